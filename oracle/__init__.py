@@ -1,0 +1,34 @@
+"""CPU oracle for the TigerCI sigma-vector path (sigma = H.c inside Davidson-Liu).
+
+TEST INFRASTRUCTURE ONLY.  Nothing in the product path (``tigerci_amd``) imports,
+calls or links anything in this package.  Only ``tests/``,
+``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of ``bench.py`` use it,
+and only as the checker (or as the timed CPU baseline), never as the thing that
+is measured or shipped.
+
+Contents
+--------
+problem   synthetic closed-shell MRSDCI problem: orbital energies, Cholesky
+          vectors L^J_pq, local pair domains, the CSF layout of the CI vector
+          (graph_mod.f90:1041-1092 layout) and seeded CI vectors.
+scepper   pack/unpack of pair blocks (scepmaker_mod.f90:60-798) and a literal
+          restatement of the 4-external ladder per (a,c) and walk
+          (three_and_four_seg_mod_1.f90:3240-3627).
+sigma     dense pair-block restatement of every sigma term of htimesc
+          (htimesc.f90:46-147) for the closed-shell single-reference CSF space,
+          plus the exact diagonal H_ii (diag_element_mod_2.f90:83 role).
+davidson  Davidson-Liu restatement (davidson.f90:47-239, 478-723).
+
+How each function is pinned (the checks live in ``tests/``):
+  * every sigma term and the diagonal: brute-force Slater-determinant CI with
+    Slater-Condon rules on tiny spaces, projected onto explicitly constructed
+    spin-adapted CSFs (tests/detci.py) -- independent of the closed forms here;
+  * the CSF basis: orthonormality and S^2 = 0 in determinant space;
+  * pack/unpack: round-trip identities and adjointness;
+  * packed ladder vs dense ladder: agreement to round-off;
+  * Davidson: converged roots equal numpy.linalg.eigh of the dense H;
+  * hermiticity c1.(H c2) == c2.(H c1) and H_ii == (H e_i)_i.
+Parity unpinned: the SGGA spin-function phase of the triplet-coupled pair
+functions and the walk order relative to the reference DRT (neither can be
+fixed without running the reference, which cannot be built here).
+"""

@@ -1,0 +1,190 @@
+"""Synthetic closed-shell MRSDCI problem and the CSF layout of the CI vector.
+
+TEST INFRASTRUCTURE ONLY (see oracle/__init__.py).
+
+Orbitals: internal (doubly occupied in the closed-shell reference) 0..n_int-1,
+then virtual n_int..n_int+n_v-1.  Virtual indices inside domains are 0-based
+over the virtual space only.
+
+Cholesky vectors follow the reference MO-integral record layout: one record
+of ``n_chol`` doubles per orbital pair p>=q (C/ThreeIndexWriter.cpp:18-35,
+cho_vectors(numcho, npair) in F/cholesky_structs.f90:35-39), pair index
+p(p+1)/2+q in 0-based form (the reference's 1-based index2(p,q) =
+max(max-1)/2+min of F/utilities_mod.f90:199 shifted by one).
+
+CI vector layout (F/graph_mod.f90:1041-1092, 1430-1450) for the closed-shell
+single reference:
+  [valence]  1 coefficient (the reference determinant, fsn(0)=1)
+  [N-1]      per internal hole i (walk order i=0..n_int-1): n_v coefficients
+             (fsn(2)=1 spin function, virt-minor; singles use all virtuals)
+  [N-2]      per strong pair walk P=(i,j), i>=j, in walk order:
+               aa block: d_P coefficients           (fsn(s)=1)
+               ab block: nsp * d_P(d_P-1)/2          (nsp=fsn(s+2): 1 for i==j,
+                                                      2 for i>j: singlet-coupled
+                                                      block first, then triplet)
+             packed ab order (2,1),(3,1),(3,2),... over domain positions, as
+             written by scepper (F/scepmaker_mod.f90:84-96).
+Weak pairs (domain smaller than the weak threshold) carry no doubles
+(WP removal, F/locist_mod.f90:204-240).
+"""
+from __future__ import annotations
+
+import dataclasses
+import numpy as np
+
+
+def pair_index(p: int, q: int) -> int:
+    """0-based packed index of orbital pair (p,q): max(max+1)/2+min."""
+    if p < q:
+        p, q = q, p
+    return p * (p + 1) // 2 + q
+
+
+@dataclasses.dataclass
+class Problem:
+    n_int: int
+    n_v: int
+    n_chol: int
+    eps: np.ndarray            # (n_orb,) orbital energies, internal first
+    L: np.ndarray              # (npair_all, n_chol) float64, pair-major records
+    pairs: np.ndarray          # (n_pairs, 2) int32 strong pairs (i>=j), walk order
+    dom_off: np.ndarray        # (n_pairs+1,) int64 offsets into dom_idx
+    dom_idx: np.ndarray        # (sum d_P,) int32 sorted virtual indices
+
+    # ---- derived -------------------------------------------------------
+    @property
+    def n_orb(self) -> int:
+        return self.n_int + self.n_v
+
+    @property
+    def n_pairs(self) -> int:
+        return int(self.pairs.shape[0])
+
+    def domain(self, P: int) -> np.ndarray:
+        return self.dom_idx[self.dom_off[P]:self.dom_off[P + 1]]
+
+    def dsize(self, P: int) -> int:
+        return int(self.dom_off[P + 1] - self.dom_off[P])
+
+    def layout(self) -> "Layout":
+        return Layout.build(self)
+
+    def L_full(self) -> np.ndarray:
+        """(n_chol, n_orb, n_orb) symmetric L^J_pq (test helper)."""
+        n = self.n_orb
+        iu = np.tril_indices(n)
+        idx = iu[0] * (iu[0] + 1) // 2 + iu[1]
+        out = np.zeros((self.n_chol, n, n))
+        out[:, iu[0], iu[1]] = self.L[idx].T
+        out[:, iu[1], iu[0]] = self.L[idx].T
+        return out
+
+
+@dataclasses.dataclass
+class Layout:
+    """Offsets of every block of the CI vector (0-based)."""
+    n_csf: int
+    ref_off: int
+    single_off: int                  # singles of hole i at single_off + i*n_v
+    aa_off: np.ndarray               # (n_pairs,)
+    ab_off: np.ndarray               # (n_pairs,)
+    nsp: np.ndarray                  # (n_pairs,) 1 or 2
+    dsz: np.ndarray                  # (n_pairs,)
+
+    @staticmethod
+    def build(prob: Problem) -> "Layout":
+        n_pairs = prob.n_pairs
+        aa_off = np.zeros(n_pairs, np.int64)
+        ab_off = np.zeros(n_pairs, np.int64)
+        nsp = np.zeros(n_pairs, np.int64)
+        dsz = np.zeros(n_pairs, np.int64)
+        off = 1 + prob.n_int * prob.n_v
+        for P in range(n_pairs):
+            i, j = prob.pairs[P]
+            d = prob.dsize(P)
+            dsz[P] = d
+            nsp[P] = 1 if i == j else 2
+            aa_off[P] = off
+            off += d
+            ab_off[P] = off
+            off += nsp[P] * (d * (d - 1) // 2)
+        return Layout(int(off), 0, 1, aa_off, ab_off, nsp, dsz)
+
+
+def make_problem(n_int: int, n_v: int, n_chol: int, seed: int,
+                 domain_range: tuple[int, int] | None = None,
+                 weak_below: int = 0,
+                 decay: float = 80.0) -> Problem:
+    """Seeded synthetic problem.
+
+    * eps_i ~ U[-1.5,-0.3], eps_a ~ U[0.5,5.0]
+    * L^J_pq = L^J_qp ~ N(0,1) * exp(-J/decay) over all orbitals
+    * full pair domains unless ``domain_range=(lo,hi)``: then each pair's
+      domain is a sorted random subset of size U[lo,hi] (inclusive) and pairs
+      with size < ``weak_below`` carry no doubles.
+    Independent sub-streams keep each piece reproducible on its own.
+    """
+    n_orb = n_int + n_v
+    rng_e = np.random.default_rng([seed, 0])
+    eps = np.concatenate([rng_e.uniform(-1.5, -0.3, n_int),
+                          rng_e.uniform(0.5, 5.0, n_v)])
+    npair_all = n_orb * (n_orb + 1) // 2
+    rng_l = np.random.default_rng([seed, 1])
+    scale = np.exp(-np.arange(n_chol) / decay)
+    L = rng_l.standard_normal((npair_all, n_chol)) * scale[None, :]
+    pairs, doms = [], []
+    rng_d = np.random.default_rng([seed, 2])
+    for i in range(n_int):
+        for j in range(i + 1):
+            if domain_range is None:
+                dom = np.arange(n_v, dtype=np.int32)
+            else:
+                size = int(rng_d.integers(domain_range[0], domain_range[1] + 1))
+                size = min(size, n_v)
+                if size < weak_below:
+                    continue
+                dom = np.sort(rng_d.choice(n_v, size, replace=False)).astype(np.int32)
+            pairs.append((i, j))
+            doms.append(dom)
+    pairs_a = np.array(pairs, dtype=np.int32).reshape(-1, 2)
+    dom_off = np.zeros(len(doms) + 1, np.int64)
+    dom_off[1:] = np.cumsum([len(d) for d in doms])
+    dom_idx = (np.concatenate(doms) if doms else np.zeros(0)).astype(np.int32)
+    return Problem(n_int, n_v, n_chol, eps, L, pairs_a, dom_off, dom_idx)
+
+
+def random_ci_vector(prob: Problem, seed: int) -> np.ndarray:
+    """N(0,1) CI vector over every CSF, normalised to 1."""
+    n = prob.layout().n_csf
+    c = np.random.default_rng([seed, 3]).standard_normal(n)
+    return c / np.linalg.norm(c)
+
+
+# ---- integral blocks from L (pq|rs) = sum_J L^J_pq L^J_rs ------------------
+# (F/make_all_integrals.f90:174-282 makeOneIntegral; F/make_abcd.f90:498-526)
+
+def eri_full(prob: Problem) -> np.ndarray:
+    """(n_orb,)*4 chemist-notation ERIs, tiny problems only."""
+    Lf = prob.L_full()
+    n = prob.n_orb
+    M = Lf.reshape(prob.n_chol, n * n)
+    return (M.T @ M).reshape(n, n, n, n)
+
+
+def core_hamiltonian(prob: Problem, eri: np.ndarray | None = None) -> np.ndarray:
+    """h_pq such that the closed-shell Fock matrix is diag(eps):
+    f_pq = h_pq + sum_k [2(pq|kk) - (pk|kq)]  ->  h = diag(eps) - G."""
+    if eri is None:
+        eri = eri_full(prob)
+    o = slice(0, prob.n_int)
+    G = 2.0 * np.einsum('pqkk->pq', eri[:, :, o, o]) - np.einsum('pkkq->pq', eri[:, o, o, :])
+    return np.diag(prob.eps) - G
+
+
+def reference_energy(prob: Problem, eri: np.ndarray | None = None) -> float:
+    """E_ref = sum_i (h_ii + eps_i) of the closed-shell reference."""
+    if eri is None:
+        eri = eri_full(prob)
+    h = core_hamiltonian(prob, eri)
+    o = np.arange(prob.n_int)
+    return float(np.sum(h[o, o] + prob.eps[o]))
