@@ -1,0 +1,166 @@
+/*
+ * tigersigma -- MI355X (gfx950) sigma-vector build sigma = H.c and the
+ * Davidson-Liu solver around it, for local MRSDCI with Cholesky-decomposed
+ * two-electron integrals (closed-shell reference, valence + N-1 + N-2 CSF
+ * space, local pair domains).
+ *
+ * C ABI: plain pointers and sizes, int status (0 = ok), text of the last
+ * error from tci_last_error().  No exception crosses this boundary.
+ * Vectors passed to tci_sigma_apply / tci_diagonal live in device memory;
+ * tci_htimesc takes host vectors.
+ *
+ * CI vector layout (one entry per CSF, 0-based):
+ *   [0]                    valence (closed-shell reference)
+ *   [1 + i*n_v + a]        N-1 walk of hole i, virtual a
+ *   per strong pair walk P = (i,j), i >= j, in the order given:
+ *       aa block  d_P entries                       (virtual doubly occupied)
+ *       ab block  nsp * d_P (d_P-1)/2 entries       (nsp = 1 if i==j else 2:
+ *                 singlet-coupled block, then triplet-coupled block; packed
+ *                 pair order (2,1),(3,1),(3,2),... over domain positions)
+ * This is the layout documented at graph_mod.f90:1041-1092 with the packing
+ * of scepmaker_mod.f90:84-96.
+ */
+#ifndef TIGERSIGMA_H
+#define TIGERSIGMA_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum tci_status {
+  TCI_OK = 0,
+  TCI_ERR_ARG = 1,     /* invalid argument or inconsistent sizes */
+  TCI_ERR_HIP = 2,     /* HIP runtime failure */
+  TCI_ERR_OOM = 3,     /* device allocation failed */
+  TCI_ERR_NUMERIC = 4  /* self-test mismatch / non-finite result */
+};
+
+enum tci_dtype { TCI_F64 = 0, TCI_F32 = 1 };
+
+/* arithmetic of the pair contractions (4-external ladder) in a sigma build */
+enum tci_precision {
+  TCI_SIGMA_FP64 = 0,    /* FP64 MFMA throughout */
+  TCI_SIGMA_BF16X3 = 1,  /* ladder as hi*hi + hi*lo + lo*hi BF16 MFMA, FP32 accumulate */
+  TCI_SIGMA_FP16X3 = 2   /* same split in scaled FP16 */
+};
+
+/* flag bits for tci_sigma_apply (low byte = tci_precision) */
+#define TCI_ACCUMULATE 0x100 /* sigma += H c (htimesc semantics) instead of sigma = H c */
+#define TCI_ADD_EREF   0x200 /* include E_ref on the diagonal (bare H) */
+
+/* Problem description.  All pointers are host pointers, read during
+ * tci_plan_create only.  Orbitals: internal 0..n_int-1, virtual
+ * n_int..n_int+n_v-1.  `chol` holds one record of n_chol values per orbital
+ * pair p>=q, record index p(p+1)/2+q (the MO Cholesky record layout of
+ * ThreeIndexWriter.cpp:18-35 / cholesky_structs.f90:35-39, 0-based).
+ * eps are the orbital energies: the core Hamiltonian is the one whose
+ * closed-shell Fock matrix is diag(eps).  Domains are sorted virtual indices
+ * (0-based over the virtual space) per strong pair walk (allowed virtuals,
+ * allowed_virtuals_mod.f90:163-224); pairs left out carry no doubles. */
+typedef struct tci_problem {
+  int64_t n_int;
+  int64_t n_v;
+  int64_t n_chol;
+  const double* eps;       /* n_int + n_v */
+  const void* chol;        /* (n_orb(n_orb+1)/2) x n_chol, dtype chol_dtype */
+  int32_t chol_dtype;      /* TCI_F64 or TCI_F32 */
+  int32_t ladder_dtype;    /* device storage of L^J_ab for the ladder: TCI_F64 or TCI_F32 */
+  int64_t n_pairs;         /* strong pair walks */
+  const int32_t* pairs;    /* n_pairs x 2: (i, j), i >= j */
+  const int64_t* dom_off;  /* n_pairs + 1 */
+  const int32_t* dom_idx;  /* dom_off[n_pairs] entries */
+} tci_problem;
+
+typedef struct tci_plan tci_plan;
+
+typedef struct tci_plan_info {
+  int64_t n_csf;           /* length of the CI vector */
+  int64_t n_ladder_items;  /* work items of the grouped ladder launch */
+  double e_ref;            /* closed-shell reference energy */
+  double flops_ladder;     /* algorithmic flops of the ladder per sigma: sum_P 4 d_P^3 n_chol */
+  double flops_sigma;      /* algorithmic flops of one FP64 sigma (all contractions) */
+  double bytes_ladder;     /* algorithmic HBM bytes of the ladder per sigma */
+  int64_t device_bytes;    /* device memory held by the plan */
+} tci_plan_info;
+
+/* Build a plan: uploads L^J, assembles (ia|jb), (ij|ab), (ij|kl), (ij|ka)
+ * and the exact diagonal on the device, and builds the grouped launch
+ * tables.  Replaces the setup that htimesc performs on its first call
+ * (htimesc.f90:77-82 first_time; makeIntegrals make_all_integrals.f90:23;
+ * record dry runs three_and_four_seg_mod_1.f90:3142-3238,
+ * three_and_four_seg_mod_2.f90:1095-1195).  Uses the current HIP device. */
+int tci_plan_create(const tci_problem* prob, tci_plan** out);
+void tci_plan_destroy(tci_plan* plan);
+int tci_plan_get_info(const tci_plan* plan, tci_plan_info* info);
+
+/* sigma = (H - E_ref) c  (or += with TCI_ACCUMULATE; bare H with TCI_ADD_EREF).
+ * c and sigma are device arrays of n = n_csf doubles; stream may be NULL.
+ * Replaces htimesc(civec, sigmavec, data) (htimesc.f90:46-147). */
+int tci_sigma_apply(tci_plan* plan, const double* c, double* sigma, int64_t n,
+                    int32_t flags, void* stream);
+
+/* Host-vector drop-in with htimesc semantics: sigmavec += H civec (bare H,
+ * FP64).  Copies over PCIe; synchronous. */
+int tci_htimesc(tci_plan* plan, const double* civec, double* sigmavec, int64_t n);
+
+/* diag[k] = H_kk - E_ref (device array of n doubles); the values
+ * diag_element (diag_element_mod_2.f90:83) writes to iodiagstore. */
+int tci_diagonal(tci_plan* plan, double* diag, int64_t n, void* stream);
+
+typedef struct tci_davidson_params {
+  int32_t roots;          /* number of lowest roots */
+  int32_t maxit;          /* 0 -> 50*roots (davidson.f90:80) */
+  double energy_tol;      /* |dE| convergence (ENERGY TOLERANCE, default 1e-6) */
+  double norm_tol;        /* residual norm convergence (RESIDUAL TOLERANCE) */
+  int32_t precision;      /* tci_precision of the expansion sigmas */
+  double switch_tol;      /* mixed: switch to FP64 sigma once every |dE| < switch_tol */
+  int32_t max_vectors;    /* subspace collapse budget (0 -> maxit*roots) */
+  int32_t verbose;        /* 1: print the reference-style iteration table */
+} tci_davidson_params;
+
+typedef struct tci_davidson_result {
+  int32_t iterations;
+  int32_t sigma_fp64;     /* number of FP64 sigma builds */
+  int32_t sigma_mixed;    /* number of mixed-precision sigma builds */
+  int32_t converged;      /* all roots converged */
+  double seconds;         /* wall time of the solve */
+  double seconds_sigma;   /* of which inside sigma builds */
+} tci_davidson_result;
+
+/* Davidson-Liu for the lowest roots (davidson_liu, davidson.f90:47-239).
+ * energies: host array of `roots` total energies (E_ref included).
+ * vectors: device array roots x n_csf for the converged vectors, or NULL. */
+int tci_davidson_run(tci_plan* plan, const tci_davidson_params* params,
+                     double* energies, double* vectors, tci_davidson_result* result);
+
+/* Device memory for callers without a GPU runtime of their own (a Fortran or
+ * C Davidson driver).  No reference counterpart. */
+int tci_device_alloc(int64_t bytes, void** ptr);
+int tci_device_free(void* ptr);
+int tci_memcpy_h2d(void* dst, const void* src, int64_t bytes);
+int tci_memcpy_d2h(void* dst, const void* src, int64_t bytes);
+int tci_memcpy_d2d(void* dst, const void* src, int64_t bytes);
+int tci_device_synchronize(void);
+
+/* Kernel timing with HIP events on the stream of each sigma build: when
+ * enabled, every tci_sigma_apply records the duration of its ladder launch and
+ * of the whole build.  tci_profile_read returns the sums since enabling.
+ * No reference counterpart (the reference's per-phase wall clocks live in
+ * time_var_mod.f90). */
+int tci_profile_enable(tci_plan* plan, int32_t on);
+int tci_profile_read(const tci_plan* plan, double* ladder_ms, int64_t* ladder_launches,
+                     double* sigma_ms, int64_t* sigma_calls);
+
+/* Checks the MFMA operand/result lane layouts the kernels assume against a
+ * scalar product on the device; 0 when they agree.  No reference counterpart. */
+int tci_selftest_mfma(void);
+
+const char* tci_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TIGERSIGMA_H */

@@ -1,0 +1,37 @@
+"""GPU Davidson: converged roots vs dense eigh and vs the oracle Davidson."""
+import os
+
+import numpy as np
+import pytest
+
+import tigerci_amd as tci
+from oracle.problem import Problem, make_problem
+from oracle.sigma import Integrals, sigma as oracle_sigma, diagonal as oracle_diagonal, dense_hamiltonian
+from oracle.davidson import davidson as oracle_davidson
+
+pytestmark = pytest.mark.gpu
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def test_davidson_golden_roots():
+    print("case: davidson golden", flush=True)
+    z = np.load(os.path.join(ROOT, "tests", "golden", "full_n3_v12.npz"))
+    prob = Problem(int(z["n_int"]), int(z["n_v"]), int(z["n_chol"]), z["eps"], z["L"], z["pairs"],
+                   z["dom_off"], z["dom_idx"])
+    plan = tci.Plan.from_problem(prob)
+    e, res = plan.davidson(roots=3, energy_tol=1e-11, norm_tol=1e-8)
+    assert res["converged"]
+    assert np.abs(e - plan.info.e_ref - z["eigvals"][:3]).max() < 1e-8
+
+
+def test_davidson_matches_oracle_trajectory():
+    print("case: davidson vs oracle", flush=True)
+    prob = make_problem(5, 30, 40, 3)
+    ints = Integrals(prob)
+    plan = tci.Plan.from_problem(prob)
+    E, X, hist = oracle_davidson(lambda v: oracle_sigma(prob, v, ints), oracle_diagonal(prob, ints),
+                                 roots=2, energy_tol=1e-6, norm_tol=1e-6)
+    e, res = plan.davidson(roots=2, energy_tol=1e-6, norm_tol=1e-6)
+    assert res["converged"]
+    assert np.abs(e - plan.info.e_ref - E).max() < 1e-8
+    assert res["iterations"] == hist[-1][0]

@@ -1,0 +1,299 @@
+"""tigerci_amd -- MI355X sigma = H.c and Davidson-Liu for local MRSDCI with
+Cholesky-decomposed integrals.
+
+Host-side mirror of the reference's sigma interface over the C ABI of
+``libtigersigma.so`` (include/tigersigma.h):
+
+  Plan(problem arrays)            set-up that htimesc does on its first call
+                                  (htimesc.f90:77-82) plus makeIntegrals
+                                  (make_all_integrals.f90:23)
+  Plan.sigma(c, out)              htimesc(civec, sigmavec, data) on device
+                                  vectors (htimesc.f90:46-147)
+  htimesc(civec, sigmavec, plan)  the reference's argument order and
+                                  semantics: sigmavec += H civec, host arrays
+  Plan.diagonal()                 H_ii (diag_element_mod_2.f90:83)
+  Plan.davidson(...)              davidson_liu (davidson.f90:47-239)
+
+The HIP library is required: nothing here falls back to a CPU path.  Build it
+with ``python -m tigerci_amd.build``.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libtigersigma.so")
+
+TCI_F64, TCI_F32 = 0, 1
+PRECISION = {"fp64": 0, "bf16x3": 1, "fp16x3": 2}
+TCI_ACCUMULATE = 0x100
+TCI_ADD_EREF = 0x200
+
+
+class TigerSigmaError(RuntimeError):
+    pass
+
+
+class _Problem(ctypes.Structure):
+    _fields_ = [("n_int", ctypes.c_int64), ("n_v", ctypes.c_int64), ("n_chol", ctypes.c_int64),
+                ("eps", ctypes.c_void_p), ("chol", ctypes.c_void_p), ("chol_dtype", ctypes.c_int32),
+                ("ladder_dtype", ctypes.c_int32), ("n_pairs", ctypes.c_int64), ("pairs", ctypes.c_void_p),
+                ("dom_off", ctypes.c_void_p), ("dom_idx", ctypes.c_void_p)]
+
+
+class _Info(ctypes.Structure):
+    _fields_ = [("n_csf", ctypes.c_int64), ("n_ladder_items", ctypes.c_int64), ("e_ref", ctypes.c_double),
+                ("flops_ladder", ctypes.c_double), ("flops_sigma", ctypes.c_double),
+                ("bytes_ladder", ctypes.c_double), ("device_bytes", ctypes.c_int64)]
+
+
+class _DavParams(ctypes.Structure):
+    _fields_ = [("roots", ctypes.c_int32), ("maxit", ctypes.c_int32), ("energy_tol", ctypes.c_double),
+                ("norm_tol", ctypes.c_double), ("precision", ctypes.c_int32), ("switch_tol", ctypes.c_double),
+                ("max_vectors", ctypes.c_int32), ("verbose", ctypes.c_int32)]
+
+
+class _DavResult(ctypes.Structure):
+    _fields_ = [("iterations", ctypes.c_int32), ("sigma_fp64", ctypes.c_int32), ("sigma_mixed", ctypes.c_int32),
+                ("converged", ctypes.c_int32), ("seconds", ctypes.c_double), ("seconds_sigma", ctypes.c_double)]
+
+
+_lib = None
+
+
+def load_library(path: str | None = None) -> ctypes.CDLL:
+    """Load libtigersigma.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = path or LIB_PATH
+    if not os.path.exists(p):
+        raise TigerSigmaError(f"{p} is missing: build it with `python -m tigerci_amd.build`")
+    lib = ctypes.CDLL(p)
+    vp, i32, i64 = ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64
+    lib.tci_plan_create.argtypes = [ctypes.POINTER(_Problem), ctypes.POINTER(vp)]
+    lib.tci_plan_destroy.argtypes = [vp]
+    lib.tci_plan_destroy.restype = None
+    lib.tci_plan_get_info.argtypes = [vp, ctypes.POINTER(_Info)]
+    lib.tci_sigma_apply.argtypes = [vp, vp, vp, i64, i32, vp]
+    lib.tci_htimesc.argtypes = [vp, vp, vp, i64]
+    lib.tci_diagonal.argtypes = [vp, vp, i64, vp]
+    lib.tci_davidson_run.argtypes = [vp, ctypes.POINTER(_DavParams), vp, vp, ctypes.POINTER(_DavResult)]
+    lib.tci_selftest_mfma.argtypes = []
+    lib.tci_device_alloc.argtypes = [i64, ctypes.POINTER(vp)]
+    lib.tci_device_free.argtypes = [vp]
+    lib.tci_memcpy_h2d.argtypes = [vp, vp, i64]
+    lib.tci_memcpy_d2h.argtypes = [vp, vp, i64]
+    lib.tci_memcpy_d2d.argtypes = [vp, vp, i64]
+    lib.tci_device_synchronize.argtypes = []
+    lib.tci_profile_enable.argtypes = [vp, i32]
+    lib.tci_profile_read.argtypes = [vp, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(i64),
+                                     ctypes.POINTER(ctypes.c_double), ctypes.POINTER(i64)]
+    lib.tci_last_error.restype = ctypes.c_char_p
+    for f in ("tci_plan_create", "tci_plan_get_info", "tci_sigma_apply", "tci_htimesc", "tci_diagonal",
+              "tci_davidson_run", "tci_selftest_mfma", "tci_device_alloc", "tci_device_free", "tci_memcpy_h2d",
+              "tci_memcpy_d2h", "tci_memcpy_d2d", "tci_device_synchronize", "tci_profile_enable",
+              "tci_profile_read"):
+        getattr(lib, f).restype = ctypes.c_int
+    if path is None:
+        _lib = lib
+    return lib
+
+
+def _check(rc: int):
+    if rc != 0:
+        msg = _lib.tci_last_error().decode(errors="replace") if _lib is not None else ""
+        raise TigerSigmaError(f"tigersigma error {rc}: {msg}")
+
+
+def selftest_mfma() -> None:
+    lib = load_library()
+    _check(lib.tci_selftest_mfma())
+
+
+def synchronize() -> None:
+    _check(load_library().tci_device_synchronize())
+
+
+class DeviceArray:
+    """A float64 vector in device memory, allocated through the library's own
+    HIP runtime (so no second GPU runtime enters the process)."""
+
+    def __init__(self, n: int):
+        self._lib = load_library()
+        self.n = int(n)
+        p = ctypes.c_void_p()
+        _check(self._lib.tci_device_alloc(8 * self.n, ctypes.byref(p)))
+        self.ptr = p.value
+
+    @classmethod
+    def from_numpy(cls, a: np.ndarray) -> "DeviceArray":
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        d = cls(a.size)
+        d.upload(a)
+        return d
+
+    def upload(self, a: np.ndarray) -> None:
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        if a.size != self.n:
+            raise TigerSigmaError("size mismatch")
+        _check(self._lib.tci_memcpy_h2d(self.ptr, a.ctypes.data, 8 * self.n))
+
+    def numpy(self) -> np.ndarray:
+        out = np.empty(self.n)
+        _check(self._lib.tci_memcpy_d2h(out.ctypes.data, self.ptr, 8 * self.n))
+        return out
+
+    def copy_from(self, other: "DeviceArray") -> None:
+        _check(self._lib.tci_memcpy_d2d(self.ptr, other.ptr, 8 * self.n))
+
+    def free(self) -> None:
+        if getattr(self, "ptr", None):
+            self._lib.tci_device_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def _ptr(t) -> int:
+    """Device address of a DeviceArray, of an object exposing data_ptr(), or an int."""
+    if isinstance(t, int):
+        return t
+    if isinstance(t, DeviceArray):
+        return t.ptr
+    return t.data_ptr()
+
+
+def _stream(stream):
+    if stream is None:
+        return None
+    return stream if isinstance(stream, int) else stream.cuda_stream
+
+
+@dataclass
+class PlanInfo:
+    n_csf: int
+    n_ladder_items: int
+    e_ref: float
+    flops_ladder: float
+    flops_sigma: float
+    bytes_ladder: float
+    device_bytes: int
+
+
+class Plan:
+    """Device plan of one CI problem (see the module docstring)."""
+
+    def __init__(self, n_int: int, n_v: int, n_chol: int, eps, chol, pairs, dom_off, dom_idx,
+                 ladder_dtype: str = "f64"):
+        self._lib = load_library()
+        eps = np.ascontiguousarray(eps, dtype=np.float64)
+        chol = np.ascontiguousarray(chol)
+        if chol.dtype not in (np.float64, np.float32):
+            chol = chol.astype(np.float64)
+        n_orb = n_int + n_v
+        if chol.size != n_orb * (n_orb + 1) // 2 * n_chol:
+            raise TigerSigmaError("chol must hold n_orb(n_orb+1)/2 records of n_chol values")
+        pairs = np.ascontiguousarray(pairs, dtype=np.int32).reshape(-1, 2)
+        dom_off = np.ascontiguousarray(dom_off, dtype=np.int64)
+        dom_idx = np.ascontiguousarray(dom_idx, dtype=np.int32)
+        self._keep = (eps, chol, pairs, dom_off, dom_idx)
+        pr = _Problem(n_int, n_v, n_chol, eps.ctypes.data, chol.ctypes.data,
+                      TCI_F64 if chol.dtype == np.float64 else TCI_F32,
+                      TCI_F64 if ladder_dtype == "f64" else TCI_F32,
+                      pairs.shape[0], pairs.ctypes.data if pairs.size else None,
+                      dom_off.ctypes.data, dom_idx.ctypes.data if dom_idx.size else None)
+        h = ctypes.c_void_p()
+        _check(self._lib.tci_plan_create(ctypes.byref(pr), ctypes.byref(h)))
+        self._h = h
+        self._keep = None
+        inf = _Info()
+        _check(self._lib.tci_plan_get_info(self._h, ctypes.byref(inf)))
+        self.info = PlanInfo(inf.n_csf, inf.n_ladder_items, inf.e_ref, inf.flops_ladder, inf.flops_sigma,
+                             inf.bytes_ladder, inf.device_bytes)
+        self.n_csf = inf.n_csf
+
+    @classmethod
+    def from_problem(cls, prob, ladder_dtype: str = "f64") -> "Plan":
+        """Build from any object with the attributes n_int, n_v, n_chol, eps, L,
+        pairs, dom_off, dom_idx (L: records x n_chol)."""
+        return cls(prob.n_int, prob.n_v, prob.n_chol, prob.eps, prob.L, prob.pairs, prob.dom_off,
+                   prob.dom_idx, ladder_dtype=ladder_dtype)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.tci_plan_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def sigma(self, c, out, precision: str = "fp64", accumulate: bool = False, add_eref: bool = False,
+              stream=None) -> None:
+        """out = (H - E_ref) c on device (DeviceArray or device addresses)."""
+        flags = PRECISION[precision] | (TCI_ACCUMULATE if accumulate else 0) | (TCI_ADD_EREF if add_eref else 0)
+        _check(self._lib.tci_sigma_apply(self._h, _ptr(c), _ptr(out), self.n_csf, flags, _stream(stream)))
+
+    def diagonal(self, out=None, stream=None):
+        """Exact H_ii - E_ref into ``out`` (device); returns numpy when out is None."""
+        if out is None:
+            d = DeviceArray(self.n_csf)
+            _check(self._lib.tci_diagonal(self._h, d.ptr, self.n_csf, None))
+            return d.numpy()
+        _check(self._lib.tci_diagonal(self._h, _ptr(out), self.n_csf, _stream(stream)))
+        return None
+
+    def sigma_numpy(self, x: np.ndarray, precision: str = "fp64") -> np.ndarray:
+        """Convenience: host vector in, host sigma out (device work in between)."""
+        c = DeviceArray.from_numpy(x)
+        y = DeviceArray(self.n_csf)
+        self.sigma(c, y, precision=precision)
+        return y.numpy()
+
+    def profile(self, on: bool = True) -> None:
+        _check(self._lib.tci_profile_enable(self._h, 1 if on else 0))
+
+    def profile_read(self) -> dict:
+        lm, sm = ctypes.c_double(), ctypes.c_double()
+        ln, sn = ctypes.c_int64(), ctypes.c_int64()
+        _check(self._lib.tci_profile_read(self._h, ctypes.byref(lm), ctypes.byref(ln), ctypes.byref(sm),
+                                          ctypes.byref(sn)))
+        return dict(ladder_ms=lm.value, ladder_launches=ln.value, sigma_ms=sm.value, sigma_calls=sn.value)
+
+    def davidson(self, roots: int = 1, energy_tol: float = 1e-6, norm_tol: float = 1e-6,
+                 precision: str = "fp64", switch_tol: float = 1e-4, maxit: int = 0,
+                 max_vectors: int = 0, vectors=None, verbose: bool = False):
+        """Lowest roots; returns (total energies, result dict).  ``vectors`` is an
+        optional DeviceArray of roots*n_csf doubles for the eigenvectors."""
+        p = _DavParams(roots, maxit, energy_tol, norm_tol, PRECISION[precision], switch_tol, max_vectors,
+                       1 if verbose else 0)
+        e = np.zeros(roots)
+        r = _DavResult()
+        _check(self._lib.tci_davidson_run(self._h, ctypes.byref(p), e.ctypes.data,
+                                          _ptr(vectors) if vectors is not None else None, ctypes.byref(r)))
+        return e, dict(iterations=r.iterations, sigma_fp64=r.sigma_fp64, sigma_mixed=r.sigma_mixed,
+                       converged=bool(r.converged), seconds=r.seconds, seconds_sigma=r.seconds_sigma)
+
+    def htimesc_host(self, civec: np.ndarray, sigmavec: np.ndarray) -> None:
+        if civec.dtype != np.float64 or sigmavec.dtype != np.float64 or not sigmavec.flags.c_contiguous:
+            raise TigerSigmaError("htimesc needs contiguous float64 arrays")
+        civec = np.ascontiguousarray(civec)
+        _check(self._lib.tci_htimesc(self._h, civec.ctypes.data, sigmavec.ctypes.data, self.n_csf))
+
+
+def htimesc(civec: np.ndarray, sigmavec: np.ndarray, data: Plan) -> None:
+    """sigmavec += H civec (bare Hamiltonian), the argument order of
+    ``subroutine htimesc(civec, sigmavec, data)`` (htimesc.f90:46): civec is
+    read-only, sigmavec accumulates (the caller zeroes it, davidson.f90:126)."""
+    data.htimesc_host(civec, sigmavec)

@@ -1,0 +1,117 @@
+// Shared definitions of the tigersigma HIP library (gfx950 only).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <hip/hip_bf16.h>
+#include <hip/hip_fp16.h>
+#include <cstdint>
+#include <cstdio>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/tigersigma.h"
+
+namespace tci {
+
+struct Error : std::runtime_error {
+  int code;
+  Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+void set_last_error(const std::string& msg);
+
+#define TCI_HIP(expr)                                                              \
+  do {                                                                             \
+    hipError_t _e = (expr);                                                        \
+    if (_e != hipSuccess)                                                          \
+      throw ::tci::Error(TCI_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(_e) + \
+                                          " at " + __FILE__ + ":" + std::to_string(__LINE__)); \
+  } while (0)
+
+#define TCI_REQUIRE(cond, msg)                                                     \
+  do {                                                                             \
+    if (!(cond)) throw ::tci::Error(TCI_ERR_ARG, msg);                             \
+  } while (0)
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef _Float16 h8 __attribute__((ext_vector_type(8)));
+
+// Device allocation owned by a plan.
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  ~DevBuf() { reset(); }
+  void reset() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  void alloc(size_t n) {
+    reset();
+    if (n == 0) n = 256;
+    hipError_t e = hipMalloc(&p, n);
+    if (e != hipSuccess) {
+      p = nullptr;
+      throw Error(TCI_ERR_OOM, "hipMalloc of " + std::to_string(n) + " bytes failed: " +
+                                   hipGetErrorString(e));
+    }
+    bytes = n;
+  }
+  template <class T> T* as() const { return static_cast<T*>(p); }
+};
+
+template <class T>
+void upload(DevBuf& b, const std::vector<T>& h, hipStream_t s = nullptr) {
+  b.alloc(h.size() * sizeof(T));
+  if (!h.empty()) TCI_HIP(hipMemcpyAsync(b.p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, s));
+}
+
+inline int64_t cdiv(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+// ---------------------------------------------------------------------------
+// Generic grouped gather-GEMM (FP64 MFMA):  C[tile] (+)= sum_seg alpha A_seg B_seg
+// An operand element (r,k) is base[rmap(r)*rs + cmap(k)*cs]; a map entry < 0
+// (or a null base) reads as zero; a null map is the identity.
+struct Op {
+  const void* base;   // double, or float when f32 != 0
+  int64_t rs, cs;
+  const int32_t* rmap;
+  const int32_t* cmap;
+  int32_t f32;
+};
+struct Seg {
+  Op A, B;
+  int32_t K;
+  double alpha;
+};
+struct Task {
+  double* C;
+  int64_t ldc;
+  int32_t M, N;     // full output extent (bounds)
+  int32_t m0, n0;   // this tile's origin
+  int32_t seg0, seg1;
+  int32_t accumulate;
+};
+
+// A list of tasks and their segments, kept on the host while being built and
+// mirrored on the device.
+struct GemmProgram {
+  std::vector<Task> tasks;
+  std::vector<Seg> segs;
+  DevBuf d_tasks, d_segs;
+  void clear() { tasks.clear(); segs.clear(); }
+  // add one output matrix: tiles over (M,N), all sharing `segs_for_output`
+  void add(double* C, int64_t ldc, int M, int N, const std::vector<Seg>& s, bool accumulate);
+  void upload_tables(hipStream_t st);
+  void launch(hipStream_t st) const;
+};
+
+void launch_ggemm(const Task* tasks, const Seg* segs, int64_t ntasks, hipStream_t st);
+
+}  // namespace tci

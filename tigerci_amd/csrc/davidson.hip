@@ -1,0 +1,478 @@
+// Davidson-Liu on device (davidson_liu, davidson.f90:47-239).
+//
+// The expansion vectors c_k and sigma_k = H c_k stay in HBM (the reference
+// keeps them in files, davidson_io.f90:26-52).  Per iteration and root:
+//   h(k,idx) = <c_k, sigma_idx>                  one multi-dot pass
+//   r = sum_k y_k (sigma_k - E c_k)              one multi-axpy pass
+//   r <- -r / (H_ii - E), 0 where H_ii - E == 0  (build_next_vector_std_eig)
+//   orthonormalise against all c_k              classical Gram-Schmidt applied
+//                                                twice, then normalise; zero if
+//                                                the norm < 1e-14.  The
+//                                                reference's modified GS with
+//                                                renormalisation after every
+//                                                projection spans the same vector.
+// The m x m subspace problem (m <= 50 roots) is solved on the host (Jacobi).
+// Mixed precision: expansion sigmas use the split ladder until every root's
+// |dE| < switch_tol; then the subspace collapses to the current Ritz vectors,
+// their sigmas are rebuilt in FP64 and the solve finishes in FP64.  The first
+// iteration after the switch never counts as converged (its dE compares a
+// mixed-precision eigenvalue with an FP64 one).
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <numeric>
+
+#include "plan.hpp"
+
+namespace tci {
+namespace {
+
+constexpr int MD_GROUP = 8;
+
+// partial[k][b] = sum over block b's elements of X_k . v
+__global__ void mdot_kernel(const double* const* __restrict__ X, int m, const double* __restrict__ v, int64_t n,
+                            double* __restrict__ partial, int nblk) {
+  const int k0 = blockIdx.y * MD_GROUP;
+  double acc[MD_GROUP];
+#pragma unroll
+  for (int q = 0; q < MD_GROUP; ++q) acc[q] = 0.0;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n; e += (int64_t)nblk * blockDim.x) {
+    const double ve = v[e];
+#pragma unroll
+    for (int q = 0; q < MD_GROUP; ++q)
+      if (k0 + q < m) acc[q] += X[k0 + q][e] * ve;
+  }
+  __shared__ double red[MD_GROUP][256];
+#pragma unroll
+  for (int q = 0; q < MD_GROUP; ++q) red[q][threadIdx.x] = acc[q];
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w)
+#pragma unroll
+      for (int q = 0; q < MD_GROUP; ++q) red[q][threadIdx.x] += red[q][threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x < MD_GROUP && k0 + (int)threadIdx.x < m) partial[(int64_t)(k0 + threadIdx.x) * nblk + blockIdx.x] = red[threadIdx.x][0];
+}
+
+__global__ void mdot_reduce_kernel(const double* __restrict__ partial, int nblk, int m, double* __restrict__ out) {
+  const int k = blockIdx.x;
+  if (k >= m) return;
+  __shared__ double red[256];
+  double a = 0.0;
+  for (int b = threadIdx.x; b < nblk; b += blockDim.x) a += partial[(int64_t)k * nblk + b];
+  red[threadIdx.x] = a;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[k] = red[0];
+}
+
+// out = beta*out + sum_k coef[k] X_k
+__global__ void lincomb_kernel(double* __restrict__ out, double beta, const double* const* __restrict__ X,
+                               const double* __restrict__ coef, int m, int64_t n) {
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x) {
+    double a = beta == 0.0 ? 0.0 : beta * out[e];
+    for (int k = 0; k < m; ++k) a += coef[k] * X[k][e];
+    out[e] = a;
+  }
+}
+
+__global__ void precond_kernel(double* __restrict__ r, const double* __restrict__ diag, double E, int64_t n) {
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x) {
+    const double p = diag[e] - E;
+    r[e] = (p == 0.0) ? 0.0 : -r[e] / p;
+  }
+}
+
+__global__ void scale_kernel(double* __restrict__ v, double a, int64_t n) {
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x)
+    v[e] *= a;
+}
+
+__global__ void unit_kernel(double* __restrict__ v, int64_t n, int64_t k) {
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x)
+    v[e] = (e == k) ? 1.0 : 0.0;
+}
+
+// cyclic Jacobi eigen-solver for a small symmetric matrix (ascending eigenvalues)
+void jacobi_eigh(int m, std::vector<double> A, std::vector<double>& w, std::vector<double>& V) {
+  V.assign((size_t)m * m, 0.0);
+  for (int i = 0; i < m; ++i) V[(size_t)i * m + i] = 1.0;
+  auto a = [&](int i, int j) -> double& { return A[(size_t)i * m + j]; };
+  for (int sweep = 0; sweep < 100; ++sweep) {
+    double off = 0.0, tot = 0.0;
+    for (int i = 0; i < m; ++i)
+      for (int j = 0; j < m; ++j) {
+        tot += a(i, j) * a(i, j);
+        if (i != j) off += a(i, j) * a(i, j);
+      }
+    if (off <= 1e-30 * std::max(tot, 1e-300)) break;
+    for (int p = 0; p < m; ++p)
+      for (int q = p + 1; q < m; ++q) {
+        const double apq = a(p, q);
+        if (std::fabs(apq) < 1e-300) continue;
+        const double theta = (a(q, q) - a(p, p)) / (2.0 * apq);
+        const double t = (theta >= 0 ? 1.0 : -1.0) / (std::fabs(theta) + std::sqrt(theta * theta + 1.0));
+        const double c = 1.0 / std::sqrt(t * t + 1.0), s = t * c;
+        for (int k = 0; k < m; ++k) {
+          const double akp = a(k, p), akq = a(k, q);
+          a(k, p) = c * akp - s * akq;
+          a(k, q) = s * akp + c * akq;
+        }
+        for (int k = 0; k < m; ++k) {
+          const double apk = a(p, k), aqk = a(q, k);
+          a(p, k) = c * apk - s * aqk;
+          a(q, k) = s * apk + c * aqk;
+        }
+        for (int k = 0; k < m; ++k) {
+          const double vkp = V[(size_t)k * m + p], vkq = V[(size_t)k * m + q];
+          V[(size_t)k * m + p] = c * vkp - s * vkq;
+          V[(size_t)k * m + q] = s * vkp + c * vkq;
+        }
+      }
+  }
+  std::vector<int> ord(m);
+  std::iota(ord.begin(), ord.end(), 0);
+  std::stable_sort(ord.begin(), ord.end(), [&](int x, int y) { return a(x, x) < a(y, y); });
+  w.resize(m);
+  std::vector<double> Vs((size_t)m * m);
+  for (int c = 0; c < m; ++c) {
+    w[c] = a(ord[c], ord[c]);
+    for (int k = 0; k < m; ++k) Vs[(size_t)k * m + c] = V[(size_t)k * m + ord[c]];
+  }
+  V.swap(Vs);
+}
+
+struct Dav {
+  tci_plan& P;
+  hipStream_t st = nullptr;
+  int64_t n;
+  int nblk = 1024;
+  DevBuf ptrs, coefs, partial, dots;
+  std::vector<double*> hptr;
+  explicit Dav(tci_plan& p) : P(p), n(p.n_csf) {
+    nblk = (int)std::min<int64_t>(1024, std::max<int64_t>(1, cdiv(n, 256 * 4)));
+    ptrs.alloc(1024 * sizeof(double*));
+    coefs.alloc(1024 * sizeof(double));
+    partial.alloc((size_t)1024 * nblk * sizeof(double));
+    dots.alloc(1024 * sizeof(double));
+  }
+  // out[k] = X_k . v for k < m
+  void mdot(const std::vector<double*>& X, const double* v, std::vector<double>& out) {
+    const int m = (int)X.size();
+    out.assign(m, 0.0);
+    if (m == 0) return;
+    TCI_REQUIRE(m <= 1024, "too many vectors");
+    TCI_HIP(hipMemcpyAsync(ptrs.p, X.data(), m * sizeof(double*), hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(mdot_kernel, dim3(nblk, (unsigned)cdiv(m, MD_GROUP)), dim3(256), 0, st,
+                       (const double* const*)ptrs.p, m, v, n, partial.as<double>(), nblk);
+    TCI_HIP(hipGetLastError());
+    hipLaunchKernelGGL(mdot_reduce_kernel, dim3(m), dim3(256), 0, st, partial.as<double>(), nblk, m,
+                       dots.as<double>());
+    TCI_HIP(hipGetLastError());
+    TCI_HIP(hipMemcpyAsync(out.data(), dots.p, m * sizeof(double), hipMemcpyDeviceToHost, st));
+    TCI_HIP(hipStreamSynchronize(st));
+  }
+  void lincomb(double* out, double beta, const std::vector<double*>& X, const std::vector<double>& c) {
+    const int m = (int)X.size();
+    TCI_REQUIRE(m <= 1024, "too many vectors");
+    if (m > 0) {
+      TCI_HIP(hipMemcpyAsync(ptrs.p, X.data(), m * sizeof(double*), hipMemcpyHostToDevice, st));
+      TCI_HIP(hipMemcpyAsync(coefs.p, c.data(), m * sizeof(double), hipMemcpyHostToDevice, st));
+    }
+    hipLaunchKernelGGL(lincomb_kernel, dim3((unsigned)std::min<int64_t>(cdiv(n, 256), 8192)), dim3(256), 0, st, out, beta,
+                       (const double* const*)ptrs.p, coefs.as<double>(), m, n);
+    TCI_HIP(hipGetLastError());
+    // the pointer/coef tables are reused by the next call: keep ordering simple
+    TCI_HIP(hipStreamSynchronize(st));
+  }
+  double norm(const double* v) {
+    std::vector<double> o;
+    mdot({const_cast<double*>(v)}, v, o);
+    return std::sqrt(std::max(o[0], 0.0));
+  }
+  void scale(double* v, double a) {
+    hipLaunchKernelGGL(scale_kernel, dim3((unsigned)std::min<int64_t>(cdiv(n, 256), 8192)), dim3(256), 0, st, v, a, n);
+    TCI_HIP(hipGetLastError());
+  }
+  // orthonormalise v against X (CGS2); returns false if v vanished
+  bool orthonormalize(double* v, const std::vector<double*>& X) {
+    std::vector<double> ov;
+    for (int pass = 0; pass < 2 && !X.empty(); ++pass) {
+      mdot(X, v, ov);
+      for (double& o : ov) o = -o;
+      lincomb(v, 1.0, X, ov);
+    }
+    const double nr = norm(v);
+    if (nr < 1e-14) {
+      scale(v, 0.0);
+      return false;
+    }
+    scale(v, 1.0 / nr);
+    return true;
+  }
+};
+
+}  // namespace
+}  // namespace tci
+
+using namespace tci;
+
+extern "C" int tci_davidson_run(tci_plan* plan, const tci_davidson_params* prm, double* energies, double* vectors,
+                                tci_davidson_result* result) {
+  try {
+    TCI_REQUIRE(plan && prm && energies, "null argument");
+    const int roots = prm->roots;
+    TCI_REQUIRE(roots >= 1 && roots <= 64 && roots <= plan->n_csf, "roots out of range");
+    const int maxit = prm->maxit > 0 ? prm->maxit : 50 * roots;
+    int mv = prm->max_vectors > 0 ? prm->max_vectors : maxit * roots + roots;
+    mv = std::min(mv, 1000);
+    TCI_REQUIRE(mv >= 5 * roots, "max_vectors must be at least 5*roots");
+    const int prec0 = prm->precision;
+    TCI_REQUIRE(prec0 == TCI_SIGMA_FP64 || prec0 == TCI_SIGMA_BF16X3 || prec0 == TCI_SIGMA_FP16X3, "unknown precision");
+    const int64_t n = plan->n_csf;
+    Dav D(*plan);
+    hipStream_t st = D.st;
+    auto t_start = std::chrono::steady_clock::now();
+    double t_sigma = 0.0;
+    int n64 = 0, nmix = 0;
+    DevBuf Vb, Sb, work;
+    Vb.alloc((size_t)mv * n * 8);
+    Sb.alloc((size_t)mv * n * 8);
+    work.alloc((size_t)2 * n * 8);
+    double* Vp = Vb.as<double>();
+    double* Sp = Sb.as<double>();
+    auto Vk = [&](int k) { return Vp + (int64_t)k * n; };
+    auto Sk = [&](int k) { return Sp + (int64_t)k * n; };
+    double* res = work.as<double>();
+    const double* diag = plan->diag.as<double>();
+    int precision = prec0;
+    auto sigma = [&](int k) {
+      auto t0 = std::chrono::steady_clock::now();
+      plan->sigma(Vk(k), Sk(k), precision, st);
+      TCI_HIP(hipStreamSynchronize(st));
+      t_sigma += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+      if (precision == TCI_SIGMA_FP64) ++n64; else ++nmix;
+    };
+    // initial guess: unit vectors at the lowest diagonal elements (init_from_diag)
+    std::vector<double> hd(n);
+    TCI_HIP(hipMemcpy(hd.data(), diag, n * 8, hipMemcpyDeviceToHost));
+    std::vector<int64_t> idx(n);
+    std::iota(idx.begin(), idx.end(), 0);
+    std::partial_sort(idx.begin(), idx.begin() + roots, idx.end(), [&](int64_t a, int64_t b) {
+      return hd[a] < hd[b] || (hd[a] == hd[b] && a < b);
+    });
+    int m = 0;
+    for (int r = 0; r < roots; ++r) {
+      hipLaunchKernelGGL(unit_kernel, dim3((unsigned)std::min<int64_t>(cdiv(n, 256), 8192)), dim3(256), 0, st, Vk(m), n,
+                         idx[r]);
+      TCI_HIP(hipGetLastError());
+      ++m;
+    }
+    std::vector<int> root2idx(roots);
+    std::iota(root2idx.begin(), root2idx.end(), 0);
+    std::vector<char> conv(roots, 0);
+    std::vector<double> energy(roots, 0.0), dE(roots, 0.0);
+    std::vector<double> h((size_t)mv * mv, 0.0);  // row-major, leading dim mv
+    std::vector<double> w, Y, Yprev;
+    std::vector<char> restarts;
+    bool skip_conv_once = false;
+    int it = 0;
+    bool all_conv = false;
+    for (it = 1; it <= maxit; ++it) {
+      for (int r = 0; r < roots; ++r) {
+        if (conv[r]) continue;
+        const int k = root2idx[r];
+        sigma(k);
+        std::vector<double*> X;
+        for (int q = 0; q <= k; ++q) X.push_back(Vk(q));
+        std::vector<double> hv;
+        D.mdot(X, Sk(k), hv);
+        for (int q = 0; q <= k; ++q) h[(size_t)q * mv + k] = h[(size_t)k * mv + q] = hv[q];
+      }
+      std::vector<double> hm((size_t)m * m);
+      for (int a = 0; a < m; ++a)
+        for (int b = 0; b < m; ++b) hm[(size_t)a * m + b] = h[(size_t)a * mv + b];
+      jacobi_eigh(m, hm, w, Y);
+      for (int r = 0; r < roots; ++r) {
+        dE[r] = std::fabs(w[r] - energy[r]);
+        energy[r] = w[r];
+      }
+      bool restarted = false;
+      // mixed -> FP64 switch
+      bool do_switch = false;
+      if (precision != TCI_SIGMA_FP64 && it > 1) {
+        do_switch = true;
+        for (int r = 0; r < roots; ++r)
+          if (!conv[r] && !(dE[r] < prm->switch_tol)) do_switch = false;
+      }
+      const bool budget = (m > 4 * roots && m + 3 * roots > mv);
+      if (do_switch || budget) {
+        // collapse to the current (and, for a budget restart, previous) Ritz vectors
+        int newm = 0;
+        // build new basis in tmp slots at the end of V/S, then move
+        std::vector<std::vector<double>> cols;
+        for (int r = 0; r < roots; ++r) {
+          std::vector<double> c(m);
+          for (int a = 0; a < m; ++a) c[a] = Y[(size_t)a * m + r];
+          cols.push_back(c);
+          if (budget && !do_switch && !Yprev.empty()) {
+            const int mp = (int)std::lround(std::sqrt((double)Yprev.size()));
+            std::vector<double> cp(m, 0.0);
+            for (int a = 0; a < std::min(m, mp); ++a) cp[a] = Yprev[(size_t)a * mp + r];
+            cols.push_back(cp);
+          }
+        }
+        // new vectors go to slots m .. m+ncols-1 (space reserved by the budget check)
+        TCI_REQUIRE(m + (int)cols.size() <= mv, "subspace budget too small for a collapse");
+        std::vector<double*> Xv, Xs;
+        for (int q = 0; q < m; ++q) {
+          Xv.push_back(Vk(q));
+          Xs.push_back(Sk(q));
+        }
+        std::vector<int> slots;
+        for (auto& c : cols) {
+          const int slot = m + (int)slots.size();
+          D.lincomb(Vk(slot), 0.0, Xv, c);
+          std::vector<double*> prevV;
+          for (int sidx : slots) prevV.push_back(Vk(sidx));
+          if (!do_switch) D.lincomb(Sk(slot), 0.0, Xs, c);
+          // orthonormalise against already kept vectors (and carry sigma along)
+          bool ok = true;
+          if (!prevV.empty()) {
+            std::vector<double> ov;
+            D.mdot(prevV, Vk(slot), ov);
+            for (double& o : ov) o = -o;
+            D.lincomb(Vk(slot), 1.0, prevV, ov);
+            if (!do_switch) {
+              std::vector<double*> prevS;
+              for (int sidx : slots) prevS.push_back(Sk(sidx));
+              D.lincomb(Sk(slot), 1.0, prevS, ov);
+            }
+          }
+          const double nr = D.norm(Vk(slot));
+          if (nr < 1e-8) ok = false;
+          if (ok) {
+            D.scale(Vk(slot), 1.0 / nr);
+            if (!do_switch) D.scale(Sk(slot), 1.0 / nr);
+            slots.push_back(slot);
+          }
+        }
+        // move kept vectors to the front
+        for (size_t q = 0; q < slots.size(); ++q) {
+          TCI_HIP(hipMemcpyAsync(Vk((int)q), Vk(slots[q]), n * 8, hipMemcpyDeviceToDevice, st));
+          if (!do_switch) TCI_HIP(hipMemcpyAsync(Sk((int)q), Sk(slots[q]), n * 8, hipMemcpyDeviceToDevice, st));
+        }
+        newm = (int)slots.size();
+        m = newm;
+        if (do_switch) {
+          precision = TCI_SIGMA_FP64;
+          for (int q = 0; q < m; ++q) sigma(q);
+          skip_conv_once = true;
+        }
+        for (int a = 0; a < m; ++a) {
+          std::vector<double*> X;
+          for (int q = 0; q < m; ++q) X.push_back(Vk(q));
+          std::vector<double> hv;
+          D.mdot(X, Sk(a), hv);
+          for (int q = 0; q < m; ++q) h[(size_t)q * mv + a] = hv[q];
+        }
+        for (int a = 0; a < m; ++a)
+          for (int b = a + 1; b < m; ++b) {
+            const double s2 = 0.5 * (h[(size_t)a * mv + b] + h[(size_t)b * mv + a]);
+            h[(size_t)a * mv + b] = h[(size_t)b * mv + a] = s2;
+          }
+        hm.assign((size_t)m * m, 0.0);
+        for (int a = 0; a < m; ++a)
+          for (int b = 0; b < m; ++b) hm[(size_t)a * m + b] = h[(size_t)a * mv + b];
+        jacobi_eigh(m, hm, w, Y);
+        for (int r = 0; r < roots; ++r) energy[r] = w[r];
+        restarted = !do_switch;
+      }
+      restarts.push_back(restarted ? 1 : 0);
+      Yprev = Y;
+      if (prm->verbose) std::printf("  Iteration %3d has finished (m=%d, %s)\n", it, m, precision == TCI_SIGMA_FP64 ? "fp64" : "mixed");
+      const int mcur = m;
+      std::vector<double*> Xv, Xs;
+      for (int q = 0; q < mcur; ++q) {
+        Xv.push_back(Vk(q));
+        Xs.push_back(Sk(q));
+      }
+      for (int r = 0; r < roots; ++r) {
+        if (conv[r]) continue;
+        // residual r = sum_k y_k (sigma_k - E c_k)
+        std::vector<double*> X;
+        std::vector<double> c;
+        for (int q = 0; q < mcur; ++q) {
+          X.push_back(Sk(q));
+          c.push_back(Y[(size_t)q * mcur + r]);
+          X.push_back(Vk(q));
+          c.push_back(-energy[r] * Y[(size_t)q * mcur + r]);
+        }
+        D.lincomb(res, 0.0, X, c);
+        const double rn = D.norm(res);
+        if (prm->verbose)
+          std::printf("  %5d %5d %22.12f %15.8e %15.8e\n", it, r + 1, energy[r] + plan->e_ref, rn, dE[r]);
+        // is_converged (davidson.f90:564-602)
+        bool halt = false;
+        const bool in_restart = restarts.back() || (restarts.size() >= 2 && restarts[restarts.size() - 2]);
+        if (!in_restart && !skip_conv_once) {
+          if (rn < prm->norm_tol) halt = true;
+          else if (dE[r] < prm->energy_tol && it > 1 && precision == TCI_SIGMA_FP64) halt = true;
+        }
+        if (it == maxit) halt = true;
+        if (halt) {
+          conv[r] = 1;
+          continue;
+        }
+        hipLaunchKernelGGL(precond_kernel, dim3((unsigned)std::min<int64_t>(cdiv(n, 256), 8192)), dim3(256), 0, st, res,
+                           diag, energy[r], n);
+        TCI_HIP(hipGetLastError());
+        TCI_REQUIRE(m < mv, "Davidson subspace budget exhausted");
+        TCI_HIP(hipMemcpyAsync(Vk(m), res, n * 8, hipMemcpyDeviceToDevice, st));
+        std::vector<double*> prev;
+        for (int q = 0; q < m; ++q) prev.push_back(Vk(q));
+        D.orthonormalize(Vk(m), prev);
+        root2idx[r] = m;
+        ++m;
+      }
+      skip_conv_once = false;
+      all_conv = std::all_of(conv.begin(), conv.end(), [](char c) { return c != 0; });
+      if (all_conv) break;
+    }
+    // final vectors
+    for (int r = 0; r < roots; ++r) energies[r] = energy[r] + plan->e_ref;
+    if (vectors) {
+      const int mcur = (int)std::lround(std::sqrt((double)Y.size()));
+      std::vector<double*> X;
+      for (int q = 0; q < mcur; ++q) X.push_back(Vk(q));
+      for (int r = 0; r < roots; ++r) {
+        std::vector<double> c(mcur);
+        for (int q = 0; q < mcur; ++q) c[q] = Y[(size_t)q * mcur + r];
+        double* out = vectors + (int64_t)r * n;
+        D.lincomb(out, 0.0, X, c);
+        const double nr = D.norm(out);
+        if (nr > 0) D.scale(out, 1.0 / nr);
+      }
+    }
+    TCI_HIP(hipStreamSynchronize(st));
+    if (result) {
+      result->iterations = std::min(it, maxit);
+      result->sigma_fp64 = n64;
+      result->sigma_mixed = nmix;
+      result->converged = all_conv ? 1 : 0;
+      result->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
+      result->seconds_sigma = t_sigma;
+    }
+    return TCI_OK;
+  } catch (const tci::Error& e) {
+    tci::set_last_error(e.what());
+    return e.code;
+  } catch (const std::exception& e) {
+    tci::set_last_error(e.what());
+    return TCI_ERR_ARG;
+  }
+}

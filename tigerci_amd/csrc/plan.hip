@@ -1,0 +1,694 @@
+// Plan construction, sigma orchestration and the C ABI.
+//
+// Per sigma build (all device work on the caller's stream):
+//   unpack        CSF vector -> dense C^{ij}, Ct^{ij} = 2C - C^T, singles s
+//   ladder        R^P  = sum_J L^J C^P L^J                 (grouped MFMA launch)
+//   rterms        R   += c0 K^{ij} - (ki|bj) s_k^a - (kj|ai) s_k^b
+//   hole          R   += sum_kl (ki|lj) C^{kl}
+//   Y             Y[J][a][i] = sum_c L^J_ac s_i^c
+//   ring          R   += Z^{ij} + Z^{ji T} + (ac|bj) s_i^c + (bc|ai) s_j^c   (grouped gather-GEMM)
+//   singles       g = (2K - J) s + sum (ab|jc) Ct^{ij}_bc - sum (ji|kb) Ct^{jk}_ab
+//   reference     sigma_0 = sum_ij <K^{ij}, Ct^{ij}>
+//   pack          sigma = S^T R (CSF factors) + Fock diagonal
+// The formulas are those of the closed-shell MRSDCI sigma of htimesc
+// (htimesc.f90:46-147) written in pair-block form.
+#include "plan.hpp"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <mutex>
+#include <set>
+
+namespace tci {
+
+static thread_local std::string g_last_error;
+void set_last_error(const std::string& m) { g_last_error = m; }
+
+namespace {
+
+inline int64_t rec(int64_t p, int64_t q) {  // 0-based packed orbital-pair record
+  if (p < q) std::swap(p, q);
+  return p * (p + 1) / 2 + q;
+}
+
+Op op(const void* base, int64_t rs, int64_t cs, const int32_t* rmap = nullptr, const int32_t* cmap = nullptr,
+      bool f32 = false) {
+  Op o;
+  o.base = base;
+  o.rs = rs;
+  o.cs = cs;
+  o.rmap = rmap;
+  o.cmap = cmap;
+  o.f32 = f32 ? 1 : 0;
+  return o;
+}
+
+Seg seg(const Op& A, const Op& B, int K, double alpha) {
+  Seg s;
+  s.A = A;
+  s.B = B;
+  s.K = K;
+  s.alpha = alpha;
+  return s;
+}
+
+template <class T>
+void upload_now(DevBuf& b, const std::vector<T>& h) {
+  upload(b, h, nullptr);
+}
+
+}  // namespace
+}  // namespace tci
+
+using namespace tci;
+
+void tci_plan::build(const tci_problem& p) {
+  TCI_REQUIRE(p.n_int >= 1 && p.n_v >= 1 && p.n_chol >= 1, "n_int, n_v and n_chol must be positive");
+  TCI_REQUIRE(p.n_int <= 256 && p.n_v <= 65536, "orbital counts out of range");
+  TCI_REQUIRE(p.eps && p.chol, "eps and chol are required");
+  TCI_REQUIRE(p.chol_dtype == TCI_F64 || p.chol_dtype == TCI_F32, "chol_dtype must be TCI_F64 or TCI_F32");
+  TCI_REQUIRE(p.ladder_dtype == TCI_F64 || p.ladder_dtype == TCI_F32, "ladder_dtype must be TCI_F64 or TCI_F32");
+  TCI_REQUIRE(p.n_pairs >= 0 && (p.n_pairs == 0 || (p.pairs && p.dom_off && p.dom_idx)), "pair tables missing");
+  n_int = (int)p.n_int;
+  n_v = (int)p.n_v;
+  n_chol = p.n_chol;
+  n_orb = n_int + n_v;
+  npair_all = n_orb * (n_orb + 1) / 2;
+  n_pairs = (int)p.n_pairs;
+  lvv_f32 = p.ladder_dtype == TCI_F32;
+  h_eps.assign(p.eps, p.eps + n_orb);
+  h_pairs.assign(p.pairs, p.pairs + 2 * n_pairs);
+  h_dom_off.assign(p.dom_off, p.dom_off + n_pairs + 1);
+  TCI_REQUIRE(h_dom_off[0] == 0, "dom_off[0] must be 0");
+  h_dom_idx.assign(p.dom_idx, p.dom_idx + h_dom_off[n_pairs]);
+  pairid.assign((size_t)n_int * n_int, -1);
+  for (int P = 0; P < n_pairs; ++P) {
+    const int i = h_pairs[2 * P], j = h_pairs[2 * P + 1];
+    TCI_REQUIRE(0 <= j && j <= i && i < n_int, "pair walks must satisfy 0 <= j <= i < n_int");
+    TCI_REQUIRE(pairid[(size_t)i * n_int + j] < 0, "duplicate pair walk");
+    pairid[(size_t)i * n_int + j] = pairid[(size_t)j * n_int + i] = P;
+    const int64_t d = h_dom_off[P + 1] - h_dom_off[P];
+    TCI_REQUIRE(d >= 1 && d <= 512, "pair domain size must be within [1, 512]");
+    for (int64_t q = h_dom_off[P]; q < h_dom_off[P + 1]; ++q) {
+      TCI_REQUIRE(h_dom_idx[q] >= 0 && h_dom_idx[q] < n_v, "domain index out of range");
+      if (q > h_dom_off[P]) TCI_REQUIRE(h_dom_idx[q] > h_dom_idx[q - 1], "domains must be sorted and unique");
+    }
+    dmax = std::max<int>(dmax, (int)d);
+  }
+  // ---- CSF layout ----------------------------------------------------------
+  h_aa_off.resize(n_pairs);
+  h_ab_off.resize(n_pairs);
+  h_cd_off.resize(n_pairs + 1);
+  int64_t off = 1 + (int64_t)n_int * n_v;
+  sum_d2 = 0;
+  for (int P = 0; P < n_pairs; ++P) {
+    const int64_t d = h_dom_off[P + 1] - h_dom_off[P];
+    const int nsp = h_pairs[2 * P] == h_pairs[2 * P + 1] ? 1 : 2;
+    h_aa_off[P] = off;
+    off += d;
+    h_ab_off[P] = off;
+    off += nsp * (d * (d - 1) / 2);
+    h_cd_off[P] = sum_d2;
+    sum_d2 += d * d;
+  }
+  h_cd_off[n_pairs] = sum_d2;
+  n_csf = off;
+
+  hipStream_t st = nullptr;
+  upload_now(pairs, h_pairs);
+  upload_now(dom_off, h_dom_off);
+  upload_now(dom_idx, h_dom_idx);
+  upload_now(cd_off, h_cd_off);
+  upload_now(aa_off, h_aa_off);
+  upload_now(ab_off, h_ab_off);
+  upload_now(eps, h_eps);
+  tab.n_pairs = n_pairs;
+  tab.n_int = n_int;
+  tab.n_v = n_v;
+  tab.pairs = pairs.as<int32_t>();
+  tab.dom_off = dom_off.as<int64_t>();
+  tab.dom_idx = dom_idx.as<int32_t>();
+  tab.cd_off = cd_off.as<int64_t>();
+  tab.aa_off = aa_off.as<int64_t>();
+  tab.ab_off = ab_off.as<int64_t>();
+  tab.eps = eps.as<double>();
+
+  // ---- Cholesky records -> device layouts ----------------------------------
+  const int64_t niv = (int64_t)n_int * n_v;
+  DevBuf Lpm;
+  Lpm.alloc((size_t)npair_all * n_chol * sizeof(double));
+  {
+    const size_t esz = p.chol_dtype == TCI_F64 ? 8 : 4;
+    const size_t total = (size_t)npair_all * n_chol;
+    if (esz == 8) {
+      TCI_HIP(hipMemcpy(Lpm.p, p.chol, total * 8, hipMemcpyHostToDevice));
+    } else {
+      // convert FP32 records on the host in bounded chunks
+      const float* src = static_cast<const float*>(p.chol);
+      const size_t chunk = (size_t)1 << 24;
+      std::vector<double> buf(std::min(chunk, total));
+      for (size_t o = 0; o < total; o += chunk) {
+        const size_t m = std::min(chunk, total - o);
+        for (size_t q = 0; q < m; ++q) buf[q] = (double)src[o + q];
+        TCI_HIP(hipMemcpy(Lpm.as<double>() + o, buf.data(), m * 8, hipMemcpyHostToDevice));
+      }
+    }
+  }
+  Lvv.alloc((size_t)n_chol * n_v * n_v * (lvv_f32 ? 4 : 8));
+  launch_gather_Lvv(Lpm.as<double>(), n_chol, n_int, n_v, Lvv.p, lvv_f32, st);
+  Lov.alloc((size_t)n_chol * niv * 8);
+  launch_gather_Lov(Lpm.as<double>(), n_chol, n_int, n_v, Lov.as<double>(), st);
+
+  // ---- one-off integral assembly on MFMA -------------------------------------
+  {
+    std::vector<int32_t> rec_ov(niv), rec_oo((size_t)n_int * n_int), rec_vo(niv), rec_aa(n_v),
+        rec_pairs((size_t)n_int * (n_int + 1) / 2), rec_vv((size_t)n_v * n_v);
+    for (int i = 0; i < n_int; ++i)
+      for (int a = 0; a < n_v; ++a) {
+        rec_ov[(size_t)i * n_v + a] = (int32_t)rec(n_int + a, i);
+        rec_vo[(size_t)a * n_int + i] = (int32_t)rec(n_int + a, i);
+      }
+    for (int k = 0; k < n_int; ++k)
+      for (int i = 0; i < n_int; ++i) rec_oo[(size_t)k * n_int + i] = (int32_t)rec(k, i);
+    for (int k = 0; k < n_int; ++k)
+      for (int l = 0; l <= k; ++l) rec_pairs[(size_t)k * (k + 1) / 2 + l] = (int32_t)rec(k, l);
+    for (int a = 0; a < n_v; ++a) {
+      rec_aa[a] = (int32_t)rec(n_int + a, n_int + a);
+      for (int b = 0; b < n_v; ++b) rec_vv[(size_t)a * n_v + b] = (int32_t)rec(n_int + a, n_int + b);
+    }
+    TCI_REQUIRE(npair_all < ((int64_t)1 << 31), "too many orbital pairs");
+    DevBuf d_ov, d_oo, d_vo, d_aa, d_pr, d_vv;
+    upload_now(d_ov, rec_ov);
+    upload_now(d_oo, rec_oo);
+    upload_now(d_vo, rec_vo);
+    upload_now(d_aa, rec_aa);
+    upload_now(d_pr, rec_pairs);
+    upload_now(d_vv, rec_vv);
+    const double* L = Lpm.as<double>();
+    const int npr = n_int * (n_int + 1) / 2;
+    Kov.alloc((size_t)niv * niv * 8);
+    Jvv.alloc((size_t)npr * n_v * n_v * 8);
+    O4.alloc((size_t)n_int * n_int * n_int * n_int * 8);
+    O3.alloc((size_t)n_int * n_int * niv * 8);
+    vvd.alloc((size_t)n_v * n_v * 8);
+    vvx.alloc((size_t)n_v * n_v * 8);
+    GemmProgram asmb;
+    const int K = (int)n_chol;
+    // rows: records (k contiguous, cs = 1); cols: records transposed (rs = 1, cs = n_chol)
+    asmb.add(Kov.as<double>(), niv, (int)niv, (int)niv,
+             {seg(op(L, n_chol, 1, d_ov.as<int32_t>()), op(L, 1, n_chol, nullptr, d_ov.as<int32_t>()), K, 1.0)},
+             false);
+    asmb.add(Jvv.as<double>(), (int64_t)n_v * n_v, npr, n_v * n_v,
+             {seg(op(L, n_chol, 1, d_pr.as<int32_t>()), op(L, 1, n_chol, nullptr, d_vv.as<int32_t>()), K, 1.0)},
+             false);
+    asmb.add(O4.as<double>(), (int64_t)n_int * n_int, n_int * n_int, n_int * n_int,
+             {seg(op(L, n_chol, 1, d_oo.as<int32_t>()), op(L, 1, n_chol, nullptr, d_oo.as<int32_t>()), K, 1.0)},
+             false);
+    asmb.add(O3.as<double>(), niv, n_int * n_int, (int)niv,
+             {seg(op(L, n_chol, 1, d_oo.as<int32_t>()), op(L, 1, n_chol, nullptr, d_vo.as<int32_t>()), K, 1.0)},
+             false);
+    asmb.add(vvd.as<double>(), n_v, n_v, n_v,
+             {seg(op(L, n_chol, 1, d_aa.as<int32_t>()), op(L, 1, n_chol, nullptr, d_aa.as<int32_t>()), K, 1.0)},
+             false);
+    asmb.upload_tables(st);
+    asmb.launch(st);
+    launch_vv_exch(Lvv.p, lvv_f32, n_chol, n_v, vvx.as<double>(), st);
+    Mss.alloc((size_t)niv * niv * 8);
+    launch_build_mss(Kov.as<double>(), Jvv.as<double>(), n_int, n_v, Mss.as<double>(), st);
+    // E_ref = 2 sum_i eps_i - sum_ik [2 (ii|kk) - (ik|ik)]  (closed shell, Fock = diag(eps))
+    std::vector<double> o4((size_t)n_int * n_int * n_int * n_int);
+    TCI_HIP(hipMemcpy(o4.data(), O4.p, o4.size() * 8, hipMemcpyDeviceToHost));
+    auto O = [&](int a, int b, int c, int d) { return o4[(((size_t)a * n_int + b) * n_int + c) * n_int + d]; };
+    double e = 0.0;
+    for (int i = 0; i < n_int; ++i) {
+      e += 2.0 * h_eps[i];
+      for (int k = 0; k < n_int; ++k) e -= 2.0 * O(i, i, k, k) - O(i, k, i, k);
+    }
+    e_ref = e;
+    TCI_HIP(hipDeviceSynchronize());
+  }
+  Lpm.reset();
+
+  diag.alloc((size_t)n_csf * 8);
+  launch_diag(tab, dmax, Kov.as<double>(), Jvv.as<double>(), O4.as<double>(), vvd.as<double>(), vvx.as<double>(),
+              diag.as<double>(), st);
+
+  // ---- domain maps ---------------------------------------------------------
+  std::vector<int32_t> h_pos((size_t)std::max(n_pairs, 1) * n_v, -1);
+  for (int Q = 0; Q < n_pairs; ++Q)
+    for (int64_t q = h_dom_off[Q]; q < h_dom_off[Q + 1]; ++q)
+      h_pos[(size_t)Q * n_v + h_dom_idx[q]] = (int32_t)(q - h_dom_off[Q]);
+  upload_now(pos, h_pos);
+  // composite maps D_P -> position in D_Q for every (P,Q) sharing an internal index
+  std::map<std::pair<int, int>, int64_t> xoff;
+  std::vector<int32_t> h_xmap;
+  auto need = [&](int P, int Q) {
+    auto key = std::make_pair(P, Q);
+    if (xoff.count(key)) return;
+    xoff[key] = (int64_t)h_xmap.size();
+    for (int64_t q = h_dom_off[P]; q < h_dom_off[P + 1]; ++q) h_xmap.push_back(h_pos[(size_t)Q * n_v + h_dom_idx[q]]);
+  };
+  for (int P = 0; P < n_pairs; ++P) {
+    const int i = h_pairs[2 * P], j = h_pairs[2 * P + 1];
+    for (int k = 0; k < n_int; ++k) {
+      int Q1 = pairid[(size_t)i * n_int + k], Q2 = pairid[(size_t)j * n_int + k];
+      if (Q1 >= 0) need(P, Q1);
+      if (Q2 >= 0) need(P, Q2);
+    }
+  }
+  upload_now(xmap, h_xmap);
+  auto XM = [&](int P, int Q) { return xmap.as<int32_t>() + xoff.at(std::make_pair(P, Q)); };
+  auto DL = [&](int P) { return dom_idx.as<int32_t>() + h_dom_off[P]; };
+  auto dsz = [&](int P) { return (int)(h_dom_off[P + 1] - h_dom_off[P]); };
+  // intersections for the 4-internal term, sorted by position in P
+  {
+    std::vector<int64_t> ioff((size_t)n_pairs * n_pairs + 1, 0);
+    std::vector<int32_t> is;
+    for (int P = 0; P < n_pairs; ++P)
+      for (int Q = 0; Q < n_pairs; ++Q) {
+        ioff[(size_t)P * n_pairs + Q] = (int64_t)is.size() / 2;
+        int64_t a = h_dom_off[P], b = h_dom_off[Q];
+        while (a < h_dom_off[P + 1] && b < h_dom_off[Q + 1]) {
+          if (h_dom_idx[a] < h_dom_idx[b]) ++a;
+          else if (h_dom_idx[a] > h_dom_idx[b]) ++b;
+          else {
+            is.push_back((int32_t)(a - h_dom_off[P]));
+            is.push_back((int32_t)(b - h_dom_off[Q]));
+            ++a;
+            ++b;
+          }
+        }
+      }
+    ioff[(size_t)n_pairs * n_pairs] = (int64_t)is.size() / 2;
+    upload_now(isect_off, ioff);
+    upload_now(isect, is);
+    // inverse lists: virtual a -> (Q, position)
+    std::vector<int64_t> ivo(n_v + 1, 0);
+    for (int64_t q = 0; q < (int64_t)h_dom_idx.size(); ++q) ivo[h_dom_idx[q] + 1]++;
+    for (int a = 0; a < n_v; ++a) ivo[a + 1] += ivo[a];
+    std::vector<int32_t> iq(h_dom_idx.size()), ip(h_dom_idx.size());
+    std::vector<int64_t> fill(ivo.begin(), ivo.end() - 1);
+    for (int Q = 0; Q < n_pairs; ++Q)
+      for (int64_t q = h_dom_off[Q]; q < h_dom_off[Q + 1]; ++q) {
+        const int a = h_dom_idx[q];
+        iq[fill[a]] = Q;
+        ip[fill[a]] = (int32_t)(q - h_dom_off[Q]);
+        fill[a]++;
+      }
+    upload_now(inv_off, ivo);
+    upload_now(inv_q, iq);
+    upload_now(inv_pos, ip);
+    std::vector<int64_t> go(2 * (size_t)n_pairs + 1, 0);
+    for (int Q = 0; Q < n_pairs; ++Q) {
+      go[2 * Q + 1] = go[2 * Q] + (int64_t)n_int * dsz(Q);
+      go[2 * Q + 2] = go[2 * Q + 1] + (int64_t)n_int * dsz(Q);
+    }
+    upload_now(gpart_off, go);
+    gpart.alloc((size_t)go[2 * n_pairs] * 8);
+  }
+
+  // ---- work buffers ----------------------------------------------------------
+  C.alloc((size_t)sum_d2 * 8);
+  Ct.alloc((size_t)sum_d2 * 8);
+  R.alloc((size_t)sum_d2 * 8);
+  s.alloc((size_t)niv * 8);
+  g.alloc((size_t)niv * 8);
+  gss.alloc((size_t)niv * 8);
+  Y.alloc((size_t)n_chol * niv * 8);
+  W.alloc((size_t)n_chol * niv * 8);
+  part.alloc((size_t)std::max(n_pairs, 1) * 8);
+  sig0.alloc(8);
+  n_split = (int)std::min<int64_t>(std::max<int64_t>(1, n_chol / 64), 64);
+  gW.alloc((size_t)n_split * niv * 8);
+
+  // ---- grouped task tables -----------------------------------------------------
+  const double* Kp = Kov.as<double>();
+  const double* Jp = Jvv.as<double>();
+  const double* Cp = C.as<double>();
+  const double* Ctp = Ct.as<double>();
+  auto Kop = [&](int k, int l, const int32_t* rm, const int32_t* cm) {  // K^{kl}[rows, cols] = (ka|lb)
+    return op(Kp + (int64_t)k * n_v * niv + (int64_t)l * n_v, niv, 1, rm, cm);
+  };
+  auto Jop = [&](int k, int l, const int32_t* rm, const int32_t* cm) {  // J^{kl}[rows, cols] = (kl|ab)
+    return op(Jp + rec(k, l) * n_v * n_v, n_v, 1, rm, cm);
+  };
+  // dense block of strong walk Q seen as the ordered pair (u,v): element [r][c]
+  // of X^{uv} = (u >= v) ? X_Q[r][c] : X_Q[c][r]
+  auto blk = [&](const double* X, int Q, bool transposed, const int32_t* rm, const int32_t* cm) {
+    const int d = dsz(Q);
+    const double* b = X + h_cd_off[Q];
+    return transposed ? op(b, 1, d, rm, cm) : op(b, d, 1, rm, cm);
+  };
+  ring.clear();
+  for (int P = 0; P < n_pairs; ++P) {
+    const int i = h_pairs[2 * P], j = h_pairs[2 * P + 1];
+    const int d = dsz(P);
+    const int32_t* DP = DL(P);
+    std::vector<Seg> sg;
+    for (int k = 0; k < n_int; ++k) {
+      const int Q1 = pairid[(size_t)i * n_int + k];
+      if (Q1 >= 0) {  // Z^{ij}: Ct^{ik} K^{kj} - C^{ik} J^{kj} - J^{kj} C^{ik}
+        const int dq = dsz(Q1);
+        const bool tr = i < k;
+        const int32_t* m = XM(P, Q1);
+        sg.push_back(seg(blk(Ctp, Q1, tr, m, nullptr), Kop(k, j, DL(Q1), DP), dq, 1.0));
+        sg.push_back(seg(blk(Cp, Q1, tr, m, nullptr), Jop(k, j, DL(Q1), DP), dq, -1.0));
+        sg.push_back(seg(Jop(k, j, DP, DL(Q1)), blk(Cp, Q1, tr, nullptr, m), dq, -1.0));
+      }
+      const int Q2 = pairid[(size_t)j * n_int + k];
+      if (Q2 >= 0) {  // (Z^{ji})^T: K^{ik} Ct^{kj} - J^{ki} C^{kj} - C^{kj} J^{ki}
+        const int dq = dsz(Q2);
+        // X^{kj}[c][b] = X^{jk}[b][c]: element [c][b] of the ordered pair (k,j)
+        const bool tr = k < j;
+        const int32_t* m = XM(P, Q2);
+        sg.push_back(seg(Kop(i, k, DP, DL(Q2)), blk(Ctp, Q2, tr, nullptr, m), dq, 1.0));
+        sg.push_back(seg(Jop(k, i, DP, DL(Q2)), blk(Cp, Q2, tr, nullptr, m), dq, -1.0));
+        sg.push_back(seg(blk(Cp, Q2, tr, m, nullptr), Jop(k, i, DL(Q2), DP), dq, -1.0));
+      }
+    }
+    // 3-external singles: X^{ij}_ab + X^{ji}_ba, X^{ij}_ab = sum_J Y[J][a][i] L^J_{jb}
+    const double* Yp = Y.as<double>();
+    const double* Lo = Lov.as<double>();
+    sg.push_back(seg(op(Yp + i, n_int, niv, DP, nullptr), op(Lo + (int64_t)j * n_v, niv, 1, nullptr, DP), (int)n_chol, 1.0));
+    sg.push_back(seg(op(Lo + (int64_t)i * n_v, 1, niv, DP, nullptr), op(Yp + j, niv, n_int, nullptr, DP), (int)n_chol, 1.0));
+    ring.add(R.as<double>() + h_cd_off[P], d, d, d, sg, true);
+  }
+  ring.upload_tables(st);
+  // Y[(J,a)][i] = sum_c L^J_ac s_i^c
+  progY.clear();
+  progY.add(Y.as<double>(), n_int, (int)(n_chol * n_v), n_int,
+            {seg(op(Lvv.p, n_v, 1, nullptr, nullptr, lvv_f32), op(s.as<double>(), 1, n_v), n_v, 1.0)}, false);
+  progY.upload_tables(st);
+  // W[i][J][b] = sum_j sum_c Ct^{ij}[pos(b)][c] L^J_{j D[c]}
+  progW.clear();
+  for (int i = 0; i < n_int; ++i) {
+    std::vector<Seg> sg;
+    for (int j = 0; j < n_int; ++j) {
+      const int Q = pairid[(size_t)i * n_int + j];
+      if (Q < 0) continue;
+      const int dq = dsz(Q);
+      const bool tr = i < j;  // Ct^{ij} = Ct_Q^T when i < j
+      const double* b = Ctp + h_cd_off[Q];
+      // B(c, b) = Ct^{ij}[pos(b)][c]
+      Op B = tr ? op(b, dq, 1, nullptr, pos.as<int32_t>() + (size_t)Q * n_v)
+                : op(b, 1, dq, nullptr, pos.as<int32_t>() + (size_t)Q * n_v);
+      sg.push_back(seg(op(Lov.as<double>() + (int64_t)j * n_v, niv, 1, nullptr, DL(Q)), B, dq, 1.0));
+    }
+    progW.add(W.as<double>() + (int64_t)i * n_chol * n_v, n_v, (int)n_chol, n_v, sg, false);
+  }
+  progW.upload_tables(st);
+  // gW[s][i][a] = sum_{J in split s} sum_b W[i][J][b] L^J_ba
+  progGW.clear();
+  for (int sp = 0; sp < n_split; ++sp) {
+    const int64_t J0 = n_chol * sp / n_split, J1 = n_chol * (sp + 1) / n_split;
+    const void* Lb = lvv_f32 ? (const void*)(Lvv.as<float>() + J0 * n_v * n_v) : (const void*)(Lvv.as<double>() + J0 * n_v * n_v);
+    progGW.add(gW.as<double>() + (int64_t)sp * niv, n_v, n_int, n_v,
+               {seg(op(W.as<double>() + J0 * n_v, n_chol * n_v, 1), op(Lb, n_v, 1, nullptr, nullptr, lvv_f32),
+                    (int)((J1 - J0) * n_v), 1.0)},
+               false);
+  }
+  progGW.upload_tables(st);
+  // gss = (2K - J) s
+  progGss.clear();
+  progGss.add(gss.as<double>(), 1, (int)niv, 1, {seg(op(Mss.as<double>(), niv, 1), op(s.as<double>(), 1, 0), (int)niv, 1.0)},
+              false);
+  progGss.upload_tables(st);
+
+  // ---- ladder work items -------------------------------------------------------
+  {
+    int64_t panels = 0;
+    for (int P = 0; P < n_pairs; ++P) panels += cdiv(dsz(P), 64);
+    const int64_t target = 2048;
+    lad_chunks = (int)std::max<int64_t>(1, std::min<int64_t>(std::min<int64_t>(16, n_chol), cdiv(target, std::max<int64_t>(panels, 1))));
+    std::vector<LadderItem> items;
+    std::vector<double> cost;
+    for (int P = 0; P < n_pairs; ++P) {
+      const int d = dsz(P);
+      const double dp = 16.0 * cdiv(d, 16);
+      for (int r0 = 0; r0 < d; r0 += 64)
+        for (int c = 0; c < lad_chunks; ++c) {
+          LadderItem it{};
+          it.P = P;
+          it.r0 = r0;
+          it.J0 = (int)(n_chol * c / lad_chunks);
+          it.J1 = (int)(n_chol * (c + 1) / lad_chunks);
+          it.chunk = c;
+          items.push_back(it);
+          cost.push_back(std::min(64, d - r0) * dp * dp * (it.J1 - it.J0));
+        }
+    }
+    ladder.build(items, cost, dmax, st);
+    lad_part.alloc((size_t)lad_chunks * sum_d2 * 8);
+  }
+
+  // ---- accounting ---------------------------------------------------------------
+  flops_ladder = 0.0;
+  double f_ring = 0.0, bytes = 0.0;
+  for (int P = 0; P < n_pairs; ++P) {
+    const double d = dsz(P);
+    flops_ladder += 4.0 * d * d * d * (double)n_chol;
+    bytes += (double)n_chol * d * d * (lvv_f32 ? 4 : 8) + 2.0 * 8.0 * d * d;
+    const int i = h_pairs[2 * P], j = h_pairs[2 * P + 1];
+    for (int k = 0; k < n_int; ++k) {
+      const int Q1 = pairid[(size_t)i * n_int + k], Q2 = pairid[(size_t)j * n_int + k];
+      if (Q1 >= 0) f_ring += 3.0 * 2.0 * d * d * dsz(Q1);
+      if (Q2 >= 0) f_ring += 3.0 * 2.0 * d * d * dsz(Q2);
+    }
+    f_ring += 2.0 * 2.0 * d * d * (double)n_chol;
+  }
+  bytes_ladder = bytes;
+  flops_sigma = flops_ladder + f_ring + 2.0 * (double)n_chol * n_v * niv * 3.0 + 2.0 * (double)niv * niv;
+  TCI_HIP(hipDeviceSynchronize());
+  device_bytes = 0;
+  for (DevBuf* b : {&pairs, &dom_off, &dom_idx, &cd_off, &aa_off, &ab_off, &eps, &pos, &xmap, &isect_off, &isect,
+                    &inv_off, &inv_q, &inv_pos, &gpart_off, &Lvv, &Lov, &Kov, &Jvv, &Mss, &O4, &O3, &vvd, &vvx,
+                    &diag, &C, &Ct, &R, &s, &g, &gss, &gW, &Y, &W, &gpart, &part, &sig0, &lad_part})
+    device_bytes += (int64_t)b->bytes;
+}
+
+tci_plan::~tci_plan() {
+  for (hipEvent_t& e : ev)
+    if (e) (void)hipEventDestroy(e);
+}
+
+void tci_plan::unpack_and_ladder(const double* x, int precision, hipStream_t st) {
+  launch_unpack(tab, dmax, x, C.as<double>(), Ct.as<double>(), s.as<double>(), st);
+  if (profiling) TCI_HIP(hipEventRecord(ev[2], st));
+  if (precision == TCI_SIGMA_FP64) {
+    LadderArgs a;
+    a.dom_off = dom_off.as<int64_t>();
+    a.dom_idx = dom_idx.as<int32_t>();
+    a.cd_off = cd_off.as<int64_t>();
+    a.Cd = C.as<double>();
+    a.Lvv = Lvv.p;
+    a.lvv_f32 = lvv_f32;
+    a.n_v = n_v;
+    a.partial = lad_part.as<double>();
+    a.chunk_stride = sum_d2;
+    ladder.launch(a, st);
+  } else {
+    mixed_ladder(*this, precision, st);
+  }
+  if (profiling) TCI_HIP(hipEventRecord(ev[3], st));
+  ladder_reduce(lad_part.as<double>(), sum_d2, lad_chunks, R.as<double>(), sum_d2, st);
+}
+
+void tci_plan::sigma(const double* x, double* y, int flags, hipStream_t st) {
+  const int precision = flags & 0xff;
+  TCI_REQUIRE(precision == TCI_SIGMA_FP64 || precision == TCI_SIGMA_BF16X3 || precision == TCI_SIGMA_FP16X3,
+              "unknown precision");
+  if (profiling) TCI_HIP(hipEventRecord(ev[0], st));
+  unpack_and_ladder(x, precision, st);
+  launch_rterms(tab, dmax, x, s.as<double>(), Kov.as<double>(), O3.as<double>(), R.as<double>(), st);
+  HoleTab H{isect_off.as<int64_t>(), isect.as<int32_t>()};
+  launch_hole(tab, H, dmax, C.as<double>(), O4.as<double>(), R.as<double>(), st);
+  progY.launch(st);
+  ring.launch(st);
+  progGss.launch(st);
+  progW.launch(st);
+  progGW.launch(st);
+  launch_s3int_singles(tab, gpart_off.as<int64_t>(), Ct.as<double>(), O3.as<double>(), gpart.as<double>(), st);
+  InvTab I{inv_off.as<int64_t>(), inv_q.as<int32_t>(), inv_pos.as<int32_t>()};
+  launch_singles_reduce(tab, I, gpart_off.as<int64_t>(), gpart.as<double>(), gss.as<double>(), gW.as<double>(), n_split,
+                        g.as<double>(), st);
+  launch_ref_dot(tab, Kov.as<double>(), Ct.as<double>(), part.as<double>(), sig0.as<double>(), st);
+  launch_pack(tab, dmax, x, R.as<double>(), g.as<double>(), sig0.as<double>(), e_ref, flags, y, st);
+  if (profiling) {
+    TCI_HIP(hipEventRecord(ev[1], st));
+    TCI_HIP(hipEventSynchronize(ev[1]));
+    float ms = 0.f;
+    TCI_HIP(hipEventElapsedTime(&ms, ev[0], ev[1]));
+    prof_sigma_ms += ms;
+    ++prof_sigma_n;
+    TCI_HIP(hipEventElapsedTime(&ms, ev[2], ev[3]));
+    prof_ladder_ms += ms;
+    ++prof_ladder_n;
+  }
+}
+
+// ============================================================================
+// C ABI
+// ============================================================================
+#define TCI_GUARD_BEGIN try {
+#define TCI_GUARD_END                                   \
+  }                                                     \
+  catch (const tci::Error& e) {                         \
+    tci::set_last_error(e.what());                      \
+    return e.code;                                      \
+  }                                                     \
+  catch (const std::exception& e) {                     \
+    tci::set_last_error(e.what());                      \
+    return TCI_ERR_ARG;                                 \
+  }                                                     \
+  catch (...) {                                         \
+    tci::set_last_error("unknown error");               \
+    return TCI_ERR_ARG;                                 \
+  }
+
+extern "C" {
+
+const char* tci_last_error(void) { return tci::g_last_error.c_str(); }
+
+int tci_plan_create(const tci_problem* prob, tci_plan** out) {
+  TCI_GUARD_BEGIN
+  TCI_REQUIRE(prob && out, "null argument");
+  *out = nullptr;
+  tci_plan* p = new tci_plan();
+  try {
+    p->build(*prob);
+  } catch (...) {
+    delete p;
+    throw;
+  }
+  *out = p;
+  return TCI_OK;
+  TCI_GUARD_END
+}
+
+void tci_plan_destroy(tci_plan* plan) {
+  if (plan) {
+    (void)hipDeviceSynchronize();
+    delete plan;
+  }
+}
+
+int tci_plan_get_info(const tci_plan* plan, tci_plan_info* info) {
+  TCI_GUARD_BEGIN
+  TCI_REQUIRE(plan && info, "null argument");
+  info->n_csf = plan->n_csf;
+  info->n_ladder_items = plan->ladder.n_items;
+  info->e_ref = plan->e_ref;
+  info->flops_ladder = plan->flops_ladder;
+  info->flops_sigma = plan->flops_sigma;
+  info->bytes_ladder = plan->bytes_ladder;
+  info->device_bytes = plan->device_bytes;
+  return TCI_OK;
+  TCI_GUARD_END
+}
+
+int tci_sigma_apply(tci_plan* plan, const double* c, double* sigma, int64_t n, int32_t flags, void* stream) {
+  TCI_GUARD_BEGIN
+  TCI_REQUIRE(plan && c && sigma, "null argument");
+  TCI_REQUIRE(n == plan->n_csf, "vector length does not match the plan's CSF count");
+  TCI_REQUIRE((const void*)c != (const void*)sigma, "c and sigma must not alias");
+  plan->sigma(c, sigma, flags, static_cast<hipStream_t>(stream));
+  return TCI_OK;
+  TCI_GUARD_END
+}
+
+int tci_htimesc(tci_plan* plan, const double* civec, double* sigmavec, int64_t n) {
+  TCI_GUARD_BEGIN
+  TCI_REQUIRE(plan && civec && sigmavec, "null argument");
+  TCI_REQUIRE(n == plan->n_csf, "vector length does not match the plan's CSF count");
+  DevBuf dc, ds;
+  dc.alloc((size_t)n * 8);
+  ds.alloc((size_t)n * 8);
+  TCI_HIP(hipMemcpy(dc.p, civec, (size_t)n * 8, hipMemcpyHostToDevice));
+  TCI_HIP(hipMemcpy(ds.p, sigmavec, (size_t)n * 8, hipMemcpyHostToDevice));
+  plan->sigma(dc.as<double>(), ds.as<double>(), TCI_SIGMA_FP64 | TCI_ACCUMULATE | TCI_ADD_EREF, nullptr);
+  TCI_HIP(hipMemcpy(sigmavec, ds.p, (size_t)n * 8, hipMemcpyDeviceToHost));
+  return TCI_OK;
+  TCI_GUARD_END
+}
+
+int tci_diagonal(tci_plan* plan, double* diag, int64_t n, void* stream) {
+  TCI_GUARD_BEGIN
+  TCI_REQUIRE(plan && diag, "null argument");
+  TCI_REQUIRE(n == plan->n_csf, "vector length does not match the plan's CSF count");
+  TCI_HIP(hipMemcpyAsync(diag, plan->diag.p, (size_t)n * 8, hipMemcpyDeviceToDevice, static_cast<hipStream_t>(stream)));
+  return TCI_OK;
+  TCI_GUARD_END
+}
+
+int tci_profile_enable(tci_plan* plan, int32_t on) {
+  TCI_GUARD_BEGIN
+  TCI_REQUIRE(plan, "null argument");
+  for (hipEvent_t& e : plan->ev)
+    if (!e) TCI_HIP(hipEventCreate(&e));
+  plan->profiling = on != 0;
+  plan->prof_ladder_ms = plan->prof_sigma_ms = 0.0;
+  plan->prof_ladder_n = plan->prof_sigma_n = 0;
+  return TCI_OK;
+  TCI_GUARD_END
+}
+
+int tci_profile_read(const tci_plan* plan, double* ladder_ms, int64_t* ladder_launches, double* sigma_ms,
+                     int64_t* sigma_calls) {
+  TCI_GUARD_BEGIN
+  TCI_REQUIRE(plan, "null argument");
+  if (ladder_ms) *ladder_ms = plan->prof_ladder_ms;
+  if (ladder_launches) *ladder_launches = plan->prof_ladder_n;
+  if (sigma_ms) *sigma_ms = plan->prof_sigma_ms;
+  if (sigma_calls) *sigma_calls = plan->prof_sigma_n;
+  return TCI_OK;
+  TCI_GUARD_END
+}
+
+int tci_device_alloc(int64_t bytes, void** ptr) {
+  TCI_GUARD_BEGIN
+  TCI_REQUIRE(ptr && bytes >= 0, "bad argument");
+  *ptr = nullptr;
+  hipError_t e = hipMalloc(ptr, (size_t)std::max<int64_t>(bytes, 8));
+  if (e != hipSuccess) throw tci::Error(TCI_ERR_OOM, std::string("hipMalloc: ") + hipGetErrorString(e));
+  return TCI_OK;
+  TCI_GUARD_END
+}
+
+int tci_device_free(void* ptr) {
+  TCI_GUARD_BEGIN
+  if (ptr) TCI_HIP(hipFree(ptr));
+  return TCI_OK;
+  TCI_GUARD_END
+}
+
+int tci_memcpy_h2d(void* dst, const void* src, int64_t bytes) {
+  TCI_GUARD_BEGIN
+  TCI_HIP(hipMemcpy(dst, src, (size_t)bytes, hipMemcpyHostToDevice));
+  return TCI_OK;
+  TCI_GUARD_END
+}
+
+int tci_memcpy_d2h(void* dst, const void* src, int64_t bytes) {
+  TCI_GUARD_BEGIN
+  TCI_HIP(hipMemcpy(dst, src, (size_t)bytes, hipMemcpyDeviceToHost));
+  return TCI_OK;
+  TCI_GUARD_END
+}
+
+int tci_memcpy_d2d(void* dst, const void* src, int64_t bytes) {
+  TCI_GUARD_BEGIN
+  TCI_HIP(hipMemcpy(dst, src, (size_t)bytes, hipMemcpyDeviceToDevice));
+  return TCI_OK;
+  TCI_GUARD_END
+}
+
+int tci_device_synchronize(void) {
+  TCI_GUARD_BEGIN
+  TCI_HIP(hipDeviceSynchronize());
+  return TCI_OK;
+  TCI_GUARD_END
+}
+
+}  // extern "C"

@@ -1,0 +1,58 @@
+// Plan object behind the C ABI (see plan.hip).
+#pragma once
+
+#include <map>
+
+#include "common.hpp"
+#include "ladder.hpp"
+#include "terms.hpp"
+
+struct tci_plan {
+  // sizes
+  int n_int = 0, n_v = 0, n_pairs = 0, dmax = 0;
+  int64_t n_chol = 0, n_orb = 0, npair_all = 0, n_csf = 0, sum_d2 = 0;
+  double e_ref = 0.0;
+  bool lvv_f32 = false;
+  // host tables
+  std::vector<int32_t> h_pairs, h_dom_idx;
+  std::vector<int64_t> h_dom_off, h_cd_off, h_aa_off, h_ab_off;
+  std::vector<double> h_eps;
+  std::vector<int32_t> pairid;   // [i*n_int+j] -> strong walk index or -1 (both orders)
+  // device tables
+  tci::DevBuf pairs, dom_off, dom_idx, cd_off, aa_off, ab_off, eps;
+  tci::DevBuf pos;               // [Q][n_v] position of virtual in D_Q or -1
+  tci::DevBuf xmap;              // composite maps D_P -> positions in D_Q
+  tci::DevBuf isect_off, isect, inv_off, inv_q, inv_pos, gpart_off;
+  // integrals (device)
+  tci::DevBuf Lvv, Lov, Kov, Jvv, Mss, O4, O3, vvd, vvx, diag;
+  // mixed-precision copies of L_vv (hi/lo split)
+  tci::DevBuf Lvv_hi, Lvv_lo;
+  // work buffers
+  tci::DevBuf C, Ct, R, s, g, gss, gW, Y, W, gpart, part, sig0, lad_part;
+  tci::DevBuf C_hi, C_lo;
+  int n_split = 1;
+  int lad_chunks = 1;
+  // programs
+  tci::GemmProgram ring, progY, progW, progGW, progGss;
+  tci::LadderLaunch ladder;
+  tci::PairTab tab{};
+  // accounting
+  double flops_ladder = 0.0, flops_sigma = 0.0, bytes_ladder = 0.0;
+  int64_t device_bytes = 0;
+  // event timing (tci_profile_enable)
+  bool profiling = false;
+  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  double prof_ladder_ms = 0.0, prof_sigma_ms = 0.0;
+  int64_t prof_ladder_n = 0, prof_sigma_n = 0;
+  ~tci_plan();
+
+  void build(const tci_problem& p);
+  void sigma(const double* x, double* y, int flags, hipStream_t st);
+  void unpack_and_ladder(const double* x, int precision, hipStream_t st);
+};
+
+namespace tci {
+// mixed-precision ladder (ladder_mixed.hip)
+void mixed_prepare_L(tci_plan& P, hipStream_t st);
+void mixed_ladder(tci_plan& P, int precision, hipStream_t st);
+}  // namespace tci

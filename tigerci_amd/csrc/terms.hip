@@ -1,0 +1,505 @@
+// Elementwise / gather kernels of the sigma build.
+//
+// Dense pair blocks: for strong walk P = (i,j), i >= j, with domain D (sorted
+// virtuals, d = |D|), C^{ij} is a d x d row-major block whose row index pairs
+// with internal orbital i.  From the CSF vector (scepper, scepmaker_mod.f90:
+// 121-212):  i > j:  C = S+(x_ab+, x_aa)/2 + S-(x_ab-)/(2 sqrt3)
+//            i = j:  C = S+(x_ab, x_aa)/sqrt2
+// S+ puts v at (r,c) and (c,r) and sqrt2*w on the diagonal; S- puts +v above
+// and -v below the diagonal.  Packing applies the adjoints (unscepper,
+// scepmaker_mod.f90:633-733) with factors 1 (i>j, singlet), sqrt3 (i>j,
+// triplet), 1/sqrt2 (i=j).  Packed index of (hi, lo), hi > lo: hi(hi-1)/2+lo.
+#include "terms.hpp"
+
+#include <cmath>
+
+namespace tci {
+
+namespace {
+
+constexpr double SQRT2 = 1.41421356237309504880;
+constexpr double INV_SQRT2 = 0.70710678118654752440;
+constexpr double SQRT3 = 1.73205080756887729353;
+constexpr double INV_2SQRT3 = 0.28867513459481288225;  // 1/(2 sqrt3)
+
+__device__ __forceinline__ void tri_decode(int64_t idx, int& hi, int& lo) {
+  // largest hi with hi(hi-1)/2 <= idx
+  int h = (int)((1.0 + sqrt(1.0 + 8.0 * (double)idx)) * 0.5);
+  while ((int64_t)h * (h - 1) / 2 > idx) --h;
+  while ((int64_t)(h + 1) * h / 2 <= idx) ++h;
+  hi = h;
+  lo = (int)(idx - (int64_t)h * (h - 1) / 2);
+}
+
+__global__ void unpack_kernel(PairTab T, const double* __restrict__ x, double* __restrict__ C,
+                              double* __restrict__ Ct, double* __restrict__ s) {
+  const int P = blockIdx.y;
+  if (P == T.n_pairs) {  // singles
+    const int64_t n = (int64_t)T.n_int * T.n_v;
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x)
+      s[e] = x[1 + e] * INV_SQRT2;
+    return;
+  }
+  const int i = T.pairs[2 * P], j = T.pairs[2 * P + 1];
+  const int d = (int)(T.dom_off[P + 1] - T.dom_off[P]);
+  const int64_t dd = (int64_t)d * d;
+  const int64_t m2 = (int64_t)d * (d - 1) / 2;
+  const double* xa = x + T.aa_off[P];
+  const double* xb = x + T.ab_off[P];
+  double* Cb = C + T.cd_off[P];
+  double* Ctb = Ct + T.cd_off[P];
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < dd; e += (int64_t)gridDim.x * blockDim.x) {
+    const int r = (int)(e / d), c = (int)(e - (int64_t)r * d);
+    double crc, ccr;
+    if (r == c) {
+      crc = ccr = (i == j) ? xa[r] : xa[r] * INV_SQRT2;
+    } else {
+      const int hi = r > c ? r : c, lo = r > c ? c : r;
+      const int64_t idx = (int64_t)hi * (hi - 1) / 2 + lo;
+      if (i == j) {
+        crc = ccr = xb[idx] * INV_SQRT2;
+      } else {
+        const double vs = 0.5 * xb[idx];
+        const double q = xb[m2 + idx] * INV_2SQRT3;
+        crc = r < c ? vs + q : vs - q;
+        ccr = r < c ? vs - q : vs + q;
+      }
+    }
+    Cb[e] = crc;
+    Ctb[e] = 2.0 * crc - ccr;
+  }
+}
+
+__global__ void rterms_kernel(PairTab T, const double* __restrict__ x, const double* __restrict__ s,
+                              const double* __restrict__ Kov, const double* __restrict__ O3,
+                              double* __restrict__ R) {
+  const int P = blockIdx.y;
+  const int i = T.pairs[2 * P], j = T.pairs[2 * P + 1];
+  const int64_t off = T.dom_off[P];
+  const int d = (int)(T.dom_off[P + 1] - off);
+  const int64_t dd = (int64_t)d * d;
+  const int n = T.n_int, nv = T.n_v;
+  const int64_t niv = (int64_t)n * nv;
+  const double c0 = x[0];
+  double* Rb = R + T.cd_off[P];
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < dd; e += (int64_t)gridDim.x * blockDim.x) {
+    const int r = (int)(e / d), c = (int)(e - (int64_t)r * d);
+    const int a = T.dom_idx[off + r], b = T.dom_idx[off + c];
+    double v = c0 * Kov[((int64_t)i * nv + a) * niv + (int64_t)j * nv + b];
+    // O3[k][p][b][q] = (kp|bq)
+    double t = 0.0;
+    for (int k = 0; k < n; ++k) {
+      t += O3[(((int64_t)k * n + i) * nv + b) * n + j] * s[(int64_t)k * nv + a];
+      t += O3[(((int64_t)k * n + j) * nv + a) * n + i] * s[(int64_t)k * nv + b];
+    }
+    Rb[e] += v - t;
+  }
+}
+
+// block (P, row chunk of 16 rows); loops over all strong Q
+__global__ void hole_kernel(PairTab T, HoleTab H, const double* __restrict__ C, const double* __restrict__ O4,
+                            double* __restrict__ R) {
+  const int P = blockIdx.y;
+  const int i = T.pairs[2 * P], j = T.pairs[2 * P + 1];
+  const int d = (int)(T.dom_off[P + 1] - T.dom_off[P]);
+  const int r0 = blockIdx.x * 16;
+  if (r0 >= d) return;
+  const int n = T.n_int;
+  double* Rb = R + T.cd_off[P];
+  for (int Q = 0; Q < T.n_pairs; ++Q) {
+    const int k = T.pairs[2 * Q], l = T.pairs[2 * Q + 1];
+    const double cA = O4[(((int64_t)k * n + i) * n + l) * n + j];                 // (ki|lj)
+    const double cB = (k != l) ? O4[(((int64_t)l * n + i) * n + k) * n + j] : 0.0;  // (li|kj)
+    const int64_t s0 = H.isect_off[(int64_t)P * T.n_pairs + Q];
+    const int m = (int)(H.isect_off[(int64_t)P * T.n_pairs + Q + 1] - s0);
+    if (m == 0) continue;
+    const int dq = (int)(T.dom_off[Q + 1] - T.dom_off[Q]);
+    const double* Cq = C + T.cd_off[Q];
+    const int32_t* is = H.isect + 2 * s0;
+    // rows of the intersection whose P-position lies in [r0, r0+16)
+    for (int e = threadIdx.x; e < 16 * m; e += blockDim.x) {
+      const int xr = e / m, y = e - xr * m;
+      // find intersection entry with P-position r0 + xr: entries are sorted by P-position
+      // (binary search over the m entries)
+      int lo = 0, hi = m;
+      const int target = r0 + xr;
+      while (lo < hi) {
+        int mid = (lo + hi) >> 1;
+        if (is[2 * mid] < target) lo = mid + 1; else hi = mid;
+      }
+      if (lo >= m || is[2 * lo] != target) continue;
+      const int qx = is[2 * lo + 1];
+      const int py = is[2 * y], qy = is[2 * y + 1];
+      Rb[(int64_t)target * d + py] += cA * Cq[(int64_t)qx * dq + qy] + cB * Cq[(int64_t)qy * dq + qx];
+    }
+    __syncthreads();
+  }
+}
+
+__global__ void ref_dot_kernel(PairTab T, const double* __restrict__ Kov, const double* __restrict__ Ct,
+                               double* __restrict__ part) {
+  const int P = blockIdx.x;
+  const int i = T.pairs[2 * P], j = T.pairs[2 * P + 1];
+  const int64_t off = T.dom_off[P];
+  const int d = (int)(T.dom_off[P + 1] - off);
+  const int64_t dd = (int64_t)d * d;
+  const int nv = T.n_v;
+  const int64_t niv = (int64_t)T.n_int * nv;
+  const double* Cb = Ct + T.cd_off[P];
+  double acc = 0.0;
+  for (int64_t e = threadIdx.x; e < dd; e += blockDim.x) {
+    const int r = (int)(e / d), c = (int)(e - (int64_t)r * d);
+    const int a = T.dom_idx[off + r], b = T.dom_idx[off + c];
+    acc += Kov[((int64_t)i * nv + a) * niv + (int64_t)j * nv + b] * Cb[e];
+  }
+  __shared__ double red[256];
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[P] = (i == j ? 1.0 : 2.0) * red[0];
+}
+
+__global__ void sum_kernel(const double* __restrict__ part, int n, double* __restrict__ out) {
+  __shared__ double red[256];
+  double acc = 0.0;
+  for (int e = threadIdx.x; e < n; e += blockDim.x) acc += part[e];
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *out = red[0];
+}
+
+// block (Q, o): ordered pair (j,k) = o==0 ? (qi,qj) : (qj,qi)
+__global__ void s3int_singles_kernel(PairTab T, const int64_t* __restrict__ gpart_off,
+                                     const double* __restrict__ Ct, const double* __restrict__ O3,
+                                     double* __restrict__ gpart) {
+  const int Q = blockIdx.x, o = blockIdx.y;
+  const int qi = T.pairs[2 * Q], qj = T.pairs[2 * Q + 1];
+  if (o == 1 && qi == qj) return;
+  const int jj = o == 0 ? qi : qj, kk = o == 0 ? qj : qi;
+  const int64_t off = T.dom_off[Q];
+  const int d = (int)(T.dom_off[Q + 1] - off);
+  const int n = T.n_int, nv = T.n_v;
+  const double* Cb = Ct + T.cd_off[Q];
+  double* out = gpart + gpart_off[2 * Q + o];   // [n_int][d]
+  for (int e = threadIdx.x + blockIdx.z * blockDim.x; e < n * d; e += blockDim.x * gridDim.z) {
+    const int i = e / d, r = e - (e / d) * d;
+    double acc = 0.0;
+    for (int c = 0; c < d; ++c) {
+      const double ct = (o == 0) ? Cb[(int64_t)r * d + c] : Cb[(int64_t)c * d + r];
+      const int b = T.dom_idx[off + c];
+      acc += ct * O3[(((int64_t)jj * n + i) * nv + b) * n + kk];   // (ji|kb) = (ji|bk)
+    }
+    out[(int64_t)i * d + r] = -acc;
+  }
+}
+
+__global__ void singles_reduce_kernel(PairTab T, InvTab I, const int64_t* __restrict__ gpart_off,
+                                      const double* __restrict__ gpart, const double* __restrict__ gss,
+                                      const double* __restrict__ gW, int n_split, double* __restrict__ g) {
+  const int nv = T.n_v;
+  const int64_t n = (int64_t)T.n_int * nv;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x) {
+    const int i = (int)(e / nv), a = (int)(e - (int64_t)i * nv);
+    double v = gss[e];
+    for (int sp = 0; sp < n_split; ++sp) v += gW[(int64_t)sp * n + e];
+    for (int64_t q = I.inv_off[a]; q < I.inv_off[a + 1]; ++q) {
+      const int Q = I.inv_q[q], pos = I.inv_pos[q];
+      const int d = (int)(T.dom_off[Q + 1] - T.dom_off[Q]);
+      v += gpart[gpart_off[2 * Q] + (int64_t)i * d + pos];
+      if (T.pairs[2 * Q] != T.pairs[2 * Q + 1]) v += gpart[gpart_off[2 * Q + 1] + (int64_t)i * d + pos];
+    }
+    g[e] = v;
+  }
+}
+
+__global__ void pack_kernel(PairTab T, const double* __restrict__ x, const double* __restrict__ R,
+                            const double* __restrict__ g, const double* __restrict__ sig0, double e_ref,
+                            int flags, double* __restrict__ sigma) {
+  const bool accum = (flags & TCI_ACCUMULATE) != 0;
+  const double eadd = (flags & TCI_ADD_EREF) ? e_ref : 0.0;
+  const int P = blockIdx.y;
+  const int no = T.n_int, nv = T.n_v;
+  if (P == T.n_pairs) {
+    const int64_t n = 1 + (int64_t)no * nv;
+    for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < n; k += (int64_t)gridDim.x * blockDim.x) {
+      double v;
+      if (k == 0) {
+        v = *sig0 + eadd * x[0];
+      } else {
+        const int64_t e = k - 1;
+        const int i = (int)(e / nv), a = (int)(e - (int64_t)i * nv);
+        v = SQRT2 * g[e] + (T.eps[no + a] - T.eps[i] + eadd) * x[k];
+      }
+      sigma[k] = accum ? sigma[k] + v : v;
+    }
+    return;
+  }
+  const int i = T.pairs[2 * P], j = T.pairs[2 * P + 1];
+  const int64_t off = T.dom_off[P];
+  const int d = (int)(T.dom_off[P + 1] - off);
+  const int64_t m2 = (int64_t)d * (d - 1) / 2;
+  const int nsp = (i == j) ? 1 : 2;
+  const int64_t cnt = d + nsp * m2;
+  const int64_t base = T.aa_off[P];   // ab block follows the aa block
+  const double* Rb = R + T.cd_off[P];
+  const double ei = T.eps[i], ej = T.eps[j];
+  for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < cnt; q += (int64_t)gridDim.x * blockDim.x) {
+    double v, fock;
+    if (q < d) {
+      const int r = (int)q;
+      const double ea = T.eps[no + T.dom_idx[off + r]];
+      v = (i == j) ? Rb[(int64_t)r * d + r] : SQRT2 * Rb[(int64_t)r * d + r];
+      fock = 2.0 * ea - ei - ej;
+    } else {
+      int64_t idx = q - d;
+      const bool trip = idx >= m2;
+      if (trip) idx -= m2;
+      int hi, lo;
+      tri_decode(idx, hi, lo);
+      const double up = Rb[(int64_t)lo * d + hi], dn = Rb[(int64_t)hi * d + lo];
+      if (i == j) v = (up + dn) * INV_SQRT2;
+      else v = trip ? SQRT3 * (up - dn) : (up + dn);
+      fock = T.eps[no + T.dom_idx[off + hi]] + T.eps[no + T.dom_idx[off + lo]] - ei - ej;
+    }
+    const int64_t k = base + q;
+    v += (fock + eadd) * x[k];
+    sigma[k] = accum ? sigma[k] + v : v;
+  }
+}
+
+__global__ void diag_kernel(PairTab T, const double* __restrict__ Kov, const double* __restrict__ Jvv,
+                            const double* __restrict__ O4, const double* __restrict__ vvd,
+                            const double* __restrict__ vvx, double* __restrict__ diag) {
+  const int P = blockIdx.y;
+  const int no = T.n_int, nv = T.n_v;
+  const int64_t niv = (int64_t)no * nv;
+  // (ma|ma) and (mm|aa)
+  auto Kd = [&](int m, int a) { return Kov[((int64_t)m * nv + a) * niv + (int64_t)m * nv + a]; };
+  auto Jd = [&](int m, int a) {
+    const int64_t pm = (int64_t)m * (m + 1) / 2 + m;
+    return Jvv[pm * nv * nv + (int64_t)a * nv + a];
+  };
+  auto o4 = [&](int p, int q, int r, int s) { return O4[(((int64_t)p * no + q) * no + r) * no + s]; };
+  if (P == T.n_pairs) {
+    const int64_t n = 1 + niv;
+    for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < n; k += (int64_t)gridDim.x * blockDim.x) {
+      if (k == 0) { diag[0] = 0.0; continue; }
+      const int64_t e = k - 1;
+      const int i = (int)(e / nv), a = (int)(e - (int64_t)i * nv);
+      diag[k] = T.eps[no + a] - T.eps[i] + 2.0 * Kd(i, a) - Jd(i, a);
+    }
+    return;
+  }
+  const int i = T.pairs[2 * P], j = T.pairs[2 * P + 1];
+  const int64_t off = T.dom_off[P];
+  const int d = (int)(T.dom_off[P + 1] - off);
+  const int64_t m2 = (int64_t)d * (d - 1) / 2;
+  const int nsp = (i == j) ? 1 : 2;
+  const int64_t cnt = d + nsp * m2;
+  const double ei = T.eps[i], ej = T.eps[j];
+  const double iiii = o4(i, i, i, i), iijj = o4(i, i, j, j), ijij = o4(i, j, i, j);
+  for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < cnt; q += (int64_t)gridDim.x * blockDim.x) {
+    double v;
+    if (q < d) {
+      const int a = T.dom_idx[off + q];
+      const double ea = T.eps[no + a];
+      const double aaaa = vvd[(int64_t)a * nv + a];
+      if (i == j)
+        v = 2 * ea - 2 * ei + aaaa + iiii + 2 * Kd(i, a) - 4 * Jd(i, a);
+      else
+        v = 2 * ea - ei - ej + aaaa + iijj + ijij + Kd(i, a) + Kd(j, a) - 2 * Jd(i, a) - 2 * Jd(j, a);
+    } else {
+      int64_t idx = q - d;
+      const bool trip = idx >= m2;
+      if (trip) idx -= m2;
+      int hi, lo;
+      tri_decode(idx, hi, lo);
+      const int a = T.dom_idx[off + hi], b = T.dom_idx[off + lo];
+      const double de = T.eps[no + a] + T.eps[no + b] - ei - ej;
+      const double aabb = vvd[(int64_t)a * nv + b], abab = vvx[(int64_t)a * nv + b];
+      if (i == j) {
+        v = de + aabb + abab + iiii + Kd(i, a) + Kd(i, b) - 2 * Jd(i, a) - 2 * Jd(i, b);
+      } else {
+        const double ks = Kd(i, a) + Kd(i, b) + Kd(j, a) + Kd(j, b);
+        const double js = Jd(i, a) + Jd(i, b) + Jd(j, a) + Jd(j, b);
+        v = trip ? de + aabb - abab + iijj - ijij + 1.5 * ks - js
+                 : de + aabb + abab + iijj + ijij + 0.5 * ks - js;
+      }
+    }
+    diag[T.aa_off[P] + q] = v;
+  }
+}
+
+// Lpm[(p(p+1)/2+q)][J] -> Lvv[J][a][b] (both halves)
+template <class OT>
+__global__ void gather_lvv_kernel(const double* __restrict__ Lpm, int64_t n_chol, int n_int, int n_v,
+                                  OT* __restrict__ Lvv) {
+  // block: one virtual row a, a chunk of J; threads over (b <= a)
+  const int a = blockIdx.x;
+  const int64_t J0 = (int64_t)blockIdx.y * 32;
+  const int64_t pa = (int64_t)(n_int + a);
+  for (int t = threadIdx.x; t < (a + 1) * 32; t += blockDim.x) {
+    const int jj = t & 31, b = t >> 5;
+    const int64_t J = J0 + jj;
+    if (J >= n_chol) continue;
+    const int64_t rec = pa * (pa + 1) / 2 + n_int + b;
+    const double v = Lpm[rec * n_chol + J];
+    const int64_t base = J * n_v * n_v;
+    Lvv[base + (int64_t)a * n_v + b] = (OT)v;
+    Lvv[base + (int64_t)b * n_v + a] = (OT)v;
+  }
+}
+
+__global__ void gather_lov_kernel(const double* __restrict__ Lpm, int64_t n_chol, int n_int, int n_v,
+                                  double* __restrict__ Lov) {
+  const int a = blockIdx.x;
+  const int64_t J0 = (int64_t)blockIdx.y * 32;
+  const int64_t pa = (int64_t)(n_int + a);
+  for (int t = threadIdx.x; t < n_int * 32; t += blockDim.x) {
+    const int jj = t & 31, i = t >> 5;
+    const int64_t J = J0 + jj;
+    if (J >= n_chol) continue;
+    const int64_t rec = pa * (pa + 1) / 2 + i;
+    Lov[(J * n_int + i) * n_v + a] = Lpm[rec * n_chol + J];
+  }
+}
+
+template <class LT>
+__global__ void vv_exch_kernel(const LT* __restrict__ Lvv, int64_t n_chol, int n_v, double* __restrict__ out) {
+  const int64_t n = (int64_t)n_v * n_v;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x) {
+    double acc = 0.0;
+    for (int64_t J = 0; J < n_chol; ++J) {
+      const double v = (double)Lvv[J * n + e];
+      acc += v * v;
+    }
+    out[e] = acc;
+  }
+}
+
+__global__ void build_mss_kernel(const double* __restrict__ Kov, const double* __restrict__ Jvv, int n_int,
+                                 int n_v, double* __restrict__ Mss) {
+  const int64_t niv = (int64_t)n_int * n_v;
+  const int64_t n = niv * niv;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t row = e / niv, col = e - row * niv;
+    const int i = (int)(row / n_v), a = (int)(row - (int64_t)i * n_v);
+    const int j = (int)(col / n_v), b = (int)(col - (int64_t)j * n_v);
+    const int hi = i > j ? i : j, lo = i > j ? j : i;
+    const int64_t pij = (int64_t)hi * (hi + 1) / 2 + lo;
+    Mss[e] = 2.0 * Kov[e] - Jvv[pij * n_v * n_v + (int64_t)a * n_v + b];
+  }
+}
+
+__global__ void fill_kernel(double* p, int64_t n, double v) {
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x)
+    p[e] = v;
+}
+
+inline unsigned gx(int64_t work) { return (unsigned)std::min<int64_t>(std::max<int64_t>(cdiv(work, 256), 1), 4096); }
+
+}  // namespace
+
+void launch_unpack(const PairTab& T, int dmax, const double* x, double* C, double* Ct, double* s,
+                   hipStream_t st) {
+  const int64_t w = std::max<int64_t>((int64_t)dmax * dmax, (int64_t)T.n_int * T.n_v);
+  hipLaunchKernelGGL(unpack_kernel, dim3(std::min<unsigned>(gx(w), 256), T.n_pairs + 1), dim3(256), 0, st, T, x, C,
+                     Ct, s);
+  TCI_HIP(hipGetLastError());
+}
+
+void launch_rterms(const PairTab& T, int dmax, const double* x, const double* s, const double* Kov,
+                   const double* O3, double* R, hipStream_t st) {
+  if (T.n_pairs == 0) return;
+  hipLaunchKernelGGL(rterms_kernel, dim3(std::min<unsigned>(gx((int64_t)dmax * dmax), 256), T.n_pairs), dim3(256),
+                     0, st, T, x, s, Kov, O3, R);
+  TCI_HIP(hipGetLastError());
+}
+
+void launch_hole(const PairTab& T, const HoleTab& H, int dmax, const double* C, const double* O4, double* R,
+                 hipStream_t st) {
+  if (T.n_pairs == 0) return;
+  hipLaunchKernelGGL(hole_kernel, dim3((unsigned)cdiv(dmax, 16), T.n_pairs), dim3(256), 0, st, T, H, C, O4, R);
+  TCI_HIP(hipGetLastError());
+}
+
+void launch_ref_dot(const PairTab& T, const double* Kov, const double* Ct, double* pair_part, double* sig0,
+                    hipStream_t st) {
+  if (T.n_pairs > 0) {
+    hipLaunchKernelGGL(ref_dot_kernel, dim3(T.n_pairs), dim3(256), 0, st, T, Kov, Ct, pair_part);
+    TCI_HIP(hipGetLastError());
+  }
+  hipLaunchKernelGGL(sum_kernel, dim3(1), dim3(256), 0, st, pair_part, T.n_pairs, sig0);
+  TCI_HIP(hipGetLastError());
+}
+
+void launch_s3int_singles(const PairTab& T, const int64_t* gpart_off, const double* Ct, const double* O3,
+                          double* gpart, hipStream_t st) {
+  if (T.n_pairs == 0) return;
+  hipLaunchKernelGGL(s3int_singles_kernel, dim3(T.n_pairs, 2, 4), dim3(256), 0, st, T, gpart_off, Ct, O3, gpart);
+  TCI_HIP(hipGetLastError());
+}
+
+void launch_singles_reduce(const PairTab& T, const InvTab& I, const int64_t* gpart_off, const double* gpart,
+                           const double* gss, const double* gW, int n_split, double* g, hipStream_t st) {
+  hipLaunchKernelGGL(singles_reduce_kernel, dim3(gx((int64_t)T.n_int * T.n_v)), dim3(256), 0, st, T, I, gpart_off,
+                     gpart, gss, gW, n_split, g);
+  TCI_HIP(hipGetLastError());
+}
+
+void launch_pack(const PairTab& T, int dmax, const double* x, const double* R, const double* g, const double* sig0,
+                 double e_ref, int flags, double* sigma, hipStream_t st) {
+  const int64_t w = std::max<int64_t>((int64_t)dmax * dmax, 1 + (int64_t)T.n_int * T.n_v);
+  hipLaunchKernelGGL(pack_kernel, dim3(std::min<unsigned>(gx(w), 256), T.n_pairs + 1), dim3(256), 0, st, T, x, R, g,
+                     sig0, e_ref, flags, sigma);
+  TCI_HIP(hipGetLastError());
+}
+
+void launch_diag(const PairTab& T, int dmax, const double* Kov, const double* Jvv, const double* O4,
+                 const double* vv_diag, const double* vv_exch, double* diag, hipStream_t st) {
+  const int64_t w = std::max<int64_t>((int64_t)dmax * dmax, 1 + (int64_t)T.n_int * T.n_v);
+  hipLaunchKernelGGL(diag_kernel, dim3(std::min<unsigned>(gx(w), 256), T.n_pairs + 1), dim3(256), 0, st, T, Kov, Jvv,
+                     O4, vv_diag, vv_exch, diag);
+  TCI_HIP(hipGetLastError());
+}
+
+void launch_gather_Lvv(const double* Lpm, int64_t n_chol, int n_int, int n_v, void* Lvv, bool f32, hipStream_t st) {
+  dim3 grid(n_v, (unsigned)cdiv(n_chol, 32));
+  if (f32) hipLaunchKernelGGL(gather_lvv_kernel<float>, grid, dim3(256), 0, st, Lpm, n_chol, n_int, n_v, (float*)Lvv);
+  else hipLaunchKernelGGL(gather_lvv_kernel<double>, grid, dim3(256), 0, st, Lpm, n_chol, n_int, n_v, (double*)Lvv);
+  TCI_HIP(hipGetLastError());
+}
+
+void launch_gather_Lov(const double* Lpm, int64_t n_chol, int n_int, int n_v, double* Lov, hipStream_t st) {
+  dim3 grid(n_v, (unsigned)cdiv(n_chol, 32));
+  hipLaunchKernelGGL(gather_lov_kernel, grid, dim3(256), 0, st, Lpm, n_chol, n_int, n_v, Lov);
+  TCI_HIP(hipGetLastError());
+}
+
+void launch_vv_exch(const void* Lvv, bool f32, int64_t n_chol, int n_v, double* out, hipStream_t st) {
+  const unsigned g = gx((int64_t)n_v * n_v);
+  if (f32) hipLaunchKernelGGL(vv_exch_kernel<float>, dim3(g), dim3(256), 0, st, (const float*)Lvv, n_chol, n_v, out);
+  else hipLaunchKernelGGL(vv_exch_kernel<double>, dim3(g), dim3(256), 0, st, (const double*)Lvv, n_chol, n_v, out);
+  TCI_HIP(hipGetLastError());
+}
+
+void launch_build_mss(const double* Kov, const double* Jvv, int n_int, int n_v, double* Mss, hipStream_t st) {
+  const int64_t n = (int64_t)n_int * n_v * n_int * n_v;
+  hipLaunchKernelGGL(build_mss_kernel, dim3(gx(n)), dim3(256), 0, st, Kov, Jvv, n_int, n_v, Mss);
+  TCI_HIP(hipGetLastError());
+}
+
+void launch_fill(double* p, int64_t n, double v, hipStream_t st) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(fill_kernel, dim3(gx(n)), dim3(256), 0, st, p, n, v);
+  TCI_HIP(hipGetLastError());
+}
+
+}  // namespace tci
