@@ -1,0 +1,191 @@
+#!/usr/bin/env python3
+"""Benchmark of the MI355X sigma / Davidson path.
+
+Metric: Davidson time-to-|dE|<1e-6 Eh, mixed precision vs FP64 on one GPU,
+with sigma TFLOP/s and the ladder kernel's roofline.  One step = one complete
+Davidson-Liu solve (lowest root, from the init_from_diag guess) of the
+synthetic workload with the split-precision ladder and FP64 refinement.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME]
+                    [--precision bf16x3|fp16x3|fp64] [--mode davidson|sigma]
+
+Prints one JSON line.  Inputs are generated on the host and resident in HBM
+before the timed region starts (plan creation is outside it).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+METRIC = "Davidson time-to-|ΔE|<1e-6 Eh: mixed vs FP64 GPU; sigma TFLOP/s, HBM GB/s"
+PEAK = {"fp64": 78.6e12, "bf16x3": 2.5e15, "fp16x3": 2.5e15}   # dense MFMA, MI355X
+HBM_PEAK = 8.0e12
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(syn, plan, n_sigma_fp64: int, budget_s: float = 15.0) -> dict:
+    """Time the oracle's dense ladder restatement on a bounded sample of pair
+    blocks of the same workload, and scale to one FP64 Davidson solve
+    (n_sigma_fp64 sigma builds, each plan.info.flops_sigma flops)."""
+    from threadpoolctl import threadpool_info
+    from oracle.scepper import scepper_p1, scepper_m1
+    rng = np.random.default_rng(7)
+    n_int = syn.n_int
+    L = syn.L
+    order = np.argsort(-np.diff(syn.dom_off))
+    flops = 0.0
+    t0 = time.perf_counter()
+    used = 0
+    for P in order[::max(1, len(order) // 16)]:
+        D = syn.dom_idx[syn.dom_off[P]:syn.dom_off[P + 1]]
+        d = len(D)
+        a = n_int + D
+        rec = (np.maximum.outer(a, a) * (np.maximum.outer(a, a) + 1) // 2 + np.minimum.outer(a, a))
+        Lp = np.ascontiguousarray(L[rec.ravel()].T.reshape(syn.n_chol, d, d), dtype=np.float64)
+        m2 = d * (d - 1) // 2
+        C = scepper_p1(rng.standard_normal(m2), rng.standard_normal(d), d) / 2 + \
+            scepper_m1(rng.standard_normal(m2), d) / 3.4641016151377544
+        (Lp @ C @ Lp).sum(axis=0)
+        flops += 4.0 * d ** 3 * syn.n_chol
+        used += 1
+        if time.perf_counter() - t0 > budget_s:
+            break
+    dt = time.perf_counter() - t0
+    rate = flops / dt
+    threads = max([x.get("num_threads", 1) for x in threadpool_info() if x.get("user_api") == "blas"] or [1])
+    sigma_s = plan.info.flops_sigma / rate
+    return {"value": sigma_s * n_sigma_fp64, "unit": "s", "cores": int(threads), "kind": "port",
+            "sample": f"oracle dense ladder (numpy BLAS, FP64) on {used} of {len(order)} pair blocks "
+                      f"({flops:.3g} flop in {dt:.1f} s = {rate / 1e9:.1f} GFLOP/s), scaled by flops to "
+                      f"{n_sigma_fp64} FP64 sigma builds of {plan.info.flops_sigma:.3g} flop",
+            "sigma_seconds": sigma_s}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="sigma_16x300x1200_full")
+    ap.add_argument("--precision", default="bf16x3")
+    ap.add_argument("--mode", default="davidson", choices=["davidson", "sigma"])
+    ap.add_argument("--switch-tol", type=float, default=1e-4)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--seed", type=int, default=1234)
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="after the timed region, copy this many MiB device-to-device (counter calibration)")
+    a = ap.parse_args()
+    if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print(json.dumps({"error": "this build runs on one GPU only: the sigma path is not sharded "
+                                    "(see the sharding design notes); run with --gpus 1"}))
+        return 2
+
+    import tigerci_amd as tci
+    from tigerci_amd.synthetic import workload, random_vector, WORKLOADS
+
+    t = time.perf_counter()
+    syn = workload(a.workload, a.seed)
+    log(f"[bench] generated {a.workload} in {time.perf_counter() - t:.1f} s")
+    ladder_dtype = "f32" if syn.L.dtype == np.float32 else "f64"
+    t = time.perf_counter()
+    plan = tci.Plan.from_problem(syn, ladder_dtype=ladder_dtype)
+    tci.synchronize()
+    log(f"[bench] plan: n_csf={plan.n_csf} items={plan.info.n_ladder_items} "
+        f"device {plan.info.device_bytes / 2**30:.1f} GiB, built in {time.perf_counter() - t:.1f} s")
+    info = plan.info
+    cfg = {"workload": a.workload, "n_int": syn.n_int, "n_v": syn.n_v, "n_chol": syn.n_chol,
+           "pairs": int(len(syn.pairs)), "n_csf": int(plan.n_csf), "L_storage": ladder_dtype,
+           "ladder_flops_per_sigma": info.flops_ladder, "sigma_flops": info.flops_sigma}
+    out = {"metric": METRIC, "n_gpus": 1, "steps": a.steps, "warmup": a.warmup, "scaling": "weak",
+           "vs_baseline": None, "data": "synthetic (seeded N(0,1) exp(-J/80) Cholesky vectors)"}
+
+    if a.mode == "sigma":
+        x = tci.DeviceArray.from_numpy(random_vector(plan.n_csf, a.seed))
+        y = tci.DeviceArray(plan.n_csf)
+        for _ in range(a.warmup):
+            plan.sigma(x, y, precision=a.precision)
+        tci.synchronize()
+        plan.profile(True)
+        t0 = time.perf_counter()
+        for _ in range(a.steps):
+            plan.sigma(x, y, precision=a.precision)
+        tci.synchronize()
+        dt = (time.perf_counter() - t0) / a.steps
+        pr = plan.profile_read(a.precision)
+        lad = pr["ladder_ms"] / max(pr["ladder_launches"], 1) / 1e3
+        ach = info.flops_ladder / lad
+        out.update({"value": info.flops_sigma / dt / 1e12, "unit": "TFLOP/s", "higher_is_better": True,
+                    "ms_per_step": dt * 1e3, "dtype": a.precision, "config": cfg,
+                    "roofline": {"bound": "mfma", "achieved": ach / 1e12, "peak": PEAK[a.precision] / 1e12,
+                                 "unit": "TFLOP/s", "frac": ach / PEAK[a.precision], "traffic": None,
+                                 "kernel": "ladder", "avg_launch_ms": lad * 1e3}})
+        if a.copy_probe_mib:
+            n = a.copy_probe_mib * (1 << 20) // 8
+            src, dst = tci.DeviceArray(n), tci.DeviceArray(n)
+            dst.copy_from(src)
+            tci.synchronize()
+            out["copy_probe_bytes"] = 8 * n
+        print(json.dumps(out), flush=True)
+        return 0
+
+    # ---- Davidson: mixed (timed steps) and FP64 reference -------------------
+    def solve(prec):
+        return plan.davidson(roots=1, energy_tol=1e-6, norm_tol=1e-6, precision=prec, switch_tol=a.switch_tol)
+
+    for _ in range(a.warmup):
+        solve(a.precision)
+    tci.synchronize()
+    plan.profile(True)
+    t0 = time.perf_counter()
+    res_m = None
+    for s in range(a.steps):
+        e_m, res_m = solve(a.precision)
+        log(f"[bench] step {s}: {res_m}")
+    tci.synchronize()
+    t_m = (time.perf_counter() - t0) / a.steps
+    pm = plan.profile_read(a.precision) if a.precision != "fp64" else None
+    plan.profile(True)
+    t0 = time.perf_counter()
+    e_64, res_64 = solve("fp64")
+    tci.synchronize()
+    t_64 = time.perf_counter() - t0
+    p64 = plan.profile_read("fp64")
+    log(f"[bench] fp64 solve: {res_64}")
+    lad = p64["ladder_ms"] / max(p64["ladder_launches"], 1) / 1e3
+    ach = info.flops_ladder / lad
+    sig64 = p64["sigma_ms"] / max(p64["sigma_calls"], 1) / 1e3
+    out.update({
+        "value": t_m, "unit": "s", "higher_is_better": False, "ms_per_step": t_m * 1e3,
+        "dtype": "f64" if a.precision == "fp64" else f"{a.precision}+f64", "config": cfg,
+        "fp64_seconds": t_64, "speedup_vs_fp64": t_64 / t_m,
+        "energy_mixed": float(e_m[0]), "energy_fp64": float(e_64[0]),
+        "energy_diff_uEh": float(abs(e_m[0] - e_64[0]) * 1e6),
+        "iterations": {"mixed": res_m["iterations"], "fp64": res_64["iterations"],
+                       "sigma_mixed": res_m["sigma_mixed"], "sigma_fp64_in_mixed": res_m["sigma_fp64"],
+                       "sigma_fp64": res_64["sigma_fp64"]},
+        "sigma_tflops_fp64": info.flops_sigma / sig64 / 1e12,
+        "roofline": {"bound": "mfma", "achieved": ach / 1e12, "peak": PEAK["fp64"] / 1e12, "unit": "TFLOP/s",
+                     "frac": ach / PEAK["fp64"], "traffic": None, "kernel": "ladder_f64_kernel (FP64 MFMA)",
+                     "avg_launch_ms": lad * 1e3, "launches": p64["ladder_launches"]},
+    })
+    if pm and pm["ladder_launches"]:
+        lm = pm["ladder_ms"] / pm["ladder_launches"] / 1e3
+        out["roofline_mixed"] = {"bound": "mfma", "achieved": info.flops_ladder / lm / 1e12,
+                                 "peak": PEAK[a.precision] / 1e12, "unit": "TFLOP/s",
+                                 "frac": info.flops_ladder / lm / PEAK[a.precision], "avg_launch_ms": lm * 1e3}
+    if not a.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(syn, plan, res_64["sigma_fp64"])
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -145,13 +145,13 @@ int tci_memcpy_d2d(void* dst, const void* src, int64_t bytes);
 int tci_device_synchronize(void);
 
 /* Kernel timing with HIP events on the stream of each sigma build: when
- * enabled, every tci_sigma_apply records the duration of its ladder launch and
- * of the whole build.  tci_profile_read returns the sums since enabling.
- * No reference counterpart (the reference's per-phase wall clocks live in
- * time_var_mod.f90). */
+ * enabled, every sigma build records the duration of its ladder launch and of
+ * the whole build, summed per tci_precision; tci_profile_read returns the sums
+ * for one precision since enabling.  No reference counterpart (the
+ * reference's per-phase wall clocks live in time_var_mod.f90). */
 int tci_profile_enable(tci_plan* plan, int32_t on);
-int tci_profile_read(const tci_plan* plan, double* ladder_ms, int64_t* ladder_launches,
-                     double* sigma_ms, int64_t* sigma_calls);
+int tci_profile_read(const tci_plan* plan, int32_t precision, double* ladder_ms,
+                     int64_t* ladder_launches, double* sigma_ms, int64_t* sigma_calls);
 
 /* Checks the MFMA operand/result lane layouts the kernels assume against a
  * scalar product on the device; 0 when they agree.  No reference counterpart. */

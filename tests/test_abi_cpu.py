@@ -65,3 +65,15 @@ def test_golden_fixtures_match_oracle():
         for x, s in zip(z["x"], z["sigma"]):
             assert np.abs(sigma(prob, x) - s).max() < 1e-12 * np.abs(s).max()
         assert np.abs(diagonal(prob) - z["diag"]).max() < 1e-12 * np.abs(z["diag"]).max()
+
+
+def test_synthetic_generator_matches_oracle():
+    """The bench's chunked generator reproduces the oracle's problem bit for bit."""
+    from oracle.problem import make_problem
+    from tigerci_amd.synthetic import make_synthetic
+    for args in [dict(n_int=3, n_v=9, n_chol=7, seed=5),
+                 dict(n_int=5, n_v=30, n_chol=11, seed=6, domain_range=(4, 25), weak_below=8)]:
+        a = make_problem(**args)
+        b = make_synthetic(**args, chunk_rows=17)
+        for f in ("eps", "L", "pairs", "dom_off", "dom_idx"):
+            assert np.array_equal(getattr(a, f), getattr(b, f)), f

@@ -91,7 +91,7 @@ def load_library(path: str | None = None) -> ctypes.CDLL:
     lib.tci_memcpy_d2d.argtypes = [vp, vp, i64]
     lib.tci_device_synchronize.argtypes = []
     lib.tci_profile_enable.argtypes = [vp, i32]
-    lib.tci_profile_read.argtypes = [vp, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(i64),
+    lib.tci_profile_read.argtypes = [vp, i32, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(i64),
                                      ctypes.POINTER(ctypes.c_double), ctypes.POINTER(i64)]
     lib.tci_last_error.restype = ctypes.c_char_p
     for f in ("tci_plan_create", "tci_plan_get_info", "tci_sigma_apply", "tci_htimesc", "tci_diagonal",
@@ -264,11 +264,11 @@ class Plan:
     def profile(self, on: bool = True) -> None:
         _check(self._lib.tci_profile_enable(self._h, 1 if on else 0))
 
-    def profile_read(self) -> dict:
+    def profile_read(self, precision: str = "fp64") -> dict:
         lm, sm = ctypes.c_double(), ctypes.c_double()
         ln, sn = ctypes.c_int64(), ctypes.c_int64()
-        _check(self._lib.tci_profile_read(self._h, ctypes.byref(lm), ctypes.byref(ln), ctypes.byref(sm),
-                                          ctypes.byref(sn)))
+        _check(self._lib.tci_profile_read(self._h, PRECISION[precision], ctypes.byref(lm), ctypes.byref(ln),
+                                          ctypes.byref(sm), ctypes.byref(sn)))
         return dict(ladder_ms=lm.value, ladder_launches=ln.value, sigma_ms=sm.value, sigma_calls=sn.value)
 
     def davidson(self, roots: int = 1, energy_tol: float = 1e-6, norm_tol: float = 1e-6,

@@ -46,6 +46,20 @@ struct DevBuf {
   DevBuf() = default;
   DevBuf(const DevBuf&) = delete;
   DevBuf& operator=(const DevBuf&) = delete;
+  DevBuf(DevBuf&& o) noexcept : p(o.p), bytes(o.bytes) {
+    o.p = nullptr;
+    o.bytes = 0;
+  }
+  DevBuf& operator=(DevBuf&& o) noexcept {
+    if (this != &o) {
+      reset();
+      p = o.p;
+      bytes = o.bytes;
+      o.p = nullptr;
+      o.bytes = 0;
+    }
+    return *this;
+  }
   ~DevBuf() { reset(); }
   void reset() {
     if (p) (void)hipFree(p);
@@ -74,15 +88,26 @@ void upload(DevBuf& b, const std::vector<T>& h, hipStream_t s = nullptr) {
 
 inline int64_t cdiv(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
+// Load through the global address space.  Pointers read from launch tables are
+// generic to the compiler, which would otherwise emit flat loads; those also
+// count against the LDS wait counter and would make every LDS read wait for
+// all outstanding prefetches.
+template <class T>
+__device__ __forceinline__ T gld(const T* p) {
+  return *(const __attribute__((address_space(1))) T*)p;
+}
+
 // ---------------------------------------------------------------------------
 // Generic grouped gather-GEMM (FP64 MFMA):  C[tile] (+)= sum_seg alpha A_seg B_seg
-// An operand element (r,k) is base[rmap(r)*rs + cmap(k)*cs]; a map entry < 0
-// (or a null base) reads as zero; a null map is the identity.
+// An operand element (r,k) is base[J0*jstride + rmap(r)*rs + cmap(k)*cs]; a map
+// entry < 0 (or a null base) reads as zero; a null map is the identity.  J0 is
+// a launch argument, so one table serves every Cholesky-vector chunk.
 struct Op {
   const void* base;   // double, or float when f32 != 0
   int64_t rs, cs;
   const int32_t* rmap;
   const int32_t* cmap;
+  int64_t jstride;
   int32_t f32;
 };
 struct Seg {
@@ -97,21 +122,32 @@ struct Task {
   int32_t m0, n0;   // this tile's origin
   int32_t seg0, seg1;
   int32_t accumulate;
+  int32_t out_f32;   // C is float* (split-precision kernel only)
+  int32_t orow_div;  // dense kernels: if > 0, logical row r is stored at row (r % div) * (M / div) + r / div
 };
 
 // A list of tasks and their segments, kept on the host while being built and
 // mirrored on the device.
+// tile = 64: 64x64 tiles, 4 waves of 32x32 (small or skinny outputs);
+// tile = 128: 128x128 tiles, double-buffered LDS, 4 waves of 64x64 with
+// wave-uniform skipping of 16x16 blocks outside the output.
 struct GemmProgram {
+  int tile = 64;
+  bool dense = false;  // contiguous row-major operands only: dense_gemm.hip kernel (tile 64 or 128)
   std::vector<Task> tasks;
   std::vector<Seg> segs;
   DevBuf d_tasks, d_segs;
   void clear() { tasks.clear(); segs.clear(); }
   // add one output matrix: tiles over (M,N), all sharing `segs_for_output`
-  void add(double* C, int64_t ldc, int M, int N, const std::vector<Seg>& s, bool accumulate);
+  void add(double* C, int64_t ldc, int M, int N, const std::vector<Seg>& s, bool accumulate,
+           bool out_f32 = false, int orow_div = 0);
   void upload_tables(hipStream_t st);
-  void launch(hipStream_t st) const;
+  void launch(hipStream_t st, int64_t J0 = 0) const;
+  // split-precision kernel (gemm_split.hip): 128 x 128 tiles only
+  void launch_split(int precision, hipStream_t st, int64_t J0 = 0, float scale_a = 1.f, float scale_b = 1.f) const;
 };
 
-void launch_ggemm(const Task* tasks, const Seg* segs, int64_t ntasks, hipStream_t st);
+void launch_ggemm(int tile, const Task* tasks, const Seg* segs, int64_t ntasks, int64_t J0, hipStream_t st);
+void launch_dgemm(int tile, const Task* tasks, const Seg* segs, int64_t ntasks, int64_t J0, hipStream_t st);
 
 }  // namespace tci

@@ -34,13 +34,14 @@ inline int64_t rec(int64_t p, int64_t q) {  // 0-based packed orbital-pair recor
 }
 
 Op op(const void* base, int64_t rs, int64_t cs, const int32_t* rmap = nullptr, const int32_t* cmap = nullptr,
-      bool f32 = false) {
+      bool f32 = false, int64_t jstride = 0) {
   Op o;
   o.base = base;
   o.rs = rs;
   o.cs = cs;
   o.rmap = rmap;
   o.cmap = cmap;
+  o.jstride = jstride;
   o.f32 = f32 ? 1 : 0;
   return o;
 }
@@ -195,6 +196,7 @@ void tci_plan::build(const tci_problem& p) {
     vvd.alloc((size_t)n_v * n_v * 8);
     vvx.alloc((size_t)n_v * n_v * 8);
     GemmProgram asmb;
+    asmb.tile = 128;
     const int K = (int)n_chol;
     // rows: records (k contiguous, cs = 1); cols: records transposed (rs = 1, cs = n_chol)
     asmb.add(Kov.as<double>(), niv, (int)niv, (int)niv,
@@ -342,6 +344,7 @@ void tci_plan::build(const tci_problem& p) {
     return transposed ? op(b, 1, d, rm, cm) : op(b, d, 1, rm, cm);
   };
   ring.clear();
+  ring.tile = 128;
   for (int P = 0; P < n_pairs; ++P) {
     const int i = h_pairs[2 * P], j = h_pairs[2 * P + 1];
     const int d = dsz(P);
@@ -383,6 +386,7 @@ void tci_plan::build(const tci_problem& p) {
   progY.upload_tables(st);
   // W[i][J][b] = sum_j sum_c Ct^{ij}[pos(b)][c] L^J_{j D[c]}
   progW.clear();
+  progW.tile = 128;
   for (int i = 0; i < n_int; ++i) {
     std::vector<Seg> sg;
     for (int j = 0; j < n_int; ++j) {
@@ -439,9 +443,14 @@ void tci_plan::build(const tci_problem& p) {
           cost.push_back(std::min(64, d - r0) * dp * dp * (it.J1 - it.J0));
         }
     }
-    ladder.build(items, cost, dmax, st);
-    lad_part.alloc((size_t)lad_chunks * sum_d2 * 8);
+    const char* env = std::getenv("TCI_LADDER");
+    ladder_fused = env && std::string(env) == "fused";
+    if (ladder_fused) {
+      ladder.build(items, cost, dmax, st);
+      lad_part.alloc((size_t)lad_chunks * sum_d2 * 8);
+    }
   }
+  if (!ladder_fused) build_two_phase_ladder(st);
 
   // ---- accounting ---------------------------------------------------------------
   flops_ladder = 0.0;
@@ -473,9 +482,132 @@ tci_plan::~tci_plan() {
     if (e) (void)hipEventDestroy(e);
 }
 
+// Two-phase ladder tables (dense FP64 MFMA programs).  Per chunk of Lc
+// Cholesky vectors starting at J0:
+//   pack    Lp_P[jj] = L^{J0+jj}[D_P, D_P]  (local-domain pairs only; full-domain
+//           pairs read L^J directly)
+//   phase 1 T_P[(jj,a)][c] = sum_k Lp_P[jj][a][k] C_P[k][c]       (128-wide tiles)
+//   phase 2 R_P[a][n]    += sum_jj sum_c T_P[jj][a][c] Lp_P[jj][c][n]
+//           (64-wide tiles, one K-segment per jj, K split into partial R
+//           buffers when the outputs alone cannot fill the chip)
+// Lc bounds T (8 bytes) plus Lp (4 or 8 bytes) per element of sum_P d_P^2.
+void tci_plan::build_two_phase_ladder(hipStream_t st) {
+  lphase.clear();
+  lchunks.clear();
+  const size_t lsz = lvv_f32 ? 4 : 8;
+  int64_t sum_local = 0;
+  std::vector<int32_t> loc;
+  for (int P = 0; P < n_pairs; ++P) {
+    const int64_t d = h_dom_off[P + 1] - h_dom_off[P];
+    if (d != n_v) {
+      loc.push_back(P);
+      sum_local += d * d;
+    }
+  }
+  n_local = (int)loc.size();
+  upload(pairs_local, loc, st);
+  const double budget = 8.0e9;  // bytes of T + Lp per chunk
+  const double per_J = 8.0 * (double)std::max<int64_t>(sum_d2, 1) + (double)lsz * sum_local;
+  int64_t Lc = std::max<int64_t>(1, std::min<int64_t>(n_chol, (int64_t)(budget / per_J)));
+  for (int64_t q = Lc; q >= std::max<int64_t>(1, Lc / 2); --q)
+    if (n_chol % q == 0) { Lc = q; break; }
+  std::vector<int64_t> lens;
+  for (int64_t J0 = 0; J0 < n_chol; J0 += Lc) {
+    const int64_t len = std::min(Lc, n_chol - J0);
+    int idx = -1;
+    for (size_t q = 0; q < lens.size(); ++q)
+      if (lens[q] == len) idx = (int)q;
+    if (idx < 0) {
+      idx = (int)lens.size();
+      lens.push_back(len);
+    }
+    lchunks.push_back({J0, idx});
+  }
+  int64_t tiles2 = 0;
+  for (int P = 0; P < n_pairs; ++P) tiles2 += cdiv(h_dom_off[P + 1] - h_dom_off[P], 64) * cdiv(h_dom_off[P + 1] - h_dom_off[P], 64);
+  rpart_n = (int)std::max<int64_t>(1, std::min<int64_t>(16, cdiv(2048, std::max<int64_t>(tiles2, 1))));
+  Tbuf.alloc((size_t)Lc * sum_d2 * 8);
+  if (sum_local) Lp.alloc((size_t)Lc * sum_local * lsz);
+  if (rpart_n > 1) Rpart.alloc((size_t)rpart_n * sum_d2 * 8);
+  const int64_t nv2 = (int64_t)n_v * n_v;
+  for (int64_t len : lens) {
+    lphase.emplace_back();
+    LadderPhase& ph = lphase.back();
+    ph.Lc = len;
+    ph.nsplit = (int)std::min<int64_t>(rpart_n, len);
+    std::vector<int64_t> lpo(n_local + 1, 0);
+    for (int q = 0; q < n_local; ++q) {
+      const int64_t d = h_dom_off[loc[q] + 1] - h_dom_off[loc[q]];
+      lpo[q + 1] = lpo[q] + len * d * d;
+    }
+    upload(ph.lp_off, lpo, st);
+    ph.p1.clear();
+    ph.p2.clear();
+    ph.p1.dense = ph.p2.dense = true;
+    ph.p1.tile = 128;
+    ph.p2.tile = 64;
+    int64_t toff = 0;
+    int qloc = 0;
+    for (int P = 0; P < n_pairs; ++P) {
+      const int d = (int)(h_dom_off[P + 1] - h_dom_off[P]);
+      const bool full = d == n_v;
+      double* T = Tbuf.as<double>() + toff;
+      // L operand of this pair: rows (jj, a) with leading dimension ld, one
+      // jj-block every jstep elements; full domains read L^J itself (J0 via jstride)
+      const void* Lb;
+      int64_t ld, jstep, jstride;
+      if (full) {
+        Lb = Lvv.p;
+        ld = n_v;
+        jstep = nv2;
+        jstride = nv2;
+      } else {
+        Lb = static_cast<const char*>(Lp.p) + (size_t)lpo[qloc++] * lsz;
+        ld = d;
+        jstep = (int64_t)d * d;
+        jstride = 0;
+      }
+      // phase 1: rows (jj, a) of L (M = len*d); rows jj*ld.. are contiguous when
+      // ld == d, and for full domains ld == n_v == d as well
+      ph.p1.add(T, d, (int)(len * d), d, {seg(op(Lb, ld, 1, nullptr, nullptr, lvv_f32, jstride),
+                                              op(C.as<double>() + h_cd_off[P], d, 1), d, 1.0)},
+                false);
+      for (int sp = 0; sp < ph.nsplit; ++sp) {
+        const int64_t j0 = len * sp / ph.nsplit, j1 = len * (sp + 1) / ph.nsplit;
+        std::vector<Seg> sg;
+        for (int64_t jj = j0; jj < j1; ++jj) {
+          const void* Lj = static_cast<const char*>(Lb) + (size_t)(jj * jstep) * lsz;
+          sg.push_back(seg(op(T + jj * d * d, d, 1), op(Lj, ld, 1, nullptr, nullptr, lvv_f32, jstride), d, 1.0));
+        }
+        double* Rout = (rpart_n > 1 ? Rpart.as<double>() + (int64_t)sp * sum_d2 : R.as<double>()) + h_cd_off[P];
+        ph.p2.add(Rout, d, d, d, sg, true);
+      }
+      toff += len * (int64_t)d * d;
+    }
+    ph.p1.upload_tables(st);
+    ph.p2.upload_tables(st);
+  }
+}
+
 void tci_plan::unpack_and_ladder(const double* x, int precision, hipStream_t st) {
   launch_unpack(tab, dmax, x, C.as<double>(), Ct.as<double>(), s.as<double>(), st);
   if (profiling) TCI_HIP(hipEventRecord(ev[2], st));
+  if (precision == TCI_SIGMA_FP64 && !ladder_fused) {
+    double* Rz = rpart_n > 1 ? Rpart.as<double>() : R.as<double>();
+    launch_fill(Rz, (int64_t)(rpart_n > 1 ? rpart_n : 1) * sum_d2, 0.0, st);
+    for (const auto& ch : lchunks) {
+      const LadderPhase& ph = lphase[ch.second];
+      if (n_local) {
+        PackTab K{pairs_local.as<int32_t>(), ph.lp_off.as<int64_t>(), n_local};
+        launch_pack_L(tab, K, Lvv.p, lvv_f32, ch.first, ph.Lc, Lp.p, st);
+      }
+      ph.p1.launch(st, ch.first);
+      ph.p2.launch(st, ch.first);
+    }
+    if (profiling) TCI_HIP(hipEventRecord(ev[3], st));
+    if (rpart_n > 1) ladder_reduce(Rpart.as<double>(), sum_d2, rpart_n, R.as<double>(), sum_d2, st);
+    return;
+  }
   if (precision == TCI_SIGMA_FP64) {
     LadderArgs a;
     a.dom_off = dom_off.as<int64_t>();
@@ -520,11 +652,11 @@ void tci_plan::sigma(const double* x, double* y, int flags, hipStream_t st) {
     TCI_HIP(hipEventSynchronize(ev[1]));
     float ms = 0.f;
     TCI_HIP(hipEventElapsedTime(&ms, ev[0], ev[1]));
-    prof_sigma_ms += ms;
-    ++prof_sigma_n;
+    prof_sigma_ms[precision] += ms;
+    ++prof_sigma_n[precision];
     TCI_HIP(hipEventElapsedTime(&ms, ev[2], ev[3]));
-    prof_ladder_ms += ms;
-    ++prof_ladder_n;
+    prof_ladder_ms[precision] += ms;
+    ++prof_ladder_n[precision];
   }
 }
 
@@ -628,20 +760,23 @@ int tci_profile_enable(tci_plan* plan, int32_t on) {
   for (hipEvent_t& e : plan->ev)
     if (!e) TCI_HIP(hipEventCreate(&e));
   plan->profiling = on != 0;
-  plan->prof_ladder_ms = plan->prof_sigma_ms = 0.0;
-  plan->prof_ladder_n = plan->prof_sigma_n = 0;
+  for (int q = 0; q < 3; ++q) {
+    plan->prof_ladder_ms[q] = plan->prof_sigma_ms[q] = 0.0;
+    plan->prof_ladder_n[q] = plan->prof_sigma_n[q] = 0;
+  }
   return TCI_OK;
   TCI_GUARD_END
 }
 
-int tci_profile_read(const tci_plan* plan, double* ladder_ms, int64_t* ladder_launches, double* sigma_ms,
-                     int64_t* sigma_calls) {
+int tci_profile_read(const tci_plan* plan, int32_t precision, double* ladder_ms, int64_t* ladder_launches,
+                     double* sigma_ms, int64_t* sigma_calls) {
   TCI_GUARD_BEGIN
   TCI_REQUIRE(plan, "null argument");
-  if (ladder_ms) *ladder_ms = plan->prof_ladder_ms;
-  if (ladder_launches) *ladder_launches = plan->prof_ladder_n;
-  if (sigma_ms) *sigma_ms = plan->prof_sigma_ms;
-  if (sigma_calls) *sigma_calls = plan->prof_sigma_n;
+  TCI_REQUIRE(precision >= 0 && precision < 3, "unknown precision");
+  if (ladder_ms) *ladder_ms = plan->prof_ladder_ms[precision];
+  if (ladder_launches) *ladder_launches = plan->prof_ladder_n[precision];
+  if (sigma_ms) *sigma_ms = plan->prof_sigma_ms[precision];
+  if (sigma_calls) *sigma_calls = plan->prof_sigma_n[precision];
   return TCI_OK;
   TCI_GUARD_END
 }

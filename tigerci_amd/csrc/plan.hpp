@@ -34,7 +34,21 @@ struct tci_plan {
   int lad_chunks = 1;
   // programs
   tci::GemmProgram ring, progY, progW, progGW, progGss;
-  tci::LadderLaunch ladder;
+  tci::LadderLaunch ladder;        // fused single-kernel ladder (TCI_LADDER=fused)
+  bool ladder_fused = false;
+  // two-phase ladder: per J-chunk of length Lc, T_P = L[D,D] C_P then R_P += sum T L
+  struct LadderPhase {
+    int64_t Lc = 0;                 // chunk length these tables are built for
+    int nsplit = 1;                 // K-splits of phase 2 (partial R buffers)
+    tci::GemmProgram p1, p2;        // FP64 dense programs
+    tci::DevBuf lp_off;             // packed-L offsets of the local-domain pairs
+  };
+  tci::DevBuf Lp, pairs_local;      // packed L^J[D,D] of local-domain pairs (one chunk)
+  int n_local = 0;
+  std::vector<LadderPhase> lphase;  // one per distinct chunk length
+  std::vector<std::pair<int64_t, int>> lchunks;  // (J0, index into lphase)
+  tci::DevBuf Tbuf, Rpart;
+  int rpart_n = 1;
   tci::PairTab tab{};
   // accounting
   double flops_ladder = 0.0, flops_sigma = 0.0, bytes_ladder = 0.0;
@@ -42,13 +56,14 @@ struct tci_plan {
   // event timing (tci_profile_enable)
   bool profiling = false;
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
-  double prof_ladder_ms = 0.0, prof_sigma_ms = 0.0;
-  int64_t prof_ladder_n = 0, prof_sigma_n = 0;
+  double prof_ladder_ms[3] = {0, 0, 0}, prof_sigma_ms[3] = {0, 0, 0};
+  int64_t prof_ladder_n[3] = {0, 0, 0}, prof_sigma_n[3] = {0, 0, 0};
   ~tci_plan();
 
   void build(const tci_problem& p);
   void sigma(const double* x, double* y, int flags, hipStream_t st);
   void unpack_and_ladder(const double* x, int precision, hipStream_t st);
+  void build_two_phase_ladder(hipStream_t st);
 };
 
 namespace tci {

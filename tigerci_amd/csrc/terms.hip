@@ -398,6 +398,24 @@ __global__ void build_mss_kernel(const double* __restrict__ Kov, const double* _
   }
 }
 
+// block (local pair q, jj): copy the d x d gathered block of L^{J0+jj}
+template <class LT>
+__global__ void pack_L_kernel(PairTab T, PackTab K, const LT* __restrict__ Lvv, int64_t J0, int64_t len,
+                              LT* __restrict__ Lp) {
+  const int q = blockIdx.x;
+  const int64_t jj = blockIdx.y;
+  const int P = K.pairs_local[q];
+  const int64_t off = T.dom_off[P];
+  const int d = (int)(T.dom_off[P + 1] - off);
+  const int64_t nv = T.n_v;
+  const LT* L = Lvv + (J0 + jj) * nv * nv;
+  LT* out = Lp + K.lp_off[q] + jj * (int64_t)d * d;
+  for (int r = threadIdx.y; r < d; r += blockDim.y) {
+    const int64_t row = (int64_t)T.dom_idx[off + r] * nv;
+    for (int c = threadIdx.x; c < d; c += blockDim.x) out[(int64_t)r * d + c] = L[row + T.dom_idx[off + c]];
+  }
+}
+
 __global__ void fill_kernel(double* p, int64_t n, double v) {
   for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x)
     p[e] = v;
@@ -493,6 +511,15 @@ void launch_vv_exch(const void* Lvv, bool f32, int64_t n_chol, int n_v, double* 
 void launch_build_mss(const double* Kov, const double* Jvv, int n_int, int n_v, double* Mss, hipStream_t st) {
   const int64_t n = (int64_t)n_int * n_v * n_int * n_v;
   hipLaunchKernelGGL(build_mss_kernel, dim3(gx(n)), dim3(256), 0, st, Kov, Jvv, n_int, n_v, Mss);
+  TCI_HIP(hipGetLastError());
+}
+
+void launch_pack_L(const PairTab& T, const PackTab& K, const void* Lvv, bool f32, int64_t J0, int64_t len, void* Lp,
+                   hipStream_t st) {
+  if (K.n_local == 0 || len == 0) return;
+  dim3 grid(K.n_local, (unsigned)len), block(64, 4);
+  if (f32) hipLaunchKernelGGL(pack_L_kernel<float>, grid, block, 0, st, T, K, (const float*)Lvv, J0, len, (float*)Lp);
+  else hipLaunchKernelGGL(pack_L_kernel<double>, grid, block, 0, st, T, K, (const double*)Lvv, J0, len, (double*)Lp);
   TCI_HIP(hipGetLastError());
 }
 

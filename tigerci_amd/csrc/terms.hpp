@@ -70,5 +70,13 @@ void launch_vv_exch(const void* Lvv, bool f32, int64_t n_chol, int n_v, double* 
 void launch_build_mss(const double* Kov, const double* Jvv, int n_int, int n_v, double* Mss,
                       hipStream_t st);
 void launch_fill(double* p, int64_t n, double v, hipStream_t st);
+// Lp[P][jj][r][c] = L^{J0+jj}[D_P r][D_P c] for the local-domain pairs of a chunk
+struct PackTab {
+  const int32_t* pairs_local;  // local-domain pair indices
+  const int64_t* lp_off;       // [n_local+1] element offsets into Lp (per chunk length)
+  int32_t n_local;
+};
+void launch_pack_L(const PairTab& T, const PackTab& K, const void* Lvv, bool f32, int64_t J0, int64_t len,
+                   void* Lp, hipStream_t st);
 
 }  // namespace tci
