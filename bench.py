@@ -3,11 +3,13 @@
 
 Metric: Davidson time-to-|dE|<1e-6 Eh, mixed precision vs FP64 on one GPU,
 with sigma TFLOP/s and the ladder kernel's roofline.  One step = one complete
-Davidson-Liu solve (lowest root, from the init_from_diag guess) of the
-synthetic workload with the split-precision ladder and FP64 refinement.
+Davidson-Liu solve (lowest --roots roots, from the init_from_diag guess) of
+the synthetic workload with the split-precision ladder and FP64 refinement;
+the FP64 solve of the same problem is timed once beside it.  --mode sigma
+times single sigma builds instead.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME]
-                    [--precision bf16x3|fp16x3|fp64] [--mode davidson|sigma]
+                    [--precision bf16x3|fp64] [--mode davidson|sigma] [--roots R]
 
 Prints one JSON line.  Inputs are generated on the host and resident in HBM
 before the timed region starts (plan creation is outside it).
@@ -74,10 +76,12 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="sigma_16x300x1200_full")
+    ap.add_argument("--workload", default=None,
+                    help="default: davidson_16x300x1200_full (davidson), sigma_16x300x1200_full (sigma)")
+    ap.add_argument("--roots", type=int, default=3)
     ap.add_argument("--precision", default="bf16x3")
     ap.add_argument("--mode", default="davidson", choices=["davidson", "sigma"])
-    ap.add_argument("--switch-tol", type=float, default=1e-4)
+    ap.add_argument("--switch-tol", type=float, default=1e-5)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--seed", type=int, default=1234)
     ap.add_argument("--copy-probe-mib", type=int, default=0,
@@ -91,6 +95,8 @@ def main() -> int:
     import tigerci_amd as tci
     from tigerci_amd.synthetic import workload, random_vector, WORKLOADS
 
+    if a.workload is None:
+        a.workload = "davidson_16x300x1200_full" if a.mode == "davidson" else "sigma_16x300x1200_full"
     t = time.perf_counter()
     syn = workload(a.workload, a.seed)
     log(f"[bench] generated {a.workload} in {time.perf_counter() - t:.1f} s")
@@ -105,7 +111,8 @@ def main() -> int:
            "pairs": int(len(syn.pairs)), "n_csf": int(plan.n_csf), "L_storage": ladder_dtype,
            "ladder_flops_per_sigma": info.flops_ladder, "sigma_flops": info.flops_sigma}
     out = {"metric": METRIC, "n_gpus": 1, "steps": a.steps, "warmup": a.warmup, "scaling": "weak",
-           "vs_baseline": None, "data": "synthetic (seeded N(0,1) exp(-J/80) Cholesky vectors)"}
+           "vs_baseline": None,
+           "data": f"synthetic (seeded N(0,1) exp(-J/80) x {WORKLOADS[a.workload][6]} Cholesky vectors)"}
 
     if a.mode == "sigma":
         x = tci.DeviceArray.from_numpy(random_vector(plan.n_csf, a.seed))
@@ -136,9 +143,22 @@ def main() -> int:
         print(json.dumps(out), flush=True)
         return 0
 
+    # ---- sigma precision check on one random vector ---------------------------
+    xr = random_vector(plan.n_csf, a.seed)
+    xd = tci.DeviceArray.from_numpy(xr)
+    yd = tci.DeviceArray(plan.n_csf)
+    plan.sigma(xd, yd, precision="fp64")
+    s64 = yd.numpy()
+    plan.sigma(xd, yd, precision=a.precision)
+    smx = yd.numpy()
+    sig_err = {"sigma_rel_err": float(np.linalg.norm(smx - s64) / np.linalg.norm(s64)),
+               "rayleigh_err_Eh": float(abs(xr @ smx - xr @ s64))}
+    del xd, yd
+
     # ---- Davidson: mixed (timed steps) and FP64 reference -------------------
     def solve(prec):
-        return plan.davidson(roots=1, energy_tol=1e-6, norm_tol=1e-6, precision=prec, switch_tol=a.switch_tol)
+        return plan.davidson(roots=a.roots, energy_tol=1e-6, norm_tol=1e-5, precision=prec,
+                             switch_tol=a.switch_tol)
 
     for _ in range(a.warmup):
         solve(a.precision)
@@ -148,39 +168,52 @@ def main() -> int:
     res_m = None
     for s in range(a.steps):
         e_m, res_m = solve(a.precision)
-        log(f"[bench] step {s}: {res_m}")
+        log(f"[bench] step {s}: {res_m} E={e_m}")
     tci.synchronize()
     t_m = (time.perf_counter() - t0) / a.steps
-    pm = plan.profile_read(a.precision) if a.precision != "fp64" else None
+    pm = plan.profile_read(a.precision)
     plan.profile(True)
     t0 = time.perf_counter()
     e_64, res_64 = solve("fp64")
     tci.synchronize()
     t_64 = time.perf_counter() - t0
     p64 = plan.profile_read("fp64")
-    log(f"[bench] fp64 solve: {res_64}")
+    log(f"[bench] fp64 solve: {res_64} E={e_64}")
     lad = p64["ladder_ms"] / max(p64["ladder_launches"], 1) / 1e3
     ach = info.flops_ladder / lad
     sig64 = p64["sigma_ms"] / max(p64["sigma_calls"], 1) / 1e3
     out.update({
         "value": t_m, "unit": "s", "higher_is_better": False, "ms_per_step": t_m * 1e3,
-        "dtype": "f64" if a.precision == "fp64" else f"{a.precision}+f64", "config": cfg,
+        "dtype": "f64" if a.precision == "fp64" else f"{a.precision}+f64", "config": dict(cfg, roots=a.roots),
         "fp64_seconds": t_64, "speedup_vs_fp64": t_64 / t_m,
-        "energy_mixed": float(e_m[0]), "energy_fp64": float(e_64[0]),
-        "energy_diff_uEh": float(abs(e_m[0] - e_64[0]) * 1e6),
+        "energies_mixed": [float(v) for v in e_m], "energies_fp64": [float(v) for v in e_64],
+        "energy_diff_uEh": float(np.abs(e_m - e_64).max() * 1e6),
+        "converged": {"mixed": res_m["converged"], "fp64": res_64["converged"]},
         "iterations": {"mixed": res_m["iterations"], "fp64": res_64["iterations"],
                        "sigma_mixed": res_m["sigma_mixed"], "sigma_fp64_in_mixed": res_m["sigma_fp64"],
                        "sigma_fp64": res_64["sigma_fp64"]},
+        "sigma_check": sig_err,
         "sigma_tflops_fp64": info.flops_sigma / sig64 / 1e12,
-        "roofline": {"bound": "mfma", "achieved": ach / 1e12, "peak": PEAK["fp64"] / 1e12, "unit": "TFLOP/s",
-                     "frac": ach / PEAK["fp64"], "traffic": None, "kernel": "ladder_f64_kernel (FP64 MFMA)",
-                     "avg_launch_ms": lad * 1e3, "launches": p64["ladder_launches"]},
     })
-    if pm and pm["ladder_launches"]:
+    roof64 = {"bound": "mfma", "achieved": ach / 1e12, "peak": PEAK["fp64"] / 1e12, "unit": "TFLOP/s",
+              "frac": ach / PEAK["fp64"], "traffic": None,
+              "kernel": "FP64 ladder (dgemm_kernel phases 1+2, per sigma)",
+              "avg_launch_ms": lad * 1e3, "launches": p64["ladder_launches"]}
+    if a.precision != "fp64" and pm["ladder_launches"]:
         lm = pm["ladder_ms"] / pm["ladder_launches"] / 1e3
-        out["roofline_mixed"] = {"bound": "mfma", "achieved": info.flops_ladder / lm / 1e12,
-                                 "peak": PEAK[a.precision] / 1e12, "unit": "TFLOP/s",
-                                 "frac": info.flops_ladder / lm / PEAK[a.precision], "avg_launch_ms": lm * 1e3}
+        sgm = pm["sigma_ms"] / max(pm["sigma_calls"], 1) / 1e3
+        out["sigma_tflops_mixed"] = info.flops_sigma / sgm / 1e12
+        # algorithmic (FP64-equivalent) ladder flops against the dense bf16
+        # peak; each product costs 3 bf16 MFMAs, so MFMA busy = 3 x frac
+        out["roofline"] = {"bound": "mfma", "achieved": info.flops_ladder / lm / 1e12,
+                           "peak": PEAK[a.precision] / 1e12, "unit": "TFLOP/s",
+                           "frac": info.flops_ladder / lm / PEAK[a.precision], "traffic": None,
+                           "kernel": "bf16x3 ladder (xgemm_kernel phases 1+2, per sigma)",
+                           "mfma_flops_per_algorithmic_flop": 3,
+                           "avg_launch_ms": lm * 1e3, "launches": pm["ladder_launches"]}
+        out["roofline_fp64"] = roof64
+    else:
+        out["roofline"] = roof64
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(syn, plan, res_64["sigma_fp64"])
     print(json.dumps(out), flush=True)

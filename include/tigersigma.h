@@ -43,7 +43,7 @@ enum tci_dtype { TCI_F64 = 0, TCI_F32 = 1 };
 enum tci_precision {
   TCI_SIGMA_FP64 = 0,    /* FP64 MFMA throughout */
   TCI_SIGMA_BF16X3 = 1,  /* ladder as hi*hi + hi*lo + lo*hi BF16 MFMA, FP32 accumulate */
-  TCI_SIGMA_FP16X3 = 2   /* same split in scaled FP16 */
+  TCI_SIGMA_FP16X3 = 2   /* reserved (scaled FP16 split): rejected with TCI_ERR_ARG */
 };
 
 /* flag bits for tci_sigma_apply (low byte = tci_precision) */
@@ -115,7 +115,8 @@ typedef struct tci_davidson_params {
   double energy_tol;      /* |dE| convergence (ENERGY TOLERANCE, default 1e-6) */
   double norm_tol;        /* residual norm convergence (RESIDUAL TOLERANCE) */
   int32_t precision;      /* tci_precision of the expansion sigmas */
-  double switch_tol;      /* mixed: switch to FP64 sigma once every |dE| < switch_tol */
+  double switch_tol;      /* mixed: switch to FP64 sigma once every root has |dE| < switch_tol
+                             or a residual below norm_tol (forced at maxit/2) */
   int32_t max_vectors;    /* subspace collapse budget (0 -> maxit*roots) */
   int32_t verbose;        /* 1: print the reference-style iteration table */
 } tci_davidson_params;
@@ -124,7 +125,7 @@ typedef struct tci_davidson_result {
   int32_t iterations;
   int32_t sigma_fp64;     /* number of FP64 sigma builds */
   int32_t sigma_mixed;    /* number of mixed-precision sigma builds */
-  int32_t converged;      /* all roots converged */
+  int32_t converged;      /* all roots met a convergence criterion before maxit */
   double seconds;         /* wall time of the solve */
   double seconds_sigma;   /* of which inside sigma builds */
 } tci_davidson_result;

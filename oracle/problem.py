@@ -114,11 +114,13 @@ class Layout:
 def make_problem(n_int: int, n_v: int, n_chol: int, seed: int,
                  domain_range: tuple[int, int] | None = None,
                  weak_below: int = 0,
-                 decay: float = 80.0) -> Problem:
+                 decay: float = 80.0, l_scale: float = 1.0) -> Problem:
     """Seeded synthetic problem.
 
     * eps_i ~ U[-1.5,-0.3], eps_a ~ U[0.5,5.0]
-    * L^J_pq = L^J_qp ~ N(0,1) * exp(-J/decay) over all orbitals
+    * L^J_pq = L^J_qp ~ N(0,1) * exp(-J/decay) * l_scale over all orbitals
+      (l_scale < 1 gives molecule-sized two-electron integrals and a
+      diagonally dominant H; the same draws for every l_scale)
     * full pair domains unless ``domain_range=(lo,hi)``: then each pair's
       domain is a sorted random subset of size U[lo,hi] (inclusive) and pairs
       with size < ``weak_below`` carry no doubles.
@@ -130,7 +132,7 @@ def make_problem(n_int: int, n_v: int, n_chol: int, seed: int,
                           rng_e.uniform(0.5, 5.0, n_v)])
     npair_all = n_orb * (n_orb + 1) // 2
     rng_l = np.random.default_rng([seed, 1])
-    scale = np.exp(-np.arange(n_chol) / decay)
+    scale = np.exp(-np.arange(n_chol) / decay) * l_scale
     L = rng_l.standard_normal((npair_all, n_chol)) * scale[None, :]
     pairs, doms = [], []
     rng_d = np.random.default_rng([seed, 2])

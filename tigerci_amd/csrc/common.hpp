@@ -122,7 +122,7 @@ struct Task {
   int32_t m0, n0;   // this tile's origin
   int32_t seg0, seg1;
   int32_t accumulate;
-  int32_t out_f32;   // C is float* (split-precision kernel only)
+  int32_t out_split;  // split kernel only: C is a split-row 16-bit matrix, ldc in 16-bit elements
   int32_t orow_div;  // dense kernels: if > 0, logical row r is stored at row (r % div) * (M / div) + r / div
 };
 
@@ -134,20 +134,43 @@ struct Task {
 struct GemmProgram {
   int tile = 64;
   bool dense = false;  // contiguous row-major operands only: dense_gemm.hip kernel (tile 64 or 128)
+  bool split = false;  // split-row operands, C = sum A B^T: dense_split.hip kernel (tile 64 or 128)
   std::vector<Task> tasks;
   std::vector<Seg> segs;
   DevBuf d_tasks, d_segs;
   void clear() { tasks.clear(); segs.clear(); }
   // add one output matrix: tiles over (M,N), all sharing `segs_for_output`
   void add(double* C, int64_t ldc, int M, int N, const std::vector<Seg>& s, bool accumulate,
-           bool out_f32 = false, int orow_div = 0);
+           bool out_split = false, int orow_div = 0);
   void upload_tables(hipStream_t st);
   void launch(hipStream_t st, int64_t J0 = 0) const;
-  // split-precision kernel (gemm_split.hip): 128 x 128 tiles only
-  void launch_split(int precision, hipStream_t st, int64_t J0 = 0, float scale_a = 1.f, float scale_b = 1.f) const;
+  void launch_split(int precision, hipStream_t st, int64_t J0 = 0) const;
 };
 
 void launch_ggemm(int tile, const Task* tasks, const Seg* segs, int64_t ntasks, int64_t J0, hipStream_t st);
 void launch_dgemm(int tile, const Task* tasks, const Seg* segs, int64_t ntasks, int64_t J0, hipStream_t st);
+void launch_xgemm(int precision, int tile, const Task* tasks, const Seg* segs, int64_t ntasks, int64_t J0,
+                  hipStream_t st);
+
+// ---------------------------------------------------------------------------
+// Split-row storage of a matrix for the split-precision GEMM.  A row of K
+// values is padded with zeros to Kp = K rounded up to 8 and kept as Kp / 8
+// chunks of 16 bf16 values: the high parts hi = bf16(x) of 8 consecutive
+// elements, then their low parts lo = bf16(x - hi).  Row stride 2 Kp; every
+// chunk is 32-byte aligned when the matrix base is.
+inline int64_t xpad(int64_t K) { return (K + 7) & ~(int64_t)7; }
+__device__ __forceinline__ int64_t xidx(int64_t k) { return ((k >> 3) << 4) + (k & 7); }
+// write the split form of v[0..7] as one chunk
+__device__ __forceinline__ void put_split8(uint16_t* chunk, const float* v) {
+  bf16x8 h, l;
+#pragma unroll
+  for (int e = 0; e < 8; ++e) {
+    h[e] = (__bf16)v[e];
+    l[e] = (__bf16)(v[e] - (float)h[e]);
+  }
+  __attribute__((address_space(1))) bf16x8* p = (__attribute__((address_space(1))) bf16x8*)chunk;
+  p[0] = h;
+  p[1] = l;
+}
 
 }  // namespace tci

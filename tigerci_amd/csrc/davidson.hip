@@ -274,7 +274,7 @@ extern "C" int tci_davidson_run(tci_plan* plan, const tci_davidson_params* prm, 
     }
     std::vector<int> root2idx(roots);
     std::iota(root2idx.begin(), root2idx.end(), 0);
-    std::vector<char> conv(roots, 0);
+    std::vector<char> conv(roots, 0), met(roots, 0), ready(roots, 0);
     std::vector<double> energy(roots, 0.0), dE(roots, 0.0);
     std::vector<double> h((size_t)mv * mv, 0.0);  // row-major, leading dim mv
     std::vector<double> w, Y, Yprev;
@@ -304,10 +304,15 @@ extern "C" int tci_davidson_run(tci_plan* plan, const tci_davidson_params* prm, 
       bool restarted = false;
       // mixed -> FP64 switch
       bool do_switch = false;
+      // in the mixed phase no root is declared converged: once every root has
+      // settled (|dE| < switch_tol now, or a residual below norm_tol in the
+      // previous iteration) the subspace collapses to the Ritz vectors and the
+      // solve continues with FP64 sigma; half of maxit forces the switch
       if (precision != TCI_SIGMA_FP64 && it > 1) {
         do_switch = true;
         for (int r = 0; r < roots; ++r)
-          if (!conv[r] && !(dE[r] < prm->switch_tol)) do_switch = false;
+          if (!conv[r] && !(dE[r] < prm->switch_tol) && !ready[r]) do_switch = false;
+        if (it >= std::max(2, maxit / 2)) do_switch = true;
       }
       const bool budget = (m > 4 * roots && m + 3 * roots > mv);
       if (do_switch || budget) {
@@ -419,10 +424,15 @@ extern "C" int tci_davidson_run(tci_plan* plan, const tci_davidson_params* prm, 
         // is_converged (davidson.f90:564-602)
         bool halt = false;
         const bool in_restart = restarts.back() || (restarts.size() >= 2 && restarts[restarts.size() - 2]);
-        if (!in_restart && !skip_conv_once) {
+        if (precision != TCI_SIGMA_FP64) {
+          ready[r] = rn < prm->norm_tol ? 1 : 0;
+        } else if (!in_restart) {
+          // right after the switch dE compares an FP64 with a mixed energy:
+          // only the residual criterion applies
           if (rn < prm->norm_tol) halt = true;
-          else if (dE[r] < prm->energy_tol && it > 1 && precision == TCI_SIGMA_FP64) halt = true;
+          else if (!skip_conv_once && dE[r] < prm->energy_tol && it > 1) halt = true;
         }
+        if (halt) met[r] = 1;
         if (it == maxit) halt = true;
         if (halt) {
           conv[r] = 1;
@@ -463,7 +473,7 @@ extern "C" int tci_davidson_run(tci_plan* plan, const tci_davidson_params* prm, 
       result->iterations = std::min(it, maxit);
       result->sigma_fp64 = n64;
       result->sigma_mixed = nmix;
-      result->converged = all_conv ? 1 : 0;
+      result->converged = std::all_of(met.begin(), met.end(), [](char c) { return c != 0; }) ? 1 : 0;
       result->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
       result->seconds_sigma = t_sigma;
     }

@@ -1,0 +1,225 @@
+// Dense grouped GEMM in split precision on BF16 MFMA:
+//   C[tile] (+)= sum_seg A_seg B_seg^T,   A_seg: M x K,  B_seg: N x K,
+// both operands in split-row storage (hi/lo bf16 chunks, common.hpp), so
+// A B^T ~= Ahi Bhi^T + Ahi Blo^T + Alo Bhi^T with FP32 accumulation
+// (v_mfma_f32_16x16x32_bf16).  hi + lo keeps ~17 significant bits of each
+// operand; the dropped lo lo^T term is of the same relative size (2^-17).
+//
+// Because both operands are k-contiguous and pre-split, staging is a pure
+// copy: per 32-wide k-step a thread moves 32-byte chunks (hi 8 + lo 8) from
+// global memory into LDS, no conversion in the loop.  LDS keeps each operand
+// as a hi plane and a lo plane of [row][32] bf16 (64 B per row) with the
+// 16-byte chunk index XOR-swizzled by (row >> 1) & 3: any 8 consecutive lanes
+// of a fragment read or of a staging write then touch 8 distinct 16-byte bank
+// groups.  A lane's fragment (row = lane % 16, k-chunk = lane / 16) is one
+// 16-byte LDS read.
+//
+// Tiles TS x TS: TS = 128 (4 waves of 64 x 64, 64 KiB LDS, 2 workgroups per
+// CU) or TS = 64 (4 waves of 32 x 32, 32 KiB, 4 workgroups per CU); two LDS
+// stages with the global loads of stage s+1 in flight during the MFMAs of
+// stage s; each wave's loop specialised on its count of 16 x 16 blocks inside
+// the output.  Output: FP64 (optionally accumulated) or, with out_split, a
+// split-row matrix (the operand of a following split GEMM).
+// MFMA 16x16x32 bf16: A[i][k] lane i + 16(k/8) element k%8, B[k][j] lane
+// j + 16(k/8) element k%8, D[i][j] lane j + 16(i/4) register i%4.
+#include "common.hpp"
+
+namespace tci {
+
+namespace {
+
+constexpr int XK = 32;
+typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+
+// element offset of k-chunk ch (8 values) of LDS row `row`
+__device__ __forceinline__ int xsw(int row, int ch) { return row * XK + (((ch ^ (row >> 1)) & 3) << 3); }
+
+__device__ __forceinline__ bool xadvance(const Seg* segs, int seg1, int& seg, int& kb) {
+  kb += XK;
+  if (kb >= segs[seg].K) {
+    kb = 0;
+    ++seg;
+    while (seg < seg1 && segs[seg].K <= 0) ++seg;
+  }
+  return seg < seg1;
+}
+
+// Rows r0 + (tid >> 2) + 64 q (q < TS / 64), k-chunk tid & 3 of one operand;
+// rows beyond `rows` and chunks beyond K read as zero (addresses clamped).
+template <int TS>
+__device__ __forceinline__ void xload(const Op& O, int64_t J0, int rows, int r0, int kb, int K, int tid,
+                                      u32x4 (*v)[2]) {
+  constexpr int NL = TS / 64;
+  const uint16_t* base = static_cast<const uint16_t*>(O.base) + J0 * O.jstride;
+  const int k = kb + 8 * (tid & 3);
+  const bool kok = k < K;
+  const int64_t koff = (int64_t)(min(k, K - 1) >> 3) << 4;
+#pragma unroll
+  for (int q = 0; q < NL; ++q) {
+    const int r = r0 + (tid >> 2) + 64 * q;
+    const bool ok = kok && r < rows;
+    const u32x4* p = reinterpret_cast<const u32x4*>(base + (int64_t)min(r, rows - 1) * O.rs + koff);
+    const u32x4 h = gld(p), l = gld(p + 1);
+    const u32x4 z = {0u, 0u, 0u, 0u};
+    v[q][0] = ok ? h : z;
+    v[q][1] = ok ? l : z;
+  }
+}
+
+template <int TS>
+__device__ __forceinline__ void xstore(int tid, const u32x4 (*v)[2], uint16_t* hi, uint16_t* lo) {
+  constexpr int NL = TS / 64;
+#pragma unroll
+  for (int q = 0; q < NL; ++q) {
+    const int e = xsw((tid >> 2) + 64 * q, tid & 3);
+    *reinterpret_cast<u32x4*>(hi + e) = v[q][0];
+    *reinterpret_cast<u32x4*>(lo + e) = v[q][1];
+  }
+}
+
+template <int TS>
+__device__ __forceinline__ void xstage(const Seg& S, const Task& t, int64_t J0, int kb, int tid, u32x4 (*va)[2],
+                                       u32x4 (*vb)[2]) {
+  xload<TS>(S.A, J0, t.M, t.m0, kb, S.K, tid, va);
+  xload<TS>(S.B, J0, t.N, t.n0, kb, S.K, tid, vb);
+}
+
+template <int TS, int NBM, int NBN>
+__device__ __forceinline__ void xgemm_body(const Task& t, const Seg* __restrict__ segs, int64_t J0, int seg0,
+                                           uint16_t* sm, int tid, int wm, int wn, int li, int lg) {
+  constexpr int NL = TS / 64, PL = TS * XK, STG = 4 * PL;
+  f4 acc[NBM > 0 ? NBM : 1][NBN > 0 ? NBN : 1];
+#pragma unroll
+  for (int i = 0; i < NBM; ++i)
+#pragma unroll
+    for (int j = 0; j < NBN; ++j) acc[i][j] = f4{0.f, 0.f, 0.f, 0.f};
+  u32x4 va[NL][2], vb[NL][2];
+  int seg = seg0, kb = 0;
+  xstage<TS>(segs[seg], t, J0, kb, tid, va, vb);
+  xstore<TS>(tid, va, sm, sm + PL);
+  xstore<TS>(tid, vb, sm + 2 * PL, sm + 3 * PL);
+  bool more = xadvance(segs, t.seg1, seg, kb);
+  __syncthreads();
+  int buf = 0;
+  while (true) {
+    if (more) xstage<TS>(segs[seg], t, J0, kb, tid, va, vb);
+    if (NBM > 0 && NBN > 0) {
+      const uint16_t* base = sm + buf * STG;
+      bf16x8 ah[NBM > 0 ? NBM : 1], al[NBM > 0 ? NBM : 1], bh[NBN > 0 ? NBN : 1], bl[NBN > 0 ? NBN : 1];
+#pragma unroll
+      for (int i = 0; i < NBM; ++i) {
+        const int e = xsw(wm + i * 16 + li, lg);
+        ah[i] = *reinterpret_cast<const bf16x8*>(base + e);
+        al[i] = *reinterpret_cast<const bf16x8*>(base + PL + e);
+      }
+#pragma unroll
+      for (int j = 0; j < NBN; ++j) {
+        const int e = xsw(wn + j * 16 + li, lg);
+        bh[j] = *reinterpret_cast<const bf16x8*>(base + 2 * PL + e);
+        bl[j] = *reinterpret_cast<const bf16x8*>(base + 3 * PL + e);
+      }
+#pragma unroll
+      for (int i = 0; i < NBM; ++i)
+#pragma unroll
+        for (int j = 0; j < NBN; ++j) {
+          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[i], bh[j], acc[i][j], 0, 0, 0);
+          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bl[j], acc[i][j], 0, 0, 0);
+          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bh[j], acc[i][j], 0, 0, 0);
+        }
+    }
+    if (!more) break;
+    uint16_t* nb = sm + (buf ^ 1) * STG;
+    xstore<TS>(tid, va, nb, nb + PL);
+    xstore<TS>(tid, vb, nb + 2 * PL, nb + 3 * PL);
+    __syncthreads();
+    buf ^= 1;
+    more = xadvance(segs, t.seg1, seg, kb);
+  }
+#pragma unroll
+  for (int i = 0; i < NBM; ++i)
+#pragma unroll
+    for (int j = 0; j < NBN; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = t.m0 + wm + i * 16 + 4 * lg + r;
+        const int col = t.n0 + wn + j * 16 + li;
+        if (row < t.M && col < t.N) {
+          const float v = acc[i][j][r];
+          if (t.out_split) {
+            __attribute__((address_space(1))) __bf16* p = (__attribute__((address_space(1))) __bf16*)(
+                reinterpret_cast<uint16_t*>(t.C) + (int64_t)row * t.ldc + xidx(col));
+            const __bf16 h = (__bf16)v;
+            p[0] = h;
+            p[8] = (__bf16)(v - (float)h);
+          } else {
+            __attribute__((address_space(1))) double* p =
+                (__attribute__((address_space(1))) double*)(t.C + (int64_t)row * t.ldc + col);
+            *p = t.accumulate ? *p + (double)v : (double)v;
+          }
+        }
+      }
+}
+
+template <int TS>
+__global__ __launch_bounds__(256, TS == 128 ? 2 : 4) void xgemm_kernel(const Task* __restrict__ tasks,
+                                                                       const Seg* __restrict__ segs, int64_t J0) {
+  constexpr int WB = TS / 32;  // 16x16 blocks per wave side
+  const Task t = tasks[blockIdx.x];
+  __shared__ __attribute__((aligned(16))) uint16_t sm[2 * 4 * TS * XK];
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, w = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int li = lane & 15, lg = lane >> 4;
+  const int wm = (w >> 1) * (TS / 2), wn = (w & 1) * (TS / 2);
+  const int nbm = __builtin_amdgcn_readfirstlane(max(0, min(WB, (t.M - (t.m0 + wm) + 15) >> 4)));
+  const int nbn = __builtin_amdgcn_readfirstlane(max(0, min(WB, (t.N - (t.n0 + wn) + 15) >> 4)));
+  int seg0 = t.seg0;
+  while (seg0 < t.seg1 && segs[seg0].K <= 0) ++seg0;
+  if (seg0 >= t.seg1) {
+    if (!t.accumulate)
+      for (int e = tid; e < TS * TS; e += 256) {
+        const int row = t.m0 + e / TS, col = t.n0 + e % TS;
+        if (row < t.M && col < t.N) {
+          if (t.out_split) {
+            uint16_t* p = reinterpret_cast<uint16_t*>(t.C) + (int64_t)row * t.ldc + xidx(col);
+            p[0] = 0;
+            p[8] = 0;
+          } else {
+            t.C[(int64_t)row * t.ldc + col] = 0.0;
+          }
+        }
+      }
+    return;
+  }
+#define TCI_XB(M_, N_) \
+  case M_ * 5 + N_:    \
+    if constexpr (M_ <= WB && N_ <= WB) xgemm_body<TS, M_, N_>(t, segs, J0, seg0, sm, tid, wm, wn, li, lg); \
+    break;
+  switch (nbm * 5 + nbn) {
+    TCI_XB(0, 0) TCI_XB(0, 1) TCI_XB(0, 2) TCI_XB(0, 3) TCI_XB(0, 4)
+    TCI_XB(1, 0) TCI_XB(1, 1) TCI_XB(1, 2) TCI_XB(1, 3) TCI_XB(1, 4)
+    TCI_XB(2, 0) TCI_XB(2, 1) TCI_XB(2, 2) TCI_XB(2, 3) TCI_XB(2, 4)
+    TCI_XB(3, 0) TCI_XB(3, 1) TCI_XB(3, 2) TCI_XB(3, 3) TCI_XB(3, 4)
+    TCI_XB(4, 0) TCI_XB(4, 1) TCI_XB(4, 2) TCI_XB(4, 3) TCI_XB(4, 4)
+    default: break;
+  }
+#undef TCI_XB
+}
+
+}  // namespace
+
+void launch_xgemm(int precision, int tile, const Task* tasks, const Seg* segs, int64_t ntasks, int64_t J0,
+                  hipStream_t st) {
+  TCI_REQUIRE(precision == TCI_SIGMA_BF16X3, "the split GEMM implements bf16x3 only");
+  if (ntasks <= 0) return;
+  TCI_REQUIRE(ntasks < ((int64_t)1 << 31), "too many gemm tasks");
+  if (tile == 128) hipLaunchKernelGGL(xgemm_kernel<128>, dim3((unsigned)ntasks), dim3(256), 0, st, tasks, segs, J0);
+  else hipLaunchKernelGGL(xgemm_kernel<64>, dim3((unsigned)ntasks), dim3(256), 0, st, tasks, segs, J0);
+  TCI_HIP(hipGetLastError());
+}
+
+void GemmProgram::launch_split(int precision, hipStream_t st, int64_t J0) const {
+  TCI_REQUIRE(split, "not a split program");
+  launch_xgemm(precision, tile, d_tasks.as<Task>(), d_segs.as<Seg>(), (int64_t)tasks.size(), J0, st);
+}
+
+}  // namespace tci

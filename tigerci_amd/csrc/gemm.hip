@@ -381,10 +381,18 @@ void launch_ggemm(int tile, const Task* tasks, const Seg* segs, int64_t ntasks, 
 }
 
 void GemmProgram::add(double* C, int64_t ldc, int M, int N, const std::vector<Seg>& s, bool accumulate,
-                      bool out_f32, int orow_div) {
+                      bool out_split, int orow_div) {
   if (M <= 0 || N <= 0) return;
+  TCI_REQUIRE(!out_split || (split && !accumulate), "split-row output needs a split program and no accumulation");
+  if (split)
+    for (const Seg& x : s)
+      TCI_REQUIRE(x.alpha == 1.0 && orow_div == 0 && x.A.base && x.B.base && !x.A.rmap && !x.A.cmap && !x.B.rmap &&
+                      !x.B.cmap && x.A.rs % 16 == 0 && x.B.rs % 16 == 0 && x.A.jstride % 16 == 0 &&
+                      x.B.jstride % 16 == 0 && (reinterpret_cast<uintptr_t>(x.A.base) & 31) == 0 &&
+                      (reinterpret_cast<uintptr_t>(x.B.base) & 31) == 0,
+                  "split GEMM segments need 32-byte aligned split-row operands and alpha = 1");
   TCI_REQUIRE(orow_div == 0 || (dense && M % orow_div == 0), "row permutation needs a dense program and M % div == 0");
-  if (dense)
+  if (dense && !split)
     for (const Seg& x : s)
       TCI_REQUIRE(x.A.cs == 1 && x.B.cs == 1 && !x.A.rmap && !x.A.cmap && !x.B.rmap && !x.B.cmap && x.A.base &&
                       x.B.base,
@@ -404,7 +412,7 @@ void GemmProgram::add(double* C, int64_t ldc, int M, int N, const std::vector<Se
       t.seg0 = seg0;
       t.seg1 = seg1;
       t.accumulate = accumulate ? 1 : 0;
-      t.out_f32 = out_f32 ? 1 : 0;
+      t.out_split = out_split ? 1 : 0;
       t.orow_div = orow_div;
       tasks.push_back(t);
     }
@@ -416,8 +424,10 @@ void GemmProgram::upload_tables(hipStream_t st) {
 }
 
 void GemmProgram::launch(hipStream_t st, int64_t J0) const {
+  TCI_REQUIRE(!split, "split program launched without a precision");
   if (dense) launch_dgemm(tile, d_tasks.as<Task>(), d_segs.as<Seg>(), (int64_t)tasks.size(), J0, st);
   else launch_ggemm(tile, d_tasks.as<Task>(), d_segs.as<Seg>(), (int64_t)tasks.size(), J0, st);
 }
 
 }  // namespace tci
+

@@ -1,19 +1,283 @@
-// Split-precision ladder (BF16x3 / scaled FP16x3) -- filled in below.
+// Split-precision (bf16x3) two-phase ladder  R_P = sum_J L^J[D,D] C_P L^J[D,D].
+//
+// Both phases are C = A B^T products with k-contiguous split-row operands
+// (dense_split.hip).  With T' = L C^T per Cholesky vector,
+//   phase 1  T'_P[(jj,a)][c] = sum_k L^J[a][k] C_P[c][k]      (A = L rows, B = C rows)
+//   phase 2  R_P[n][a]      += sum_jj sum_c L^J[n][c] T'_P[jj][a][c]
+//                                                             (A = L rows, B = T' rows)
+// and L^J symmetric, so R = L T'^T = L C L.  Phase 1 writes T' directly in
+// split-row form; phase 2 accumulates FP32 tile sums into FP64 R (split into
+// partial R buffers along K when the outputs alone cannot fill the chip).
+// Split inputs: L^J[v][v] of all J once per plan (full-domain pairs), packed
+// L^J[D,D] of local-domain pairs per chunk, C_P per sigma call.
 #include "plan.hpp"
 
 namespace tci {
 
-void mixed_prepare_L(tci_plan& P, hipStream_t st) {
-  (void)P;
-  (void)st;
-  throw Error(TCI_ERR_ARG, "mixed-precision ladder not built yet");
+namespace {
+
+// Cx[P] rows r of C_P (d x d, FP64) in split-row form, row stride 2 xpad(d)
+__global__ void split_blocks_kernel(PairTab T, const int64_t* __restrict__ cx_off, const double* __restrict__ C,
+                                    uint16_t* __restrict__ Cx) {
+  const int P = blockIdx.y;
+  const int d = (int)(T.dom_off[P + 1] - T.dom_off[P]);
+  const int nch = (d + 7) >> 3;
+  const double* src = C + T.cd_off[P];
+  uint16_t* dst = Cx + cx_off[P];
+  for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < d * nch; e += gridDim.x * blockDim.x) {
+    const int r = e / nch, ch = e - r * nch;
+    float v[8];
+#pragma unroll
+    for (int t = 0; t < 8; ++t) {
+      const int k = 8 * ch + t;
+      v[t] = k < d ? (float)src[(int64_t)r * d + k] : 0.f;
+    }
+    put_split8(dst + (int64_t)r * 16 * nch + 16 * ch, v);
+  }
+}
+
+// Lx[(J, a)] = split row a of L^J[v][v] for all J, row stride 2 xpad(n_v)
+template <class LT>
+__global__ void split_lvv_kernel(const LT* __restrict__ Lvv, int64_t rows, int n_v, uint16_t* __restrict__ Lx) {
+  const int nch = (n_v + 7) >> 3;
+  const int64_t n = rows * nch;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t r = e / nch;
+    const int ch = (int)(e - r * nch);
+    float v[8];
+#pragma unroll
+    for (int t = 0; t < 8; ++t) {
+      const int k = 8 * ch + t;
+      v[t] = k < n_v ? (float)Lvv[r * n_v + k] : 0.f;
+    }
+    put_split8(Lx + r * 16 * nch + 16 * ch, v);
+  }
+}
+
+// Lpx[q] rows (jj, r) = split row r of L^{J0+jj}[D_P, D_P] for local pair q
+template <class LT>
+__global__ void pack_L_split_kernel(PairTab T, const int32_t* __restrict__ pairs_local,
+                                    const int64_t* __restrict__ lp_off, const LT* __restrict__ Lvv, int64_t J0,
+                                    int len, uint16_t* __restrict__ Lpx) {
+  const int q = blockIdx.y;
+  const int P = pairs_local[q];
+  const int64_t off = T.dom_off[P];
+  const int d = (int)(T.dom_off[P + 1] - off);
+  const int nch = (d + 7) >> 3;
+  const int64_t nv = T.n_v;
+  const int64_t n = (int64_t)len * d * nch;
+  uint16_t* out = Lpx + lp_off[q];
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t row = e / nch;  // jj * d + r
+    const int ch = (int)(e - row * nch);
+    const int64_t jj = row / d;
+    const int r = (int)(row - jj * d);
+    const LT* L = Lvv + (J0 + jj) * nv * nv + (int64_t)T.dom_idx[off + r] * nv;
+    float v[8];
+#pragma unroll
+    for (int t = 0; t < 8; ++t) {
+      const int k = 8 * ch + t;
+      v[t] = k < d ? (float)L[T.dom_idx[off + min(k, d - 1)]] : 0.f;
+    }
+    put_split8(out + row * 16 * nch + 16 * ch, v);
+  }
+}
+
+inline unsigned blocks_for(int64_t work, unsigned cap) {
+  return (unsigned)std::min<int64_t>(std::max<int64_t>(cdiv(work, 256), 1), cap);
+}
+
+Op xop(const void* base, int64_t rs, int64_t jstride = 0) {
+  Op o;
+  o.base = base;
+  o.rs = rs;
+  o.cs = 1;
+  o.rmap = nullptr;
+  o.cmap = nullptr;
+  o.jstride = jstride;
+  o.f32 = 0;
+  return o;
+}
+
+Seg xseg(const Op& A, const Op& B, int K) {
+  Seg s;
+  s.A = A;
+  s.B = B;
+  s.K = K;
+  s.alpha = 1.0;
+  return s;
+}
+
+}  // namespace
+
+void mixed_prepare(tci_plan& P, hipStream_t st) {
+  tci_plan::MixedLadder& M = P.mx;
+  if (M.ready) return;
+  const int n_pairs = P.n_pairs, n_v = P.n_v;
+  const int64_t nvp = xpad(n_v), n_chol = P.n_chol;
+  std::vector<int64_t> cxo(n_pairs + 1, 0);
+  std::vector<int32_t> loc;
+  int64_t sum_x = 0, sum_local_x = 0;  // sum over pairs of d * 2 xpad(d) (16-bit elements)
+  bool any_full = false;
+  for (int Q = 0; Q < n_pairs; ++Q) {
+    const int64_t d = P.h_dom_off[Q + 1] - P.h_dom_off[Q];
+    const int64_t x = d * 2 * xpad(d);
+    cxo[Q + 1] = cxo[Q] + x;
+    sum_x += x;
+    if (d == n_v) {
+      any_full = true;
+    } else {
+      loc.push_back(Q);
+      sum_local_x += x;
+    }
+  }
+  M.bytes = 0;
+  upload(M.cx_off, cxo, st);
+  M.Cx.alloc((size_t)std::max<int64_t>(sum_x, 16) * 2);
+  M.n_local = (int)loc.size();
+  upload(M.pairs_local, loc, st);
+  M.bytes += (int64_t)M.Cx.bytes;
+  if (any_full) {
+    const int64_t rows = n_chol * n_v;
+    M.Lx.alloc((size_t)rows * 2 * nvp * 2);
+    if (P.lvv_f32)
+      hipLaunchKernelGGL(split_lvv_kernel<float>, dim3(blocks_for(rows * (nvp / 8), 65536)), dim3(256), 0, st,
+                         static_cast<const float*>(P.Lvv.p), rows, n_v, M.Lx.as<uint16_t>());
+    else
+      hipLaunchKernelGGL(split_lvv_kernel<double>, dim3(blocks_for(rows * (nvp / 8), 65536)), dim3(256), 0, st,
+                         static_cast<const double*>(P.Lvv.p), rows, n_v, M.Lx.as<uint16_t>());
+    TCI_HIP(hipGetLastError());
+    M.bytes += (int64_t)M.Lx.bytes;
+  }
+  // chunk length: T' (all pairs) + packed L (local pairs), 2 bytes per element
+  const double budget = 8.0e9;
+  const double per_J = 2.0 * (double)(sum_x + sum_local_x);
+  int64_t Lc = std::max<int64_t>(1, std::min<int64_t>(n_chol, (int64_t)(budget / std::max(per_J, 1.0))));
+  for (int64_t q = Lc; q >= std::max<int64_t>(1, Lc / 2); --q)
+    if (n_chol % q == 0) {
+      Lc = q;
+      break;
+    }
+  M.Tx.alloc((size_t)Lc * sum_x * 2);
+  // padding columns of T' are never written by phase 1 and must read as zero
+  launch_fill(M.Tx.as<double>(), (int64_t)(M.Tx.bytes / 8), 0.0, st);
+  M.bytes += (int64_t)M.Tx.bytes;
+  if (sum_local_x) {
+    M.Lpx.alloc((size_t)Lc * sum_local_x * 2);
+    M.bytes += (int64_t)M.Lpx.bytes;
+  }
+  const int tile2 = 64;
+  int64_t tiles2 = 0;
+  for (int Q = 0; Q < n_pairs; ++Q) {
+    const int64_t d = P.h_dom_off[Q + 1] - P.h_dom_off[Q];
+    tiles2 += cdiv(d, tile2) * cdiv(d, tile2);
+  }
+  M.rpart_n = (int)std::max<int64_t>(1, std::min<int64_t>(16, cdiv(2048, std::max<int64_t>(tiles2, 1))));
+  if (M.rpart_n > 1) {
+    M.Rpart.alloc((size_t)M.rpart_n * P.sum_d2 * 8);
+    M.bytes += (int64_t)M.Rpart.bytes;
+  }
+  M.phase.clear();
+  M.chunks.clear();
+  std::vector<int64_t> lens;
+  for (int64_t J0 = 0; J0 < n_chol; J0 += Lc) {
+    const int64_t len = std::min(Lc, n_chol - J0);
+    int idx = -1;
+    for (size_t q = 0; q < lens.size(); ++q)
+      if (lens[q] == len) idx = (int)q;
+    if (idx < 0) {
+      idx = (int)lens.size();
+      lens.push_back(len);
+    }
+    M.chunks.push_back({J0, idx});
+  }
+  const uint16_t* Lx = M.Lx.as<uint16_t>();
+  const uint16_t* Cx = M.Cx.as<uint16_t>();
+  uint16_t* Tx = M.Tx.as<uint16_t>();
+  const int64_t jstride_full = (int64_t)n_v * 2 * nvp;
+  for (int64_t len : lens) {
+    M.phase.emplace_back();
+    tci_plan::LadderPhase& ph = M.phase.back();
+    ph.Lc = len;
+    ph.nsplit = (int)std::min<int64_t>(M.rpart_n, len);
+    std::vector<int64_t> lpo(loc.size() + 1, 0);
+    for (size_t q = 0; q < loc.size(); ++q) {
+      const int64_t d = P.h_dom_off[loc[q] + 1] - P.h_dom_off[loc[q]];
+      lpo[q + 1] = lpo[q] + len * d * 2 * xpad(d);
+    }
+    upload(ph.lp_off, lpo, st);
+    ph.p1.clear();
+    ph.p2.clear();
+    ph.p1.split = ph.p2.split = true;
+    ph.p1.tile = 128;
+    ph.p2.tile = tile2;
+    int64_t toff = 0;
+    int qloc = 0;
+    for (int Q = 0; Q < n_pairs; ++Q) {
+      const int d = (int)(P.h_dom_off[Q + 1] - P.h_dom_off[Q]);
+      const int64_t ldx = 2 * xpad(d);
+      const bool full = d == n_v;
+      // L operand: rows (jj, a), one jj-block every jstep elements
+      const uint16_t* Lb;
+      int64_t jstep, jstride;
+      if (full) {
+        Lb = Lx;
+        jstep = jstride_full;
+        jstride = jstride_full;
+      } else {
+        Lb = M.Lpx.as<uint16_t>() + lpo[qloc++];
+        jstep = (int64_t)d * ldx;
+        jstride = 0;
+      }
+      uint16_t* Tq = Tx + toff;
+      ph.p1.add(reinterpret_cast<double*>(Tq), ldx, (int)(len * d), d,
+                {xseg(xop(Lb, ldx, jstride), xop(Cx + cxo[Q], ldx), d)}, false, true);
+      for (int sp = 0; sp < ph.nsplit; ++sp) {
+        const int64_t j0 = len * sp / ph.nsplit, j1 = len * (sp + 1) / ph.nsplit;
+        std::vector<Seg> sg;
+        for (int64_t jj = j0; jj < j1; ++jj)
+          sg.push_back(xseg(xop(Lb + jj * jstep, ldx, jstride), xop(Tq + jj * d * ldx, ldx), d));
+        double* Rout = (M.rpart_n > 1 ? M.Rpart.as<double>() + (int64_t)sp * P.sum_d2 : P.R.as<double>()) +
+                       P.h_cd_off[Q];
+        ph.p2.add(Rout, d, d, d, sg, true);
+      }
+      toff += len * d * ldx;
+    }
+    ph.p1.upload_tables(st);
+    ph.p2.upload_tables(st);
+  }
+  TCI_HIP(hipStreamSynchronize(st));
+  M.ready = true;
 }
 
 void mixed_ladder(tci_plan& P, int precision, hipStream_t st) {
-  (void)P;
-  (void)precision;
-  (void)st;
-  throw Error(TCI_ERR_ARG, "mixed-precision ladder not built yet");
+  TCI_REQUIRE(precision == TCI_SIGMA_BF16X3, "split-precision sigma implements bf16x3 only");
+  mixed_prepare(P, st);
+  tci_plan::MixedLadder& M = P.mx;
+  hipLaunchKernelGGL(split_blocks_kernel, dim3(blocks_for((int64_t)P.dmax * ((P.dmax + 7) / 8), 256), P.n_pairs),
+                     dim3(256), 0, st, P.tab, M.cx_off.as<int64_t>(), P.C.as<double>(), M.Cx.as<uint16_t>());
+  TCI_HIP(hipGetLastError());
+  double* Rz = M.rpart_n > 1 ? M.Rpart.as<double>() : P.R.as<double>();
+  launch_fill(Rz, (int64_t)M.rpart_n * P.sum_d2, 0.0, st);
+  for (const auto& ch : M.chunks) {
+    const tci_plan::LadderPhase& ph = M.phase[ch.second];
+    if (M.n_local) {
+      const unsigned gxb = 1024;
+      if (P.lvv_f32)
+        hipLaunchKernelGGL(pack_L_split_kernel<float>, dim3(gxb, M.n_local), dim3(256), 0, st, P.tab,
+                           M.pairs_local.as<int32_t>(), ph.lp_off.as<int64_t>(), static_cast<const float*>(P.Lvv.p),
+                           ch.first, (int)ph.Lc, M.Lpx.as<uint16_t>());
+      else
+        hipLaunchKernelGGL(pack_L_split_kernel<double>, dim3(gxb, M.n_local), dim3(256), 0, st, P.tab,
+                           M.pairs_local.as<int32_t>(), ph.lp_off.as<int64_t>(), static_cast<const double*>(P.Lvv.p),
+                           ch.first, (int)ph.Lc, M.Lpx.as<uint16_t>());
+      TCI_HIP(hipGetLastError());
+    }
+    ph.p1.launch_split(precision, st, ch.first);
+    ph.p2.launch_split(precision, st, ch.first);
+  }
+  if (P.profiling) TCI_HIP(hipEventRecord(P.ev[3], st));
+  if (M.rpart_n > 1) ladder_reduce(M.Rpart.as<double>(), P.sum_d2, M.rpart_n, P.R.as<double>(), P.sum_d2, st);
 }
 
 }  // namespace tci

@@ -592,7 +592,11 @@ void tci_plan::build_two_phase_ladder(hipStream_t st) {
 void tci_plan::unpack_and_ladder(const double* x, int precision, hipStream_t st) {
   launch_unpack(tab, dmax, x, C.as<double>(), Ct.as<double>(), s.as<double>(), st);
   if (profiling) TCI_HIP(hipEventRecord(ev[2], st));
-  if (precision == TCI_SIGMA_FP64 && !ladder_fused) {
+  if (precision != TCI_SIGMA_FP64) {
+    mixed_ladder(*this, precision, st);
+    return;
+  }
+  if (!ladder_fused) {
     double* Rz = rpart_n > 1 ? Rpart.as<double>() : R.as<double>();
     launch_fill(Rz, (int64_t)(rpart_n > 1 ? rpart_n : 1) * sum_d2, 0.0, st);
     for (const auto& ch : lchunks) {
@@ -608,21 +612,17 @@ void tci_plan::unpack_and_ladder(const double* x, int precision, hipStream_t st)
     if (rpart_n > 1) ladder_reduce(Rpart.as<double>(), sum_d2, rpart_n, R.as<double>(), sum_d2, st);
     return;
   }
-  if (precision == TCI_SIGMA_FP64) {
-    LadderArgs a;
-    a.dom_off = dom_off.as<int64_t>();
-    a.dom_idx = dom_idx.as<int32_t>();
-    a.cd_off = cd_off.as<int64_t>();
-    a.Cd = C.as<double>();
-    a.Lvv = Lvv.p;
-    a.lvv_f32 = lvv_f32;
-    a.n_v = n_v;
-    a.partial = lad_part.as<double>();
-    a.chunk_stride = sum_d2;
-    ladder.launch(a, st);
-  } else {
-    mixed_ladder(*this, precision, st);
-  }
+  LadderArgs a;
+  a.dom_off = dom_off.as<int64_t>();
+  a.dom_idx = dom_idx.as<int32_t>();
+  a.cd_off = cd_off.as<int64_t>();
+  a.Cd = C.as<double>();
+  a.Lvv = Lvv.p;
+  a.lvv_f32 = lvv_f32;
+  a.n_v = n_v;
+  a.partial = lad_part.as<double>();
+  a.chunk_stride = sum_d2;
+  ladder.launch(a, st);
   if (profiling) TCI_HIP(hipEventRecord(ev[3], st));
   ladder_reduce(lad_part.as<double>(), sum_d2, lad_chunks, R.as<double>(), sum_d2, st);
 }

@@ -25,11 +25,8 @@ struct tci_plan {
   tci::DevBuf isect_off, isect, inv_off, inv_q, inv_pos, gpart_off;
   // integrals (device)
   tci::DevBuf Lvv, Lov, Kov, Jvv, Mss, O4, O3, vvd, vvx, diag;
-  // mixed-precision copies of L_vv (hi/lo split)
-  tci::DevBuf Lvv_hi, Lvv_lo;
   // work buffers
   tci::DevBuf C, Ct, R, s, g, gss, gW, Y, W, gpart, part, sig0, lad_part;
-  tci::DevBuf C_hi, C_lo;
   int n_split = 1;
   int lad_chunks = 1;
   // programs
@@ -40,7 +37,7 @@ struct tci_plan {
   struct LadderPhase {
     int64_t Lc = 0;                 // chunk length these tables are built for
     int nsplit = 1;                 // K-splits of phase 2 (partial R buffers)
-    tci::GemmProgram p1, p2;        // FP64 dense programs
+    tci::GemmProgram p1, p2;        // dense programs (FP64, or split for the mixed ladder)
     tci::DevBuf lp_off;             // packed-L offsets of the local-domain pairs
   };
   tci::DevBuf Lp, pairs_local;      // packed L^J[D,D] of local-domain pairs (one chunk)
@@ -49,6 +46,24 @@ struct tci_plan {
   std::vector<std::pair<int64_t, int>> lchunks;  // (J0, index into lphase)
   tci::DevBuf Tbuf, Rpart;
   int rpart_n = 1;
+  // split-precision (bf16x3) two-phase ladder, built on first use:
+  //   phase 1 T'_P[(jj,a)][c] = sum_k L^J[a][k] C_P[c][k]   (T' = L C^T, split rows)
+  //   phase 2 R_P[n][a]     += sum_jj sum_c L^J[n][c] T'_P[jj][a][c]
+  // every operand k-contiguous in split-row storage (L^J symmetric)
+  struct MixedLadder {
+    bool ready = false;
+    tci::DevBuf Lx;                 // split rows of L^J[v][v] for all J (full-domain pairs present)
+    tci::DevBuf Lpx;                // split rows of L^J[D,D] of local-domain pairs (one chunk)
+    tci::DevBuf pairs_local;        // indices of the local-domain pairs
+    int n_local = 0;
+    tci::DevBuf Cx, cx_off;         // split rows of the C_P blocks
+    tci::DevBuf Tx;                 // T' of one chunk
+    tci::DevBuf Rpart;              // partial R buffers of phase 2 K-splits
+    int rpart_n = 1;
+    std::vector<LadderPhase> phase;
+    std::vector<std::pair<int64_t, int>> chunks;
+    int64_t bytes = 0;
+  } mx;
   tci::PairTab tab{};
   // accounting
   double flops_ladder = 0.0, flops_sigma = 0.0, bytes_ladder = 0.0;
@@ -67,7 +82,9 @@ struct tci_plan {
 };
 
 namespace tci {
-// mixed-precision ladder (ladder_mixed.hip)
-void mixed_prepare_L(tci_plan& P, hipStream_t st);
+// split-precision ladder (ladder_mixed.hip): R (zeroed inside) = sum_J L^J C L^J
+// from the FP64 C blocks of the plan; records ev[3] before the final reduction
+// when profiling
+void mixed_prepare(tci_plan& P, hipStream_t st);
 void mixed_ladder(tci_plan& P, int precision, hipStream_t st);
 }  // namespace tci

@@ -2,8 +2,8 @@
 
 Every array follows the tci_problem layout of include/tigersigma.h:
   * eps_i ~ U[-1.5,-0.3] (internal), eps_a ~ U[0.5,5.0] (virtual)
-  * L^J_pq = L^J_qp ~ N(0,1) exp(-J/80), one record of n_chol values per
-    orbital pair p>=q (record p(p+1)/2+q)
+  * L^J_pq = L^J_qp ~ N(0,1) exp(-J/80) l_scale, one record of n_chol values
+    per orbital pair p>=q (record p(p+1)/2+q)
   * pair walks (i,j), i>=j, in the order (0,0),(1,0),(1,1),(2,0),...; full
     domains, or seeded random sorted subsets of size U[lo,hi] with pairs below
     ``weak_below`` dropped (singles only)
@@ -18,10 +18,14 @@ from dataclasses import dataclass
 import numpy as np
 
 WORKLOADS = {
-    # name: (n_int, n_v, n_chol, domain_range, weak_below, chol dtype)
-    "sigma_10x100x400_full": (10, 100, 400, None, 0, np.float64),
-    "sigma_16x300x1200_full": (16, 300, 1200, None, 0, np.float64),
-    "local_40x1000x3000_dom40-400": (40, 1000, 3000, (40, 400), 80, np.float32),
+    # name: (n_int, n_v, n_chol, domain_range, weak_below, chol dtype, l_scale)
+    "sigma_10x100x400_full": (10, 100, 400, None, 0, np.float64, 1.0),
+    "sigma_16x300x1200_full": (16, 300, 1200, None, 0, np.float64, 1.0),
+    "local_40x1000x3000_dom40-400": (40, 1000, 3000, (40, 400), 80, np.float32, 1.0),
+    # Davidson solves: the same shapes with L scaled to molecule-sized
+    # integrals (diagonally dominant H, reference weight ~0.95)
+    "davidson_16x300x1200_full": (16, 300, 1200, None, 0, np.float64, 0.03),
+    "davidson_40x1000x3000_dom40-400": (40, 1000, 3000, (40, 400), 80, np.float32, 0.03),
 }
 
 
@@ -38,13 +42,14 @@ class Synthetic:
 
 
 def make_synthetic(n_int: int, n_v: int, n_chol: int, seed: int, domain_range=None, weak_below: int = 0,
-                   decay: float = 80.0, dtype=np.float64, chunk_rows: int = 1 << 14) -> Synthetic:
+                   decay: float = 80.0, dtype=np.float64, chunk_rows: int = 1 << 14,
+                   l_scale: float = 1.0) -> Synthetic:
     n_orb = n_int + n_v
     rng_e = np.random.default_rng([seed, 0])
     eps = np.concatenate([rng_e.uniform(-1.5, -0.3, n_int), rng_e.uniform(0.5, 5.0, n_v)])
     npair = n_orb * (n_orb + 1) // 2
     rng_l = np.random.default_rng([seed, 1])
-    scale = np.exp(-np.arange(n_chol) / decay)
+    scale = np.exp(-np.arange(n_chol) / decay) * l_scale
     L = np.empty((npair, n_chol), dtype=dtype)
     for r0 in range(0, npair, chunk_rows):
         r1 = min(npair, r0 + chunk_rows)
@@ -69,8 +74,8 @@ def make_synthetic(n_int: int, n_v: int, n_chol: int, seed: int, domain_range=No
 
 
 def workload(name: str, seed: int = 1234) -> Synthetic:
-    n_int, n_v, n_chol, dr, weak, dt = WORKLOADS[name]
-    return make_synthetic(n_int, n_v, n_chol, seed, domain_range=dr, weak_below=weak, dtype=dt)
+    n_int, n_v, n_chol, dr, weak, dt, ls = WORKLOADS[name]
+    return make_synthetic(n_int, n_v, n_chol, seed, domain_range=dr, weak_below=weak, dtype=dt, l_scale=ls)
 
 
 def random_vector(n: int, seed: int) -> np.ndarray:
