@@ -105,3 +105,24 @@ def test_sigma_config1_size():
     x = random_ci_vector(prob, 1)
     ref = oracle_sigma(prob, x)
     assert _rel(_gpu_sigma(plan, x), ref) < TOL
+
+
+def test_dense_pair_path_matches_gather_path():
+    """Full domains take the dense ordered-pair products for the ring and
+    4-internal terms; TCI_DENSE_PAIRS=0 forces the per-walk gather GEMMs.
+    Both must agree with the oracle (and so with each other)."""
+    _say("case: dense pair products vs gather path")
+    prob = make_problem(5, 37, 50, 21)
+    ints = Integrals(prob)
+    x = random_ci_vector(prob, 5)
+    ref = oracle_sigma(prob, x, ints)
+    dense = tci.Plan.from_problem(prob)
+    os.environ["TCI_DENSE_PAIRS"] = "0"
+    try:
+        gather = tci.Plan.from_problem(prob)
+    finally:
+        del os.environ["TCI_DENSE_PAIRS"]
+    a, b = _gpu_sigma(dense, x), _gpu_sigma(gather, x)
+    assert _rel(a, ref) < TOL and _rel(b, ref) < TOL
+    am, bm = _gpu_sigma(dense, x, precision="bf16x3"), _gpu_sigma(gather, x, precision="bf16x3")
+    assert _rel(am, ref) < 5e-5 and _rel(bm, ref) < 5e-5

@@ -21,7 +21,11 @@
 // the output.  Output: FP64 (optionally accumulated) or, with out_split, a
 // split-row matrix (the operand of a following split GEMM).
 // MFMA 16x16x32 bf16: A[i][k] lane i + 16(k/8) element k%8, B[k][j] lane
-// j + 16(k/8) element k%8, D[i][j] lane j + 16(i/4) register i%4.
+// j + 16(k/8) element k%8, D[i][j] lane j + 16(i/4) register i%4.  The
+// kernel feeds the B fragment as the MFMA's A operand, so a lane's four
+// result registers are four consecutive output columns of one row: FP64
+// results leave as 32-byte vectors and split results as 8-byte hi and lo
+// groups.
 #include "common.hpp"
 
 namespace tci {
@@ -44,8 +48,11 @@ __device__ __forceinline__ bool xadvance(const Seg* segs, int seg1, int& seg, in
   return seg < seg1;
 }
 
-// Rows r0 + (tid >> 2) + 64 q (q < TS / 64), k-chunk tid & 3 of one operand;
-// rows beyond `rows` and chunks beyond K read as zero (addresses clamped).
+// Rows r0 + (tid >> 2) + 64 q (q < TS / 64), k-chunk tid & 3 of one operand.
+// Rows beyond `rows` and chunks beyond K load from a zero page instead, so no
+// select waits on the loaded data.
+__device__ __attribute__((aligned(32))) u32x4 xzero[2];
+
 template <int TS>
 __device__ __forceinline__ void xload(const Op& O, int64_t J0, int rows, int r0, int kb, int K, int tid,
                                       u32x4 (*v)[2]) {
@@ -53,16 +60,13 @@ __device__ __forceinline__ void xload(const Op& O, int64_t J0, int rows, int r0,
   const uint16_t* base = static_cast<const uint16_t*>(O.base) + J0 * O.jstride;
   const int k = kb + 8 * (tid & 3);
   const bool kok = k < K;
-  const int64_t koff = (int64_t)(min(k, K - 1) >> 3) << 4;
+  const int64_t koff = (int64_t)(k >> 3) << 4;
 #pragma unroll
   for (int q = 0; q < NL; ++q) {
     const int r = r0 + (tid >> 2) + 64 * q;
-    const bool ok = kok && r < rows;
-    const u32x4* p = reinterpret_cast<const u32x4*>(base + (int64_t)min(r, rows - 1) * O.rs + koff);
-    const u32x4 h = gld(p), l = gld(p + 1);
-    const u32x4 z = {0u, 0u, 0u, 0u};
-    v[q][0] = ok ? h : z;
-    v[q][1] = ok ? l : z;
+    const u32x4* p = (kok && r < rows) ? reinterpret_cast<const u32x4*>(base + (int64_t)r * O.rs + koff) : xzero;
+    v[q][0] = gld(p);
+    v[q][1] = gld(p + 1);
   }
 }
 
@@ -118,14 +122,21 @@ __device__ __forceinline__ void xgemm_body(const Task& t, const Seg* __restrict_
         bh[j] = *reinterpret_cast<const bf16x8*>(base + 2 * PL + e);
         bl[j] = *reinterpret_cast<const bf16x8*>(base + 3 * PL + e);
       }
+      // B fragment as the MFMA's first operand: acc[i][j] holds the transposed
+      // 16 x 16 block, so each lane ends with 4 consecutive output columns.
+      // Small terms first; the three passes keep 16 independent chains.
 #pragma unroll
       for (int i = 0; i < NBM; ++i)
 #pragma unroll
-        for (int j = 0; j < NBN; ++j) {
-          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[i], bh[j], acc[i][j], 0, 0, 0);
-          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bl[j], acc[i][j], 0, 0, 0);
-          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bh[j], acc[i][j], 0, 0, 0);
-        }
+        for (int j = 0; j < NBN; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh[j], al[i], acc[i][j], 0, 0, 0);
+#pragma unroll
+      for (int i = 0; i < NBM; ++i)
+#pragma unroll
+        for (int j = 0; j < NBN; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bl[j], ah[i], acc[i][j], 0, 0, 0);
+#pragma unroll
+      for (int i = 0; i < NBM; ++i)
+#pragma unroll
+        for (int j = 0; j < NBN; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh[j], ah[i], acc[i][j], 0, 0, 0);
     }
     if (!more) break;
     uint16_t* nb = sm + (buf ^ 1) * STG;
@@ -135,29 +146,45 @@ __device__ __forceinline__ void xgemm_body(const Task& t, const Seg* __restrict_
     buf ^= 1;
     more = xadvance(segs, t.seg1, seg, kb);
   }
+  // lane (li, lg) of block (i, j) holds output row 16i + li, columns 16j + 4lg .. +3
 #pragma unroll
   for (int i = 0; i < NBM; ++i)
 #pragma unroll
-    for (int j = 0; j < NBN; ++j)
+    for (int j = 0; j < NBN; ++j) {
+      const int row = t.m0 + wm + i * 16 + li;
+      const int col = t.n0 + wn + j * 16 + 4 * lg;
+      if (row < t.M && col < t.N) {
+        const f4 v = acc[i][j];
+        if (t.out_split) {
+          // split-row output: padding columns past N (< the padded row length)
+          // receive the exact zeros of the masked B rows
+          typedef __bf16 b4 __attribute__((ext_vector_type(4)));
+          b4 h, l;
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int row = t.m0 + wm + i * 16 + 4 * lg + r;
-        const int col = t.n0 + wn + j * 16 + li;
-        if (row < t.M && col < t.N) {
-          const float v = acc[i][j][r];
-          if (t.out_split) {
-            __attribute__((address_space(1))) __bf16* p = (__attribute__((address_space(1))) __bf16*)(
-                reinterpret_cast<uint16_t*>(t.C) + (int64_t)row * t.ldc + xidx(col));
-            const __bf16 h = (__bf16)v;
-            p[0] = h;
-            p[8] = (__bf16)(v - (float)h);
-          } else {
-            __attribute__((address_space(1))) double* p =
-                (__attribute__((address_space(1))) double*)(t.C + (int64_t)row * t.ldc + col);
-            *p = t.accumulate ? *p + (double)v : (double)v;
+          for (int r = 0; r < 4; ++r) {
+            h[r] = (__bf16)v[r];
+            l[r] = (__bf16)(v[r] - (float)h[r]);
           }
+          __attribute__((address_space(1))) b4* p = (__attribute__((address_space(1))) b4*)(
+              reinterpret_cast<uint16_t*>(t.C) + (int64_t)row * t.ldc + xidx(col));
+          p[0] = h;
+          p[2] = l;
+        } else if (col + 3 < t.N) {
+          typedef double d4u __attribute__((ext_vector_type(4), aligned(8)));  // ldc is arbitrary
+          __attribute__((address_space(1))) d4u* p =
+              (__attribute__((address_space(1))) d4u*)(t.C + (int64_t)row * t.ldc + col);
+          const d4u o = {(double)v[0], (double)v[1], (double)v[2], (double)v[3]};
+          if (t.accumulate) *p = *p + o;
+          else *p = o;
+        } else {
+          __attribute__((address_space(1))) double* p =
+              (__attribute__((address_space(1))) double*)(t.C + (int64_t)row * t.ldc + col);
+#pragma unroll
+          for (int r = 0; r < 4; ++r)
+            if (col + r < t.N) p[r] = t.accumulate ? p[r] + (double)v[r] : (double)v[r];
         }
       }
+    }
 }
 
 template <int TS>

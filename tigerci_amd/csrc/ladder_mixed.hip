@@ -246,6 +246,7 @@ void mixed_prepare(tci_plan& P, hipStream_t st) {
     ph.p1.upload_tables(st);
     ph.p2.upload_tables(st);
   }
+  if (P.dense_pairs) dense_pairs_build_split(P, st);
   TCI_HIP(hipStreamSynchronize(st));
   M.ready = true;
 }

@@ -17,6 +17,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 #include <set>
@@ -163,6 +164,7 @@ void tci_plan::build(const tci_problem& p) {
   launch_gather_Lov(Lpm.as<double>(), n_chol, n_int, n_v, Lov.as<double>(), st);
 
   // ---- one-off integral assembly on MFMA -------------------------------------
+  std::vector<double> o4((size_t)n_int * n_int * n_int * n_int);
   {
     std::vector<int32_t> rec_ov(niv), rec_oo((size_t)n_int * n_int), rec_vo(niv), rec_aa(n_v),
         rec_pairs((size_t)n_int * (n_int + 1) / 2), rec_vv((size_t)n_v * n_v);
@@ -220,7 +222,6 @@ void tci_plan::build(const tci_problem& p) {
     Mss.alloc((size_t)niv * niv * 8);
     launch_build_mss(Kov.as<double>(), Jvv.as<double>(), n_int, n_v, Mss.as<double>(), st);
     // E_ref = 2 sum_i eps_i - sum_ik [2 (ii|kk) - (ik|ik)]  (closed shell, Fock = diag(eps))
-    std::vector<double> o4((size_t)n_int * n_int * n_int * n_int);
     TCI_HIP(hipMemcpy(o4.data(), O4.p, o4.size() * 8, hipMemcpyDeviceToHost));
     auto O = [&](int a, int b, int c, int d) { return o4[(((size_t)a * n_int + b) * n_int + c) * n_int + d]; };
     double e = 0.0;
@@ -343,6 +344,10 @@ void tci_plan::build(const tci_problem& p) {
     const double* b = X + h_cd_off[Q];
     return transposed ? op(b, 1, d, rm, cm) : op(b, d, 1, rm, cm);
   };
+  {
+    const char* env = std::getenv("TCI_DENSE_PAIRS");
+    dense_pairs = dense_pairs_eligible(*this) && !(env && env[0] == '0');
+  }
   ring.clear();
   ring.tile = 128;
   for (int P = 0; P < n_pairs; ++P) {
@@ -350,7 +355,7 @@ void tci_plan::build(const tci_problem& p) {
     const int d = dsz(P);
     const int32_t* DP = DL(P);
     std::vector<Seg> sg;
-    for (int k = 0; k < n_int; ++k) {
+    for (int k = 0; k < n_int && !dense_pairs; ++k) {
       const int Q1 = pairid[(size_t)i * n_int + k];
       if (Q1 >= 0) {  // Z^{ij}: Ct^{ik} K^{kj} - C^{ik} J^{kj} - J^{kj} C^{ik}
         const int dq = dsz(Q1);
@@ -379,6 +384,7 @@ void tci_plan::build(const tci_problem& p) {
     ring.add(R.as<double>() + h_cd_off[P], d, d, d, sg, true);
   }
   ring.upload_tables(st);
+  if (dense_pairs) dense_pairs_build(*this, o4, st);
   // Y[(J,a)][i] = sum_c L^J_ac s_i^c
   progY.clear();
   progY.add(Y.as<double>(), n_int, (int)(n_chol * n_v), n_int,
@@ -468,12 +474,17 @@ void tci_plan::build(const tci_problem& p) {
     f_ring += 2.0 * 2.0 * d * d * (double)n_chol;
   }
   bytes_ladder = bytes;
+  if (dense_pairs) {  // executed products: Z (two N^3 terms), Z3, the 4-internal GEMM, the X terms
+    const double N = (double)niv, d2 = (double)n_v * n_v;
+    f_ring = 6.0 * N * N * N + 2.0 * n_pairs * 2.0 * n_pairs * d2 + n_pairs * 2.0 * 2.0 * d2 * (double)n_chol;
+  }
   flops_sigma = flops_ladder + f_ring + 2.0 * (double)n_chol * n_v * niv * 3.0 + 2.0 * (double)niv * niv;
   TCI_HIP(hipDeviceSynchronize());
   device_bytes = 0;
   for (DevBuf* b : {&pairs, &dom_off, &dom_idx, &cd_off, &aa_off, &ab_off, &eps, &pos, &xmap, &isect_off, &isect,
                     &inv_off, &inv_q, &inv_pos, &gpart_off, &Lvv, &Lov, &Kov, &Jvv, &Mss, &O4, &O3, &vvd, &vvx,
-                    &diag, &C, &Ct, &R, &s, &g, &gss, &gW, &Y, &W, &gpart, &part, &sig0, &lad_part})
+                    &diag, &C, &Ct, &R, &s, &g, &gss, &gW, &Y, &W, &gpart, &part, &sig0, &lad_part, &Jbig, &CRb,
+                    &CtRb, &CCb, &Zb, &Z3b, &CTb, &holeA})
     device_bytes += (int64_t)b->bytes;
 }
 
@@ -634,8 +645,12 @@ void tci_plan::sigma(const double* x, double* y, int flags, hipStream_t st) {
   if (profiling) TCI_HIP(hipEventRecord(ev[0], st));
   unpack_and_ladder(x, precision, st);
   launch_rterms(tab, dmax, x, s.as<double>(), Kov.as<double>(), O3.as<double>(), R.as<double>(), st);
-  HoleTab H{isect_off.as<int64_t>(), isect.as<int32_t>()};
-  launch_hole(tab, H, dmax, C.as<double>(), O4.as<double>(), R.as<double>(), st);
+  if (dense_pairs) {
+    dense_pairs_terms(*this, precision, st);
+  } else {
+    HoleTab H{isect_off.as<int64_t>(), isect.as<int32_t>()};
+    launch_hole(tab, H, dmax, C.as<double>(), O4.as<double>(), R.as<double>(), st);
+  }
   progY.launch(st);
   ring.launch(st);
   progGss.launch(st);

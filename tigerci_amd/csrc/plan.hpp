@@ -31,6 +31,10 @@ struct tci_plan {
   int lad_chunks = 1;
   // programs
   tci::GemmProgram ring, progY, progW, progGW, progGss;
+  // full-domain pair terms as dense products (pairs_full.hip)
+  bool dense_pairs = false;
+  tci::DevBuf Jbig, CRb, CtRb, CCb, Zb, Z3b, CTb, holeA;
+  tci::GemmProgram ringZ, ringZ3, holeG;
   tci::LadderLaunch ladder;        // fused single-kernel ladder (TCI_LADDER=fused)
   bool ladder_fused = false;
   // two-phase ladder: per J-chunk of length Lc, T_P = L[D,D] C_P then R_P += sum T L
@@ -60,6 +64,9 @@ struct tci_plan {
     tci::DevBuf Tx;                 // T' of one chunk
     tci::DevBuf Rpart;              // partial R buffers of phase 2 K-splits
     int rpart_n = 1;
+    // dense pair terms in split precision (full domains)
+    tci::DevBuf Kx, Jx, CRx, CtRx, CCx;
+    tci::GemmProgram ringZx, ringZ3x;
     std::vector<LadderPhase> phase;
     std::vector<std::pair<int64_t, int>> chunks;
     int64_t bytes = 0;
@@ -87,4 +94,9 @@ namespace tci {
 // when profiling
 void mixed_prepare(tci_plan& P, hipStream_t st);
 void mixed_ladder(tci_plan& P, int precision, hipStream_t st);
+// dense pair terms (pairs_full.hip)
+bool dense_pairs_eligible(const tci_plan& P);
+void dense_pairs_build(tci_plan& P, const std::vector<double>& o4, hipStream_t st);
+void dense_pairs_build_split(tci_plan& P, hipStream_t st);
+void dense_pairs_terms(tci_plan& P, int precision, hipStream_t st);
 }  // namespace tci

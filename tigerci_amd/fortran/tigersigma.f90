@@ -1,0 +1,141 @@
+! Fortran 2003 binding of the tigersigma C ABI (include/tigersigma.h).
+!
+! A Fortran Davidson driver (davidson_liu, davidson.f90:47) reaches the GPU
+! sigma build through these interfaces; the drop-in htimesc body that uses
+! them is shown in the integration notes of this repository.  All sizes are
+! explicit 64-bit integers, so the module compiles unchanged with or without
+! -fdefault-integer-8 (the reference builds with 8-byte default integers).
+module tigersigma
+  use, intrinsic :: iso_c_binding
+  implicit none
+  private
+
+  integer(c_int), parameter, public :: TCI_OK = 0, TCI_ERR_ARG = 1, TCI_ERR_HIP = 2, &
+                                       TCI_ERR_OOM = 3, TCI_ERR_NUMERIC = 4
+  integer(c_int32_t), parameter, public :: TCI_F64 = 0, TCI_F32 = 1
+  integer(c_int32_t), parameter, public :: TCI_SIGMA_FP64 = 0, TCI_SIGMA_BF16X3 = 1
+  integer(c_int32_t), parameter, public :: TCI_ACCUMULATE = 256, TCI_ADD_EREF = 512
+
+  type, bind(c), public :: tci_problem
+    integer(c_int64_t) :: n_int, n_v, n_chol
+    type(c_ptr) :: eps            ! n_int + n_v orbital energies
+    type(c_ptr) :: chol           ! (n_orb(n_orb+1)/2) records of n_chol values
+    integer(c_int32_t) :: chol_dtype, ladder_dtype
+    integer(c_int64_t) :: n_pairs
+    type(c_ptr) :: pairs          ! n_pairs x 2 int32, (i, j) with i >= j, 0-based
+    type(c_ptr) :: dom_off        ! n_pairs + 1 int64
+    type(c_ptr) :: dom_idx        ! int32 sorted virtual indices, 0-based
+  end type tci_problem
+
+  type, bind(c), public :: tci_plan_info
+    integer(c_int64_t) :: n_csf, n_ladder_items
+    real(c_double) :: e_ref, flops_ladder, flops_sigma, bytes_ladder
+    integer(c_int64_t) :: device_bytes
+  end type tci_plan_info
+
+  type, bind(c), public :: tci_davidson_params
+    integer(c_int32_t) :: roots, maxit
+    real(c_double) :: energy_tol, norm_tol
+    integer(c_int32_t) :: precision
+    real(c_double) :: switch_tol
+    integer(c_int32_t) :: max_vectors, verbose
+  end type tci_davidson_params
+
+  type, bind(c), public :: tci_davidson_result
+    integer(c_int32_t) :: iterations, sigma_fp64, sigma_mixed, converged
+    real(c_double) :: seconds, seconds_sigma
+  end type tci_davidson_result
+
+  public :: tci_plan_create, tci_plan_destroy, tci_plan_get_info, tci_sigma_apply, tci_htimesc, &
+            tci_diagonal, tci_davidson_run, tci_device_alloc, tci_device_free, tci_memcpy_h2d, &
+            tci_memcpy_d2h, tci_device_synchronize, tci_last_error_string
+
+  interface
+    integer(c_int) function tci_plan_create(prob, plan) bind(c, name="tci_plan_create")
+      import :: c_int, c_ptr, tci_problem
+      type(tci_problem), intent(in) :: prob
+      type(c_ptr), intent(out) :: plan
+    end function
+    subroutine tci_plan_destroy(plan) bind(c, name="tci_plan_destroy")
+      import :: c_ptr
+      type(c_ptr), value :: plan
+    end subroutine
+    integer(c_int) function tci_plan_get_info(plan, info) bind(c, name="tci_plan_get_info")
+      import :: c_int, c_ptr, tci_plan_info
+      type(c_ptr), value :: plan
+      type(tci_plan_info), intent(out) :: info
+    end function
+    ! device vectors c, sigma; flags = tci_precision + TCI_ACCUMULATE / TCI_ADD_EREF
+    integer(c_int) function tci_sigma_apply(plan, c, sigma, n, flags, stream) bind(c, name="tci_sigma_apply")
+      import :: c_int, c_ptr, c_int64_t, c_int32_t
+      type(c_ptr), value :: plan, c, sigma, stream
+      integer(c_int64_t), value :: n
+      integer(c_int32_t), value :: flags
+    end function
+    ! host vectors, htimesc semantics: sigmavec += H civec
+    integer(c_int) function tci_htimesc(plan, civec, sigmavec, n) bind(c, name="tci_htimesc")
+      import :: c_int, c_ptr, c_int64_t, c_double
+      type(c_ptr), value :: plan
+      real(c_double), intent(in) :: civec(*)
+      real(c_double), intent(inout) :: sigmavec(*)
+      integer(c_int64_t), value :: n
+    end function
+    integer(c_int) function tci_diagonal(plan, diag, n, stream) bind(c, name="tci_diagonal")
+      import :: c_int, c_ptr, c_int64_t
+      type(c_ptr), value :: plan, diag, stream
+      integer(c_int64_t), value :: n
+    end function
+    integer(c_int) function tci_davidson_run(plan, params, energies, vectors, result) &
+        bind(c, name="tci_davidson_run")
+      import :: c_int, c_ptr, c_double, tci_davidson_params, tci_davidson_result
+      type(c_ptr), value :: plan, vectors
+      type(tci_davidson_params), intent(in) :: params
+      real(c_double), intent(out) :: energies(*)
+      type(tci_davidson_result), intent(out) :: result
+    end function
+    integer(c_int) function tci_device_alloc(bytes, ptr) bind(c, name="tci_device_alloc")
+      import :: c_int, c_ptr, c_int64_t
+      integer(c_int64_t), value :: bytes
+      type(c_ptr), intent(out) :: ptr
+    end function
+    integer(c_int) function tci_device_free(ptr) bind(c, name="tci_device_free")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ptr
+    end function
+    integer(c_int) function tci_memcpy_h2d(dst, src, bytes) bind(c, name="tci_memcpy_h2d")
+      import :: c_int, c_ptr, c_int64_t
+      type(c_ptr), value :: dst, src
+      integer(c_int64_t), value :: bytes
+    end function
+    integer(c_int) function tci_memcpy_d2h(dst, src, bytes) bind(c, name="tci_memcpy_d2h")
+      import :: c_int, c_ptr, c_int64_t
+      type(c_ptr), value :: dst, src
+      integer(c_int64_t), value :: bytes
+    end function
+    integer(c_int) function tci_device_synchronize() bind(c, name="tci_device_synchronize")
+      import :: c_int
+    end function
+    type(c_ptr) function tci_last_error_c() bind(c, name="tci_last_error")
+      import :: c_ptr
+    end function
+  end interface
+
+contains
+
+  ! text of the last error as a Fortran string (trailing blanks trimmed by the caller)
+  function tci_last_error_string() result(msg)
+    character(len=512) :: msg
+    character(kind=c_char), pointer :: p(:)
+    type(c_ptr) :: cp
+    integer :: k
+    msg = ' '
+    cp = tci_last_error_c()
+    if (.not. c_associated(cp)) return
+    call c_f_pointer(cp, p, [512])
+    do k = 1, 512
+      if (p(k) == c_null_char) exit
+      msg(k:k) = p(k)
+    end do
+  end function tci_last_error_string
+
+end module tigersigma
