@@ -126,3 +126,24 @@ def test_dense_pair_path_matches_gather_path():
     assert _rel(a, ref) < TOL and _rel(b, ref) < TOL
     am, bm = _gpu_sigma(dense, x, precision="bf16x3"), _gpu_sigma(gather, x, precision="bf16x3")
     assert _rel(am, ref) < 5e-5 and _rel(bm, ref) < 5e-5
+
+
+@pytest.mark.parametrize("env", [{}, {"TCI_LADDER_MERGED": "0"}, {"TCI_LADDER_MERGED": "0", "TCI_DENSE_PAIRS": "0"}])
+def test_full_domain_variants(env):
+    """Full domains run the ladder as single products over all walks; the
+    per-walk programs (TCI_LADDER_MERGED=0) must give the same sigma, in FP64
+    and bf16x3, including a domain size that is not a multiple of 4 or 8."""
+    _say("case: full-domain variants", env)
+    for args in [dict(n_int=4, n_v=37, n_chol=30, seed=31), dict(n_int=3, n_v=64, n_chol=20, seed=32)]:
+        prob = make_problem(**args)
+        ints = Integrals(prob)
+        x = random_ci_vector(prob, 6)
+        ref = oracle_sigma(prob, x, ints)
+        os.environ.update(env)
+        try:
+            plan = tci.Plan.from_problem(prob)
+        finally:
+            for k in env:
+                del os.environ[k]
+        assert _rel(_gpu_sigma(plan, x), ref) < TOL
+        assert _rel(_gpu_sigma(plan, x, precision="bf16x3"), ref) < 5e-5

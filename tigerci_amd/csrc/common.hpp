@@ -124,6 +124,15 @@ struct Task {
   int32_t accumulate;
   int32_t out_split;  // split kernel only: C is a split-row 16-bit matrix, ldc in 16-bit elements
   int32_t orow_div;  // dense kernels: if > 0, logical row r is stored at row (r % div) * (M / div) + r / div
+  // dense kernels, blocked output: element (row, col) is stored at
+  //   (row / rdiv) rstride + (row % rdiv) ldc + (col / cdiv) cstride + (col % cdiv)
+  // (column offset through xidx for split-row output); rdiv / cdiv = 0: no blocking
+  int32_t rdiv, cdiv;
+  int64_t rstride, cstride;
+};
+struct Blocking {
+  int32_t rdiv = 0, cdiv = 0;
+  int64_t rstride = 0, cstride = 0;
 };
 
 // A list of tasks and their segments, kept on the host while being built and
@@ -141,7 +150,7 @@ struct GemmProgram {
   void clear() { tasks.clear(); segs.clear(); }
   // add one output matrix: tiles over (M,N), all sharing `segs_for_output`
   void add(double* C, int64_t ldc, int M, int N, const std::vector<Seg>& s, bool accumulate,
-           bool out_split = false, int orow_div = 0);
+           bool out_split = false, int orow_div = 0, Blocking blk = Blocking());
   void upload_tables(hipStream_t st);
   void launch(hipStream_t st, int64_t J0 = 0) const;
   void launch_split(int precision, hipStream_t st, int64_t J0 = 0) const;

@@ -11,7 +11,8 @@
 // workgroups per CU; TS = 64: 4 waves of 32 x 32, 17 KiB, for small outputs
 // such as 300 x 300 pair blocks where 128-wide tiles would leave a third of
 // the waves idle), BK = 16, two LDS stages, global loads of stage s+1 in
-// flight during the MFMAs of stage s.  Each wave's loop is specialised on its
+// flight during the MFMAs of stage s.  Outputs may be blocked (Task rdiv /
+// cdiv): one launch then scatters row and column blocks to separate places.  Each wave's loop is specialised on its
 // count of 16 x 16 blocks inside the output (no predicated MFMAs, no
 // accumulator copies).  An optional output-row permutation lets one launch
 // write its rows in a different order (row r to (r % div)(M/div) + r/div).
@@ -137,21 +138,32 @@ __device__ __forceinline__ void dgemm_body(const Task& t, const Seg* __restrict_
     buf ^= 1;
     more = dadvance(segs, t.seg1, nxt.seg, nxt.kb);
   }
+  // column offsets of this lane (one per j), then one row offset per (i, r)
+  int64_t coff[NBN > 0 ? NBN : 1];
+#pragma unroll
+  for (int j = 0; j < NBN; ++j) {
+    const int col = t.n0 + wn + j * 16 + li;
+    const int cb = t.cdiv ? col / t.cdiv : 0;
+    coff[j] = (int64_t)cb * t.cstride + (col - cb * t.cdiv);
+  }
 #pragma unroll
   for (int i = 0; i < NBM; ++i)
 #pragma unroll
-    for (int j = 0; j < NBN; ++j)
+    for (int r = 0; r < 4; ++r) {
+      const int row = t.m0 + wm + i * 16 + lg + 4 * r;
+      if (row >= t.M) continue;
+      const int prow = t.orow_div > 0 ? (row % t.orow_div) * (t.M / t.orow_div) + row / t.orow_div : row;
+      const int64_t ro = t.rdiv ? (int64_t)(row / t.rdiv) * t.rstride + (int64_t)(row % t.rdiv) * t.ldc
+                                : (int64_t)prow * t.ldc;
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int row = t.m0 + wm + i * 16 + lg + 4 * r;
+      for (int j = 0; j < NBN; ++j) {
         const int col = t.n0 + wn + j * 16 + li;
-        if (row < t.M && col < t.N) {
-          const int prow = t.orow_div > 0 ? (row % t.orow_div) * (t.M / t.orow_div) + row / t.orow_div : row;
-          __attribute__((address_space(1))) double* p =
-              (__attribute__((address_space(1))) double*)(t.C + (int64_t)prow * t.ldc + col);
+        if (col < t.N) {
+          __attribute__((address_space(1))) double* p = (__attribute__((address_space(1))) double*)(t.C + ro + coff[j]);
           *p = t.accumulate ? *p + acc[i][j][r] : acc[i][j][r];
         }
       }
+    }
 }
 
 template <int TS>
@@ -174,7 +186,10 @@ __global__ __launch_bounds__(256, TS == 128 ? 2 : 4) void dgemm_kernel(const Tas
         const int row = t.m0 + e / TS, col = t.n0 + e % TS;
         if (row < t.M && col < t.N) {
           const int prow = t.orow_div > 0 ? (row % t.orow_div) * (t.M / t.orow_div) + row / t.orow_div : row;
-          t.C[(int64_t)prow * t.ldc + col] = 0.0;
+          const int64_t ro = t.rdiv ? (int64_t)(row / t.rdiv) * t.rstride + (int64_t)(row % t.rdiv) * t.ldc
+                                    : (int64_t)prow * t.ldc;
+          const int cb = t.cdiv ? col / t.cdiv : 0;
+          t.C[ro + (int64_t)cb * t.cstride + (col - cb * t.cdiv)] = 0.0;
         }
       }
     return;

@@ -147,17 +147,29 @@ __device__ __forceinline__ void xgemm_body(const Task& t, const Seg* __restrict_
     more = xadvance(segs, t.seg1, seg, kb);
   }
   // lane (li, lg) of block (i, j) holds output row 16i + li, columns 16j + 4lg .. +3
+  int64_t roff[NBM > 0 ? NBM : 1];
 #pragma unroll
-  for (int i = 0; i < NBM; ++i)
+  for (int i = 0; i < NBM; ++i) {
+    const int row = t.m0 + wm + i * 16 + li;
+    roff[i] = t.rdiv ? (int64_t)(row / t.rdiv) * t.rstride + (int64_t)(row % t.rdiv) * t.ldc : (int64_t)row * t.ldc;
+  }
 #pragma unroll
-    for (int j = 0; j < NBN; ++j) {
+  for (int j = 0; j < NBN; ++j) {
+    const int col = t.n0 + wn + j * 16 + 4 * lg;
+    if (col >= t.N) continue;
+    const int cb = t.cdiv ? col / t.cdiv : 0, cc = col - cb * t.cdiv;
+    const int64_t coff = (int64_t)cb * t.cstride;
+    // one vector store when the 4 columns stay in one column block (and, for a
+    // split row, in one 8-wide chunk); split rows without blocking may spill
+    // into their zero padding, which receives the exact zeros of masked B rows
+    const bool whole = t.cdiv ? ((cc & 3) == 0 && cc + 3 < t.cdiv) : (t.out_split || col + 3 < t.N);
+#pragma unroll
+    for (int i = 0; i < NBM; ++i) {
       const int row = t.m0 + wm + i * 16 + li;
-      const int col = t.n0 + wn + j * 16 + 4 * lg;
-      if (row < t.M && col < t.N) {
-        const f4 v = acc[i][j];
+      if (row >= t.M) continue;
+      const f4 v = acc[i][j];
+      if (whole) {
         if (t.out_split) {
-          // split-row output: padding columns past N (< the padded row length)
-          // receive the exact zeros of the masked B rows
           typedef __bf16 b4 __attribute__((ext_vector_type(4)));
           b4 h, l;
 #pragma unroll
@@ -166,25 +178,37 @@ __device__ __forceinline__ void xgemm_body(const Task& t, const Seg* __restrict_
             l[r] = (__bf16)(v[r] - (float)h[r]);
           }
           __attribute__((address_space(1))) b4* p = (__attribute__((address_space(1))) b4*)(
-              reinterpret_cast<uint16_t*>(t.C) + (int64_t)row * t.ldc + xidx(col));
+              reinterpret_cast<uint16_t*>(t.C) + roff[i] + coff + xidx(cc));
           p[0] = h;
           p[2] = l;
-        } else if (col + 3 < t.N) {
+        } else {
           typedef double d4u __attribute__((ext_vector_type(4), aligned(8)));  // ldc is arbitrary
-          __attribute__((address_space(1))) d4u* p =
-              (__attribute__((address_space(1))) d4u*)(t.C + (int64_t)row * t.ldc + col);
+          __attribute__((address_space(1))) d4u* p = (__attribute__((address_space(1))) d4u*)(t.C + roff[i] + coff + cc);
           const d4u o = {(double)v[0], (double)v[1], (double)v[2], (double)v[3]};
           if (t.accumulate) *p = *p + o;
           else *p = o;
-        } else {
-          __attribute__((address_space(1))) double* p =
-              (__attribute__((address_space(1))) double*)(t.C + (int64_t)row * t.ldc + col);
+        }
+      } else {
 #pragma unroll
-          for (int r = 0; r < 4; ++r)
-            if (col + r < t.N) p[r] = t.accumulate ? p[r] + (double)v[r] : (double)v[r];
+        for (int r = 0; r < 4; ++r) {
+          const int cr = col + r;
+          if (cr >= t.N) break;
+          const int cbr = t.cdiv ? cr / t.cdiv : 0, ccr = cr - cbr * t.cdiv;
+          const int64_t o = roff[i] + (int64_t)cbr * t.cstride;
+          if (t.out_split) {
+            __attribute__((address_space(1))) __bf16* p =
+                (__attribute__((address_space(1))) __bf16*)(reinterpret_cast<uint16_t*>(t.C) + o + xidx(ccr));
+            const __bf16 h = (__bf16)v[r];
+            p[0] = h;
+            p[8] = (__bf16)(v[r] - (float)h);
+          } else {
+            __attribute__((address_space(1))) double* p = (__attribute__((address_space(1))) double*)(t.C + o + ccr);
+            *p = t.accumulate ? *p + (double)v[r] : (double)v[r];
+          }
         }
       }
     }
+  }
 }
 
 template <int TS>
@@ -206,12 +230,15 @@ __global__ __launch_bounds__(256, TS == 128 ? 2 : 4) void xgemm_kernel(const Tas
       for (int e = tid; e < TS * TS; e += 256) {
         const int row = t.m0 + e / TS, col = t.n0 + e % TS;
         if (row < t.M && col < t.N) {
+          const int64_t ro = t.rdiv ? (int64_t)(row / t.rdiv) * t.rstride + (int64_t)(row % t.rdiv) * t.ldc
+                                    : (int64_t)row * t.ldc;
+          const int cb = t.cdiv ? col / t.cdiv : 0, cc = col - cb * t.cdiv;
           if (t.out_split) {
-            uint16_t* p = reinterpret_cast<uint16_t*>(t.C) + (int64_t)row * t.ldc + xidx(col);
+            uint16_t* p = reinterpret_cast<uint16_t*>(t.C) + ro + (int64_t)cb * t.cstride + xidx(cc);
             p[0] = 0;
             p[8] = 0;
           } else {
-            t.C[(int64_t)row * t.ldc + col] = 0.0;
+            t.C[ro + (int64_t)cb * t.cstride + cc] = 0.0;
           }
         }
       }

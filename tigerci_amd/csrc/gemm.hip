@@ -381,8 +381,10 @@ void launch_ggemm(int tile, const Task* tasks, const Seg* segs, int64_t ntasks, 
 }
 
 void GemmProgram::add(double* C, int64_t ldc, int M, int N, const std::vector<Seg>& s, bool accumulate,
-                      bool out_split, int orow_div) {
+                      bool out_split, int orow_div, Blocking blk) {
   if (M <= 0 || N <= 0) return;
+  TCI_REQUIRE((blk.rdiv == 0 && blk.cdiv == 0) || ((dense || split) && orow_div == 0 && blk.rdiv >= 0 && blk.cdiv >= 0),
+              "blocked output needs a dense or split program");
   TCI_REQUIRE(!out_split || (split && !accumulate), "split-row output needs a split program and no accumulation");
   if (split)
     for (const Seg& x : s)
@@ -414,6 +416,10 @@ void GemmProgram::add(double* C, int64_t ldc, int M, int N, const std::vector<Se
       t.accumulate = accumulate ? 1 : 0;
       t.out_split = out_split ? 1 : 0;
       t.orow_div = orow_div;
+      t.rdiv = blk.rdiv;
+      t.cdiv = blk.cdiv;
+      t.rstride = blk.rstride;
+      t.cstride = blk.cstride;
       tasks.push_back(t);
     }
 }

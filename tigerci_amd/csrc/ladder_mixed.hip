@@ -12,6 +12,8 @@
 // L^J[D,D] of local-domain pairs per chunk, C_P per sigma call.
 #include "plan.hpp"
 
+#include <cstdlib>
+
 namespace tci {
 
 namespace {
@@ -33,6 +35,35 @@ __global__ void split_blocks_kernel(PairTab T, const int64_t* __restrict__ cx_of
       v[t] = k < d ? (float)src[(int64_t)r * d + k] : 0.f;
     }
     put_split8(dst + (int64_t)r * 16 * nch + 16 * ch, v);
+  }
+}
+
+// Cx[P] rows c = split column c of C_P (row c of C_P^T), 32 x 32 tiles via LDS
+__global__ void split_blocks_T_kernel(PairTab T, const int64_t* __restrict__ cx_off, const double* __restrict__ C,
+                                      uint16_t* __restrict__ Cx) {
+  __shared__ float tile[32][33];
+  const int P = blockIdx.y;
+  const int d = (int)(T.dom_off[P + 1] - T.dom_off[P]);
+  const int nt = (d + 31) >> 5;
+  if ((int)blockIdx.x >= nt * nt) return;
+  const int k0 = (blockIdx.x / nt) * 32, c0 = (blockIdx.x % nt) * 32;
+  const double* src = C + T.cd_off[P];
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+  for (int yy = ty; yy < 32; yy += 8) {
+    const int k = k0 + yy, c = c0 + tx;
+    tile[yy][tx] = (k < d && c < d) ? (float)src[(int64_t)k * d + c] : 0.f;
+  }
+  __syncthreads();
+  if (threadIdx.x < 128) {
+    const int nch = (d + 7) >> 3;
+    const int cy = threadIdx.x >> 2, ch = threadIdx.x & 3;
+    const int c = c0 + cy, k = k0 + 8 * ch;
+    if (c < d && k < 8 * nch) {
+      float v[8];
+#pragma unroll
+      for (int t = 0; t < 8; ++t) v[t] = tile[8 * ch + t][cy];
+      put_split8(Cx + cx_off[P] + (int64_t)c * 16 * nch + 2 * k, v);
+    }
   }
 }
 
@@ -166,13 +197,18 @@ void mixed_prepare(tci_plan& P, hipStream_t st) {
     M.Lpx.alloc((size_t)Lc * sum_local_x * 2);
     M.bytes += (int64_t)M.Lpx.bytes;
   }
-  const int tile2 = 64;
+  {
+    const char* env = std::getenv("TCI_LADDER_MERGED");
+    M.merged = loc.empty() && n_pairs > 0 && !(env && env[0] == '0');
+  }
+  const int tile2 = 128;
   int64_t tiles2 = 0;
   for (int Q = 0; Q < n_pairs; ++Q) {
     const int64_t d = P.h_dom_off[Q + 1] - P.h_dom_off[Q];
     tiles2 += cdiv(d, tile2) * cdiv(d, tile2);
   }
-  M.rpart_n = (int)std::max<int64_t>(1, std::min<int64_t>(16, cdiv(2048, std::max<int64_t>(tiles2, 1))));
+  if (M.merged) tiles2 = cdiv((int64_t)n_pairs * n_v, tile2) * cdiv(n_v, tile2);
+  M.rpart_n = (int)std::max<int64_t>(1, std::min<int64_t>(16, cdiv(512, std::max<int64_t>(tiles2, 1))));
   if (M.rpart_n > 1) {
     M.Rpart.alloc((size_t)M.rpart_n * P.sum_d2 * 8);
     M.bytes += (int64_t)M.Rpart.bytes;
@@ -211,6 +247,32 @@ void mixed_prepare(tci_plan& P, hipStream_t st) {
     ph.p1.split = ph.p2.split = true;
     ph.p1.tile = 128;
     ph.p2.tile = tile2;
+    if (M.merged) {
+      // every walk has the full domain: one product per phase over all walks,
+      //   phase 1 T''[(jj,a)][(Q,c)] = sum_k L^J[a][k] C_Q[k][c]   (B = rows of C_Q^T),
+      //            stored as T''[jj][(Q,a)][c]
+      //   phase 2 R[(Q,a)][n] += sum_jj sum_c T''[jj][(Q,a)][c] L^J[n][c]
+      const int d = n_v, NP = n_pairs * n_v;
+      const int64_t ldx = 2 * nvp, Sjj = (int64_t)NP * ldx;
+      Blocking bk;
+      bk.rdiv = d;
+      bk.rstride = Sjj;
+      bk.cdiv = d;
+      bk.cstride = (int64_t)d * ldx;
+      ph.p1.add(reinterpret_cast<double*>(Tx), ldx, (int)(len * d), NP,
+                {xseg(xop(Lx, ldx, jstride_full), xop(Cx, ldx), d)}, false, true, 0, bk);
+      for (int sp = 0; sp < ph.nsplit; ++sp) {
+        const int64_t j0 = len * sp / ph.nsplit, j1 = len * (sp + 1) / ph.nsplit;
+        std::vector<Seg> sg;
+        for (int64_t jj = j0; jj < j1; ++jj)
+          sg.push_back(xseg(xop(Tx + jj * Sjj, ldx), xop(Lx + jj * jstride_full, ldx, jstride_full), d));
+        double* Rout = M.rpart_n > 1 ? M.Rpart.as<double>() + (int64_t)sp * P.sum_d2 : P.R.as<double>();
+        ph.p2.add(Rout, d, NP, d, sg, true);
+      }
+      ph.p1.upload_tables(st);
+      ph.p2.upload_tables(st);
+      continue;
+    }
     int64_t toff = 0;
     int qloc = 0;
     for (int Q = 0; Q < n_pairs; ++Q) {
@@ -255,8 +317,14 @@ void mixed_ladder(tci_plan& P, int precision, hipStream_t st) {
   TCI_REQUIRE(precision == TCI_SIGMA_BF16X3, "split-precision sigma implements bf16x3 only");
   mixed_prepare(P, st);
   tci_plan::MixedLadder& M = P.mx;
-  hipLaunchKernelGGL(split_blocks_kernel, dim3(blocks_for((int64_t)P.dmax * ((P.dmax + 7) / 8), 256), P.n_pairs),
-                     dim3(256), 0, st, P.tab, M.cx_off.as<int64_t>(), P.C.as<double>(), M.Cx.as<uint16_t>());
+  if (M.merged) {
+    const int nt = (P.dmax + 31) / 32;
+    hipLaunchKernelGGL(split_blocks_T_kernel, dim3(nt * nt, P.n_pairs), dim3(256), 0, st, P.tab,
+                       M.cx_off.as<int64_t>(), P.C.as<double>(), M.Cx.as<uint16_t>());
+  } else {
+    hipLaunchKernelGGL(split_blocks_kernel, dim3(blocks_for((int64_t)P.dmax * ((P.dmax + 7) / 8), 256), P.n_pairs),
+                       dim3(256), 0, st, P.tab, M.cx_off.as<int64_t>(), P.C.as<double>(), M.Cx.as<uint16_t>());
+  }
   TCI_HIP(hipGetLastError());
   double* Rz = M.rpart_n > 1 ? M.Rpart.as<double>() : P.R.as<double>();
   launch_fill(Rz, (int64_t)M.rpart_n * P.sum_d2, 0.0, st);

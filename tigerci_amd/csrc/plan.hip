@@ -484,7 +484,7 @@ void tci_plan::build(const tci_problem& p) {
   for (DevBuf* b : {&pairs, &dom_off, &dom_idx, &cd_off, &aa_off, &ab_off, &eps, &pos, &xmap, &isect_off, &isect,
                     &inv_off, &inv_q, &inv_pos, &gpart_off, &Lvv, &Lov, &Kov, &Jvv, &Mss, &O4, &O3, &vvd, &vvx,
                     &diag, &C, &Ct, &R, &s, &g, &gss, &gW, &Y, &W, &gpart, &part, &sig0, &lad_part, &Jbig, &CRb,
-                    &CtRb, &CCb, &Zb, &Z3b, &CTb, &holeA})
+                    &CtRb, &CCb, &Zb, &Z3b, &CTb, &holeA, &Tbuf, &Rpart, &Lp, &Cw})
     device_bytes += (int64_t)b->bytes;
 }
 
@@ -517,6 +517,10 @@ void tci_plan::build_two_phase_ladder(hipStream_t st) {
   }
   n_local = (int)loc.size();
   upload(pairs_local, loc, st);
+  {
+    const char* env = std::getenv("TCI_LADDER_MERGED");
+    ladder_merged = n_local == 0 && n_pairs > 0 && !(env && env[0] == '0');
+  }
   const double budget = 8.0e9;  // bytes of T + Lp per chunk
   const double per_J = 8.0 * (double)std::max<int64_t>(sum_d2, 1) + (double)lsz * sum_local;
   int64_t Lc = std::max<int64_t>(1, std::min<int64_t>(n_chol, (int64_t)(budget / per_J)));
@@ -537,6 +541,11 @@ void tci_plan::build_two_phase_ladder(hipStream_t st) {
   int64_t tiles2 = 0;
   for (int P = 0; P < n_pairs; ++P) tiles2 += cdiv(h_dom_off[P + 1] - h_dom_off[P], 64) * cdiv(h_dom_off[P + 1] - h_dom_off[P], 64);
   rpart_n = (int)std::max<int64_t>(1, std::min<int64_t>(16, cdiv(2048, std::max<int64_t>(tiles2, 1))));
+  if (ladder_merged) {
+    const int64_t tm = cdiv((int64_t)n_pairs * n_v, 128) * cdiv(n_v, 128);
+    rpart_n = (int)std::max<int64_t>(1, std::min<int64_t>(16, cdiv(512, tm)));
+    Cw.alloc((size_t)sum_d2 * 8);
+  }
   Tbuf.alloc((size_t)Lc * sum_d2 * 8);
   if (sum_local) Lp.alloc((size_t)Lc * sum_local * lsz);
   if (rpart_n > 1) Rpart.alloc((size_t)rpart_n * sum_d2 * 8);
@@ -557,6 +566,34 @@ void tci_plan::build_two_phase_ladder(hipStream_t st) {
     ph.p1.dense = ph.p2.dense = true;
     ph.p1.tile = 128;
     ph.p2.tile = 64;
+    if (ladder_merged) {
+      // phase 1: T[(jj,a)][(P,c)] = sum_k L^J[a][k] Cw[k][(P,c)], stored as T[jj][P][a][c]
+      // phase 2: R[(P,a)][n] += sum_jj sum_c T[jj][(P,a)][c] L^J[c][n]  (R blocks back to back)
+      const int d = n_v, NP = n_pairs * n_v;
+      const int64_t Sjj = (int64_t)n_pairs * d * d;
+      Blocking bk;
+      bk.rdiv = d;
+      bk.rstride = Sjj;
+      bk.cdiv = d;
+      bk.cstride = (int64_t)d * d;
+      ph.p1.add(Tbuf.as<double>(), d, (int)(len * d), NP,
+                {seg(op(Lvv.p, n_v, 1, nullptr, nullptr, lvv_f32, nv2), op(Cw.p, NP, 1), d, 1.0)}, false, false, 0, bk);
+      ph.p2.tile = 128;
+      for (int sp = 0; sp < ph.nsplit; ++sp) {
+        const int64_t j0 = len * sp / ph.nsplit, j1 = len * (sp + 1) / ph.nsplit;
+        std::vector<Seg> sg;
+        for (int64_t jj = j0; jj < j1; ++jj)
+          sg.push_back(seg(op(Tbuf.as<double>() + jj * Sjj, d, 1),
+                           op(static_cast<const char*>(Lvv.p) + (size_t)(jj * nv2) * lsz, n_v, 1, nullptr, nullptr,
+                              lvv_f32, nv2),
+                           d, 1.0));
+        double* Rout = rpart_n > 1 ? Rpart.as<double>() + (int64_t)sp * sum_d2 : R.as<double>();
+        ph.p2.add(Rout, d, NP, d, sg, true);
+      }
+      ph.p1.upload_tables(st);
+      ph.p2.upload_tables(st);
+      continue;
+    }
     int64_t toff = 0;
     int qloc = 0;
     for (int P = 0; P < n_pairs; ++P) {
@@ -610,6 +647,7 @@ void tci_plan::unpack_and_ladder(const double* x, int precision, hipStream_t st)
   if (!ladder_fused) {
     double* Rz = rpart_n > 1 ? Rpart.as<double>() : R.as<double>();
     launch_fill(Rz, (int64_t)(rpart_n > 1 ? rpart_n : 1) * sum_d2, 0.0, st);
+    if (ladder_merged) launch_concat_cols(C.as<double>(), n_pairs, n_v, Cw.as<double>(), st);
     for (const auto& ch : lchunks) {
       const LadderPhase& ph = lphase[ch.second];
       if (n_local) {

@@ -50,6 +50,10 @@ struct tci_plan {
   std::vector<std::pair<int64_t, int>> lchunks;  // (J0, index into lphase)
   tci::DevBuf Tbuf, Rpart;
   int rpart_n = 1;
+  // every walk has the full domain: both phases run as single products over all
+  // walks (C blocks side by side in Cw)
+  bool ladder_merged = false;
+  tci::DevBuf Cw;
   // split-precision (bf16x3) two-phase ladder, built on first use:
   //   phase 1 T'_P[(jj,a)][c] = sum_k L^J[a][k] C_P[c][k]   (T' = L C^T, split rows)
   //   phase 2 R_P[n][a]     += sum_jj sum_c L^J[n][c] T'_P[jj][a][c]
@@ -60,7 +64,8 @@ struct tci_plan {
     tci::DevBuf Lpx;                // split rows of L^J[D,D] of local-domain pairs (one chunk)
     tci::DevBuf pairs_local;        // indices of the local-domain pairs
     int n_local = 0;
-    tci::DevBuf Cx, cx_off;         // split rows of the C_P blocks
+    tci::DevBuf Cx, cx_off;         // split rows of the C_P blocks (of C_P^T when merged)
+    bool merged = false;            // all walks full-domain: single products over all walks
     tci::DevBuf Tx;                 // T' of one chunk
     tci::DevBuf Rpart;              // partial R buffers of phase 2 K-splits
     int rpart_n = 1;

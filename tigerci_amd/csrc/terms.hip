@@ -416,6 +416,16 @@ __global__ void pack_L_kernel(PairTab T, PackTab K, const LT* __restrict__ Lvv, 
   }
 }
 
+// Cw[k][P d + c] = C_P[k][c]: the C blocks of equal size d side by side
+__global__ void concat_cols_kernel(const double* __restrict__ C, int n_pairs, int d, double* __restrict__ Cw) {
+  const int64_t d2 = (int64_t)d * d, n = (int64_t)n_pairs * d2, ld = (int64_t)n_pairs * d;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t P = e / d2, rem = e - P * d2;
+    const int64_t k = rem / d, c = rem - k * d;
+    Cw[k * ld + P * d + c] = C[e];
+  }
+}
+
 __global__ void fill_kernel(double* p, int64_t n, double v) {
   for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x)
     p[e] = v;
@@ -520,6 +530,13 @@ void launch_pack_L(const PairTab& T, const PackTab& K, const void* Lvv, bool f32
   dim3 grid(K.n_local, (unsigned)len), block(64, 4);
   if (f32) hipLaunchKernelGGL(pack_L_kernel<float>, grid, block, 0, st, T, K, (const float*)Lvv, J0, len, (float*)Lp);
   else hipLaunchKernelGGL(pack_L_kernel<double>, grid, block, 0, st, T, K, (const double*)Lvv, J0, len, (double*)Lp);
+  TCI_HIP(hipGetLastError());
+}
+
+void launch_concat_cols(const double* C, int n_pairs, int d, double* Cw, hipStream_t st) {
+  const int64_t n = (int64_t)n_pairs * d * d;
+  if (n <= 0) return;
+  hipLaunchKernelGGL(concat_cols_kernel, dim3(gx(n)), dim3(256), 0, st, C, n_pairs, d, Cw);
   TCI_HIP(hipGetLastError());
 }
 
