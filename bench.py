@@ -79,9 +79,10 @@ def main() -> int:
     ap.add_argument("--workload", default=None,
                     help="default: davidson_16x300x1200_full (davidson), sigma_16x300x1200_full (sigma)")
     ap.add_argument("--roots", type=int, default=3)
+    ap.add_argument("--l-scale", type=float, default=None, help="override the workload's Cholesky-vector scale")
     ap.add_argument("--precision", default="bf16x3")
     ap.add_argument("--mode", default="davidson", choices=["davidson", "sigma"])
-    ap.add_argument("--switch-tol", type=float, default=1e-5)
+    ap.add_argument("--switch-tol", type=float, default=1e-6)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--seed", type=int, default=1234)
     ap.add_argument("--copy-probe-mib", type=int, default=0,
@@ -98,7 +99,7 @@ def main() -> int:
     if a.workload is None:
         a.workload = "davidson_16x300x1200_full" if a.mode == "davidson" else "sigma_16x300x1200_full"
     t = time.perf_counter()
-    syn = workload(a.workload, a.seed)
+    syn = workload(a.workload, a.seed, l_scale=a.l_scale)
     log(f"[bench] generated {a.workload} in {time.perf_counter() - t:.1f} s")
     ladder_dtype = "f32" if syn.L.dtype == np.float32 else "f64"
     t = time.perf_counter()
@@ -112,7 +113,8 @@ def main() -> int:
            "ladder_flops_per_sigma": info.flops_ladder, "sigma_flops": info.flops_sigma}
     out = {"metric": METRIC, "n_gpus": 1, "steps": a.steps, "warmup": a.warmup, "scaling": "weak",
            "vs_baseline": None,
-           "data": f"synthetic (seeded N(0,1) exp(-J/80) x {WORKLOADS[a.workload][6]} Cholesky vectors)"}
+           "data": f"synthetic (seeded N(0,1) exp(-J/80) x "
+                   f"{a.l_scale if a.l_scale is not None else WORKLOADS[a.workload][6]} Cholesky vectors)"}
 
     if a.mode == "sigma":
         x = tci.DeviceArray.from_numpy(random_vector(plan.n_csf, a.seed))

@@ -117,7 +117,7 @@ typedef struct tci_davidson_params {
   int32_t precision;      /* tci_precision of the expansion sigmas */
   double switch_tol;      /* mixed: switch to FP64 sigma once every root has |dE| < switch_tol
                              or a residual below norm_tol (forced at maxit/2) */
-  int32_t max_vectors;    /* subspace collapse budget (0 -> maxit*roots) */
+  int32_t max_vectors;    /* subspace collapse budget (0 -> maxit*roots + roots, capped at 32 GB of vectors) */
   int32_t verbose;        /* 1: print the reference-style iteration table */
 } tci_davidson_params;
 

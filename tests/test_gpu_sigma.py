@@ -147,3 +147,23 @@ def test_full_domain_variants(env):
                 del os.environ[k]
         assert _rel(_gpu_sigma(plan, x), ref) < TOL
         assert _rel(_gpu_sigma(plan, x, precision="bf16x3"), ref) < 5e-5
+
+
+@pytest.mark.parametrize("dense", ["0", "1"])
+def test_local_domain_pair_paths(dense):
+    """Local domains with weak pairs: the ring, 3-external and 4-internal terms
+    as products over embedded blocks (TCI_DENSE_PAIRS=1) and as per-walk
+    gathered programs (=0) both reproduce the oracle."""
+    _say("case: local-domain pair paths, TCI_DENSE_PAIRS =", dense)
+    prob = make_problem(6, 45, 40, 41, domain_range=(9, 45), weak_below=14)
+    ints = Integrals(prob)
+    os.environ["TCI_DENSE_PAIRS"] = dense
+    try:
+        plan = tci.Plan.from_problem(prob)
+    finally:
+        del os.environ["TCI_DENSE_PAIRS"]
+    for k in range(2):
+        x = random_ci_vector(prob, 60 + k)
+        ref = oracle_sigma(prob, x, ints)
+        assert _rel(_gpu_sigma(plan, x), ref) < TOL
+        assert _rel(_gpu_sigma(plan, x, precision="bf16x3"), ref) < 5e-5

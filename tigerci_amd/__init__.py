@@ -272,10 +272,12 @@ class Plan:
         return dict(ladder_ms=lm.value, ladder_launches=ln.value, sigma_ms=sm.value, sigma_calls=sn.value)
 
     def davidson(self, roots: int = 1, energy_tol: float = 1e-6, norm_tol: float = 1e-6,
-                 precision: str = "fp64", switch_tol: float = 1e-4, maxit: int = 0,
+                 precision: str = "fp64", switch_tol: float | None = None, maxit: int = 0,
                  max_vectors: int = 0, vectors=None, verbose: bool = False):
         """Lowest roots; returns (total energies, result dict).  ``vectors`` is an
         optional DeviceArray of roots*n_csf doubles for the eigenvectors."""
+        if switch_tol is None:
+            switch_tol = energy_tol
         p = _DavParams(roots, maxit, energy_tol, norm_tol, PRECISION[precision], switch_tol, max_vectors,
                        1 if verbose else 0)
         e = np.zeros(roots)

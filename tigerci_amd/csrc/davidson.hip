@@ -228,7 +228,14 @@ extern "C" int tci_davidson_run(tci_plan* plan, const tci_davidson_params* prm, 
     const int roots = prm->roots;
     TCI_REQUIRE(roots >= 1 && roots <= 64 && roots <= plan->n_csf, "roots out of range");
     const int maxit = prm->maxit > 0 ? prm->maxit : 50 * roots;
-    int mv = prm->max_vectors > 0 ? prm->max_vectors : maxit * roots + roots;
+    // default subspace budget: every iteration's vectors, capped so that the
+    // basis and its sigma vectors take at most 32 GB (the reference's disk
+    // budget rule, davidson_restart.f90:26-60, with device memory as the store)
+    int mv = prm->max_vectors;
+    if (mv <= 0) {
+      const int64_t cap = (int64_t)(32.0e9 / (16.0 * (double)plan->n_csf));
+      mv = (int)std::max<int64_t>(5 * roots, std::min<int64_t>((int64_t)maxit * roots + roots, cap));
+    }
     mv = std::min(mv, 1000);
     TCI_REQUIRE(mv >= 5 * roots, "max_vectors must be at least 5*roots");
     const int prec0 = prm->precision;
@@ -279,12 +286,11 @@ extern "C" int tci_davidson_run(tci_plan* plan, const tci_davidson_params* prm, 
     std::vector<double> h((size_t)mv * mv, 0.0);  // row-major, leading dim mv
     std::vector<double> w, Y, Yprev;
     std::vector<char> restarts;
-    bool skip_conv_once = false;
     int it = 0;
     bool all_conv = false;
     for (it = 1; it <= maxit; ++it) {
       for (int r = 0; r < roots; ++r) {
-        if (conv[r]) continue;
+        if (conv[r] || ready[r]) continue;  // no new vector for a settled root
         const int k = root2idx[r];
         sigma(k);
         std::vector<double*> X;
@@ -304,14 +310,15 @@ extern "C" int tci_davidson_run(tci_plan* plan, const tci_davidson_params* prm, 
       bool restarted = false;
       // mixed -> FP64 switch
       bool do_switch = false;
-      // in the mixed phase no root is declared converged: once every root has
-      // settled (|dE| < switch_tol now, or a residual below norm_tol in the
-      // previous iteration) the subspace collapses to the Ritz vectors and the
+      // in the mixed phase no root is declared converged: a root that meets
+      // the convergence rules with the split sigma (|dE| < switch_tol or a
+      // residual below norm_tol) is settled and gets no new vectors; once every
+      // root has settled the subspace collapses to the Ritz vectors and the
       // solve continues with FP64 sigma; half of maxit forces the switch
       if (precision != TCI_SIGMA_FP64 && it > 1) {
         do_switch = true;
         for (int r = 0; r < roots; ++r)
-          if (!conv[r] && !(dE[r] < prm->switch_tol) && !ready[r]) do_switch = false;
+          if (!conv[r] && !ready[r]) do_switch = false;
         if (it >= std::max(2, maxit / 2)) do_switch = true;
       }
       const bool budget = (m > 4 * roots && m + 3 * roots > mv);
@@ -375,8 +382,8 @@ extern "C" int tci_davidson_run(tci_plan* plan, const tci_davidson_params* prm, 
         m = newm;
         if (do_switch) {
           precision = TCI_SIGMA_FP64;
+          std::fill(ready.begin(), ready.end(), 0);
           for (int q = 0; q < m; ++q) sigma(q);
-          skip_conv_once = true;
         }
         for (int a = 0; a < m; ++a) {
           std::vector<double*> X;
@@ -394,7 +401,13 @@ extern "C" int tci_davidson_run(tci_plan* plan, const tci_davidson_params* prm, 
         for (int a = 0; a < m; ++a)
           for (int b = 0; b < m; ++b) hm[(size_t)a * m + b] = h[(size_t)a * mv + b];
         jacobi_eigh(m, hm, w, Y);
-        for (int r = 0; r < roots; ++r) energy[r] = w[r];
+        // after the switch |dE| compares the FP64 Ritz values of the collapsed
+        // space with the mixed ones: the Rayleigh-quotient error of the split
+        // sigma (1e-10 Eh scale), so the energy criterion stays meaningful
+        for (int r = 0; r < roots; ++r) {
+          if (do_switch) dE[r] = std::fabs(w[r] - energy[r]);
+          energy[r] = w[r];
+        }
         restarted = !do_switch;
       }
       restarts.push_back(restarted ? 1 : 0);
@@ -407,7 +420,7 @@ extern "C" int tci_davidson_run(tci_plan* plan, const tci_davidson_params* prm, 
         Xs.push_back(Sk(q));
       }
       for (int r = 0; r < roots; ++r) {
-        if (conv[r]) continue;
+        if (conv[r] || ready[r]) continue;
         // residual r = sum_k y_k (sigma_k - E c_k)
         std::vector<double*> X;
         std::vector<double> c;
@@ -425,12 +438,13 @@ extern "C" int tci_davidson_run(tci_plan* plan, const tci_davidson_params* prm, 
         bool halt = false;
         const bool in_restart = restarts.back() || (restarts.size() >= 2 && restarts[restarts.size() - 2]);
         if (precision != TCI_SIGMA_FP64) {
-          ready[r] = rn < prm->norm_tol ? 1 : 0;
+          if (!in_restart && (rn < prm->norm_tol || (dE[r] < prm->switch_tol && it > 1))) {
+            ready[r] = 1;
+            continue;
+          }
         } else if (!in_restart) {
-          // right after the switch dE compares an FP64 with a mixed energy:
-          // only the residual criterion applies
           if (rn < prm->norm_tol) halt = true;
-          else if (!skip_conv_once && dE[r] < prm->energy_tol && it > 1) halt = true;
+          else if (dE[r] < prm->energy_tol && it > 1) halt = true;
         }
         if (halt) met[r] = 1;
         if (it == maxit) halt = true;
@@ -449,7 +463,6 @@ extern "C" int tci_davidson_run(tci_plan* plan, const tci_davidson_params* prm, 
         root2idx[r] = m;
         ++m;
       }
-      skip_conv_once = false;
       all_conv = std::all_of(conv.begin(), conv.end(), [](char c) { return c != 0; });
       if (all_conv) break;
     }

@@ -180,9 +180,31 @@ void mixed_prepare(tci_plan& P, hipStream_t st) {
     TCI_HIP(hipGetLastError());
     M.bytes += (int64_t)M.Lx.bytes;
   }
-  // chunk length: T' (all pairs) + packed L (local pairs), 2 bytes per element
+  // local-domain L^J[D,D] for all J packed once when it fits 64 GB; otherwise
+  // packed per chunk of Cholesky vectors inside every sigma
+  M.prepacked = sum_local_x > 0 && 2.0 * (double)sum_local_x * (double)n_chol <= 64.0e9;
+  std::vector<int64_t> lpo_all(loc.size() + 1, 0);
+  if (M.prepacked) {
+    for (size_t q = 0; q < loc.size(); ++q) {
+      const int64_t d = P.h_dom_off[loc[q] + 1] - P.h_dom_off[loc[q]];
+      lpo_all[q + 1] = lpo_all[q] + n_chol * d * 2 * xpad(d);
+    }
+    upload(M.lp_off_all, lpo_all, st);
+    M.Lpx.alloc((size_t)lpo_all.back() * 2);
+    M.bytes += (int64_t)M.Lpx.bytes;
+    if (P.lvv_f32)
+      hipLaunchKernelGGL(pack_L_split_kernel<float>, dim3(4096, (unsigned)loc.size()), dim3(256), 0, st, P.tab,
+                         M.pairs_local.as<int32_t>(), M.lp_off_all.as<int64_t>(), static_cast<const float*>(P.Lvv.p),
+                         (int64_t)0, (int)n_chol, M.Lpx.as<uint16_t>());
+    else
+      hipLaunchKernelGGL(pack_L_split_kernel<double>, dim3(4096, (unsigned)loc.size()), dim3(256), 0, st, P.tab,
+                         M.pairs_local.as<int32_t>(), M.lp_off_all.as<int64_t>(), static_cast<const double*>(P.Lvv.p),
+                         (int64_t)0, (int)n_chol, M.Lpx.as<uint16_t>());
+    TCI_HIP(hipGetLastError());
+  }
+  // chunk length: T' (all pairs) + packed L (local pairs, unless prepacked), 2 bytes per element
   const double budget = 8.0e9;
-  const double per_J = 2.0 * (double)(sum_x + sum_local_x);
+  const double per_J = 2.0 * (double)(sum_x + (M.prepacked ? 0 : sum_local_x));
   int64_t Lc = std::max<int64_t>(1, std::min<int64_t>(n_chol, (int64_t)(budget / std::max(per_J, 1.0))));
   for (int64_t q = Lc; q >= std::max<int64_t>(1, Lc / 2); --q)
     if (n_chol % q == 0) {
@@ -193,7 +215,7 @@ void mixed_prepare(tci_plan& P, hipStream_t st) {
   // padding columns of T' are never written by phase 1 and must read as zero
   launch_fill(M.Tx.as<double>(), (int64_t)(M.Tx.bytes / 8), 0.0, st);
   M.bytes += (int64_t)M.Tx.bytes;
-  if (sum_local_x) {
+  if (sum_local_x && !M.prepacked) {
     M.Lpx.alloc((size_t)Lc * sum_local_x * 2);
     M.bytes += (int64_t)M.Lpx.bytes;
   }
@@ -286,6 +308,10 @@ void mixed_prepare(tci_plan& P, hipStream_t st) {
         Lb = Lx;
         jstep = jstride_full;
         jstride = jstride_full;
+      } else if (M.prepacked) {  // [q][J][r][k]: chunk start through jstride
+        Lb = M.Lpx.as<uint16_t>() + lpo_all[qloc++];
+        jstep = (int64_t)d * ldx;
+        jstride = jstep;
       } else {
         Lb = M.Lpx.as<uint16_t>() + lpo[qloc++];
         jstep = (int64_t)d * ldx;
@@ -330,7 +356,7 @@ void mixed_ladder(tci_plan& P, int precision, hipStream_t st) {
   launch_fill(Rz, (int64_t)M.rpart_n * P.sum_d2, 0.0, st);
   for (const auto& ch : M.chunks) {
     const tci_plan::LadderPhase& ph = M.phase[ch.second];
-    if (M.n_local) {
+    if (M.n_local && !M.prepacked) {
       const unsigned gxb = 1024;
       if (P.lvv_f32)
         hipLaunchKernelGGL(pack_L_split_kernel<float>, dim3(gxb, M.n_local), dim3(256), 0, st, P.tab,

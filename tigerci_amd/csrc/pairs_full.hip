@@ -1,84 +1,109 @@
-// Pair-pair terms as dense products when every internal pair i >= j is a
-// strong walk whose domain is the whole virtual space (the untruncated
-// synthetic problems).  N = n_int n_v; an ordered internal pair (p, q) names an
-// n_v x n_v block, X^{pq} = X_Q for p >= q and X_Q^T for p < q, with
-// Q = p(p+1)/2 + q the walk of the unordered pair.  Per sigma:
+// Pair-pair terms as dense products over ordered internal pairs, with every
+// pair block embedded in the full virtual space.
+//
+// N = n_int n_v.  An ordered internal pair (p, q) names an n_v x n_v block;
+// X^{pq} is the walk Q = {p, q}'s dense block X_Q (p >= q) or X_Q^T (p < q),
+// scattered to the virtuals of its domain D_Q (zero elsewhere, and zero for
+// pairs without doubles).  Per sigma:
 //   CR [(p,r)][(q,c)] = C^{pq}[r][c],   CtR likewise from Ct = 2C - C^T,
-//   CC [(p,r)][(q,c)] = C^{qp}[r][c]    (CR and CC are symmetric matrices)
+//   CC [(p,r)][(q,c)] = C^{qp}[r][c]    (CR, CtR and CC are symmetric),
+//   Yt [(i,a)][J] = sum_c L^J_ac s_i^c,
 // and once per plan
-//   Kov[(k,c)][(j,b)] = (kc|jb),  Jbig[(k,c)][(j,b)] = (kj|cb)  (both symmetric).
-// Then Z = CtR Kov - CR Jbig and Z3 = Jbig CC hold, for every ordered (i, j),
-//   Z (i,j) = sum_k [Ct^{ik} K^{kj} - C^{ik} J^{kj}],  Z3(j,i) = sum_k J^{kj} C^{ik},
-// so the ring terms Z^{ij} + Z^{ji T} of R^{ij} are
-//   R^{ij}[a][b] += Z(i,j)[a][b] - Z3(j,i)[a][b] + Z(j,i)[b][a] - Z3(i,j)[b][a].
-// The 4-internal term sum_{Q=(k,l)} [(ki|lj) C_Q + (li|kj) C_Q^T] is one
-// product of an n_pairs x 2 n_pairs coefficient matrix with [C; C^T] seen as
-// 2 n_pairs x n_v^2.  All products run on the dense FP64 kernel, or for the
-// split precisions Z and Z3 on the split-row kernel (k padded per block to a
-// multiple of 8).
+//   Kov[(k,c)][(j,b)] = (kc|jb),  Jbig[(k,c)][(j,b)] = (kj|cb)  (both symmetric),
+//   Lov[J][(j,b)] = L^J_jb.
+// Then
+//   Z  = CtR Kov - CR Jbig + Yt Lov,   Z3 = Jbig CC,   W[i][J][b] = (Lov CtR)[J][(i,b)]
+// hold for every ordered (i, j): Z(i,j) = sum_k [Ct^{ik} K^{kj} - C^{ik} J^{kj}] + X^{ij},
+// Z3(j,i) = sum_k J^{kj} C^{ik}, so the ring and 3-external terms of the walk
+// P = (i, j) are, at positions x, y of D_P (a = D_P[x], b = D_P[y]),
+//   R_P[x][y] += Z(i,j)[a][b] - Z3(j,i)[a][b] + Z(j,i)[b][a] - Z3(i,j)[b][a].
+// A restriction of an embedded product equals the product over the domain
+// intersections, which is what the per-walk gather programs compute.  The
+// 4-internal term sum_{Q=(k,l)} [(ki|lj) C_Q + (li|kj) C_Q^T] is one product of
+// an n_pairs x 2 n_pairs coefficient matrix with the embedded [C; C^T]
+// (2 n_pairs x n_v^2 rows), gathered back the same way.  All products run on
+// the dense FP64 kernel, or for the split precisions Z, Z3 and W on the
+// split-row kernel (k padded per n_v block to a multiple of 8).  The plan
+// takes this path when the products cost at most a few times the gathered
+// per-walk work (full or wide domains).
 #include "plan.hpp"
+
+#include <cstdlib>
 
 namespace tci {
 
 namespace {
 
-__device__ __forceinline__ int64_t walk_of(int p, int q) {
+struct EmbedTab {
+  const int32_t* pairid;  // [p n_int + q] -> strong walk of the unordered pair {p, q}, or -1
+  const int32_t* pos;     // [Q n_v + a]   -> position of virtual a in D_Q, or -1
+};
+
+__device__ __forceinline__ int64_t rec_of(int p, int q) {
   const int hi = max(p, q), lo = min(p, q);
   return (int64_t)hi * (hi + 1) / 2 + lo;
 }
 
-// Jbig[(p,r)][(q,s)] = (pq|rs) from Jvv[walk(p,q)][r][s]
+// Jbig[(p,r)][(q,s)] = (pq|rs) from Jvv[rec(p,q)][r][s]
 __global__ void jbig_kernel(const double* __restrict__ Jvv, int n_int, int n_v, double* __restrict__ Jbig) {
   const int64_t N = (int64_t)n_int * n_v;
   for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < N * N; e += (int64_t)gridDim.x * blockDim.x) {
     const int64_t row = e / N, col = e - row * N;
     const int p = (int)(row / n_v), r = (int)(row - (int64_t)p * n_v);
     const int q = (int)(col / n_v), s = (int)(col - (int64_t)q * n_v);
-    Jbig[e] = Jvv[(walk_of(p, q) * n_v + r) * n_v + s];
+    Jbig[e] = Jvv[(rec_of(p, q) * n_v + r) * n_v + s];
   }
 }
 
-// Output block (p, q) = sign * X^{pq} (cc = false) or sign * X^{qp} = sign * (X^{pq})^T
-// (cc = true), from the dense d x d blocks of src (walk order, d^2 apart).
-// 32 x 32 tiles through LDS so both orientations read and write coalesced.
-// FP64: out[(p d + r) ldo + q d + c];  split rows: out16[(p d + r) ldo + xidx(q dp + c)],
-// dp = d rounded up to 8, padding columns zero.
+// Output block (p, q) = sign X^{pq} (cc = false) or sign X^{qp} = sign (X^{pq})^T
+// (cc = true), embedded.  32 x 32 tiles through LDS so both orientations read
+// along rows of the source block.  FP64: out[(p n_v + r) ldo + q n_v + c];
+// split rows: out16[(p n_v + r) ldo + xidx(q vp + c)], vp = n_v rounded up to 8.
 template <bool SPLIT>
-__global__ void relayout_kernel(const double* __restrict__ src, int d, bool cc, double sign, void* __restrict__ out,
-                                int64_t ldo) {
+__global__ void relayout_kernel(PairTab T, EmbedTab E, const double* __restrict__ src, bool cc, double sign,
+                                void* __restrict__ out, int64_t ldo) {
   __shared__ double tile[32][33];
-  const int nt = (d + 31) >> 5;
+  const int nv = T.n_v, nt = (nv + 31) >> 5;
   const int q = blockIdx.x / nt, ct = blockIdx.x - q * nt;
   const int p = blockIdx.y / nt, rt = blockIdx.y - p * nt;
-  const double* blk = src + walk_of(p, q) * d * d;
+  const int Q = E.pairid[p * T.n_int + q];
   const bool direct = (p >= q) != cc;
   const int r0 = rt * 32, c0 = ct * 32;
   const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
-  for (int yy = ty; yy < 32; yy += 8) {
-    if (direct) {
-      const int r = r0 + yy, c = c0 + tx;
-      tile[yy][tx] = (r < d && c < d) ? blk[(int64_t)r * d + c] : 0.0;
-    } else {  // element (r, c) = blk[c][r]: read row c, column r
-      const int c = c0 + yy, r = r0 + tx;
-      tile[tx][yy] = (r < d && c < d) ? blk[(int64_t)c * d + r] : 0.0;
+  if (Q >= 0) {
+    const double* blk = src + T.cd_off[Q];
+    const int d = (int)(T.dom_off[Q + 1] - T.dom_off[Q]);
+    const int32_t* ps = E.pos + (int64_t)Q * nv;
+    for (int yy = ty; yy < 32; yy += 8) {
+      if (direct) {  // element (r, c) = blk[pos r][pos c]
+        const int r = r0 + yy, c = c0 + tx;
+        const int pr = r < nv ? ps[r] : -1, pc = c < nv ? ps[c] : -1;
+        tile[yy][tx] = (pr >= 0 && pc >= 0) ? blk[(int64_t)pr * d + pc] : 0.0;
+      } else {  // element (r, c) = blk[pos c][pos r]: walk along row pos c
+        const int c = c0 + yy, r = r0 + tx;
+        const int pr = r < nv ? ps[r] : -1, pc = c < nv ? ps[c] : -1;
+        tile[tx][yy] = (pr >= 0 && pc >= 0) ? blk[(int64_t)pc * d + pr] : 0.0;
+      }
     }
+  } else {
+    for (int yy = ty; yy < 32; yy += 8) tile[yy][tx] = 0.0;
   }
   __syncthreads();
   if (!SPLIT) {
     double* o = static_cast<double*>(out);
     for (int yy = ty; yy < 32; yy += 8) {
       const int r = r0 + yy, c = c0 + tx;
-      if (r < d && c < d) o[((int64_t)p * d + r) * ldo + (int64_t)q * d + c] = sign * tile[yy][tx];
+      if (r < nv && c < nv) o[((int64_t)p * nv + r) * ldo + (int64_t)q * nv + c] = sign * tile[yy][tx];
     }
   } else if (threadIdx.x < 128) {
-    const int dp = (d + 7) & ~7;
+    const int vp = (nv + 7) & ~7;
     const int yy = threadIdx.x >> 2, c = c0 + 8 * (threadIdx.x & 3);
     const int r = r0 + yy;
-    if (r < d && c < dp) {
+    if (r < nv && c < vp) {
       float v[8];
 #pragma unroll
-      for (int t = 0; t < 8; ++t) v[t] = (c + t < d) ? (float)(sign * tile[yy][c - c0 + t]) : 0.f;
-      put_split8(static_cast<uint16_t*>(out) + ((int64_t)p * d + r) * ldo + xidx((int64_t)q * dp + c), v);
+      for (int t = 0; t < 8; ++t) v[t] = (c + t < nv) ? (float)(sign * tile[yy][c - c0 + t]) : 0.f;
+      put_split8(static_cast<uint16_t*>(out) + ((int64_t)p * nv + r) * ldo + xidx((int64_t)q * vp + c), v);
     }
   }
 }
@@ -101,52 +126,111 @@ __global__ void split_blocked_kernel(const double* __restrict__ src, int64_t row
   }
 }
 
-// R_P[a][b] += Z(i,j)[a][b] - Z3(j,i)[a][b] + Z(j,i)[b][a] - Z3(i,j)[b][a]
-__global__ void ring_combine_kernel(PairTab T, const double* __restrict__ Z, const double* __restrict__ Z3,
-                                    double* __restrict__ R) {
-  __shared__ double tile[32][33];
-  const int d = T.n_v;
-  const int64_t N = (int64_t)T.n_int * d;
-  const int nt = (d + 31) >> 5;
-  const int P = blockIdx.y;
-  const int i = T.pairs[2 * P], j = T.pairs[2 * P + 1];
-  const int a0 = (blockIdx.x / nt) * 32, b0 = (blockIdx.x % nt) * 32;
+// split rows of a transposed FP64 matrix: out16[c ldo + xidx(r)] = src[r ld + c],
+// r < nr, c < nc; k (= r) padded with zeros to a multiple of 8
+__global__ void transpose_split_kernel(const double* __restrict__ src, int64_t nr, int64_t nc, int64_t ld,
+                                       uint16_t* __restrict__ out, int64_t ldo) {
+  __shared__ float tile[32][33];
+  const int64_t r0 = (int64_t)blockIdx.y * 32, c0 = (int64_t)blockIdx.x * 32;
   const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
-  for (int yy = ty; yy < 32; yy += 8) {  // transposed part, read along a
-    const int b = b0 + yy, a = a0 + tx;
-    tile[yy][tx] = (a < d && b < d) ? Z[((int64_t)j * d + b) * N + (int64_t)i * d + a] -
-                                          Z3[((int64_t)i * d + b) * N + (int64_t)j * d + a]
-                                    : 0.0;
+  for (int yy = ty; yy < 32; yy += 8) {
+    const int64_t r = r0 + yy, c = c0 + tx;
+    tile[yy][tx] = (r < nr && c < nc) ? (float)src[r * ld + c] : 0.f;
   }
   __syncthreads();
-  double* Rb = R + T.cd_off[P];
-  for (int yy = ty; yy < 32; yy += 8) {
-    const int a = a0 + yy, b = b0 + tx;
-    if (a < d && b < d)
-      Rb[(int64_t)a * d + b] += Z[((int64_t)i * d + a) * N + (int64_t)j * d + b] -
-                                Z3[((int64_t)j * d + a) * N + (int64_t)i * d + b] + tile[tx][yy];
+  if (threadIdx.x < 128) {
+    const int cy = threadIdx.x >> 2, ch = threadIdx.x & 3;
+    const int64_t c = c0 + cy, k = r0 + 8 * ch;
+    if (c < nc && k < ((nr + 7) & ~(int64_t)7)) {
+      float v[8];
+#pragma unroll
+      for (int t = 0; t < 8; ++t) v[t] = tile[8 * ch + t][cy];
+      put_split8(out + c * ldo + xidx(k), v);
+    }
   }
 }
 
-// CT_Q = C_Q^T for every walk (d_Q x d_Q blocks at cd_off)
-__global__ void transpose_blocks_kernel(PairTab T, const double* __restrict__ C, double* __restrict__ CT) {
+// Yt[(i,a)][J] = Y[(J,a)][i]  (32 x 32 tiles of (J, a) per i)
+__global__ void y_transpose_kernel(const double* __restrict__ Y, int n_int, int n_v, int64_t n_chol,
+                                   double* __restrict__ Yt) {
   __shared__ double tile[32][33];
+  const int i = blockIdx.z;
+  const int64_t J0 = (int64_t)blockIdx.x * 32;
+  const int a0 = blockIdx.y * 32;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+  for (int yy = ty; yy < 32; yy += 8) {
+    const int64_t J = J0 + yy;
+    const int a = a0 + tx;
+    tile[yy][tx] = (J < n_chol && a < n_v) ? Y[(J * n_v + a) * n_int + i] : 0.0;
+  }
+  __syncthreads();
+  for (int yy = ty; yy < 32; yy += 8) {
+    const int a = a0 + yy;
+    const int64_t J = J0 + tx;
+    if (J < n_chol && a < n_v) Yt[((int64_t)i * n_v + a) * n_chol + J] = tile[tx][yy];
+  }
+}
+
+// Cemb[Q][a][b] = C_Q[pos a][pos b], CTemb[Q][b][a] = the same value (0 outside D_Q)
+__global__ void embed_blocks_kernel(PairTab T, EmbedTab E, const double* __restrict__ C, double* __restrict__ Cemb,
+                                    double* __restrict__ CTemb) {
+  __shared__ double tile[32][33];
+  const int nv = T.n_v, nt = (nv + 31) >> 5;
+  const int Q = blockIdx.y;
+  const int a0 = (blockIdx.x / nt) * 32, b0 = (blockIdx.x % nt) * 32;
+  const int d = (int)(T.dom_off[Q + 1] - T.dom_off[Q]);
+  const double* blk = C + T.cd_off[Q];
+  const int32_t* ps = E.pos + (int64_t)Q * nv;
+  const int64_t base = (int64_t)Q * nv * nv;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+  for (int yy = ty; yy < 32; yy += 8) {
+    const int a = a0 + yy, b = b0 + tx;
+    const int pa = a < nv ? ps[a] : -1, pb = b < nv ? ps[b] : -1;
+    const double v = (pa >= 0 && pb >= 0) ? blk[(int64_t)pa * d + pb] : 0.0;
+    tile[yy][tx] = v;
+    if (a < nv && b < nv) Cemb[base + (int64_t)a * nv + b] = v;
+  }
+  __syncthreads();
+  for (int yy = ty; yy < 32; yy += 8) {
+    const int b = b0 + yy, a = a0 + tx;
+    if (a < nv && b < nv) CTemb[base + (int64_t)b * nv + a] = tile[tx][yy];
+  }
+}
+
+// R_P[x][y] += Remb[P][a][b] + Z(i,j)[a][b] - Z3(j,i)[a][b] + Z(j,i)[b][a] - Z3(i,j)[b][a],
+// a = D_P[x], b = D_P[y]
+__global__ void ring_combine_kernel(PairTab T, const double* __restrict__ Z, const double* __restrict__ Z3,
+                                    const double* __restrict__ Remb, double* __restrict__ R) {
+  __shared__ double tile[32][33];
+  const int nv = T.n_v;
+  const int64_t N = (int64_t)T.n_int * nv;
   const int P = blockIdx.y;
   const int d = (int)(T.dom_off[P + 1] - T.dom_off[P]);
   const int nt = (d + 31) >> 5;
   if ((int)blockIdx.x >= nt * nt) return;
-  const int r0 = (blockIdx.x / nt) * 32, c0 = (blockIdx.x % nt) * 32;
+  const int i = T.pairs[2 * P], j = T.pairs[2 * P + 1];
+  const int32_t* D = T.dom_idx + T.dom_off[P];
+  const int x0 = (blockIdx.x / nt) * 32, y0 = (blockIdx.x % nt) * 32;
   const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
-  const double* src = C + T.cd_off[P];
-  double* dst = CT + T.cd_off[P];
-  for (int yy = ty; yy < 32; yy += 8) {
-    const int r = r0 + yy, c = c0 + tx;
-    tile[yy][tx] = (r < d && c < d) ? src[(int64_t)r * d + c] : 0.0;
+  for (int yy = ty; yy < 32; yy += 8) {  // transposed part: rows b, columns a
+    const int y = y0 + yy, x = x0 + tx;
+    double v = 0.0;
+    if (x < d && y < d) {
+      const int a = D[x], b = D[y];
+      v = Z[((int64_t)j * nv + b) * N + (int64_t)i * nv + a] - Z3[((int64_t)i * nv + b) * N + (int64_t)j * nv + a];
+    }
+    tile[yy][tx] = v;
   }
   __syncthreads();
+  double* Rb = R + T.cd_off[P];
+  const double* Hp = Remb + (int64_t)P * nv * nv;
   for (int yy = ty; yy < 32; yy += 8) {
-    const int c = c0 + yy, r = r0 + tx;
-    if (r < d && c < d) dst[(int64_t)c * d + r] = tile[tx][yy];
+    const int x = x0 + yy, y = y0 + tx;
+    if (x < d && y < d) {
+      const int a = D[x], b = D[y];
+      Rb[(int64_t)x * d + y] += Hp[(int64_t)a * nv + b] + Z[((int64_t)i * nv + a) * N + (int64_t)j * nv + b] -
+                                Z3[((int64_t)j * nv + a) * N + (int64_t)i * nv + b] + tile[tx][yy];
+    }
   }
 }
 
@@ -175,31 +259,41 @@ Seg dseg(const Op& A, const Op& B, int K, double alpha) {
   return s;
 }
 
+EmbedTab etab(const tci_plan& P) { return EmbedTab{P.pairid_d.as<int32_t>(), P.pos.as<int32_t>()}; }
+
 }  // namespace
 
-bool dense_pairs_eligible(const tci_plan& P) {
-  const int n = P.n_int;
-  if (P.n_pairs != n * (n + 1) / 2) return false;
-  if ((int64_t)n * P.n_v >= ((int64_t)1 << 31) / 2) return false;
-  for (int Q = 0; Q < P.n_pairs; ++Q) {
-    const int i = P.h_pairs[2 * Q], j = P.h_pairs[2 * Q + 1];
-    if ((int64_t)i * (i + 1) / 2 + j != Q) return false;
-    if (P.h_dom_off[Q + 1] - P.h_dom_off[Q] != P.n_v) return false;
-  }
-  return true;
+// Embedded products when they cost at most 8x the per-walk gathered ring
+// flops (f_gather) and their FP64 work matrices fit in 48 GB; TCI_DENSE_PAIRS=0
+// turns them off, =1 forces them within the memory bound.
+bool dense_pairs_choose(const tci_plan& P, double f_gather) {
+  const double N = (double)P.n_int * P.n_v;
+  if (P.n_pairs == 0 || N >= 46340.0) return false;  // N^2 must index with int32 tiles
+  const double bytes = 8.0 * (6.0 * N * N + 3.0 * P.n_pairs * (double)P.n_v * P.n_v);
+  if (bytes > 48.0e9) return false;
+  const char* env = std::getenv("TCI_DENSE_PAIRS");
+  if (env && env[0] == '0') return false;
+  if (env && env[0] == '1') return true;
+  const double f_dense = 6.0 * N * N * N + 4.0 * N * N * (double)P.n_chol;
+  return f_dense <= 8.0 * f_gather;
 }
 
-// FP64 tables: Jbig, the work matrices and the three dense programs; o4 is the
-// host copy of (ki|lj) = o4[((k n + i) n + l) n + j]
+// FP64 tables: Jbig, work matrices, coefficient matrix and the dense programs;
+// o4 is the host copy of (ki|lj) = o4[((k n + i) n + l) n + j]
 void dense_pairs_build(tci_plan& P, const std::vector<double>& o4, hipStream_t st) {
-  const int n = P.n_int, d = P.n_v, np = P.n_pairs;
-  const int64_t N = (int64_t)n * d, d2 = (int64_t)d * d;
+  const int n = P.n_int, nv = P.n_v, np = P.n_pairs;
+  const int64_t N = (int64_t)n * nv, v2 = (int64_t)nv * nv;
+  std::vector<int32_t> pid((size_t)n * n);
+  for (int p = 0; p < n; ++p)
+    for (int q = 0; q < n; ++q) pid[(size_t)p * n + q] = P.pairid[(size_t)p * n + q];
+  upload(P.pairid_d, pid, st);
   P.Jbig.alloc((size_t)N * N * 8);
-  hipLaunchKernelGGL(jbig_kernel, dim3(grid_for(N * N)), dim3(256), 0, st, P.Jvv.as<double>(), n, d,
+  hipLaunchKernelGGL(jbig_kernel, dim3(grid_for(N * N)), dim3(256), 0, st, P.Jvv.as<double>(), n, nv,
                      P.Jbig.as<double>());
   TCI_HIP(hipGetLastError());
   for (DevBuf* b : {&P.CRb, &P.CtRb, &P.CCb, &P.Zb, &P.Z3b}) b->alloc((size_t)N * N * 8);
-  P.CTb.alloc((size_t)P.sum_d2 * 8);
+  for (DevBuf* b : {&P.Cemb, &P.CTemb, &P.Remb}) b->alloc((size_t)np * v2 * 8);
+  P.Yt.alloc((size_t)N * P.n_chol * 8);
   std::vector<double> hA((size_t)np * 2 * np, 0.0);
   for (int X = 0; X < np; ++X) {
     const int i = P.h_pairs[2 * X], j = P.h_pairs[2 * X + 1];
@@ -210,12 +304,13 @@ void dense_pairs_build(tci_plan& P, const std::vector<double>& o4, hipStream_t s
     }
   }
   upload(P.holeA, hA, st);
-  const int iN = (int)N;
+  const int iN = (int)N, K = (int)P.n_chol;
   P.ringZ.clear();
   P.ringZ.dense = true;
   P.ringZ.tile = 128;
   P.ringZ.add(P.Zb.as<double>(), N, iN, iN,
-              {dseg(dop(P.CtRb.p, N), dop(P.Kov.p, N), iN, 1.0), dseg(dop(P.CRb.p, N), dop(P.Jbig.p, N), iN, -1.0)},
+              {dseg(dop(P.CtRb.p, N), dop(P.Kov.p, N), iN, 1.0), dseg(dop(P.CRb.p, N), dop(P.Jbig.p, N), iN, -1.0),
+               dseg(dop(P.Yt.p, P.n_chol), dop(P.Lov.p, N), K, 1.0)},
               false);
   P.ringZ.upload_tables(st);
   P.ringZ3.clear();
@@ -223,76 +318,113 @@ void dense_pairs_build(tci_plan& P, const std::vector<double>& o4, hipStream_t s
   P.ringZ3.tile = 128;
   P.ringZ3.add(P.Z3b.as<double>(), N, iN, iN, {dseg(dop(P.Jbig.p, N), dop(P.CCb.p, N), iN, 1.0)}, false);
   P.ringZ3.upload_tables(st);
+  // W[i][J][b] = (Lov CtR)[J][(i,b)]: column block i of n_v goes to W + i n_chol n_v
+  Blocking bw;
+  bw.cdiv = nv;
+  bw.cstride = P.n_chol * nv;
+  P.progWd.clear();
+  P.progWd.dense = true;
+  P.progWd.tile = 128;
+  P.progWd.add(P.W.as<double>(), nv, K, iN, {dseg(dop(P.Lov.p, N), dop(P.CtRb.p, N), iN, 1.0)}, false, false, 0, bw);
+  P.progWd.upload_tables(st);
   P.holeG.clear();
   P.holeG.dense = true;
   P.holeG.tile = 128;
-  P.holeG.add(P.R.as<double>(), d2, np, (int)d2,
-              {dseg(dop(P.holeA.p, 2 * np), dop(P.C.p, d2), np, 1.0),
-               dseg(dop(P.holeA.as<double>() + np, 2 * np), dop(P.CTb.p, d2), np, 1.0)},
-              true);
+  P.holeG.add(P.Remb.as<double>(), v2, np, (int)v2,
+              {dseg(dop(P.holeA.p, 2 * np), dop(P.Cemb.p, v2), np, 1.0),
+               dseg(dop(P.holeA.as<double>() + np, 2 * np), dop(P.CTemb.p, v2), np, 1.0)},
+              false);
   P.holeG.upload_tables(st);
 }
 
-// split copies of Kov and Jbig and the split programs for Z and Z3
+// split copies of Kov, Jbig, Lov, Lov^T and the split programs for Z, Z3 and W
 void dense_pairs_build_split(tci_plan& P, hipStream_t st) {
   tci_plan::MixedLadder& M = P.mx;
-  const int n = P.n_int, d = P.n_v;
-  const int64_t N = (int64_t)n * d, ldx = 2 * (int64_t)n * xpad(d);
+  const int n = P.n_int, nv = P.n_v;
+  const int64_t N = (int64_t)n * nv, ldx = 2 * (int64_t)n * xpad(nv), ldj = 2 * xpad(P.n_chol);
   for (DevBuf* b : {&M.Kx, &M.Jx, &M.CRx, &M.CtRx, &M.CCx}) {
     b->alloc((size_t)N * ldx * 2);
     M.bytes += (int64_t)b->bytes;
   }
-  const int64_t work = N * n * (xpad(d) / 8);
-  hipLaunchKernelGGL(split_blocked_kernel, dim3(grid_for(work)), dim3(256), 0, st, P.Kov.as<double>(), N, N, n, d,
+  M.Lovx.alloc((size_t)P.n_chol * ldx * 2);
+  M.LovTx.alloc((size_t)N * ldj * 2);
+  M.Ytx.alloc((size_t)N * ldj * 2);
+  M.bytes += (int64_t)(M.Lovx.bytes + M.LovTx.bytes + M.Ytx.bytes);
+  const int64_t work = N * n * (xpad(nv) / 8);
+  hipLaunchKernelGGL(split_blocked_kernel, dim3(grid_for(work)), dim3(256), 0, st, P.Kov.as<double>(), N, N, n, nv,
                      M.Kx.as<uint16_t>(), ldx);
-  hipLaunchKernelGGL(split_blocked_kernel, dim3(grid_for(work)), dim3(256), 0, st, P.Jbig.as<double>(), N, N, n, d,
+  hipLaunchKernelGGL(split_blocked_kernel, dim3(grid_for(work)), dim3(256), 0, st, P.Jbig.as<double>(), N, N, n, nv,
                      M.Jx.as<uint16_t>(), ldx);
+  hipLaunchKernelGGL(split_blocked_kernel, dim3(grid_for(P.n_chol * n * (xpad(nv) / 8))), dim3(256), 0, st,
+                     P.Lov.as<double>(), P.n_chol, N, n, nv, M.Lovx.as<uint16_t>(), ldx);
+  hipLaunchKernelGGL(transpose_split_kernel, dim3((unsigned)cdiv(N, 32), (unsigned)cdiv(P.n_chol, 32)), dim3(256), 0, st,
+                     P.Lov.as<double>(), P.n_chol, N, N, M.LovTx.as<uint16_t>(), ldj);
   TCI_HIP(hipGetLastError());
-  const int iN = (int)N, K = (int)(n * xpad(d));
-  auto xop = [&](const DevBuf& b) { return dop(b.p, ldx); };
+  const int iN = (int)N, K = (int)(n * xpad(nv)), KJ = (int)xpad(P.n_chol);
+  auto xop = [&](const DevBuf& b, int64_t ld) { return dop(b.p, ld); };
+  // B operands as [n][k] rows: Kov, Jbig and CC are symmetric; Lov enters transposed
   M.ringZx.clear();
   M.ringZx.split = true;
   M.ringZx.tile = 128;
-  // B operands as [n][k] rows: Kov and Jbig are symmetric, CC is symmetric
-  M.ringZx.add(P.Zb.as<double>(), N, iN, iN, {dseg(xop(M.CtRx), xop(M.Kx), K, 1.0), dseg(xop(M.CRx), xop(M.Jx), K, 1.0)},
+  M.ringZx.add(P.Zb.as<double>(), N, iN, iN,
+               {dseg(xop(M.CtRx, ldx), xop(M.Kx, ldx), K, 1.0), dseg(xop(M.CRx, ldx), xop(M.Jx, ldx), K, 1.0),
+                dseg(xop(M.Ytx, ldj), xop(M.LovTx, ldj), KJ, 1.0)},
                false);
   M.ringZx.upload_tables(st);
   M.ringZ3x.clear();
   M.ringZ3x.split = true;
   M.ringZ3x.tile = 128;
-  M.ringZ3x.add(P.Z3b.as<double>(), N, iN, iN, {dseg(xop(M.Jx), xop(M.CCx), K, 1.0)}, false);
+  M.ringZ3x.add(P.Z3b.as<double>(), N, iN, iN, {dseg(xop(M.Jx, ldx), xop(M.CCx, ldx), K, 1.0)}, false);
   M.ringZ3x.upload_tables(st);
+  Blocking bw;
+  bw.cdiv = nv;
+  bw.cstride = P.n_chol * nv;
+  M.progWx.clear();
+  M.progWx.split = true;
+  M.progWx.tile = 128;
+  M.progWx.add(P.W.as<double>(), nv, (int)P.n_chol, iN, {dseg(xop(M.Lovx, ldx), xop(M.CtRx, ldx), K, 1.0)}, false,
+               false, 0, bw);
+  M.progWx.upload_tables(st);
 }
 
-// ring (Z, Z3 and their combination) and 4-internal terms into R
+// ring, 3-external (X and W) and 4-internal terms into R and W; needs Y
 void dense_pairs_terms(tci_plan& P, int precision, hipStream_t st) {
-  const int n = P.n_int, d = P.n_v;
-  const int64_t N = (int64_t)n * d;
-  const int nt = (d + 31) / 32;
+  const int n = P.n_int, nv = P.n_v;
+  const int64_t N = (int64_t)n * nv;
+  const int nt = (nv + 31) / 32;
   const dim3 g(n * nt, n * nt), b(256);
+  const EmbedTab E = etab(P);
+  hipLaunchKernelGGL(y_transpose_kernel, dim3((unsigned)cdiv(P.n_chol, 32), nt, n), b, 0, st, P.Y.as<double>(), n, nv,
+                     P.n_chol, P.Yt.as<double>());
   if (precision == TCI_SIGMA_FP64) {
-    hipLaunchKernelGGL(relayout_kernel<false>, g, b, 0, st, P.Ct.as<double>(), d, false, 1.0, P.CtRb.p, N);
-    hipLaunchKernelGGL(relayout_kernel<false>, g, b, 0, st, P.C.as<double>(), d, false, 1.0, P.CRb.p, N);
-    hipLaunchKernelGGL(relayout_kernel<false>, g, b, 0, st, P.C.as<double>(), d, true, 1.0, P.CCb.p, N);
+    hipLaunchKernelGGL(relayout_kernel<false>, g, b, 0, st, P.tab, E, P.Ct.as<double>(), false, 1.0, P.CtRb.p, N);
+    hipLaunchKernelGGL(relayout_kernel<false>, g, b, 0, st, P.tab, E, P.C.as<double>(), false, 1.0, P.CRb.p, N);
+    hipLaunchKernelGGL(relayout_kernel<false>, g, b, 0, st, P.tab, E, P.C.as<double>(), true, 1.0, P.CCb.p, N);
     TCI_HIP(hipGetLastError());
     P.ringZ.launch(st);
     P.ringZ3.launch(st);
+    P.progWd.launch(st);
   } else {
     tci_plan::MixedLadder& M = P.mx;
-    const int64_t ldx = 2 * (int64_t)n * xpad(d);
-    hipLaunchKernelGGL(relayout_kernel<true>, g, b, 0, st, P.Ct.as<double>(), d, false, 1.0, M.CtRx.p, ldx);
-    hipLaunchKernelGGL(relayout_kernel<true>, g, b, 0, st, P.C.as<double>(), d, false, -1.0, M.CRx.p, ldx);
-    hipLaunchKernelGGL(relayout_kernel<true>, g, b, 0, st, P.C.as<double>(), d, true, 1.0, M.CCx.p, ldx);
+    const int64_t ldx = 2 * (int64_t)n * xpad(nv), ldj = 2 * xpad(P.n_chol);
+    hipLaunchKernelGGL(relayout_kernel<true>, g, b, 0, st, P.tab, E, P.Ct.as<double>(), false, 1.0, M.CtRx.p, ldx);
+    hipLaunchKernelGGL(relayout_kernel<true>, g, b, 0, st, P.tab, E, P.C.as<double>(), false, -1.0, M.CRx.p, ldx);
+    hipLaunchKernelGGL(relayout_kernel<true>, g, b, 0, st, P.tab, E, P.C.as<double>(), true, 1.0, M.CCx.p, ldx);
+    hipLaunchKernelGGL(split_blocked_kernel, dim3(grid_for(N * (xpad(P.n_chol) / 8))), b, 0, st, P.Yt.as<double>(), N,
+                       P.n_chol, 1, (int)P.n_chol, M.Ytx.as<uint16_t>(), ldj);
     TCI_HIP(hipGetLastError());
     M.ringZx.launch_split(precision, st);
     M.ringZ3x.launch_split(precision, st);
+    M.progWx.launch_split(precision, st);
   }
-  hipLaunchKernelGGL(ring_combine_kernel, dim3(nt * nt, P.n_pairs), b, 0, st, P.tab, P.Zb.as<double>(),
-                     P.Z3b.as<double>(), P.R.as<double>());
-  hipLaunchKernelGGL(transpose_blocks_kernel, dim3(nt * nt, P.n_pairs), b, 0, st, P.tab, P.C.as<double>(),
-                     P.CTb.as<double>());
+  hipLaunchKernelGGL(embed_blocks_kernel, dim3(nt * nt, P.n_pairs), b, 0, st, P.tab, E, P.C.as<double>(),
+                     P.Cemb.as<double>(), P.CTemb.as<double>());
   TCI_HIP(hipGetLastError());
   P.holeG.launch(st);
+  const int ntd = (P.dmax + 31) / 32;
+  hipLaunchKernelGGL(ring_combine_kernel, dim3(ntd * ntd, P.n_pairs), b, 0, st, P.tab, P.Zb.as<double>(),
+                     P.Z3b.as<double>(), P.Remb.as<double>(), P.R.as<double>());
+  TCI_HIP(hipGetLastError());
 }
 
 }  // namespace tci

@@ -345,8 +345,19 @@ void tci_plan::build(const tci_problem& p) {
     return transposed ? op(b, 1, d, rm, cm) : op(b, d, 1, rm, cm);
   };
   {
-    const char* env = std::getenv("TCI_DENSE_PAIRS");
-    dense_pairs = dense_pairs_eligible(*this) && !(env && env[0] == '0');
+    // flops of the per-walk gathered ring and 3-external products
+    double fg = 0.0;
+    for (int P = 0; P < n_pairs; ++P) {
+      const double d = dsz(P);
+      const int i = h_pairs[2 * P], j = h_pairs[2 * P + 1];
+      for (int k = 0; k < n_int; ++k) {
+        const int Q1 = pairid[(size_t)i * n_int + k], Q2 = pairid[(size_t)j * n_int + k];
+        if (Q1 >= 0) fg += 3.0 * 2.0 * d * d * dsz(Q1);
+        if (Q2 >= 0) fg += 3.0 * 2.0 * d * d * dsz(Q2);
+      }
+      fg += 2.0 * 2.0 * d * d * (double)n_chol + 2.0 * d * n_v * (double)n_chol;
+    }
+    dense_pairs = dense_pairs_choose(*this, fg);
   }
   ring.clear();
   ring.tile = 128;
@@ -376,6 +387,7 @@ void tci_plan::build(const tci_problem& p) {
         sg.push_back(seg(blk(Cp, Q2, tr, m, nullptr), Jop(k, i, DL(Q2), DP), dq, -1.0));
       }
     }
+    if (dense_pairs) continue;
     // 3-external singles: X^{ij}_ab + X^{ji}_ba, X^{ij}_ab = sum_J Y[J][a][i] L^J_{jb}
     const double* Yp = Y.as<double>();
     const double* Lo = Lov.as<double>();
@@ -393,7 +405,7 @@ void tci_plan::build(const tci_problem& p) {
   // W[i][J][b] = sum_j sum_c Ct^{ij}[pos(b)][c] L^J_{j D[c]}
   progW.clear();
   progW.tile = 128;
-  for (int i = 0; i < n_int; ++i) {
+  for (int i = 0; i < n_int && !dense_pairs; ++i) {
     std::vector<Seg> sg;
     for (int j = 0; j < n_int; ++j) {
       const int Q = pairid[(size_t)i * n_int + j];
@@ -474,9 +486,9 @@ void tci_plan::build(const tci_problem& p) {
     f_ring += 2.0 * 2.0 * d * d * (double)n_chol;
   }
   bytes_ladder = bytes;
-  if (dense_pairs) {  // executed products: Z (two N^3 terms), Z3, the 4-internal GEMM, the X terms
+  if (dense_pairs) {  // executed products: Z (three terms), Z3, W, the 4-internal GEMM
     const double N = (double)niv, d2 = (double)n_v * n_v;
-    f_ring = 6.0 * N * N * N + 2.0 * n_pairs * 2.0 * n_pairs * d2 + n_pairs * 2.0 * 2.0 * d2 * (double)n_chol;
+    f_ring = 6.0 * N * N * N + 4.0 * N * N * (double)n_chol + 2.0 * n_pairs * 2.0 * n_pairs * d2;
   }
   flops_sigma = flops_ladder + f_ring + 2.0 * (double)n_chol * n_v * niv * 3.0 + 2.0 * (double)niv * niv;
   TCI_HIP(hipDeviceSynchronize());
@@ -484,7 +496,7 @@ void tci_plan::build(const tci_problem& p) {
   for (DevBuf* b : {&pairs, &dom_off, &dom_idx, &cd_off, &aa_off, &ab_off, &eps, &pos, &xmap, &isect_off, &isect,
                     &inv_off, &inv_q, &inv_pos, &gpart_off, &Lvv, &Lov, &Kov, &Jvv, &Mss, &O4, &O3, &vvd, &vvx,
                     &diag, &C, &Ct, &R, &s, &g, &gss, &gW, &Y, &W, &gpart, &part, &sig0, &lad_part, &Jbig, &CRb,
-                    &CtRb, &CCb, &Zb, &Z3b, &CTb, &holeA, &Tbuf, &Rpart, &Lp, &Cw})
+                    &CtRb, &CCb, &Zb, &Z3b, &Cemb, &CTemb, &Remb, &Yt, &holeA, &Tbuf, &Rpart, &Lp, &Cw})
     device_bytes += (int64_t)b->bytes;
 }
 
@@ -683,16 +695,16 @@ void tci_plan::sigma(const double* x, double* y, int flags, hipStream_t st) {
   if (profiling) TCI_HIP(hipEventRecord(ev[0], st));
   unpack_and_ladder(x, precision, st);
   launch_rterms(tab, dmax, x, s.as<double>(), Kov.as<double>(), O3.as<double>(), R.as<double>(), st);
+  progY.launch(st);
   if (dense_pairs) {
     dense_pairs_terms(*this, precision, st);
   } else {
     HoleTab H{isect_off.as<int64_t>(), isect.as<int32_t>()};
     launch_hole(tab, H, dmax, C.as<double>(), O4.as<double>(), R.as<double>(), st);
+    ring.launch(st);
+    progW.launch(st);
   }
-  progY.launch(st);
-  ring.launch(st);
   progGss.launch(st);
-  progW.launch(st);
   progGW.launch(st);
   launch_s3int_singles(tab, gpart_off.as<int64_t>(), Ct.as<double>(), O3.as<double>(), gpart.as<double>(), st);
   InvTab I{inv_off.as<int64_t>(), inv_q.as<int32_t>(), inv_pos.as<int32_t>()};

@@ -31,10 +31,10 @@ struct tci_plan {
   int lad_chunks = 1;
   // programs
   tci::GemmProgram ring, progY, progW, progGW, progGss;
-  // full-domain pair terms as dense products (pairs_full.hip)
+  // pair terms as dense products over embedded pair blocks (pairs_full.hip)
   bool dense_pairs = false;
-  tci::DevBuf Jbig, CRb, CtRb, CCb, Zb, Z3b, CTb, holeA;
-  tci::GemmProgram ringZ, ringZ3, holeG;
+  tci::DevBuf pairid_d, Jbig, CRb, CtRb, CCb, Zb, Z3b, Cemb, CTemb, Remb, Yt, holeA;
+  tci::GemmProgram ringZ, ringZ3, holeG, progWd;
   tci::LadderLaunch ladder;        // fused single-kernel ladder (TCI_LADDER=fused)
   bool ladder_fused = false;
   // two-phase ladder: per J-chunk of length Lc, T_P = L[D,D] C_P then R_P += sum T L
@@ -64,14 +64,16 @@ struct tci_plan {
     tci::DevBuf Lpx;                // split rows of L^J[D,D] of local-domain pairs (one chunk)
     tci::DevBuf pairs_local;        // indices of the local-domain pairs
     int n_local = 0;
+    bool prepacked = false;         // Lpx holds every J (packed once per plan)
+    tci::DevBuf lp_off_all;         // offsets of the local pairs in a prepacked Lpx
     tci::DevBuf Cx, cx_off;         // split rows of the C_P blocks (of C_P^T when merged)
     bool merged = false;            // all walks full-domain: single products over all walks
     tci::DevBuf Tx;                 // T' of one chunk
     tci::DevBuf Rpart;              // partial R buffers of phase 2 K-splits
     int rpart_n = 1;
     // dense pair terms in split precision (full domains)
-    tci::DevBuf Kx, Jx, CRx, CtRx, CCx;
-    tci::GemmProgram ringZx, ringZ3x;
+    tci::DevBuf Kx, Jx, CRx, CtRx, CCx, Lovx, LovTx, Ytx;
+    tci::GemmProgram ringZx, ringZ3x, progWx;
     std::vector<LadderPhase> phase;
     std::vector<std::pair<int64_t, int>> chunks;
     int64_t bytes = 0;
@@ -100,7 +102,7 @@ namespace tci {
 void mixed_prepare(tci_plan& P, hipStream_t st);
 void mixed_ladder(tci_plan& P, int precision, hipStream_t st);
 // dense pair terms (pairs_full.hip)
-bool dense_pairs_eligible(const tci_plan& P);
+bool dense_pairs_choose(const tci_plan& P, double f_gather);
 void dense_pairs_build(tci_plan& P, const std::vector<double>& o4, hipStream_t st);
 void dense_pairs_build_split(tci_plan& P, hipStream_t st);
 void dense_pairs_terms(tci_plan& P, int precision, hipStream_t st);

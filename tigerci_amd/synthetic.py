@@ -26,6 +26,10 @@ WORKLOADS = {
     # integrals (diagonally dominant H, reference weight ~0.95)
     "davidson_16x300x1200_full": (16, 300, 1200, None, 0, np.float64, 0.03),
     "davidson_40x1000x3000_dom40-400": (40, 1000, 3000, (40, 400), 80, np.float32, 0.03),
+    # the shape of a cc-pVDZ polyene C12H14 local MRSDCI (34 correlated
+    # internal, 192 virtual orbitals), local pair domains of 40..192 virtuals,
+    # pairs below 60 singles-only
+    "davidson_34x192x1200_dom40-192": (34, 192, 1200, (40, 192), 60, np.float64, 0.02),
 }
 
 
@@ -73,8 +77,9 @@ def make_synthetic(n_int: int, n_v: int, n_chol: int, seed: int, domain_range=No
     return Synthetic(n_int, n_v, n_chol, eps, L, np.array(pairs, np.int32).reshape(-1, 2), dom_off, dom_idx)
 
 
-def workload(name: str, seed: int = 1234) -> Synthetic:
+def workload(name: str, seed: int = 1234, l_scale: float | None = None) -> Synthetic:
     n_int, n_v, n_chol, dr, weak, dt, ls = WORKLOADS[name]
+    ls = ls if l_scale is None else l_scale
     return make_synthetic(n_int, n_v, n_chol, seed, domain_range=dr, weak_below=weak, dtype=dt, l_scale=ls)
 
 
