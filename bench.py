@@ -33,6 +33,19 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def measured_traffic(workload: str, precision: str):
+    """HBM bytes of one sigma's ladder section, measured with rocprofv3 --pmc
+    (profiles/ladder_traffic.txt), or None for an unmeasured workload."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "ladder_traffic.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        f = line.split()
+        if len(f) == 3 and not line.startswith("#") and f[0] == workload and f[1] == precision:
+            return float(f[2])
+    return None
+
+
 def cpu_baseline(syn, plan, n_sigma_fp64: int, budget_s: float = 15.0) -> dict:
     """Time the oracle's dense ladder restatement on a bounded sample of pair
     blocks of the same workload, and scale to one FP64 Davidson solve
@@ -43,10 +56,12 @@ def cpu_baseline(syn, plan, n_sigma_fp64: int, budget_s: float = 15.0) -> dict:
     n_int = syn.n_int
     L = syn.L
     order = np.argsort(-np.diff(syn.dom_off))
+    sample = list(order[::max(1, len(order) // 16)])
     flops = 0.0
     t0 = time.perf_counter()
     used = 0
-    for P in order[::max(1, len(order) // 16)]:
+    while time.perf_counter() - t0 < budget_s:
+        P = sample[used % len(sample)]
         D = syn.dom_idx[syn.dom_off[P]:syn.dom_off[P + 1]]
         d = len(D)
         a = n_int + D
@@ -58,14 +73,13 @@ def cpu_baseline(syn, plan, n_sigma_fp64: int, budget_s: float = 15.0) -> dict:
         (Lp @ C @ Lp).sum(axis=0)
         flops += 4.0 * d ** 3 * syn.n_chol
         used += 1
-        if time.perf_counter() - t0 > budget_s:
-            break
     dt = time.perf_counter() - t0
     rate = flops / dt
     threads = max([x.get("num_threads", 1) for x in threadpool_info() if x.get("user_api") == "blas"] or [1])
     sigma_s = plan.info.flops_sigma / rate
     return {"value": sigma_s * n_sigma_fp64, "unit": "s", "cores": int(threads), "kind": "port",
-            "sample": f"oracle dense ladder (numpy BLAS, FP64) on {used} of {len(order)} pair blocks "
+            "sample": f"oracle dense ladder (numpy BLAS, FP64), {used} pair-block ladders drawn from "
+                      f"{len(sample)} of {len(order)} walks "
                       f"({flops:.3g} flop in {dt:.1f} s = {rate / 1e9:.1f} GFLOP/s), scaled by flops to "
                       f"{n_sigma_fp64} FP64 sigma builds of {plan.info.flops_sigma:.3g} flop",
             "sigma_seconds": sigma_s}
@@ -77,7 +91,7 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=1)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default=None,
-                    help="default: davidson_16x300x1200_full (davidson), sigma_16x300x1200_full (sigma)")
+                    help="default: davidson_34x192x1200_dom40-192 (davidson), sigma_16x300x1200_full (sigma)")
     ap.add_argument("--roots", type=int, default=3)
     ap.add_argument("--l-scale", type=float, default=None, help="override the workload's Cholesky-vector scale")
     ap.add_argument("--precision", default="bf16x3")
@@ -97,7 +111,7 @@ def main() -> int:
     from tigerci_amd.synthetic import workload, random_vector, WORKLOADS
 
     if a.workload is None:
-        a.workload = "davidson_16x300x1200_full" if a.mode == "davidson" else "sigma_16x300x1200_full"
+        a.workload = "davidson_34x192x1200_dom40-192" if a.mode == "davidson" else "sigma_16x300x1200_full"
     t = time.perf_counter()
     syn = workload(a.workload, a.seed, l_scale=a.l_scale)
     log(f"[bench] generated {a.workload} in {time.perf_counter() - t:.1f} s")
@@ -134,7 +148,8 @@ def main() -> int:
         out.update({"value": info.flops_sigma / dt / 1e12, "unit": "TFLOP/s", "higher_is_better": True,
                     "ms_per_step": dt * 1e3, "dtype": a.precision, "config": cfg,
                     "roofline": {"bound": "mfma", "achieved": ach / 1e12, "peak": PEAK[a.precision] / 1e12,
-                                 "unit": "TFLOP/s", "frac": ach / PEAK[a.precision], "traffic": None,
+                                 "unit": "TFLOP/s", "frac": ach / PEAK[a.precision],
+                                 "traffic": measured_traffic(a.workload, a.precision),
                                  "kernel": "ladder", "avg_launch_ms": lad * 1e3}})
         if a.copy_probe_mib:
             n = a.copy_probe_mib * (1 << 20) // 8
@@ -197,8 +212,9 @@ def main() -> int:
         "sigma_check": sig_err,
         "sigma_tflops_fp64": info.flops_sigma / sig64 / 1e12,
     })
+    tb64 = measured_traffic(a.workload, "fp64")
     roof64 = {"bound": "mfma", "achieved": ach / 1e12, "peak": PEAK["fp64"] / 1e12, "unit": "TFLOP/s",
-              "frac": ach / PEAK["fp64"], "traffic": None,
+              "frac": ach / PEAK["fp64"], "traffic": tb64, "traffic_gbs": tb64 / lad / 1e9 if tb64 else None,
               "kernel": "FP64 ladder (dgemm_kernel phases 1+2, per sigma)",
               "avg_launch_ms": lad * 1e3, "launches": p64["ladder_launches"]}
     if a.precision != "fp64" and pm["ladder_launches"]:
@@ -207,9 +223,11 @@ def main() -> int:
         out["sigma_tflops_mixed"] = info.flops_sigma / sgm / 1e12
         # algorithmic (FP64-equivalent) ladder flops against the dense bf16
         # peak; each product costs 3 bf16 MFMAs, so MFMA busy = 3 x frac
+        tb = measured_traffic(a.workload, a.precision)
         out["roofline"] = {"bound": "mfma", "achieved": info.flops_ladder / lm / 1e12,
                            "peak": PEAK[a.precision] / 1e12, "unit": "TFLOP/s",
-                           "frac": info.flops_ladder / lm / PEAK[a.precision], "traffic": None,
+                           "frac": info.flops_ladder / lm / PEAK[a.precision], "traffic": tb,
+                           "traffic_gbs": tb / lm / 1e9 if tb else None,
                            "kernel": "bf16x3 ladder (xgemm_kernel phases 1+2, per sigma)",
                            "mfma_flops_per_algorithmic_flop": 3,
                            "avg_launch_ms": lm * 1e3, "launches": pm["ladder_launches"]}

@@ -323,7 +323,12 @@ void tci_plan::build(const tci_problem& p) {
   W.alloc((size_t)n_chol * niv * 8);
   part.alloc((size_t)std::max(n_pairs, 1) * 8);
   sig0.alloc(8);
-  n_split = (int)std::min<int64_t>(std::max<int64_t>(1, n_chol / 64), 64);
+  {
+    // K-splits of gW = W L: enough 64 x 64 tiles for the chip, at least 16
+    // Cholesky vectors per split
+    const int64_t tiles = cdiv(n_int, 64) * cdiv(n_v, 64);
+    n_split = (int)std::max<int64_t>(1, std::min<int64_t>(std::min<int64_t>(n_chol / 16, 256), cdiv(1024, tiles)));
+  }
   gW.alloc((size_t)n_split * niv * 8);
 
   // ---- grouped task tables -----------------------------------------------------
@@ -432,11 +437,7 @@ void tci_plan::build(const tci_problem& p) {
                false);
   }
   progGW.upload_tables(st);
-  // gss = (2K - J) s
-  progGss.clear();
-  progGss.add(gss.as<double>(), 1, (int)niv, 1, {seg(op(Mss.as<double>(), niv, 1), op(s.as<double>(), 1, 0), (int)niv, 1.0)},
-              false);
-  progGss.upload_tables(st);
+  // gss = (2K - J) s runs as a GEMV (launch_gemv) inside every sigma
 
   // ---- ladder work items -------------------------------------------------------
   {
@@ -704,7 +705,7 @@ void tci_plan::sigma(const double* x, double* y, int flags, hipStream_t st) {
     ring.launch(st);
     progW.launch(st);
   }
-  progGss.launch(st);
+  launch_gemv(Mss.as<double>(), (int64_t)n_int * n_v, s.as<double>(), gss.as<double>(), st);
   progGW.launch(st);
   launch_s3int_singles(tab, gpart_off.as<int64_t>(), Ct.as<double>(), O3.as<double>(), gpart.as<double>(), st);
   InvTab I{inv_off.as<int64_t>(), inv_q.as<int32_t>(), inv_pos.as<int32_t>()};

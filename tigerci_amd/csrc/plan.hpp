@@ -30,7 +30,7 @@ struct tci_plan {
   int n_split = 1;
   int lad_chunks = 1;
   // programs
-  tci::GemmProgram ring, progY, progW, progGW, progGss;
+  tci::GemmProgram ring, progY, progW, progGW;
   // pair terms as dense products over embedded pair blocks (pairs_full.hip)
   bool dense_pairs = false;
   tci::DevBuf pairid_d, Jbig, CRb, CtRb, CCb, Zb, Z3b, Cemb, CTemb, Remb, Yt, holeA;

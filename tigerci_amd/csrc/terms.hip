@@ -426,6 +426,21 @@ __global__ void concat_cols_kernel(const double* __restrict__ C, int n_pairs, in
   }
 }
 
+// y = A x for a square row-major n x n matrix: one wave per row
+__global__ void gemv_kernel(const double* __restrict__ A, int64_t n, const double* __restrict__ x,
+                            double* __restrict__ y) {
+  const int lane = threadIdx.x & 63;
+  for (int64_t r = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); r < n;
+       r += (int64_t)gridDim.x * (blockDim.x >> 6)) {
+    const double* a = A + r * n;
+    double acc = 0.0;
+    for (int64_t c = lane; c < n; c += 64) acc += a[c] * x[c];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
+    if (lane == 0) y[r] = acc;
+  }
+}
+
 __global__ void fill_kernel(double* p, int64_t n, double v) {
   for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x)
     p[e] = v;
@@ -537,6 +552,12 @@ void launch_concat_cols(const double* C, int n_pairs, int d, double* Cw, hipStre
   const int64_t n = (int64_t)n_pairs * d * d;
   if (n <= 0) return;
   hipLaunchKernelGGL(concat_cols_kernel, dim3(gx(n)), dim3(256), 0, st, C, n_pairs, d, Cw);
+  TCI_HIP(hipGetLastError());
+}
+
+void launch_gemv(const double* A, int64_t n, const double* x, double* y, hipStream_t st) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(gemv_kernel, dim3((unsigned)std::min<int64_t>(cdiv(n, 4), 8192)), dim3(256), 0, st, A, n, x, y);
   TCI_HIP(hipGetLastError());
 }
 

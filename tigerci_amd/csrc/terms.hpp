@@ -70,6 +70,8 @@ void launch_vv_exch(const void* Lvv, bool f32, int64_t n_chol, int n_v, double* 
 void launch_build_mss(const double* Kov, const double* Jvv, int n_int, int n_v, double* Mss,
                       hipStream_t st);
 void launch_fill(double* p, int64_t n, double v, hipStream_t st);
+// y = A x, A n x n row-major
+void launch_gemv(const double* A, int64_t n, const double* x, double* y, hipStream_t st);
 // Cw[k][P d + c] = C_P[k][c] for n_pairs blocks of size d x d stored back to back
 void launch_concat_cols(const double* C, int n_pairs, int d, double* Cw, hipStream_t st);
 // Lp[P][jj][r][c] = L^{J0+jj}[D_P r][D_P c] for the local-domain pairs of a chunk

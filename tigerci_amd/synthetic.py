@@ -29,7 +29,7 @@ WORKLOADS = {
     # the shape of a cc-pVDZ polyene C12H14 local MRSDCI (34 correlated
     # internal, 192 virtual orbitals), local pair domains of 40..192 virtuals,
     # pairs below 60 singles-only
-    "davidson_34x192x1200_dom40-192": (34, 192, 1200, (40, 192), 60, np.float64, 0.02),
+    "davidson_34x192x1200_dom40-192": (34, 192, 1200, (40, 192), 60, np.float64, 0.01),
 }
 
 
