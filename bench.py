@@ -145,6 +145,13 @@ def main() -> int:
         pr = plan.profile_read(a.precision)
         lad = pr["ladder_ms"] / max(pr["ladder_launches"], 1) / 1e3
         ach = info.flops_ladder / lad
+        if a.precision != "fp64":  # precision study: this sigma against the FP64 one
+            s_mx = y.numpy()
+            plan.sigma(x, y, precision="fp64")
+            s_64 = y.numpy()
+            xv = x.numpy()
+            out["sigma_check"] = {"sigma_rel_err": float(np.linalg.norm(s_mx - s_64) / np.linalg.norm(s_64)),
+                                  "rayleigh_err_Eh": float(abs(xv @ s_mx - xv @ s_64))}
         out.update({"value": info.flops_sigma / dt / 1e12, "unit": "TFLOP/s", "higher_is_better": True,
                     "ms_per_step": dt * 1e3, "dtype": a.precision, "config": cfg,
                     "roofline": {"bound": "mfma", "achieved": ach / 1e12, "peak": PEAK[a.precision] / 1e12,

@@ -43,7 +43,7 @@ enum tci_dtype { TCI_F64 = 0, TCI_F32 = 1 };
 enum tci_precision {
   TCI_SIGMA_FP64 = 0,    /* FP64 MFMA throughout */
   TCI_SIGMA_BF16X3 = 1,  /* ladder as hi*hi + hi*lo + lo*hi BF16 MFMA, FP32 accumulate */
-  TCI_SIGMA_FP16X3 = 2   /* reserved (scaled FP16 split): rejected with TCI_ERR_ARG */
+  TCI_SIGMA_FP16X3 = 2   /* the same split in FP16 with power-of-two operand scales, FP32 accumulate */
 };
 
 /* flag bits for tci_sigma_apply (low byte = tci_precision) */

@@ -2,7 +2,7 @@
 
 The bf16x3 products keep ~17 significant bits per operand (hi + lo bf16) and
 accumulate in FP32, so sigma agrees with the oracle to a few 1e-6 relative
-(bound used: 5e-5 in the 2-norm).  Split precision covers the ladder and,
+(bound used: 5e-5 in the 2-norm); scaled fp16x3 keeps ~22 bits (bound 2e-6).  Split precision covers the ladder and,
 where the plan runs them as dense products, the ring and 3-external terms;
 the remaining terms stay FP64.  The mixed
 Davidson refines in FP64 after the switch, so its roots must match the FP64
@@ -37,6 +37,10 @@ def _sigma(plan, x, **kw):
     return out.numpy()
 
 
+SPLIT_TOL = {"bf16x3": TOL_MIXED, "fp16x3": 2e-6}
+
+
+@pytest.mark.parametrize("precision", ["bf16x3", "fp16x3"])
 @pytest.mark.parametrize("args", [
     dict(n_int=2, n_v=5, n_chol=7, seed=1),
     dict(n_int=4, n_v=37, n_chol=45, seed=2),
@@ -44,26 +48,26 @@ def _sigma(plan, x, **kw):
     dict(n_int=3, n_v=300, n_chol=30, seed=4, domain_range=(130, 300)),
     dict(n_int=4, n_v=150, n_chol=300, seed=5),
 ])
-def test_sigma_bf16x3(args):
-    _say("case: bf16x3 sigma", args)
+def test_sigma_split(args, precision):
+    _say("case:", precision, "sigma", args)
     prob = make_problem(**args)
     ints = Integrals(prob)
     plan = tci.Plan.from_problem(prob)
     for k in range(2):
         x = random_ci_vector(prob, 70 + k)
         ref = oracle_sigma(prob, x, ints)
-        mixed = _sigma(plan, x, precision="bf16x3")
+        mixed = _sigma(plan, x, precision=precision)
         fp64 = _sigma(plan, x)
         err = _rel(mixed, ref)
-        _say(f"  bf16x3 rel err {err:.3e}  fp64 rel err {_rel(fp64, ref):.3e}")
-        assert err < TOL_MIXED
+        _say(f"  {precision} rel err {err:.3e}  fp64 rel err {_rel(fp64, ref):.3e}")
+        assert err < SPLIT_TOL[precision]
         assert _rel(fp64, ref) < 1e-10
     # a vector without doubles (reference + singles) exercises the split
     # 3-external products alone
     lay = prob.layout()
     x = np.zeros(lay.n_csf)
     x[: 1 + prob.n_int * prob.n_v] = random_ci_vector(prob, 90)[: 1 + prob.n_int * prob.n_v]
-    assert _rel(_sigma(plan, x, precision="bf16x3"), oracle_sigma(prob, x, ints)) < TOL_MIXED
+    assert _rel(_sigma(plan, x, precision=precision), oracle_sigma(prob, x, ints)) < SPLIT_TOL[precision]
 
 
 def test_sigma_bf16x3_flags_and_repeat():
@@ -80,12 +84,20 @@ def test_sigma_bf16x3_flags_and_repeat():
     assert np.allclose(out.numpy(), 1.0 + a + plan.info.e_ref * x, rtol=0, atol=1e-12 * max(1, np.abs(a).max()))
 
 
-def test_fp16x3_is_rejected():
-    _say("case: fp16x3 rejected")
-    prob = make_problem(2, 6, 5, 1)
+def test_switching_split_precisions():
+    """One plan serves both split precisions (the split copies are rebuilt on a switch)."""
+    _say("case: bf16x3 -> fp16x3 -> bf16x3 on one plan")
+    prob = make_problem(3, 30, 25, 8, domain_range=(10, 30), weak_below=12)
+    ints = Integrals(prob)
     plan = tci.Plan.from_problem(prob)
-    with pytest.raises(tci.TigerSigmaError):
-        _sigma(plan, random_ci_vector(prob, 1), precision="fp16x3")
+    x = random_ci_vector(prob, 4)
+    ref = oracle_sigma(prob, x, ints)
+    a = _sigma(plan, x, precision="bf16x3")
+    b = _sigma(plan, x, precision="fp16x3")
+    c = _sigma(plan, x, precision="bf16x3")
+    assert np.array_equal(a, c)
+    assert _rel(b, ref) < SPLIT_TOL["fp16x3"] and _rel(a, ref) < SPLIT_TOL["bf16x3"]
+    assert _rel(b, ref) < _rel(a, ref)
 
 
 @pytest.mark.parametrize("args,roots", [

@@ -129,6 +129,8 @@ struct Task {
   // (column offset through xidx for split-row output); rdiv / cdiv = 0: no blocking
   int32_t rdiv, cdiv;
   int64_t rstride, cstride;
+  int32_t oscale;    // split kernel: result multiplied by scales[oscale] (-1: not scaled)
+  int32_t pad_;
 };
 struct Blocking {
   int32_t rdiv = 0, cdiv = 0;
@@ -144,22 +146,24 @@ struct GemmProgram {
   int tile = 64;
   bool dense = false;  // contiguous row-major operands only: dense_gemm.hip kernel (tile 64 or 128)
   bool split = false;  // split-row operands, C = sum A B^T: dense_split.hip kernel (tile 64 or 128)
+  bool scaled = false; // some task has an output scale (set by upload_tables)
   std::vector<Task> tasks;
   std::vector<Seg> segs;
   DevBuf d_tasks, d_segs;
   void clear() { tasks.clear(); segs.clear(); }
   // add one output matrix: tiles over (M,N), all sharing `segs_for_output`
   void add(double* C, int64_t ldc, int M, int N, const std::vector<Seg>& s, bool accumulate,
-           bool out_split = false, int orow_div = 0, Blocking blk = Blocking());
+           bool out_split = false, int orow_div = 0, Blocking blk = Blocking(), int oscale = -1);
   void upload_tables(hipStream_t st);
   void launch(hipStream_t st, int64_t J0 = 0) const;
-  void launch_split(int precision, hipStream_t st, int64_t J0 = 0) const;
+  // scales: device array the tasks' oscale index into (may be null when unused)
+  void launch_split(int precision, hipStream_t st, int64_t J0 = 0, const float* scales = nullptr) const;
 };
 
 void launch_ggemm(int tile, const Task* tasks, const Seg* segs, int64_t ntasks, int64_t J0, hipStream_t st);
 void launch_dgemm(int tile, const Task* tasks, const Seg* segs, int64_t ntasks, int64_t J0, hipStream_t st);
 void launch_xgemm(int precision, int tile, const Task* tasks, const Seg* segs, int64_t ntasks, int64_t J0,
-                  hipStream_t st);
+                  const float* scales, hipStream_t st);
 
 // ---------------------------------------------------------------------------
 // Split-row storage of a matrix for the split-precision GEMM.  A row of K
@@ -169,17 +173,34 @@ void launch_xgemm(int precision, int tile, const Task* tasks, const Seg* segs, i
 // chunk is 32-byte aligned when the matrix base is.
 inline int64_t xpad(int64_t K) { return (K + 7) & ~(int64_t)7; }
 __device__ __forceinline__ int64_t xidx(int64_t k) { return ((k >> 3) << 4) + (k & 7); }
-// write the split form of v[0..7] as one chunk
-__device__ __forceinline__ void put_split8(uint16_t* chunk, const float* v) {
-  bf16x8 h, l;
+// write the split form of s v[0..7] as one chunk: BF16 parts, or FP16 parts
+// when f16 (scaled FP16: s is a power of two that keeps |s v| below 2^14)
+__device__ __forceinline__ void put_split8(uint16_t* chunk, const float* v, float s = 1.f, bool f16 = false) {
+  if (f16) {
+    h8 h, l;
 #pragma unroll
-  for (int e = 0; e < 8; ++e) {
-    h[e] = (__bf16)v[e];
-    l[e] = (__bf16)(v[e] - (float)h[e]);
+    for (int e = 0; e < 8; ++e) {
+      const float x = v[e] * s;
+      h[e] = (_Float16)x;
+      l[e] = (_Float16)(x - (float)h[e]);
+    }
+    __attribute__((address_space(1))) h8* p = (__attribute__((address_space(1))) h8*)chunk;
+    p[0] = h;
+    p[1] = l;
+  } else {
+    bf16x8 h, l;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      const float x = v[e] * s;
+      h[e] = (__bf16)x;
+      l[e] = (__bf16)(x - (float)h[e]);
+    }
+    __attribute__((address_space(1))) bf16x8* p = (__attribute__((address_space(1))) bf16x8*)chunk;
+    p[0] = h;
+    p[1] = l;
   }
-  __attribute__((address_space(1))) bf16x8* p = (__attribute__((address_space(1))) bf16x8*)chunk;
-  p[0] = h;
-  p[1] = l;
 }
+// scale read from device memory (null: 1)
+__device__ __forceinline__ float scale_at(const float* p) { return p ? *p : 1.f; }
 
 }  // namespace tci

@@ -1,9 +1,12 @@
-// Dense grouped GEMM in split precision on BF16 MFMA:
-//   C[tile] (+)= sum_seg A_seg B_seg^T,   A_seg: M x K,  B_seg: N x K,
-// both operands in split-row storage (hi/lo bf16 chunks, common.hpp), so
-// A B^T ~= Ahi Bhi^T + Ahi Blo^T + Alo Bhi^T with FP32 accumulation
-// (v_mfma_f32_16x16x32_bf16).  hi + lo keeps ~17 significant bits of each
-// operand; the dropped lo lo^T term is of the same relative size (2^-17).
+// Dense grouped GEMM in split precision on 16-bit MFMA:
+//   C[tile] (+)= s sum_seg A_seg B_seg^T,   A_seg: M x K,  B_seg: N x K,
+// both operands in split-row storage (hi/lo chunks, common.hpp), so
+// A B^T ~= Ahi Bhi^T + Ahi Blo^T + Alo Bhi^T with FP32 accumulation.
+// bf16x3 (v_mfma_f32_16x16x32_bf16): hi + lo keeps ~17 significant bits of
+// each operand and the dropped lo lo^T term is of the same relative size.
+// Scaled fp16x3 (v_mfma_f32_16x16x32_f16): ~22 bits; operands carry
+// power-of-two scales chosen by their producers, and the task's output scale s
+// (scales[oscale]) undoes them exactly.
 //
 // Because both operands are k-contiguous and pre-split, staging is a pure
 // copy: per 32-wide k-step a thread moves 32-byte chunks (hi 8 + lo 8) from
@@ -34,6 +37,21 @@ namespace {
 
 constexpr int XK = 32;
 typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+
+template <int PREC>
+struct XT;
+template <>
+struct XT<TCI_SIGMA_BF16X3> {
+  typedef __bf16 T;
+  typedef bf16x8 V;
+  static __device__ __forceinline__ f4 mfma(V a, V b, f4 c) { return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0); }
+};
+template <>
+struct XT<TCI_SIGMA_FP16X3> {
+  typedef _Float16 T;
+  typedef h8 V;
+  static __device__ __forceinline__ f4 mfma(V a, V b, f4 c) { return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0); }
+};
 
 // element offset of k-chunk ch (8 values) of LDS row `row`
 __device__ __forceinline__ int xsw(int row, int ch) { return row * XK + (((ch ^ (row >> 1)) & 3) << 3); }
@@ -88,9 +106,12 @@ __device__ __forceinline__ void xstage(const Seg& S, const Task& t, int64_t J0, 
   xload<TS>(S.B, J0, t.N, t.n0, kb, S.K, tid, vb);
 }
 
-template <int TS, int NBM, int NBN>
+template <int TS, int PREC, int NBM, int NBN>
 __device__ __forceinline__ void xgemm_body(const Task& t, const Seg* __restrict__ segs, int64_t J0, int seg0,
-                                           uint16_t* sm, int tid, int wm, int wn, int li, int lg) {
+                                           const float* __restrict__ scales, uint16_t* sm, int tid, int wm, int wn,
+                                           int li, int lg) {
+  typedef typename XT<PREC>::T T16;
+  typedef typename XT<PREC>::V V;
   constexpr int NL = TS / 64, PL = TS * XK, STG = 4 * PL;
   f4 acc[NBM > 0 ? NBM : 1][NBN > 0 ? NBN : 1];
 #pragma unroll
@@ -109,18 +130,18 @@ __device__ __forceinline__ void xgemm_body(const Task& t, const Seg* __restrict_
     if (more) xstage<TS>(segs[seg], t, J0, kb, tid, va, vb);
     if (NBM > 0 && NBN > 0) {
       const uint16_t* base = sm + buf * STG;
-      bf16x8 ah[NBM > 0 ? NBM : 1], al[NBM > 0 ? NBM : 1], bh[NBN > 0 ? NBN : 1], bl[NBN > 0 ? NBN : 1];
+      V ah[NBM > 0 ? NBM : 1], al[NBM > 0 ? NBM : 1], bh[NBN > 0 ? NBN : 1], bl[NBN > 0 ? NBN : 1];
 #pragma unroll
       for (int i = 0; i < NBM; ++i) {
         const int e = xsw(wm + i * 16 + li, lg);
-        ah[i] = *reinterpret_cast<const bf16x8*>(base + e);
-        al[i] = *reinterpret_cast<const bf16x8*>(base + PL + e);
+        ah[i] = *reinterpret_cast<const V*>(base + e);
+        al[i] = *reinterpret_cast<const V*>(base + PL + e);
       }
 #pragma unroll
       for (int j = 0; j < NBN; ++j) {
         const int e = xsw(wn + j * 16 + li, lg);
-        bh[j] = *reinterpret_cast<const bf16x8*>(base + 2 * PL + e);
-        bl[j] = *reinterpret_cast<const bf16x8*>(base + 3 * PL + e);
+        bh[j] = *reinterpret_cast<const V*>(base + 2 * PL + e);
+        bl[j] = *reinterpret_cast<const V*>(base + 3 * PL + e);
       }
       // B fragment as the MFMA's first operand: acc[i][j] holds the transposed
       // 16 x 16 block, so each lane ends with 4 consecutive output columns.
@@ -128,15 +149,15 @@ __device__ __forceinline__ void xgemm_body(const Task& t, const Seg* __restrict_
 #pragma unroll
       for (int i = 0; i < NBM; ++i)
 #pragma unroll
-        for (int j = 0; j < NBN; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh[j], al[i], acc[i][j], 0, 0, 0);
+        for (int j = 0; j < NBN; ++j) acc[i][j] = XT<PREC>::mfma(bh[j], al[i], acc[i][j]);
 #pragma unroll
       for (int i = 0; i < NBM; ++i)
 #pragma unroll
-        for (int j = 0; j < NBN; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bl[j], ah[i], acc[i][j], 0, 0, 0);
+        for (int j = 0; j < NBN; ++j) acc[i][j] = XT<PREC>::mfma(bl[j], ah[i], acc[i][j]);
 #pragma unroll
       for (int i = 0; i < NBM; ++i)
 #pragma unroll
-        for (int j = 0; j < NBN; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh[j], ah[i], acc[i][j], 0, 0, 0);
+        for (int j = 0; j < NBN; ++j) acc[i][j] = XT<PREC>::mfma(bh[j], ah[i], acc[i][j]);
     }
     if (!more) break;
     uint16_t* nb = sm + (buf ^ 1) * STG;
@@ -147,6 +168,7 @@ __device__ __forceinline__ void xgemm_body(const Task& t, const Seg* __restrict_
     more = xadvance(segs, t.seg1, seg, kb);
   }
   // lane (li, lg) of block (i, j) holds output row 16i + li, columns 16j + 4lg .. +3
+  const float osc = t.oscale >= 0 ? scales[t.oscale] : 1.f;
   int64_t roff[NBM > 0 ? NBM : 1];
 #pragma unroll
   for (int i = 0; i < NBM; ++i) {
@@ -167,15 +189,15 @@ __device__ __forceinline__ void xgemm_body(const Task& t, const Seg* __restrict_
     for (int i = 0; i < NBM; ++i) {
       const int row = t.m0 + wm + i * 16 + li;
       if (row >= t.M) continue;
-      const f4 v = acc[i][j];
+      const f4 v = acc[i][j] * osc;
       if (whole) {
         if (t.out_split) {
-          typedef __bf16 b4 __attribute__((ext_vector_type(4)));
+          typedef T16 b4 __attribute__((ext_vector_type(4)));
           b4 h, l;
 #pragma unroll
           for (int r = 0; r < 4; ++r) {
-            h[r] = (__bf16)v[r];
-            l[r] = (__bf16)(v[r] - (float)h[r]);
+            h[r] = (T16)v[r];
+            l[r] = (T16)(v[r] - (float)h[r]);
           }
           __attribute__((address_space(1))) b4* p = (__attribute__((address_space(1))) b4*)(
               reinterpret_cast<uint16_t*>(t.C) + roff[i] + coff + xidx(cc));
@@ -196,11 +218,11 @@ __device__ __forceinline__ void xgemm_body(const Task& t, const Seg* __restrict_
           const int cbr = t.cdiv ? cr / t.cdiv : 0, ccr = cr - cbr * t.cdiv;
           const int64_t o = roff[i] + (int64_t)cbr * t.cstride;
           if (t.out_split) {
-            __attribute__((address_space(1))) __bf16* p =
-                (__attribute__((address_space(1))) __bf16*)(reinterpret_cast<uint16_t*>(t.C) + o + xidx(ccr));
-            const __bf16 h = (__bf16)v[r];
+            __attribute__((address_space(1))) T16* p =
+                (__attribute__((address_space(1))) T16*)(reinterpret_cast<uint16_t*>(t.C) + o + xidx(ccr));
+            const T16 h = (T16)v[r];
             p[0] = h;
-            p[8] = (__bf16)(v[r] - (float)h);
+            p[8] = (T16)(v[r] - (float)h);
           } else {
             __attribute__((address_space(1))) double* p = (__attribute__((address_space(1))) double*)(t.C + o + ccr);
             *p = t.accumulate ? *p + (double)v[r] : (double)v[r];
@@ -211,9 +233,10 @@ __device__ __forceinline__ void xgemm_body(const Task& t, const Seg* __restrict_
   }
 }
 
-template <int TS>
+template <int TS, int PREC>
 __global__ __launch_bounds__(256, TS == 128 ? 2 : 4) void xgemm_kernel(const Task* __restrict__ tasks,
-                                                                       const Seg* __restrict__ segs, int64_t J0) {
+                                                                       const Seg* __restrict__ segs, int64_t J0,
+                                                                       const float* __restrict__ scales) {
   constexpr int WB = TS / 32;  // 16x16 blocks per wave side
   const Task t = tasks[blockIdx.x];
   __shared__ __attribute__((aligned(16))) uint16_t sm[2 * 4 * TS * XK];
@@ -246,7 +269,7 @@ __global__ __launch_bounds__(256, TS == 128 ? 2 : 4) void xgemm_kernel(const Tas
   }
 #define TCI_XB(M_, N_) \
   case M_ * 5 + N_:    \
-    if constexpr (M_ <= WB && N_ <= WB) xgemm_body<TS, M_, N_>(t, segs, J0, seg0, sm, tid, wm, wn, li, lg); \
+    if constexpr (M_ <= WB && N_ <= WB) xgemm_body<TS, PREC, M_, N_>(t, segs, J0, seg0, scales, sm, tid, wm, wn, li, lg); \
     break;
   switch (nbm * 5 + nbn) {
     TCI_XB(0, 0) TCI_XB(0, 1) TCI_XB(0, 2) TCI_XB(0, 3) TCI_XB(0, 4)
@@ -262,18 +285,25 @@ __global__ __launch_bounds__(256, TS == 128 ? 2 : 4) void xgemm_kernel(const Tas
 }  // namespace
 
 void launch_xgemm(int precision, int tile, const Task* tasks, const Seg* segs, int64_t ntasks, int64_t J0,
-                  hipStream_t st) {
-  TCI_REQUIRE(precision == TCI_SIGMA_BF16X3, "the split GEMM implements bf16x3 only");
+                  const float* scales, hipStream_t st) {
+  TCI_REQUIRE(precision == TCI_SIGMA_BF16X3 || precision == TCI_SIGMA_FP16X3, "split GEMM needs a split precision");
   if (ntasks <= 0) return;
   TCI_REQUIRE(ntasks < ((int64_t)1 << 31), "too many gemm tasks");
-  if (tile == 128) hipLaunchKernelGGL(xgemm_kernel<128>, dim3((unsigned)ntasks), dim3(256), 0, st, tasks, segs, J0);
-  else hipLaunchKernelGGL(xgemm_kernel<64>, dim3((unsigned)ntasks), dim3(256), 0, st, tasks, segs, J0);
+  const dim3 g((unsigned)ntasks), b(256);
+  if (precision == TCI_SIGMA_BF16X3) {
+    if (tile == 128) hipLaunchKernelGGL((xgemm_kernel<128, TCI_SIGMA_BF16X3>), g, b, 0, st, tasks, segs, J0, scales);
+    else hipLaunchKernelGGL((xgemm_kernel<64, TCI_SIGMA_BF16X3>), g, b, 0, st, tasks, segs, J0, scales);
+  } else {
+    if (tile == 128) hipLaunchKernelGGL((xgemm_kernel<128, TCI_SIGMA_FP16X3>), g, b, 0, st, tasks, segs, J0, scales);
+    else hipLaunchKernelGGL((xgemm_kernel<64, TCI_SIGMA_FP16X3>), g, b, 0, st, tasks, segs, J0, scales);
+  }
   TCI_HIP(hipGetLastError());
 }
 
-void GemmProgram::launch_split(int precision, hipStream_t st, int64_t J0) const {
+void GemmProgram::launch_split(int precision, hipStream_t st, int64_t J0, const float* scales) const {
   TCI_REQUIRE(split, "not a split program");
-  launch_xgemm(precision, tile, d_tasks.as<Task>(), d_segs.as<Seg>(), (int64_t)tasks.size(), J0, st);
+  TCI_REQUIRE(!scaled || scales, "scaled split tasks need a scales array");
+  launch_xgemm(precision, tile, d_tasks.as<Task>(), d_segs.as<Seg>(), (int64_t)tasks.size(), J0, scales, st);
 }
 
 }  // namespace tci

@@ -381,10 +381,11 @@ void launch_ggemm(int tile, const Task* tasks, const Seg* segs, int64_t ntasks, 
 }
 
 void GemmProgram::add(double* C, int64_t ldc, int M, int N, const std::vector<Seg>& s, bool accumulate,
-                      bool out_split, int orow_div, Blocking blk) {
+                      bool out_split, int orow_div, Blocking blk, int oscale) {
   if (M <= 0 || N <= 0) return;
   TCI_REQUIRE((blk.rdiv == 0 && blk.cdiv == 0) || ((dense || split) && orow_div == 0 && blk.rdiv >= 0 && blk.cdiv >= 0),
               "blocked output needs a dense or split program");
+  TCI_REQUIRE(oscale < 0 || split, "output scales belong to split programs");
   TCI_REQUIRE(!out_split || (split && !accumulate), "split-row output needs a split program and no accumulation");
   if (split)
     for (const Seg& x : s)
@@ -420,11 +421,15 @@ void GemmProgram::add(double* C, int64_t ldc, int M, int N, const std::vector<Se
       t.cdiv = blk.cdiv;
       t.rstride = blk.rstride;
       t.cstride = blk.cstride;
+      t.oscale = oscale;
+      t.pad_ = 0;
       tasks.push_back(t);
     }
 }
 
 void GemmProgram::upload_tables(hipStream_t st) {
+  scaled = false;
+  for (const Task& t : tasks) scaled = scaled || t.oscale >= 0;
   upload(d_tasks, tasks, st);
   upload(d_segs, segs, st);
 }

@@ -1,4 +1,5 @@
-// Split-precision (bf16x3) two-phase ladder  R_P = sum_J L^J[D,D] C_P L^J[D,D].
+// Split-precision (bf16x3 / scaled fp16x3) two-phase ladder
+//   R_P = sum_J L^J[D,D] C_P L^J[D,D].
 //
 // Both phases are C = A B^T products with k-contiguous split-row operands
 // (dense_split.hip).  With T' = L C^T per Cholesky vector,
@@ -20,7 +21,8 @@ namespace {
 
 // Cx[P] rows r of C_P (d x d, FP64) in split-row form, row stride 2 xpad(d)
 __global__ void split_blocks_kernel(PairTab T, const int64_t* __restrict__ cx_off, const double* __restrict__ C,
-                                    uint16_t* __restrict__ Cx) {
+                                    uint16_t* __restrict__ Cx, const float* scl, int f16) {
+  const float sc = scale_at(scl);
   const int P = blockIdx.y;
   const int d = (int)(T.dom_off[P + 1] - T.dom_off[P]);
   const int nch = (d + 7) >> 3;
@@ -34,13 +36,13 @@ __global__ void split_blocks_kernel(PairTab T, const int64_t* __restrict__ cx_of
       const int k = 8 * ch + t;
       v[t] = k < d ? (float)src[(int64_t)r * d + k] : 0.f;
     }
-    put_split8(dst + (int64_t)r * 16 * nch + 16 * ch, v);
+    put_split8(dst + (int64_t)r * 16 * nch + 16 * ch, v, sc, f16);
   }
 }
 
 // Cx[P] rows c = split column c of C_P (row c of C_P^T), 32 x 32 tiles via LDS
 __global__ void split_blocks_T_kernel(PairTab T, const int64_t* __restrict__ cx_off, const double* __restrict__ C,
-                                      uint16_t* __restrict__ Cx) {
+                                      uint16_t* __restrict__ Cx, const float* scl, int f16) {
   __shared__ float tile[32][33];
   const int P = blockIdx.y;
   const int d = (int)(T.dom_off[P + 1] - T.dom_off[P]);
@@ -62,14 +64,16 @@ __global__ void split_blocks_T_kernel(PairTab T, const int64_t* __restrict__ cx_
       float v[8];
 #pragma unroll
       for (int t = 0; t < 8; ++t) v[t] = tile[8 * ch + t][cy];
-      put_split8(Cx + cx_off[P] + (int64_t)c * 16 * nch + 2 * k, v);
+      put_split8(Cx + cx_off[P] + (int64_t)c * 16 * nch + 2 * k, v, scale_at(scl), f16);
     }
   }
 }
 
 // Lx[(J, a)] = split row a of L^J[v][v] for all J, row stride 2 xpad(n_v)
 template <class LT>
-__global__ void split_lvv_kernel(const LT* __restrict__ Lvv, int64_t rows, int n_v, uint16_t* __restrict__ Lx) {
+__global__ void split_lvv_kernel(const LT* __restrict__ Lvv, int64_t rows, int n_v, uint16_t* __restrict__ Lx,
+                                 const float* scl, int f16) {
+  const float sc = scale_at(scl);
   const int nch = (n_v + 7) >> 3;
   const int64_t n = rows * nch;
   for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x) {
@@ -81,7 +85,7 @@ __global__ void split_lvv_kernel(const LT* __restrict__ Lvv, int64_t rows, int n
       const int k = 8 * ch + t;
       v[t] = k < n_v ? (float)Lvv[r * n_v + k] : 0.f;
     }
-    put_split8(Lx + r * 16 * nch + 16 * ch, v);
+    put_split8(Lx + r * 16 * nch + 16 * ch, v, sc, f16);
   }
 }
 
@@ -89,7 +93,8 @@ __global__ void split_lvv_kernel(const LT* __restrict__ Lvv, int64_t rows, int n
 template <class LT>
 __global__ void pack_L_split_kernel(PairTab T, const int32_t* __restrict__ pairs_local,
                                     const int64_t* __restrict__ lp_off, const LT* __restrict__ Lvv, int64_t J0,
-                                    int len, uint16_t* __restrict__ Lpx) {
+                                    int len, uint16_t* __restrict__ Lpx, const float* scl, int f16) {
+  const float sc = scale_at(scl);
   const int q = blockIdx.y;
   const int P = pairs_local[q];
   const int64_t off = T.dom_off[P];
@@ -110,7 +115,53 @@ __global__ void pack_L_split_kernel(PairTab T, const int32_t* __restrict__ pairs
       const int k = 8 * ch + t;
       v[t] = k < d ? (float)L[T.dom_idx[off + min(k, d - 1)]] : 0.f;
     }
-    put_split8(out + row * 16 * nch + 16 * ch, v);
+    put_split8(out + row * 16 * nch + 16 * ch, v, sc, f16);
+  }
+}
+
+// out = max(out, max |x|) as float bits (non-negative floats order like their bits)
+template <class T>
+__global__ void absmax_kernel(const T* __restrict__ x, int64_t n, unsigned* __restrict__ out) {
+  __shared__ float part[4];
+  float m = 0.f;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x)
+    m = fmaxf(m, fabsf((float)x[e]));
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_down(m, off, 64));
+  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    m = fmaxf(fmaxf(part[0], part[1]), fmaxf(part[2], part[3]));
+    atomicMax(out, __float_as_uint(m));
+  }
+}
+
+__global__ void zero_u32_kernel(unsigned* p, int n) {
+  if ((int)threadIdx.x < n) p[threadIdx.x] = 0u;
+}
+
+// power of two s with m s in [2^13, 2^14): FP16 parts keep 22 bits with room
+__device__ __forceinline__ float p2scale(float m) { return m > 0.f ? ldexpf(1.f, 13 - ilogbf(m)) : 1.f; }
+
+// Scales of the scaled-FP16 split (indices documented at MixedLadder::scales).
+//   mode 0: constants from mx[2]     mode 1: ladder from mx[0] = max|C|
+//   mode 2: pair products from mx[0], mx[1] = max|Yt|
+__global__ void set_scales_kernel(int mode, const unsigned* __restrict__ mx, float* __restrict__ sc, float dmax) {
+  if (threadIdx.x != 0) return;
+  if (mode == 0) {
+    const float mi = __uint_as_float(mx[2]);
+    sc[0] = p2scale(mi);
+    sc[8] = mi;
+  } else if (mode == 1) {
+    const float mc = __uint_as_float(mx[0]);
+    sc[1] = p2scale(mc);
+    sc[2] = p2scale(dmax * sc[8] * mc);  // |T| <= d max|L| max|C|
+    sc[3] = sc[2] / (sc[0] * sc[1]);
+    sc[4] = 1.f / (sc[2] * sc[0]);
+  } else {
+    const float mv = fmaxf(3.f * __uint_as_float(mx[0]), __uint_as_float(mx[1]));  // |Ct| <= 3 max|C|
+    sc[6] = p2scale(mv);
+    sc[5] = 1.f / (sc[0] * sc[6]);
   }
 }
 
@@ -141,9 +192,51 @@ Seg xseg(const Op& A, const Op& B, int K) {
 
 }  // namespace
 
-void mixed_prepare(tci_plan& P, hipStream_t st) {
+void launch_split_scales(int mode, const double* x, int64_t n, unsigned* mx, int slot, float* sc, float dmax,
+                         hipStream_t st) {
+  hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(64), 0, st, mx + slot, 1);
+  if (n > 0) hipLaunchKernelGGL(absmax_kernel<double>, dim3(blocks_for(n, 8192)), dim3(256), 0, st, x, n, mx + slot);
+  hipLaunchKernelGGL(set_scales_kernel, dim3(1), dim3(64), 0, st, mode, mx, sc, dmax);
+  TCI_HIP(hipGetLastError());
+}
+
+void mixed_prepare(tci_plan& P, int precision, hipStream_t st) {
   tci_plan::MixedLadder& M = P.mx;
-  if (M.ready) return;
+  TCI_REQUIRE(precision == TCI_SIGMA_BF16X3 || precision == TCI_SIGMA_FP16X3, "not a split precision");
+  if (M.ready && M.prec == precision) return;
+  M.ready = false;
+  M.prec = precision;
+  const int f16 = precision == TCI_SIGMA_FP16X3 ? 1 : 0;
+  // scales: all 1 for bf16x3 (FP32 exponent range); for fp16x3 the constants'
+  // scale now, the per-sigma ones inside every sigma
+  {
+    std::vector<float> one(16, 1.f);
+    upload(M.scales, one, st);
+    std::vector<unsigned> z(4, 0u);
+    upload(M.maxbits, z, st);
+  }
+  float* sc = M.scales.as<float>();
+  if (f16) {
+    unsigned* mx = M.maxbits.as<unsigned>();
+    const int64_t nvv = P.n_chol * (int64_t)P.n_v * P.n_v, nov = P.n_chol * (int64_t)P.n_int * P.n_v;
+    const int64_t niv = (int64_t)P.n_int * P.n_v;
+    if (P.lvv_f32)
+      hipLaunchKernelGGL(absmax_kernel<float>, dim3(blocks_for(nvv, 8192)), dim3(256), 0, st, P.Lvv.as<float>(), nvv,
+                         mx + 2);
+    else
+      hipLaunchKernelGGL(absmax_kernel<double>, dim3(blocks_for(nvv, 8192)), dim3(256), 0, st, P.Lvv.as<double>(), nvv,
+                         mx + 2);
+    if (P.dense_pairs) {
+      hipLaunchKernelGGL(absmax_kernel<double>, dim3(blocks_for(niv * niv, 8192)), dim3(256), 0, st,
+                         P.Kov.as<double>(), niv * niv, mx + 2);
+      hipLaunchKernelGGL(absmax_kernel<double>, dim3(blocks_for(niv * niv, 8192)), dim3(256), 0, st,
+                         P.Jbig.as<double>(), niv * niv, mx + 2);
+      hipLaunchKernelGGL(absmax_kernel<double>, dim3(blocks_for(nov, 8192)), dim3(256), 0, st, P.Lov.as<double>(),
+                         nov, mx + 2);
+    }
+    hipLaunchKernelGGL(set_scales_kernel, dim3(1), dim3(64), 0, st, 0, mx, sc, (float)P.dmax);
+    TCI_HIP(hipGetLastError());
+  }
   const int n_pairs = P.n_pairs, n_v = P.n_v;
   const int64_t nvp = xpad(n_v), n_chol = P.n_chol;
   std::vector<int64_t> cxo(n_pairs + 1, 0);
@@ -173,10 +266,10 @@ void mixed_prepare(tci_plan& P, hipStream_t st) {
     M.Lx.alloc((size_t)rows * 2 * nvp * 2);
     if (P.lvv_f32)
       hipLaunchKernelGGL(split_lvv_kernel<float>, dim3(blocks_for(rows * (nvp / 8), 65536)), dim3(256), 0, st,
-                         static_cast<const float*>(P.Lvv.p), rows, n_v, M.Lx.as<uint16_t>());
+                         static_cast<const float*>(P.Lvv.p), rows, n_v, M.Lx.as<uint16_t>(), sc, f16);
     else
       hipLaunchKernelGGL(split_lvv_kernel<double>, dim3(blocks_for(rows * (nvp / 8), 65536)), dim3(256), 0, st,
-                         static_cast<const double*>(P.Lvv.p), rows, n_v, M.Lx.as<uint16_t>());
+                         static_cast<const double*>(P.Lvv.p), rows, n_v, M.Lx.as<uint16_t>(), sc, f16);
     TCI_HIP(hipGetLastError());
     M.bytes += (int64_t)M.Lx.bytes;
   }
@@ -195,11 +288,11 @@ void mixed_prepare(tci_plan& P, hipStream_t st) {
     if (P.lvv_f32)
       hipLaunchKernelGGL(pack_L_split_kernel<float>, dim3(4096, (unsigned)loc.size()), dim3(256), 0, st, P.tab,
                          M.pairs_local.as<int32_t>(), M.lp_off_all.as<int64_t>(), static_cast<const float*>(P.Lvv.p),
-                         (int64_t)0, (int)n_chol, M.Lpx.as<uint16_t>());
+                         (int64_t)0, (int)n_chol, M.Lpx.as<uint16_t>(), sc, f16);
     else
       hipLaunchKernelGGL(pack_L_split_kernel<double>, dim3(4096, (unsigned)loc.size()), dim3(256), 0, st, P.tab,
                          M.pairs_local.as<int32_t>(), M.lp_off_all.as<int64_t>(), static_cast<const double*>(P.Lvv.p),
-                         (int64_t)0, (int)n_chol, M.Lpx.as<uint16_t>());
+                         (int64_t)0, (int)n_chol, M.Lpx.as<uint16_t>(), sc, f16);
     TCI_HIP(hipGetLastError());
   }
   // chunk length: T' (all pairs) + packed L (local pairs, unless prepacked), 2 bytes per element
@@ -282,14 +375,14 @@ void mixed_prepare(tci_plan& P, hipStream_t st) {
       bk.cdiv = d;
       bk.cstride = (int64_t)d * ldx;
       ph.p1.add(reinterpret_cast<double*>(Tx), ldx, (int)(len * d), NP,
-                {xseg(xop(Lx, ldx, jstride_full), xop(Cx, ldx), d)}, false, true, 0, bk);
+                {xseg(xop(Lx, ldx, jstride_full), xop(Cx, ldx), d)}, false, true, 0, bk, 3);
       for (int sp = 0; sp < ph.nsplit; ++sp) {
         const int64_t j0 = len * sp / ph.nsplit, j1 = len * (sp + 1) / ph.nsplit;
         std::vector<Seg> sg;
         for (int64_t jj = j0; jj < j1; ++jj)
           sg.push_back(xseg(xop(Tx + jj * Sjj, ldx), xop(Lx + jj * jstride_full, ldx, jstride_full), d));
         double* Rout = M.rpart_n > 1 ? M.Rpart.as<double>() + (int64_t)sp * P.sum_d2 : P.R.as<double>();
-        ph.p2.add(Rout, d, NP, d, sg, true);
+        ph.p2.add(Rout, d, NP, d, sg, true, false, 0, Blocking(), 4);
       }
       ph.p1.upload_tables(st);
       ph.p2.upload_tables(st);
@@ -319,7 +412,7 @@ void mixed_prepare(tci_plan& P, hipStream_t st) {
       }
       uint16_t* Tq = Tx + toff;
       ph.p1.add(reinterpret_cast<double*>(Tq), ldx, (int)(len * d), d,
-                {xseg(xop(Lb, ldx, jstride), xop(Cx + cxo[Q], ldx), d)}, false, true);
+                {xseg(xop(Lb, ldx, jstride), xop(Cx + cxo[Q], ldx), d)}, false, true, 0, Blocking(), 3);
       for (int sp = 0; sp < ph.nsplit; ++sp) {
         const int64_t j0 = len * sp / ph.nsplit, j1 = len * (sp + 1) / ph.nsplit;
         std::vector<Seg> sg;
@@ -327,7 +420,7 @@ void mixed_prepare(tci_plan& P, hipStream_t st) {
           sg.push_back(xseg(xop(Lb + jj * jstep, ldx, jstride), xop(Tq + jj * d * ldx, ldx), d));
         double* Rout = (M.rpart_n > 1 ? M.Rpart.as<double>() + (int64_t)sp * P.sum_d2 : P.R.as<double>()) +
                        P.h_cd_off[Q];
-        ph.p2.add(Rout, d, d, d, sg, true);
+        ph.p2.add(Rout, d, d, d, sg, true, false, 0, Blocking(), 4);
       }
       toff += len * d * ldx;
     }
@@ -340,16 +433,26 @@ void mixed_prepare(tci_plan& P, hipStream_t st) {
 }
 
 void mixed_ladder(tci_plan& P, int precision, hipStream_t st) {
-  TCI_REQUIRE(precision == TCI_SIGMA_BF16X3, "split-precision sigma implements bf16x3 only");
-  mixed_prepare(P, st);
+  mixed_prepare(P, precision, st);
   tci_plan::MixedLadder& M = P.mx;
+  const int f16 = precision == TCI_SIGMA_FP16X3 ? 1 : 0;
+  float* sc = M.scales.as<float>();
+  if (f16) {  // scale of C and of the intermediate from max|C|
+    unsigned* mx = M.maxbits.as<unsigned>();
+    hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(64), 0, st, mx, 2);
+    hipLaunchKernelGGL(absmax_kernel<double>, dim3(blocks_for(P.sum_d2, 8192)), dim3(256), 0, st, P.C.as<double>(),
+                       P.sum_d2, mx);
+    hipLaunchKernelGGL(set_scales_kernel, dim3(1), dim3(64), 0, st, 1, mx, sc, (float)P.dmax);
+    TCI_HIP(hipGetLastError());
+  }
   if (M.merged) {
     const int nt = (P.dmax + 31) / 32;
     hipLaunchKernelGGL(split_blocks_T_kernel, dim3(nt * nt, P.n_pairs), dim3(256), 0, st, P.tab,
-                       M.cx_off.as<int64_t>(), P.C.as<double>(), M.Cx.as<uint16_t>());
+                       M.cx_off.as<int64_t>(), P.C.as<double>(), M.Cx.as<uint16_t>(), sc + 1, f16);
   } else {
     hipLaunchKernelGGL(split_blocks_kernel, dim3(blocks_for((int64_t)P.dmax * ((P.dmax + 7) / 8), 256), P.n_pairs),
-                       dim3(256), 0, st, P.tab, M.cx_off.as<int64_t>(), P.C.as<double>(), M.Cx.as<uint16_t>());
+                       dim3(256), 0, st, P.tab, M.cx_off.as<int64_t>(), P.C.as<double>(), M.Cx.as<uint16_t>(), sc + 1,
+                       f16);
   }
   TCI_HIP(hipGetLastError());
   double* Rz = M.rpart_n > 1 ? M.Rpart.as<double>() : P.R.as<double>();
@@ -361,15 +464,15 @@ void mixed_ladder(tci_plan& P, int precision, hipStream_t st) {
       if (P.lvv_f32)
         hipLaunchKernelGGL(pack_L_split_kernel<float>, dim3(gxb, M.n_local), dim3(256), 0, st, P.tab,
                            M.pairs_local.as<int32_t>(), ph.lp_off.as<int64_t>(), static_cast<const float*>(P.Lvv.p),
-                           ch.first, (int)ph.Lc, M.Lpx.as<uint16_t>());
+                           ch.first, (int)ph.Lc, M.Lpx.as<uint16_t>(), sc, f16);
       else
         hipLaunchKernelGGL(pack_L_split_kernel<double>, dim3(gxb, M.n_local), dim3(256), 0, st, P.tab,
                            M.pairs_local.as<int32_t>(), ph.lp_off.as<int64_t>(), static_cast<const double*>(P.Lvv.p),
-                           ch.first, (int)ph.Lc, M.Lpx.as<uint16_t>());
+                           ch.first, (int)ph.Lc, M.Lpx.as<uint16_t>(), sc, f16);
       TCI_HIP(hipGetLastError());
     }
-    ph.p1.launch_split(precision, st, ch.first);
-    ph.p2.launch_split(precision, st, ch.first);
+    ph.p1.launch_split(precision, st, ch.first, sc);
+    ph.p2.launch_split(precision, st, ch.first, sc);
   }
   if (P.profiling) TCI_HIP(hipEventRecord(P.ev[3], st));
   if (M.rpart_n > 1) ladder_reduce(M.Rpart.as<double>(), P.sum_d2, M.rpart_n, P.R.as<double>(), P.sum_d2, st);

@@ -61,7 +61,7 @@ __global__ void jbig_kernel(const double* __restrict__ Jvv, int n_int, int n_v, 
 // split rows: out16[(p n_v + r) ldo + xidx(q vp + c)], vp = n_v rounded up to 8.
 template <bool SPLIT>
 __global__ void relayout_kernel(PairTab T, EmbedTab E, const double* __restrict__ src, bool cc, double sign,
-                                void* __restrict__ out, int64_t ldo) {
+                                void* __restrict__ out, int64_t ldo, const float* scl, int f16) {
   __shared__ double tile[32][33];
   const int nv = T.n_v, nt = (nv + 31) >> 5;
   const int q = blockIdx.x / nt, ct = blockIdx.x - q * nt;
@@ -103,7 +103,8 @@ __global__ void relayout_kernel(PairTab T, EmbedTab E, const double* __restrict_
       float v[8];
 #pragma unroll
       for (int t = 0; t < 8; ++t) v[t] = (c + t < nv) ? (float)(sign * tile[yy][c - c0 + t]) : 0.f;
-      put_split8(static_cast<uint16_t*>(out) + ((int64_t)p * nv + r) * ldo + xidx((int64_t)q * vp + c), v);
+      put_split8(static_cast<uint16_t*>(out) + ((int64_t)p * nv + r) * ldo + xidx((int64_t)q * vp + c), v,
+                 scale_at(scl), f16);
     }
   }
 }
@@ -111,7 +112,8 @@ __global__ void relayout_kernel(PairTab T, EmbedTab E, const double* __restrict_
 // split rows of an FP64 matrix whose columns are nb blocks of d, each block's
 // k padded to dp: out16[row ldo + xidx(q dp + c)] = src[row ld + q d + c]
 __global__ void split_blocked_kernel(const double* __restrict__ src, int64_t rows, int64_t ld, int nb, int d,
-                                     uint16_t* __restrict__ out, int64_t ldo) {
+                                     uint16_t* __restrict__ out, int64_t ldo, const float* scl, int f16) {
+  const float sc = scale_at(scl);
   const int dp = (d + 7) & ~7, nch = dp >> 3;
   const int64_t per_row = (int64_t)nb * nch;
   for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < rows * per_row;
@@ -122,14 +124,14 @@ __global__ void split_blocked_kernel(const double* __restrict__ src, int64_t row
     float v[8];
 #pragma unroll
     for (int t = 0; t < 8; ++t) v[t] = (c + t < d) ? (float)src[row * ld + (int64_t)q * d + c + t] : 0.f;
-    put_split8(out + row * ldo + xidx((int64_t)q * dp + c), v);
+    put_split8(out + row * ldo + xidx((int64_t)q * dp + c), v, sc, f16);
   }
 }
 
 // split rows of a transposed FP64 matrix: out16[c ldo + xidx(r)] = src[r ld + c],
 // r < nr, c < nc; k (= r) padded with zeros to a multiple of 8
 __global__ void transpose_split_kernel(const double* __restrict__ src, int64_t nr, int64_t nc, int64_t ld,
-                                       uint16_t* __restrict__ out, int64_t ldo) {
+                                       uint16_t* __restrict__ out, int64_t ldo, const float* scl, int f16) {
   __shared__ float tile[32][33];
   const int64_t r0 = (int64_t)blockIdx.y * 32, c0 = (int64_t)blockIdx.x * 32;
   const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
@@ -145,7 +147,7 @@ __global__ void transpose_split_kernel(const double* __restrict__ src, int64_t n
       float v[8];
 #pragma unroll
       for (int t = 0; t < 8; ++t) v[t] = tile[8 * ch + t][cy];
-      put_split8(out + c * ldo + xidx(k), v);
+      put_split8(out + c * ldo + xidx(k), v, scale_at(scl), f16);
     }
   }
 }
@@ -351,14 +353,16 @@ void dense_pairs_build_split(tci_plan& P, hipStream_t st) {
   M.Ytx.alloc((size_t)N * ldj * 2);
   M.bytes += (int64_t)(M.Lovx.bytes + M.LovTx.bytes + M.Ytx.bytes);
   const int64_t work = N * n * (xpad(nv) / 8);
+  const float* s0 = M.scales.as<float>();  // constants' scale
+  const int f16 = M.prec == TCI_SIGMA_FP16X3 ? 1 : 0;
   hipLaunchKernelGGL(split_blocked_kernel, dim3(grid_for(work)), dim3(256), 0, st, P.Kov.as<double>(), N, N, n, nv,
-                     M.Kx.as<uint16_t>(), ldx);
+                     M.Kx.as<uint16_t>(), ldx, s0, f16);
   hipLaunchKernelGGL(split_blocked_kernel, dim3(grid_for(work)), dim3(256), 0, st, P.Jbig.as<double>(), N, N, n, nv,
-                     M.Jx.as<uint16_t>(), ldx);
+                     M.Jx.as<uint16_t>(), ldx, s0, f16);
   hipLaunchKernelGGL(split_blocked_kernel, dim3(grid_for(P.n_chol * n * (xpad(nv) / 8))), dim3(256), 0, st,
-                     P.Lov.as<double>(), P.n_chol, N, n, nv, M.Lovx.as<uint16_t>(), ldx);
+                     P.Lov.as<double>(), P.n_chol, N, n, nv, M.Lovx.as<uint16_t>(), ldx, s0, f16);
   hipLaunchKernelGGL(transpose_split_kernel, dim3((unsigned)cdiv(N, 32), (unsigned)cdiv(P.n_chol, 32)), dim3(256), 0, st,
-                     P.Lov.as<double>(), P.n_chol, N, N, M.LovTx.as<uint16_t>(), ldj);
+                     P.Lov.as<double>(), P.n_chol, N, N, M.LovTx.as<uint16_t>(), ldj, s0, f16);
   TCI_HIP(hipGetLastError());
   const int iN = (int)N, K = (int)(n * xpad(nv)), KJ = (int)xpad(P.n_chol);
   auto xop = [&](const DevBuf& b, int64_t ld) { return dop(b.p, ld); };
@@ -369,12 +373,13 @@ void dense_pairs_build_split(tci_plan& P, hipStream_t st) {
   M.ringZx.add(P.Zb.as<double>(), N, iN, iN,
                {dseg(xop(M.CtRx, ldx), xop(M.Kx, ldx), K, 1.0), dseg(xop(M.CRx, ldx), xop(M.Jx, ldx), K, 1.0),
                 dseg(xop(M.Ytx, ldj), xop(M.LovTx, ldj), KJ, 1.0)},
-               false);
+               false, false, 0, Blocking(), 5);
   M.ringZx.upload_tables(st);
   M.ringZ3x.clear();
   M.ringZ3x.split = true;
   M.ringZ3x.tile = 128;
-  M.ringZ3x.add(P.Z3b.as<double>(), N, iN, iN, {dseg(xop(M.Jx, ldx), xop(M.CCx, ldx), K, 1.0)}, false);
+  M.ringZ3x.add(P.Z3b.as<double>(), N, iN, iN, {dseg(xop(M.Jx, ldx), xop(M.CCx, ldx), K, 1.0)}, false, false, 0,
+                Blocking(), 5);
   M.ringZ3x.upload_tables(st);
   Blocking bw;
   bw.cdiv = nv;
@@ -383,7 +388,7 @@ void dense_pairs_build_split(tci_plan& P, hipStream_t st) {
   M.progWx.split = true;
   M.progWx.tile = 128;
   M.progWx.add(P.W.as<double>(), nv, (int)P.n_chol, iN, {dseg(xop(M.Lovx, ldx), xop(M.CtRx, ldx), K, 1.0)}, false,
-               false, 0, bw);
+               false, 0, bw, 5);
   M.progWx.upload_tables(st);
 }
 
@@ -397,9 +402,12 @@ void dense_pairs_terms(tci_plan& P, int precision, hipStream_t st) {
   hipLaunchKernelGGL(y_transpose_kernel, dim3((unsigned)cdiv(P.n_chol, 32), nt, n), b, 0, st, P.Y.as<double>(), n, nv,
                      P.n_chol, P.Yt.as<double>());
   if (precision == TCI_SIGMA_FP64) {
-    hipLaunchKernelGGL(relayout_kernel<false>, g, b, 0, st, P.tab, E, P.Ct.as<double>(), false, 1.0, P.CtRb.p, N);
-    hipLaunchKernelGGL(relayout_kernel<false>, g, b, 0, st, P.tab, E, P.C.as<double>(), false, 1.0, P.CRb.p, N);
-    hipLaunchKernelGGL(relayout_kernel<false>, g, b, 0, st, P.tab, E, P.C.as<double>(), true, 1.0, P.CCb.p, N);
+    hipLaunchKernelGGL(relayout_kernel<false>, g, b, 0, st, P.tab, E, P.Ct.as<double>(), false, 1.0, P.CtRb.p, N,
+                       nullptr, 0);
+    hipLaunchKernelGGL(relayout_kernel<false>, g, b, 0, st, P.tab, E, P.C.as<double>(), false, 1.0, P.CRb.p, N,
+                       nullptr, 0);
+    hipLaunchKernelGGL(relayout_kernel<false>, g, b, 0, st, P.tab, E, P.C.as<double>(), true, 1.0, P.CCb.p, N,
+                       nullptr, 0);
     TCI_HIP(hipGetLastError());
     P.ringZ.launch(st);
     P.ringZ3.launch(st);
@@ -407,15 +415,22 @@ void dense_pairs_terms(tci_plan& P, int precision, hipStream_t st) {
   } else {
     tci_plan::MixedLadder& M = P.mx;
     const int64_t ldx = 2 * (int64_t)n * xpad(nv), ldj = 2 * xpad(P.n_chol);
-    hipLaunchKernelGGL(relayout_kernel<true>, g, b, 0, st, P.tab, E, P.Ct.as<double>(), false, 1.0, M.CtRx.p, ldx);
-    hipLaunchKernelGGL(relayout_kernel<true>, g, b, 0, st, P.tab, E, P.C.as<double>(), false, -1.0, M.CRx.p, ldx);
-    hipLaunchKernelGGL(relayout_kernel<true>, g, b, 0, st, P.tab, E, P.C.as<double>(), true, 1.0, M.CCx.p, ldx);
+    float* sc = M.scales.as<float>();
+    const int f16 = precision == TCI_SIGMA_FP16X3 ? 1 : 0;
+    // fp16x3: one scale for C, Ct and Yt (|Ct| <= 3 max|C|; max|C| from this sigma's ladder)
+    if (f16) launch_split_scales(2, P.Yt.as<double>(), N * P.n_chol, M.maxbits.as<unsigned>(), 1, sc, (float)P.dmax, st);
+    const float* sv = sc + 6;
+    hipLaunchKernelGGL(relayout_kernel<true>, g, b, 0, st, P.tab, E, P.Ct.as<double>(), false, 1.0, M.CtRx.p, ldx, sv,
+                       f16);
+    hipLaunchKernelGGL(relayout_kernel<true>, g, b, 0, st, P.tab, E, P.C.as<double>(), false, -1.0, M.CRx.p, ldx, sv,
+                       f16);
+    hipLaunchKernelGGL(relayout_kernel<true>, g, b, 0, st, P.tab, E, P.C.as<double>(), true, 1.0, M.CCx.p, ldx, sv, f16);
     hipLaunchKernelGGL(split_blocked_kernel, dim3(grid_for(N * (xpad(P.n_chol) / 8))), b, 0, st, P.Yt.as<double>(), N,
-                       P.n_chol, 1, (int)P.n_chol, M.Ytx.as<uint16_t>(), ldj);
+                       P.n_chol, 1, (int)P.n_chol, M.Ytx.as<uint16_t>(), ldj, sv, f16);
     TCI_HIP(hipGetLastError());
-    M.ringZx.launch_split(precision, st);
-    M.ringZ3x.launch_split(precision, st);
-    M.progWx.launch_split(precision, st);
+    M.ringZx.launch_split(precision, st, 0, sc);
+    M.ringZ3x.launch_split(precision, st, 0, sc);
+    M.progWx.launch_split(precision, st, 0, sc);
   }
   hipLaunchKernelGGL(embed_blocks_kernel, dim3(nt * nt, P.n_pairs), b, 0, st, P.tab, E, P.C.as<double>(),
                      P.Cemb.as<double>(), P.CTemb.as<double>());

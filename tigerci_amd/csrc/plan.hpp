@@ -65,6 +65,13 @@ struct tci_plan {
     tci::DevBuf pairs_local;        // indices of the local-domain pairs
     int n_local = 0;
     bool prepacked = false;         // Lpx holds every J (packed once per plan)
+    int prec = -1;                  // split precision the copies are made for
+    // scales (fp16x3; all 1 for bf16x3): [0] constants (L, Kov, Jbig, Lov),
+    // [1] C of the ladder, [2] ladder intermediate, [3] phase-1 output,
+    // [4] phase-2 output, [5] pair-product output, [6] C and Yt of the pair
+    // products, [8] max of the constants; maxbits: [0] max|C|, [1] max|Yt|,
+    // [2] max over the constants (float bits)
+    tci::DevBuf scales, maxbits;
     tci::DevBuf lp_off_all;         // offsets of the local pairs in a prepacked Lpx
     tci::DevBuf Cx, cx_off;         // split rows of the C_P blocks (of C_P^T when merged)
     bool merged = false;            // all walks full-domain: single products over all walks
@@ -99,8 +106,11 @@ namespace tci {
 // split-precision ladder (ladder_mixed.hip): R (zeroed inside) = sum_J L^J C L^J
 // from the FP64 C blocks of the plan; records ev[3] before the final reduction
 // when profiling
-void mixed_prepare(tci_plan& P, hipStream_t st);
+void mixed_prepare(tci_plan& P, int precision, hipStream_t st);
 void mixed_ladder(tci_plan& P, int precision, hipStream_t st);
+// fp16x3 scales: mx[slot] = max |x[0..n)|, then set_scales mode (1: ladder, 2: pair products)
+void launch_split_scales(int mode, const double* x, int64_t n, unsigned* mx, int slot, float* sc, float dmax,
+                         hipStream_t st);
 // dense pair terms (pairs_full.hip)
 bool dense_pairs_choose(const tci_plan& P, double f_gather);
 void dense_pairs_build(tci_plan& P, const std::vector<double>& o4, hipStream_t st);
