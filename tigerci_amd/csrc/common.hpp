@@ -144,6 +144,7 @@ struct Blocking {
 // wave-uniform skipping of 16x16 blocks outside the output.
 struct GemmProgram {
   int tile = 64;
+  int tile_n = 0;      // split programs: tile width in N (0: square tiles)
   bool dense = false;  // contiguous row-major operands only: dense_gemm.hip kernel (tile 64 or 128)
   bool split = false;  // split-row operands, C = sum A B^T: dense_split.hip kernel (tile 64 or 128)
   bool scaled = false; // some task has an output scale (set by upload_tables)
@@ -162,7 +163,7 @@ struct GemmProgram {
 
 void launch_ggemm(int tile, const Task* tasks, const Seg* segs, int64_t ntasks, int64_t J0, hipStream_t st);
 void launch_dgemm(int tile, const Task* tasks, const Seg* segs, int64_t ntasks, int64_t J0, hipStream_t st);
-void launch_xgemm(int precision, int tile, const Task* tasks, const Seg* segs, int64_t ntasks, int64_t J0,
+void launch_xgemm(int precision, int tm, int tn, const Task* tasks, const Seg* segs, int64_t ntasks, int64_t J0,
                   const float* scales, hipStream_t st);
 
 // ---------------------------------------------------------------------------

@@ -17,8 +17,10 @@
 // groups.  A lane's fragment (row = lane % 16, k-chunk = lane / 16) is one
 // 16-byte LDS read.
 //
-// Tiles TS x TS: TS = 128 (4 waves of 64 x 64, 64 KiB LDS, 2 workgroups per
-// CU) or TS = 64 (4 waves of 32 x 32, 32 KiB, 4 workgroups per CU); two LDS
+// Tiles TM x TN (4 waves of TM/2 x TN/2): 128 x 128 (64 KiB LDS, 2 workgroups
+// per CU), 128 x 192 and 128 x 64 (for outputs whose short side is a domain of
+// up to 192 or 64, so that one tile spans it and each operand panel is read
+// once), 64 x 64 (32 KiB, 4 workgroups per CU); two LDS
 // stages with the global loads of stage s+1 in flight during the MFMAs of
 // stage s; each wave's loop specialised on its count of 16 x 16 blocks inside
 // the output.  Output: FP64 (optionally accumulated) or, with out_split, a
@@ -66,15 +68,15 @@ __device__ __forceinline__ bool xadvance(const Seg* segs, int seg1, int& seg, in
   return seg < seg1;
 }
 
-// Rows r0 + (tid >> 2) + 64 q (q < TS / 64), k-chunk tid & 3 of one operand.
+// Rows r0 + (tid >> 2) + 64 q (q < R / 64), k-chunk tid & 3 of one operand.
 // Rows beyond `rows` and chunks beyond K load from a zero page instead, so no
 // select waits on the loaded data.
 __device__ __attribute__((aligned(32))) u32x4 xzero[2];
 
-template <int TS>
+template <int R>
 __device__ __forceinline__ void xload(const Op& O, int64_t J0, int rows, int r0, int kb, int K, int tid,
                                       u32x4 (*v)[2]) {
-  constexpr int NL = TS / 64;
+  constexpr int NL = R / 64;
   const uint16_t* base = static_cast<const uint16_t*>(O.base) + J0 * O.jstride;
   const int k = kb + 8 * (tid & 3);
   const bool kok = k < K;
@@ -88,9 +90,9 @@ __device__ __forceinline__ void xload(const Op& O, int64_t J0, int rows, int r0,
   }
 }
 
-template <int TS>
+template <int R>
 __device__ __forceinline__ void xstore(int tid, const u32x4 (*v)[2], uint16_t* hi, uint16_t* lo) {
-  constexpr int NL = TS / 64;
+  constexpr int NL = R / 64;
 #pragma unroll
   for (int q = 0; q < NL; ++q) {
     const int e = xsw((tid >> 2) + 64 * q, tid & 3);
@@ -99,35 +101,36 @@ __device__ __forceinline__ void xstore(int tid, const u32x4 (*v)[2], uint16_t* h
   }
 }
 
-template <int TS>
+template <int TM, int TN>
 __device__ __forceinline__ void xstage(const Seg& S, const Task& t, int64_t J0, int kb, int tid, u32x4 (*va)[2],
                                        u32x4 (*vb)[2]) {
-  xload<TS>(S.A, J0, t.M, t.m0, kb, S.K, tid, va);
-  xload<TS>(S.B, J0, t.N, t.n0, kb, S.K, tid, vb);
+  xload<TM>(S.A, J0, t.M, t.m0, kb, S.K, tid, va);
+  xload<TN>(S.B, J0, t.N, t.n0, kb, S.K, tid, vb);
 }
 
-template <int TS, int PREC, int NBM, int NBN>
+template <int TM, int TN, int PREC, int NBM, int NBN>
 __device__ __forceinline__ void xgemm_body(const Task& t, const Seg* __restrict__ segs, int64_t J0, int seg0,
                                            const float* __restrict__ scales, uint16_t* sm, int tid, int wm, int wn,
                                            int li, int lg) {
   typedef typename XT<PREC>::T T16;
   typedef typename XT<PREC>::V V;
-  constexpr int NL = TS / 64, PL = TS * XK, STG = 4 * PL;
+  // stage layout: A hi, A lo (TM rows each), B hi, B lo (TN rows each)
+  constexpr int PA = TM * XK, PB = TN * XK, STG = 2 * PA + 2 * PB;
   f4 acc[NBM > 0 ? NBM : 1][NBN > 0 ? NBN : 1];
 #pragma unroll
   for (int i = 0; i < NBM; ++i)
 #pragma unroll
     for (int j = 0; j < NBN; ++j) acc[i][j] = f4{0.f, 0.f, 0.f, 0.f};
-  u32x4 va[NL][2], vb[NL][2];
+  u32x4 va[TM / 64][2], vb[TN / 64][2];
   int seg = seg0, kb = 0;
-  xstage<TS>(segs[seg], t, J0, kb, tid, va, vb);
-  xstore<TS>(tid, va, sm, sm + PL);
-  xstore<TS>(tid, vb, sm + 2 * PL, sm + 3 * PL);
+  xstage<TM, TN>(segs[seg], t, J0, kb, tid, va, vb);
+  xstore<TM>(tid, va, sm, sm + PA);
+  xstore<TN>(tid, vb, sm + 2 * PA, sm + 2 * PA + PB);
   bool more = xadvance(segs, t.seg1, seg, kb);
   __syncthreads();
   int buf = 0;
   while (true) {
-    if (more) xstage<TS>(segs[seg], t, J0, kb, tid, va, vb);
+    if (more) xstage<TM, TN>(segs[seg], t, J0, kb, tid, va, vb);
     if (NBM > 0 && NBN > 0) {
       const uint16_t* base = sm + buf * STG;
       V ah[NBM > 0 ? NBM : 1], al[NBM > 0 ? NBM : 1], bh[NBN > 0 ? NBN : 1], bl[NBN > 0 ? NBN : 1];
@@ -135,13 +138,13 @@ __device__ __forceinline__ void xgemm_body(const Task& t, const Seg* __restrict_
       for (int i = 0; i < NBM; ++i) {
         const int e = xsw(wm + i * 16 + li, lg);
         ah[i] = *reinterpret_cast<const V*>(base + e);
-        al[i] = *reinterpret_cast<const V*>(base + PL + e);
+        al[i] = *reinterpret_cast<const V*>(base + PA + e);
       }
 #pragma unroll
       for (int j = 0; j < NBN; ++j) {
         const int e = xsw(wn + j * 16 + li, lg);
-        bh[j] = *reinterpret_cast<const V*>(base + 2 * PL + e);
-        bl[j] = *reinterpret_cast<const V*>(base + 3 * PL + e);
+        bh[j] = *reinterpret_cast<const V*>(base + 2 * PA + e);
+        bl[j] = *reinterpret_cast<const V*>(base + 2 * PA + PB + e);
       }
       // B fragment as the MFMA's first operand: acc[i][j] holds the transposed
       // 16 x 16 block, so each lane ends with 4 consecutive output columns.
@@ -161,8 +164,8 @@ __device__ __forceinline__ void xgemm_body(const Task& t, const Seg* __restrict_
     }
     if (!more) break;
     uint16_t* nb = sm + (buf ^ 1) * STG;
-    xstore<TS>(tid, va, nb, nb + PL);
-    xstore<TS>(tid, vb, nb + 2 * PL, nb + 3 * PL);
+    xstore<TM>(tid, va, nb, nb + PA);
+    xstore<TN>(tid, vb, nb + 2 * PA, nb + 2 * PA + PB);
     __syncthreads();
     buf ^= 1;
     more = xadvance(segs, t.seg1, seg, kb);
@@ -233,25 +236,25 @@ __device__ __forceinline__ void xgemm_body(const Task& t, const Seg* __restrict_
   }
 }
 
-template <int TS, int PREC>
-__global__ __launch_bounds__(256, TS == 128 ? 2 : 4) void xgemm_kernel(const Task* __restrict__ tasks,
-                                                                       const Seg* __restrict__ segs, int64_t J0,
-                                                                       const float* __restrict__ scales) {
-  constexpr int WB = TS / 32;  // 16x16 blocks per wave side
+template <int TM, int TN, int PREC>
+__global__ __launch_bounds__(256, TM == 64 ? 4 : 2) void xgemm_kernel(const Task* __restrict__ tasks,
+                                                                      const Seg* __restrict__ segs, int64_t J0,
+                                                                      const float* __restrict__ scales) {
+  constexpr int WBM = TM / 32, WBN = TN / 32;  // 16x16 blocks per wave side
   const Task t = tasks[blockIdx.x];
-  __shared__ __attribute__((aligned(16))) uint16_t sm[2 * 4 * TS * XK];
+  __shared__ __attribute__((aligned(16))) uint16_t sm[2 * 2 * (TM + TN) * XK];
   const int tid = threadIdx.x;
   const int lane = tid & 63, w = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int li = lane & 15, lg = lane >> 4;
-  const int wm = (w >> 1) * (TS / 2), wn = (w & 1) * (TS / 2);
-  const int nbm = __builtin_amdgcn_readfirstlane(max(0, min(WB, (t.M - (t.m0 + wm) + 15) >> 4)));
-  const int nbn = __builtin_amdgcn_readfirstlane(max(0, min(WB, (t.N - (t.n0 + wn) + 15) >> 4)));
+  const int wm = (w >> 1) * (TM / 2), wn = (w & 1) * (TN / 2);
+  const int nbm = __builtin_amdgcn_readfirstlane(max(0, min(WBM, (t.M - (t.m0 + wm) + 15) >> 4)));
+  const int nbn = __builtin_amdgcn_readfirstlane(max(0, min(WBN, (t.N - (t.n0 + wn) + 15) >> 4)));
   int seg0 = t.seg0;
   while (seg0 < t.seg1 && segs[seg0].K <= 0) ++seg0;
   if (seg0 >= t.seg1) {
     if (!t.accumulate)
-      for (int e = tid; e < TS * TS; e += 256) {
-        const int row = t.m0 + e / TS, col = t.n0 + e % TS;
+      for (int e = tid; e < TM * TN; e += 256) {
+        const int row = t.m0 + e / TN, col = t.n0 + e % TN;
         if (row < t.M && col < t.N) {
           const int64_t ro = t.rdiv ? (int64_t)(row / t.rdiv) * t.rstride + (int64_t)(row % t.rdiv) * t.ldc
                                     : (int64_t)row * t.ldc;
@@ -268,42 +271,47 @@ __global__ __launch_bounds__(256, TS == 128 ? 2 : 4) void xgemm_kernel(const Tas
     return;
   }
 #define TCI_XB(M_, N_) \
-  case M_ * 5 + N_:    \
-    if constexpr (M_ <= WB && N_ <= WB) xgemm_body<TS, PREC, M_, N_>(t, segs, J0, seg0, scales, sm, tid, wm, wn, li, lg); \
+  case M_ * 8 + N_:    \
+    if constexpr (M_ <= WBM && N_ <= WBN)                                                              \
+      xgemm_body<TM, TN, PREC, M_, N_>(t, segs, J0, seg0, scales, sm, tid, wm, wn, li, lg);            \
     break;
-  switch (nbm * 5 + nbn) {
-    TCI_XB(0, 0) TCI_XB(0, 1) TCI_XB(0, 2) TCI_XB(0, 3) TCI_XB(0, 4)
-    TCI_XB(1, 0) TCI_XB(1, 1) TCI_XB(1, 2) TCI_XB(1, 3) TCI_XB(1, 4)
-    TCI_XB(2, 0) TCI_XB(2, 1) TCI_XB(2, 2) TCI_XB(2, 3) TCI_XB(2, 4)
-    TCI_XB(3, 0) TCI_XB(3, 1) TCI_XB(3, 2) TCI_XB(3, 3) TCI_XB(3, 4)
-    TCI_XB(4, 0) TCI_XB(4, 1) TCI_XB(4, 2) TCI_XB(4, 3) TCI_XB(4, 4)
+#define TCI_XROW(M_) TCI_XB(M_, 0) TCI_XB(M_, 1) TCI_XB(M_, 2) TCI_XB(M_, 3) TCI_XB(M_, 4) TCI_XB(M_, 5) TCI_XB(M_, 6)
+  switch (nbm * 8 + nbn) {
+    TCI_XROW(0) TCI_XROW(1) TCI_XROW(2) TCI_XROW(3) TCI_XROW(4)
     default: break;
   }
+#undef TCI_XROW
 #undef TCI_XB
 }
 
 }  // namespace
 
-void launch_xgemm(int precision, int tile, const Task* tasks, const Seg* segs, int64_t ntasks, int64_t J0,
+template <int PREC>
+void launch_xgemm_t(int tm, int tn, const Task* tasks, const Seg* segs, int64_t ntasks, int64_t J0, const float* scales,
+                    hipStream_t st) {
+  const dim3 g((unsigned)ntasks), b(256);
+  if (tm == 128 && tn == 128) hipLaunchKernelGGL((xgemm_kernel<128, 128, PREC>), g, b, 0, st, tasks, segs, J0, scales);
+  else if (tm == 128 && tn == 192) hipLaunchKernelGGL((xgemm_kernel<128, 192, PREC>), g, b, 0, st, tasks, segs, J0, scales);
+  else if (tm == 128 && tn == 64) hipLaunchKernelGGL((xgemm_kernel<128, 64, PREC>), g, b, 0, st, tasks, segs, J0, scales);
+  else if (tm == 64 && tn == 64) hipLaunchKernelGGL((xgemm_kernel<64, 64, PREC>), g, b, 0, st, tasks, segs, J0, scales);
+  else throw Error(TCI_ERR_ARG, "unsupported split GEMM tile " + std::to_string(tm) + "x" + std::to_string(tn));
+}
+
+void launch_xgemm(int precision, int tm, int tn, const Task* tasks, const Seg* segs, int64_t ntasks, int64_t J0,
                   const float* scales, hipStream_t st) {
   TCI_REQUIRE(precision == TCI_SIGMA_BF16X3 || precision == TCI_SIGMA_FP16X3, "split GEMM needs a split precision");
   if (ntasks <= 0) return;
   TCI_REQUIRE(ntasks < ((int64_t)1 << 31), "too many gemm tasks");
-  const dim3 g((unsigned)ntasks), b(256);
-  if (precision == TCI_SIGMA_BF16X3) {
-    if (tile == 128) hipLaunchKernelGGL((xgemm_kernel<128, TCI_SIGMA_BF16X3>), g, b, 0, st, tasks, segs, J0, scales);
-    else hipLaunchKernelGGL((xgemm_kernel<64, TCI_SIGMA_BF16X3>), g, b, 0, st, tasks, segs, J0, scales);
-  } else {
-    if (tile == 128) hipLaunchKernelGGL((xgemm_kernel<128, TCI_SIGMA_FP16X3>), g, b, 0, st, tasks, segs, J0, scales);
-    else hipLaunchKernelGGL((xgemm_kernel<64, TCI_SIGMA_FP16X3>), g, b, 0, st, tasks, segs, J0, scales);
-  }
+  if (precision == TCI_SIGMA_BF16X3) launch_xgemm_t<TCI_SIGMA_BF16X3>(tm, tn, tasks, segs, ntasks, J0, scales, st);
+  else launch_xgemm_t<TCI_SIGMA_FP16X3>(tm, tn, tasks, segs, ntasks, J0, scales, st);
   TCI_HIP(hipGetLastError());
 }
 
 void GemmProgram::launch_split(int precision, hipStream_t st, int64_t J0, const float* scales) const {
   TCI_REQUIRE(split, "not a split program");
   TCI_REQUIRE(!scaled || scales, "scaled split tasks need a scales array");
-  launch_xgemm(precision, tile, d_tasks.as<Task>(), d_segs.as<Seg>(), (int64_t)tasks.size(), J0, scales, st);
+  launch_xgemm(precision, tile, tile_n > 0 ? tile_n : tile, d_tasks.as<Task>(), d_segs.as<Seg>(),
+               (int64_t)tasks.size(), J0, scales, st);
 }
 
 }  // namespace tci

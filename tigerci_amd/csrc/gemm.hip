@@ -403,8 +403,9 @@ void GemmProgram::add(double* C, int64_t ldc, int M, int N, const std::vector<Se
   int seg0 = (int)segs.size();
   for (const Seg& x : s) segs.push_back(x);
   int seg1 = (int)segs.size();
+  const int tn = (split && tile_n > 0) ? tile_n : tile;
   for (int m0 = 0; m0 < M; m0 += tile)
-    for (int n0 = 0; n0 < N; n0 += tile) {
+    for (int n0 = 0; n0 < N; n0 += tn) {
       Task t;
       t.C = C;
       t.ldc = ldc;

@@ -324,6 +324,26 @@ void mixed_prepare(tci_plan& P, int precision, hipStream_t st) {
   }
   if (M.merged) tiles2 = cdiv((int64_t)n_pairs * n_v, tile2) * cdiv(n_v, tile2);
   M.rpart_n = (int)std::max<int64_t>(1, std::min<int64_t>(16, cdiv(512, std::max<int64_t>(tiles2, 1))));
+  // per-walk tile shapes (local domains): phase 2 of walk Q runs on tiles
+  // t2m[v2] x t2n[v2]; every shape gets enough K-splits to fill the chip in its
+  // own launch
+  static const int t2m[3] = {64, 128, 128}, t2n[3] = {64, 128, 192};
+  std::vector<int> v2of(n_pairs, 1);
+  int nsplit_v[3] = {1, 1, 1};
+  if (!M.merged) {
+    int64_t tv[3] = {0, 0, 0};
+    for (int Q = 0; Q < n_pairs; ++Q) {
+      const int64_t d = P.h_dom_off[Q + 1] - P.h_dom_off[Q];
+      const int v = d <= 64 ? 0 : (d <= 128 ? 1 : ((double)d / (192.0 * cdiv(d, 192)) >= (double)d / (128.0 * cdiv(d, 128)) ? 2 : 1));
+      v2of[Q] = v;
+      tv[v] += cdiv(d, t2m[v]) * cdiv(d, t2n[v]);
+    }
+    M.rpart_n = 1;
+    for (int v = 0; v < 3; ++v) {
+      nsplit_v[v] = tv[v] ? (int)std::max<int64_t>(1, std::min<int64_t>(16, cdiv(512, tv[v]))) : 1;
+      M.rpart_n = std::max(M.rpart_n, nsplit_v[v]);
+    }
+  }
   if (M.rpart_n > 1) {
     M.Rpart.alloc((size_t)M.rpart_n * P.sum_d2 * 8);
     M.bytes += (int64_t)M.Rpart.bytes;
@@ -388,6 +408,22 @@ void mixed_prepare(tci_plan& P, int precision, hipStream_t st) {
       ph.p2.upload_tables(st);
       continue;
     }
+    // per-walk tile shapes: phase 1 (M = len d rows) 128 x {64,128,192} and
+    // phase 2 (d x d) {64x64, 128x128, 128x192}, chosen so one tile spans the
+    // short side of the domain where possible (each operand panel read once)
+    ph.p1x.clear();
+    ph.p2x.clear();
+    ph.p1x.resize(3);
+    ph.p2x.resize(3);
+    static const int tn1[3] = {64, 128, 192};
+    for (int v = 0; v < 3; ++v) {
+      ph.p1x[v].split = ph.p2x[v].split = true;
+      ph.p1x[v].tile = 128;
+      ph.p1x[v].tile_n = tn1[v];
+      ph.p2x[v].tile = t2m[v];
+      ph.p2x[v].tile_n = t2n[v];
+    }
+    auto util = [](int64_t d, int t) { return (double)d / (double)(t * cdiv(d, t)); };
     int64_t toff = 0;
     int qloc = 0;
     for (int Q = 0; Q < n_pairs; ++Q) {
@@ -411,21 +447,26 @@ void mixed_prepare(tci_plan& P, int precision, hipStream_t st) {
         jstride = 0;
       }
       uint16_t* Tq = Tx + toff;
-      ph.p1.add(reinterpret_cast<double*>(Tq), ldx, (int)(len * d), d,
-                {xseg(xop(Lb, ldx, jstride), xop(Cx + cxo[Q], ldx), d)}, false, true, 0, Blocking(), 3);
-      for (int sp = 0; sp < ph.nsplit; ++sp) {
-        const int64_t j0 = len * sp / ph.nsplit, j1 = len * (sp + 1) / ph.nsplit;
+      int v1 = 0;
+      for (int v = 1; v < 3; ++v)
+        if (util(d, tn1[v]) >= util(d, tn1[v1])) v1 = v;
+      const int v2 = v2of[Q];
+      const int nsp = (int)std::min<int64_t>(nsplit_v[v2], len);
+      ph.p1x[v1].add(reinterpret_cast<double*>(Tq), ldx, (int)(len * d), d,
+                     {xseg(xop(Lb, ldx, jstride), xop(Cx + cxo[Q], ldx), d)}, false, true, 0, Blocking(), 3);
+      for (int sp = 0; sp < nsp; ++sp) {
+        const int64_t j0 = len * sp / nsp, j1 = len * (sp + 1) / nsp;
         std::vector<Seg> sg;
         for (int64_t jj = j0; jj < j1; ++jj)
           sg.push_back(xseg(xop(Lb + jj * jstep, ldx, jstride), xop(Tq + jj * d * ldx, ldx), d));
         double* Rout = (M.rpart_n > 1 ? M.Rpart.as<double>() + (int64_t)sp * P.sum_d2 : P.R.as<double>()) +
                        P.h_cd_off[Q];
-        ph.p2.add(Rout, d, d, d, sg, true, false, 0, Blocking(), 4);
+        ph.p2x[v2].add(Rout, d, d, d, sg, true, false, 0, Blocking(), 4);
       }
       toff += len * d * ldx;
     }
-    ph.p1.upload_tables(st);
-    ph.p2.upload_tables(st);
+    for (auto& g : ph.p1x) g.upload_tables(st);
+    for (auto& g : ph.p2x) g.upload_tables(st);
   }
   if (P.dense_pairs) dense_pairs_build_split(P, st);
   TCI_HIP(hipStreamSynchronize(st));
@@ -471,8 +512,13 @@ void mixed_ladder(tci_plan& P, int precision, hipStream_t st) {
                            ch.first, (int)ph.Lc, M.Lpx.as<uint16_t>(), sc, f16);
       TCI_HIP(hipGetLastError());
     }
-    ph.p1.launch_split(precision, st, ch.first, sc);
-    ph.p2.launch_split(precision, st, ch.first, sc);
+    if (M.merged) {
+      ph.p1.launch_split(precision, st, ch.first, sc);
+      ph.p2.launch_split(precision, st, ch.first, sc);
+    } else {
+      for (const auto& g : ph.p1x) g.launch_split(precision, st, ch.first, sc);
+      for (const auto& g : ph.p2x) g.launch_split(precision, st, ch.first, sc);
+    }
   }
   if (P.profiling) TCI_HIP(hipEventRecord(P.ev[3], st));
   if (M.rpart_n > 1) ladder_reduce(M.Rpart.as<double>(), P.sum_d2, M.rpart_n, P.R.as<double>(), P.sum_d2, st);

@@ -607,11 +607,25 @@ void tci_plan::build_two_phase_ladder(hipStream_t st) {
       ph.p2.upload_tables(st);
       continue;
     }
+    // per-walk programs: phase 1 (M = len d) on 64- or 128-wide tiles by domain
+    // size; phase 2 on 64-wide tiles (4 workgroups per CU measure faster than
+    // 128-wide tiles for the long-K d x d outputs)
+    ph.p1x.clear();
+    ph.p2x.clear();
+    ph.p1x.resize(2);
+    ph.p2x.resize(1);
+    for (int v = 0; v < 2; ++v) {
+      ph.p1x[v].dense = true;
+      ph.p1x[v].tile = v ? 128 : 64;
+    }
+    ph.p2x[0].dense = true;
+    ph.p2x[0].tile = 64;
     int64_t toff = 0;
     int qloc = 0;
     for (int P = 0; P < n_pairs; ++P) {
       const int d = (int)(h_dom_off[P + 1] - h_dom_off[P]);
       const bool full = d == n_v;
+      const int v = d <= 64 ? 0 : 1;
       double* T = Tbuf.as<double>() + toff;
       // L operand of this pair: rows (jj, a) with leading dimension ld, one
       // jj-block every jstep elements; full domains read L^J itself (J0 via jstride)
@@ -630,9 +644,9 @@ void tci_plan::build_two_phase_ladder(hipStream_t st) {
       }
       // phase 1: rows (jj, a) of L (M = len*d); rows jj*ld.. are contiguous when
       // ld == d, and for full domains ld == n_v == d as well
-      ph.p1.add(T, d, (int)(len * d), d, {seg(op(Lb, ld, 1, nullptr, nullptr, lvv_f32, jstride),
-                                              op(C.as<double>() + h_cd_off[P], d, 1), d, 1.0)},
-                false);
+      ph.p1x[v].add(T, d, (int)(len * d), d, {seg(op(Lb, ld, 1, nullptr, nullptr, lvv_f32, jstride),
+                                                  op(C.as<double>() + h_cd_off[P], d, 1), d, 1.0)},
+                    false);
       for (int sp = 0; sp < ph.nsplit; ++sp) {
         const int64_t j0 = len * sp / ph.nsplit, j1 = len * (sp + 1) / ph.nsplit;
         std::vector<Seg> sg;
@@ -641,12 +655,12 @@ void tci_plan::build_two_phase_ladder(hipStream_t st) {
           sg.push_back(seg(op(T + jj * d * d, d, 1), op(Lj, ld, 1, nullptr, nullptr, lvv_f32, jstride), d, 1.0));
         }
         double* Rout = (rpart_n > 1 ? Rpart.as<double>() + (int64_t)sp * sum_d2 : R.as<double>()) + h_cd_off[P];
-        ph.p2.add(Rout, d, d, d, sg, true);
+        ph.p2x[0].add(Rout, d, d, d, sg, true);
       }
       toff += len * (int64_t)d * d;
     }
-    ph.p1.upload_tables(st);
-    ph.p2.upload_tables(st);
+    for (auto& g : ph.p1x) g.upload_tables(st);
+    for (auto& g : ph.p2x) g.upload_tables(st);
   }
 }
 
@@ -667,8 +681,13 @@ void tci_plan::unpack_and_ladder(const double* x, int precision, hipStream_t st)
         PackTab K{pairs_local.as<int32_t>(), ph.lp_off.as<int64_t>(), n_local};
         launch_pack_L(tab, K, Lvv.p, lvv_f32, ch.first, ph.Lc, Lp.p, st);
       }
-      ph.p1.launch(st, ch.first);
-      ph.p2.launch(st, ch.first);
+      if (ladder_merged) {
+        ph.p1.launch(st, ch.first);
+        ph.p2.launch(st, ch.first);
+      } else {
+        for (const auto& g : ph.p1x) g.launch(st, ch.first);
+        for (const auto& g : ph.p2x) g.launch(st, ch.first);
+      }
     }
     if (profiling) TCI_HIP(hipEventRecord(ev[3], st));
     if (rpart_n > 1) ladder_reduce(Rpart.as<double>(), sum_d2, rpart_n, R.as<double>(), sum_d2, st);

@@ -42,6 +42,7 @@ struct tci_plan {
     int64_t Lc = 0;                 // chunk length these tables are built for
     int nsplit = 1;                 // K-splits of phase 2 (partial R buffers)
     tci::GemmProgram p1, p2;        // dense programs (FP64, or split for the mixed ladder)
+    std::vector<tci::GemmProgram> p1x, p2x;  // split per-walk programs, one per tile shape
     tci::DevBuf lp_off;             // packed-L offsets of the local-domain pairs
   };
   tci::DevBuf Lp, pairs_local;      // packed L^J[D,D] of local-domain pairs (one chunk)
