@@ -94,7 +94,12 @@ def main() -> int:
                     help="default: davidson_34x192x1200_dom40-192 (davidson), sigma_16x300x1200_full (sigma)")
     ap.add_argument("--roots", type=int, default=3)
     ap.add_argument("--l-scale", type=float, default=None, help="override the workload's Cholesky-vector scale")
+    ap.add_argument("--ladder-launch", default="grouped", choices=["grouped", "per-pair"],
+                    help="split ladder of local domains: grouped launches or one launch per walk (comparison)")
     ap.add_argument("--precision", default="bf16x3")
+    ap.add_argument("--graph", action="store_true",
+                    help="replay every sigma build from a captured HIP graph (ladder times then come from "
+                         "separate eager profiled builds)")
     ap.add_argument("--mode", default="davidson", choices=["davidson", "sigma"])
     ap.add_argument("--switch-tol", type=float, default=1e-6)
     ap.add_argument("--no-cpu", action="store_true")
@@ -107,6 +112,8 @@ def main() -> int:
                                     "(see the sharding design notes); run with --gpus 1"}))
         return 2
 
+    if a.ladder_launch == "per-pair":
+        os.environ["TCI_LADDER_PER_PAIR"] = "1"
     import tigerci_amd as tci
     from tigerci_amd.synthetic import workload, random_vector, WORKLOADS
 
@@ -122,7 +129,7 @@ def main() -> int:
     log(f"[bench] plan: n_csf={plan.n_csf} items={plan.info.n_ladder_items} "
         f"device {plan.info.device_bytes / 2**30:.1f} GiB, built in {time.perf_counter() - t:.1f} s")
     info = plan.info
-    cfg = {"workload": a.workload, "n_int": syn.n_int, "n_v": syn.n_v, "n_chol": syn.n_chol,
+    cfg = {"workload": a.workload, "ladder_launch": a.ladder_launch, "n_int": syn.n_int, "n_v": syn.n_v, "n_chol": syn.n_chol,
            "pairs": int(len(syn.pairs)), "n_csf": int(plan.n_csf), "L_storage": ladder_dtype,
            "ladder_flops_per_sigma": info.flops_ladder, "sigma_flops": info.flops_sigma}
     out = {"metric": METRIC, "n_gpus": 1, "steps": a.steps, "warmup": a.warmup, "scaling": "weak",
@@ -130,19 +137,32 @@ def main() -> int:
            "data": f"synthetic (seeded N(0,1) exp(-J/80) x "
                    f"{a.l_scale if a.l_scale is not None else WORKLOADS[a.workload][6]} Cholesky vectors)"}
 
+    def ladder_probe(prec, n):
+        """ladder and sigma event times of n eager builds (graph replays time only the whole build)"""
+        xp = tci.DeviceArray.from_numpy(random_vector(plan.n_csf, a.seed + 1))
+        yp = tci.DeviceArray(plan.n_csf)
+        plan.sigma(xp, yp, precision=prec)
+        plan.profile(True)
+        for _ in range(n):
+            plan.sigma(xp, yp, precision=prec)
+        r = plan.profile_read(prec)
+        plan.profile(False)
+        return r
+
+    cfg["sigma_graph"] = a.graph
     if a.mode == "sigma":
         x = tci.DeviceArray.from_numpy(random_vector(plan.n_csf, a.seed))
         y = tci.DeviceArray(plan.n_csf)
         for _ in range(a.warmup):
-            plan.sigma(x, y, precision=a.precision)
+            plan.sigma(x, y, precision=a.precision, graph=a.graph)
         tci.synchronize()
-        plan.profile(True)
+        plan.profile(not a.graph)
         t0 = time.perf_counter()
         for _ in range(a.steps):
-            plan.sigma(x, y, precision=a.precision)
+            plan.sigma(x, y, precision=a.precision, graph=a.graph)
         tci.synchronize()
         dt = (time.perf_counter() - t0) / a.steps
-        pr = plan.profile_read(a.precision)
+        pr = ladder_probe(a.precision, a.steps) if a.graph else plan.profile_read(a.precision)
         lad = pr["ladder_ms"] / max(pr["ladder_launches"], 1) / 1e3
         ach = info.flops_ladder / lad
         if a.precision != "fp64":  # precision study: this sigma against the FP64 one
@@ -182,7 +202,7 @@ def main() -> int:
     # ---- Davidson: mixed (timed steps) and FP64 reference -------------------
     def solve(prec):
         return plan.davidson(roots=a.roots, energy_tol=1e-6, norm_tol=1e-5, precision=prec,
-                             switch_tol=a.switch_tol)
+                             switch_tol=a.switch_tol, graph=a.graph)
 
     for _ in range(a.warmup):
         solve(a.precision)
@@ -203,6 +223,12 @@ def main() -> int:
     t_64 = time.perf_counter() - t0
     p64 = plan.profile_read("fp64")
     log(f"[bench] fp64 solve: {res_64} E={e_64}")
+    if a.graph:
+        sig_ms = {k: v["sigma_ms"] / max(v["sigma_calls"], 1) for k, v in (("fp64", p64), ("mixed", pm))}
+        p64 = dict(ladder_probe("fp64", 3), sigma_ms=p64["sigma_ms"], sigma_calls=p64["sigma_calls"])
+        if a.precision != "fp64":
+            pm = dict(ladder_probe(a.precision, 3), sigma_ms=pm["sigma_ms"], sigma_calls=pm["sigma_calls"])
+        log(f"[bench] graph sigma ms {sig_ms}")
     lad = p64["ladder_ms"] / max(p64["ladder_launches"], 1) / 1e3
     ach = info.flops_ladder / lad
     sig64 = p64["sigma_ms"] / max(p64["sigma_calls"], 1) / 1e3

@@ -49,6 +49,12 @@ enum tci_precision {
 /* flag bits for tci_sigma_apply (low byte = tci_precision) */
 #define TCI_ACCUMULATE 0x100 /* sigma += H c (htimesc semantics) instead of sigma = H c */
 #define TCI_ADD_EREF   0x200 /* include E_ref on the diagonal (bare H) */
+/* Run the build as one HIP graph launch.  The first call with a given
+ * precision/ACCUMULATE/ADD_EREF combination captures the kernel sequence
+ * on a private stream; later calls replay it on `stream` for
+ * any civec/sigmavec pair.  Calls on one plan must be ordered on one stream.
+ * With profiling on, only the whole build is timed (no ladder time). */
+#define TCI_SIGMA_GRAPH 0x400
 
 /* Problem description.  All pointers are host pointers, read during
  * tci_plan_create only.  Orbitals: internal 0..n_int-1, virtual
@@ -114,7 +120,8 @@ typedef struct tci_davidson_params {
   int32_t maxit;          /* 0 -> 50*roots (davidson.f90:80) */
   double energy_tol;      /* |dE| convergence (ENERGY TOLERANCE, default 1e-6) */
   double norm_tol;        /* residual norm convergence (RESIDUAL TOLERANCE) */
-  int32_t precision;      /* tci_precision of the expansion sigmas */
+  int32_t precision;      /* tci_precision of the expansion sigmas, | TCI_SIGMA_GRAPH to replay
+                             every sigma build from a captured HIP graph */
   double switch_tol;      /* mixed: switch to FP64 sigma once every root has |dE| < switch_tol
                              or a residual below norm_tol (forced at maxit/2) */
   int32_t max_vectors;    /* subspace collapse budget (0 -> maxit*roots + roots, capped at 32 GB of vectors) */

@@ -8,6 +8,7 @@ the remaining terms stay FP64.  The mixed
 Davidson refines in FP64 after the switch, so its roots must match the FP64
 solver's to 1 microhartree (they typically agree to 1e-9).
 """
+import os
 import sys
 
 import numpy as np
@@ -115,3 +116,20 @@ def test_mixed_davidson_matches_fp64(args, roots):
     assert r64["converged"] and rmx["converged"]
     assert rmx["sigma_mixed"] > 0
     assert np.abs(emx - e64).max() < 1e-6
+
+
+def test_per_pair_launch_matches_grouped():
+    """One launch per walk and phase (TCI_LADDER_PER_PAIR=1) computes the same
+    tiles in the same order as the grouped launches: identical sigma."""
+    _say("case: per-walk ladder launches vs grouped")
+    prob = make_problem(5, 60, 30, 9, domain_range=(10, 60), weak_below=12)
+    x = random_ci_vector(prob, 5)
+    # the split ladder is laid out on a plan's first split sigma, which is
+    # when the launch mode is read
+    a = _sigma(tci.Plan.from_problem(prob), x, precision="bf16x3")
+    os.environ["TCI_LADDER_PER_PAIR"] = "1"
+    try:
+        b = _sigma(tci.Plan.from_problem(prob), x, precision="bf16x3")
+    finally:
+        del os.environ["TCI_LADDER_PER_PAIR"]
+    assert np.array_equal(a, b)

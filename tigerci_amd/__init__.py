@@ -32,6 +32,7 @@ TCI_F64, TCI_F32 = 0, 1
 PRECISION = {"fp64": 0, "bf16x3": 1, "fp16x3": 2}
 TCI_ACCUMULATE = 0x100
 TCI_ADD_EREF = 0x200
+TCI_SIGMA_GRAPH = 0x400
 
 
 class TigerSigmaError(RuntimeError):
@@ -240,9 +241,11 @@ class Plan:
             pass
 
     def sigma(self, c, out, precision: str = "fp64", accumulate: bool = False, add_eref: bool = False,
-              stream=None) -> None:
-        """out = (H - E_ref) c on device (DeviceArray or device addresses)."""
+              stream=None, graph: bool = False) -> None:
+        """out = (H - E_ref) c on device (DeviceArray or device addresses).
+        ``graph``: replay the build from a HIP graph captured on first use."""
         flags = PRECISION[precision] | (TCI_ACCUMULATE if accumulate else 0) | (TCI_ADD_EREF if add_eref else 0)
+        flags |= TCI_SIGMA_GRAPH if graph else 0
         _check(self._lib.tci_sigma_apply(self._h, _ptr(c), _ptr(out), self.n_csf, flags, _stream(stream)))
 
     def diagonal(self, out=None, stream=None):
@@ -273,12 +276,14 @@ class Plan:
 
     def davidson(self, roots: int = 1, energy_tol: float = 1e-6, norm_tol: float = 1e-6,
                  precision: str = "fp64", switch_tol: float | None = None, maxit: int = 0,
-                 max_vectors: int = 0, vectors=None, verbose: bool = False):
+                 max_vectors: int = 0, vectors=None, verbose: bool = False, graph: bool = False):
         """Lowest roots; returns (total energies, result dict).  ``vectors`` is an
-        optional DeviceArray of roots*n_csf doubles for the eigenvectors."""
+        optional DeviceArray of roots*n_csf doubles for the eigenvectors;
+        ``graph`` replays every sigma build from a captured HIP graph."""
         if switch_tol is None:
             switch_tol = energy_tol
-        p = _DavParams(roots, maxit, energy_tol, norm_tol, PRECISION[precision], switch_tol, max_vectors,
+        prec = PRECISION[precision] | (TCI_SIGMA_GRAPH if graph else 0)
+        p = _DavParams(roots, maxit, energy_tol, norm_tol, prec, switch_tol, max_vectors,
                        1 if verbose else 0)
         e = np.zeros(roots)
         r = _DavResult()

@@ -238,7 +238,11 @@ extern "C" int tci_davidson_run(tci_plan* plan, const tci_davidson_params* prm, 
     }
     mv = std::min(mv, 1000);
     TCI_REQUIRE(mv >= 5 * roots, "max_vectors must be at least 5*roots");
-    const int prec0 = prm->precision;
+    // low byte: tci_precision of the expansion sigmas; TCI_SIGMA_GRAPH replays
+    // every sigma build from a captured HIP graph
+    TCI_REQUIRE((prm->precision & ~(0xff | TCI_SIGMA_GRAPH)) == 0, "unknown bits in precision");
+    const int graph_flag = prm->precision & TCI_SIGMA_GRAPH;
+    const int prec0 = prm->precision & 0xff;
     TCI_REQUIRE(prec0 == TCI_SIGMA_FP64 || prec0 == TCI_SIGMA_BF16X3 || prec0 == TCI_SIGMA_FP16X3, "unknown precision");
     const int64_t n = plan->n_csf;
     Dav D(*plan);
@@ -259,7 +263,7 @@ extern "C" int tci_davidson_run(tci_plan* plan, const tci_davidson_params* prm, 
     int precision = prec0;
     auto sigma = [&](int k) {
       auto t0 = std::chrono::steady_clock::now();
-      plan->sigma(Vk(k), Sk(k), precision, st);
+      plan->sigma(Vk(k), Sk(k), precision | graph_flag, st);
       TCI_HIP(hipStreamSynchronize(st));
       t_sigma += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
       if (precision == TCI_SIGMA_FP64) ++n64; else ++nmix;

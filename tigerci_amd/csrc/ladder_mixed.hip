@@ -204,6 +204,9 @@ void mixed_prepare(tci_plan& P, int precision, hipStream_t st) {
   tci_plan::MixedLadder& M = P.mx;
   TCI_REQUIRE(precision == TCI_SIGMA_BF16X3 || precision == TCI_SIGMA_FP16X3, "not a split precision");
   if (M.ready && M.prec == precision) return;
+  TCI_REQUIRE(!P.capturing, "split operands are prepared outside graph capture");
+  // captured graphs point into the copies rebuilt below
+  P.drop_graphs();
   M.ready = false;
   M.prec = precision;
   const int f16 = precision == TCI_SIGMA_FP16X3 ? 1 : 0;
@@ -315,6 +318,9 @@ void mixed_prepare(tci_plan& P, int precision, hipStream_t st) {
   {
     const char* env = std::getenv("TCI_LADDER_MERGED");
     M.merged = loc.empty() && n_pairs > 0 && !(env && env[0] == '0');
+    const char* pp = std::getenv("TCI_LADDER_PER_PAIR");
+    M.per_pair = pp && pp[0] == '1';
+    if (M.per_pair) M.merged = false;
   }
   const int tile2 = 128;
   int64_t tiles2 = 0;
@@ -413,6 +419,8 @@ void mixed_prepare(tci_plan& P, int precision, hipStream_t st) {
     // short side of the domain where possible (each operand panel read once)
     ph.p1x.clear();
     ph.p2x.clear();
+    ph.pp1.clear();
+    ph.pp2.clear();
     ph.p1x.resize(3);
     ph.p2x.resize(3);
     static const int tn1[3] = {64, 128, 192};
@@ -452,8 +460,23 @@ void mixed_prepare(tci_plan& P, int precision, hipStream_t st) {
         if (util(d, tn1[v]) >= util(d, tn1[v1])) v1 = v;
       const int v2 = v2of[Q];
       const int nsp = (int)std::min<int64_t>(nsplit_v[v2], len);
-      ph.p1x[v1].add(reinterpret_cast<double*>(Tq), ldx, (int)(len * d), d,
-                     {xseg(xop(Lb, ldx, jstride), xop(Cx + cxo[Q], ldx), d)}, false, true, 0, Blocking(), 3);
+      // one program per walk and phase instead (a launch per walk) with
+      // TCI_LADDER_PER_PAIR=1, to compare against the grouped launches
+      GemmProgram* g1 = &ph.p1x[v1];
+      GemmProgram* g2 = &ph.p2x[v2];
+      if (M.per_pair) {
+        ph.pp1.emplace_back();
+        ph.pp2.emplace_back();
+        g1 = &ph.pp1.back();
+        g2 = &ph.pp2.back();
+        g1->split = g2->split = true;
+        g1->tile = 128;
+        g1->tile_n = tn1[v1];
+        g2->tile = ph.p2x[v2].tile;
+        g2->tile_n = ph.p2x[v2].tile_n;
+      }
+      g1->add(reinterpret_cast<double*>(Tq), ldx, (int)(len * d), d,
+              {xseg(xop(Lb, ldx, jstride), xop(Cx + cxo[Q], ldx), d)}, false, true, 0, Blocking(), 3);
       for (int sp = 0; sp < nsp; ++sp) {
         const int64_t j0 = len * sp / nsp, j1 = len * (sp + 1) / nsp;
         std::vector<Seg> sg;
@@ -461,12 +484,14 @@ void mixed_prepare(tci_plan& P, int precision, hipStream_t st) {
           sg.push_back(xseg(xop(Lb + jj * jstep, ldx, jstride), xop(Tq + jj * d * ldx, ldx), d));
         double* Rout = (M.rpart_n > 1 ? M.Rpart.as<double>() + (int64_t)sp * P.sum_d2 : P.R.as<double>()) +
                        P.h_cd_off[Q];
-        ph.p2x[v2].add(Rout, d, d, d, sg, true, false, 0, Blocking(), 4);
+        g2->add(Rout, d, d, d, sg, true, false, 0, Blocking(), 4);
       }
       toff += len * d * ldx;
     }
     for (auto& g : ph.p1x) g.upload_tables(st);
     for (auto& g : ph.p2x) g.upload_tables(st);
+    for (auto& g : ph.pp1) g.upload_tables(st);
+    for (auto& g : ph.pp2) g.upload_tables(st);
   }
   if (P.dense_pairs) dense_pairs_build_split(P, st);
   TCI_HIP(hipStreamSynchronize(st));
@@ -517,10 +542,12 @@ void mixed_ladder(tci_plan& P, int precision, hipStream_t st) {
       ph.p2.launch_split(precision, st, ch.first, sc);
     } else {
       for (const auto& g : ph.p1x) g.launch_split(precision, st, ch.first, sc);
+      for (const auto& g : ph.pp1) g.launch_split(precision, st, ch.first, sc);
       for (const auto& g : ph.p2x) g.launch_split(precision, st, ch.first, sc);
+      for (const auto& g : ph.pp2) g.launch_split(precision, st, ch.first, sc);
     }
   }
-  if (P.profiling) TCI_HIP(hipEventRecord(P.ev[3], st));
+  if (P.timing()) TCI_HIP(hipEventRecord(P.ev[3], st));
   if (M.rpart_n > 1) ladder_reduce(M.Rpart.as<double>(), P.sum_d2, M.rpart_n, P.R.as<double>(), P.sum_d2, st);
 }
 

@@ -501,7 +501,17 @@ void tci_plan::build(const tci_problem& p) {
     device_bytes += (int64_t)b->bytes;
 }
 
+void tci_plan::drop_graphs() {
+  if (graphs.empty()) return;
+  // a replay may still be in flight on any stream
+  TCI_HIP(hipDeviceSynchronize());
+  for (auto& kv : graphs) (void)hipGraphExecDestroy(kv.second);
+  graphs.clear();
+}
+
 tci_plan::~tci_plan() {
+  for (auto& kv : graphs) (void)hipGraphExecDestroy(kv.second);
+  if (gstream) (void)hipStreamDestroy(gstream);
   for (hipEvent_t& e : ev)
     if (e) (void)hipEventDestroy(e);
 }
@@ -664,9 +674,9 @@ void tci_plan::build_two_phase_ladder(hipStream_t st) {
   }
 }
 
-void tci_plan::unpack_and_ladder(const double* x, int precision, hipStream_t st) {
-  launch_unpack(tab, dmax, x, C.as<double>(), Ct.as<double>(), s.as<double>(), st);
-  if (profiling) TCI_HIP(hipEventRecord(ev[2], st));
+void tci_plan::unpack_and_ladder(const double* x, const IoSlot* io, int precision, hipStream_t st) {
+  launch_unpack(tab, dmax, x, C.as<double>(), Ct.as<double>(), s.as<double>(), st, io);
+  if (timing()) TCI_HIP(hipEventRecord(ev[2], st));
   if (precision != TCI_SIGMA_FP64) {
     mixed_ladder(*this, precision, st);
     return;
@@ -689,7 +699,7 @@ void tci_plan::unpack_and_ladder(const double* x, int precision, hipStream_t st)
         for (const auto& g : ph.p2x) g.launch(st, ch.first);
       }
     }
-    if (profiling) TCI_HIP(hipEventRecord(ev[3], st));
+    if (timing()) TCI_HIP(hipEventRecord(ev[3], st));
     if (rpart_n > 1) ladder_reduce(Rpart.as<double>(), sum_d2, rpart_n, R.as<double>(), sum_d2, st);
     return;
   }
@@ -704,17 +714,14 @@ void tci_plan::unpack_and_ladder(const double* x, int precision, hipStream_t st)
   a.partial = lad_part.as<double>();
   a.chunk_stride = sum_d2;
   ladder.launch(a, st);
-  if (profiling) TCI_HIP(hipEventRecord(ev[3], st));
+  if (timing()) TCI_HIP(hipEventRecord(ev[3], st));
   ladder_reduce(lad_part.as<double>(), sum_d2, lad_chunks, R.as<double>(), sum_d2, st);
 }
 
-void tci_plan::sigma(const double* x, double* y, int flags, hipStream_t st) {
+void tci_plan::sigma_body(const double* x, double* y, const IoSlot* io, int flags, hipStream_t st) {
   const int precision = flags & 0xff;
-  TCI_REQUIRE(precision == TCI_SIGMA_FP64 || precision == TCI_SIGMA_BF16X3 || precision == TCI_SIGMA_FP16X3,
-              "unknown precision");
-  if (profiling) TCI_HIP(hipEventRecord(ev[0], st));
-  unpack_and_ladder(x, precision, st);
-  launch_rterms(tab, dmax, x, s.as<double>(), Kov.as<double>(), O3.as<double>(), R.as<double>(), st);
+  unpack_and_ladder(x, io, precision, st);
+  launch_rterms(tab, dmax, x, s.as<double>(), Kov.as<double>(), O3.as<double>(), R.as<double>(), st, io);
   progY.launch(st);
   if (dense_pairs) {
     dense_pairs_terms(*this, precision, st);
@@ -731,7 +738,65 @@ void tci_plan::sigma(const double* x, double* y, int flags, hipStream_t st) {
   launch_singles_reduce(tab, I, gpart_off.as<int64_t>(), gpart.as<double>(), gss.as<double>(), gW.as<double>(), n_split,
                         g.as<double>(), st);
   launch_ref_dot(tab, Kov.as<double>(), Ct.as<double>(), part.as<double>(), sig0.as<double>(), st);
-  launch_pack(tab, dmax, x, R.as<double>(), g.as<double>(), sig0.as<double>(), e_ref, flags, y, st);
+  launch_pack(tab, dmax, x, R.as<double>(), g.as<double>(), sig0.as<double>(), e_ref, flags, y, st, io);
+}
+
+// The whole build as one graph launch: the first call with a flag set captures
+// the launch sequence of sigma_body on a private stream (x and y read from the
+// io slot), later calls set the slot and replay.  Split precisions prepare
+// their operand copies eagerly first: a precision switch rebuilds those copies
+// and drops every captured graph.  Every buffer a graph touches belongs to the
+// plan and keeps its address for the plan's lifetime otherwise.
+void tci_plan::sigma_graph(const double* x, double* y, int flags, hipStream_t st) {
+  const int precision = flags & 0xff;
+  if (precision != TCI_SIGMA_FP64) mixed_prepare(*this, precision, st);
+  auto it = graphs.find(flags);
+  if (it == graphs.end()) {
+    if (!io.p) io.alloc(sizeof(IoSlot));
+    if (!gstream) TCI_HIP(hipStreamCreateWithFlags(&gstream, hipStreamNonBlocking));
+    hipGraph_t g = nullptr;
+    TCI_HIP(hipStreamBeginCapture(gstream, hipStreamCaptureModeThreadLocal));
+    capturing = true;
+    try {
+      sigma_body(nullptr, nullptr, io.as<IoSlot>(), flags, gstream);
+    } catch (...) {
+      capturing = false;
+      (void)hipStreamEndCapture(gstream, &g);
+      if (g) (void)hipGraphDestroy(g);
+      throw;
+    }
+    capturing = false;
+    TCI_HIP(hipStreamEndCapture(gstream, &g));
+    hipGraphExec_t ex = nullptr;
+    const hipError_t e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    TCI_HIP(e);
+    it = graphs.emplace(flags, ex).first;
+  }
+  if (profiling) TCI_HIP(hipEventRecord(ev[0], st));
+  launch_set_io(io.as<IoSlot>(), x, y, st);
+  TCI_HIP(hipGraphLaunch(it->second, st));
+  if (profiling) {
+    TCI_HIP(hipEventRecord(ev[1], st));
+    TCI_HIP(hipEventSynchronize(ev[1]));
+    float ms = 0.f;
+    TCI_HIP(hipEventElapsedTime(&ms, ev[0], ev[1]));
+    prof_sigma_ms[precision] += ms;
+    ++prof_sigma_n[precision];
+  }
+}
+
+void tci_plan::sigma(const double* x, double* y, int flags, hipStream_t st) {
+  const int precision = flags & 0xff;
+  TCI_REQUIRE(precision == TCI_SIGMA_FP64 || precision == TCI_SIGMA_BF16X3 || precision == TCI_SIGMA_FP16X3,
+              "unknown precision");
+  TCI_REQUIRE((flags & ~(0xff | TCI_ACCUMULATE | TCI_ADD_EREF | TCI_SIGMA_GRAPH)) == 0, "unknown sigma flag bits");
+  if (flags & TCI_SIGMA_GRAPH) {
+    sigma_graph(x, y, flags & ~TCI_SIGMA_GRAPH, st);
+    return;
+  }
+  if (profiling) TCI_HIP(hipEventRecord(ev[0], st));
+  sigma_body(x, y, nullptr, flags, st);
   if (profiling) {
     TCI_HIP(hipEventRecord(ev[1], st));
     TCI_HIP(hipEventSynchronize(ev[1]));

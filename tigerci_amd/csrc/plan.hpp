@@ -43,6 +43,7 @@ struct tci_plan {
     int nsplit = 1;                 // K-splits of phase 2 (partial R buffers)
     tci::GemmProgram p1, p2;        // dense programs (FP64, or split for the mixed ladder)
     std::vector<tci::GemmProgram> p1x, p2x;  // split per-walk programs, one per tile shape
+    std::vector<tci::GemmProgram> pp1, pp2;  // one program per walk (TCI_LADDER_PER_PAIR=1)
     tci::DevBuf lp_off;             // packed-L offsets of the local-domain pairs
   };
   tci::DevBuf Lp, pairs_local;      // packed L^J[D,D] of local-domain pairs (one chunk)
@@ -76,6 +77,7 @@ struct tci_plan {
     tci::DevBuf lp_off_all;         // offsets of the local pairs in a prepacked Lpx
     tci::DevBuf Cx, cx_off;         // split rows of the C_P blocks (of C_P^T when merged)
     bool merged = false;            // all walks full-domain: single products over all walks
+    bool per_pair = false;          // one launch per walk and phase (comparison mode)
     tci::DevBuf Tx;                 // T' of one chunk
     tci::DevBuf Rpart;              // partial R buffers of phase 2 K-splits
     int rpart_n = 1;
@@ -95,11 +97,21 @@ struct tci_plan {
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   double prof_ladder_ms[3] = {0, 0, 0}, prof_sigma_ms[3] = {0, 0, 0};
   int64_t prof_ladder_n[3] = {0, 0, 0}, prof_sigma_n[3] = {0, 0, 0};
+  // sigma builds captured as HIP graphs (TCI_SIGMA_GRAPH), one executable
+  // graph per flag set; the graphs read x and y from the device slot `io`
+  tci::DevBuf io;
+  hipStream_t gstream = nullptr;    // capture stream (default priority)
+  std::map<int, hipGraphExec_t> graphs;
+  bool capturing = false;
+  bool timing() const { return profiling && !capturing; }
+  void drop_graphs();
   ~tci_plan();
 
   void build(const tci_problem& p);
   void sigma(const double* x, double* y, int flags, hipStream_t st);
-  void unpack_and_ladder(const double* x, int precision, hipStream_t st);
+  void sigma_body(const double* x, double* y, const tci::IoSlot* io, int flags, hipStream_t st);
+  void sigma_graph(const double* x, double* y, int flags, hipStream_t st);
+  void unpack_and_ladder(const double* x, const tci::IoSlot* io, int precision, hipStream_t st);
   void build_two_phase_ladder(hipStream_t st);
 };
 

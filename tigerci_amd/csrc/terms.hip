@@ -32,7 +32,8 @@ __device__ __forceinline__ void tri_decode(int64_t idx, int& hi, int& lo) {
 }
 
 __global__ void unpack_kernel(PairTab T, const double* __restrict__ x, double* __restrict__ C,
-                              double* __restrict__ Ct, double* __restrict__ s) {
+                              double* __restrict__ Ct, double* __restrict__ s, const IoSlot* __restrict__ io) {
+  if (io) x = io->x;
   const int P = blockIdx.y;
   if (P == T.n_pairs) {  // singles
     const int64_t n = (int64_t)T.n_int * T.n_v;
@@ -72,7 +73,8 @@ __global__ void unpack_kernel(PairTab T, const double* __restrict__ x, double* _
 
 __global__ void rterms_kernel(PairTab T, const double* __restrict__ x, const double* __restrict__ s,
                               const double* __restrict__ Kov, const double* __restrict__ O3,
-                              double* __restrict__ R) {
+                              double* __restrict__ R, const IoSlot* __restrict__ io) {
+  if (io) x = io->x;
   const int P = blockIdx.y;
   const int i = T.pairs[2 * P], j = T.pairs[2 * P + 1];
   const int64_t off = T.dom_off[P];
@@ -219,9 +221,20 @@ __global__ void singles_reduce_kernel(PairTab T, InvTab I, const int64_t* __rest
   }
 }
 
+__global__ void set_io_kernel(IoSlot* io, const double* x, double* y) {
+  if (threadIdx.x == 0) {
+    io->x = x;
+    io->y = y;
+  }
+}
+
 __global__ void pack_kernel(PairTab T, const double* __restrict__ x, const double* __restrict__ R,
                             const double* __restrict__ g, const double* __restrict__ sig0, double e_ref,
-                            int flags, double* __restrict__ sigma) {
+                            int flags, double* __restrict__ sigma, const IoSlot* __restrict__ io) {
+  if (io) {
+    x = io->x;
+    sigma = io->y;
+  }
   const bool accum = (flags & TCI_ACCUMULATE) != 0;
   const double eadd = (flags & TCI_ADD_EREF) ? e_ref : 0.0;
   const int P = blockIdx.y;
@@ -451,18 +464,18 @@ inline unsigned gx(int64_t work) { return (unsigned)std::min<int64_t>(std::max<i
 }  // namespace
 
 void launch_unpack(const PairTab& T, int dmax, const double* x, double* C, double* Ct, double* s,
-                   hipStream_t st) {
+                   hipStream_t st, const IoSlot* io) {
   const int64_t w = std::max<int64_t>((int64_t)dmax * dmax, (int64_t)T.n_int * T.n_v);
   hipLaunchKernelGGL(unpack_kernel, dim3(std::min<unsigned>(gx(w), 256), T.n_pairs + 1), dim3(256), 0, st, T, x, C,
-                     Ct, s);
+                     Ct, s, io);
   TCI_HIP(hipGetLastError());
 }
 
 void launch_rterms(const PairTab& T, int dmax, const double* x, const double* s, const double* Kov,
-                   const double* O3, double* R, hipStream_t st) {
+                   const double* O3, double* R, hipStream_t st, const IoSlot* io) {
   if (T.n_pairs == 0) return;
   hipLaunchKernelGGL(rterms_kernel, dim3(std::min<unsigned>(gx((int64_t)dmax * dmax), 256), T.n_pairs), dim3(256),
-                     0, st, T, x, s, Kov, O3, R);
+                     0, st, T, x, s, Kov, O3, R, io);
   TCI_HIP(hipGetLastError());
 }
 
@@ -498,10 +511,15 @@ void launch_singles_reduce(const PairTab& T, const InvTab& I, const int64_t* gpa
 }
 
 void launch_pack(const PairTab& T, int dmax, const double* x, const double* R, const double* g, const double* sig0,
-                 double e_ref, int flags, double* sigma, hipStream_t st) {
+                 double e_ref, int flags, double* sigma, hipStream_t st, const IoSlot* io) {
   const int64_t w = std::max<int64_t>((int64_t)dmax * dmax, 1 + (int64_t)T.n_int * T.n_v);
   hipLaunchKernelGGL(pack_kernel, dim3(std::min<unsigned>(gx(w), 256), T.n_pairs + 1), dim3(256), 0, st, T, x, R, g,
-                     sig0, e_ref, flags, sigma);
+                     sig0, e_ref, flags, sigma, io);
+  TCI_HIP(hipGetLastError());
+}
+
+void launch_set_io(IoSlot* io, const double* x, double* y, hipStream_t st) {
+  hipLaunchKernelGGL(set_io_kernel, dim3(1), dim3(64), 0, st, io, x, y);
   TCI_HIP(hipGetLastError());
 }
 

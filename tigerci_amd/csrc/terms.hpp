@@ -19,13 +19,21 @@ struct PairTab {
   const double* eps;       // [n_int + n_v]
 };
 
+// Device slot holding the CSF vectors of a sigma build.  A sigma replayed from
+// a HIP graph reads x and y from here, so one captured graph serves any pair
+// of vectors; the eager path passes io = nullptr and the pointers directly.
+struct IoSlot {
+  const double* x;
+  double* y;
+};
+
 // x (CSF vector) -> dense C, Ct = 2C - C^T blocks, singles s = x_s / sqrt2
 void launch_unpack(const PairTab& T, int dmax, const double* x, double* C, double* Ct, double* s,
-                   hipStream_t st);
+                   hipStream_t st, const IoSlot* io = nullptr);
 
 // R += c0 K^{ij} - sum_k [(ki|bj) s_k^a + (kj|ai) s_k^b]   (c0 read from x[0])
 void launch_rterms(const PairTab& T, int dmax, const double* x, const double* s, const double* Kov,
-                   const double* O3, double* R, hipStream_t st);
+                   const double* O3, double* R, hipStream_t st, const IoSlot* io = nullptr);
 
 // R_P += sum_Q [(ki|lj) C_Q(pos) + (li|kj) C_Q(pos)^T] over domain intersections
 struct HoleTab {
@@ -55,7 +63,11 @@ void launch_singles_reduce(const PairTab& T, const InvTab& I, const int64_t* gpa
 
 // sigma = S^T R (with CSF factors) + Fock diagonal + optional E_ref; flags as tci_sigma_apply
 void launch_pack(const PairTab& T, int dmax, const double* x, const double* R, const double* g,
-                 const double* sig0, double e_ref, int flags, double* sigma, hipStream_t st);
+                 const double* sig0, double e_ref, int flags, double* sigma, hipStream_t st,
+                 const IoSlot* io = nullptr);
+
+// io->x = x, io->y = y, in stream order (before a graph replay)
+void launch_set_io(IoSlot* io, const double* x, double* y, hipStream_t st);
 
 // exact diagonal H_kk - E_ref
 void launch_diag(const PairTab& T, int dmax, const double* Kov, const double* Jvv, const double* O4,

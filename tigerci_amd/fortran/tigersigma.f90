@@ -14,7 +14,7 @@ module tigersigma
                                        TCI_ERR_OOM = 3, TCI_ERR_NUMERIC = 4
   integer(c_int32_t), parameter, public :: TCI_F64 = 0, TCI_F32 = 1
   integer(c_int32_t), parameter, public :: TCI_SIGMA_FP64 = 0, TCI_SIGMA_BF16X3 = 1
-  integer(c_int32_t), parameter, public :: TCI_ACCUMULATE = 256, TCI_ADD_EREF = 512
+  integer(c_int32_t), parameter, public :: TCI_ACCUMULATE = 256, TCI_ADD_EREF = 512, TCI_SIGMA_GRAPH = 1024
 
   type, bind(c), public :: tci_problem
     integer(c_int64_t) :: n_int, n_v, n_chol
@@ -65,7 +65,7 @@ module tigersigma
       type(c_ptr), value :: plan
       type(tci_plan_info), intent(out) :: info
     end function
-    ! device vectors c, sigma; flags = tci_precision + TCI_ACCUMULATE / TCI_ADD_EREF
+    ! device vectors c, sigma; flags = tci_precision + TCI_ACCUMULATE / TCI_ADD_EREF / TCI_SIGMA_GRAPH
     integer(c_int) function tci_sigma_apply(plan, c, sigma, n, flags, stream) bind(c, name="tci_sigma_apply")
       import :: c_int, c_ptr, c_int64_t, c_int32_t
       type(c_ptr), value :: plan, c, sigma, stream
