@@ -150,6 +150,7 @@ def main() -> int:
         return r
 
     cfg["sigma_graph"] = a.graph
+    cfg["autotune"] = os.environ.get("TCI_AUTOTUNE", "1") != "0"
     if a.mode == "sigma":
         x = tci.DeviceArray.from_numpy(random_vector(plan.n_csf, a.seed))
         y = tci.DeviceArray(plan.n_csf)
@@ -163,6 +164,7 @@ def main() -> int:
         tci.synchronize()
         dt = (time.perf_counter() - t0) / a.steps
         pr = ladder_probe(a.precision, a.steps) if a.graph else plan.profile_read(a.precision)
+        cfg["schedule"] = plan.tuning().strip().splitlines()
         lad = pr["ladder_ms"] / max(pr["ladder_launches"], 1) / 1e3
         ach = info.flops_ladder / lad
         if a.precision != "fp64":  # precision study: this sigma against the FP64 one
@@ -245,6 +247,7 @@ def main() -> int:
         "sigma_check": sig_err,
         "sigma_tflops_fp64": info.flops_sigma / sig64 / 1e12,
     })
+    out["config"]["schedule"] = plan.tuning().strip().splitlines()
     tb64 = measured_traffic(a.workload, "fp64")
     roof64 = {"bound": "mfma", "achieved": ach / 1e12, "peak": PEAK["fp64"] / 1e12, "unit": "TFLOP/s",
               "frac": ach / PEAK["fp64"], "traffic": tb64, "traffic_gbs": tb64 / lad / 1e9 if tb64 else None,

@@ -161,6 +161,13 @@ int tci_profile_enable(tci_plan* plan, int32_t on);
 int tci_profile_read(const tci_plan* plan, int32_t precision, double* ladder_ms,
                      int64_t* ladder_launches, double* sigma_ms, int64_t* sigma_calls);
 
+/* Launch schedules the plan picked by timing candidates on the device: K-split
+ * fill targets and workgroup orders of the ladder programs (FP64 at plan
+ * creation, split precisions when first used), one text line per ladder.
+ * TCI_AUTOTUNE=0 in the environment keeps fixed defaults instead.  Copies at
+ * most len-1 characters plus a terminating 0.  No reference counterpart. */
+int tci_plan_tuning(const tci_plan* plan, char* buf, int64_t len);
+
 /* Checks the MFMA operand/result lane layouts the kernels assume against a
  * scalar product on the device; 0 when they agree.  No reference counterpart. */
 int tci_selftest_mfma(void);

@@ -125,11 +125,13 @@ def test_per_pair_launch_matches_grouped():
     prob = make_problem(5, 60, 30, 9, domain_range=(10, 60), weak_below=12)
     x = random_ci_vector(prob, 5)
     # the split ladder is laid out on a plan's first split sigma, which is
-    # when the launch mode is read
-    a = _sigma(tci.Plan.from_problem(prob), x, precision="bf16x3")
-    os.environ["TCI_LADDER_PER_PAIR"] = "1"
+    # when the launch mode is read; both plans keep the untuned K-splits
+    os.environ["TCI_AUTOTUNE"] = "0"
     try:
+        a = _sigma(tci.Plan.from_problem(prob), x, precision="bf16x3")
+        os.environ["TCI_LADDER_PER_PAIR"] = "1"
         b = _sigma(tci.Plan.from_problem(prob), x, precision="bf16x3")
     finally:
-        del os.environ["TCI_LADDER_PER_PAIR"]
+        os.environ.pop("TCI_LADDER_PER_PAIR", None)
+        del os.environ["TCI_AUTOTUNE"]
     assert np.array_equal(a, b)

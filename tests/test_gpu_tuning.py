@@ -1,0 +1,66 @@
+"""Autotuned launch schedules.
+
+A plan times K-split fill targets and workgroup orders of its ladder programs
+on the device (FP64 at creation, split precisions on first use) and keeps the
+fastest.  The schedule changes the order of FP64 partial sums only, so tuned
+and untuned plans agree to rounding, and both reproduce the oracle.
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import tigerci_amd as tci
+from oracle.problem import make_problem, random_ci_vector
+from oracle.sigma import Integrals, sigma as oracle_sigma
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def _sigma(plan, x, **kw):
+    c = tci.DeviceArray.from_numpy(x)
+    out = tci.DeviceArray(plan.n_csf)
+    plan.sigma(c, out, **kw)
+    return out.numpy()
+
+
+def _rel(a, b):
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+@pytest.mark.parametrize("args", [
+    dict(n_int=5, n_v=60, n_chol=40, seed=51, domain_range=(10, 60), weak_below=12),
+    dict(n_int=4, n_v=40, n_chol=30, seed=52),
+])
+def test_tuned_schedule_matches_default(args):
+    _say("case: tuned vs default schedule", args)
+    prob = make_problem(**args)
+    ints = Integrals(prob)
+    x = random_ci_vector(prob, 9)
+    ref = oracle_sigma(prob, x, ints)
+    tuned = tci.Plan.from_problem(prob)
+    os.environ["TCI_AUTOTUNE"] = "0"
+    try:
+        fixed = tci.Plan.from_problem(prob)
+        f64, fmx = _sigma(fixed, x), _sigma(fixed, x, precision="bf16x3")
+    finally:
+        del os.environ["TCI_AUTOTUNE"]
+    t64, tmx = _sigma(tuned, x), _sigma(tuned, x, precision="bf16x3")
+    log = tuned.tuning()
+    _say(log)
+    assert "fp64 ladder" in log and "split ladder" in log
+    assert fixed.tuning() == ""
+    assert _rel(t64, ref) < 1e-10 and _rel(f64, ref) < 1e-10
+    assert _rel(t64, f64) < 1e-12
+    assert _rel(tmx, ref) < 5e-5 and _rel(fmx, ref) < 5e-5
+    # the schedule is kept: repeated and re-prepared sigmas are identical
+    assert np.array_equal(_sigma(tuned, x, precision="bf16x3"), tmx)
+    _sigma(tuned, x, precision="fp16x3")
+    assert np.array_equal(_sigma(tuned, x, precision="bf16x3"), tmx)
+    assert tuned.tuning().count("split ladder") == 1

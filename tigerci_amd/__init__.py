@@ -94,11 +94,12 @@ def load_library(path: str | None = None) -> ctypes.CDLL:
     lib.tci_profile_enable.argtypes = [vp, i32]
     lib.tci_profile_read.argtypes = [vp, i32, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(i64),
                                      ctypes.POINTER(ctypes.c_double), ctypes.POINTER(i64)]
+    lib.tci_plan_tuning.argtypes = [vp, ctypes.c_char_p, i64]
     lib.tci_last_error.restype = ctypes.c_char_p
     for f in ("tci_plan_create", "tci_plan_get_info", "tci_sigma_apply", "tci_htimesc", "tci_diagonal",
               "tci_davidson_run", "tci_selftest_mfma", "tci_device_alloc", "tci_device_free", "tci_memcpy_h2d",
               "tci_memcpy_d2h", "tci_memcpy_d2d", "tci_device_synchronize", "tci_profile_enable",
-              "tci_profile_read"):
+              "tci_profile_read", "tci_plan_tuning"):
         getattr(lib, f).restype = ctypes.c_int
     if path is None:
         _lib = lib
@@ -263,6 +264,12 @@ class Plan:
         y = DeviceArray(self.n_csf)
         self.sigma(c, y, precision=precision)
         return y.numpy()
+
+    def tuning(self) -> str:
+        """Launch schedules picked by the plan's autotuning (empty when off)."""
+        buf = ctypes.create_string_buffer(4096)
+        _check(self._lib.tci_plan_tuning(self._h, buf, len(buf)))
+        return buf.value.decode()
 
     def profile(self, on: bool = True) -> None:
         _check(self._lib.tci_profile_enable(self._h, 1 if on else 0))

@@ -6,6 +6,7 @@
 #include <hip/hip_fp16.h>
 #include <cstdint>
 #include <cstdio>
+#include <functional>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -88,6 +89,23 @@ void upload(DevBuf& b, const std::vector<T>& h, hipStream_t s = nullptr) {
 
 inline int64_t cdiv(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
+// Workgroups of a launch are dispatched round-robin over the 8 XCDs of the
+// chip (workgroup b runs on XCD b % 8), and every XCD has an L2 of its own.
+// Tasks listed next to each other in a program share operand panels (the
+// tiles of one output block, the K-splits of one walk), so task lists are
+// dealt out in contiguous ranges: XCD x runs tasks [start_x, start_x + count_x)
+// in order, and neighbouring tiles meet in one L2 instead of eight.
+constexpr int kXcds = 8;
+__device__ __forceinline__ int64_t xcd_task(int64_t b, int64_t G) {
+  const int64_t q = G / kXcds, r = G % kXcds, x = b % kXcds, i = b / kXcds;
+  return (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + i;
+}
+// Order of a program that has not been tuned: TCI_XCD_REMAP=0/1 forces plain
+// or XCD-contiguous order for every program (and switches tuning of the order
+// off); the default is plain order.
+int xcd_remap_default();
+bool xcd_remap_forced();
+
 // Load through the global address space.  Pointers read from launch tables are
 // generic to the compiler, which would otherwise emit flat loads; those also
 // count against the LDS wait counter and would make every LDS read wait for
@@ -148,6 +166,7 @@ struct GemmProgram {
   bool dense = false;  // contiguous row-major operands only: dense_gemm.hip kernel (tile 64 or 128)
   bool split = false;  // split-row operands, C = sum A B^T: dense_split.hip kernel (tile 64 or 128)
   bool scaled = false; // some task has an output scale (set by upload_tables)
+  int remap = -1;      // workgroup -> task order: 0 plain, 1 XCD-contiguous, -1 default
   std::vector<Task> tasks;
   std::vector<Seg> segs;
   DevBuf d_tasks, d_segs;
@@ -161,10 +180,21 @@ struct GemmProgram {
   void launch_split(int precision, hipStream_t st, int64_t J0 = 0, const float* scales = nullptr) const;
 };
 
-void launch_ggemm(int tile, const Task* tasks, const Seg* segs, int64_t ntasks, int64_t J0, hipStream_t st);
-void launch_dgemm(int tile, const Task* tasks, const Seg* segs, int64_t ntasks, int64_t J0, hipStream_t st);
+// Autotuning helpers (tune.hip).  time_ms: mean duration in ms of `reps` runs
+// of fn on st after one untimed run (HIP events).  tune_order: sets g.remap to
+// the faster workgroup order of g's launch and returns it (unchanged when the
+// order is forced from the environment).  autotune_enabled: TCI_AUTOTUNE != 0.
+bool autotune_enabled();
+std::string fmt_ms(double ms);
+double time_ms(hipStream_t st, int reps, const std::function<void()>& fn);
+int tune_order(GemmProgram& g, hipStream_t st, const std::function<void()>& launch);
+
+void launch_ggemm(int tile, const Task* tasks, const Seg* segs, int64_t ntasks, int64_t J0, hipStream_t st,
+                  int remap);
+void launch_dgemm(int tile, const Task* tasks, const Seg* segs, int64_t ntasks, int64_t J0, hipStream_t st,
+                  int remap);
 void launch_xgemm(int precision, int tm, int tn, const Task* tasks, const Seg* segs, int64_t ntasks, int64_t J0,
-                  const float* scales, hipStream_t st);
+                  const float* scales, hipStream_t st, int remap);
 
 // ---------------------------------------------------------------------------
 // Split-row storage of a matrix for the split-precision GEMM.  A row of K

@@ -168,9 +168,10 @@ __device__ __forceinline__ void dgemm_body(const Task& t, const Seg* __restrict_
 
 template <int TS>
 __global__ __launch_bounds__(256, TS == 128 ? 2 : 4) void dgemm_kernel(const Task* __restrict__ tasks,
-                                                                       const Seg* __restrict__ segs, int64_t J0) {
+                                                                       const Seg* __restrict__ segs, int64_t J0,
+                                                                       int remap) {
   constexpr int WB = TS / 32;  // 16x16 blocks per wave side
-  const Task t = tasks[blockIdx.x];
+  const Task t = tasks[remap ? xcd_task(blockIdx.x, gridDim.x) : blockIdx.x];
   __shared__ double smem[4 * DK * (TS + 2)];
   const int tid = threadIdx.x;
   const int lane = tid & 63, w = __builtin_amdgcn_readfirstlane(tid >> 6);
@@ -211,11 +212,14 @@ __global__ __launch_bounds__(256, TS == 128 ? 2 : 4) void dgemm_kernel(const Tas
 
 }  // namespace
 
-void launch_dgemm(int tile, const Task* tasks, const Seg* segs, int64_t ntasks, int64_t J0, hipStream_t st) {
+void launch_dgemm(int tile, const Task* tasks, const Seg* segs, int64_t ntasks, int64_t J0, hipStream_t st,
+                  int remap) {
   if (ntasks <= 0) return;
   TCI_REQUIRE(ntasks < ((int64_t)1 << 31), "too many gemm tasks");
-  if (tile == 128) hipLaunchKernelGGL(dgemm_kernel<128>, dim3((unsigned)ntasks), dim3(256), 0, st, tasks, segs, J0);
-  else hipLaunchKernelGGL(dgemm_kernel<64>, dim3((unsigned)ntasks), dim3(256), 0, st, tasks, segs, J0);
+  if (tile == 128)
+    hipLaunchKernelGGL(dgemm_kernel<128>, dim3((unsigned)ntasks), dim3(256), 0, st, tasks, segs, J0, remap);
+  else
+    hipLaunchKernelGGL(dgemm_kernel<64>, dim3((unsigned)ntasks), dim3(256), 0, st, tasks, segs, J0, remap);
   TCI_HIP(hipGetLastError());
 }
 

@@ -239,9 +239,9 @@ __device__ __forceinline__ void xgemm_body(const Task& t, const Seg* __restrict_
 template <int TM, int TN, int PREC>
 __global__ __launch_bounds__(256, TM == 64 ? 4 : 2) void xgemm_kernel(const Task* __restrict__ tasks,
                                                                       const Seg* __restrict__ segs, int64_t J0,
-                                                                      const float* __restrict__ scales) {
+                                                                      const float* __restrict__ scales, int remap) {
   constexpr int WBM = TM / 32, WBN = TN / 32;  // 16x16 blocks per wave side
-  const Task t = tasks[blockIdx.x];
+  const Task t = tasks[remap ? xcd_task(blockIdx.x, gridDim.x) : blockIdx.x];
   __shared__ __attribute__((aligned(16))) uint16_t sm[2 * 2 * (TM + TN) * XK];
   const int tid = threadIdx.x;
   const int lane = tid & 63, w = __builtin_amdgcn_readfirstlane(tid >> 6);
@@ -288,30 +288,31 @@ __global__ __launch_bounds__(256, TM == 64 ? 4 : 2) void xgemm_kernel(const Task
 
 template <int PREC>
 void launch_xgemm_t(int tm, int tn, const Task* tasks, const Seg* segs, int64_t ntasks, int64_t J0, const float* scales,
-                    hipStream_t st) {
+                    hipStream_t st, int remap) {
   const dim3 g((unsigned)ntasks), b(256);
-  if (tm == 128 && tn == 128) hipLaunchKernelGGL((xgemm_kernel<128, 128, PREC>), g, b, 0, st, tasks, segs, J0, scales);
-  else if (tm == 128 && tn == 192) hipLaunchKernelGGL((xgemm_kernel<128, 192, PREC>), g, b, 0, st, tasks, segs, J0, scales);
-  else if (tm == 128 && tn == 64) hipLaunchKernelGGL((xgemm_kernel<128, 64, PREC>), g, b, 0, st, tasks, segs, J0, scales);
-  else if (tm == 64 && tn == 64) hipLaunchKernelGGL((xgemm_kernel<64, 64, PREC>), g, b, 0, st, tasks, segs, J0, scales);
+  if (tm == 128 && tn == 128) hipLaunchKernelGGL((xgemm_kernel<128, 128, PREC>), g, b, 0, st, tasks, segs, J0, scales, remap);
+  else if (tm == 128 && tn == 192) hipLaunchKernelGGL((xgemm_kernel<128, 192, PREC>), g, b, 0, st, tasks, segs, J0, scales, remap);
+  else if (tm == 128 && tn == 64) hipLaunchKernelGGL((xgemm_kernel<128, 64, PREC>), g, b, 0, st, tasks, segs, J0, scales, remap);
+  else if (tm == 64 && tn == 64) hipLaunchKernelGGL((xgemm_kernel<64, 64, PREC>), g, b, 0, st, tasks, segs, J0, scales, remap);
   else throw Error(TCI_ERR_ARG, "unsupported split GEMM tile " + std::to_string(tm) + "x" + std::to_string(tn));
 }
 
 void launch_xgemm(int precision, int tm, int tn, const Task* tasks, const Seg* segs, int64_t ntasks, int64_t J0,
-                  const float* scales, hipStream_t st) {
+                  const float* scales, hipStream_t st, int remap) {
   TCI_REQUIRE(precision == TCI_SIGMA_BF16X3 || precision == TCI_SIGMA_FP16X3, "split GEMM needs a split precision");
   if (ntasks <= 0) return;
   TCI_REQUIRE(ntasks < ((int64_t)1 << 31), "too many gemm tasks");
-  if (precision == TCI_SIGMA_BF16X3) launch_xgemm_t<TCI_SIGMA_BF16X3>(tm, tn, tasks, segs, ntasks, J0, scales, st);
-  else launch_xgemm_t<TCI_SIGMA_FP16X3>(tm, tn, tasks, segs, ntasks, J0, scales, st);
+  if (precision == TCI_SIGMA_BF16X3) launch_xgemm_t<TCI_SIGMA_BF16X3>(tm, tn, tasks, segs, ntasks, J0, scales, st, remap);
+  else launch_xgemm_t<TCI_SIGMA_FP16X3>(tm, tn, tasks, segs, ntasks, J0, scales, st, remap);
   TCI_HIP(hipGetLastError());
 }
 
 void GemmProgram::launch_split(int precision, hipStream_t st, int64_t J0, const float* scales) const {
   TCI_REQUIRE(split, "not a split program");
   TCI_REQUIRE(!scaled || scales, "scaled split tasks need a scales array");
+  const int rm = remap < 0 || xcd_remap_forced() ? xcd_remap_default() : remap;
   launch_xgemm(precision, tile, tile_n > 0 ? tile_n : tile, d_tasks.as<Task>(), d_segs.as<Seg>(),
-               (int64_t)tasks.size(), J0, scales, st);
+               (int64_t)tasks.size(), J0, scales, st, rm);
 }
 
 }  // namespace tci

@@ -19,6 +19,8 @@
 // MFMA 16x16x4 f64 lane layout (checked by tci_selftest_mfma):
 //   A[i][k] at lane i + 16k, B[k][j] at lane j + 16k,
 //   D[i][j] at lane j + 16*(i%4), register i/4.
+#include <cstdlib>
+
 #include "common.hpp"
 
 namespace tci {
@@ -41,8 +43,8 @@ __device__ __forceinline__ double fetch(const Op& o, int64_t J0, int r, int k, i
 constexpr int TM = 64, TN = 64, TK = 16;
 
 __global__ __launch_bounds__(256) void ggemm64_kernel(const Task* __restrict__ tasks, const Seg* __restrict__ segs,
-                                                      int64_t J0) {
-  const Task t = tasks[blockIdx.x];
+                                                      int64_t J0, int remap) {
+  const Task t = tasks[remap ? xcd_task(blockIdx.x, gridDim.x) : blockIdx.x];
   __shared__ double As[TK][TM + 1];
   __shared__ double Bs[TK][TN + 1];
   const int tid = threadIdx.x;
@@ -332,8 +334,8 @@ __device__ __forceinline__ void ggemm128_body(const Task& t, const Seg* __restri
 }
 
 __global__ __launch_bounds__(256, 1) void ggemm128_kernel(const Task* __restrict__ tasks,
-                                                          const Seg* __restrict__ segs, int64_t J0) {
-  const Task t = tasks[blockIdx.x];
+                                                          const Seg* __restrict__ segs, int64_t J0, int remap) {
+  const Task t = tasks[remap ? xcd_task(blockIdx.x, gridDim.x) : blockIdx.x];
   __shared__ double smem[4 * BK * LP];
   const int tid = threadIdx.x;
   const int lane = tid & 63, w = __builtin_amdgcn_readfirstlane(tid >> 6);
@@ -370,13 +372,27 @@ __global__ __launch_bounds__(256, 1) void ggemm128_kernel(const Task* __restrict
 
 }  // namespace
 
-void launch_ggemm(int tile, const Task* tasks, const Seg* segs, int64_t ntasks, int64_t J0, hipStream_t st) {
+namespace {
+int remap_env() {
+  static const int v = [] {
+    const char* e = std::getenv("TCI_XCD_REMAP");
+    return e && (e[0] == '0' || e[0] == '1') ? e[0] - '0' : -1;
+  }();
+  return v;
+}
+}  // namespace
+
+int xcd_remap_default() { return remap_env() < 0 ? 0 : remap_env(); }
+bool xcd_remap_forced() { return remap_env() >= 0; }
+
+void launch_ggemm(int tile, const Task* tasks, const Seg* segs, int64_t ntasks, int64_t J0, hipStream_t st,
+                  int remap) {
   if (ntasks <= 0) return;
   TCI_REQUIRE(ntasks < ((int64_t)1 << 31), "too many gemm tasks");
   if (tile == 128)
-    hipLaunchKernelGGL(ggemm128_kernel, dim3((unsigned)ntasks), dim3(256), 0, st, tasks, segs, J0);
+    hipLaunchKernelGGL(ggemm128_kernel, dim3((unsigned)ntasks), dim3(256), 0, st, tasks, segs, J0, remap);
   else
-    hipLaunchKernelGGL(ggemm64_kernel, dim3((unsigned)ntasks), dim3(256), 0, st, tasks, segs, J0);
+    hipLaunchKernelGGL(ggemm64_kernel, dim3((unsigned)ntasks), dim3(256), 0, st, tasks, segs, J0, remap);
   TCI_HIP(hipGetLastError());
 }
 
@@ -437,8 +453,9 @@ void GemmProgram::upload_tables(hipStream_t st) {
 
 void GemmProgram::launch(hipStream_t st, int64_t J0) const {
   TCI_REQUIRE(!split, "split program launched without a precision");
-  if (dense) launch_dgemm(tile, d_tasks.as<Task>(), d_segs.as<Seg>(), (int64_t)tasks.size(), J0, st);
-  else launch_ggemm(tile, d_tasks.as<Task>(), d_segs.as<Seg>(), (int64_t)tasks.size(), J0, st);
+  const int rm = remap < 0 || xcd_remap_forced() ? xcd_remap_default() : remap;
+  if (dense) launch_dgemm(tile, d_tasks.as<Task>(), d_segs.as<Seg>(), (int64_t)tasks.size(), J0, st, rm);
+  else launch_ggemm(tile, d_tasks.as<Task>(), d_segs.as<Seg>(), (int64_t)tasks.size(), J0, st, rm);
 }
 
 }  // namespace tci

@@ -200,6 +200,36 @@ void launch_split_scales(int mode, const double* x, int64_t n, unsigned* mx, int
   TCI_HIP(hipGetLastError());
 }
 
+// One chunk of Cholesky vectors of the split ladder: pack L^J[D,D] of the
+// local-domain walks unless prepacked, then phase 1 and phase 2.
+void mixed_ladder_chunk(tci_plan& P, int precision, const std::pair<int64_t, int>& ch, hipStream_t st) {
+  tci_plan::MixedLadder& M = P.mx;
+  const int f16 = precision == TCI_SIGMA_FP16X3 ? 1 : 0;
+  const float* sc = M.scales.as<float>();
+  const tci_plan::LadderPhase& ph = M.phase[ch.second];
+  if (M.n_local && !M.prepacked) {
+    const unsigned gxb = 1024;
+    if (P.lvv_f32)
+      hipLaunchKernelGGL(pack_L_split_kernel<float>, dim3(gxb, M.n_local), dim3(256), 0, st, P.tab,
+                         M.pairs_local.as<int32_t>(), ph.lp_off.as<int64_t>(), static_cast<const float*>(P.Lvv.p),
+                         ch.first, (int)ph.Lc, M.Lpx.as<uint16_t>(), sc, f16);
+    else
+      hipLaunchKernelGGL(pack_L_split_kernel<double>, dim3(gxb, M.n_local), dim3(256), 0, st, P.tab,
+                         M.pairs_local.as<int32_t>(), ph.lp_off.as<int64_t>(), static_cast<const double*>(P.Lvv.p),
+                         ch.first, (int)ph.Lc, M.Lpx.as<uint16_t>(), sc, f16);
+    TCI_HIP(hipGetLastError());
+  }
+  if (M.merged) {
+    ph.p1.launch_split(precision, st, ch.first, sc);
+    ph.p2.launch_split(precision, st, ch.first, sc);
+  } else {
+    for (const auto& g : ph.p1x) g.launch_split(precision, st, ch.first, sc);
+    for (const auto& g : ph.pp1) g.launch_split(precision, st, ch.first, sc);
+    for (const auto& g : ph.p2x) g.launch_split(precision, st, ch.first, sc);
+    for (const auto& g : ph.pp2) g.launch_split(precision, st, ch.first, sc);
+  }
+}
+
 void mixed_prepare(tci_plan& P, int precision, hipStream_t st) {
   tci_plan::MixedLadder& M = P.mx;
   TCI_REQUIRE(precision == TCI_SIGMA_BF16X3 || precision == TCI_SIGMA_FP16X3, "not a split precision");
@@ -322,176 +352,231 @@ void mixed_prepare(tci_plan& P, int precision, hipStream_t st) {
     M.per_pair = pp && pp[0] == '1';
     if (M.per_pair) M.merged = false;
   }
-  const int tile2 = 128;
-  int64_t tiles2 = 0;
-  for (int Q = 0; Q < n_pairs; ++Q) {
-    const int64_t d = P.h_dom_off[Q + 1] - P.h_dom_off[Q];
-    tiles2 += cdiv(d, tile2) * cdiv(d, tile2);
-  }
-  if (M.merged) tiles2 = cdiv((int64_t)n_pairs * n_v, tile2) * cdiv(n_v, tile2);
-  M.rpart_n = (int)std::max<int64_t>(1, std::min<int64_t>(16, cdiv(512, std::max<int64_t>(tiles2, 1))));
-  // per-walk tile shapes (local domains): phase 2 of walk Q runs on tiles
-  // t2m[v2] x t2n[v2]; every shape gets enough K-splits to fill the chip in its
-  // own launch
-  static const int t2m[3] = {64, 128, 128}, t2n[3] = {64, 128, 192};
-  std::vector<int> v2of(n_pairs, 1);
-  int nsplit_v[3] = {1, 1, 1};
-  if (!M.merged) {
-    int64_t tv[3] = {0, 0, 0};
-    for (int Q = 0; Q < n_pairs; ++Q) {
-      const int64_t d = P.h_dom_off[Q + 1] - P.h_dom_off[Q];
-      const int v = d <= 64 ? 0 : (d <= 128 ? 1 : ((double)d / (192.0 * cdiv(d, 192)) >= (double)d / (128.0 * cdiv(d, 128)) ? 2 : 1));
-      v2of[Q] = v;
-      tv[v] += cdiv(d, t2m[v]) * cdiv(d, t2n[v]);
+  // programs for a K-split fill target: phase 2 of each tile shape is split
+  // along K until it has about `target` workgroups (at most 16 splits)
+  auto build = [&](int64_t target) {
+    const int tile2 = 128;
+    const int64_t tiles2m = cdiv((int64_t)n_pairs * n_v, tile2) * cdiv(n_v, tile2);
+    M.rpart_n = (int)std::max<int64_t>(1, std::min<int64_t>(16, cdiv(target, std::max<int64_t>(tiles2m, 1))));
+    // per-walk tile shapes (local domains): phase 2 of walk Q runs on tiles
+    // t2m[v2] x t2n[v2]; every shape gets enough K-splits to fill the chip in its
+    // own launch
+    static const int t2m[3] = {64, 128, 128}, t2n[3] = {64, 128, 192};
+    std::vector<int> v2of(n_pairs, 1);
+    int nsplit_v[3] = {1, 1, 1};
+    if (!M.merged) {
+      int64_t tv[3] = {0, 0, 0};
+      for (int Q = 0; Q < n_pairs; ++Q) {
+        const int64_t d = P.h_dom_off[Q + 1] - P.h_dom_off[Q];
+        const int v = d <= 64 ? 0 : (d <= 128 ? 1 : ((double)d / (192.0 * cdiv(d, 192)) >= (double)d / (128.0 * cdiv(d, 128)) ? 2 : 1));
+        v2of[Q] = v;
+        tv[v] += cdiv(d, t2m[v]) * cdiv(d, t2n[v]);
+      }
+      M.rpart_n = 1;
+      for (int v = 0; v < 3; ++v) {
+        nsplit_v[v] = tv[v] ? (int)std::max<int64_t>(1, std::min<int64_t>(16, cdiv(target, tv[v]))) : 1;
+        M.rpart_n = std::max(M.rpart_n, nsplit_v[v]);
+      }
     }
-    M.rpart_n = 1;
-    for (int v = 0; v < 3; ++v) {
-      nsplit_v[v] = tv[v] ? (int)std::max<int64_t>(1, std::min<int64_t>(16, cdiv(512, tv[v]))) : 1;
-      M.rpart_n = std::max(M.rpart_n, nsplit_v[v]);
+    if (M.rpart_n > 1 && M.Rpart.bytes < (size_t)M.rpart_n * P.sum_d2 * 8) {
+      M.bytes -= (int64_t)M.Rpart.bytes;
+      M.Rpart.alloc((size_t)M.rpart_n * P.sum_d2 * 8);
+      M.bytes += (int64_t)M.Rpart.bytes;
     }
-  }
-  if (M.rpart_n > 1) {
-    M.Rpart.alloc((size_t)M.rpart_n * P.sum_d2 * 8);
-    M.bytes += (int64_t)M.Rpart.bytes;
-  }
-  M.phase.clear();
-  M.chunks.clear();
-  std::vector<int64_t> lens;
-  for (int64_t J0 = 0; J0 < n_chol; J0 += Lc) {
-    const int64_t len = std::min(Lc, n_chol - J0);
-    int idx = -1;
-    for (size_t q = 0; q < lens.size(); ++q)
-      if (lens[q] == len) idx = (int)q;
-    if (idx < 0) {
-      idx = (int)lens.size();
-      lens.push_back(len);
+    M.phase.clear();
+    M.chunks.clear();
+    std::vector<int64_t> lens;
+    for (int64_t J0 = 0; J0 < n_chol; J0 += Lc) {
+      const int64_t len = std::min(Lc, n_chol - J0);
+      int idx = -1;
+      for (size_t q = 0; q < lens.size(); ++q)
+        if (lens[q] == len) idx = (int)q;
+      if (idx < 0) {
+        idx = (int)lens.size();
+        lens.push_back(len);
+      }
+      M.chunks.push_back({J0, idx});
     }
-    M.chunks.push_back({J0, idx});
-  }
-  const uint16_t* Lx = M.Lx.as<uint16_t>();
-  const uint16_t* Cx = M.Cx.as<uint16_t>();
-  uint16_t* Tx = M.Tx.as<uint16_t>();
-  const int64_t jstride_full = (int64_t)n_v * 2 * nvp;
-  for (int64_t len : lens) {
-    M.phase.emplace_back();
-    tci_plan::LadderPhase& ph = M.phase.back();
-    ph.Lc = len;
-    ph.nsplit = (int)std::min<int64_t>(M.rpart_n, len);
-    std::vector<int64_t> lpo(loc.size() + 1, 0);
-    for (size_t q = 0; q < loc.size(); ++q) {
-      const int64_t d = P.h_dom_off[loc[q] + 1] - P.h_dom_off[loc[q]];
-      lpo[q + 1] = lpo[q] + len * d * 2 * xpad(d);
+    const uint16_t* Lx = M.Lx.as<uint16_t>();
+    const uint16_t* Cx = M.Cx.as<uint16_t>();
+    uint16_t* Tx = M.Tx.as<uint16_t>();
+    const int64_t jstride_full = (int64_t)n_v * 2 * nvp;
+    for (int64_t len : lens) {
+      M.phase.emplace_back();
+      tci_plan::LadderPhase& ph = M.phase.back();
+      ph.Lc = len;
+      ph.nsplit = (int)std::min<int64_t>(M.rpart_n, len);
+      std::vector<int64_t> lpo(loc.size() + 1, 0);
+      for (size_t q = 0; q < loc.size(); ++q) {
+        const int64_t d = P.h_dom_off[loc[q] + 1] - P.h_dom_off[loc[q]];
+        lpo[q + 1] = lpo[q] + len * d * 2 * xpad(d);
+      }
+      upload(ph.lp_off, lpo, st);
+      ph.p1.clear();
+      ph.p2.clear();
+      ph.p1.split = ph.p2.split = true;
+      ph.p1.tile = 128;
+      ph.p2.tile = tile2;
+      if (M.merged) {
+        // every walk has the full domain: one product per phase over all walks,
+        //   phase 1 T''[(jj,a)][(Q,c)] = sum_k L^J[a][k] C_Q[k][c]   (B = rows of C_Q^T),
+        //            stored as T''[jj][(Q,a)][c]
+        //   phase 2 R[(Q,a)][n] += sum_jj sum_c T''[jj][(Q,a)][c] L^J[n][c]
+        const int d = n_v, NP = n_pairs * n_v;
+        const int64_t ldx = 2 * nvp, Sjj = (int64_t)NP * ldx;
+        Blocking bk;
+        bk.rdiv = d;
+        bk.rstride = Sjj;
+        bk.cdiv = d;
+        bk.cstride = (int64_t)d * ldx;
+        ph.p1.add(reinterpret_cast<double*>(Tx), ldx, (int)(len * d), NP,
+                  {xseg(xop(Lx, ldx, jstride_full), xop(Cx, ldx), d)}, false, true, 0, bk, 3);
+        for (int sp = 0; sp < ph.nsplit; ++sp) {
+          const int64_t j0 = len * sp / ph.nsplit, j1 = len * (sp + 1) / ph.nsplit;
+          std::vector<Seg> sg;
+          for (int64_t jj = j0; jj < j1; ++jj)
+            sg.push_back(xseg(xop(Tx + jj * Sjj, ldx), xop(Lx + jj * jstride_full, ldx, jstride_full), d));
+          double* Rout = M.rpart_n > 1 ? M.Rpart.as<double>() + (int64_t)sp * P.sum_d2 : P.R.as<double>();
+          ph.p2.add(Rout, d, NP, d, sg, true, false, 0, Blocking(), 4);
+        }
+        ph.p1.upload_tables(st);
+        ph.p2.upload_tables(st);
+        continue;
+      }
+      // per-walk tile shapes: phase 1 (M = len d rows) 128 x {64,128,192} and
+      // phase 2 (d x d) {64x64, 128x128, 128x192}, chosen so one tile spans the
+      // short side of the domain where possible (each operand panel read once)
+      ph.p1x.clear();
+      ph.p2x.clear();
+      ph.pp1.clear();
+      ph.pp2.clear();
+      ph.p1x.resize(3);
+      ph.p2x.resize(3);
+      static const int tn1[3] = {64, 128, 192};
+      for (int v = 0; v < 3; ++v) {
+        ph.p1x[v].split = ph.p2x[v].split = true;
+        ph.p1x[v].tile = 128;
+        ph.p1x[v].tile_n = tn1[v];
+        ph.p2x[v].tile = t2m[v];
+        ph.p2x[v].tile_n = t2n[v];
+      }
+      auto util = [](int64_t d, int t) { return (double)d / (double)(t * cdiv(d, t)); };
+      int64_t toff = 0;
+      int qloc = 0;
+      for (int Q = 0; Q < n_pairs; ++Q) {
+        const int d = (int)(P.h_dom_off[Q + 1] - P.h_dom_off[Q]);
+        const int64_t ldx = 2 * xpad(d);
+        const bool full = d == n_v;
+        // L operand: rows (jj, a), one jj-block every jstep elements
+        const uint16_t* Lb;
+        int64_t jstep, jstride;
+        if (full) {
+          Lb = Lx;
+          jstep = jstride_full;
+          jstride = jstride_full;
+        } else if (M.prepacked) {  // [q][J][r][k]: chunk start through jstride
+          Lb = M.Lpx.as<uint16_t>() + lpo_all[qloc++];
+          jstep = (int64_t)d * ldx;
+          jstride = jstep;
+        } else {
+          Lb = M.Lpx.as<uint16_t>() + lpo[qloc++];
+          jstep = (int64_t)d * ldx;
+          jstride = 0;
+        }
+        uint16_t* Tq = Tx + toff;
+        int v1 = 0;
+        for (int v = 1; v < 3; ++v)
+          if (util(d, tn1[v]) >= util(d, tn1[v1])) v1 = v;
+        const int v2 = v2of[Q];
+        const int nsp = (int)std::min<int64_t>(nsplit_v[v2], len);
+        // one program per walk and phase instead (a launch per walk) with
+        // TCI_LADDER_PER_PAIR=1, to compare against the grouped launches
+        GemmProgram* g1 = &ph.p1x[v1];
+        GemmProgram* g2 = &ph.p2x[v2];
+        if (M.per_pair) {
+          ph.pp1.emplace_back();
+          ph.pp2.emplace_back();
+          g1 = &ph.pp1.back();
+          g2 = &ph.pp2.back();
+          g1->split = g2->split = true;
+          g1->tile = 128;
+          g1->tile_n = tn1[v1];
+          g2->tile = ph.p2x[v2].tile;
+          g2->tile_n = ph.p2x[v2].tile_n;
+        }
+        g1->add(reinterpret_cast<double*>(Tq), ldx, (int)(len * d), d,
+                {xseg(xop(Lb, ldx, jstride), xop(Cx + cxo[Q], ldx), d)}, false, true, 0, Blocking(), 3);
+        for (int sp = 0; sp < nsp; ++sp) {
+          const int64_t j0 = len * sp / nsp, j1 = len * (sp + 1) / nsp;
+          std::vector<Seg> sg;
+          for (int64_t jj = j0; jj < j1; ++jj)
+            sg.push_back(xseg(xop(Lb + jj * jstep, ldx, jstride), xop(Tq + jj * d * ldx, ldx), d));
+          double* Rout = (M.rpart_n > 1 ? M.Rpart.as<double>() + (int64_t)sp * P.sum_d2 : P.R.as<double>()) +
+                         P.h_cd_off[Q];
+          g2->add(Rout, d, d, d, sg, true, false, 0, Blocking(), 4);
+        }
+        toff += len * d * ldx;
+      }
+      for (auto& g : ph.p1x) g.upload_tables(st);
+      for (auto& g : ph.p2x) g.upload_tables(st);
+      for (auto& g : ph.pp1) g.upload_tables(st);
+      for (auto& g : ph.pp2) g.upload_tables(st);
     }
-    upload(ph.lp_off, lpo, st);
-    ph.p1.clear();
-    ph.p2.clear();
-    ph.p1.split = ph.p2.split = true;
-    ph.p1.tile = 128;
-    ph.p2.tile = tile2;
+  };
+  const int64_t def_target = 512;
+  auto progs = [&](tci_plan::LadderPhase& ph) {
+    std::vector<GemmProgram*> v;
     if (M.merged) {
-      // every walk has the full domain: one product per phase over all walks,
-      //   phase 1 T''[(jj,a)][(Q,c)] = sum_k L^J[a][k] C_Q[k][c]   (B = rows of C_Q^T),
-      //            stored as T''[jj][(Q,a)][c]
-      //   phase 2 R[(Q,a)][n] += sum_jj sum_c T''[jj][(Q,a)][c] L^J[n][c]
-      const int d = n_v, NP = n_pairs * n_v;
-      const int64_t ldx = 2 * nvp, Sjj = (int64_t)NP * ldx;
-      Blocking bk;
-      bk.rdiv = d;
-      bk.rstride = Sjj;
-      bk.cdiv = d;
-      bk.cstride = (int64_t)d * ldx;
-      ph.p1.add(reinterpret_cast<double*>(Tx), ldx, (int)(len * d), NP,
-                {xseg(xop(Lx, ldx, jstride_full), xop(Cx, ldx), d)}, false, true, 0, bk, 3);
-      for (int sp = 0; sp < ph.nsplit; ++sp) {
-        const int64_t j0 = len * sp / ph.nsplit, j1 = len * (sp + 1) / ph.nsplit;
-        std::vector<Seg> sg;
-        for (int64_t jj = j0; jj < j1; ++jj)
-          sg.push_back(xseg(xop(Tx + jj * Sjj, ldx), xop(Lx + jj * jstride_full, ldx, jstride_full), d));
-        double* Rout = M.rpart_n > 1 ? M.Rpart.as<double>() + (int64_t)sp * P.sum_d2 : P.R.as<double>();
-        ph.p2.add(Rout, d, NP, d, sg, true, false, 0, Blocking(), 4);
-      }
-      ph.p1.upload_tables(st);
-      ph.p2.upload_tables(st);
-      continue;
+      v.push_back(&ph.p1);
+      v.push_back(&ph.p2);
     }
-    // per-walk tile shapes: phase 1 (M = len d rows) 128 x {64,128,192} and
-    // phase 2 (d x d) {64x64, 128x128, 128x192}, chosen so one tile spans the
-    // short side of the domain where possible (each operand panel read once)
-    ph.p1x.clear();
-    ph.p2x.clear();
-    ph.pp1.clear();
-    ph.pp2.clear();
-    ph.p1x.resize(3);
-    ph.p2x.resize(3);
-    static const int tn1[3] = {64, 128, 192};
-    for (int v = 0; v < 3; ++v) {
-      ph.p1x[v].split = ph.p2x[v].split = true;
-      ph.p1x[v].tile = 128;
-      ph.p1x[v].tile_n = tn1[v];
-      ph.p2x[v].tile = t2m[v];
-      ph.p2x[v].tile_n = t2n[v];
+    for (auto& g : ph.p1x) v.push_back(&g);
+    for (auto& g : ph.p2x) v.push_back(&g);
+    return v;
+  };
+  if (M.per_pair || !autotune_enabled()) {
+    build(def_target);
+  } else if (M.tuned_target > 0) {
+    // tuned before for the other split precision: same schedule, so one plan
+    // gives the same sigma for a precision every time it is used
+    build(M.tuned_target);
+    for (auto& ph : M.phase) {
+      std::vector<GemmProgram*> g = progs(ph);
+      for (size_t k = 0; k < g.size() && k < M.tuned_orders.size(); ++k) g[k]->remap = M.tuned_orders[k];
     }
-    auto util = [](int64_t d, int t) { return (double)d / (double)(t * cdiv(d, t)); };
-    int64_t toff = 0;
-    int qloc = 0;
-    for (int Q = 0; Q < n_pairs; ++Q) {
-      const int d = (int)(P.h_dom_off[Q + 1] - P.h_dom_off[Q]);
-      const int64_t ldx = 2 * xpad(d);
-      const bool full = d == n_v;
-      // L operand: rows (jj, a), one jj-block every jstep elements
-      const uint16_t* Lb;
-      int64_t jstep, jstride;
-      if (full) {
-        Lb = Lx;
-        jstep = jstride_full;
-        jstride = jstride_full;
-      } else if (M.prepacked) {  // [q][J][r][k]: chunk start through jstride
-        Lb = M.Lpx.as<uint16_t>() + lpo_all[qloc++];
-        jstep = (int64_t)d * ldx;
-        jstride = jstep;
-      } else {
-        Lb = M.Lpx.as<uint16_t>() + lpo[qloc++];
-        jstep = (int64_t)d * ldx;
-        jstride = 0;
+  } else {
+    // autotuning: time the first chunk of the ladder for several fill
+    // targets, keep the fastest, then pick each program's workgroup order
+    const int64_t cand[4] = {512, 1024, 2048, 4096};
+    double best = 1e300;
+    int64_t bt = def_target;
+    std::string log = "split ladder: K-split target";
+    for (int64_t t : cand) {
+      build(t);
+      TCI_HIP(hipStreamSynchronize(st));
+      const double ms = time_ms(st, 2, [&] { mixed_ladder_chunk(P, precision, M.chunks[0], st); });
+      log += " " + std::to_string(t) + ":" + fmt_ms(ms);
+      if (ms < best * 0.98) {
+        best = ms;
+        bt = t;
       }
-      uint16_t* Tq = Tx + toff;
-      int v1 = 0;
-      for (int v = 1; v < 3; ++v)
-        if (util(d, tn1[v]) >= util(d, tn1[v1])) v1 = v;
-      const int v2 = v2of[Q];
-      const int nsp = (int)std::min<int64_t>(nsplit_v[v2], len);
-      // one program per walk and phase instead (a launch per walk) with
-      // TCI_LADDER_PER_PAIR=1, to compare against the grouped launches
-      GemmProgram* g1 = &ph.p1x[v1];
-      GemmProgram* g2 = &ph.p2x[v2];
-      if (M.per_pair) {
-        ph.pp1.emplace_back();
-        ph.pp2.emplace_back();
-        g1 = &ph.pp1.back();
-        g2 = &ph.pp2.back();
-        g1->split = g2->split = true;
-        g1->tile = 128;
-        g1->tile_n = tn1[v1];
-        g2->tile = ph.p2x[v2].tile;
-        g2->tile_n = ph.p2x[v2].tile_n;
-      }
-      g1->add(reinterpret_cast<double*>(Tq), ldx, (int)(len * d), d,
-              {xseg(xop(Lb, ldx, jstride), xop(Cx + cxo[Q], ldx), d)}, false, true, 0, Blocking(), 3);
-      for (int sp = 0; sp < nsp; ++sp) {
-        const int64_t j0 = len * sp / nsp, j1 = len * (sp + 1) / nsp;
-        std::vector<Seg> sg;
-        for (int64_t jj = j0; jj < j1; ++jj)
-          sg.push_back(xseg(xop(Lb + jj * jstep, ldx, jstride), xop(Tq + jj * d * ldx, ldx), d));
-        double* Rout = (M.rpart_n > 1 ? M.Rpart.as<double>() + (int64_t)sp * P.sum_d2 : P.R.as<double>()) +
-                       P.h_cd_off[Q];
-        g2->add(Rout, d, d, d, sg, true, false, 0, Blocking(), 4);
-      }
-      toff += len * d * ldx;
     }
-    for (auto& g : ph.p1x) g.upload_tables(st);
-    for (auto& g : ph.p2x) g.upload_tables(st);
-    for (auto& g : ph.pp1) g.upload_tables(st);
-    for (auto& g : ph.pp2) g.upload_tables(st);
+    build(bt);
+    TCI_HIP(hipStreamSynchronize(st));
+    log += " -> " + std::to_string(bt) + "; order";
+    const auto& ch = M.chunks[0];
+    std::vector<GemmProgram*> g0 = progs(M.phase[ch.second]);
+    const float* scl = M.scales.as<float>();
+    M.tuned_orders.assign(g0.size(), -1);
+    for (size_t k = 0; k < g0.size(); ++k) {
+      if (g0[k]->tasks.empty()) continue;
+      const int o = tune_order(*g0[k], st, [&] { g0[k]->launch_split(precision, st, ch.first, scl); });
+      log += " " + std::to_string(o);
+      M.tuned_orders[k] = o;
+      for (auto& ph : M.phase) progs(ph)[k]->remap = o;  // same role in the other chunk lengths
+    }
+    M.tuned_target = bt;
+    P.tuning += log + "\n";
   }
   if (P.dense_pairs) dense_pairs_build_split(P, st);
   TCI_HIP(hipStreamSynchronize(st));
@@ -523,30 +608,7 @@ void mixed_ladder(tci_plan& P, int precision, hipStream_t st) {
   TCI_HIP(hipGetLastError());
   double* Rz = M.rpart_n > 1 ? M.Rpart.as<double>() : P.R.as<double>();
   launch_fill(Rz, (int64_t)M.rpart_n * P.sum_d2, 0.0, st);
-  for (const auto& ch : M.chunks) {
-    const tci_plan::LadderPhase& ph = M.phase[ch.second];
-    if (M.n_local && !M.prepacked) {
-      const unsigned gxb = 1024;
-      if (P.lvv_f32)
-        hipLaunchKernelGGL(pack_L_split_kernel<float>, dim3(gxb, M.n_local), dim3(256), 0, st, P.tab,
-                           M.pairs_local.as<int32_t>(), ph.lp_off.as<int64_t>(), static_cast<const float*>(P.Lvv.p),
-                           ch.first, (int)ph.Lc, M.Lpx.as<uint16_t>(), sc, f16);
-      else
-        hipLaunchKernelGGL(pack_L_split_kernel<double>, dim3(gxb, M.n_local), dim3(256), 0, st, P.tab,
-                           M.pairs_local.as<int32_t>(), ph.lp_off.as<int64_t>(), static_cast<const double*>(P.Lvv.p),
-                           ch.first, (int)ph.Lc, M.Lpx.as<uint16_t>(), sc, f16);
-      TCI_HIP(hipGetLastError());
-    }
-    if (M.merged) {
-      ph.p1.launch_split(precision, st, ch.first, sc);
-      ph.p2.launch_split(precision, st, ch.first, sc);
-    } else {
-      for (const auto& g : ph.p1x) g.launch_split(precision, st, ch.first, sc);
-      for (const auto& g : ph.pp1) g.launch_split(precision, st, ch.first, sc);
-      for (const auto& g : ph.p2x) g.launch_split(precision, st, ch.first, sc);
-      for (const auto& g : ph.pp2) g.launch_split(precision, st, ch.first, sc);
-    }
-  }
+  for (const auto& ch : M.chunks) mixed_ladder_chunk(P, precision, ch, st);
   if (P.timing()) TCI_HIP(hipEventRecord(P.ev[3], st));
   if (M.rpart_n > 1) ladder_reduce(M.Rpart.as<double>(), P.sum_d2, M.rpart_n, P.R.as<double>(), P.sum_d2, st);
 }

@@ -561,116 +561,181 @@ void tci_plan::build_two_phase_ladder(hipStream_t st) {
     }
     lchunks.push_back({J0, idx});
   }
-  int64_t tiles2 = 0;
-  for (int P = 0; P < n_pairs; ++P) tiles2 += cdiv(h_dom_off[P + 1] - h_dom_off[P], 64) * cdiv(h_dom_off[P + 1] - h_dom_off[P], 64);
-  rpart_n = (int)std::max<int64_t>(1, std::min<int64_t>(16, cdiv(2048, std::max<int64_t>(tiles2, 1))));
-  if (ladder_merged) {
-    const int64_t tm = cdiv((int64_t)n_pairs * n_v, 128) * cdiv(n_v, 128);
-    rpart_n = (int)std::max<int64_t>(1, std::min<int64_t>(16, cdiv(512, tm)));
-    Cw.alloc((size_t)sum_d2 * 8);
-  }
+  if (ladder_merged) Cw.alloc((size_t)sum_d2 * 8);
   Tbuf.alloc((size_t)Lc * sum_d2 * 8);
   if (sum_local) Lp.alloc((size_t)Lc * sum_local * lsz);
-  if (rpart_n > 1) Rpart.alloc((size_t)rpart_n * sum_d2 * 8);
-  const int64_t nv2 = (int64_t)n_v * n_v;
-  for (int64_t len : lens) {
-    lphase.emplace_back();
-    LadderPhase& ph = lphase.back();
-    ph.Lc = len;
-    ph.nsplit = (int)std::min<int64_t>(rpart_n, len);
-    std::vector<int64_t> lpo(n_local + 1, 0);
-    for (int q = 0; q < n_local; ++q) {
-      const int64_t d = h_dom_off[loc[q] + 1] - h_dom_off[loc[q]];
-      lpo[q + 1] = lpo[q] + len * d * d;
-    }
-    upload(ph.lp_off, lpo, st);
-    ph.p1.clear();
-    ph.p2.clear();
-    ph.p1.dense = ph.p2.dense = true;
-    ph.p1.tile = 128;
-    ph.p2.tile = 64;
+  // programs for a K-split fill target: phase 2 is split along K until it has
+  // about `target` workgroups (at most 16 splits)
+  auto build = [&](int64_t target) {
+    int64_t tiles2 = 0;
+    for (int P = 0; P < n_pairs; ++P) tiles2 += cdiv(h_dom_off[P + 1] - h_dom_off[P], 64) * cdiv(h_dom_off[P + 1] - h_dom_off[P], 64);
+    rpart_n = (int)std::max<int64_t>(1, std::min<int64_t>(16, cdiv(target, std::max<int64_t>(tiles2, 1))));
     if (ladder_merged) {
-      // phase 1: T[(jj,a)][(P,c)] = sum_k L^J[a][k] Cw[k][(P,c)], stored as T[jj][P][a][c]
-      // phase 2: R[(P,a)][n] += sum_jj sum_c T[jj][(P,a)][c] L^J[c][n]  (R blocks back to back)
-      const int d = n_v, NP = n_pairs * n_v;
-      const int64_t Sjj = (int64_t)n_pairs * d * d;
-      Blocking bk;
-      bk.rdiv = d;
-      bk.rstride = Sjj;
-      bk.cdiv = d;
-      bk.cstride = (int64_t)d * d;
-      ph.p1.add(Tbuf.as<double>(), d, (int)(len * d), NP,
-                {seg(op(Lvv.p, n_v, 1, nullptr, nullptr, lvv_f32, nv2), op(Cw.p, NP, 1), d, 1.0)}, false, false, 0, bk);
-      ph.p2.tile = 128;
-      for (int sp = 0; sp < ph.nsplit; ++sp) {
-        const int64_t j0 = len * sp / ph.nsplit, j1 = len * (sp + 1) / ph.nsplit;
-        std::vector<Seg> sg;
-        for (int64_t jj = j0; jj < j1; ++jj)
-          sg.push_back(seg(op(Tbuf.as<double>() + jj * Sjj, d, 1),
-                           op(static_cast<const char*>(Lvv.p) + (size_t)(jj * nv2) * lsz, n_v, 1, nullptr, nullptr,
-                              lvv_f32, nv2),
-                           d, 1.0));
-        double* Rout = rpart_n > 1 ? Rpart.as<double>() + (int64_t)sp * sum_d2 : R.as<double>();
-        ph.p2.add(Rout, d, NP, d, sg, true);
-      }
-      ph.p1.upload_tables(st);
-      ph.p2.upload_tables(st);
-      continue;
+      const int64_t tm = cdiv((int64_t)n_pairs * n_v, 128) * cdiv(n_v, 128);
+      rpart_n = (int)std::max<int64_t>(1, std::min<int64_t>(16, cdiv(target, tm)));
     }
-    // per-walk programs: phase 1 (M = len d) on 64- or 128-wide tiles by domain
-    // size; phase 2 on 64-wide tiles (4 workgroups per CU measure faster than
-    // 128-wide tiles for the long-K d x d outputs)
-    ph.p1x.clear();
-    ph.p2x.clear();
-    ph.p1x.resize(2);
-    ph.p2x.resize(1);
-    for (int v = 0; v < 2; ++v) {
-      ph.p1x[v].dense = true;
-      ph.p1x[v].tile = v ? 128 : 64;
-    }
-    ph.p2x[0].dense = true;
-    ph.p2x[0].tile = 64;
-    int64_t toff = 0;
-    int qloc = 0;
-    for (int P = 0; P < n_pairs; ++P) {
-      const int d = (int)(h_dom_off[P + 1] - h_dom_off[P]);
-      const bool full = d == n_v;
-      const int v = d <= 64 ? 0 : 1;
-      double* T = Tbuf.as<double>() + toff;
-      // L operand of this pair: rows (jj, a) with leading dimension ld, one
-      // jj-block every jstep elements; full domains read L^J itself (J0 via jstride)
-      const void* Lb;
-      int64_t ld, jstep, jstride;
-      if (full) {
-        Lb = Lvv.p;
-        ld = n_v;
-        jstep = nv2;
-        jstride = nv2;
-      } else {
-        Lb = static_cast<const char*>(Lp.p) + (size_t)lpo[qloc++] * lsz;
-        ld = d;
-        jstep = (int64_t)d * d;
-        jstride = 0;
+    if (rpart_n > 1 && Rpart.bytes < (size_t)rpart_n * sum_d2 * 8) Rpart.alloc((size_t)rpart_n * sum_d2 * 8);
+    lphase.clear();
+    const int64_t nv2 = (int64_t)n_v * n_v;
+    for (int64_t len : lens) {
+      lphase.emplace_back();
+      LadderPhase& ph = lphase.back();
+      ph.Lc = len;
+      ph.nsplit = (int)std::min<int64_t>(rpart_n, len);
+      std::vector<int64_t> lpo(n_local + 1, 0);
+      for (int q = 0; q < n_local; ++q) {
+        const int64_t d = h_dom_off[loc[q] + 1] - h_dom_off[loc[q]];
+        lpo[q + 1] = lpo[q] + len * d * d;
       }
-      // phase 1: rows (jj, a) of L (M = len*d); rows jj*ld.. are contiguous when
-      // ld == d, and for full domains ld == n_v == d as well
-      ph.p1x[v].add(T, d, (int)(len * d), d, {seg(op(Lb, ld, 1, nullptr, nullptr, lvv_f32, jstride),
-                                                  op(C.as<double>() + h_cd_off[P], d, 1), d, 1.0)},
-                    false);
-      for (int sp = 0; sp < ph.nsplit; ++sp) {
-        const int64_t j0 = len * sp / ph.nsplit, j1 = len * (sp + 1) / ph.nsplit;
-        std::vector<Seg> sg;
-        for (int64_t jj = j0; jj < j1; ++jj) {
-          const void* Lj = static_cast<const char*>(Lb) + (size_t)(jj * jstep) * lsz;
-          sg.push_back(seg(op(T + jj * d * d, d, 1), op(Lj, ld, 1, nullptr, nullptr, lvv_f32, jstride), d, 1.0));
+      upload(ph.lp_off, lpo, st);
+      ph.p1.clear();
+      ph.p2.clear();
+      ph.p1.dense = ph.p2.dense = true;
+      ph.p1.tile = 128;
+      ph.p2.tile = 64;
+      if (ladder_merged) {
+        // phase 1: T[(jj,a)][(P,c)] = sum_k L^J[a][k] Cw[k][(P,c)], stored as T[jj][P][a][c]
+        // phase 2: R[(P,a)][n] += sum_jj sum_c T[jj][(P,a)][c] L^J[c][n]  (R blocks back to back)
+        const int d = n_v, NP = n_pairs * n_v;
+        const int64_t Sjj = (int64_t)n_pairs * d * d;
+        Blocking bk;
+        bk.rdiv = d;
+        bk.rstride = Sjj;
+        bk.cdiv = d;
+        bk.cstride = (int64_t)d * d;
+        ph.p1.add(Tbuf.as<double>(), d, (int)(len * d), NP,
+                  {seg(op(Lvv.p, n_v, 1, nullptr, nullptr, lvv_f32, nv2), op(Cw.p, NP, 1), d, 1.0)}, false, false, 0, bk);
+        ph.p2.tile = 128;
+        for (int sp = 0; sp < ph.nsplit; ++sp) {
+          const int64_t j0 = len * sp / ph.nsplit, j1 = len * (sp + 1) / ph.nsplit;
+          std::vector<Seg> sg;
+          for (int64_t jj = j0; jj < j1; ++jj)
+            sg.push_back(seg(op(Tbuf.as<double>() + jj * Sjj, d, 1),
+                             op(static_cast<const char*>(Lvv.p) + (size_t)(jj * nv2) * lsz, n_v, 1, nullptr, nullptr,
+                                lvv_f32, nv2),
+                             d, 1.0));
+          double* Rout = rpart_n > 1 ? Rpart.as<double>() + (int64_t)sp * sum_d2 : R.as<double>();
+          ph.p2.add(Rout, d, NP, d, sg, true);
         }
-        double* Rout = (rpart_n > 1 ? Rpart.as<double>() + (int64_t)sp * sum_d2 : R.as<double>()) + h_cd_off[P];
-        ph.p2x[0].add(Rout, d, d, d, sg, true);
+        ph.p1.upload_tables(st);
+        ph.p2.upload_tables(st);
+        continue;
       }
-      toff += len * (int64_t)d * d;
+      // per-walk programs: phase 1 (M = len d) on 64- or 128-wide tiles by domain
+      // size; phase 2 on 64-wide tiles (4 workgroups per CU measure faster than
+      // 128-wide tiles for the long-K d x d outputs)
+      ph.p1x.clear();
+      ph.p2x.clear();
+      ph.p1x.resize(2);
+      ph.p2x.resize(1);
+      for (int v = 0; v < 2; ++v) {
+        ph.p1x[v].dense = true;
+        ph.p1x[v].tile = v ? 128 : 64;
+      }
+      ph.p2x[0].dense = true;
+      ph.p2x[0].tile = 64;
+      int64_t toff = 0;
+      int qloc = 0;
+      for (int P = 0; P < n_pairs; ++P) {
+        const int d = (int)(h_dom_off[P + 1] - h_dom_off[P]);
+        const bool full = d == n_v;
+        const int v = d <= 64 ? 0 : 1;
+        double* T = Tbuf.as<double>() + toff;
+        // L operand of this pair: rows (jj, a) with leading dimension ld, one
+        // jj-block every jstep elements; full domains read L^J itself (J0 via jstride)
+        const void* Lb;
+        int64_t ld, jstep, jstride;
+        if (full) {
+          Lb = Lvv.p;
+          ld = n_v;
+          jstep = nv2;
+          jstride = nv2;
+        } else {
+          Lb = static_cast<const char*>(Lp.p) + (size_t)lpo[qloc++] * lsz;
+          ld = d;
+          jstep = (int64_t)d * d;
+          jstride = 0;
+        }
+        // phase 1: rows (jj, a) of L (M = len*d); rows jj*ld.. are contiguous when
+        // ld == d, and for full domains ld == n_v == d as well
+        ph.p1x[v].add(T, d, (int)(len * d), d, {seg(op(Lb, ld, 1, nullptr, nullptr, lvv_f32, jstride),
+                                                    op(C.as<double>() + h_cd_off[P], d, 1), d, 1.0)},
+                      false);
+        for (int sp = 0; sp < ph.nsplit; ++sp) {
+          const int64_t j0 = len * sp / ph.nsplit, j1 = len * (sp + 1) / ph.nsplit;
+          std::vector<Seg> sg;
+          for (int64_t jj = j0; jj < j1; ++jj) {
+            const void* Lj = static_cast<const char*>(Lb) + (size_t)(jj * jstep) * lsz;
+            sg.push_back(seg(op(T + jj * d * d, d, 1), op(Lj, ld, 1, nullptr, nullptr, lvv_f32, jstride), d, 1.0));
+          }
+          double* Rout = (rpart_n > 1 ? Rpart.as<double>() + (int64_t)sp * sum_d2 : R.as<double>()) + h_cd_off[P];
+          ph.p2x[0].add(Rout, d, d, d, sg, true);
+        }
+        toff += len * (int64_t)d * d;
+      }
+      for (auto& g : ph.p1x) g.upload_tables(st);
+      for (auto& g : ph.p2x) g.upload_tables(st);
     }
-    for (auto& g : ph.p1x) g.upload_tables(st);
-    for (auto& g : ph.p2x) g.upload_tables(st);
+  };
+  // 64-wide phase-2 tiles run 4 workgroups per CU, 128-wide ones 2
+  const int64_t def_target = ladder_merged ? 512 : 2048;
+  if (!autotune_enabled()) {
+    build(def_target);
+    return;
+  }
+  // autotuning on the first chunk (C holds no vector yet; the timing does
+  // not depend on the values), then each program's workgroup order
+  double best = 1e300;
+  int64_t bt = def_target;
+  std::string log = "fp64 ladder: K-split target";
+  for (int64_t t : {def_target / 2, def_target, 2 * def_target, 4 * def_target}) {
+    build(t);
+    TCI_HIP(hipStreamSynchronize(st));
+    const double ms = time_ms(st, 2, [&] { ladder_chunk(lchunks[0], st); });
+    log += " " + std::to_string(t) + ":" + fmt_ms(ms);
+    if (ms < best * 0.98) {
+      best = ms;
+      bt = t;
+    }
+  }
+  build(bt);
+  TCI_HIP(hipStreamSynchronize(st));
+  log += " -> " + std::to_string(bt) + "; order";
+  auto progs = [&](LadderPhase& ph) {
+    std::vector<GemmProgram*> v;
+    if (ladder_merged) {
+      v.push_back(&ph.p1);
+      v.push_back(&ph.p2);
+    }
+    for (auto& g : ph.p1x) v.push_back(&g);
+    for (auto& g : ph.p2x) v.push_back(&g);
+    return v;
+  };
+  const int64_t J0 = lchunks[0].first;
+  std::vector<GemmProgram*> g0 = progs(lphase[lchunks[0].second]);
+  for (size_t k = 0; k < g0.size(); ++k) {
+    if (g0[k]->tasks.empty()) continue;
+    const int o = tune_order(*g0[k], st, [&] { g0[k]->launch(st, J0); });
+    log += " " + std::to_string(o);
+    for (auto& ph : lphase) progs(ph)[k]->remap = o;
+  }
+  tuning += log + "\n";
+}
+
+// One chunk of Cholesky vectors of the FP64 two-phase ladder: pack L^J[D,D]
+// of the local-domain walks, then phase 1 and phase 2.
+void tci_plan::ladder_chunk(const std::pair<int64_t, int>& ch, hipStream_t st) {
+  const LadderPhase& ph = lphase[ch.second];
+  if (n_local) {
+    PackTab K{pairs_local.as<int32_t>(), ph.lp_off.as<int64_t>(), n_local};
+    launch_pack_L(tab, K, Lvv.p, lvv_f32, ch.first, ph.Lc, Lp.p, st);
+  }
+  if (ladder_merged) {
+    ph.p1.launch(st, ch.first);
+    ph.p2.launch(st, ch.first);
+  } else {
+    for (const auto& g : ph.p1x) g.launch(st, ch.first);
+    for (const auto& g : ph.p2x) g.launch(st, ch.first);
   }
 }
 
@@ -685,20 +750,7 @@ void tci_plan::unpack_and_ladder(const double* x, const IoSlot* io, int precisio
     double* Rz = rpart_n > 1 ? Rpart.as<double>() : R.as<double>();
     launch_fill(Rz, (int64_t)(rpart_n > 1 ? rpart_n : 1) * sum_d2, 0.0, st);
     if (ladder_merged) launch_concat_cols(C.as<double>(), n_pairs, n_v, Cw.as<double>(), st);
-    for (const auto& ch : lchunks) {
-      const LadderPhase& ph = lphase[ch.second];
-      if (n_local) {
-        PackTab K{pairs_local.as<int32_t>(), ph.lp_off.as<int64_t>(), n_local};
-        launch_pack_L(tab, K, Lvv.p, lvv_f32, ch.first, ph.Lc, Lp.p, st);
-      }
-      if (ladder_merged) {
-        ph.p1.launch(st, ch.first);
-        ph.p2.launch(st, ch.first);
-      } else {
-        for (const auto& g : ph.p1x) g.launch(st, ch.first);
-        for (const auto& g : ph.p2x) g.launch(st, ch.first);
-      }
-    }
+    for (const auto& ch : lchunks) ladder_chunk(ch, st);
     if (timing()) TCI_HIP(hipEventRecord(ev[3], st));
     if (rpart_n > 1) ladder_reduce(Rpart.as<double>(), sum_d2, rpart_n, R.as<double>(), sum_d2, st);
     return;
@@ -854,6 +906,17 @@ void tci_plan_destroy(tci_plan* plan) {
     (void)hipDeviceSynchronize();
     delete plan;
   }
+}
+
+int tci_plan_tuning(const tci_plan* plan, char* buf, int64_t len) {
+  TCI_GUARD_BEGIN
+  TCI_REQUIRE(plan && buf && len > 0, "null argument");
+  const std::string& t = plan->tuning;
+  const size_t n = std::min<size_t>(t.size(), (size_t)len - 1);
+  std::memcpy(buf, t.data(), n);
+  buf[n] = 0;
+  return TCI_OK;
+  TCI_GUARD_END
 }
 
 int tci_plan_get_info(const tci_plan* plan, tci_plan_info* info) {

@@ -87,11 +87,14 @@ struct tci_plan {
     std::vector<LadderPhase> phase;
     std::vector<std::pair<int64_t, int>> chunks;
     int64_t bytes = 0;
+    int64_t tuned_target = -1;      // autotuned K-split fill target (kept across precision switches)
+    std::vector<int> tuned_orders;  // autotuned workgroup order per program role
   } mx;
   tci::PairTab tab{};
   // accounting
   double flops_ladder = 0.0, flops_sigma = 0.0, bytes_ladder = 0.0;
   int64_t device_bytes = 0;
+  std::string tuning;               // autotuning record (tci_plan_tuning)
   // event timing (tci_profile_enable)
   bool profiling = false;
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
@@ -113,6 +116,7 @@ struct tci_plan {
   void sigma_graph(const double* x, double* y, int flags, hipStream_t st);
   void unpack_and_ladder(const double* x, const tci::IoSlot* io, int precision, hipStream_t st);
   void build_two_phase_ladder(hipStream_t st);
+  void ladder_chunk(const std::pair<int64_t, int>& ch, hipStream_t st);
 };
 
 namespace tci {
@@ -121,6 +125,7 @@ namespace tci {
 // when profiling
 void mixed_prepare(tci_plan& P, int precision, hipStream_t st);
 void mixed_ladder(tci_plan& P, int precision, hipStream_t st);
+void mixed_ladder_chunk(tci_plan& P, int precision, const std::pair<int64_t, int>& ch, hipStream_t st);
 // fp16x3 scales: mx[slot] = max |x[0..n)|, then set_scales mode (1: ladder, 2: pair products)
 void launch_split_scales(int mode, const double* x, int64_t n, unsigned* mx, int slot, float* sc, float dmax,
                          hipStream_t st);
