@@ -64,3 +64,29 @@ def test_tuned_schedule_matches_default(args):
     _sigma(tuned, x, precision="fp16x3")
     assert np.array_equal(_sigma(tuned, x, precision="bf16x3"), tmx)
     assert tuned.tuning().count("split ladder") == 1
+
+
+def test_phase2_192_tiles():
+    """Phase 2 of the split ladder on single 192 x 192 tiles for domains of
+    129-192 virtuals (the shape the tuner may pick) reproduces the oracle and
+    the 128-row tiles to rounding."""
+    _say("case: split ladder phase 2 on 192 x 192 tiles")
+    prob = make_problem(3, 200, 24, 53, domain_range=(100, 192), weak_below=0)
+    ints = Integrals(prob)
+    x = random_ci_vector(prob, 10)
+    ref = oracle_sigma(prob, x, ints)
+    os.environ["TCI_AUTOTUNE"] = "0"
+    try:
+        a = _sigma(tci.Plan.from_problem(prob), x, precision="bf16x3")
+        os.environ["TCI_SPLIT_P2_192"] = "1"
+        big = tci.Plan.from_problem(prob)
+        b = _sigma(big, x, precision="bf16x3")
+        c = _sigma(big, x, precision="fp16x3")
+    finally:
+        os.environ.pop("TCI_SPLIT_P2_192", None)
+        del os.environ["TCI_AUTOTUNE"]
+    d = np.diff(np.asarray(prob.dom_off))
+    assert ((d > 128) & (d <= 192)).sum() >= 2
+    _say(f"  rel err 128-tiles {_rel(a, ref):.2e}  192-tiles {_rel(b, ref):.2e}  fp16x3 {_rel(c, ref):.2e}")
+    assert _rel(a, ref) < 5e-5 and _rel(b, ref) < 5e-5 and _rel(c, ref) < 2e-6
+    assert _rel(b, a) < 1e-5

@@ -353,27 +353,29 @@ void mixed_prepare(tci_plan& P, int precision, hipStream_t st) {
     if (M.per_pair) M.merged = false;
   }
   // programs for a K-split fill target: phase 2 of each tile shape is split
-  // along K until it has about `target` workgroups (at most 16 splits)
-  auto build = [&](int64_t target) {
+  // along K until it has about `target` workgroups (at most 16 splits);
+  // big: phase 2 of domains of 129-192 on single 192 x 192 tiles
+  auto build = [&](int64_t target, bool big) {
     const int tile2 = 128;
     const int64_t tiles2m = cdiv((int64_t)n_pairs * n_v, tile2) * cdiv(n_v, tile2);
     M.rpart_n = (int)std::max<int64_t>(1, std::min<int64_t>(16, cdiv(target, std::max<int64_t>(tiles2m, 1))));
     // per-walk tile shapes (local domains): phase 2 of walk Q runs on tiles
     // t2m[v2] x t2n[v2]; every shape gets enough K-splits to fill the chip in its
     // own launch
-    static const int t2m[3] = {64, 128, 128}, t2n[3] = {64, 128, 192};
+    static const int t2m[4] = {64, 128, 128, 192}, t2n[4] = {64, 128, 192, 192};
     std::vector<int> v2of(n_pairs, 1);
-    int nsplit_v[3] = {1, 1, 1};
+    int nsplit_v[4] = {1, 1, 1, 1};
     if (!M.merged) {
-      int64_t tv[3] = {0, 0, 0};
+      int64_t tv[4] = {0, 0, 0, 0};
       for (int Q = 0; Q < n_pairs; ++Q) {
         const int64_t d = P.h_dom_off[Q + 1] - P.h_dom_off[Q];
-        const int v = d <= 64 ? 0 : (d <= 128 ? 1 : ((double)d / (192.0 * cdiv(d, 192)) >= (double)d / (128.0 * cdiv(d, 128)) ? 2 : 1));
+        int v = d <= 64 ? 0 : (d <= 128 ? 1 : ((double)d / (192.0 * cdiv(d, 192)) >= (double)d / (128.0 * cdiv(d, 128)) ? 2 : 1));
+        if (big && d > 128 && d <= 192) v = 3;
         v2of[Q] = v;
         tv[v] += cdiv(d, t2m[v]) * cdiv(d, t2n[v]);
       }
       M.rpart_n = 1;
-      for (int v = 0; v < 3; ++v) {
+      for (int v = 0; v < 4; ++v) {
         nsplit_v[v] = tv[v] ? (int)std::max<int64_t>(1, std::min<int64_t>(16, cdiv(target, tv[v]))) : 1;
         M.rpart_n = std::max(M.rpart_n, nsplit_v[v]);
       }
@@ -451,12 +453,15 @@ void mixed_prepare(tci_plan& P, int precision, hipStream_t st) {
       ph.pp1.clear();
       ph.pp2.clear();
       ph.p1x.resize(3);
-      ph.p2x.resize(3);
+      ph.p2x.resize(4);
       static const int tn1[3] = {64, 128, 192};
       for (int v = 0; v < 3; ++v) {
-        ph.p1x[v].split = ph.p2x[v].split = true;
+        ph.p1x[v].split = true;
         ph.p1x[v].tile = 128;
         ph.p1x[v].tile_n = tn1[v];
+      }
+      for (int v = 0; v < 4; ++v) {
+        ph.p2x[v].split = true;
         ph.p2x[v].tile = t2m[v];
         ph.p2x[v].tile_n = t2n[v];
       }
@@ -535,35 +540,46 @@ void mixed_prepare(tci_plan& P, int precision, hipStream_t st) {
     return v;
   };
   if (M.per_pair || !autotune_enabled()) {
-    build(def_target);
+    // untuned: 128-row phase-2 tiles unless TCI_SPLIT_P2_192=1
+    const char* e = std::getenv("TCI_SPLIT_P2_192");
+    build(def_target, e && e[0] == '1');
   } else if (M.tuned_target > 0) {
     // tuned before for the other split precision: same schedule, so one plan
     // gives the same sigma for a precision every time it is used
-    build(M.tuned_target);
+    build(M.tuned_target, M.tuned_big);
     for (auto& ph : M.phase) {
       std::vector<GemmProgram*> g = progs(ph);
       for (size_t k = 0; k < g.size() && k < M.tuned_orders.size(); ++k) g[k]->remap = M.tuned_orders[k];
     }
   } else {
     // autotuning: time the first chunk of the ladder for several fill
-    // targets, keep the fastest, then pick each program's workgroup order
+    // targets and phase-2 tile shapes, keep the fastest, then pick each
+    // program's workgroup order
+    bool any_big = false;
+    for (int Q = 0; Q < n_pairs; ++Q) {
+      const int64_t d = P.h_dom_off[Q + 1] - P.h_dom_off[Q];
+      any_big = any_big || (d > 128 && d <= 192 && d != n_v);
+    }
     const int64_t cand[4] = {512, 1024, 2048, 4096};
     double best = 1e300;
     int64_t bt = def_target;
-    std::string log = "split ladder: K-split target";
-    for (int64_t t : cand) {
-      build(t);
-      TCI_HIP(hipStreamSynchronize(st));
-      const double ms = time_ms(st, 2, [&] { mixed_ladder_chunk(P, precision, M.chunks[0], st); });
-      log += " " + std::to_string(t) + ":" + fmt_ms(ms);
-      if (ms < best * 0.98) {
-        best = ms;
-        bt = t;
+    bool bb = false;
+    std::string log = "split ladder: (K-split target, 192-tiles)";
+    for (int big = 0; big <= (any_big && !M.merged ? 1 : 0); ++big)
+      for (int64_t t : cand) {
+        build(t, big != 0);
+        TCI_HIP(hipStreamSynchronize(st));
+        const double ms = time_ms(st, 2, [&] { mixed_ladder_chunk(P, precision, M.chunks[0], st); });
+        log += " " + std::to_string(t) + (big ? "b" : "") + ":" + fmt_ms(ms);
+        if (ms < best * 0.98) {
+          best = ms;
+          bt = t;
+          bb = big != 0;
+        }
       }
-    }
-    build(bt);
+    build(bt, bb);
     TCI_HIP(hipStreamSynchronize(st));
-    log += " -> " + std::to_string(bt) + "; order";
+    log += " -> " + std::to_string(bt) + (bb ? "b" : "") + "; order";
     const auto& ch = M.chunks[0];
     std::vector<GemmProgram*> g0 = progs(M.phase[ch.second]);
     const float* scl = M.scales.as<float>();
@@ -576,6 +592,7 @@ void mixed_prepare(tci_plan& P, int precision, hipStream_t st) {
       for (auto& ph : M.phase) progs(ph)[k]->remap = o;  // same role in the other chunk lengths
     }
     M.tuned_target = bt;
+    M.tuned_big = bb;
     P.tuning += log + "\n";
   }
   if (P.dense_pairs) dense_pairs_build_split(P, st);

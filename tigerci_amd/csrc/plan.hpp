@@ -88,6 +88,7 @@ struct tci_plan {
     std::vector<std::pair<int64_t, int>> chunks;
     int64_t bytes = 0;
     int64_t tuned_target = -1;      // autotuned K-split fill target (kept across precision switches)
+    bool tuned_big = false;         // autotuned: phase 2 of domains 129-192 on 192 x 192 tiles
     std::vector<int> tuned_orders;  // autotuned workgroup order per program role
   } mx;
   tci::PairTab tab{};
