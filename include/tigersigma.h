@@ -143,6 +143,31 @@ typedef struct tci_davidson_result {
 int tci_davidson_run(tci_plan* plan, const tci_davidson_params* params,
                      double* energies, double* vectors, tci_davidson_result* result);
 
+/* ACPF-type functionals (acpf_flag of the reference input, gtimesc.f90
+ * get_gvalues) */
+enum tci_functional {
+  TCI_ACPF = 1,        /* g3 = g4 = g5 = 2/n */
+  TCI_ACPF2 = 2,       /* g3 = g4 = (4/n)(1 - 1/(2(n-1))), g5 = 2/n */
+  TCI_MRAQCC = 3,      /* g3 = g4 = g5 = (4/n)(1 - 1/(2(n-1))) */
+  TCI_ACPF_CUSTOM = 4, /* g3 = g4 = custom_g, g5 = 2/n */
+  TCI_ACPF_TEST = 9    /* all g = 1 (the standard problem) */
+};
+
+/* The five g values (g1, g2 = 1 for the reference and other valence CSFs;
+ * g3 valence CSFs with inactive holes; g4 N-1 walks; g5 N-2 walks) of a
+ * functional for n_corr_elec correlated electrons (get_gvalues,
+ * gtimesc.f90).  Host arithmetic only. */
+int tci_acpf_gvalues(int32_t functional, int32_t n_corr_elec, double custom_g, double* g_values);
+
+/* Davidson-Liu for the generalized problem (H - E_ref) c = E G c with the
+ * diagonal G of the g values per CSF class (davidson_liu_gen_eig,
+ * davidson.f90:253-469; gtimesc / build_g_diagonal, gtimesc.f90).  In the
+ * closed-shell single-reference space the valence CSF takes g1, the N-1
+ * walks g4 and the N-2 walks g5.  Same parameters, precision modes and
+ * results as tci_davidson_run; energies are E_ref + E. */
+int tci_davidson_gen_run(tci_plan* plan, const tci_davidson_params* params, const double* g_values,
+                         double* energies, double* vectors, tci_davidson_result* result);
+
 /* Device memory for callers without a GPU runtime of their own (a Fortran or
  * C Davidson driver).  No reference counterpart. */
 int tci_device_alloc(int64_t bytes, void** ptr);

@@ -32,3 +32,7 @@ def test_fortran_binding_links_and_reports(tmp_path):
     sizes = [int(v) for v in out[2].split()[1:]]
     assert sizes == [ctypes.sizeof(tci._Problem), ctypes.sizeof(tci._Info), ctypes.sizeof(tci._DavParams),
                      ctypes.sizeof(tci._DavResult)]
+    # ACPF-2, 14 correlated electrons: the g values Acetylene_SuperMol_MRACPF2.output:745-749 prints
+    acpf = out[3].split()
+    assert acpf[0] == "acpf2" and acpf[1] == "0"
+    assert [float(v) for v in acpf[2:]] == [1.0, 1.0, 0.27473, 0.27473, 0.14286]

@@ -95,11 +95,13 @@ def load_library(path: str | None = None) -> ctypes.CDLL:
     lib.tci_profile_read.argtypes = [vp, i32, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(i64),
                                      ctypes.POINTER(ctypes.c_double), ctypes.POINTER(i64)]
     lib.tci_plan_tuning.argtypes = [vp, ctypes.c_char_p, i64]
+    lib.tci_davidson_gen_run.argtypes = [vp, ctypes.POINTER(_DavParams), vp, vp, vp, ctypes.POINTER(_DavResult)]
+    lib.tci_acpf_gvalues.argtypes = [i32, i32, ctypes.c_double, vp]
     lib.tci_last_error.restype = ctypes.c_char_p
     for f in ("tci_plan_create", "tci_plan_get_info", "tci_sigma_apply", "tci_htimesc", "tci_diagonal",
               "tci_davidson_run", "tci_selftest_mfma", "tci_device_alloc", "tci_device_free", "tci_memcpy_h2d",
               "tci_memcpy_d2h", "tci_memcpy_d2d", "tci_device_synchronize", "tci_profile_enable",
-              "tci_profile_read", "tci_plan_tuning"):
+              "tci_profile_read", "tci_plan_tuning", "tci_davidson_gen_run", "tci_acpf_gvalues"):
         getattr(lib, f).restype = ctypes.c_int
     if path is None:
         _lib = lib
@@ -115,6 +117,16 @@ def _check(rc: int):
 def selftest_mfma() -> None:
     lib = load_library()
     _check(lib.tci_selftest_mfma())
+
+
+ACPF, ACPF2, MRAQCC, ACPF_CUSTOM, ACPF_TEST = 1, 2, 3, 4, 9
+
+
+def acpf_gvalues(functional: int, n_corr_elec: int, custom_g: float = 0.0) -> np.ndarray:
+    """g1..g5 of an ACPF-type functional (get_gvalues, gtimesc.f90)."""
+    g = np.zeros(5)
+    _check(load_library().tci_acpf_gvalues(functional, n_corr_elec, custom_g, g.ctypes.data))
+    return g
 
 
 def synchronize() -> None:
@@ -283,10 +295,13 @@ class Plan:
 
     def davidson(self, roots: int = 1, energy_tol: float = 1e-6, norm_tol: float = 1e-6,
                  precision: str = "fp64", switch_tol: float | None = None, maxit: int = 0,
-                 max_vectors: int = 0, vectors=None, verbose: bool = False, graph: bool = False):
+                 max_vectors: int = 0, vectors=None, verbose: bool = False, graph: bool = False,
+                 g_values=None):
         """Lowest roots; returns (total energies, result dict).  ``vectors`` is an
         optional DeviceArray of roots*n_csf doubles for the eigenvectors;
-        ``graph`` replays every sigma build from a captured HIP graph."""
+        ``graph`` replays every sigma build from a captured HIP graph;
+        ``g_values`` (five g values, see ``acpf_gvalues``) solves the
+        generalized ACPF-type problem (H - E_ref) c = E G c instead."""
         if switch_tol is None:
             switch_tol = energy_tol
         prec = PRECISION[precision] | (TCI_SIGMA_GRAPH if graph else 0)
@@ -294,8 +309,15 @@ class Plan:
                        1 if verbose else 0)
         e = np.zeros(roots)
         r = _DavResult()
-        _check(self._lib.tci_davidson_run(self._h, ctypes.byref(p), e.ctypes.data,
-                                          _ptr(vectors) if vectors is not None else None, ctypes.byref(r)))
+        vp = _ptr(vectors) if vectors is not None else None
+        if g_values is None:
+            _check(self._lib.tci_davidson_run(self._h, ctypes.byref(p), e.ctypes.data, vp, ctypes.byref(r)))
+        else:
+            g = np.ascontiguousarray(g_values, dtype=np.float64)
+            if g.shape != (5,):
+                raise ValueError("g_values must hold five values")
+            _check(self._lib.tci_davidson_gen_run(self._h, ctypes.byref(p), g.ctypes.data, e.ctypes.data, vp,
+                                                  ctypes.byref(r)))
         return e, dict(iterations=r.iterations, sigma_fp64=r.sigma_fp64, sigma_mixed=r.sigma_mixed,
                        converged=bool(r.converged), seconds=r.seconds, seconds_sigma=r.seconds_sigma)
 

@@ -87,6 +87,36 @@ __global__ void precond_kernel(double* __restrict__ r, const double* __restrict_
   }
 }
 
+// generalized problem (H - E_ref) c = E G c, G diagonal
+__global__ void precond_gen_kernel(double* __restrict__ r, const double* __restrict__ diag,
+                                   const double* __restrict__ g, double E, int64_t n) {
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x) {
+    const double p = diag[e] - E * g[e];
+    r[e] = (p == 0.0) ? 0.0 : -r[e] / p;
+  }
+}
+
+// out = g o x (elementwise)
+__global__ void gmul_kernel(double* __restrict__ out, const double* __restrict__ g, const double* __restrict__ x,
+                            int64_t n) {
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x)
+    out[e] = g[e] * x[e];
+}
+
+// r += a (g o x)
+__global__ void gaxpy_kernel(double* __restrict__ r, double a, const double* __restrict__ g,
+                             const double* __restrict__ x, int64_t n) {
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x)
+    r[e] += a * g[e] * x[e];
+}
+
+// diagonal of G by CSF class (build_g_diagonal, gtimesc.f90): the valence
+// (reference) CSF g1, the N-1 walks g4, the N-2 walks g5
+__global__ void gdiag_kernel(double* __restrict__ g, int64_t n, int64_t n_singles, double g1, double g4, double g5) {
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x)
+    g[e] = e == 0 ? g1 : (e <= n_singles ? g4 : g5);
+}
+
 __global__ void scale_kernel(double* __restrict__ v, double a, int64_t n) {
   for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x)
     v[e] *= a;
@@ -144,6 +174,51 @@ void jacobi_eigh(int m, std::vector<double> A, std::vector<double>& w, std::vect
     for (int k = 0; k < m; ++k) Vs[(size_t)k * m + c] = V[(size_t)k * m + ord[c]];
   }
   V.swap(Vs);
+}
+
+// h y = w g y for symmetric h and positive definite g (dsygv_wrapper role,
+// math_utils.f90:417-479): g = L L^T, Jacobi on L^-1 h L^-T, y = L^-T z, so
+// the columns of y are g-normalised
+void gen_eigh(int m, const std::vector<double>& h, const std::vector<double>& g, std::vector<double>& w,
+              std::vector<double>& Y) {
+  std::vector<double> L((size_t)m * m, 0.0);
+  for (int i = 0; i < m; ++i)
+    for (int j = 0; j <= i; ++j) {
+      double s = g[(size_t)i * m + j];
+      for (int k = 0; k < j; ++k) s -= L[(size_t)i * m + k] * L[(size_t)j * m + k];
+      if (i == j) {
+        TCI_REQUIRE(s > 0.0, "subspace metric G is not positive definite");
+        L[(size_t)i * m + i] = std::sqrt(s);
+      } else {
+        L[(size_t)i * m + j] = s / L[(size_t)j * m + j];
+      }
+    }
+  // B = L^-1 h (forward substitution per column), A = B L^-T
+  std::vector<double> B((size_t)m * m), A((size_t)m * m);
+  for (int c = 0; c < m; ++c)
+    for (int i = 0; i < m; ++i) {
+      double s = h[(size_t)i * m + c];
+      for (int k = 0; k < i; ++k) s -= L[(size_t)i * m + k] * B[(size_t)k * m + c];
+      B[(size_t)i * m + c] = s / L[(size_t)i * m + i];
+    }
+  for (int r = 0; r < m; ++r)
+    for (int i = 0; i < m; ++i) {
+      double s = B[(size_t)r * m + i];
+      for (int k = 0; k < i; ++k) s -= A[(size_t)r * m + k] * L[(size_t)i * m + k];
+      A[(size_t)r * m + i] = s / L[(size_t)i * m + i];
+    }
+  for (int i = 0; i < m; ++i)
+    for (int j = i + 1; j < m; ++j) A[(size_t)i * m + j] = A[(size_t)j * m + i] = 0.5 * (A[(size_t)i * m + j] + A[(size_t)j * m + i]);
+  std::vector<double> Z;
+  jacobi_eigh(m, A, w, Z);
+  // Y = L^-T Z (back substitution per column)
+  Y.assign((size_t)m * m, 0.0);
+  for (int c = 0; c < m; ++c)
+    for (int i = m - 1; i >= 0; --i) {
+      double s = Z[(size_t)i * m + c];
+      for (int k = i + 1; k < m; ++k) s -= L[(size_t)k * m + i] * Y[(size_t)k * m + c];
+      Y[(size_t)i * m + c] = s / L[(size_t)i * m + i];
+    }
 }
 
 struct Dav {
@@ -221,8 +296,12 @@ struct Dav {
 
 using namespace tci;
 
-extern "C" int tci_davidson_run(tci_plan* plan, const tci_davidson_params* prm, double* energies, double* vectors,
-                                tci_davidson_result* result) {
+namespace {
+
+// gvals: null for the standard problem (H - E_ref) c = E c, else the five g
+// values of the generalized problem (H - E_ref) c = E G c
+int davidson_impl(tci_plan* plan, const tci_davidson_params* prm, const double* gvals, double* energies,
+                  double* vectors, tci_davidson_result* result) {
   try {
     TCI_REQUIRE(plan && prm && energies, "null argument");
     const int roots = prm->roots;
@@ -259,7 +338,32 @@ extern "C" int tci_davidson_run(tci_plan* plan, const tci_davidson_params* prm, 
     auto Vk = [&](int k) { return Vp + (int64_t)k * n; };
     auto Sk = [&](int k) { return Sp + (int64_t)k * n; };
     double* res = work.as<double>();
+    double* tmp = res + n;
     const double* diag = plan->diag.as<double>();
+    const unsigned gxn = (unsigned)std::min<int64_t>(cdiv(n, 256), 8192);
+    // generalized problem: G on the device, its subspace matrix gs beside h
+    const bool gen = gvals != nullptr;
+    DevBuf gbuf;
+    const double* gd = nullptr;
+    if (gen) {
+      for (int q = 0; q < 5; ++q) TCI_REQUIRE(std::isfinite(gvals[q]) && gvals[q] > 0.0, "g values must be positive");
+      gbuf.alloc((size_t)n * 8);
+      hipLaunchKernelGGL(gdiag_kernel, dim3(gxn), dim3(256), 0, st, gbuf.as<double>(), n,
+                         (int64_t)plan->n_int * plan->n_v, gvals[0], gvals[3], gvals[4]);
+      TCI_HIP(hipGetLastError());
+      gd = gbuf.as<double>();
+    }
+    std::vector<double> gs;
+    // column k of the subspace metric: gs(q,k) = <c_q, G c_k> for q < m
+    auto g_column = [&](int k, int m) {
+      hipLaunchKernelGGL(gmul_kernel, dim3(gxn), dim3(256), 0, st, tmp, gd, Vk(k), n);
+      TCI_HIP(hipGetLastError());
+      std::vector<double*> X;
+      for (int q = 0; q < m; ++q) X.push_back(Vk(q));
+      std::vector<double> gv;
+      D.mdot(X, tmp, gv);
+      return gv;
+    };
     int precision = prec0;
     auto sigma = [&](int k) {
       auto t0 = std::chrono::steady_clock::now();
@@ -288,6 +392,21 @@ extern "C" int tci_davidson_run(tci_plan* plan, const tci_davidson_params* prm, 
     std::vector<char> conv(roots, 0), met(roots, 0), ready(roots, 0);
     std::vector<double> energy(roots, 0.0), dE(roots, 0.0);
     std::vector<double> h((size_t)mv * mv, 0.0);  // row-major, leading dim mv
+    if (gen) gs.assign((size_t)mv * mv, 0.0);
+    // subspace eigenpairs of the leading m x m blocks (ascending)
+    auto subspace_solve = [&](int m, std::vector<double>& w_, std::vector<double>& Y_) {
+      std::vector<double> hm((size_t)m * m), gm;
+      for (int a = 0; a < m; ++a)
+        for (int b = 0; b < m; ++b) hm[(size_t)a * m + b] = h[(size_t)a * mv + b];
+      if (!gen) {
+        jacobi_eigh(m, hm, w_, Y_);
+        return;
+      }
+      gm.resize((size_t)m * m);
+      for (int a = 0; a < m; ++a)
+        for (int b = 0; b < m; ++b) gm[(size_t)a * m + b] = 0.5 * (gs[(size_t)a * mv + b] + gs[(size_t)b * mv + a]);
+      gen_eigh(m, hm, gm, w_, Y_);
+    };
     std::vector<double> w, Y, Yprev;
     std::vector<char> restarts;
     int it = 0;
@@ -302,11 +421,12 @@ extern "C" int tci_davidson_run(tci_plan* plan, const tci_davidson_params* prm, 
         std::vector<double> hv;
         D.mdot(X, Sk(k), hv);
         for (int q = 0; q <= k; ++q) h[(size_t)q * mv + k] = h[(size_t)k * mv + q] = hv[q];
+        if (gen) {
+          const std::vector<double> gv = g_column(k, k + 1);
+          for (int q = 0; q <= k; ++q) gs[(size_t)q * mv + k] = gs[(size_t)k * mv + q] = gv[q];
+        }
       }
-      std::vector<double> hm((size_t)m * m);
-      for (int a = 0; a < m; ++a)
-        for (int b = 0; b < m; ++b) hm[(size_t)a * m + b] = h[(size_t)a * mv + b];
-      jacobi_eigh(m, hm, w, Y);
+      subspace_solve(m, w, Y);
       for (int r = 0; r < roots; ++r) {
         dE[r] = std::fabs(w[r] - energy[r]);
         energy[r] = w[r];
@@ -401,10 +521,12 @@ extern "C" int tci_davidson_run(tci_plan* plan, const tci_davidson_params* prm, 
             const double s2 = 0.5 * (h[(size_t)a * mv + b] + h[(size_t)b * mv + a]);
             h[(size_t)a * mv + b] = h[(size_t)b * mv + a] = s2;
           }
-        hm.assign((size_t)m * m, 0.0);
-        for (int a = 0; a < m; ++a)
-          for (int b = 0; b < m; ++b) hm[(size_t)a * m + b] = h[(size_t)a * mv + b];
-        jacobi_eigh(m, hm, w, Y);
+        if (gen)
+          for (int a = 0; a < m; ++a) {
+            const std::vector<double> gv = g_column(a, m);
+            for (int q = 0; q < m; ++q) gs[(size_t)q * mv + a] = gv[q];
+          }
+        subspace_solve(m, w, Y);
         // after the switch |dE| compares the FP64 Ritz values of the collapsed
         // space with the mixed ones: the Rayleigh-quotient error of the split
         // sigma (1e-10 Eh scale), so the energy criterion stays meaningful
@@ -425,16 +547,26 @@ extern "C" int tci_davidson_run(tci_plan* plan, const tci_davidson_params* prm, 
       }
       for (int r = 0; r < roots; ++r) {
         if (conv[r] || ready[r]) continue;
-        // residual r = sum_k y_k (sigma_k - E c_k)
-        std::vector<double*> X;
-        std::vector<double> c;
-        for (int q = 0; q < mcur; ++q) {
-          X.push_back(Sk(q));
-          c.push_back(Y[(size_t)q * mcur + r]);
-          X.push_back(Vk(q));
-          c.push_back(-energy[r] * Y[(size_t)q * mcur + r]);
+        // residual r = sum_k y_k (sigma_k - E G c_k)  (build_residual_gen_eig
+        // for the generalized problem, G = 1 otherwise)
+        if (gen) {
+          std::vector<double> c(mcur);
+          for (int q = 0; q < mcur; ++q) c[q] = Y[(size_t)q * mcur + r];
+          D.lincomb(res, 0.0, Xs, c);
+          D.lincomb(tmp, 0.0, Xv, c);
+          hipLaunchKernelGGL(gaxpy_kernel, dim3(gxn), dim3(256), 0, st, res, -energy[r], gd, tmp, n);
+          TCI_HIP(hipGetLastError());
+        } else {
+          std::vector<double*> X;
+          std::vector<double> c;
+          for (int q = 0; q < mcur; ++q) {
+            X.push_back(Sk(q));
+            c.push_back(Y[(size_t)q * mcur + r]);
+            X.push_back(Vk(q));
+            c.push_back(-energy[r] * Y[(size_t)q * mcur + r]);
+          }
+          D.lincomb(res, 0.0, X, c);
         }
-        D.lincomb(res, 0.0, X, c);
         const double rn = D.norm(res);
         if (prm->verbose)
           std::printf("  %5d %5d %22.12f %15.8e %15.8e\n", it, r + 1, energy[r] + plan->e_ref, rn, dE[r]);
@@ -456,8 +588,12 @@ extern "C" int tci_davidson_run(tci_plan* plan, const tci_davidson_params* prm, 
           conv[r] = 1;
           continue;
         }
-        hipLaunchKernelGGL(precond_kernel, dim3((unsigned)std::min<int64_t>(cdiv(n, 256), 8192)), dim3(256), 0, st, res,
-                           diag, energy[r], n);
+        // preconditioner: r_i <- -r_i / (H_ii - E_ref - E g_i), 0 where that is 0
+        // (build_next_vector_std_eig / build_next_vector_gen_eig)
+        if (gen)
+          hipLaunchKernelGGL(precond_gen_kernel, dim3(gxn), dim3(256), 0, st, res, diag, gd, energy[r], n);
+        else
+          hipLaunchKernelGGL(precond_kernel, dim3(gxn), dim3(256), 0, st, res, diag, energy[r], n);
         TCI_HIP(hipGetLastError());
         TCI_REQUIRE(m < mv, "Davidson subspace budget exhausted");
         TCI_HIP(hipMemcpyAsync(Vk(m), res, n * 8, hipMemcpyDeviceToDevice, st));
@@ -502,4 +638,43 @@ extern "C" int tci_davidson_run(tci_plan* plan, const tci_davidson_params* prm, 
     tci::set_last_error(e.what());
     return TCI_ERR_ARG;
   }
+}
+
+}  // namespace
+
+extern "C" int tci_davidson_run(tci_plan* plan, const tci_davidson_params* prm, double* energies, double* vectors,
+                                tci_davidson_result* result) {
+  return davidson_impl(plan, prm, nullptr, energies, vectors, result);
+}
+
+extern "C" int tci_davidson_gen_run(tci_plan* plan, const tci_davidson_params* prm, const double* g_values,
+                                    double* energies, double* vectors, tci_davidson_result* result) {
+  if (!g_values) {
+    tci::set_last_error("null g_values");
+    return TCI_ERR_ARG;
+  }
+  return davidson_impl(plan, prm, g_values, energies, vectors, result);
+}
+
+// get_gvalues (gtimesc.f90): g1..g5 of the ACPF-type functionals from the
+// number of correlated electrons n (electrons minus 2 per frozen orbital)
+extern "C" int tci_acpf_gvalues(int32_t functional, int32_t n_corr_elec, double custom_g, double* g) {
+  if (!g || n_corr_elec < 2) {
+    tci::set_last_error("g must be non-null and n_corr_elec >= 2");
+    return TCI_ERR_ARG;
+  }
+  const double n = (double)n_corr_elec;
+  const double acpf = 2.0 / n, acpf2 = (4.0 / n) * (1.0 - 1.0 / (2.0 * (n - 1.0)));
+  g[0] = g[1] = 1.0;
+  switch (functional) {
+    case TCI_ACPF: g[2] = g[3] = g[4] = acpf; break;
+    case TCI_ACPF2: g[2] = g[3] = acpf2; g[4] = acpf; break;
+    case TCI_MRAQCC: g[2] = g[3] = g[4] = acpf2; break;
+    case TCI_ACPF_CUSTOM: g[2] = g[3] = custom_g; g[4] = acpf; break;
+    case TCI_ACPF_TEST: g[2] = g[3] = g[4] = 1.0; break;
+    default:
+      tci::set_last_error("unknown functional");
+      return TCI_ERR_ARG;
+  }
+  return TCI_OK;
 }

@@ -566,9 +566,11 @@ void tci_plan::build_two_phase_ladder(hipStream_t st) {
   if (sum_local) Lp.alloc((size_t)Lc * sum_local * lsz);
   // programs for a K-split fill target: phase 2 is split along K until it has
   // about `target` workgroups (at most 16 splits)
-  auto build = [&](int64_t target) {
+  // tile2: phase-2 tile of the per-walk programs (64 or 128)
+  auto build = [&](int64_t target, int tile2) {
     int64_t tiles2 = 0;
-    for (int P = 0; P < n_pairs; ++P) tiles2 += cdiv(h_dom_off[P + 1] - h_dom_off[P], 64) * cdiv(h_dom_off[P + 1] - h_dom_off[P], 64);
+    for (int P = 0; P < n_pairs; ++P)
+      tiles2 += cdiv(h_dom_off[P + 1] - h_dom_off[P], tile2) * cdiv(h_dom_off[P + 1] - h_dom_off[P], tile2);
     rpart_n = (int)std::max<int64_t>(1, std::min<int64_t>(16, cdiv(target, std::max<int64_t>(tiles2, 1))));
     if (ladder_merged) {
       const int64_t tm = cdiv((int64_t)n_pairs * n_v, 128) * cdiv(n_v, 128);
@@ -633,7 +635,7 @@ void tci_plan::build_two_phase_ladder(hipStream_t st) {
         ph.p1x[v].tile = v ? 128 : 64;
       }
       ph.p2x[0].dense = true;
-      ph.p2x[0].tile = 64;
+      ph.p2x[0].tile = tile2;
       int64_t toff = 0;
       int qloc = 0;
       for (int P = 0; P < n_pairs; ++P) {
@@ -680,27 +682,34 @@ void tci_plan::build_two_phase_ladder(hipStream_t st) {
   // 64-wide phase-2 tiles run 4 workgroups per CU, 128-wide ones 2
   const int64_t def_target = ladder_merged ? 512 : 2048;
   if (!autotune_enabled()) {
-    build(def_target);
+    build(def_target, 64);
     return;
   }
   // autotuning on the first chunk (C holds no vector yet; the timing does
-  // not depend on the values), then each program's workgroup order
+  // not depend on the values) over fill targets and, for per-walk programs,
+  // the phase-2 tile; then each program's workgroup order
   double best = 1e300;
   int64_t bt = def_target;
-  std::string log = "fp64 ladder: K-split target";
-  for (int64_t t : {def_target / 2, def_target, 2 * def_target, 4 * def_target}) {
-    build(t);
-    TCI_HIP(hipStreamSynchronize(st));
-    const double ms = time_ms(st, 2, [&] { ladder_chunk(lchunks[0], st); });
-    log += " " + std::to_string(t) + ":" + fmt_ms(ms);
-    if (ms < best * 0.98) {
-      best = ms;
-      bt = t;
+  int btile = 64;
+  std::string log = "fp64 ladder: (K-split target, phase-2 tile)";
+  for (int tile2 : {64, 128}) {
+    if (ladder_merged && tile2 == 128) break;  // merged programs have their own tiles
+    const int64_t base = tile2 == 64 ? def_target : def_target / 4;
+    for (int64_t t : {base / 2, base, 2 * base, 4 * base}) {
+      build(t, tile2);
+      TCI_HIP(hipStreamSynchronize(st));
+      const double ms = time_ms(st, 2, [&] { ladder_chunk(lchunks[0], st); });
+      log += " " + std::to_string(t) + "/" + std::to_string(tile2) + ":" + fmt_ms(ms);
+      if (ms < best * 0.98) {
+        best = ms;
+        bt = t;
+        btile = tile2;
+      }
     }
   }
-  build(bt);
+  build(bt, btile);
   TCI_HIP(hipStreamSynchronize(st));
-  log += " -> " + std::to_string(bt) + "; order";
+  log += " -> " + std::to_string(bt) + "/" + std::to_string(btile) + "; order";
   auto progs = [&](LadderPhase& ph) {
     std::vector<GemmProgram*> v;
     if (ladder_merged) {

@@ -13,7 +13,10 @@ module tigersigma
   integer(c_int), parameter, public :: TCI_OK = 0, TCI_ERR_ARG = 1, TCI_ERR_HIP = 2, &
                                        TCI_ERR_OOM = 3, TCI_ERR_NUMERIC = 4
   integer(c_int32_t), parameter, public :: TCI_F64 = 0, TCI_F32 = 1
-  integer(c_int32_t), parameter, public :: TCI_SIGMA_FP64 = 0, TCI_SIGMA_BF16X3 = 1
+  integer(c_int32_t), parameter, public :: TCI_SIGMA_FP64 = 0, TCI_SIGMA_BF16X3 = 1, TCI_SIGMA_FP16X3 = 2
+  ! acpf_flag values of the reference input (gtimesc.f90 get_gvalues)
+  integer(c_int32_t), parameter, public :: TCI_ACPF = 1, TCI_ACPF2 = 2, TCI_MRAQCC = 3, TCI_ACPF_CUSTOM = 4, &
+                                           TCI_ACPF_TEST = 9
   integer(c_int32_t), parameter, public :: TCI_ACCUMULATE = 256, TCI_ADD_EREF = 512, TCI_SIGMA_GRAPH = 1024
 
   type, bind(c), public :: tci_problem
@@ -47,7 +50,8 @@ module tigersigma
   end type tci_davidson_result
 
   public :: tci_plan_create, tci_plan_destroy, tci_plan_get_info, tci_sigma_apply, tci_htimesc, &
-            tci_diagonal, tci_davidson_run, tci_device_alloc, tci_device_free, tci_memcpy_h2d, &
+            tci_diagonal, tci_davidson_run, tci_davidson_gen_run, tci_acpf_gvalues, tci_device_alloc, &
+            tci_device_free, tci_memcpy_h2d, &
             tci_memcpy_d2h, tci_device_synchronize, tci_last_error_string
 
   interface
@@ -92,6 +96,25 @@ module tigersigma
       type(tci_davidson_params), intent(in) :: params
       real(c_double), intent(out) :: energies(*)
       type(tci_davidson_result), intent(out) :: result
+    end function
+
+    ! davidson_liu_gen_eig: (H - E_ref) c = E G c, g_values(5) from tci_acpf_gvalues
+    integer(c_int) function tci_davidson_gen_run(plan, params, g_values, energies, vectors, result) &
+        bind(c, name="tci_davidson_gen_run")
+      import :: c_int, c_ptr, c_double, tci_davidson_params, tci_davidson_result
+      type(c_ptr), value :: plan, vectors
+      type(tci_davidson_params), intent(in) :: params
+      real(c_double), intent(in) :: g_values(5)
+      real(c_double), intent(out) :: energies(*)
+      type(tci_davidson_result), intent(out) :: result
+    end function
+
+    integer(c_int) function tci_acpf_gvalues(functional, n_corr_elec, custom_g, g_values) &
+        bind(c, name="tci_acpf_gvalues")
+      import :: c_int, c_int32_t, c_double
+      integer(c_int32_t), value :: functional, n_corr_elec
+      real(c_double), value :: custom_g
+      real(c_double), intent(out) :: g_values(5)
     end function
     integer(c_int) function tci_device_alloc(bytes, ptr) bind(c, name="tci_device_alloc")
       import :: c_int, c_ptr, c_int64_t
