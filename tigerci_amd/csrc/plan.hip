@@ -567,7 +567,8 @@ void tci_plan::build_two_phase_ladder(hipStream_t st) {
   // programs for a K-split fill target: phase 2 is split along K until it has
   // about `target` workgroups (at most 16 splits)
   // tile2: phase-2 tile of the per-walk programs (64 or 128)
-  auto build = [&](int64_t target, int tile2) {
+  // tile1: phase-1 tile of the per-walk programs for domains above 64
+  auto build = [&](int64_t target, int tile2, int tile1) {
     int64_t tiles2 = 0;
     for (int P = 0; P < n_pairs; ++P)
       tiles2 += cdiv(h_dom_off[P + 1] - h_dom_off[P], tile2) * cdiv(h_dom_off[P + 1] - h_dom_off[P], tile2);
@@ -632,7 +633,7 @@ void tci_plan::build_two_phase_ladder(hipStream_t st) {
       ph.p2x.resize(1);
       for (int v = 0; v < 2; ++v) {
         ph.p1x[v].dense = true;
-        ph.p1x[v].tile = v ? 128 : 64;
+        ph.p1x[v].tile = v ? tile1 : 64;
       }
       ph.p2x[0].dense = true;
       ph.p2x[0].tile = tile2;
@@ -682,7 +683,7 @@ void tci_plan::build_two_phase_ladder(hipStream_t st) {
   // 64-wide phase-2 tiles run 4 workgroups per CU, 128-wide ones 2
   const int64_t def_target = ladder_merged ? 512 : 2048;
   if (!autotune_enabled()) {
-    build(def_target, 64);
+    build(def_target, 64, 128);
     return;
   }
   // autotuning on the first chunk (C holds no vector yet; the timing does
@@ -696,7 +697,7 @@ void tci_plan::build_two_phase_ladder(hipStream_t st) {
     if (ladder_merged && tile2 == 128) break;  // merged programs have their own tiles
     const int64_t base = tile2 == 64 ? def_target : def_target / 4;
     for (int64_t t : {base / 2, base, 2 * base, 4 * base}) {
-      build(t, tile2);
+      build(t, tile2, 128);
       TCI_HIP(hipStreamSynchronize(st));
       const double ms = time_ms(st, 2, [&] { ladder_chunk(lchunks[0], st); });
       log += " " + std::to_string(t) + "/" + std::to_string(tile2) + ":" + fmt_ms(ms);
@@ -707,9 +708,20 @@ void tci_plan::build_two_phase_ladder(hipStream_t st) {
       }
     }
   }
-  build(bt, btile);
+  int btile1 = 128;
+  if (!ladder_merged && n_local > 0) {  // then the phase-1 tile of domains above 64
+    build(bt, btile, 64);
+    TCI_HIP(hipStreamSynchronize(st));
+    const double ms = time_ms(st, 2, [&] { ladder_chunk(lchunks[0], st); });
+    log += "; phase-1 tile 64:" + fmt_ms(ms);
+    if (ms < best * 0.98) {
+      best = ms;
+      btile1 = 64;
+    }
+  }
+  build(bt, btile, btile1);
   TCI_HIP(hipStreamSynchronize(st));
-  log += " -> " + std::to_string(bt) + "/" + std::to_string(btile) + "; order";
+  log += " -> " + std::to_string(bt) + "/" + std::to_string(btile) + "/p1 " + std::to_string(btile1) + "; order";
   auto progs = [&](LadderPhase& ph) {
     std::vector<GemmProgram*> v;
     if (ladder_merged) {
