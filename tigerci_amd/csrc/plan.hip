@@ -497,7 +497,7 @@ void tci_plan::build(const tci_problem& p) {
   for (DevBuf* b : {&pairs, &dom_off, &dom_idx, &cd_off, &aa_off, &ab_off, &eps, &pos, &xmap, &isect_off, &isect,
                     &inv_off, &inv_q, &inv_pos, &gpart_off, &Lvv, &Lov, &Kov, &Jvv, &Mss, &O4, &O3, &vvd, &vvx,
                     &diag, &C, &Ct, &R, &s, &g, &gss, &gW, &Y, &W, &gpart, &part, &sig0, &lad_part, &Jbig, &CRb,
-                    &CtRb, &CCb, &Zb, &Z3b, &Cemb, &CTemb, &Remb, &Yt, &holeA, &Tbuf, &Rpart, &Lp, &Cw})
+                    &CtRb, &CCb, &Zb, &Z3b, &Cemb, &CTemb, &Remb, &Yt, &holeA, &Tbuf, &Rpart, &Lp, &Cw, &lp_off_all})
     device_bytes += (int64_t)b->bytes;
 }
 
@@ -544,8 +544,29 @@ void tci_plan::build_two_phase_ladder(hipStream_t st) {
     const char* env = std::getenv("TCI_LADDER_MERGED");
     ladder_merged = n_local == 0 && n_pairs > 0 && !(env && env[0] == '0');
   }
+  // L^J[D_P, D_P] of every local walk packed once per plan when that takes at
+  // most 96 GB and leaves 64 GB free (the split ladder's copies and the
+  // Davidson vectors); otherwise packed per chunk inside every sigma
+  {
+    const char* env = std::getenv("TCI_FP64_PREPACK");
+    const double need = (double)sum_local * (double)n_chol * (double)lsz;
+    size_t freeb = 0, totalb = 0;
+    TCI_HIP(hipMemGetInfo(&freeb, &totalb));
+    lp_all = sum_local > 0 && !(env && env[0] == '0') && need <= 96.0e9 && need + 64.0e9 <= (double)freeb;
+  }
+  std::vector<int64_t> lpo_all(loc.size() + 1, 0);
+  if (lp_all) {
+    for (size_t q = 0; q < loc.size(); ++q) {
+      const int64_t d = h_dom_off[loc[q] + 1] - h_dom_off[loc[q]];
+      lpo_all[q + 1] = lpo_all[q] + n_chol * d * d;
+    }
+    upload(lp_off_all, lpo_all, st);
+    Lp.alloc((size_t)lpo_all.back() * lsz);
+    PackTab K{pairs_local.as<int32_t>(), lp_off_all.as<int64_t>(), n_local};
+    launch_pack_L(tab, K, Lvv.p, lvv_f32, 0, n_chol, Lp.p, st);
+  }
   const double budget = 8.0e9;  // bytes of T + Lp per chunk
-  const double per_J = 8.0 * (double)std::max<int64_t>(sum_d2, 1) + (double)lsz * sum_local;
+  const double per_J = 8.0 * (double)std::max<int64_t>(sum_d2, 1) + (lp_all ? 0.0 : (double)lsz * sum_local);
   int64_t Lc = std::max<int64_t>(1, std::min<int64_t>(n_chol, (int64_t)(budget / per_J)));
   for (int64_t q = Lc; q >= std::max<int64_t>(1, Lc / 2); --q)
     if (n_chol % q == 0) { Lc = q; break; }
@@ -563,7 +584,7 @@ void tci_plan::build_two_phase_ladder(hipStream_t st) {
   }
   if (ladder_merged) Cw.alloc((size_t)sum_d2 * 8);
   Tbuf.alloc((size_t)Lc * sum_d2 * 8);
-  if (sum_local) Lp.alloc((size_t)Lc * sum_local * lsz);
+  if (sum_local && !lp_all) Lp.alloc((size_t)Lc * sum_local * lsz);
   // programs for a K-split fill target: phase 2 is split along K until it has
   // about `target` workgroups (at most 16 splits)
   // tile2: phase-2 tile of the per-walk programs (64 or 128)
@@ -653,6 +674,11 @@ void tci_plan::build_two_phase_ladder(hipStream_t st) {
           ld = n_v;
           jstep = nv2;
           jstride = nv2;
+        } else if (lp_all) {  // [q][J][r][c]: chunk start through jstride
+          Lb = static_cast<const char*>(Lp.p) + (size_t)lpo_all[qloc++] * lsz;
+          ld = d;
+          jstep = (int64_t)d * d;
+          jstride = jstep;
         } else {
           Lb = static_cast<const char*>(Lp.p) + (size_t)lpo[qloc++] * lsz;
           ld = d;
@@ -747,7 +773,7 @@ void tci_plan::build_two_phase_ladder(hipStream_t st) {
 // of the local-domain walks, then phase 1 and phase 2.
 void tci_plan::ladder_chunk(const std::pair<int64_t, int>& ch, hipStream_t st) {
   const LadderPhase& ph = lphase[ch.second];
-  if (n_local) {
+  if (n_local && !lp_all) {
     PackTab K{pairs_local.as<int32_t>(), ph.lp_off.as<int64_t>(), n_local};
     launch_pack_L(tab, K, Lvv.p, lvv_f32, ch.first, ph.Lc, Lp.p, st);
   }

@@ -46,7 +46,9 @@ struct tci_plan {
     std::vector<tci::GemmProgram> pp1, pp2;  // one program per walk (TCI_LADDER_PER_PAIR=1)
     tci::DevBuf lp_off;             // packed-L offsets of the local-domain pairs
   };
-  tci::DevBuf Lp, pairs_local;      // packed L^J[D,D] of local-domain pairs (one chunk)
+  tci::DevBuf Lp, pairs_local;      // packed L^J[D,D] of local-domain pairs (one chunk, or all J when lp_all)
+  bool lp_all = false;              // Lp holds every J (packed once per plan)
+  tci::DevBuf lp_off_all;           // offsets of the local walks in an all-J Lp
   int n_local = 0;
   std::vector<LadderPhase> lphase;  // one per distinct chunk length
   std::vector<std::pair<int64_t, int>> lchunks;  // (J0, index into lphase)
