@@ -90,3 +90,26 @@ def test_phase2_192_tiles():
     _say(f"  rel err 128-tiles {_rel(a, ref):.2e}  192-tiles {_rel(b, ref):.2e}  fp16x3 {_rel(c, ref):.2e}")
     assert _rel(a, ref) < 5e-5 and _rel(b, ref) < 5e-5 and _rel(c, ref) < 2e-6
     assert _rel(b, a) < 1e-5
+
+
+def test_fp64_192_tiles():
+    """Both FP64 ladder phases of domains of 129-192 on 192 x 192 tiles
+    (eight waves per workgroup) reproduce the oracle and the default tiles."""
+    _say("case: FP64 ladder on 192 x 192 tiles")
+    prob = make_problem(3, 200, 24, 53, domain_range=(100, 192), weak_below=0)
+    ints = Integrals(prob)
+    xs = [random_ci_vector(prob, 11 + k) for k in range(2)]
+    os.environ["TCI_AUTOTUNE"] = "0"
+    try:
+        base = tci.Plan.from_problem(prob)
+        os.environ["TCI_FP64_P192"] = "1"
+        big = tci.Plan.from_problem(prob)
+    finally:
+        os.environ.pop("TCI_FP64_P192", None)
+        del os.environ["TCI_AUTOTUNE"]
+    for x in xs:
+        ref = oracle_sigma(prob, x, ints)
+        a, b = _sigma(base, x), _sigma(big, x)
+        _say(f"  rel err default {_rel(a, ref):.2e}  192-tiles {_rel(b, ref):.2e}")
+        assert _rel(a, ref) < 1e-10 and _rel(b, ref) < 1e-10
+        assert _rel(b, a) < 1e-12

@@ -589,11 +589,21 @@ void tci_plan::build_two_phase_ladder(hipStream_t st) {
   // about `target` workgroups (at most 16 splits)
   // tile2: phase-2 tile of the per-walk programs (64 or 128)
   // tile1: phase-1 tile of the per-walk programs for domains above 64
-  auto build = [&](int64_t target, int tile2, int tile1) {
-    int64_t tiles2 = 0;
-    for (int P = 0; P < n_pairs; ++P)
-      tiles2 += cdiv(h_dom_off[P + 1] - h_dom_off[P], tile2) * cdiv(h_dom_off[P + 1] - h_dom_off[P], tile2);
-    rpart_n = (int)std::max<int64_t>(1, std::min<int64_t>(16, cdiv(target, std::max<int64_t>(tiles2, 1))));
+  // big: both phases of local domains of 129-192 on 192 x 192 tiles (one
+  //      workgroup per CU, so their K-split target is a quarter)
+  auto is_big = [&](int64_t d) { return d > 128 && d <= 192 && d != n_v; };
+  auto build = [&](int64_t target, int tile2, int tile1, bool big) {
+    int64_t tiles2 = 0, tiles_big = 0;
+    for (int P = 0; P < n_pairs; ++P) {
+      const int64_t d = h_dom_off[P + 1] - h_dom_off[P];
+      if (big && is_big(d)) ++tiles_big;
+      else tiles2 += cdiv(d, tile2) * cdiv(d, tile2);
+    }
+    const int nsp_small = tiles2 ? (int)std::max<int64_t>(1, std::min<int64_t>(16, cdiv(target, tiles2))) : 1;
+    const int nsp_big =
+        tiles_big ? (int)std::max<int64_t>(1, std::min<int64_t>(16, cdiv(std::max<int64_t>(target / 4, 1), tiles_big)))
+                  : 1;
+    rpart_n = std::max(nsp_small, nsp_big);
     if (ladder_merged) {
       const int64_t tm = cdiv((int64_t)n_pairs * n_v, 128) * cdiv(n_v, 128);
       rpart_n = (int)std::max<int64_t>(1, std::min<int64_t>(16, cdiv(target, tm)));
@@ -650,20 +660,23 @@ void tci_plan::build_two_phase_ladder(hipStream_t st) {
       // 128-wide tiles for the long-K d x d outputs)
       ph.p1x.clear();
       ph.p2x.clear();
-      ph.p1x.resize(2);
-      ph.p2x.resize(1);
-      for (int v = 0; v < 2; ++v) {
+      ph.p1x.resize(3);
+      ph.p2x.resize(2);
+      for (int v = 0; v < 3; ++v) {
         ph.p1x[v].dense = true;
-        ph.p1x[v].tile = v ? tile1 : 64;
+        ph.p1x[v].tile = v == 2 ? 192 : (v ? tile1 : 64);
       }
-      ph.p2x[0].dense = true;
+      ph.p2x[0].dense = ph.p2x[1].dense = true;
       ph.p2x[0].tile = tile2;
+      ph.p2x[1].tile = 192;
       int64_t toff = 0;
       int qloc = 0;
       for (int P = 0; P < n_pairs; ++P) {
         const int d = (int)(h_dom_off[P + 1] - h_dom_off[P]);
         const bool full = d == n_v;
-        const int v = d <= 64 ? 0 : 1;
+        const bool b192 = big && is_big(d);
+        const int v = b192 ? 2 : (d <= 64 ? 0 : 1);
+        const int nsp = (int)std::min<int64_t>(b192 ? nsp_big : nsp_small, len);
         double* T = Tbuf.as<double>() + toff;
         // L operand of this pair: rows (jj, a) with leading dimension ld, one
         // jj-block every jstep elements; full domains read L^J itself (J0 via jstride)
@@ -690,15 +703,15 @@ void tci_plan::build_two_phase_ladder(hipStream_t st) {
         ph.p1x[v].add(T, d, (int)(len * d), d, {seg(op(Lb, ld, 1, nullptr, nullptr, lvv_f32, jstride),
                                                     op(C.as<double>() + h_cd_off[P], d, 1), d, 1.0)},
                       false);
-        for (int sp = 0; sp < ph.nsplit; ++sp) {
-          const int64_t j0 = len * sp / ph.nsplit, j1 = len * (sp + 1) / ph.nsplit;
+        for (int sp = 0; sp < nsp; ++sp) {
+          const int64_t j0 = len * sp / nsp, j1 = len * (sp + 1) / nsp;
           std::vector<Seg> sg;
           for (int64_t jj = j0; jj < j1; ++jj) {
             const void* Lj = static_cast<const char*>(Lb) + (size_t)(jj * jstep) * lsz;
             sg.push_back(seg(op(T + jj * d * d, d, 1), op(Lj, ld, 1, nullptr, nullptr, lvv_f32, jstride), d, 1.0));
           }
           double* Rout = (rpart_n > 1 ? Rpart.as<double>() + (int64_t)sp * sum_d2 : R.as<double>()) + h_cd_off[P];
-          ph.p2x[0].add(Rout, d, d, d, sg, true);
+          ph.p2x[b192 ? 1 : 0].add(Rout, d, d, d, sg, true);
         }
         toff += len * (int64_t)d * d;
       }
@@ -709,34 +722,42 @@ void tci_plan::build_two_phase_ladder(hipStream_t st) {
   // 64-wide phase-2 tiles run 4 workgroups per CU, 128-wide ones 2
   const int64_t def_target = ladder_merged ? 512 : 2048;
   if (!autotune_enabled()) {
-    build(def_target, 64, 128);
+    // untuned: no 192 x 192 tiles unless TCI_FP64_P192=1
+    const char* e = std::getenv("TCI_FP64_P192");
+    build(def_target, 64, 128, e && e[0] == '1');
     return;
   }
   // autotuning on the first chunk (C holds no vector yet; the timing does
   // not depend on the values) over fill targets and, for per-walk programs,
-  // the phase-2 tile; then each program's workgroup order
+  // the phase-2 tile and the 192 x 192 tiles; then the phase-1 tile and each
+  // program's workgroup order
+  bool any_big = false;
+  for (int P = 0; P < n_pairs; ++P) any_big = any_big || is_big(h_dom_off[P + 1] - h_dom_off[P]);
   double best = 1e300;
   int64_t bt = def_target;
   int btile = 64;
-  std::string log = "fp64 ladder: (K-split target, phase-2 tile)";
-  for (int tile2 : {64, 128}) {
-    if (ladder_merged && tile2 == 128) break;  // merged programs have their own tiles
-    const int64_t base = tile2 == 64 ? def_target : def_target / 4;
-    for (int64_t t : {base / 2, base, 2 * base, 4 * base}) {
-      build(t, tile2, 128);
-      TCI_HIP(hipStreamSynchronize(st));
-      const double ms = time_ms(st, 2, [&] { ladder_chunk(lchunks[0], st); });
-      log += " " + std::to_string(t) + "/" + std::to_string(tile2) + ":" + fmt_ms(ms);
-      if (ms < best * 0.98) {
-        best = ms;
-        bt = t;
-        btile = tile2;
+  bool bbig = false;
+  std::string log = "fp64 ladder: (K-split target, phase-2 tile, 192-tiles)";
+  for (int big = 0; big <= (any_big && !ladder_merged ? 1 : 0); ++big)
+    for (int tile2 : {64, 128}) {
+      if ((ladder_merged || big) && tile2 == 128) break;  // merged programs have their own tiles
+      const int64_t base = tile2 == 64 ? def_target : def_target / 4;
+      for (int64_t t : {base / 2, base, 2 * base, 4 * base}) {
+        build(t, tile2, 128, big != 0);
+        TCI_HIP(hipStreamSynchronize(st));
+        const double ms = time_ms(st, 2, [&] { ladder_chunk(lchunks[0], st); });
+        log += " " + std::to_string(t) + "/" + std::to_string(tile2) + (big ? "b" : "") + ":" + fmt_ms(ms);
+        if (ms < best * 0.98) {
+          best = ms;
+          bt = t;
+          btile = tile2;
+          bbig = big != 0;
+        }
       }
     }
-  }
   int btile1 = 128;
   if (!ladder_merged && n_local > 0) {  // then the phase-1 tile of domains above 64
-    build(bt, btile, 64);
+    build(bt, btile, 64, bbig);
     TCI_HIP(hipStreamSynchronize(st));
     const double ms = time_ms(st, 2, [&] { ladder_chunk(lchunks[0], st); });
     log += "; phase-1 tile 64:" + fmt_ms(ms);
@@ -745,9 +766,10 @@ void tci_plan::build_two_phase_ladder(hipStream_t st) {
       btile1 = 64;
     }
   }
-  build(bt, btile, btile1);
+  build(bt, btile, btile1, bbig);
   TCI_HIP(hipStreamSynchronize(st));
-  log += " -> " + std::to_string(bt) + "/" + std::to_string(btile) + "/p1 " + std::to_string(btile1) + "; order";
+  log += " -> " + std::to_string(bt) + "/" + std::to_string(btile) + (bbig ? "b" : "") + "/p1 " +
+         std::to_string(btile1) + "; order";
   auto progs = [&](LadderPhase& ph) {
     std::vector<GemmProgram*> v;
     if (ladder_merged) {
