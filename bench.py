@@ -33,6 +33,42 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def ladder_alg_bytes(syn, precision: str) -> float:
+    """Minimum HBM bytes of one sigma's two-phase ladder: per Cholesky vector,
+    each walk's intermediate T is written and read once (d^2 elements each
+    way) and its L^J[D,D] block is read by both phases; walks with the full
+    domain share L^J itself.  Elements are 4 bytes in split form (hi + lo) and
+    8 bytes in FP64 (L^J 4 bytes when stored in FP32)."""
+    d = np.diff(np.asarray(syn.dom_off)).astype(np.float64)
+    full = d == syn.n_v
+    e_t = 8.0 if precision == "fp64" else 4.0
+    e_l = (4.0 if syn.L.dtype == np.float32 else 8.0) if precision == "fp64" else 4.0
+    per_j = 2.0 * e_t * (d ** 2).sum() + 2.0 * e_l * (d[~full] ** 2).sum() + (2.0 * e_l * syn.n_v ** 2 if full.any() else 0.0)
+    return float(per_j * syn.n_chol)
+
+
+def roofline(info, syn, precision: str, seconds: float, traffic, kernel: str) -> dict:
+    """Roofline of the ladder section: the binding roof is the larger of the
+    MFMA time (algorithmic flops, x3 MFMAs for a split product) and the HBM
+    time of the algorithm's minimum bytes; achieved is that quantity over the
+    measured section time."""
+    flops = info.flops_ladder
+    mfma_flops = flops * (1 if precision == "fp64" else 3)
+    alg_bytes = ladder_alg_bytes(syn, precision)
+    t_mfma, t_hbm = mfma_flops / PEAK[precision], alg_bytes / HBM_PEAK
+    r = {"kernel": kernel, "avg_launch_ms": seconds * 1e3, "algorithmic_flops": flops, "algorithmic_bytes": alg_bytes,
+         "traffic": traffic, "traffic_gbs": traffic / seconds / 1e9 if traffic else None,
+         "tflops_algorithmic": flops / seconds / 1e12, "mfma_busy_frac": mfma_flops / seconds / PEAK[precision]}
+    if t_hbm > t_mfma:
+        ach = alg_bytes / seconds
+        r.update({"bound": "hbm", "achieved": ach / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": ach / HBM_PEAK})
+    else:
+        ach = flops / seconds  # algorithmic flops against the published dense peak of the precision
+        r.update({"bound": "mfma", "achieved": ach / 1e12, "peak": PEAK[precision] / 1e12, "unit": "TFLOP/s",
+                  "frac": ach / PEAK[precision]})
+    return r
+
+
 def measured_traffic(workload: str, precision: str):
     """HBM bytes of one sigma's ladder section, measured with rocprofv3 --pmc
     (profiles/ladder_traffic.txt), or None for an unmeasured workload."""
@@ -97,9 +133,10 @@ def main() -> int:
     ap.add_argument("--ladder-launch", default="grouped", choices=["grouped", "per-pair"],
                     help="split ladder of local domains: grouped launches or one launch per walk (comparison)")
     ap.add_argument("--precision", default="bf16x3")
-    ap.add_argument("--graph", action="store_true",
-                    help="replay every sigma build from a captured HIP graph (ladder times then come from "
-                         "separate eager profiled builds)")
+    ap.add_argument("--graph", dest="graph", action="store_true", default=True,
+                    help="replay every sigma build from a captured HIP graph (default; ladder times then come "
+                         "from separate eager profiled builds)")
+    ap.add_argument("--no-graph", dest="graph", action="store_false", help="launch every sigma build eagerly")
     ap.add_argument("--mode", default="davidson", choices=["davidson", "sigma"])
     ap.add_argument("--switch-tol", type=float, default=1e-6)
     ap.add_argument("--no-cpu", action="store_true")
@@ -166,7 +203,6 @@ def main() -> int:
         pr = ladder_probe(a.precision, a.steps) if a.graph else plan.profile_read(a.precision)
         cfg["schedule"] = plan.tuning().strip().splitlines()
         lad = pr["ladder_ms"] / max(pr["ladder_launches"], 1) / 1e3
-        ach = info.flops_ladder / lad
         if a.precision != "fp64":  # precision study: this sigma against the FP64 one
             s_mx = y.numpy()
             plan.sigma(x, y, precision="fp64")
@@ -176,10 +212,8 @@ def main() -> int:
                                   "rayleigh_err_Eh": float(abs(xv @ s_mx - xv @ s_64))}
         out.update({"value": info.flops_sigma / dt / 1e12, "unit": "TFLOP/s", "higher_is_better": True,
                     "ms_per_step": dt * 1e3, "dtype": a.precision, "config": cfg,
-                    "roofline": {"bound": "mfma", "achieved": ach / 1e12, "peak": PEAK[a.precision] / 1e12,
-                                 "unit": "TFLOP/s", "frac": ach / PEAK[a.precision],
-                                 "traffic": measured_traffic(a.workload, a.precision),
-                                 "kernel": "ladder", "avg_launch_ms": lad * 1e3}})
+                    "roofline": roofline(info, syn, a.precision, lad, measured_traffic(a.workload, a.precision),
+                                         f"{a.precision} ladder (phases 1+2, per sigma)")})
         if a.copy_probe_mib:
             n = a.copy_probe_mib * (1 << 20) // 8
             src, dst = tci.DeviceArray(n), tci.DeviceArray(n)
@@ -232,7 +266,6 @@ def main() -> int:
             pm = dict(ladder_probe(a.precision, 3), sigma_ms=pm["sigma_ms"], sigma_calls=pm["sigma_calls"])
         log(f"[bench] graph sigma ms {sig_ms}")
     lad = p64["ladder_ms"] / max(p64["ladder_launches"], 1) / 1e3
-    ach = info.flops_ladder / lad
     sig64 = p64["sigma_ms"] / max(p64["sigma_calls"], 1) / 1e3
     out.update({
         "value": t_m, "unit": "s", "higher_is_better": False, "ms_per_step": t_m * 1e3,
@@ -248,25 +281,16 @@ def main() -> int:
         "sigma_tflops_fp64": info.flops_sigma / sig64 / 1e12,
     })
     out["config"]["schedule"] = plan.tuning().strip().splitlines()
-    tb64 = measured_traffic(a.workload, "fp64")
-    roof64 = {"bound": "mfma", "achieved": ach / 1e12, "peak": PEAK["fp64"] / 1e12, "unit": "TFLOP/s",
-              "frac": ach / PEAK["fp64"], "traffic": tb64, "traffic_gbs": tb64 / lad / 1e9 if tb64 else None,
-              "kernel": "FP64 ladder (dgemm_kernel phases 1+2, per sigma)",
-              "avg_launch_ms": lad * 1e3, "launches": p64["ladder_launches"]}
+    roof64 = roofline(info, syn, "fp64", lad, measured_traffic(a.workload, "fp64"),
+                      "FP64 ladder (dgemm phases 1+2, per sigma)")
+    roof64["launches"] = p64["ladder_launches"]
     if a.precision != "fp64" and pm["ladder_launches"]:
         lm = pm["ladder_ms"] / pm["ladder_launches"] / 1e3
         sgm = pm["sigma_ms"] / max(pm["sigma_calls"], 1) / 1e3
         out["sigma_tflops_mixed"] = info.flops_sigma / sgm / 1e12
-        # algorithmic (FP64-equivalent) ladder flops against the dense bf16
-        # peak; each product costs 3 bf16 MFMAs, so MFMA busy = 3 x frac
-        tb = measured_traffic(a.workload, a.precision)
-        out["roofline"] = {"bound": "mfma", "achieved": info.flops_ladder / lm / 1e12,
-                           "peak": PEAK[a.precision] / 1e12, "unit": "TFLOP/s",
-                           "frac": info.flops_ladder / lm / PEAK[a.precision], "traffic": tb,
-                           "traffic_gbs": tb / lm / 1e9 if tb else None,
-                           "kernel": "bf16x3 ladder (xgemm_kernel phases 1+2, per sigma)",
-                           "mfma_flops_per_algorithmic_flop": 3,
-                           "avg_launch_ms": lm * 1e3, "launches": pm["ladder_launches"]}
+        out["roofline"] = roofline(info, syn, a.precision, lm, measured_traffic(a.workload, a.precision),
+                                   f"{a.precision} ladder (xgemm phases 1+2, per sigma)")
+        out["roofline"]["launches"] = pm["ladder_launches"]
         out["roofline_fp64"] = roof64
     else:
         out["roofline"] = roof64
