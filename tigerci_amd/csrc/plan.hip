@@ -166,12 +166,11 @@ void tci_plan::build(const tci_problem& p) {
   // ---- one-off integral assembly on MFMA -------------------------------------
   std::vector<double> o4((size_t)n_int * n_int * n_int * n_int);
   {
-    std::vector<int32_t> rec_ov(niv), rec_oo((size_t)n_int * n_int), rec_vo(niv), rec_aa(n_v),
+    std::vector<int32_t> rec_ov(niv), rec_oo((size_t)n_int * n_int), rec_aa(n_v),
         rec_pairs((size_t)n_int * (n_int + 1) / 2), rec_vv((size_t)n_v * n_v);
     for (int i = 0; i < n_int; ++i)
       for (int a = 0; a < n_v; ++a) {
         rec_ov[(size_t)i * n_v + a] = (int32_t)rec(n_int + a, i);
-        rec_vo[(size_t)a * n_int + i] = (int32_t)rec(n_int + a, i);
       }
     for (int k = 0; k < n_int; ++k)
       for (int i = 0; i < n_int; ++i) rec_oo[(size_t)k * n_int + i] = (int32_t)rec(k, i);
@@ -182,10 +181,9 @@ void tci_plan::build(const tci_problem& p) {
       for (int b = 0; b < n_v; ++b) rec_vv[(size_t)a * n_v + b] = (int32_t)rec(n_int + a, n_int + b);
     }
     TCI_REQUIRE(npair_all < ((int64_t)1 << 31), "too many orbital pairs");
-    DevBuf d_ov, d_oo, d_vo, d_aa, d_pr, d_vv;
+    DevBuf d_ov, d_oo, d_aa, d_pr, d_vv;
     upload_now(d_ov, rec_ov);
     upload_now(d_oo, rec_oo);
-    upload_now(d_vo, rec_vo);
     upload_now(d_aa, rec_aa);
     upload_now(d_pr, rec_pairs);
     upload_now(d_vv, rec_vv);
@@ -210,8 +208,9 @@ void tci_plan::build(const tci_problem& p) {
     asmb.add(O4.as<double>(), (int64_t)n_int * n_int, n_int * n_int, n_int * n_int,
              {seg(op(L, n_chol, 1, d_oo.as<int32_t>()), op(L, 1, n_chol, nullptr, d_oo.as<int32_t>()), K, 1.0)},
              false);
+    // O3[k][p][q][b] = (kp|bq): rows (k,p), columns (q,b) with b fastest
     asmb.add(O3.as<double>(), niv, n_int * n_int, (int)niv,
-             {seg(op(L, n_chol, 1, d_oo.as<int32_t>()), op(L, 1, n_chol, nullptr, d_vo.as<int32_t>()), K, 1.0)},
+             {seg(op(L, n_chol, 1, d_oo.as<int32_t>()), op(L, 1, n_chol, nullptr, d_ov.as<int32_t>()), K, 1.0)},
              false);
     asmb.add(vvd.as<double>(), n_v, n_v, n_v,
              {seg(op(L, n_chol, 1, d_aa.as<int32_t>()), op(L, 1, n_chol, nullptr, d_aa.as<int32_t>()), K, 1.0)},
@@ -324,10 +323,10 @@ void tci_plan::build(const tci_problem& p) {
   part.alloc((size_t)std::max(n_pairs, 1) * 8);
   sig0.alloc(8);
   {
-    // K-splits of gW = W L: enough 64 x 64 tiles for the chip, at least 16
-    // Cholesky vectors per split
+    // K-splits of gW = W L (a product with only n_int rows that streams all of
+    // L): about 2048 64 x 64 tiles, at least 4 Cholesky vectors per split
     const int64_t tiles = cdiv(n_int, 64) * cdiv(n_v, 64);
-    n_split = (int)std::max<int64_t>(1, std::min<int64_t>(std::min<int64_t>(n_chol / 16, 256), cdiv(1024, tiles)));
+    n_split = (int)std::max<int64_t>(1, std::min<int64_t>(std::min<int64_t>(n_chol / 4, 512), cdiv(2048, tiles)));
   }
   gW.alloc((size_t)n_split * niv * 8);
 
@@ -854,7 +853,7 @@ void tci_plan::sigma_body(const double* x, double* y, const IoSlot* io, int flag
   }
   launch_gemv(Mss.as<double>(), (int64_t)n_int * n_v, s.as<double>(), gss.as<double>(), st);
   progGW.launch(st);
-  launch_s3int_singles(tab, gpart_off.as<int64_t>(), Ct.as<double>(), O3.as<double>(), gpart.as<double>(), st);
+  launch_s3int_singles(tab, dmax, gpart_off.as<int64_t>(), Ct.as<double>(), O3.as<double>(), gpart.as<double>(), st);
   InvTab I{inv_off.as<int64_t>(), inv_q.as<int32_t>(), inv_pos.as<int32_t>()};
   launch_singles_reduce(tab, I, gpart_off.as<int64_t>(), gpart.as<double>(), gss.as<double>(), gW.as<double>(), n_split,
                         g.as<double>(), st);

@@ -88,11 +88,12 @@ __global__ void rterms_kernel(PairTab T, const double* __restrict__ x, const dou
     const int r = (int)(e / d), c = (int)(e - (int64_t)r * d);
     const int a = T.dom_idx[off + r], b = T.dom_idx[off + c];
     double v = c0 * Kov[((int64_t)i * nv + a) * niv + (int64_t)j * nv + b];
-    // O3[k][p][b][q] = (kp|bq)
+    // O3[k][p][q][b] = (kp|bq): for fixed k the lanes of a row read one
+    // n_v-long row of O3 and of s (gathered by the domain), the rest broadcast
     double t = 0.0;
     for (int k = 0; k < n; ++k) {
-      t += O3[(((int64_t)k * n + i) * nv + b) * n + j] * s[(int64_t)k * nv + a];
-      t += O3[(((int64_t)k * n + j) * nv + a) * n + i] * s[(int64_t)k * nv + b];
+      t += O3[(((int64_t)k * n + i) * n + j) * nv + b] * s[(int64_t)k * nv + a];
+      t += O3[(((int64_t)k * n + j) * n + i) * nv + a] * s[(int64_t)k * nv + b];
     }
     Rb[e] += v - t;
   }
@@ -178,46 +179,100 @@ __global__ void sum_kernel(const double* __restrict__ part, int n, double* __res
 }
 
 // block (Q, o): ordered pair (j,k) = o==0 ? (qi,qj) : (qj,qi)
-__global__ void s3int_singles_kernel(PairTab T, const int64_t* __restrict__ gpart_off,
-                                     const double* __restrict__ Ct, const double* __restrict__ O3,
-                                     double* __restrict__ gpart) {
-  const int Q = blockIdx.x, o = blockIdx.y;
+// out[i][r] = -sum_c Ct'[r][c] W[i][c], W[i][c] = (jj i|kk D[c]) (a row of O3),
+// Ct' = Ct (o = 0) or Ct^T (o = 1).  Block (Q, o, 64 rows of r); c in chunks of
+// 32 staged in LDS with coalesced reads of Ct, i in groups of 64 (16 per thread).
+constexpr int S3R = 64, S3C = 32;
+
+__global__ __launch_bounds__(256) void s3int_singles_kernel(PairTab T, const int64_t* __restrict__ gpart_off,
+                                                            const double* __restrict__ Ct,
+                                                            const double* __restrict__ O3,
+                                                            double* __restrict__ gpart) {
+  const int Q = blockIdx.x, o = blockIdx.y, r0 = blockIdx.z * S3R;
   const int qi = T.pairs[2 * Q], qj = T.pairs[2 * Q + 1];
-  if (o == 1 && qi == qj) return;
-  const int jj = o == 0 ? qi : qj, kk = o == 0 ? qj : qi;
   const int64_t off = T.dom_off[Q];
   const int d = (int)(T.dom_off[Q + 1] - off);
+  if ((o == 1 && qi == qj) || r0 >= d) return;
+  const int jj = o == 0 ? qi : qj, kk = o == 0 ? qj : qi;
   const int n = T.n_int, nv = T.n_v;
   const double* Cb = Ct + T.cd_off[Q];
-  double* out = gpart + gpart_off[2 * Q + o];   // [n_int][d]
-  for (int e = threadIdx.x + blockIdx.z * blockDim.x; e < n * d; e += blockDim.x * gridDim.z) {
-    const int i = e / d, r = e - (e / d) * d;
-    double acc = 0.0;
-    for (int c = 0; c < d; ++c) {
-      const double ct = (o == 0) ? Cb[(int64_t)r * d + c] : Cb[(int64_t)c * d + r];
-      const int b = T.dom_idx[off + c];
-      acc += ct * O3[(((int64_t)jj * n + i) * nv + b) * n + kk];   // (ji|kb) = (ji|bk)
+  double* out = gpart + gpart_off[2 * Q + o];  // [n_int][d]
+  __shared__ double sC[S3R][S3C + 1];
+  __shared__ double sW[64][S3C + 1];
+  const int t = threadIdx.x, r = t & 63, ig = t >> 6;
+  for (int i0 = 0; i0 < n; i0 += 64) {
+    double acc[16];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) acc[q] = 0.0;
+    for (int c0 = 0; c0 < d; c0 += S3C) {
+      __syncthreads();
+      for (int e = t; e < S3R * S3C; e += 256) {
+        int rr, cc;
+        if (o == 0) {
+          rr = e / S3C;
+          cc = e % S3C;
+        } else {
+          rr = e % S3R;
+          cc = e / S3R;
+        }
+        const int gr = r0 + rr, gc = c0 + cc;
+        sC[rr][cc] = (gr < d && gc < d) ? (o == 0 ? Cb[(int64_t)gr * d + gc] : Cb[(int64_t)gc * d + gr]) : 0.0;
+      }
+      for (int e = t; e < 64 * S3C; e += 256) {
+        const int ii = e / S3C, cc = e % S3C;
+        const int gi = i0 + ii, gc = c0 + cc;
+        sW[ii][cc] = (gi < n && gc < d) ? O3[(((int64_t)jj * n + gi) * n + kk) * nv + T.dom_idx[off + gc]] : 0.0;
+      }
+      __syncthreads();
+#pragma unroll
+      for (int q = 0; q < 16; ++q) {
+        double a = 0.0;
+#pragma unroll 8
+        for (int cc = 0; cc < S3C; ++cc) a += sW[ig + 4 * q][cc] * sC[r][cc];
+        acc[q] += a;
+      }
     }
-    out[(int64_t)i * d + r] = -acc;
+    if (r0 + r < d)
+#pragma unroll
+      for (int q = 0; q < 16; ++q) {
+        const int i = i0 + ig + 4 * q;
+        if (i < n) out[(int64_t)i * d + r0 + r] = -acc[q];
+      }
   }
 }
 
-__global__ void singles_reduce_kernel(PairTab T, InvTab I, const int64_t* __restrict__ gpart_off,
-                                      const double* __restrict__ gpart, const double* __restrict__ gss,
-                                      const double* __restrict__ gW, int n_split, double* __restrict__ g) {
-  const int nv = T.n_v;
-  const int64_t n = (int64_t)T.n_int * nv;
-  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x) {
-    const int i = (int)(e / nv), a = (int)(e - (int64_t)i * nv);
-    double v = gss[e];
-    for (int sp = 0; sp < n_split; ++sp) v += gW[(int64_t)sp * n + e];
-    for (int64_t q = I.inv_off[a]; q < I.inv_off[a + 1]; ++q) {
-      const int Q = I.inv_q[q], pos = I.inv_pos[q];
-      const int d = (int)(T.dom_off[Q + 1] - T.dom_off[Q]);
-      v += gpart[gpart_off[2 * Q] + (int64_t)i * d + pos];
-      if (T.pairs[2 * Q] != T.pairs[2 * Q + 1]) v += gpart[gpart_off[2 * Q + 1] + (int64_t)i * d + pos];
+// g[i][a] = gss + sum of the K-split partials of gW + every gpart entry of a
+// walk whose domain holds a.  One block per virtual a; thread (i, slice) sums
+// the partials and inverse-map entries of its slice for internal i (64 at a
+// time), and the four slices meet in LDS (fixed order: deterministic).
+__global__ __launch_bounds__(256) void singles_reduce_kernel(PairTab T, InvTab I, const int64_t* __restrict__ gpart_off,
+                                                             const double* __restrict__ gpart,
+                                                             const double* __restrict__ gss,
+                                                             const double* __restrict__ gW, int n_split,
+                                                             double* __restrict__ g) {
+  const int nv = T.n_v, no = T.n_int, a = blockIdx.x;
+  const int64_t n = (int64_t)no * nv;
+  const int il = threadIdx.x & 63, sl = threadIdx.x >> 6;
+  __shared__ double part[4][64];
+  const int64_t q0 = I.inv_off[a], q1 = I.inv_off[a + 1];
+  for (int i0 = 0; i0 < no; i0 += 64) {
+    const int i = i0 + il;
+    double v = 0.0;
+    if (i < no) {
+      const int64_t e = (int64_t)i * nv + a;
+      if (sl == 0) v = gss[e];
+      for (int sp = sl; sp < n_split; sp += 4) v += gW[(int64_t)sp * n + e];
+      for (int64_t q = q0 + sl; q < q1; q += 4) {
+        const int Q = I.inv_q[q], pos = I.inv_pos[q];
+        const int d = (int)(T.dom_off[Q + 1] - T.dom_off[Q]);
+        v += gpart[gpart_off[2 * Q] + (int64_t)i * d + pos];
+        if (T.pairs[2 * Q] != T.pairs[2 * Q + 1]) v += gpart[gpart_off[2 * Q + 1] + (int64_t)i * d + pos];
+      }
     }
-    g[e] = v;
+    part[sl][il] = v;
+    __syncthreads();
+    if (sl == 0 && i < no) g[(int64_t)i * nv + a] = ((part[0][il] + part[1][il]) + part[2][il]) + part[3][il];
+    __syncthreads();
   }
 }
 
@@ -496,16 +551,17 @@ void launch_ref_dot(const PairTab& T, const double* Kov, const double* Ct, doubl
   TCI_HIP(hipGetLastError());
 }
 
-void launch_s3int_singles(const PairTab& T, const int64_t* gpart_off, const double* Ct, const double* O3,
+void launch_s3int_singles(const PairTab& T, int dmax, const int64_t* gpart_off, const double* Ct, const double* O3,
                           double* gpart, hipStream_t st) {
   if (T.n_pairs == 0) return;
-  hipLaunchKernelGGL(s3int_singles_kernel, dim3(T.n_pairs, 2, 4), dim3(256), 0, st, T, gpart_off, Ct, O3, gpart);
+  hipLaunchKernelGGL(s3int_singles_kernel, dim3(T.n_pairs, 2, (unsigned)cdiv(dmax, S3R)), dim3(256), 0, st, T,
+                     gpart_off, Ct, O3, gpart);
   TCI_HIP(hipGetLastError());
 }
 
 void launch_singles_reduce(const PairTab& T, const InvTab& I, const int64_t* gpart_off, const double* gpart,
                            const double* gss, const double* gW, int n_split, double* g, hipStream_t st) {
-  hipLaunchKernelGGL(singles_reduce_kernel, dim3(gx((int64_t)T.n_int * T.n_v)), dim3(256), 0, st, T, I, gpart_off,
+  hipLaunchKernelGGL(singles_reduce_kernel, dim3((unsigned)T.n_v), dim3(256), 0, st, T, I, gpart_off,
                      gpart, gss, gW, n_split, g);
   TCI_HIP(hipGetLastError());
 }

@@ -48,7 +48,7 @@ void launch_ref_dot(const PairTab& T, const double* Kov, const double* Ct, doubl
                     double* sig0, hipStream_t st);
 
 // singles <- doubles through (ji|kb): gpart[(2Q+o)][i][pos] = -sum_c Ct^{jk}[pos][c] (ji|kb)
-void launch_s3int_singles(const PairTab& T, const int64_t* gpart_off, const double* Ct,
+void launch_s3int_singles(const PairTab& T, int dmax, const int64_t* gpart_off, const double* Ct,
                           const double* O3, double* gpart, hipStream_t st);
 
 // g[i][a] = gss[i][a] + sum_s gW[s][i][a] + sum_{(Q,o,pos): D_Q[pos]=a} gpart[(2Q+o)][i][pos]
