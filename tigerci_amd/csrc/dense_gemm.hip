@@ -94,14 +94,14 @@ __device__ __forceinline__ void dstore(int tid, const double* a, const double* b
 
 // Epilogue shared by the tile shapes: lane (li, lg) of block (i, j) holds rows
 // 16i + lg + 4r, column 16j + li; plain, row-permuted or blocked output.
-template <int NBM, int NBN>
+template <int NBM, int NBN, int BSM = 16, int BSN = 16>
 __device__ __forceinline__ void dstore_out(const Task& t, const d4 (&acc)[NBM > 0 ? NBM : 1][NBN > 0 ? NBN : 1],
                                            int wm, int wn, int li, int lg) {
   // column offsets of this lane (one per j), then one row offset per (i, r)
   int64_t coff[NBN > 0 ? NBN : 1];
 #pragma unroll
   for (int j = 0; j < NBN; ++j) {
-    const int col = t.n0 + wn + j * 16 + li;
+    const int col = t.n0 + wn + j * BSN + li;
     const int cb = t.cdiv ? col / t.cdiv : 0;
     coff[j] = (int64_t)cb * t.cstride + (col - cb * t.cdiv);
   }
@@ -109,14 +109,14 @@ __device__ __forceinline__ void dstore_out(const Task& t, const d4 (&acc)[NBM > 
   for (int i = 0; i < NBM; ++i)
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
-      const int row = t.m0 + wm + i * 16 + lg + 4 * r;
+      const int row = t.m0 + wm + i * BSM + lg + 4 * r;
       if (row >= t.M) continue;
       const int prow = t.orow_div > 0 ? (row % t.orow_div) * (t.M / t.orow_div) + row / t.orow_div : row;
       const int64_t ro = t.rdiv ? (int64_t)(row / t.rdiv) * t.rstride + (int64_t)(row % t.rdiv) * t.ldc
                                 : (int64_t)prow * t.ldc;
 #pragma unroll
       for (int j = 0; j < NBN; ++j) {
-        const int col = t.n0 + wn + j * 16 + li;
+        const int col = t.n0 + wn + j * BSN + li;
         if (col < t.N) {
           __attribute__((address_space(1))) double* p = (__attribute__((address_space(1))) double*)(t.C + ro + coff[j]);
           *p = t.accumulate ? *p + acc[i][j][r] : acc[i][j][r];
@@ -225,10 +225,10 @@ __global__ __launch_bounds__(256, TS == 128 ? 2 : 4) void dgemm_kernel(const Tas
 
 
 // ---------------------------------------------------------------- 192 x 192
-// Eight waves (2 x 4) of 96 x 48 on one 192 x 192 tile: a whole d x d pair
+// Eight waves (2 x 4) on one 192 x 192 tile: a whole d x d pair
 // block of a domain of up to 192 per workgroup, so each operand panel is read
 // once (24 flops per byte against 8 for 64-wide tiles).  99 KiB of LDS, one
-// workgroup per CU; 18 accumulator blocks per wave.  A stage is 192 x 16 of A
+// workgroup per CU; up to 18 accumulator blocks per wave.  A stage is 192 x 16 of A
 // (thread: k = tid & 15, rows tid >> 4 + 32 q) and 16 x 192 of B (thread:
 // columns tid % 64 + 64 c, rows tid / 64 + 8 r), loads clamped and masked as
 // in dload_t.
@@ -317,9 +317,9 @@ __device__ __forceinline__ void dgemm192_body(const Task& t, const Seg* __restri
         const int kr = kk * 4 + lg;
         double a[NBM > 0 ? NBM : 1], b[NBN > 0 ? NBN : 1];
 #pragma unroll
-        for (int i = 0; i < NBM; ++i) a[i] = As[kr * L192 + wm + i * 16 + li];
+        for (int i = 0; i < NBM; ++i) a[i] = As[kr * L192 + wm + i * 32 + li];
 #pragma unroll
-        for (int j = 0; j < NBN; ++j) b[j] = Bs[kr * L192 + wn + j * 16 + li];
+        for (int j = 0; j < NBN; ++j) b[j] = Bs[kr * L192 + wn + j * 64 + li];
 #pragma unroll
         for (int i = 0; i < NBM; ++i)
 #pragma unroll
@@ -332,7 +332,7 @@ __device__ __forceinline__ void dgemm192_body(const Task& t, const Seg* __restri
     buf ^= 1;
     more = dadvance(segs, t.seg1, nxt.seg, nxt.kb);
   }
-  dstore_out<NBM, NBN>(t, acc, wm, wn, li, lg);
+  dstore_out<NBM, NBN, 32, 64>(t, acc, wm, wn, li, lg);
 }
 
 __global__ __launch_bounds__(512, 1) void dgemm192_kernel(const Task* __restrict__ tasks, const Seg* __restrict__ segs,
@@ -342,9 +342,12 @@ __global__ __launch_bounds__(512, 1) void dgemm192_kernel(const Task* __restrict
   const int tid = threadIdx.x;
   const int lane = tid & 63, w = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int li = lane & 15, lg = lane >> 4;
-  const int wm = (w >> 2) * 96, wn = (w & 3) * 48;
-  const int nbm = __builtin_amdgcn_readfirstlane(max(0, min(6, (t.M - (t.m0 + wm) + 15) >> 4)));
-  const int nbn = __builtin_amdgcn_readfirstlane(max(0, min(3, (t.N - (t.n0 + wn) + 15) >> 4)));
+  // 16 x 16 blocks dealt round-robin to the waves (row block 2i + w/4, column
+  // block 4j + w%4), so a domain smaller than 192 leaves all waves about
+  // equally loaded
+  const int wm = (w >> 2) * 16, wn = (w & 3) * 16;
+  const int nbm = __builtin_amdgcn_readfirstlane(max(0, min(6, (t.M - (t.m0 + wm) + 31) / 32)));
+  const int nbn = __builtin_amdgcn_readfirstlane(max(0, min(3, (t.N - (t.n0 + wn) + 63) / 64)));
   int seg0 = t.seg0;
   while (seg0 < t.seg1 && segs[seg0].K <= 0) ++seg0;
   if (seg0 >= t.seg1) {

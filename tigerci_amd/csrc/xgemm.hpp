@@ -116,12 +116,20 @@ __device__ __forceinline__ void xstage(const Seg& S, const Task& t, int64_t J0, 
   xload<TN>(S.B, J0, t.N, t.n0, kb, S.K, tid, vb);
 }
 
+// distance between a wave's consecutive 16 x 16 blocks: 16 (quadrants) or 32
+// (round-robin over two waves, 192 x 192 tiles)
+template <int TM, int TN>
+struct XInterleave {
+  static constexpr int BS = (TM == 192 && TN == 192) ? 32 : 16;
+};
+
 template <int TM, int TN, int PREC, int NBM, int NBN>
 __device__ __forceinline__ void xgemm_body(const Task& t, const Seg* __restrict__ segs, int64_t J0, int seg0,
                                            const float* __restrict__ scales, uint16_t* sm, int tid, int wm, int wn,
                                            int li, int lg) {
   typedef typename XT<PREC>::T T16;
   typedef typename XT<PREC>::V V;
+  constexpr int BS = XInterleave<TM, TN>::BS;
   // stage layout: A hi, A lo (TM rows each), B hi, B lo (TN rows each)
   constexpr int PA = TM * XK, PB = TN * XK, STG = 2 * PA + 2 * PB;
   f4 acc[NBM > 0 ? NBM : 1][NBN > 0 ? NBN : 1];
@@ -144,13 +152,13 @@ __device__ __forceinline__ void xgemm_body(const Task& t, const Seg* __restrict_
       V ah[NBM > 0 ? NBM : 1], al[NBM > 0 ? NBM : 1], bh[NBN > 0 ? NBN : 1], bl[NBN > 0 ? NBN : 1];
 #pragma unroll
       for (int i = 0; i < NBM; ++i) {
-        const int e = xsw(wm + i * 16 + li, lg);
+        const int e = xsw(wm + i * BS + li, lg);
         ah[i] = *reinterpret_cast<const V*>(base + e);
         al[i] = *reinterpret_cast<const V*>(base + PA + e);
       }
 #pragma unroll
       for (int j = 0; j < NBN; ++j) {
-        const int e = xsw(wn + j * 16 + li, lg);
+        const int e = xsw(wn + j * BS + li, lg);
         bh[j] = *reinterpret_cast<const V*>(base + 2 * PA + e);
         bl[j] = *reinterpret_cast<const V*>(base + 2 * PA + PB + e);
       }
@@ -183,12 +191,12 @@ __device__ __forceinline__ void xgemm_body(const Task& t, const Seg* __restrict_
   int64_t roff[NBM > 0 ? NBM : 1];
 #pragma unroll
   for (int i = 0; i < NBM; ++i) {
-    const int row = t.m0 + wm + i * 16 + li;
+    const int row = t.m0 + wm + i * BS + li;
     roff[i] = t.rdiv ? (int64_t)(row / t.rdiv) * t.rstride + (int64_t)(row % t.rdiv) * t.ldc : (int64_t)row * t.ldc;
   }
 #pragma unroll
   for (int j = 0; j < NBN; ++j) {
-    const int col = t.n0 + wn + j * 16 + 4 * lg;
+    const int col = t.n0 + wn + j * BS + 4 * lg;
     if (col >= t.N) continue;
     const int cb = t.cdiv ? col / t.cdiv : 0, cc = col - cb * t.cdiv;
     const int64_t coff = (int64_t)cb * t.cstride;
@@ -198,7 +206,7 @@ __device__ __forceinline__ void xgemm_body(const Task& t, const Seg* __restrict_
     const bool whole = t.cdiv ? ((cc & 3) == 0 && cc + 3 < t.cdiv) : (t.out_split || col + 3 < t.N);
 #pragma unroll
     for (int i = 0; i < NBM; ++i) {
-      const int row = t.m0 + wm + i * 16 + li;
+      const int row = t.m0 + wm + i * BS + li;
       if (row >= t.M) continue;
       const f4 v = acc[i][j] * osc;
       if (whole) {
@@ -254,9 +262,14 @@ __global__ __launch_bounds__(256, TM == 64 ? 4 : (TM + TN > 320 ? 1 : 2)) void x
   const int tid = threadIdx.x;
   const int lane = tid & 63, w = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int li = lane & 15, lg = lane >> 4;
-  const int wm = (w >> 1) * (TM / 2), wn = (w & 1) * (TN / 2);
-  const int nbm = __builtin_amdgcn_readfirstlane(max(0, min(WBM, (t.M - (t.m0 + wm) + 15) >> 4)));
-  const int nbn = __builtin_amdgcn_readfirstlane(max(0, min(WBN, (t.N - (t.n0 + wn) + 15) >> 4)));
+  // 192 x 192 tiles deal their 16 x 16 blocks round-robin to the waves (row
+  // block 2i + w/2, column block 2j + w%2), so a domain smaller than 192 leaves
+  // the four waves about equally loaded; other tiles give each wave a quadrant
+  constexpr int BS = XInterleave<TM, TN>::BS;
+  const int wm = BS == 32 ? (w >> 1) * 16 : (w >> 1) * (TM / 2);
+  const int wn = BS == 32 ? (w & 1) * 16 : (w & 1) * (TN / 2);
+  const int nbm = __builtin_amdgcn_readfirstlane(max(0, min(WBM, (t.M - (t.m0 + wm) + BS - 1) / BS)));
+  const int nbn = __builtin_amdgcn_readfirstlane(max(0, min(WBN, (t.N - (t.n0 + wn) + BS - 1) / BS)));
   int seg0 = t.seg0;
   while (seg0 < t.seg1 && segs[seg0].K <= 0) ++seg0;
   if (seg0 >= t.seg1) {
