@@ -1,0 +1,59 @@
+"""bench.py contract on the CPU: the one-GPU guard and the roofline arithmetic."""
+import importlib.util
+import json
+import os
+import subprocess
+import sys
+from types import SimpleNamespace
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _bench():
+    spec = importlib.util.spec_from_file_location("bench_mod", os.path.join(ROOT, "bench.py"))
+    m = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(m)
+    return m
+
+
+def test_more_than_one_gpu_exits_with_message():
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 2
+    msg = json.loads(r.stdout.strip().splitlines()[-1])
+    assert "one GPU" in msg["error"]
+
+
+def _syn(n_v, doms, n_chol, f32=False):
+    off = np.concatenate([[0], np.cumsum(doms)])
+    L = np.zeros(1, dtype=np.float32 if f32 else np.float64)
+    return SimpleNamespace(n_v=n_v, n_chol=n_chol, dom_off=off, L=L)
+
+
+def test_ladder_minimum_bytes():
+    b = _bench()
+    # two local walks (d = 10, 20) and one full walk (d = n_v = 30), 7 vectors
+    syn = _syn(30, [10, 20, 30], 7)
+    d2 = 100 + 400 + 900
+    assert b.ladder_alg_bytes(syn, "bf16x3") == 7 * (2 * 4 * d2 + 2 * 4 * (100 + 400) + 2 * 4 * 900)
+    assert b.ladder_alg_bytes(syn, "fp64") == 7 * (2 * 8 * d2 + 2 * 8 * (100 + 400) + 2 * 8 * 900)
+    syn32 = _syn(30, [10, 20], 7, f32=True)
+    assert b.ladder_alg_bytes(syn32, "fp64") == 7 * (2 * 8 * 500 + 2 * 4 * 500)
+
+
+def test_roofline_picks_the_binding_roof():
+    b = _bench()
+    syn = _syn(192, [192] * 0 + [150] * 100, 1000)
+    info = SimpleNamespace(flops_ladder=4.0 * 100 * 150 ** 3 * 1000)
+    alg = b.ladder_alg_bytes(syn, "bf16x3")
+    r = b.roofline(info, syn, "bf16x3", 0.05, None, "x")
+    # 3 MFMAs per split product at 2.5 PF/s against the minimum bytes at 8 TB/s
+    if alg / 8e12 > 3 * info.flops_ladder / 2.5e15:
+        assert r["bound"] == "hbm" and abs(r["achieved"] - alg / 0.05 / 1e9) < 1e-6 and r["unit"] == "GB/s"
+    else:
+        assert r["bound"] == "mfma"
+    r64 = b.roofline(info, syn, "fp64", 0.2, 1e12, "x")
+    assert r64["bound"] == "mfma" and abs(r64["frac"] - info.flops_ladder / 0.2 / 78.6e12) < 1e-12
+    assert abs(r64["traffic_gbs"] - 1e12 / 0.2 / 1e9) < 1e-9

@@ -113,3 +113,29 @@ def test_fp64_192_tiles():
         _say(f"  rel err default {_rel(a, ref):.2e}  192-tiles {_rel(b, ref):.2e}")
         assert _rel(a, ref) < 1e-10 and _rel(b, ref) < 1e-10
         assert _rel(b, a) < 1e-12
+
+
+@pytest.mark.parametrize("precision", ["bf16x3", "fp16x3"])
+def test_fused_split_ladder(precision):
+    """The fused split ladder (T' kept on chip) for domains of 129-192
+    reproduces the oracle and the two-phase ladder to the split precision."""
+    _say("case: fused split ladder", precision)
+    prob = make_problem(3, 200, 24, 53, domain_range=(100, 192), weak_below=0)
+    ints = Integrals(prob)
+    x = random_ci_vector(prob, 12)
+    ref = oracle_sigma(prob, x, ints)
+    os.environ["TCI_AUTOTUNE"] = "0"
+    try:
+        a = _sigma(tci.Plan.from_problem(prob), x, precision=precision)
+        os.environ["TCI_SPLIT_FUSED"] = "1"
+        fused = tci.Plan.from_problem(prob)
+        b = _sigma(fused, x, precision=precision)
+        b2 = _sigma(fused, x, precision=precision)
+    finally:
+        os.environ.pop("TCI_SPLIT_FUSED", None)
+        del os.environ["TCI_AUTOTUNE"]
+    tol = 5e-5 if precision == "bf16x3" else 2e-6
+    _say(f"  rel err two-phase {_rel(a, ref):.2e}  fused {_rel(b, ref):.2e}  fused vs two-phase {_rel(b, a):.2e}")
+    assert _rel(a, ref) < tol and _rel(b, ref) < tol
+    assert _rel(b, a) < tol / 5
+    assert np.array_equal(b, b2)

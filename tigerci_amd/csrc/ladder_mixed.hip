@@ -355,7 +355,10 @@ void mixed_prepare(tci_plan& P, int precision, hipStream_t st) {
   // programs for a K-split fill target: phase 2 of each tile shape is split
   // along K until it has about `target` workgroups (at most 16 splits);
   // big: phase 2 of domains of 129-192 on single 192 x 192 tiles
-  auto build = [&](int64_t target, bool big) {
+  // fused: walks of 129-192 (prepacked) run the fused kernel over all J
+  // instead of the two phases, with about fused_target workgroups
+  auto fusable = [&](int64_t d) { return M.prepacked && d > 128 && d <= 192 && d != n_v; };
+  auto build = [&](int64_t target, bool big, bool fused, int64_t fused_target) {
     const int tile2 = 128;
     const int64_t tiles2m = cdiv((int64_t)n_pairs * n_v, tile2) * cdiv(n_v, tile2);
     M.rpart_n = (int)std::max<int64_t>(1, std::min<int64_t>(16, cdiv(target, std::max<int64_t>(tiles2m, 1))));
@@ -365,10 +368,15 @@ void mixed_prepare(tci_plan& P, int precision, hipStream_t st) {
     static const int t2m[4] = {64, 128, 128, 192}, t2n[4] = {64, 128, 192, 192};
     std::vector<int> v2of(n_pairs, 1);
     int nsplit_v[4] = {1, 1, 1, 1};
+    int nsf = 1;
     if (!M.merged) {
-      int64_t tv[4] = {0, 0, 0, 0};
+      int64_t tv[4] = {0, 0, 0, 0}, tf = 0;
       for (int Q = 0; Q < n_pairs; ++Q) {
         const int64_t d = P.h_dom_off[Q + 1] - P.h_dom_off[Q];
+        if (fused && fusable(d)) {
+          tf += cdiv(d, 64);
+          continue;
+        }
         int v = d <= 64 ? 0 : (d <= 128 ? 1 : ((double)d / (192.0 * cdiv(d, 192)) >= (double)d / (128.0 * cdiv(d, 128)) ? 2 : 1));
         if (big && d > 128 && d <= 192) v = 3;
         v2of[Q] = v;
@@ -379,6 +387,10 @@ void mixed_prepare(tci_plan& P, int precision, hipStream_t st) {
         nsplit_v[v] = tv[v] ? (int)std::max<int64_t>(1, std::min<int64_t>(16, cdiv(target, tv[v]))) : 1;
         M.rpart_n = std::max(M.rpart_n, nsplit_v[v]);
       }
+      if (tf) {
+        nsf = (int)std::max<int64_t>(1, std::min<int64_t>(16, cdiv(fused_target, tf)));
+        M.rpart_n = std::max(M.rpart_n, nsf);
+      }
     }
     if (M.rpart_n > 1 && M.Rpart.bytes < (size_t)M.rpart_n * P.sum_d2 * 8) {
       M.bytes -= (int64_t)M.Rpart.bytes;
@@ -387,6 +399,7 @@ void mixed_prepare(tci_plan& P, int precision, hipStream_t st) {
     }
     M.phase.clear();
     M.chunks.clear();
+    M.fused_h.clear();
     std::vector<int64_t> lens;
     for (int64_t J0 = 0; J0 < n_chol; J0 += Lc) {
       const int64_t len = std::min(Lc, n_chol - J0);
@@ -488,6 +501,28 @@ void mixed_prepare(tci_plan& P, int precision, hipStream_t st) {
           jstep = (int64_t)d * ldx;
           jstride = 0;
         }
+        if (fused && fusable(d)) {
+          // every J at once (prepacked), so only the first chunk length's
+          // phase builds the tasks
+          if (&ph == &M.phase.front())
+            for (int sp = 0; sp < nsf; ++sp)
+              for (int a0 = 0; a0 < d; a0 += 64) {
+                FusedTask f{};
+                f.L = Lb;
+                f.C = Cx + cxo[Q];
+                f.R = (M.rpart_n > 1 ? M.Rpart.as<double>() + (int64_t)sp * P.sum_d2 : P.R.as<double>()) +
+                      P.h_cd_off[Q];
+                f.jstep = jstep;
+                f.d = d;
+                f.ldx = (int32_t)ldx;
+                f.a0 = a0;
+                f.j0 = (int32_t)(n_chol * sp / nsf);
+                f.j1 = (int32_t)(n_chol * (sp + 1) / nsf);
+                M.fused_h.push_back(f);
+              }
+          toff += len * d * ldx;
+          continue;
+        }
         uint16_t* Tq = Tx + toff;
         int v1 = 0;
         for (int v = 1; v < 3; ++v)
@@ -527,6 +562,8 @@ void mixed_prepare(tci_plan& P, int precision, hipStream_t st) {
       for (auto& g : ph.pp1) g.upload_tables(st);
       for (auto& g : ph.pp2) g.upload_tables(st);
     }
+    M.fused = !M.fused_h.empty();
+    if (M.fused) upload(M.fused_d, M.fused_h, st);
   };
   const int64_t def_target = 512;
   auto progs = [&](tci_plan::LadderPhase& ph) {
@@ -542,11 +579,13 @@ void mixed_prepare(tci_plan& P, int precision, hipStream_t st) {
   if (M.per_pair || !autotune_enabled()) {
     // untuned: 128-row phase-2 tiles unless TCI_SPLIT_P2_192=1
     const char* e = std::getenv("TCI_SPLIT_P2_192");
-    build(def_target, e && e[0] == '1');
+    const char* f = std::getenv("TCI_SPLIT_FUSED");
+    build(def_target, e && e[0] == '1', f && f[0] == '1', 1024);
   } else if (M.tuned_target > 0) {
     // tuned before for the other split precision: same schedule, so one plan
     // gives the same sigma for a precision every time it is used
-    build(M.tuned_target, M.tuned_big);
+    build(M.tuned_target, M.tuned_big, M.tuned_fused, M.tuned_fused_target);
+    M.fused_remap = M.tuned_fused ? 0 : -1;
     for (auto& ph : M.phase) {
       std::vector<GemmProgram*> g = progs(ph);
       for (size_t k = 0; k < g.size() && k < M.tuned_orders.size(); ++k) g[k]->remap = M.tuned_orders[k];
@@ -567,7 +606,7 @@ void mixed_prepare(tci_plan& P, int precision, hipStream_t st) {
     std::string log = "split ladder: (K-split target, 192-tiles)";
     for (int big = 0; big <= (any_big && !M.merged ? 1 : 0); ++big)
       for (int64_t t : cand) {
-        build(t, big != 0);
+        build(t, big != 0, false, 0);
         TCI_HIP(hipStreamSynchronize(st));
         const double ms = time_ms(st, 2, [&] { mixed_ladder_chunk(P, precision, M.chunks[0], st); });
         log += " " + std::to_string(t) + (big ? "b" : "") + ":" + fmt_ms(ms);
@@ -577,9 +616,35 @@ void mixed_prepare(tci_plan& P, int precision, hipStream_t st) {
           bb = big != 0;
         }
       }
-    build(bt, bb);
+    build(bt, bb, false, 0);
     TCI_HIP(hipStreamSynchronize(st));
-    log += " -> " + std::to_string(bt) + (bb ? "b" : "") + "; order";
+    log += " -> " + std::to_string(bt) + (bb ? "b" : "");
+    // then the fused kernel for the walks of 129-192 (whole ladder timed: it
+    // runs over every Cholesky vector in one launch)
+    bool bf = false;
+    int64_t bft = 0;
+    bool any_fusable = false;
+    for (int Q = 0; Q < n_pairs && !M.merged; ++Q) any_fusable = any_fusable || fusable(P.h_dom_off[Q + 1] - P.h_dom_off[Q]);
+    if (any_fusable) {
+      const double t2 = time_ms(st, 2, [&] { mixed_ladder_run(P, precision, st); });
+      log += "; whole two-phase:" + fmt_ms(t2);
+      double tb = t2;
+      for (int64_t ft : {512, 1024, 2048}) {
+        build(bt, bb, true, ft);
+        TCI_HIP(hipStreamSynchronize(st));
+        const double tf = time_ms(st, 2, [&] { mixed_ladder_run(P, precision, st); });
+        log += " fused" + std::to_string(ft) + ":" + fmt_ms(tf);
+        if (tf < tb * 0.98) {
+          tb = tf;
+          bf = true;
+          bft = ft;
+        }
+      }
+      build(bt, bb, bf, bft);
+      TCI_HIP(hipStreamSynchronize(st));
+      log += bf ? " -> fused" + std::to_string(bft) : " -> two-phase";
+    }
+    log += "; order";
     const auto& ch = M.chunks[0];
     std::vector<GemmProgram*> g0 = progs(M.phase[ch.second]);
     const float* scl = M.scales.as<float>();
@@ -593,6 +658,8 @@ void mixed_prepare(tci_plan& P, int precision, hipStream_t st) {
     }
     M.tuned_target = bt;
     M.tuned_big = bb;
+    M.tuned_fused = bf;
+    M.tuned_fused_target = bft;
     P.tuning += log + "\n";
   }
   if (P.dense_pairs) dense_pairs_build_split(P, st);
@@ -600,8 +667,7 @@ void mixed_prepare(tci_plan& P, int precision, hipStream_t st) {
   M.ready = true;
 }
 
-void mixed_ladder(tci_plan& P, int precision, hipStream_t st) {
-  mixed_prepare(P, precision, st);
+void mixed_ladder_run(tci_plan& P, int precision, hipStream_t st) {
   tci_plan::MixedLadder& M = P.mx;
   const int f16 = precision == TCI_SIGMA_FP16X3 ? 1 : 0;
   float* sc = M.scales.as<float>();
@@ -625,9 +691,17 @@ void mixed_ladder(tci_plan& P, int precision, hipStream_t st) {
   TCI_HIP(hipGetLastError());
   double* Rz = M.rpart_n > 1 ? M.Rpart.as<double>() : P.R.as<double>();
   launch_fill(Rz, (int64_t)M.rpart_n * P.sum_d2, 0.0, st);
+  if (M.fused)
+    launch_xladder_fused(precision, M.fused_d.as<FusedTask>(), (int64_t)M.fused_h.size(), sc, st,
+                         M.fused_remap < 0 || xcd_remap_forced() ? xcd_remap_default() : M.fused_remap);
   for (const auto& ch : M.chunks) mixed_ladder_chunk(P, precision, ch, st);
   if (P.timing()) TCI_HIP(hipEventRecord(P.ev[3], st));
   if (M.rpart_n > 1) ladder_reduce(M.Rpart.as<double>(), P.sum_d2, M.rpart_n, P.R.as<double>(), P.sum_d2, st);
+}
+
+void mixed_ladder(tci_plan& P, int precision, hipStream_t st) {
+  mixed_prepare(P, precision, st);
+  mixed_ladder_run(P, precision, st);
 }
 
 }  // namespace tci

@@ -5,6 +5,7 @@
 
 #include "common.hpp"
 #include "ladder.hpp"
+#include "ladder_fused.hpp"
 #include "terms.hpp"
 
 struct tci_plan {
@@ -89,6 +90,13 @@ struct tci_plan {
     std::vector<LadderPhase> phase;
     std::vector<std::pair<int64_t, int>> chunks;
     int64_t bytes = 0;
+    // fused ladder (walks of 129-192 with prepacked L): task table, on or off
+    bool fused = false;
+    std::vector<tci::FusedTask> fused_h;
+    tci::DevBuf fused_d;
+    int fused_remap = -1;
+    bool tuned_fused = false;
+    int64_t tuned_fused_target = 0;
     int64_t tuned_target = -1;      // autotuned K-split fill target (kept across precision switches)
     bool tuned_big = false;         // autotuned: phase 2 of domains 129-192 on 192 x 192 tiles
     std::vector<int> tuned_orders;  // autotuned workgroup order per program role
@@ -129,6 +137,8 @@ namespace tci {
 void mixed_prepare(tci_plan& P, int precision, hipStream_t st);
 void mixed_ladder(tci_plan& P, int precision, hipStream_t st);
 void mixed_ladder_chunk(tci_plan& P, int precision, const std::pair<int64_t, int>& ch, hipStream_t st);
+// the split ladder of one sigma after preparation (C split, chunks, fused walks, reduction)
+void mixed_ladder_run(tci_plan& P, int precision, hipStream_t st);
 // fp16x3 scales: mx[slot] = max |x[0..n)|, then set_scales mode (1: ladder, 2: pair products)
 void launch_split_scales(int mode, const double* x, int64_t n, unsigned* mx, int slot, float* sc, float dmax,
                          hipStream_t st);
