@@ -5,19 +5,20 @@
 
 namespace tci {
 
-// One workgroup: output columns a0..a0+63 of R_Q, Cholesky vectors [j0, j1).
+// One workgroup: panel a0..a0+63 of the c index of walk Q (a partial sum of
+// R_Q over that panel), Cholesky vectors [j0, j1).
 struct FusedTask {
   const uint16_t* L;   // split rows of L^J[D,D] of this walk, J-block every jstep elements
   const uint16_t* C;   // split rows of C_Q (row c, k contiguous)
-  double* R;           // output block R_Q (row-major d x d) or its K-split slice
+  double* R;           // K-split slice of R_Q (row-major d x d) of this panel and J range
   int64_t jstep;       // elements between consecutive J blocks
   int32_t d, ldx;      // domain size (129..192), split row stride (elements)
-  int32_t a0, j0, j1;  // output columns a0.., Cholesky vectors [j0, j1)
+  int32_t a0, j0, j1;  // panel start in c, Cholesky vectors [j0, j1)
   int32_t pad_;
 };
 
-// R[n][a] += s4 sum_J L_J[n][c] (s3 L_J C^T)[a][c] for each task (scales[3],
-// scales[4]; accumulates into the zero-filled R or its K-split slices)
+// R[n][a] += s4 sum_J sum_{c in panel} L_J[n][c] (s3 L_J C^T)[a][c] for each
+// task (scales[3], scales[4]; accumulates into zero-filled K-split slices)
 void launch_xladder_fused(int precision, const FusedTask* tasks, int64_t ntasks, const float* scales, hipStream_t st,
                           int remap);
 

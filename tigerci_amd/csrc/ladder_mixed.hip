@@ -387,9 +387,9 @@ void mixed_prepare(tci_plan& P, int precision, hipStream_t st) {
         nsplit_v[v] = tv[v] ? (int)std::max<int64_t>(1, std::min<int64_t>(16, cdiv(target, tv[v]))) : 1;
         M.rpart_n = std::max(M.rpart_n, nsplit_v[v]);
       }
-      if (tf) {
-        nsf = (int)std::max<int64_t>(1, std::min<int64_t>(16, cdiv(fused_target, tf)));
-        M.rpart_n = std::max(M.rpart_n, nsf);
+      if (tf) {  // each (panel, J range) of a fused walk writes its own partial-R slice
+        nsf = (int)std::max<int64_t>(1, std::min<int64_t>(5, cdiv(fused_target, tf)));
+        M.rpart_n = std::max(M.rpart_n, 3 * nsf);
       }
     }
     if (M.rpart_n > 1 && M.Rpart.bytes < (size_t)M.rpart_n * P.sum_d2 * 8) {
@@ -506,12 +506,11 @@ void mixed_prepare(tci_plan& P, int precision, hipStream_t st) {
           // phase builds the tasks
           if (&ph == &M.phase.front())
             for (int sp = 0; sp < nsf; ++sp)
-              for (int a0 = 0; a0 < d; a0 += 64) {
+              for (int a0 = 0; a0 < d; a0 += 64) {  // the panels of one J range next to each other
                 FusedTask f{};
                 f.L = Lb;
                 f.C = Cx + cxo[Q];
-                f.R = (M.rpart_n > 1 ? M.Rpart.as<double>() + (int64_t)sp * P.sum_d2 : P.R.as<double>()) +
-                      P.h_cd_off[Q];
+                f.R = M.Rpart.as<double>() + (int64_t)((a0 / 64) * nsf + sp) * P.sum_d2 + P.h_cd_off[Q];
                 f.jstep = jstep;
                 f.d = d;
                 f.ldx = (int32_t)ldx;
@@ -585,7 +584,7 @@ void mixed_prepare(tci_plan& P, int precision, hipStream_t st) {
     // tuned before for the other split precision: same schedule, so one plan
     // gives the same sigma for a precision every time it is used
     build(M.tuned_target, M.tuned_big, M.tuned_fused, M.tuned_fused_target);
-    M.fused_remap = M.tuned_fused ? 0 : -1;
+    M.fused_remap = M.tuned_fused_order;
     for (auto& ph : M.phase) {
       std::vector<GemmProgram*> g = progs(ph);
       for (size_t k = 0; k < g.size() && k < M.tuned_orders.size(); ++k) g[k]->remap = M.tuned_orders[k];
@@ -629,20 +628,26 @@ void mixed_prepare(tci_plan& P, int precision, hipStream_t st) {
       const double t2 = time_ms(st, 2, [&] { mixed_ladder_run(P, precision, st); });
       log += "; whole two-phase:" + fmt_ms(t2);
       double tb = t2;
-      for (int64_t ft : {512, 1024, 2048}) {
-        build(bt, bb, true, ft);
-        TCI_HIP(hipStreamSynchronize(st));
-        const double tf = time_ms(st, 2, [&] { mixed_ladder_run(P, precision, st); });
-        log += " fused" + std::to_string(ft) + ":" + fmt_ms(tf);
-        if (tf < tb * 0.98) {
-          tb = tf;
-          bf = true;
-          bft = ft;
+      int bfo = 1;
+      for (int64_t ft : {512, 1024, 2048})
+        for (int fo : {1, 0}) {
+          build(bt, bb, true, ft);
+          M.fused_remap = fo;
+          TCI_HIP(hipStreamSynchronize(st));
+          const double tf = time_ms(st, 2, [&] { mixed_ladder_run(P, precision, st); });
+          log += " fused" + std::to_string(ft) + (fo ? "x" : "") + ":" + fmt_ms(tf);
+          if (tf < tb * 0.98) {
+            tb = tf;
+            bf = true;
+            bft = ft;
+            bfo = fo;
+          }
         }
-      }
+      M.tuned_fused_order = bfo;
       build(bt, bb, bf, bft);
+      M.fused_remap = bfo;
       TCI_HIP(hipStreamSynchronize(st));
-      log += bf ? " -> fused" + std::to_string(bft) : " -> two-phase";
+      log += bf ? " -> fused" + std::to_string(bft) + (bfo ? "x" : "") : " -> two-phase";
     }
     log += "; order";
     const auto& ch = M.chunks[0];

@@ -94,7 +94,8 @@ struct tci_plan {
     bool fused = false;
     std::vector<tci::FusedTask> fused_h;
     tci::DevBuf fused_d;
-    int fused_remap = -1;
+    int fused_remap = 1;            // XCD-contiguous: the panels of one walk and J share L2
+    int tuned_fused_order = 1;
     bool tuned_fused = false;
     int64_t tuned_fused_target = 0;
     int64_t tuned_target = -1;      // autotuned K-split fill target (kept across precision switches)
