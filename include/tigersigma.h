@@ -107,6 +107,15 @@ int tci_plan_get_info(const tci_plan* plan, tci_plan_info* info);
 int tci_sigma_apply(tci_plan* plan, const double* c, double* sigma, int64_t n,
                     int32_t flags, void* stream);
 
+/* nvec sigma builds at once: c and sigma hold nvec vectors of n doubles back to
+ * back (device memory, not overlapping); flags as tci_sigma_apply.  In a split
+ * precision with per-walk ladder programs and resident L^J blocks the ladder
+ * of all vectors runs as one batched pass that reads every L^J block once per
+ * batch; otherwise the vectors are built one after another.  No graph replay.
+ * No reference counterpart (the reference builds one vector per call). */
+int tci_sigma_apply_batch(tci_plan* plan, const double* c, double* sigma, int64_t n, int32_t nvec, int32_t flags,
+                          void* stream);
+
 /* Host-vector drop-in with htimesc semantics: sigmavec += H civec (bare H,
  * FP64).  Copies over PCIe; synchronous. */
 int tci_htimesc(tci_plan* plan, const double* civec, double* sigmavec, int64_t n);

@@ -95,13 +95,15 @@ def load_library(path: str | None = None) -> ctypes.CDLL:
     lib.tci_profile_read.argtypes = [vp, i32, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(i64),
                                      ctypes.POINTER(ctypes.c_double), ctypes.POINTER(i64)]
     lib.tci_plan_tuning.argtypes = [vp, ctypes.c_char_p, i64]
+    lib.tci_sigma_apply_batch.argtypes = [vp, vp, vp, i64, i32, i32, vp]
     lib.tci_davidson_gen_run.argtypes = [vp, ctypes.POINTER(_DavParams), vp, vp, vp, ctypes.POINTER(_DavResult)]
     lib.tci_acpf_gvalues.argtypes = [i32, i32, ctypes.c_double, vp]
     lib.tci_last_error.restype = ctypes.c_char_p
     for f in ("tci_plan_create", "tci_plan_get_info", "tci_sigma_apply", "tci_htimesc", "tci_diagonal",
               "tci_davidson_run", "tci_selftest_mfma", "tci_device_alloc", "tci_device_free", "tci_memcpy_h2d",
               "tci_memcpy_d2h", "tci_memcpy_d2d", "tci_device_synchronize", "tci_profile_enable",
-              "tci_profile_read", "tci_plan_tuning", "tci_davidson_gen_run", "tci_acpf_gvalues"):
+              "tci_profile_read", "tci_plan_tuning", "tci_davidson_gen_run", "tci_acpf_gvalues",
+              "tci_sigma_apply_batch"):
         getattr(lib, f).restype = ctypes.c_int
     if path is None:
         _lib = lib
@@ -260,6 +262,13 @@ class Plan:
         flags = PRECISION[precision] | (TCI_ACCUMULATE if accumulate else 0) | (TCI_ADD_EREF if add_eref else 0)
         flags |= TCI_SIGMA_GRAPH if graph else 0
         _check(self._lib.tci_sigma_apply(self._h, _ptr(c), _ptr(out), self.n_csf, flags, _stream(stream)))
+
+    def sigma_batch(self, c, out, nvec: int, precision: str = "fp64", accumulate: bool = False,
+                    add_eref: bool = False, stream=None) -> None:
+        """nvec sigma builds at once; c and out hold nvec vectors back to back."""
+        flags = PRECISION[precision] | (TCI_ACCUMULATE if accumulate else 0) | (TCI_ADD_EREF if add_eref else 0)
+        _check(self._lib.tci_sigma_apply_batch(self._h, _ptr(c), _ptr(out), self.n_csf, nvec, flags,
+                                               _stream(stream)))
 
     def diagonal(self, out=None, stream=None):
         """Exact H_ii - E_ref into ``out`` (device); returns numpy when out is None."""

@@ -20,6 +20,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdlib>
 #include <numeric>
 
 #include "plan.hpp"
@@ -365,12 +366,29 @@ int davidson_impl(tci_plan* plan, const tci_davidson_params* prm, const double* 
       return gv;
     };
     int precision = prec0;
-    auto sigma = [&](int k) {
+    // TCI_DAVIDSON_BATCH=1: the sigma builds of one step go as one batch (one
+    // ladder pass for all of them in split precision); off by default, the
+    // batched ladder measures slower per vector than single builds
+    const char* benv = std::getenv("TCI_DAVIDSON_BATCH");
+    const bool batch = benv && benv[0] == '1';
+    auto sigma_many = [&](const std::vector<int>& ks) {
+      if (ks.empty()) return;
       auto t0 = std::chrono::steady_clock::now();
-      plan->sigma(Vk(k), Sk(k), precision | graph_flag, st);
+      if (batch) {
+        std::vector<const double*> xs;
+        std::vector<double*> ys;
+        for (int k : ks) {
+          xs.push_back(Vk(k));
+          ys.push_back(Sk(k));
+        }
+        plan->sigma_batch(xs.data(), ys.data(), (int)ks.size(), precision | graph_flag, st);
+      } else {
+        for (int k : ks) plan->sigma(Vk(k), Sk(k), precision | graph_flag, st);
+      }
       TCI_HIP(hipStreamSynchronize(st));
       t_sigma += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-      if (precision == TCI_SIGMA_FP64) ++n64; else ++nmix;
+      if (precision == TCI_SIGMA_FP64) n64 += (int)ks.size();
+      else nmix += (int)ks.size();
     };
     // initial guess: unit vectors at the lowest diagonal elements (init_from_diag)
     std::vector<double> hd(n);
@@ -412,10 +430,15 @@ int davidson_impl(tci_plan* plan, const tci_davidson_params* prm, const double* 
     int it = 0;
     bool all_conv = false;
     for (it = 1; it <= maxit; ++it) {
+      {
+        std::vector<int> ks;
+        for (int r = 0; r < roots; ++r)
+          if (!conv[r] && !ready[r]) ks.push_back(root2idx[r]);
+        sigma_many(ks);
+      }
       for (int r = 0; r < roots; ++r) {
         if (conv[r] || ready[r]) continue;  // no new vector for a settled root
         const int k = root2idx[r];
-        sigma(k);
         std::vector<double*> X;
         for (int q = 0; q <= k; ++q) X.push_back(Vk(q));
         std::vector<double> hv;
@@ -507,7 +530,9 @@ int davidson_impl(tci_plan* plan, const tci_davidson_params* prm, const double* 
         if (do_switch) {
           precision = TCI_SIGMA_FP64;
           std::fill(ready.begin(), ready.end(), 0);
-          for (int q = 0; q < m; ++q) sigma(q);
+          std::vector<int> all(m);
+          std::iota(all.begin(), all.end(), 0);
+          sigma_many(all);
         }
         for (int a = 0; a < m; ++a) {
           std::vector<double*> X;

@@ -20,14 +20,16 @@ namespace tci {
 namespace {
 
 // Cx[P] rows r of C_P (d x d, FP64) in split-row form, row stride 2 xpad(d)
+// Cx[P] rows c = split row c of C_P; with nvec > 1 the walks hold nvec such
+// blocks ([P][v][c][k] at nvec cx_off[P]) and this writes slot vec
 __global__ void split_blocks_kernel(PairTab T, const int64_t* __restrict__ cx_off, const double* __restrict__ C,
-                                    uint16_t* __restrict__ Cx, const float* scl, int f16) {
+                                    uint16_t* __restrict__ Cx, const float* scl, int f16, int nvec, int vec) {
   const float sc = scale_at(scl);
   const int P = blockIdx.y;
   const int d = (int)(T.dom_off[P + 1] - T.dom_off[P]);
   const int nch = (d + 7) >> 3;
   const double* src = C + T.cd_off[P];
-  uint16_t* dst = Cx + cx_off[P];
+  uint16_t* dst = Cx + (int64_t)nvec * cx_off[P] + (int64_t)vec * d * 16 * nch;
   for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < d * nch; e += gridDim.x * blockDim.x) {
     const int r = e / nch, ch = e - r * nch;
     float v[8];
@@ -238,6 +240,7 @@ void mixed_prepare(tci_plan& P, int precision, hipStream_t st) {
   // captured graphs point into the copies rebuilt below
   P.drop_graphs();
   M.ready = false;
+  M.bat.nvec = 0;  // batched programs point into the copies rebuilt below
   M.prec = precision;
   const int f16 = precision == TCI_SIGMA_FP16X3 ? 1 : 0;
   // scales: all 1 for bf16x3 (FP32 exponent range); for fp16x3 the constants'
@@ -691,7 +694,7 @@ void mixed_ladder_run(tci_plan& P, int precision, hipStream_t st) {
   } else {
     hipLaunchKernelGGL(split_blocks_kernel, dim3(blocks_for((int64_t)P.dmax * ((P.dmax + 7) / 8), 256), P.n_pairs),
                        dim3(256), 0, st, P.tab, M.cx_off.as<int64_t>(), P.C.as<double>(), M.Cx.as<uint16_t>(), sc + 1,
-                       f16);
+                       f16, 1, 0);
   }
   TCI_HIP(hipGetLastError());
   double* Rz = M.rpart_n > 1 ? M.Rpart.as<double>() : P.R.as<double>();
@@ -707,6 +710,186 @@ void mixed_ladder_run(tci_plan& P, int precision, hipStream_t st) {
 void mixed_ladder(tci_plan& P, int precision, hipStream_t st) {
   mixed_prepare(P, precision, st);
   mixed_ladder_run(P, precision, st);
+}
+
+// ---------------------------------------------------------------------------
+// Batched split ladder: nvec vectors in one two-phase pass, so every L^J block
+// is read once per batch instead of once per vector.
+//   phase 1 T'[(jj,a)][(v,c)] = sum_k L^J[a][k] C_v[c][k], stored as
+//           T'[jj][v][a][c] (blocked output: rows by jj, columns by v)
+//   phase 2 R_v[n][a] += sum_jj sum_c L^J[n][c] T'[jj][v][a][c] (columns (v,a)
+//           blocked into the R_v arrays)
+// Needs per-walk programs and every L^J[D,D] resident (prepacked or the full
+// domain); the schedule reuses the single-vector tuning's fill target.
+bool mixed_batch_supported(const tci_plan& P) {
+  const tci_plan::MixedLadder& M = P.mx;
+  return M.ready && !M.merged && (M.prepacked || M.n_local == 0) && P.n_pairs > 0;
+}
+
+void mixed_batch_prepare(tci_plan& P, int precision, int nvec, hipStream_t st) {
+  mixed_prepare(P, precision, st);
+  tci_plan::MixedLadder& M = P.mx;
+  tci_plan::MixedLadder::Batch& B = M.bat;
+  if (B.nvec == nvec) return;
+  TCI_REQUIRE(mixed_batch_supported(P), "batched split ladder needs per-walk programs with resident L blocks");
+  TCI_REQUIRE(nvec >= 2 && nvec <= 64, "batch of 2..64 vectors");
+  B.nvec = 0;
+  const int n_pairs = P.n_pairs, n_v = P.n_v;
+  const int64_t n_chol = P.n_chol, nvp = xpad(n_v), jstride_full = (int64_t)n_v * 2 * nvp;
+  std::vector<int64_t> cxo(n_pairs + 1, 0), lpo_all;
+  std::vector<int32_t> locq;
+  for (int Q = 0; Q < n_pairs; ++Q) {
+    const int64_t d = P.h_dom_off[Q + 1] - P.h_dom_off[Q];
+    cxo[Q + 1] = cxo[Q] + d * 2 * xpad(d);
+    if (d != n_v) locq.push_back(Q);
+  }
+  if (M.prepacked) {  // offsets of the local walks in the prepacked Lpx, as in mixed_prepare
+    lpo_all.assign(locq.size() + 1, 0);
+    for (size_t q = 0; q < locq.size(); ++q) {
+      const int64_t d = P.h_dom_off[locq[q] + 1] - P.h_dom_off[locq[q]];
+      lpo_all[q + 1] = lpo_all[q] + n_chol * d * 2 * xpad(d);
+    }
+  }
+  const int64_t sum_x = cxo.back();
+  B.Cxb.alloc((size_t)nvec * sum_x * 2);
+  B.Rb.alloc((size_t)nvec * P.sum_d2 * 8);
+  // T' of one chunk: 8 GB per vector of the batch (the single-vector chunks)
+  int64_t Lc = std::max<int64_t>(1, std::min<int64_t>(n_chol, (int64_t)(8.0e9 / (2.0 * (double)sum_x))));
+  for (int64_t q = Lc; q >= std::max<int64_t>(1, Lc / 2); --q)
+    if (n_chol % q == 0) {
+      Lc = q;
+      break;
+    }
+  B.Txb.alloc((size_t)Lc * nvec * sum_x * 2);
+  // phase-2 tiles by domain: 64 x 64 up to 64, 128 x 128 up to 128, 192 x 192 above
+  static const int tt[3] = {64, 128, 192};
+  auto cls = [](int64_t d) { return d <= 64 ? 0 : (d <= 128 ? 1 : 2); };
+  const int64_t target = M.tuned_target > 0 ? M.tuned_target : 512;
+  int64_t tv[3] = {0, 0, 0};
+  for (int Q = 0; Q < n_pairs; ++Q) {
+    const int64_t d = P.h_dom_off[Q + 1] - P.h_dom_off[Q];
+    const int c = cls(d);
+    tv[c] += cdiv(d, tt[c]) * cdiv(nvec * d, tt[c]);
+  }
+  int nsp_v[3];
+  B.rpart_n = 1;
+  for (int c = 0; c < 3; ++c) {
+    nsp_v[c] = tv[c] ? (int)std::max<int64_t>(1, std::min<int64_t>(16, cdiv(target, tv[c]))) : 1;
+    B.rpart_n = std::max(B.rpart_n, nsp_v[c]);
+  }
+  if (B.rpart_n > 1) B.Rpartb.alloc((size_t)B.rpart_n * nvec * P.sum_d2 * 8);
+  B.phase.clear();
+  B.chunks.clear();
+  std::vector<int64_t> lens;
+  for (int64_t J0 = 0; J0 < n_chol; J0 += Lc) {
+    const int64_t len = std::min(Lc, n_chol - J0);
+    int idx = -1;
+    for (size_t q = 0; q < lens.size(); ++q)
+      if (lens[q] == len) idx = (int)q;
+    if (idx < 0) {
+      idx = (int)lens.size();
+      lens.push_back(len);
+    }
+    B.chunks.push_back({J0, idx});
+  }
+  const uint16_t* Cxb = B.Cxb.as<uint16_t>();
+  uint16_t* Tx = B.Txb.as<uint16_t>();
+  for (int64_t len : lens) {
+    B.phase.emplace_back();
+    tci_plan::LadderPhase& ph = B.phase.back();
+    ph.Lc = len;
+    ph.p1x.resize(1);
+    ph.p2x.resize(3);
+    ph.p1x[0].split = true;
+    ph.p1x[0].tile = 128;
+    ph.p1x[0].tile_n = 192;
+    for (int c = 0; c < 3; ++c) {
+      ph.p2x[c].split = true;
+      ph.p2x[c].tile = tt[c];
+      ph.p2x[c].tile_n = tt[c];
+    }
+    int64_t toff = 0;
+    int qloc = 0;
+    for (int Q = 0; Q < n_pairs; ++Q) {
+      const int d = (int)(P.h_dom_off[Q + 1] - P.h_dom_off[Q]);
+      const int64_t ldx = 2 * xpad(d);
+      const uint16_t* Lb;
+      int64_t jstep, jstride;
+      if (d == n_v) {
+        Lb = M.Lx.as<uint16_t>();
+        jstep = jstride = jstride_full;
+      } else {
+        Lb = M.Lpx.as<uint16_t>() + lpo_all[qloc++];
+        jstep = jstride = (int64_t)d * ldx;
+      }
+      uint16_t* Tq = Tx + toff;
+      const int64_t Sv = (int64_t)d * ldx;  // one vector's block of T' for one jj
+      Blocking b1;
+      b1.rdiv = d;
+      b1.rstride = nvec * Sv;
+      b1.cdiv = d;
+      b1.cstride = Sv;
+      ph.p1x[0].add(reinterpret_cast<double*>(Tq), ldx, (int)(len * d), nvec * d,
+                    {xseg(xop(Lb, ldx, jstride), xop(Cxb + nvec * cxo[Q], ldx), d)}, false, true, 0, b1, 3);
+      const int c = cls(d);
+      const int nsp = (int)std::min<int64_t>(nsp_v[c], len);
+      Blocking b2;
+      b2.cdiv = d;
+      b2.cstride = P.sum_d2;
+      for (int sp = 0; sp < nsp; ++sp) {
+        const int64_t j0 = len * sp / nsp, j1 = len * (sp + 1) / nsp;
+        std::vector<Seg> sg;
+        for (int64_t jj = j0; jj < j1; ++jj)
+          sg.push_back(xseg(xop(Lb + jj * jstep, ldx, jstride), xop(Tq + jj * nvec * Sv, ldx), d));
+        double* Rout = (B.rpart_n > 1 ? B.Rpartb.as<double>() + (int64_t)sp * nvec * P.sum_d2 : B.Rb.as<double>()) +
+                       P.h_cd_off[Q];
+        ph.p2x[c].add(Rout, d, d, nvec * d, sg, true, false, 0, b2, 4);
+      }
+      toff += len * nvec * Sv;
+    }
+    for (auto& g : ph.p1x) g.upload_tables(st);
+    for (auto& g : ph.p2x) g.upload_tables(st);
+  }
+  TCI_HIP(hipStreamSynchronize(st));
+  B.nvec = nvec;
+}
+
+void mixed_batch_absmax(tci_plan& P, bool first, hipStream_t st) {
+  unsigned* mx = P.mx.maxbits.as<unsigned>();
+  if (first) hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(64), 0, st, mx, 2);
+  hipLaunchKernelGGL(absmax_kernel<double>, dim3(blocks_for(P.sum_d2, 8192)), dim3(256), 0, st, P.C.as<double>(),
+                     P.sum_d2, mx);
+  TCI_HIP(hipGetLastError());
+}
+
+void mixed_batch_set_scales(tci_plan& P, hipStream_t st) {
+  hipLaunchKernelGGL(set_scales_kernel, dim3(1), dim3(64), 0, st, 1, P.mx.maxbits.as<unsigned>(),
+                     P.mx.scales.as<float>(), (float)P.dmax);
+  TCI_HIP(hipGetLastError());
+}
+
+void mixed_batch_split_c(tci_plan& P, int precision, int v, hipStream_t st) {
+  tci_plan::MixedLadder& M = P.mx;
+  const int f16 = precision == TCI_SIGMA_FP16X3 ? 1 : 0;
+  hipLaunchKernelGGL(split_blocks_kernel, dim3(blocks_for((int64_t)P.dmax * ((P.dmax + 7) / 8), 256), P.n_pairs),
+                     dim3(256), 0, st, P.tab, M.cx_off.as<int64_t>(), P.C.as<double>(), M.bat.Cxb.as<uint16_t>(),
+                     M.scales.as<float>() + 1, f16, M.bat.nvec, v);
+  TCI_HIP(hipGetLastError());
+}
+
+void mixed_batch_ladder(tci_plan& P, int precision, hipStream_t st) {
+  tci_plan::MixedLadder& M = P.mx;
+  tci_plan::MixedLadder::Batch& B = M.bat;
+  const int64_t n = (int64_t)B.nvec * P.sum_d2;
+  launch_fill(B.rpart_n > 1 ? B.Rpartb.as<double>() : B.Rb.as<double>(), (int64_t)B.rpart_n * n, 0.0, st);
+  const float* sc = M.scales.as<float>();
+  for (const auto& ch : B.chunks) {
+    const tci_plan::LadderPhase& ph = B.phase[ch.second];
+    for (const auto& g : ph.p1x) g.launch_split(precision, st, ch.first, sc);
+    for (const auto& g : ph.p2x) g.launch_split(precision, st, ch.first, sc);
+  }
+  if (P.timing()) TCI_HIP(hipEventRecord(P.ev[3], st));
+  if (B.rpart_n > 1) ladder_reduce(B.Rpartb.as<double>(), n, B.rpart_n, B.Rb.as<double>(), n, st);
 }
 
 }  // namespace tci

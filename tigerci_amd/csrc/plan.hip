@@ -839,8 +839,14 @@ void tci_plan::unpack_and_ladder(const double* x, const IoSlot* io, int precisio
 }
 
 void tci_plan::sigma_body(const double* x, double* y, const IoSlot* io, int flags, hipStream_t st) {
+  unpack_and_ladder(x, io, flags & 0xff, st);
+  sigma_rest(x, y, io, flags, st);
+}
+
+// every term after the ladder: needs C, Ct and s of x (unpacked) and the
+// ladder's R
+void tci_plan::sigma_rest(const double* x, double* y, const IoSlot* io, int flags, hipStream_t st) {
   const int precision = flags & 0xff;
-  unpack_and_ladder(x, io, precision, st);
   launch_rterms(tab, dmax, x, s.as<double>(), Kov.as<double>(), O3.as<double>(), R.as<double>(), st, io);
   progY.launch(st);
   if (dense_pairs) {
@@ -903,6 +909,55 @@ void tci_plan::sigma_graph(const double* x, double* y, int flags, hipStream_t st
     TCI_HIP(hipEventElapsedTime(&ms, ev[0], ev[1]));
     prof_sigma_ms[precision] += ms;
     ++prof_sigma_n[precision];
+  }
+}
+
+void tci_plan::sigma_batch(const double* const* x, double* const* y, int nvec, int flags, hipStream_t st) {
+  const int precision = flags & 0xff;
+  TCI_REQUIRE(precision == TCI_SIGMA_FP64 || precision == TCI_SIGMA_BF16X3 || precision == TCI_SIGMA_FP16X3,
+              "unknown precision");
+  TCI_REQUIRE((flags & ~(0xff | TCI_ACCUMULATE | TCI_ADD_EREF | TCI_SIGMA_GRAPH)) == 0, "unknown sigma flag bits");
+  bool batched = nvec > 1 && (precision == TCI_SIGMA_BF16X3 || precision == TCI_SIGMA_FP16X3);
+  if (batched) {
+    mixed_prepare(*this, precision, st);
+    batched = mixed_batch_supported(*this);
+  }
+  if (!batched) {
+    for (int v = 0; v < nvec; ++v) sigma(x[v], y[v], flags, st);
+    return;
+  }
+  const int fl = flags & ~TCI_SIGMA_GRAPH;
+  mixed_batch_prepare(*this, precision, nvec, st);
+  if (profiling) TCI_HIP(hipEventRecord(ev[0], st));
+  if (precision == TCI_SIGMA_FP16X3) {  // one scale set for the whole batch
+    for (int v = 0; v < nvec; ++v) {
+      launch_unpack(tab, dmax, x[v], C.as<double>(), Ct.as<double>(), s.as<double>(), st);
+      mixed_batch_absmax(*this, v == 0, st);
+    }
+    mixed_batch_set_scales(*this, st);
+  }
+  for (int v = 0; v < nvec; ++v) {
+    launch_unpack(tab, dmax, x[v], C.as<double>(), Ct.as<double>(), s.as<double>(), st);
+    mixed_batch_split_c(*this, precision, v, st);
+  }
+  if (profiling) TCI_HIP(hipEventRecord(ev[2], st));
+  mixed_batch_ladder(*this, precision, st);
+  for (int v = 0; v < nvec; ++v) {
+    launch_unpack(tab, dmax, x[v], C.as<double>(), Ct.as<double>(), s.as<double>(), st);
+    TCI_HIP(hipMemcpyAsync(R.p, mx.bat.Rb.as<double>() + (int64_t)v * sum_d2, (size_t)sum_d2 * 8,
+                           hipMemcpyDeviceToDevice, st));
+    sigma_rest(x[v], y[v], nullptr, fl, st);
+  }
+  if (profiling) {
+    TCI_HIP(hipEventRecord(ev[1], st));
+    TCI_HIP(hipEventSynchronize(ev[1]));
+    float ms = 0.f;
+    TCI_HIP(hipEventElapsedTime(&ms, ev[0], ev[1]));
+    prof_sigma_ms[precision] += ms;
+    prof_sigma_n[precision] += nvec;
+    TCI_HIP(hipEventElapsedTime(&ms, ev[2], ev[3]));
+    prof_ladder_ms[precision] += ms;
+    prof_ladder_n[precision] += nvec;
   }
 }
 
@@ -974,6 +1029,24 @@ void tci_plan_destroy(tci_plan* plan) {
     (void)hipDeviceSynchronize();
     delete plan;
   }
+}
+
+int tci_sigma_apply_batch(tci_plan* plan, const double* c, double* sigma, int64_t n, int32_t nvec, int32_t flags,
+                          void* stream) {
+  TCI_GUARD_BEGIN
+  TCI_REQUIRE(plan && c && sigma, "null argument");
+  TCI_REQUIRE(n == plan->n_csf, "vector length does not match the plan");
+  TCI_REQUIRE(nvec >= 1 && nvec <= 64, "nvec must be 1..64");
+  TCI_REQUIRE(c + (int64_t)nvec * n <= sigma || sigma + (int64_t)nvec * n <= c, "c and sigma must not overlap");
+  std::vector<const double*> xs(nvec);
+  std::vector<double*> ys(nvec);
+  for (int v = 0; v < nvec; ++v) {
+    xs[v] = c + (int64_t)v * n;
+    ys[v] = sigma + (int64_t)v * n;
+  }
+  plan->sigma_batch(xs.data(), ys.data(), nvec, flags, static_cast<hipStream_t>(stream));
+  return TCI_OK;
+  TCI_GUARD_END
 }
 
 int tci_plan_tuning(const tci_plan* plan, char* buf, int64_t len) {

@@ -98,6 +98,16 @@ struct tci_plan {
     int tuned_fused_order = 1;
     bool tuned_fused = false;
     int64_t tuned_fused_target = 0;
+    // batched programs for `nvec` vectors (local domains, two-phase): C_v split
+    // rows per walk [Q][v][c][k], T' of one chunk [Q][jj][v][a][c], R_v back to
+    // back and their K-split partials
+    struct Batch {
+      int nvec = 0;
+      tci::DevBuf Cxb, Txb, Rb, Rpartb;
+      int rpart_n = 1;
+      std::vector<LadderPhase> phase;
+      std::vector<std::pair<int64_t, int>> chunks;
+    } bat;
     int64_t tuned_target = -1;      // autotuned K-split fill target (kept across precision switches)
     bool tuned_big = false;         // autotuned: phase 2 of domains 129-192 on 192 x 192 tiles
     std::vector<int> tuned_orders;  // autotuned workgroup order per program role
@@ -125,6 +135,10 @@ struct tci_plan {
   void build(const tci_problem& p);
   void sigma(const double* x, double* y, int flags, hipStream_t st);
   void sigma_body(const double* x, double* y, const tci::IoSlot* io, int flags, hipStream_t st);
+  void sigma_rest(const double* x, double* y, const tci::IoSlot* io, int flags, hipStream_t st);
+  // several vectors at once: the split ladder of local domains runs as one
+  // batched pass (L^J read once per batch), every other term per vector
+  void sigma_batch(const double* const* x, double* const* y, int nvec, int flags, hipStream_t st);
   void sigma_graph(const double* x, double* y, int flags, hipStream_t st);
   void unpack_and_ladder(const double* x, const tci::IoSlot* io, int precision, hipStream_t st);
   void build_two_phase_ladder(hipStream_t st);
@@ -140,6 +154,15 @@ void mixed_ladder(tci_plan& P, int precision, hipStream_t st);
 void mixed_ladder_chunk(tci_plan& P, int precision, const std::pair<int64_t, int>& ch, hipStream_t st);
 // the split ladder of one sigma after preparation (C split, chunks, fused walks, reduction)
 void mixed_ladder_run(tci_plan& P, int precision, hipStream_t st);
+// batched split ladder (ladder_mixed.hip): programs for nvec vectors, the split
+// of vector v's C (after unpack) and the ladder pass into bat.Rb
+bool mixed_batch_supported(const tci_plan& P);
+void mixed_batch_prepare(tci_plan& P, int precision, int nvec, hipStream_t st);
+void mixed_batch_split_c(tci_plan& P, int precision, int v, hipStream_t st);
+// fp16x3: max |C| over the batch (call after each vector's unpack), then the scales
+void mixed_batch_absmax(tci_plan& P, bool first, hipStream_t st);
+void mixed_batch_set_scales(tci_plan& P, hipStream_t st);
+void mixed_batch_ladder(tci_plan& P, int precision, hipStream_t st);
 // fp16x3 scales: mx[slot] = max |x[0..n)|, then set_scales mode (1: ladder, 2: pair products)
 void launch_split_scales(int mode, const double* x, int64_t n, unsigned* mx, int slot, float* sc, float dmax,
                          hipStream_t st);
