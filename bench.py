@@ -19,6 +19,7 @@ from __future__ import annotations
 import argparse
 import json
 import os
+import re
 import sys
 import time
 
@@ -33,28 +34,48 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def ladder_alg_bytes(syn, precision: str) -> float:
-    """Minimum HBM bytes of one sigma's two-phase ladder: per Cholesky vector,
-    each walk's intermediate T is written and read once (d^2 elements each
-    way) and its L^J[D,D] block is read by both phases; walks with the full
-    domain share L^J itself.  Elements are 4 bytes in split form (hi + lo) and
-    8 bytes in FP64 (L^J 4 bytes when stored in FP32)."""
+def fused_ranges(tuning: str) -> list:
+    """Domain-size ranges whose split ladder ran as a fused kernel, read from
+    the plan's tuning record ("-> fused128-N" for 65-128, "-> fusedN" for
+    129-192)."""
+    out = []
+    if re.search(r"-> fused128-\d", tuning):
+        out.append((65, 128))
+    if re.search(r"-> fused\d+x?(?![\d-])", tuning):
+        out.append((129, 192))
+    return out
+
+
+def ladder_alg_bytes(syn, precision: str, fused=()) -> float:
+    """Minimum HBM bytes of one sigma's ladder.  Two-phase walks: per Cholesky
+    vector, the intermediate T is written and read once (d^2 elements each
+    way) and the L^J[D,D] block is read by both phases; walks with the full
+    domain share L^J itself.  Walks in a fused domain range (split precision,
+    T kept on chip) read their L^J[D,D] block once.  Elements are 4 bytes in
+    split form (hi + lo) and 8 bytes in FP64 (L^J 4 bytes when stored in
+    FP32)."""
     d = np.diff(np.asarray(syn.dom_off)).astype(np.float64)
     full = d == syn.n_v
+    fz = np.zeros_like(full)
+    if precision != "fp64":
+        for lo, hi in fused:
+            fz |= (d >= lo) & (d <= hi) & ~full
     e_t = 8.0 if precision == "fp64" else 4.0
     e_l = (4.0 if syn.L.dtype == np.float32 else 8.0) if precision == "fp64" else 4.0
-    per_j = 2.0 * e_t * (d ** 2).sum() + 2.0 * e_l * (d[~full] ** 2).sum() + (2.0 * e_l * syn.n_v ** 2 if full.any() else 0.0)
+    two = ~full & ~fz
+    per_j = (2.0 * e_t * (d[~fz] ** 2).sum() + 2.0 * e_l * (d[two] ** 2).sum() + e_l * (d[fz] ** 2).sum()
+             + (2.0 * e_l * syn.n_v ** 2 if full.any() else 0.0))
     return float(per_j * syn.n_chol)
 
 
-def roofline(info, syn, precision: str, seconds: float, traffic, kernel: str) -> dict:
+def roofline(info, syn, precision: str, seconds: float, traffic, kernel: str, fused=()) -> dict:
     """Roofline of the ladder section: the binding roof is the larger of the
     MFMA time (algorithmic flops, x3 MFMAs for a split product) and the HBM
     time of the algorithm's minimum bytes; achieved is that quantity over the
     measured section time."""
     flops = info.flops_ladder
     mfma_flops = flops * (1 if precision == "fp64" else 3)
-    alg_bytes = ladder_alg_bytes(syn, precision)
+    alg_bytes = ladder_alg_bytes(syn, precision, fused)
     t_mfma, t_hbm = mfma_flops / PEAK[precision], alg_bytes / HBM_PEAK
     r = {"kernel": kernel, "avg_launch_ms": seconds * 1e3, "algorithmic_flops": flops, "algorithmic_bytes": alg_bytes,
          "traffic": traffic, "traffic_gbs": traffic / seconds / 1e9 if traffic else None,
@@ -213,13 +234,18 @@ def main() -> int:
         out.update({"value": info.flops_sigma / dt / 1e12, "unit": "TFLOP/s", "higher_is_better": True,
                     "ms_per_step": dt * 1e3, "dtype": a.precision, "config": cfg,
                     "roofline": roofline(info, syn, a.precision, lad, measured_traffic(a.workload, a.precision),
-                                         f"{a.precision} ladder (phases 1+2, per sigma)")})
+                                         f"{a.precision} ladder (per sigma)", fused_ranges(plan.tuning()))})
         if a.copy_probe_mib:
             n = a.copy_probe_mib * (1 << 20) // 8
             src, dst = tci.DeviceArray(n), tci.DeviceArray(n)
             dst.copy_from(src)
             tci.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(5):
+                dst.copy_from(src)
+            tci.synchronize()
             out["copy_probe_bytes"] = 8 * n
+            out["copy_probe_gbs"] = 2 * 8 * n * 5 / (time.perf_counter() - t0) / 1e9  # read + write
         print(json.dumps(out), flush=True)
         return 0
 
@@ -288,8 +314,10 @@ def main() -> int:
         lm = pm["ladder_ms"] / pm["ladder_launches"] / 1e3
         sgm = pm["sigma_ms"] / max(pm["sigma_calls"], 1) / 1e3
         out["sigma_tflops_mixed"] = info.flops_sigma / sgm / 1e12
+        fz = fused_ranges(plan.tuning())
+        kind = "two-phase xgemm" + "".join(f", fused {lo}-{hi}" for lo, hi in fz)
         out["roofline"] = roofline(info, syn, a.precision, lm, measured_traffic(a.workload, a.precision),
-                                   f"{a.precision} ladder (xgemm phases 1+2, per sigma)")
+                                   f"{a.precision} ladder ({kind}; per sigma)", fz)
         out["roofline"]["launches"] = pm["ladder_launches"]
         out["roofline_fp64"] = roof64
     else:

@@ -41,6 +41,24 @@ def test_ladder_minimum_bytes():
     assert b.ladder_alg_bytes(syn, "fp64") == 7 * (2 * 8 * d2 + 2 * 8 * (100 + 400) + 2 * 8 * 900)
     syn32 = _syn(30, [10, 20], 7, f32=True)
     assert b.ladder_alg_bytes(syn32, "fp64") == 7 * (2 * 8 * 500 + 2 * 4 * 500)
+    # fused walks (split only) read L^J[D,D] once and move no T: walks of 70
+    # and 130 fused, 50 two-phase, one full walk of n_v = 300
+    syn = _syn(300, [50, 70, 130, 300], 3)
+    fz = [(65, 128), (129, 192)]
+    assert b.ladder_alg_bytes(syn, "bf16x3", fz) == 3 * (2 * 4 * (2500 + 90000) + 2 * 4 * 2500
+                                                          + 4 * (4900 + 16900) + 2 * 4 * 90000)
+    assert b.ladder_alg_bytes(syn, "fp64", fz) == b.ladder_alg_bytes(syn, "fp64")
+
+
+def test_fused_ranges_from_tuning_record():
+    b = _bench()
+    t = ("split ladder: ... whole two-phase:51.3ms fused512x:60.9ms -> two-phase; 65-128 two-phase:51.2ms "
+         "fused128-512:43.5ms -> fused128-512; order 0 1")
+    assert b.fused_ranges(t) == [(65, 128)]
+    t2 = "split ladder: ... whole two-phase:51.3ms fused512x:40.9ms -> fused512x; 65-128 two-phase:40ms -> two-phase"
+    assert b.fused_ranges(t2) == [(129, 192)]
+    assert b.fused_ranges(t2.replace("-> two-phase", "-> fused128-1024x")) == [(65, 128), (129, 192)]
+    assert b.fused_ranges("fp64 ladder: -> 8192/64/p1 128") == []
 
 
 def test_roofline_picks_the_binding_roof():

@@ -139,3 +139,40 @@ def test_fused_split_ladder(precision):
     assert _rel(a, ref) < tol and _rel(b, ref) < tol
     assert _rel(b, a) < tol / 5
     assert np.array_equal(b, b2)
+
+
+@pytest.mark.parametrize("precision,dr,n_chol", [
+    ("bf16x3", (40, 140), 24),
+    ("fp16x3", (40, 140), 24),
+    ("bf16x3", (65, 65), 7),
+    ("bf16x3", (128, 128), 40),
+])
+def test_fused128_split_ladder(precision, dr, n_chol):
+    """The whole-walk fused split ladder for domains of 65-128 (T' kept in
+    registers) reproduces the oracle and the two-phase ladder to the split
+    precision, next to two-phase walks of other sizes; J ranges of one and of
+    several Cholesky vectors."""
+    _say("case: fused128 split ladder", precision, dr, n_chol)
+    prob = make_problem(5, 200, n_chol, 61, domain_range=dr, weak_below=0)
+    ints = Integrals(prob)
+    x = random_ci_vector(prob, 13)
+    ref = oracle_sigma(prob, x, ints)
+    os.environ["TCI_AUTOTUNE"] = "0"
+    try:
+        a = _sigma(tci.Plan.from_problem(prob), x, precision=precision)
+        os.environ["TCI_SPLIT_FUSED128"] = "1"
+        fused = tci.Plan.from_problem(prob)
+        b = _sigma(fused, x, precision=precision)
+        b2 = _sigma(fused, x, precision=precision)
+        os.environ["TCI_SPLIT_FUSED"] = "1"  # both fused kernels and the two-phase walks below 65
+        c = _sigma(tci.Plan.from_problem(prob), x, precision=precision)
+    finally:
+        os.environ.pop("TCI_SPLIT_FUSED", None)
+        os.environ.pop("TCI_SPLIT_FUSED128", None)
+        del os.environ["TCI_AUTOTUNE"]
+    tol = 5e-5 if precision == "bf16x3" else 2e-6
+    _say(f"  rel err two-phase {_rel(a, ref):.2e}  fused128 {_rel(b, ref):.2e}  both fused {_rel(c, ref):.2e}"
+         f"  fused128 vs two-phase {_rel(b, a):.2e}")
+    assert _rel(a, ref) < tol and _rel(b, ref) < tol and _rel(c, ref) < tol
+    assert _rel(b, a) < tol / 5
+    assert np.array_equal(b, b2)

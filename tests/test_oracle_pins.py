@@ -124,3 +124,30 @@ def test_csf_counts_closed_form():
     assert lay.n_csf == 1 + 1000 + 500500
     prob = make_problem(16, 300, 4, 1)
     assert prob.layout().n_csf - 1 - 16 * 300 == 11522400
+
+
+def test_csf_counts_match_reference_methane_output():
+    """Counts printed by the reference for methane SDCI, 5 internal and 30
+    virtual orbitals, no truncation (tests/Tests/Methane_SDCI/
+    Methane_SDCI.output:380-396): valence 1; N-1 150; N-2 configurations
+    150 (0 singles, 1 spin function) + 2475 (2 singles, 1) + 4350 (4 singles,
+    2) = 11325 CSFs; 7126 configurations, 11476 CSFs in total."""
+    prob = make_problem(5, 30, 4, 1)
+    lay = prob.layout()
+    n_v = prob.n_v
+    assert lay.single_off + prob.n_int * n_v == lay.aa_off[0]
+    assert lay.single_off - lay.ref_off == 1 and prob.n_int * n_v == 150
+    conf0 = conf2 = conf4 = 0
+    for P, (i, j) in enumerate(prob.pairs):
+        d = int(lay.dsz[P])
+        if i == j:                    # a=b: 0 open shells; a<b: 2
+            conf0 += d
+            conf2 += d * (d - 1) // 2
+        else:                         # a=b: 2 open shells; a<b: 4
+            conf2 += d
+            conf4 += d * (d - 1) // 2
+            assert lay.nsp[P] == 2
+    assert (conf0, conf2, conf4) == (150, 2475, 4350)
+    assert lay.n_csf - lay.aa_off[0] == conf0 + conf2 + 2 * conf4 == 11325
+    assert 1 + 150 + conf0 + conf2 + conf4 == 7126
+    assert lay.n_csf == 11476

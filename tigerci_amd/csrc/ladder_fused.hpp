@@ -22,4 +22,10 @@ struct FusedTask {
 void launch_xladder_fused(int precision, const FusedTask* tasks, int64_t ntasks, const float* scales, hipStream_t st,
                           int remap);
 
+// Domains of 65..128 (ladder_fused128.hip): one task is a whole walk (a0 = 0)
+// for Cholesky vectors [j0, j1); R[n][a] += s4 sum_J sum_c L_J[n][c]
+// (s3 L_J C^T)[a][c] into the task's zero-filled K-split slice.
+void launch_xladder_fused128(int precision, const FusedTask* tasks, int64_t ntasks, const float* scales,
+                             hipStream_t st, int remap);
+
 }  // namespace tci

@@ -13,6 +13,7 @@
 // L^J[D,D] of local-domain pairs per chunk, C_P per sigma call.
 #include "plan.hpp"
 
+#include <algorithm>
 #include <cstdlib>
 
 namespace tci {
@@ -340,6 +341,10 @@ void mixed_prepare(tci_plan& P, int precision, hipStream_t st) {
       Lc = q;
       break;
     }
+  if (const char* e = std::getenv("TCI_SPLIT_CHUNK")) {  // Cholesky vectors per chunk, for sweeps
+    const long long v = std::atoll(e);
+    if (v > 0) Lc = std::min<int64_t>(n_chol, (int64_t)std::min<long long>(v, (long long)Lc));
+  }
   M.Tx.alloc((size_t)Lc * sum_x * 2);
   // padding columns of T' are never written by phase 1 and must read as zero
   launch_fill(M.Tx.as<double>(), (int64_t)(M.Tx.bytes / 8), 0.0, st);
@@ -360,8 +365,11 @@ void mixed_prepare(tci_plan& P, int precision, hipStream_t st) {
   // big: phase 2 of domains of 129-192 on single 192 x 192 tiles
   // fused: walks of 129-192 (prepacked) run the fused kernel over all J
   // instead of the two phases, with about fused_target workgroups
+  // fused2_target > 0: walks of 65-128 (prepacked) run the whole-walk fused
+  // kernel, split along J into about fused2_target workgroups
   auto fusable = [&](int64_t d) { return M.prepacked && d > 128 && d <= 192 && d != n_v; };
-  auto build = [&](int64_t target, bool big, bool fused, int64_t fused_target) {
+  auto fusable2 = [&](int64_t d) { return M.prepacked && d > 64 && d <= 128 && d != n_v; };
+  auto build = [&](int64_t target, bool big, bool fused, int64_t fused_target, int64_t fused2_target) {
     const int tile2 = 128;
     const int64_t tiles2m = cdiv((int64_t)n_pairs * n_v, tile2) * cdiv(n_v, tile2);
     M.rpart_n = (int)std::max<int64_t>(1, std::min<int64_t>(16, cdiv(target, std::max<int64_t>(tiles2m, 1))));
@@ -371,13 +379,17 @@ void mixed_prepare(tci_plan& P, int precision, hipStream_t st) {
     static const int t2m[4] = {64, 128, 128, 192}, t2n[4] = {64, 128, 192, 192};
     std::vector<int> v2of(n_pairs, 1);
     int nsplit_v[4] = {1, 1, 1, 1};
-    int nsf = 1;
+    int nsf = 1, nsf2 = 1;
     if (!M.merged) {
-      int64_t tv[4] = {0, 0, 0, 0}, tf = 0;
+      int64_t tv[4] = {0, 0, 0, 0}, tf = 0, tf2 = 0;
       for (int Q = 0; Q < n_pairs; ++Q) {
         const int64_t d = P.h_dom_off[Q + 1] - P.h_dom_off[Q];
         if (fused && fusable(d)) {
           tf += cdiv(d, 64);
+          continue;
+        }
+        if (fused2_target > 0 && fusable2(d)) {
+          ++tf2;
           continue;
         }
         int v = d <= 64 ? 0 : (d <= 128 ? 1 : ((double)d / (192.0 * cdiv(d, 192)) >= (double)d / (128.0 * cdiv(d, 128)) ? 2 : 1));
@@ -394,6 +406,10 @@ void mixed_prepare(tci_plan& P, int precision, hipStream_t st) {
         nsf = (int)std::max<int64_t>(1, std::min<int64_t>(5, cdiv(fused_target, tf)));
         M.rpart_n = std::max(M.rpart_n, 3 * nsf);
       }
+      if (tf2) {  // each J range of a whole-walk fused task writes its own slice
+        nsf2 = (int)std::max<int64_t>(1, std::min<int64_t>(std::min<int64_t>(16, n_chol), cdiv(fused2_target, tf2)));
+        M.rpart_n = std::max(M.rpart_n, nsf2);
+      }
     }
     if (M.rpart_n > 1 && M.Rpart.bytes < (size_t)M.rpart_n * P.sum_d2 * 8) {
       M.bytes -= (int64_t)M.Rpart.bytes;
@@ -403,6 +419,7 @@ void mixed_prepare(tci_plan& P, int precision, hipStream_t st) {
     M.phase.clear();
     M.chunks.clear();
     M.fused_h.clear();
+    M.fused2_h.clear();
     std::vector<int64_t> lens;
     for (int64_t J0 = 0; J0 < n_chol; J0 += Lc) {
       const int64_t len = std::min(Lc, n_chol - J0);
@@ -525,6 +542,24 @@ void mixed_prepare(tci_plan& P, int precision, hipStream_t st) {
           toff += len * d * ldx;
           continue;
         }
+        if (fused2_target > 0 && fusable2(d)) {
+          if (&ph == &M.phase.front())
+            for (int sp = 0; sp < nsf2; ++sp) {
+              FusedTask f{};
+              f.L = Lb;
+              f.C = Cx + cxo[Q];
+              f.R = (M.rpart_n > 1 ? M.Rpart.as<double>() + (int64_t)sp * P.sum_d2 : P.R.as<double>()) +
+                    P.h_cd_off[Q];
+              f.jstep = jstep;
+              f.d = d;
+              f.ldx = (int32_t)ldx;
+              f.j0 = (int32_t)(n_chol * sp / nsf2);
+              f.j1 = (int32_t)(n_chol * (sp + 1) / nsf2);
+              M.fused2_h.push_back(f);
+            }
+          toff += len * d * ldx;
+          continue;
+        }
         uint16_t* Tq = Tx + toff;
         int v1 = 0;
         for (int v = 1; v < 3; ++v)
@@ -566,6 +601,11 @@ void mixed_prepare(tci_plan& P, int precision, hipStream_t st) {
     }
     M.fused = !M.fused_h.empty();
     if (M.fused) upload(M.fused_d, M.fused_h, st);
+    // largest domains first, so the smallest tasks fill the tail of the launch
+    std::stable_sort(M.fused2_h.begin(), M.fused2_h.end(),
+                     [](const FusedTask& x, const FusedTask& y) { return x.d > y.d; });
+    M.fused2 = !M.fused2_h.empty();
+    if (M.fused2) upload(M.fused2_d, M.fused2_h, st);
   };
   const int64_t def_target = 512;
   auto progs = [&](tci_plan::LadderPhase& ph) {
@@ -582,12 +622,14 @@ void mixed_prepare(tci_plan& P, int precision, hipStream_t st) {
     // untuned: 128-row phase-2 tiles unless TCI_SPLIT_P2_192=1
     const char* e = std::getenv("TCI_SPLIT_P2_192");
     const char* f = std::getenv("TCI_SPLIT_FUSED");
-    build(def_target, e && e[0] == '1', f && f[0] == '1', 1024);
+    const char* f2 = std::getenv("TCI_SPLIT_FUSED128");
+    build(def_target, e && e[0] == '1', f && f[0] == '1', 1024, f2 && f2[0] == '1' ? 1024 : 0);
   } else if (M.tuned_target > 0) {
     // tuned before for the other split precision: same schedule, so one plan
     // gives the same sigma for a precision every time it is used
-    build(M.tuned_target, M.tuned_big, M.tuned_fused, M.tuned_fused_target);
+    build(M.tuned_target, M.tuned_big, M.tuned_fused, M.tuned_fused_target, M.tuned_fused2_target);
     M.fused_remap = M.tuned_fused_order;
+    M.fused2_remap = M.tuned_fused2_order;
     for (auto& ph : M.phase) {
       std::vector<GemmProgram*> g = progs(ph);
       for (size_t k = 0; k < g.size() && k < M.tuned_orders.size(); ++k) g[k]->remap = M.tuned_orders[k];
@@ -608,7 +650,7 @@ void mixed_prepare(tci_plan& P, int precision, hipStream_t st) {
     std::string log = "split ladder: (K-split target, 192-tiles)";
     for (int big = 0; big <= (any_big && !M.merged ? 1 : 0); ++big)
       for (int64_t t : cand) {
-        build(t, big != 0, false, 0);
+        build(t, big != 0, false, 0, 0);
         TCI_HIP(hipStreamSynchronize(st));
         const double ms = time_ms(st, 2, [&] { mixed_ladder_chunk(P, precision, M.chunks[0], st); });
         log += " " + std::to_string(t) + (big ? "b" : "") + ":" + fmt_ms(ms);
@@ -618,7 +660,7 @@ void mixed_prepare(tci_plan& P, int precision, hipStream_t st) {
           bb = big != 0;
         }
       }
-    build(bt, bb, false, 0);
+    build(bt, bb, false, 0, 0);
     TCI_HIP(hipStreamSynchronize(st));
     log += " -> " + std::to_string(bt) + (bb ? "b" : "");
     // then the fused kernel for the walks of 129-192 (whole ladder timed: it
@@ -634,7 +676,7 @@ void mixed_prepare(tci_plan& P, int precision, hipStream_t st) {
       int bfo = 1;
       for (int64_t ft : {512, 1024, 2048})
         for (int fo : {1, 0}) {
-          build(bt, bb, true, ft);
+          build(bt, bb, true, ft, 0);
           M.fused_remap = fo;
           TCI_HIP(hipStreamSynchronize(st));
           const double tf = time_ms(st, 2, [&] { mixed_ladder_run(P, precision, st); });
@@ -647,11 +689,39 @@ void mixed_prepare(tci_plan& P, int precision, hipStream_t st) {
           }
         }
       M.tuned_fused_order = bfo;
-      build(bt, bb, bf, bft);
+      build(bt, bb, bf, bft, 0);
       M.fused_remap = bfo;
       TCI_HIP(hipStreamSynchronize(st));
       log += bf ? " -> fused" + std::to_string(bft) + (bfo ? "x" : "") : " -> two-phase";
     }
+    // then the whole-walk fused kernel for the walks of 65-128
+    int64_t bft2 = 0;
+    int bfo2 = 0;
+    bool any_fusable2 = false;
+    for (int Q = 0; Q < n_pairs && !M.merged; ++Q) any_fusable2 = any_fusable2 || fusable2(P.h_dom_off[Q + 1] - P.h_dom_off[Q]);
+    if (any_fusable2) {
+      double tb = time_ms(st, 2, [&] { mixed_ladder_run(P, precision, st); });
+      log += "; 65-128 two-phase:" + fmt_ms(tb);
+      for (int64_t ft : {512, 1024, 2048})
+        for (int fo : {0, 1}) {
+          build(bt, bb, bf, bft, ft);
+          M.fused2_remap = fo;
+          TCI_HIP(hipStreamSynchronize(st));
+          const double tf = time_ms(st, 2, [&] { mixed_ladder_run(P, precision, st); });
+          log += " fused128-" + std::to_string(ft) + (fo ? "x" : "") + ":" + fmt_ms(tf);
+          if (tf < tb * 0.98) {
+            tb = tf;
+            bft2 = ft;
+            bfo2 = fo;
+          }
+        }
+      build(bt, bb, bf, bft, bft2);
+      M.fused2_remap = bfo2;
+      TCI_HIP(hipStreamSynchronize(st));
+      log += bft2 ? " -> fused128-" + std::to_string(bft2) + (bfo2 ? "x" : "") : " -> two-phase";
+    }
+    M.tuned_fused2_target = bft2;
+    M.tuned_fused2_order = bfo2;
     log += "; order";
     const auto& ch = M.chunks[0];
     std::vector<GemmProgram*> g0 = progs(M.phase[ch.second]);
@@ -702,6 +772,9 @@ void mixed_ladder_run(tci_plan& P, int precision, hipStream_t st) {
   if (M.fused)
     launch_xladder_fused(precision, M.fused_d.as<FusedTask>(), (int64_t)M.fused_h.size(), sc, st,
                          M.fused_remap < 0 || xcd_remap_forced() ? xcd_remap_default() : M.fused_remap);
+  if (M.fused2)
+    launch_xladder_fused128(precision, M.fused2_d.as<FusedTask>(), (int64_t)M.fused2_h.size(), sc, st,
+                            xcd_remap_forced() ? xcd_remap_default() : M.fused2_remap);
   for (const auto& ch : M.chunks) mixed_ladder_chunk(P, precision, ch, st);
   if (P.timing()) TCI_HIP(hipEventRecord(P.ev[3], st));
   if (M.rpart_n > 1) ladder_reduce(M.Rpart.as<double>(), P.sum_d2, M.rpart_n, P.R.as<double>(), P.sum_d2, st);

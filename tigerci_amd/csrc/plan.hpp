@@ -98,6 +98,13 @@ struct tci_plan {
     int tuned_fused_order = 1;
     bool tuned_fused = false;
     int64_t tuned_fused_target = 0;
+    // fused ladder of whole walks of 65-128 (prepacked L), by descending domain
+    bool fused2 = false;
+    std::vector<tci::FusedTask> fused2_h;
+    tci::DevBuf fused2_d;
+    int fused2_remap = 0;
+    int tuned_fused2_order = 0;
+    int64_t tuned_fused2_target = 0;
     // batched programs for `nvec` vectors (local domains, two-phase): C_v split
     // rows per walk [Q][v][c][k], T' of one chunk [Q][jj][v][a][c], R_v back to
     // back and their K-split partials
