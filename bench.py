@@ -36,12 +36,12 @@ def log(*a):
 
 def fused_ranges(tuning: str) -> list:
     """Domain-size ranges whose split ladder ran as a fused kernel, read from
-    the plan's tuning record ("-> fused128-N" for 65-128, "-> fusedN" for
-    129-192)."""
+    the plan's tuning record ("-> fused128-N" for 65-128, "-> fusedN" or
+    "-> fused192-N" for 129-192)."""
     out = []
     if re.search(r"-> fused128-\d", tuning):
         out.append((65, 128))
-    if re.search(r"-> fused\d+x?(?![\d-])", tuning):
+    if re.search(r"-> fused\d+x?(?![\d-])", tuning) or re.search(r"-> fused192-\d", tuning):
         out.append((129, 192))
     return out
 

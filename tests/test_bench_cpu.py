@@ -59,6 +59,8 @@ def test_fused_ranges_from_tuning_record():
     assert b.fused_ranges(t2) == [(129, 192)]
     assert b.fused_ranges(t2.replace("-> two-phase", "-> fused128-1024x")) == [(65, 128), (129, 192)]
     assert b.fused_ranges("fp64 ladder: -> 8192/64/p1 128") == []
+    t3 = t + "; 129-192 current:43.4ms fused192-512:30.1ms -> fused192-512"
+    assert b.fused_ranges(t3) == [(65, 128), (129, 192)]
 
 
 def test_roofline_picks_the_binding_roof():

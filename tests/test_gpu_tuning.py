@@ -176,3 +176,40 @@ def test_fused128_split_ladder(precision, dr, n_chol):
     assert _rel(a, ref) < tol and _rel(b, ref) < tol and _rel(c, ref) < tol
     assert _rel(b, a) < tol / 5
     assert np.array_equal(b, b2)
+
+
+@pytest.mark.parametrize("precision,dr,n_chol", [
+    ("bf16x3", (100, 192), 24),
+    ("fp16x3", (100, 192), 24),
+    ("bf16x3", (129, 129), 7),
+    ("bf16x3", (192, 192), 40),
+])
+def test_fused192_split_ladder(precision, dr, n_chol):
+    """The whole-walk fused split ladder for domains of 129-192 (C in LDS, L^J
+    from global memory, T' in registers) reproduces the oracle and the
+    two-phase ladder to the split precision, alone and next to the fused
+    kernel for 65-128; J ranges of one and of several Cholesky vectors."""
+    _say("case: fused192 split ladder", precision, dr, n_chol)
+    prob = make_problem(5, 200, n_chol, 62, domain_range=dr, weak_below=0)
+    ints = Integrals(prob)
+    x = random_ci_vector(prob, 14)
+    ref = oracle_sigma(prob, x, ints)
+    os.environ["TCI_AUTOTUNE"] = "0"
+    try:
+        a = _sigma(tci.Plan.from_problem(prob), x, precision=precision)
+        os.environ["TCI_SPLIT_FUSED192"] = "1"
+        fused = tci.Plan.from_problem(prob)
+        b = _sigma(fused, x, precision=precision)
+        b2 = _sigma(fused, x, precision=precision)
+        os.environ["TCI_SPLIT_FUSED128"] = "1"
+        c = _sigma(tci.Plan.from_problem(prob), x, precision=precision)
+    finally:
+        os.environ.pop("TCI_SPLIT_FUSED192", None)
+        os.environ.pop("TCI_SPLIT_FUSED128", None)
+        del os.environ["TCI_AUTOTUNE"]
+    tol = 5e-5 if precision == "bf16x3" else 2e-6
+    _say(f"  rel err two-phase {_rel(a, ref):.2e}  fused192 {_rel(b, ref):.2e}  both fused {_rel(c, ref):.2e}"
+         f"  fused192 vs two-phase {_rel(b, a):.2e}")
+    assert _rel(a, ref) < tol and _rel(b, ref) < tol and _rel(c, ref) < tol
+    assert _rel(b, a) < tol / 5
+    assert np.array_equal(b, b2)

@@ -28,4 +28,10 @@ void launch_xladder_fused(int precision, const FusedTask* tasks, int64_t ntasks,
 void launch_xladder_fused128(int precision, const FusedTask* tasks, int64_t ntasks, const float* scales,
                              hipStream_t st, int remap);
 
+// Domains of 129..192 (ladder_fused192.hip): one task is a whole walk (a0 = 0)
+// for Cholesky vectors [j0, j1); C_Q in LDS, L_J read from global memory.
+// The L blocks must be followed by 16 KiB of readable memory.
+void launch_xladder_fused192(int precision, const FusedTask* tasks, int64_t ntasks, const float* scales,
+                             hipStream_t st, int remap);
+
 }  // namespace tci

@@ -320,7 +320,8 @@ void mixed_prepare(tci_plan& P, int precision, hipStream_t st) {
       lpo_all[q + 1] = lpo_all[q] + n_chol * d * 2 * xpad(d);
     }
     upload(M.lp_off_all, lpo_all, st);
-    M.Lpx.alloc((size_t)lpo_all.back() * 2);
+    // 16 KiB tail: the fused kernel for 129-192 reads rows past a block's end
+    M.Lpx.alloc((size_t)lpo_all.back() * 2 + 16384);
     M.bytes += (int64_t)M.Lpx.bytes;
     if (P.lvv_f32)
       hipLaunchKernelGGL(pack_L_split_kernel<float>, dim3(4096, (unsigned)loc.size()), dim3(256), 0, st, P.tab,
@@ -369,7 +370,11 @@ void mixed_prepare(tci_plan& P, int precision, hipStream_t st) {
   // kernel, split along J into about fused2_target workgroups
   auto fusable = [&](int64_t d) { return M.prepacked && d > 128 && d <= 192 && d != n_v; };
   auto fusable2 = [&](int64_t d) { return M.prepacked && d > 64 && d <= 128 && d != n_v; };
-  auto build = [&](int64_t target, bool big, bool fused, int64_t fused_target, int64_t fused2_target) {
+  // fused3_target > 0: walks of 129-192 (prepacked) run the whole-walk kernel
+  // with C in LDS (before the C-panel kernel, which then gets none of them)
+  auto fusable3 = [&](int64_t d) { return M.prepacked && d > 128 && d <= 192 && d != n_v; };
+  auto build = [&](int64_t target, bool big, bool fused, int64_t fused_target, int64_t fused2_target,
+                   int64_t fused3_target) {
     const int tile2 = 128;
     const int64_t tiles2m = cdiv((int64_t)n_pairs * n_v, tile2) * cdiv(n_v, tile2);
     M.rpart_n = (int)std::max<int64_t>(1, std::min<int64_t>(16, cdiv(target, std::max<int64_t>(tiles2m, 1))));
@@ -379,11 +384,15 @@ void mixed_prepare(tci_plan& P, int precision, hipStream_t st) {
     static const int t2m[4] = {64, 128, 128, 192}, t2n[4] = {64, 128, 192, 192};
     std::vector<int> v2of(n_pairs, 1);
     int nsplit_v[4] = {1, 1, 1, 1};
-    int nsf = 1, nsf2 = 1;
+    int nsf = 1, nsf2 = 1, nsf3 = 1;
     if (!M.merged) {
-      int64_t tv[4] = {0, 0, 0, 0}, tf = 0, tf2 = 0;
+      int64_t tv[4] = {0, 0, 0, 0}, tf = 0, tf2 = 0, tf3 = 0;
       for (int Q = 0; Q < n_pairs; ++Q) {
         const int64_t d = P.h_dom_off[Q + 1] - P.h_dom_off[Q];
+        if (fused3_target > 0 && fusable3(d)) {
+          ++tf3;
+          continue;
+        }
         if (fused && fusable(d)) {
           tf += cdiv(d, 64);
           continue;
@@ -410,6 +419,10 @@ void mixed_prepare(tci_plan& P, int precision, hipStream_t st) {
         nsf2 = (int)std::max<int64_t>(1, std::min<int64_t>(std::min<int64_t>(16, n_chol), cdiv(fused2_target, tf2)));
         M.rpart_n = std::max(M.rpart_n, nsf2);
       }
+      if (tf3) {
+        nsf3 = (int)std::max<int64_t>(1, std::min<int64_t>(std::min<int64_t>(16, n_chol), cdiv(fused3_target, tf3)));
+        M.rpart_n = std::max(M.rpart_n, nsf3);
+      }
     }
     if (M.rpart_n > 1 && M.Rpart.bytes < (size_t)M.rpart_n * P.sum_d2 * 8) {
       M.bytes -= (int64_t)M.Rpart.bytes;
@@ -420,6 +433,7 @@ void mixed_prepare(tci_plan& P, int precision, hipStream_t st) {
     M.chunks.clear();
     M.fused_h.clear();
     M.fused2_h.clear();
+    M.fused3_h.clear();
     std::vector<int64_t> lens;
     for (int64_t J0 = 0; J0 < n_chol; J0 += Lc) {
       const int64_t len = std::min(Lc, n_chol - J0);
@@ -521,6 +535,24 @@ void mixed_prepare(tci_plan& P, int precision, hipStream_t st) {
           jstep = (int64_t)d * ldx;
           jstride = 0;
         }
+        if (fused3_target > 0 && fusable3(d)) {
+          if (&ph == &M.phase.front())
+            for (int sp = 0; sp < nsf3; ++sp) {
+              FusedTask f{};
+              f.L = Lb;
+              f.C = Cx + cxo[Q];
+              f.R = (M.rpart_n > 1 ? M.Rpart.as<double>() + (int64_t)sp * P.sum_d2 : P.R.as<double>()) +
+                    P.h_cd_off[Q];
+              f.jstep = jstep;
+              f.d = d;
+              f.ldx = (int32_t)ldx;
+              f.j0 = (int32_t)(n_chol * sp / nsf3);
+              f.j1 = (int32_t)(n_chol * (sp + 1) / nsf3);
+              M.fused3_h.push_back(f);
+            }
+          toff += len * d * ldx;
+          continue;
+        }
         if (fused && fusable(d)) {
           // every J at once (prepacked), so only the first chunk length's
           // phase builds the tasks
@@ -606,6 +638,10 @@ void mixed_prepare(tci_plan& P, int precision, hipStream_t st) {
                      [](const FusedTask& x, const FusedTask& y) { return x.d > y.d; });
     M.fused2 = !M.fused2_h.empty();
     if (M.fused2) upload(M.fused2_d, M.fused2_h, st);
+    std::stable_sort(M.fused3_h.begin(), M.fused3_h.end(),
+                     [](const FusedTask& x, const FusedTask& y) { return x.d > y.d; });
+    M.fused3 = !M.fused3_h.empty();
+    if (M.fused3) upload(M.fused3_d, M.fused3_h, st);
   };
   const int64_t def_target = 512;
   auto progs = [&](tci_plan::LadderPhase& ph) {
@@ -623,13 +659,17 @@ void mixed_prepare(tci_plan& P, int precision, hipStream_t st) {
     const char* e = std::getenv("TCI_SPLIT_P2_192");
     const char* f = std::getenv("TCI_SPLIT_FUSED");
     const char* f2 = std::getenv("TCI_SPLIT_FUSED128");
-    build(def_target, e && e[0] == '1', f && f[0] == '1', 1024, f2 && f2[0] == '1' ? 1024 : 0);
+    const char* f3 = std::getenv("TCI_SPLIT_FUSED192");
+    build(def_target, e && e[0] == '1', f && f[0] == '1', 1024, f2 && f2[0] == '1' ? 1024 : 0,
+          f3 && f3[0] == '1' ? 1024 : 0);
   } else if (M.tuned_target > 0) {
     // tuned before for the other split precision: same schedule, so one plan
     // gives the same sigma for a precision every time it is used
-    build(M.tuned_target, M.tuned_big, M.tuned_fused, M.tuned_fused_target, M.tuned_fused2_target);
+    build(M.tuned_target, M.tuned_big, M.tuned_fused, M.tuned_fused_target, M.tuned_fused2_target,
+          M.tuned_fused3_target);
     M.fused_remap = M.tuned_fused_order;
     M.fused2_remap = M.tuned_fused2_order;
+    M.fused3_remap = M.tuned_fused3_order;
     for (auto& ph : M.phase) {
       std::vector<GemmProgram*> g = progs(ph);
       for (size_t k = 0; k < g.size() && k < M.tuned_orders.size(); ++k) g[k]->remap = M.tuned_orders[k];
@@ -650,7 +690,7 @@ void mixed_prepare(tci_plan& P, int precision, hipStream_t st) {
     std::string log = "split ladder: (K-split target, 192-tiles)";
     for (int big = 0; big <= (any_big && !M.merged ? 1 : 0); ++big)
       for (int64_t t : cand) {
-        build(t, big != 0, false, 0, 0);
+        build(t, big != 0, false, 0, 0, 0);
         TCI_HIP(hipStreamSynchronize(st));
         const double ms = time_ms(st, 2, [&] { mixed_ladder_chunk(P, precision, M.chunks[0], st); });
         log += " " + std::to_string(t) + (big ? "b" : "") + ":" + fmt_ms(ms);
@@ -660,7 +700,7 @@ void mixed_prepare(tci_plan& P, int precision, hipStream_t st) {
           bb = big != 0;
         }
       }
-    build(bt, bb, false, 0, 0);
+    build(bt, bb, false, 0, 0, 0);
     TCI_HIP(hipStreamSynchronize(st));
     log += " -> " + std::to_string(bt) + (bb ? "b" : "");
     // then the fused kernel for the walks of 129-192 (whole ladder timed: it
@@ -676,7 +716,7 @@ void mixed_prepare(tci_plan& P, int precision, hipStream_t st) {
       int bfo = 1;
       for (int64_t ft : {512, 1024, 2048})
         for (int fo : {1, 0}) {
-          build(bt, bb, true, ft, 0);
+          build(bt, bb, true, ft, 0, 0);
           M.fused_remap = fo;
           TCI_HIP(hipStreamSynchronize(st));
           const double tf = time_ms(st, 2, [&] { mixed_ladder_run(P, precision, st); });
@@ -689,7 +729,7 @@ void mixed_prepare(tci_plan& P, int precision, hipStream_t st) {
           }
         }
       M.tuned_fused_order = bfo;
-      build(bt, bb, bf, bft, 0);
+      build(bt, bb, bf, bft, 0, 0);
       M.fused_remap = bfo;
       TCI_HIP(hipStreamSynchronize(st));
       log += bf ? " -> fused" + std::to_string(bft) + (bfo ? "x" : "") : " -> two-phase";
@@ -704,7 +744,7 @@ void mixed_prepare(tci_plan& P, int precision, hipStream_t st) {
       log += "; 65-128 two-phase:" + fmt_ms(tb);
       for (int64_t ft : {512, 1024, 2048})
         for (int fo : {0, 1}) {
-          build(bt, bb, bf, bft, ft);
+          build(bt, bb, bf, bft, ft, 0);
           M.fused2_remap = fo;
           TCI_HIP(hipStreamSynchronize(st));
           const double tf = time_ms(st, 2, [&] { mixed_ladder_run(P, precision, st); });
@@ -715,13 +755,18 @@ void mixed_prepare(tci_plan& P, int precision, hipStream_t st) {
             bfo2 = fo;
           }
         }
-      build(bt, bb, bf, bft, bft2);
+      build(bt, bb, bf, bft, bft2, 0);
       M.fused2_remap = bfo2;
       TCI_HIP(hipStreamSynchronize(st));
       log += bft2 ? " -> fused128-" + std::to_string(bft2) + (bfo2 ? "x" : "") : " -> two-phase";
     }
     M.tuned_fused2_target = bft2;
     M.tuned_fused2_order = bfo2;
+    // the whole-walk kernel with C in LDS for the walks of 129-192
+    // (TCI_SPLIT_FUSED192) is not a candidate: at 34/192/1200 it measured 15%
+    // slower than the two phases for those walks
+    M.tuned_fused3_target = 0;
+    M.tuned_fused3_order = 0;
     log += "; order";
     const auto& ch = M.chunks[0];
     std::vector<GemmProgram*> g0 = progs(M.phase[ch.second]);
@@ -775,6 +820,9 @@ void mixed_ladder_run(tci_plan& P, int precision, hipStream_t st) {
   if (M.fused2)
     launch_xladder_fused128(precision, M.fused2_d.as<FusedTask>(), (int64_t)M.fused2_h.size(), sc, st,
                             xcd_remap_forced() ? xcd_remap_default() : M.fused2_remap);
+  if (M.fused3)
+    launch_xladder_fused192(precision, M.fused3_d.as<FusedTask>(), (int64_t)M.fused3_h.size(), sc, st,
+                            xcd_remap_forced() ? xcd_remap_default() : M.fused3_remap);
   for (const auto& ch : M.chunks) mixed_ladder_chunk(P, precision, ch, st);
   if (P.timing()) TCI_HIP(hipEventRecord(P.ev[3], st));
   if (M.rpart_n > 1) ladder_reduce(M.Rpart.as<double>(), P.sum_d2, M.rpart_n, P.R.as<double>(), P.sum_d2, st);

@@ -105,6 +105,13 @@ struct tci_plan {
     int fused2_remap = 0;
     int tuned_fused2_order = 0;
     int64_t tuned_fused2_target = 0;
+    // fused ladder of whole walks of 129-192 (prepacked L, C in LDS), by descending domain
+    bool fused3 = false;
+    std::vector<tci::FusedTask> fused3_h;
+    tci::DevBuf fused3_d;
+    int fused3_remap = 0;
+    int tuned_fused3_order = 0;
+    int64_t tuned_fused3_target = 0;
     // batched programs for `nvec` vectors (local domains, two-phase): C_v split
     // rows per walk [Q][v][c][k], T' of one chunk [Q][jj][v][a][c], R_v back to
     // back and their K-split partials
