@@ -92,20 +92,22 @@ def test_phase2_192_tiles():
     assert _rel(b, a) < 1e-5
 
 
-def test_fp64_192_tiles():
-    """Both FP64 ladder phases of domains of 129-192 on 192 x 192 tiles
-    (eight waves per workgroup) reproduce the oracle and the default tiles."""
-    _say("case: FP64 ladder on 192 x 192 tiles")
+@pytest.mark.parametrize("env", ["TCI_FP64_P192", "TCI_FP64_P1_192"])
+def test_fp64_192_tiles(env):
+    """Both FP64 ladder phases (TCI_FP64_P192), or phase 1 alone
+    (TCI_FP64_P1_192), of domains of 129-192 on 192 x 192 tiles (eight waves
+    per workgroup) reproduce the oracle and the default tiles."""
+    _say("case: FP64 ladder on 192 x 192 tiles", env)
     prob = make_problem(3, 200, 24, 53, domain_range=(100, 192), weak_below=0)
     ints = Integrals(prob)
     xs = [random_ci_vector(prob, 11 + k) for k in range(2)]
     os.environ["TCI_AUTOTUNE"] = "0"
     try:
         base = tci.Plan.from_problem(prob)
-        os.environ["TCI_FP64_P192"] = "1"
+        os.environ[env] = "1"
         big = tci.Plan.from_problem(prob)
     finally:
-        os.environ.pop("TCI_FP64_P192", None)
+        os.environ.pop(env, None)
         del os.environ["TCI_AUTOTUNE"]
     for x in xs:
         ref = oracle_sigma(prob, x, ints)

@@ -366,10 +366,14 @@ void dense_pairs_build_split(tci_plan& P, hipStream_t st) {
   TCI_HIP(hipGetLastError());
   const int iN = (int)N, K = (int)(n * xpad(nv)), KJ = (int)xpad(P.n_chol);
   auto xop = [&](const DevBuf& b, int64_t ld) { return dop(b.p, ld); };
-  // B operands as [n][k] rows: Kov, Jbig and CC are symmetric; Lov enters transposed
+  // B operands as [n][k] rows: Kov, Jbig and CC are symmetric; Lov enters transposed.
+  // Z and Z3 are square N x N outputs: 128 x 128 tiles, or 192 x 192 with
+  // TCI_PAIRS_TILE=192 (each operand panel fetched a third less often)
+  const char* te = std::getenv("TCI_PAIRS_TILE");
+  const int ztile = te && std::atoi(te) == 192 ? 192 : 128;
   M.ringZx.clear();
   M.ringZx.split = true;
-  M.ringZx.tile = 128;
+  M.ringZx.tile = ztile;
   M.ringZx.add(P.Zb.as<double>(), N, iN, iN,
                {dseg(xop(M.CtRx, ldx), xop(M.Kx, ldx), K, 1.0), dseg(xop(M.CRx, ldx), xop(M.Jx, ldx), K, 1.0),
                 dseg(xop(M.Ytx, ldj), xop(M.LovTx, ldj), KJ, 1.0)},
@@ -377,7 +381,7 @@ void dense_pairs_build_split(tci_plan& P, hipStream_t st) {
   M.ringZx.upload_tables(st);
   M.ringZ3x.clear();
   M.ringZ3x.split = true;
-  M.ringZ3x.tile = 128;
+  M.ringZ3x.tile = ztile;
   M.ringZ3x.add(P.Z3b.as<double>(), N, iN, iN, {dseg(xop(M.Jx, ldx), xop(M.CCx, ldx), K, 1.0)}, false, false, 0,
                 Blocking(), 5);
   M.ringZ3x.upload_tables(st);

@@ -587,7 +587,8 @@ void tci_plan::build_two_phase_ladder(hipStream_t st) {
   // programs for a K-split fill target: phase 2 is split along K until it has
   // about `target` workgroups (at most 16 splits)
   // tile2: phase-2 tile of the per-walk programs (64 or 128)
-  // tile1: phase-1 tile of the per-walk programs for domains above 64
+  // tile1: phase-1 tile of the per-walk programs for domains above 64 (64 or
+  //        128; 192 puts the domains of 129-192 on 192 x 192 tiles, 65-128 on 128)
   // big: both phases of local domains of 129-192 on 192 x 192 tiles (one
   //      workgroup per CU, so their K-split target is a quarter)
   auto is_big = [&](int64_t d) { return d > 128 && d <= 192 && d != n_v; };
@@ -663,7 +664,7 @@ void tci_plan::build_two_phase_ladder(hipStream_t st) {
       ph.p2x.resize(2);
       for (int v = 0; v < 3; ++v) {
         ph.p1x[v].dense = true;
-        ph.p1x[v].tile = v == 2 ? 192 : (v ? tile1 : 64);
+        ph.p1x[v].tile = v == 2 ? 192 : (v ? (tile1 == 192 ? 128 : tile1) : 64);
       }
       ph.p2x[0].dense = ph.p2x[1].dense = true;
       ph.p2x[0].tile = tile2;
@@ -674,7 +675,8 @@ void tci_plan::build_two_phase_ladder(hipStream_t st) {
         const int d = (int)(h_dom_off[P + 1] - h_dom_off[P]);
         const bool full = d == n_v;
         const bool b192 = big && is_big(d);
-        const int v = b192 ? 2 : (d <= 64 ? 0 : 1);
+        // tile1 == 192: phase 1 of the domains of 129-192 on 192 x 192 tiles, phase 2 on tile2
+        const int v = (b192 || (tile1 == 192 && is_big(d))) ? 2 : (d <= 64 ? 0 : 1);
         const int nsp = (int)std::min<int64_t>(b192 ? nsp_big : nsp_small, len);
         double* T = Tbuf.as<double>() + toff;
         // L operand of this pair: rows (jj, a) with leading dimension ld, one
@@ -721,9 +723,11 @@ void tci_plan::build_two_phase_ladder(hipStream_t st) {
   // 64-wide phase-2 tiles run 4 workgroups per CU, 128-wide ones 2
   const int64_t def_target = ladder_merged ? 512 : 2048;
   if (!autotune_enabled()) {
-    // untuned: no 192 x 192 tiles unless TCI_FP64_P192=1
+    // untuned: no 192 x 192 tiles unless TCI_FP64_P192=1 (both phases of
+    // 129-192) or TCI_FP64_P1_192=1 (their phase 1)
     const char* e = std::getenv("TCI_FP64_P192");
-    build(def_target, 64, 128, e && e[0] == '1');
+    const char* e1 = std::getenv("TCI_FP64_P1_192");
+    build(def_target, 64, e1 && e1[0] == '1' ? 192 : 128, e && e[0] == '1');
     return;
   }
   // autotuning on the first chunk (C holds no vector yet; the timing does
@@ -763,6 +767,16 @@ void tci_plan::build_two_phase_ladder(hipStream_t st) {
     if (ms < best * 0.98) {
       best = ms;
       btile1 = 64;
+    }
+    if (any_big && !bbig) {  // phase 1 of 129-192 alone on 192 x 192 tiles
+      build(bt, btile, 192, bbig);
+      TCI_HIP(hipStreamSynchronize(st));
+      const double m2 = time_ms(st, 2, [&] { ladder_chunk(lchunks[0], st); });
+      log += " 192:" + fmt_ms(m2);
+      if (m2 < best * 0.98) {
+        best = m2;
+        btile1 = 192;
+      }
     }
   }
   build(bt, btile, btile1, bbig);
