@@ -166,8 +166,9 @@ def main() -> int:
                     help="after the timed region, copy this many MiB device-to-device (counter calibration)")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print(json.dumps({"error": "this build runs on one GPU only: the sigma path is not sharded "
-                                    "(see the sharding design notes); run with --gpus 1"}))
+        print(json.dumps({"error": "the sigma path runs on one GPU: it is not sharded across GPUs "
+                                    "(by pair walk, with one all-gather of c per sigma, is how it would be); "
+                                    "run with --gpus 1"}))
         return 2
 
     if a.ladder_launch == "per-pair":

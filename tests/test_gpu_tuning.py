@@ -215,3 +215,33 @@ def test_fused192_split_ladder(precision, dr, n_chol):
     assert _rel(a, ref) < tol and _rel(b, ref) < tol and _rel(c, ref) < tol
     assert _rel(b, a) < tol / 5
     assert np.array_equal(b, b2)
+
+
+@pytest.mark.parametrize("precision", ["bf16x3", "fp16x3"])
+def test_split_direct_load_stages(precision):
+    """Phase 2 of the domains of 129-192 on 192 x 192 tiles fed by direct
+    global-to-LDS loads (three stages) gives the register-staged result up to
+    FP32 summation order and the oracle to the split precision; K-splits of
+    one and of several Cholesky vectors, domains below and at 192."""
+    _say("case: split 192 tiles with direct-load stages", precision)
+    prob = make_problem(4, 200, 30, 63, domain_range=(129, 192), weak_below=0)
+    ints = Integrals(prob)
+    x = random_ci_vector(prob, 15)
+    ref = oracle_sigma(prob, x, ints)
+    os.environ["TCI_AUTOTUNE"] = "0"
+    os.environ["TCI_SPLIT_P2_192"] = "1"
+    try:
+        a = _sigma(tci.Plan.from_problem(prob), x, precision=precision)
+        os.environ["TCI_SPLIT_DMA"] = "1"
+        plan = tci.Plan.from_problem(prob)
+        b = _sigma(plan, x, precision=precision)
+        b2 = _sigma(plan, x, precision=precision)
+    finally:
+        os.environ.pop("TCI_SPLIT_DMA", None)
+        os.environ.pop("TCI_SPLIT_P2_192", None)
+        del os.environ["TCI_AUTOTUNE"]
+    tol = 5e-5 if precision == "bf16x3" else 2e-6
+    _say(f"  rel err register-staged {_rel(a, ref):.2e}  direct loads {_rel(b, ref):.2e}  vs each other {_rel(b, a):.2e}")
+    assert _rel(a, ref) < tol and _rel(b, ref) < tol
+    assert _rel(b, a) < tol / 5
+    assert np.array_equal(b, b2)

@@ -654,6 +654,12 @@ void mixed_prepare(tci_plan& P, int precision, hipStream_t st) {
     for (auto& g : ph.p2x) v.push_back(&g);
     return v;
   };
+  // phase 2 of the domains of 129-192 on 192 x 192 tiles: register-staged or
+  // fed by direct global-to-LDS loads (three stages)
+  auto set_dma = [&](bool on) {
+    for (auto& ph : M.phase)
+      if (ph.p2x.size() > 3) ph.p2x[3].dma = on && ph.p2x[3].tile == 192;
+  };
   if (M.per_pair || !autotune_enabled()) {
     // untuned: 128-row phase-2 tiles unless TCI_SPLIT_P2_192=1
     const char* e = std::getenv("TCI_SPLIT_P2_192");
@@ -662,6 +668,8 @@ void mixed_prepare(tci_plan& P, int precision, hipStream_t st) {
     const char* f3 = std::getenv("TCI_SPLIT_FUSED192");
     build(def_target, e && e[0] == '1', f && f[0] == '1', 1024, f2 && f2[0] == '1' ? 1024 : 0,
           f3 && f3[0] == '1' ? 1024 : 0);
+    const char* fd = std::getenv("TCI_SPLIT_DMA");
+    set_dma(fd && fd[0] == '1');
   } else if (M.tuned_target > 0) {
     // tuned before for the other split precision: same schedule, so one plan
     // gives the same sigma for a precision every time it is used
@@ -670,6 +678,7 @@ void mixed_prepare(tci_plan& P, int precision, hipStream_t st) {
     M.fused_remap = M.tuned_fused_order;
     M.fused2_remap = M.tuned_fused2_order;
     M.fused3_remap = M.tuned_fused3_order;
+    set_dma(M.tuned_dma);
     for (auto& ph : M.phase) {
       std::vector<GemmProgram*> g = progs(ph);
       for (size_t k = 0; k < g.size() && k < M.tuned_orders.size(); ++k) g[k]->remap = M.tuned_orders[k];
@@ -703,6 +712,15 @@ void mixed_prepare(tci_plan& P, int precision, hipStream_t st) {
     build(bt, bb, false, 0, 0, 0);
     TCI_HIP(hipStreamSynchronize(st));
     log += " -> " + std::to_string(bt) + (bb ? "b" : "");
+    bool bdma = false;
+    if (bb) {  // the 192 x 192 phase-2 tiles with direct-load stages
+      set_dma(true);
+      const double md = time_ms(st, 2, [&] { mixed_ladder_chunk(P, precision, M.chunks[0], st); });
+      log += "; direct-load stages:" + fmt_ms(md);
+      bdma = md < best * 0.98;
+      set_dma(bdma);
+      log += bdma ? " -> on" : " -> off";
+    }
     // then the fused kernel for the walks of 129-192 (whole ladder timed: it
     // runs over every Cholesky vector in one launch)
     bool bf = false;
@@ -717,6 +735,7 @@ void mixed_prepare(tci_plan& P, int precision, hipStream_t st) {
       for (int64_t ft : {512, 1024, 2048})
         for (int fo : {1, 0}) {
           build(bt, bb, true, ft, 0, 0);
+          set_dma(bdma);
           M.fused_remap = fo;
           TCI_HIP(hipStreamSynchronize(st));
           const double tf = time_ms(st, 2, [&] { mixed_ladder_run(P, precision, st); });
@@ -730,6 +749,7 @@ void mixed_prepare(tci_plan& P, int precision, hipStream_t st) {
         }
       M.tuned_fused_order = bfo;
       build(bt, bb, bf, bft, 0, 0);
+      set_dma(bdma);
       M.fused_remap = bfo;
       TCI_HIP(hipStreamSynchronize(st));
       log += bf ? " -> fused" + std::to_string(bft) + (bfo ? "x" : "") : " -> two-phase";
@@ -745,6 +765,7 @@ void mixed_prepare(tci_plan& P, int precision, hipStream_t st) {
       for (int64_t ft : {512, 1024, 2048})
         for (int fo : {0, 1}) {
           build(bt, bb, bf, bft, ft, 0);
+          set_dma(bdma);
           M.fused2_remap = fo;
           TCI_HIP(hipStreamSynchronize(st));
           const double tf = time_ms(st, 2, [&] { mixed_ladder_run(P, precision, st); });
@@ -756,6 +777,7 @@ void mixed_prepare(tci_plan& P, int precision, hipStream_t st) {
           }
         }
       build(bt, bb, bf, bft, bft2, 0);
+      set_dma(bdma);
       M.fused2_remap = bfo2;
       TCI_HIP(hipStreamSynchronize(st));
       log += bft2 ? " -> fused128-" + std::to_string(bft2) + (bfo2 ? "x" : "") : " -> two-phase";
@@ -781,6 +803,7 @@ void mixed_prepare(tci_plan& P, int precision, hipStream_t st) {
     }
     M.tuned_target = bt;
     M.tuned_big = bb;
+    M.tuned_dma = bdma;
     M.tuned_fused = bf;
     M.tuned_fused_target = bft;
     P.tuning += log + "\n";

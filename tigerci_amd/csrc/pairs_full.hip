@@ -368,9 +368,11 @@ void dense_pairs_build_split(tci_plan& P, hipStream_t st) {
   auto xop = [&](const DevBuf& b, int64_t ld) { return dop(b.p, ld); };
   // B operands as [n][k] rows: Kov, Jbig and CC are symmetric; Lov enters transposed.
   // Z and Z3 are square N x N outputs: 128 x 128 tiles, or 192 x 192 with
-  // TCI_PAIRS_TILE=192 (each operand panel fetched a third less often)
+  // TCI_PAIRS_TILE=192 (each operand panel fetched a third less often), fed by
+  // direct-load stages with TCI_PAIRS_TILE=192d
   const char* te = std::getenv("TCI_PAIRS_TILE");
   const int ztile = te && std::atoi(te) == 192 ? 192 : 128;
+  const bool zdma = ztile == 192 && te && te[3] == 'd';
   M.ringZx.clear();
   M.ringZx.split = true;
   M.ringZx.tile = ztile;
@@ -378,12 +380,14 @@ void dense_pairs_build_split(tci_plan& P, hipStream_t st) {
                {dseg(xop(M.CtRx, ldx), xop(M.Kx, ldx), K, 1.0), dseg(xop(M.CRx, ldx), xop(M.Jx, ldx), K, 1.0),
                 dseg(xop(M.Ytx, ldj), xop(M.LovTx, ldj), KJ, 1.0)},
                false, false, 0, Blocking(), 5);
+  M.ringZx.dma = zdma;
   M.ringZx.upload_tables(st);
   M.ringZ3x.clear();
   M.ringZ3x.split = true;
   M.ringZ3x.tile = ztile;
   M.ringZ3x.add(P.Z3b.as<double>(), N, iN, iN, {dseg(xop(M.Jx, ldx), xop(M.CCx, ldx), K, 1.0)}, false, false, 0,
                 Blocking(), 5);
+  M.ringZ3x.dma = zdma;
   M.ringZ3x.upload_tables(st);
   Blocking bw;
   bw.cdiv = nv;

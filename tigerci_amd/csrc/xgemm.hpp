@@ -123,69 +123,53 @@ struct XInterleave {
   static constexpr int BS = (TM == 192 && TN == 192) ? 32 : 16;
 };
 
+// One 32-wide k-step of a wave's blocks from an LDS stage (A hi, A lo, B hi,
+// B lo planes of PA / PB elements).
 template <int TM, int TN, int PREC, int NBM, int NBN>
-__device__ __forceinline__ void xgemm_body(const Task& t, const Seg* __restrict__ segs, int64_t J0, int seg0,
-                                           const float* __restrict__ scales, uint16_t* sm, int tid, int wm, int wn,
-                                           int li, int lg) {
-  typedef typename XT<PREC>::T T16;
+__device__ __forceinline__ void xgemm_step(const uint16_t* base, f4 (&acc)[NBM > 0 ? NBM : 1][NBN > 0 ? NBN : 1],
+                                           int wm, int wn, int li, int lg) {
   typedef typename XT<PREC>::V V;
   constexpr int BS = XInterleave<TM, TN>::BS;
-  // stage layout: A hi, A lo (TM rows each), B hi, B lo (TN rows each)
-  constexpr int PA = TM * XK, PB = TN * XK, STG = 2 * PA + 2 * PB;
-  f4 acc[NBM > 0 ? NBM : 1][NBN > 0 ? NBN : 1];
+  constexpr int PA = TM * XK, PB = TN * XK;
+  if (NBM > 0 && NBN > 0) {
+    V ah[NBM > 0 ? NBM : 1], al[NBM > 0 ? NBM : 1], bh[NBN > 0 ? NBN : 1], bl[NBN > 0 ? NBN : 1];
 #pragma unroll
-  for (int i = 0; i < NBM; ++i)
-#pragma unroll
-    for (int j = 0; j < NBN; ++j) acc[i][j] = f4{0.f, 0.f, 0.f, 0.f};
-  u32x4 va[TM / 64][2], vb[TN / 64][2];
-  int seg = seg0, kb = 0;
-  xstage<TM, TN>(segs[seg], t, J0, kb, tid, va, vb);
-  xstore<TM>(tid, va, sm, sm + PA);
-  xstore<TN>(tid, vb, sm + 2 * PA, sm + 2 * PA + PB);
-  bool more = xadvance(segs, t.seg1, seg, kb);
-  __syncthreads();
-  int buf = 0;
-  while (true) {
-    if (more) xstage<TM, TN>(segs[seg], t, J0, kb, tid, va, vb);
-    if (NBM > 0 && NBN > 0) {
-      const uint16_t* base = sm + buf * STG;
-      V ah[NBM > 0 ? NBM : 1], al[NBM > 0 ? NBM : 1], bh[NBN > 0 ? NBN : 1], bl[NBN > 0 ? NBN : 1];
-#pragma unroll
-      for (int i = 0; i < NBM; ++i) {
-        const int e = xsw(wm + i * BS + li, lg);
-        ah[i] = *reinterpret_cast<const V*>(base + e);
-        al[i] = *reinterpret_cast<const V*>(base + PA + e);
-      }
-#pragma unroll
-      for (int j = 0; j < NBN; ++j) {
-        const int e = xsw(wn + j * BS + li, lg);
-        bh[j] = *reinterpret_cast<const V*>(base + 2 * PA + e);
-        bl[j] = *reinterpret_cast<const V*>(base + 2 * PA + PB + e);
-      }
-      // B fragment as the MFMA's first operand: acc[i][j] holds the transposed
-      // 16 x 16 block, so each lane ends with 4 consecutive output columns.
-      // Small terms first; the three passes keep 16 independent chains.
-#pragma unroll
-      for (int i = 0; i < NBM; ++i)
-#pragma unroll
-        for (int j = 0; j < NBN; ++j) acc[i][j] = XT<PREC>::mfma(bh[j], al[i], acc[i][j]);
-#pragma unroll
-      for (int i = 0; i < NBM; ++i)
-#pragma unroll
-        for (int j = 0; j < NBN; ++j) acc[i][j] = XT<PREC>::mfma(bl[j], ah[i], acc[i][j]);
-#pragma unroll
-      for (int i = 0; i < NBM; ++i)
-#pragma unroll
-        for (int j = 0; j < NBN; ++j) acc[i][j] = XT<PREC>::mfma(bh[j], ah[i], acc[i][j]);
+    for (int i = 0; i < NBM; ++i) {
+      const int e = xsw(wm + i * BS + li, lg);
+      ah[i] = *reinterpret_cast<const V*>(base + e);
+      al[i] = *reinterpret_cast<const V*>(base + PA + e);
     }
-    if (!more) break;
-    uint16_t* nb = sm + (buf ^ 1) * STG;
-    xstore<TM>(tid, va, nb, nb + PA);
-    xstore<TN>(tid, vb, nb + 2 * PA, nb + 2 * PA + PB);
-    __syncthreads();
-    buf ^= 1;
-    more = xadvance(segs, t.seg1, seg, kb);
+#pragma unroll
+    for (int j = 0; j < NBN; ++j) {
+      const int e = xsw(wn + j * BS + li, lg);
+      bh[j] = *reinterpret_cast<const V*>(base + 2 * PA + e);
+      bl[j] = *reinterpret_cast<const V*>(base + 2 * PA + PB + e);
+    }
+    // B fragment as the MFMA's first operand: acc[i][j] holds the transposed
+    // 16 x 16 block, so each lane ends with 4 consecutive output columns.
+    // Small terms first; the three passes keep 16 independent chains.
+#pragma unroll
+    for (int i = 0; i < NBM; ++i)
+#pragma unroll
+      for (int j = 0; j < NBN; ++j) acc[i][j] = XT<PREC>::mfma(bh[j], al[i], acc[i][j]);
+#pragma unroll
+    for (int i = 0; i < NBM; ++i)
+#pragma unroll
+      for (int j = 0; j < NBN; ++j) acc[i][j] = XT<PREC>::mfma(bl[j], ah[i], acc[i][j]);
+#pragma unroll
+    for (int i = 0; i < NBM; ++i)
+#pragma unroll
+      for (int j = 0; j < NBN; ++j) acc[i][j] = XT<PREC>::mfma(bh[j], ah[i], acc[i][j]);
   }
+}
+
+// Epilogue: FP64 (optionally accumulated) or split-row output of a wave's blocks.
+template <int TM, int TN, int PREC, int NBM, int NBN>
+__device__ __forceinline__ void xgemm_store(const Task& t, const float* __restrict__ scales,
+                                            const f4 (&acc)[NBM > 0 ? NBM : 1][NBN > 0 ? NBN : 1], int wm, int wn,
+                                            int li, int lg) {
+  typedef typename XT<PREC>::T T16;
+  constexpr int BS = XInterleave<TM, TN>::BS;
   // lane (li, lg) of block (i, j) holds output row 16i + li, columns 16j + 4lg .. +3
   const float osc = t.oscale >= 0 ? scales[t.oscale] : 1.f;
   int64_t roff[NBM > 0 ? NBM : 1];
@@ -252,6 +236,60 @@ __device__ __forceinline__ void xgemm_body(const Task& t, const Seg* __restrict_
   }
 }
 
+template <int TM, int TN, int PREC, int NBM, int NBN>
+__device__ __forceinline__ void xgemm_body(const Task& t, const Seg* __restrict__ segs, int64_t J0, int seg0,
+                                           const float* __restrict__ scales, uint16_t* sm, int tid, int wm, int wn,
+                                           int li, int lg) {
+  // stage layout: A hi, A lo (TM rows each), B hi, B lo (TN rows each)
+  constexpr int PA = TM * XK, PB = TN * XK, STG = 2 * PA + 2 * PB;
+  f4 acc[NBM > 0 ? NBM : 1][NBN > 0 ? NBN : 1];
+#pragma unroll
+  for (int i = 0; i < NBM; ++i)
+#pragma unroll
+    for (int j = 0; j < NBN; ++j) acc[i][j] = f4{0.f, 0.f, 0.f, 0.f};
+  u32x4 va[TM / 64][2], vb[TN / 64][2];
+  int seg = seg0, kb = 0;
+  xstage<TM, TN>(segs[seg], t, J0, kb, tid, va, vb);
+  xstore<TM>(tid, va, sm, sm + PA);
+  xstore<TN>(tid, vb, sm + 2 * PA, sm + 2 * PA + PB);
+  bool more = xadvance(segs, t.seg1, seg, kb);
+  __syncthreads();
+  int buf = 0;
+  while (true) {
+    if (more) xstage<TM, TN>(segs[seg], t, J0, kb, tid, va, vb);
+    xgemm_step<TM, TN, PREC, NBM, NBN>(sm + buf * STG, acc, wm, wn, li, lg);
+    if (!more) break;
+    uint16_t* nb = sm + (buf ^ 1) * STG;
+    xstore<TM>(tid, va, nb, nb + PA);
+    xstore<TN>(tid, vb, nb + 2 * PA, nb + 2 * PA + PB);
+    __syncthreads();
+    buf ^= 1;
+    more = xadvance(segs, t.seg1, seg, kb);
+  }
+  xgemm_store<TM, TN, PREC, NBM, NBN>(t, scales, acc, wm, wn, li, lg);
+}
+
+// Output tile of a task without K-segments: zeros unless accumulating.
+template <int TM, int TN>
+__device__ __forceinline__ void xgemm_zero_tile(const Task& t, int tid) {
+  if (!t.accumulate)
+    for (int e = tid; e < TM * TN; e += 256) {
+      const int row = t.m0 + e / TN, col = t.n0 + e % TN;
+      if (row < t.M && col < t.N) {
+        const int64_t ro = t.rdiv ? (int64_t)(row / t.rdiv) * t.rstride + (int64_t)(row % t.rdiv) * t.ldc
+                                  : (int64_t)row * t.ldc;
+        const int cb = t.cdiv ? col / t.cdiv : 0, cc = col - cb * t.cdiv;
+        if (t.out_split) {
+          uint16_t* p = reinterpret_cast<uint16_t*>(t.C) + ro + (int64_t)cb * t.cstride + xidx(cc);
+          p[0] = 0;
+          p[8] = 0;
+        } else {
+          t.C[ro + (int64_t)cb * t.cstride + cc] = 0.0;
+        }
+      }
+    }
+}
+
 template <int TM, int TN, int PREC>
 __global__ __launch_bounds__(256, TM == 64 ? 4 : (TM + TN > 320 ? 1 : 2)) void xgemm_kernel(const Task* __restrict__ tasks,
                                                                       const Seg* __restrict__ segs, int64_t J0,
@@ -273,22 +311,7 @@ __global__ __launch_bounds__(256, TM == 64 ? 4 : (TM + TN > 320 ? 1 : 2)) void x
   int seg0 = t.seg0;
   while (seg0 < t.seg1 && segs[seg0].K <= 0) ++seg0;
   if (seg0 >= t.seg1) {
-    if (!t.accumulate)
-      for (int e = tid; e < TM * TN; e += 256) {
-        const int row = t.m0 + e / TN, col = t.n0 + e % TN;
-        if (row < t.M && col < t.N) {
-          const int64_t ro = t.rdiv ? (int64_t)(row / t.rdiv) * t.rstride + (int64_t)(row % t.rdiv) * t.ldc
-                                    : (int64_t)row * t.ldc;
-          const int cb = t.cdiv ? col / t.cdiv : 0, cc = col - cb * t.cdiv;
-          if (t.out_split) {
-            uint16_t* p = reinterpret_cast<uint16_t*>(t.C) + ro + (int64_t)cb * t.cstride + xidx(cc);
-            p[0] = 0;
-            p[8] = 0;
-          } else {
-            t.C[ro + (int64_t)cb * t.cstride + cc] = 0.0;
-          }
-        }
-      }
+    xgemm_zero_tile<TM, TN>(t, tid);
     return;
   }
 #define TCI_XB(M_, N_) \
@@ -319,5 +342,8 @@ void launch_xgemm192_bf16(const Task* tasks, const Seg* segs, int64_t ntasks, in
                           hipStream_t st, int remap);
 void launch_xgemm192_f16(const Task* tasks, const Seg* segs, int64_t ntasks, int64_t J0, const float* scales,
                          hipStream_t st, int remap);
+// 192 x 192 tiles, three LDS stages filled by direct loads (xgemm192d.hip)
+void launch_xgemm192d(int precision, const Task* tasks, const Seg* segs, int64_t ntasks, int64_t J0,
+                      const float* scales, hipStream_t st, int remap);
 
 }  // namespace tci
