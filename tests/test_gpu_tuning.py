@@ -217,13 +217,20 @@ def test_fused192_split_ladder(precision, dr, n_chol):
     assert np.array_equal(b, b2)
 
 
-@pytest.mark.parametrize("precision", ["bf16x3", "fp16x3"])
-def test_split_direct_load_stages(precision):
-    """Phase 2 of the domains of 129-192 on 192 x 192 tiles fed by direct
-    global-to-LDS loads (three stages) gives the register-staged result up to
-    FP32 summation order and the oracle to the split precision; K-splits of
-    one and of several Cholesky vectors, domains below and at 192."""
-    _say("case: split 192 tiles with direct-load stages", precision)
+@pytest.mark.parametrize("precision,env", [
+    ("bf16x3", {"TCI_SPLIT_DMA": "1"}),
+    ("fp16x3", {"TCI_SPLIT_DMA": "1"}),
+    ("bf16x3", {"TCI_SPLIT_DMA1": "2"}),
+    ("bf16x3", {"TCI_SPLIT_DMA1": "3"}),
+    ("fp16x3", {"TCI_SPLIT_DMA": "1", "TCI_SPLIT_DMA1": "2"}),
+])
+def test_split_direct_load_stages(precision, env):
+    """The domains of 129-192 with their phase-2 192 x 192 tiles
+    (TCI_SPLIT_DMA) or phase-1 128 x 192 tiles (TCI_SPLIT_DMA1: 2 or 3 stages)
+    fed by direct global-to-LDS loads give the register-staged result up to
+    FP32 summation order and the oracle to the split precision; domains below
+    and at 192."""
+    _say("case: split tiles with direct-load stages", precision, env)
     prob = make_problem(4, 200, 30, 63, domain_range=(129, 192), weak_below=0)
     ints = Integrals(prob)
     x = random_ci_vector(prob, 15)
@@ -232,12 +239,13 @@ def test_split_direct_load_stages(precision):
     os.environ["TCI_SPLIT_P2_192"] = "1"
     try:
         a = _sigma(tci.Plan.from_problem(prob), x, precision=precision)
-        os.environ["TCI_SPLIT_DMA"] = "1"
+        os.environ.update(env)
         plan = tci.Plan.from_problem(prob)
         b = _sigma(plan, x, precision=precision)
         b2 = _sigma(plan, x, precision=precision)
     finally:
-        os.environ.pop("TCI_SPLIT_DMA", None)
+        for k in env:
+            os.environ.pop(k, None)
         os.environ.pop("TCI_SPLIT_P2_192", None)
         del os.environ["TCI_AUTOTUNE"]
     tol = 5e-5 if precision == "bf16x3" else 2e-6

@@ -166,7 +166,7 @@ struct GemmProgram {
   bool dense = false;  // contiguous row-major operands only: dense_gemm.hip kernel (tile 64 or 128)
   bool split = false;  // split-row operands, C = sum A B^T: dense_split.hip kernel (tile 64 or 128)
   bool scaled = false; // some task has an output scale (set by upload_tables)
-  bool dma = false;    // split 192 x 192 programs: three LDS stages filled by direct loads
+  int dma = 0;         // split programs: LDS stages filled by direct loads (0: register-staged)
   int remap = -1;      // workgroup -> task order: 0 plain, 1 XCD-contiguous, -1 default
   std::vector<Task> tasks;
   std::vector<Seg> segs;

@@ -32,9 +32,9 @@ void GemmProgram::launch_split(int precision, hipStream_t st, int64_t J0, const 
   TCI_REQUIRE(!scaled || scales, "scaled split tasks need a scales array");
   const int rm = remap < 0 || xcd_remap_forced() ? xcd_remap_default() : remap;
   if (dma) {
-    TCI_REQUIRE(tile == 192 && (tile_n == 0 || tile_n == 192), "direct-load stages need 192 x 192 tiles");
     TCI_REQUIRE(precision == TCI_SIGMA_BF16X3 || precision == TCI_SIGMA_FP16X3, "split GEMM needs a split precision");
-    launch_xgemm192d(precision, d_tasks.as<Task>(), d_segs.as<Seg>(), (int64_t)tasks.size(), J0, scales, st, rm);
+    launch_xgemmd(precision, tile, tile_n > 0 ? tile_n : tile, dma, d_tasks.as<Task>(), d_segs.as<Seg>(),
+                  (int64_t)tasks.size(), J0, scales, st, rm);
     return;
   }
   launch_xgemm(precision, tile, tile_n > 0 ? tile_n : tile, d_tasks.as<Task>(), d_segs.as<Seg>(),

@@ -658,7 +658,12 @@ void mixed_prepare(tci_plan& P, int precision, hipStream_t st) {
   // fed by direct global-to-LDS loads (three stages)
   auto set_dma = [&](bool on) {
     for (auto& ph : M.phase)
-      if (ph.p2x.size() > 3) ph.p2x[3].dma = on && ph.p2x[3].tile == 192;
+      if (ph.p2x.size() > 3) ph.p2x[3].dma = on && ph.p2x[3].tile == 192 ? 3 : 0;
+  };
+  // phase 1 of the domains of 129-192 (128 x 192 tiles): direct-load stages (0: off, 2 or 3)
+  auto set_dma1 = [&](int nst) {
+    for (auto& ph : M.phase)
+      if (ph.p1x.size() > 2 && ph.p1x[2].tile == 128 && ph.p1x[2].tile_n == 192) ph.p1x[2].dma = nst;
   };
   if (M.per_pair || !autotune_enabled()) {
     // untuned: 128-row phase-2 tiles unless TCI_SPLIT_P2_192=1
@@ -670,6 +675,8 @@ void mixed_prepare(tci_plan& P, int precision, hipStream_t st) {
           f3 && f3[0] == '1' ? 1024 : 0);
     const char* fd = std::getenv("TCI_SPLIT_DMA");
     set_dma(fd && fd[0] == '1');
+    const char* fd1 = std::getenv("TCI_SPLIT_DMA1");
+    set_dma1(fd1 && (fd1[0] == '2' || fd1[0] == '3') ? fd1[0] - '0' : 0);
   } else if (M.tuned_target > 0) {
     // tuned before for the other split precision: same schedule, so one plan
     // gives the same sigma for a precision every time it is used
@@ -679,6 +686,7 @@ void mixed_prepare(tci_plan& P, int precision, hipStream_t st) {
     M.fused2_remap = M.tuned_fused2_order;
     M.fused3_remap = M.tuned_fused3_order;
     set_dma(M.tuned_dma);
+    set_dma1(M.tuned_dma1);
     for (auto& ph : M.phase) {
       std::vector<GemmProgram*> g = progs(ph);
       for (size_t k = 0; k < g.size() && k < M.tuned_orders.size(); ++k) g[k]->remap = M.tuned_orders[k];
@@ -718,9 +726,21 @@ void mixed_prepare(tci_plan& P, int precision, hipStream_t st) {
       const double md = time_ms(st, 2, [&] { mixed_ladder_chunk(P, precision, M.chunks[0], st); });
       log += "; direct-load stages:" + fmt_ms(md);
       bdma = md < best * 0.98;
+      if (bdma) best = md;
       set_dma(bdma);
       log += bdma ? " -> on" : " -> off";
     }
+    int bdma1 = 0;  // then direct-load stages for the 128 x 192 phase-1 tiles
+    for (int nst : {2, 3}) {
+      set_dma1(nst);
+      const double m1 = time_ms(st, 2, [&] { mixed_ladder_chunk(P, precision, M.chunks[0], st); });
+      log += "; phase-1 direct loads " + std::to_string(nst) + ":" + fmt_ms(m1);
+      if (m1 < best * 0.98) {
+        best = m1;
+        bdma1 = nst;
+      }
+    }
+    set_dma1(bdma1);
     // then the fused kernel for the walks of 129-192 (whole ladder timed: it
     // runs over every Cholesky vector in one launch)
     bool bf = false;
@@ -736,6 +756,7 @@ void mixed_prepare(tci_plan& P, int precision, hipStream_t st) {
         for (int fo : {1, 0}) {
           build(bt, bb, true, ft, 0, 0);
           set_dma(bdma);
+          set_dma1(bdma1);
           M.fused_remap = fo;
           TCI_HIP(hipStreamSynchronize(st));
           const double tf = time_ms(st, 2, [&] { mixed_ladder_run(P, precision, st); });
@@ -750,6 +771,7 @@ void mixed_prepare(tci_plan& P, int precision, hipStream_t st) {
       M.tuned_fused_order = bfo;
       build(bt, bb, bf, bft, 0, 0);
       set_dma(bdma);
+      set_dma1(bdma1);
       M.fused_remap = bfo;
       TCI_HIP(hipStreamSynchronize(st));
       log += bf ? " -> fused" + std::to_string(bft) + (bfo ? "x" : "") : " -> two-phase";
@@ -766,6 +788,7 @@ void mixed_prepare(tci_plan& P, int precision, hipStream_t st) {
         for (int fo : {0, 1}) {
           build(bt, bb, bf, bft, ft, 0);
           set_dma(bdma);
+          set_dma1(bdma1);
           M.fused2_remap = fo;
           TCI_HIP(hipStreamSynchronize(st));
           const double tf = time_ms(st, 2, [&] { mixed_ladder_run(P, precision, st); });
@@ -778,6 +801,7 @@ void mixed_prepare(tci_plan& P, int precision, hipStream_t st) {
         }
       build(bt, bb, bf, bft, bft2, 0);
       set_dma(bdma);
+      set_dma1(bdma1);
       M.fused2_remap = bfo2;
       TCI_HIP(hipStreamSynchronize(st));
       log += bft2 ? " -> fused128-" + std::to_string(bft2) + (bfo2 ? "x" : "") : " -> two-phase";
@@ -804,6 +828,7 @@ void mixed_prepare(tci_plan& P, int precision, hipStream_t st) {
     M.tuned_target = bt;
     M.tuned_big = bb;
     M.tuned_dma = bdma;
+    M.tuned_dma1 = bdma1;
     M.tuned_fused = bf;
     M.tuned_fused_target = bft;
     P.tuning += log + "\n";

@@ -372,7 +372,7 @@ void dense_pairs_build_split(tci_plan& P, hipStream_t st) {
   // direct-load stages with TCI_PAIRS_TILE=192d
   const char* te = std::getenv("TCI_PAIRS_TILE");
   const int ztile = te && std::atoi(te) == 192 ? 192 : 128;
-  const bool zdma = ztile == 192 && te && te[3] == 'd';
+  const int zdma = ztile == 192 && te && te[3] == 'd' ? 3 : 0;
   M.ringZx.clear();
   M.ringZx.split = true;
   M.ringZx.tile = ztile;

@@ -125,6 +125,7 @@ struct tci_plan {
     int64_t tuned_target = -1;      // autotuned K-split fill target (kept across precision switches)
     bool tuned_big = false;         // autotuned: phase 2 of domains 129-192 on 192 x 192 tiles
     bool tuned_dma = false;         // autotuned: those 192 x 192 tiles fed by direct global-to-LDS loads
+    int tuned_dma1 = 0;             // autotuned: stages of direct loads for the 128 x 192 phase-1 tiles (0: off)
     std::vector<int> tuned_orders;  // autotuned workgroup order per program role
   } mx;
   tci::PairTab tab{};

@@ -342,8 +342,9 @@ void launch_xgemm192_bf16(const Task* tasks, const Seg* segs, int64_t ntasks, in
                           hipStream_t st, int remap);
 void launch_xgemm192_f16(const Task* tasks, const Seg* segs, int64_t ntasks, int64_t J0, const float* scales,
                          hipStream_t st, int remap);
-// 192 x 192 tiles, three LDS stages filled by direct loads (xgemm192d.hip)
-void launch_xgemm192d(int precision, const Task* tasks, const Seg* segs, int64_t ntasks, int64_t J0,
-                      const float* scales, hipStream_t st, int remap);
+// LDS stages filled by direct loads (xgemm192d.hip): 192 x 192 with 3 stages,
+// 128 x 192 with 2 or 3
+void launch_xgemmd(int precision, int tm, int tn, int nst, const Task* tasks, const Seg* segs, int64_t ntasks,
+                   int64_t J0, const float* scales, hipStream_t st, int remap);
 
 }  // namespace tci
