@@ -143,18 +143,20 @@ def test_fused_split_ladder(precision):
     assert np.array_equal(b, b2)
 
 
-@pytest.mark.parametrize("precision,dr,n_chol", [
-    ("bf16x3", (40, 140), 24),
-    ("fp16x3", (40, 140), 24),
-    ("bf16x3", (65, 65), 7),
-    ("bf16x3", (128, 128), 40),
+@pytest.mark.parametrize("precision,dr,n_chol,waves", [
+    ("bf16x3", (40, 140), 24, "1"),
+    ("fp16x3", (40, 140), 24, "1"),
+    ("bf16x3", (65, 65), 7, "1"),
+    ("bf16x3", (128, 128), 40, "1"),
+    ("bf16x3", (40, 140), 24, "8"),
+    ("fp16x3", (65, 128), 9, "8"),
 ])
-def test_fused128_split_ladder(precision, dr, n_chol):
+def test_fused128_split_ladder(precision, dr, n_chol, waves):
     """The whole-walk fused split ladder for domains of 65-128 (T' kept in
     registers) reproduces the oracle and the two-phase ladder to the split
     precision, next to two-phase walks of other sizes; J ranges of one and of
     several Cholesky vectors."""
-    _say("case: fused128 split ladder", precision, dr, n_chol)
+    _say("case: fused128 split ladder", precision, dr, n_chol, "eight waves" if waves == "8" else "four waves")
     prob = make_problem(5, 200, n_chol, 61, domain_range=dr, weak_below=0)
     ints = Integrals(prob)
     x = random_ci_vector(prob, 13)
@@ -162,7 +164,7 @@ def test_fused128_split_ladder(precision, dr, n_chol):
     os.environ["TCI_AUTOTUNE"] = "0"
     try:
         a = _sigma(tci.Plan.from_problem(prob), x, precision=precision)
-        os.environ["TCI_SPLIT_FUSED128"] = "1"
+        os.environ["TCI_SPLIT_FUSED128"] = waves
         fused = tci.Plan.from_problem(prob)
         b = _sigma(fused, x, precision=precision)
         b2 = _sigma(fused, x, precision=precision)

@@ -26,7 +26,7 @@ void launch_xladder_fused(int precision, const FusedTask* tasks, int64_t ntasks,
 // for Cholesky vectors [j0, j1); R[n][a] += s4 sum_J sum_c L_J[n][c]
 // (s3 L_J C^T)[a][c] into the task's zero-filled K-split slice.
 void launch_xladder_fused128(int precision, const FusedTask* tasks, int64_t ntasks, const float* scales,
-                             hipStream_t st, int remap);
+                             hipStream_t st, int remap, int waves = 4);
 
 // Domains of 129..192 (ladder_fused192.hip): one task is a whole walk (a0 = 0)
 // for Cholesky vectors [j0, j1); C_Q in LDS, L_J read from global memory.

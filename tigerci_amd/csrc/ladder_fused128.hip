@@ -17,8 +17,10 @@
 // L_J operand in the same order (two 8-byte LDS reads per plane).  The k
 // order inside one MFMA does not change the sum.
 //
-// Four waves; wave w owns the 16-column blocks w and w + 4 of a (T' and R),
-// so it keeps T'[a][0..127] of its 32 a's and R[0..127][a] in registers.
+// Four waves, wave w owning the 16-column blocks w and w + 4 of a (T' and R),
+// or eight waves owning one block each (two waves per SIMD hide each other's
+// LDS and barrier waits, for twice the LDS reads); a wave keeps T'[a][0..127]
+// and R[0..127][a] of its blocks in registers.
 // The body is specialised on the domain's block count (5..8) and on the
 // wave's block count (1 or 2), so every loop is unrolled and branch-free; a
 // domain of d costs ceil(d/16) blocks per side and ceil(d/32) k-steps.  Rows
@@ -43,45 +45,55 @@ constexpr int GRS = 2 * GW + 8;       // LDS row stride (16-bit elements): hi/lo
 // pairs e = tid + 256 q, row e / GCH, chunk e % GCH (hi 16 B, lo 16 B); rows
 // >= d and chunks starting at or beyond d come from the zero page.  Rows from
 // 16 NB on are never read.
-template <int NB>
+// (NT threads: passes of NT chunk pairs; the 256 NB chunk pairs of rows
+// 0..16 NB - 1 take (256 NB + NT - 1) / NT of them.)
+template <int NB, int NT>
 __device__ __forceinline__ void gload(const uint16_t* base, int64_t rs, int d, int tid, u32x4 (*v)[2]) {
+  constexpr int Q = (256 * NB + NT - 1) / NT;
 #pragma unroll
-  for (int q = 0; q < NB; ++q) {
-    const int e = tid + 256 * q, r = e / GCH, ch = e % GCH;
-    const u32x4* p =
-        (r < d && 8 * ch < d) ? reinterpret_cast<const u32x4*>(base + (int64_t)r * rs + 16 * ch) : xzero;
+  for (int q = 0; q < Q; ++q) {
+    const int e = tid + NT * q, r = e / GCH, ch = e % GCH;
+    const u32x4* p = (e < 256 * NB && r < d && 8 * ch < d)
+                         ? reinterpret_cast<const u32x4*>(base + (int64_t)r * rs + 16 * ch)
+                         : xzero;
     v[q][0] = gld(p);
     v[q][1] = gld(p + 1);
   }
 }
 
-template <int NB>
+template <int NB, int NT>
 __device__ __forceinline__ void gstore(int tid, const u32x4 (*v)[2], uint16_t* s) {
+  constexpr int Q = (256 * NB + NT - 1) / NT;
 #pragma unroll
-  for (int q = 0; q < NB; ++q) {
-    const int e = tid + 256 * q, r = e / GCH, ch = e % GCH;
-    u32x4* p = reinterpret_cast<u32x4*>(s + r * GRS + 16 * ch);
-    p[0] = v[q][0];
-    p[1] = v[q][1];
+  for (int q = 0; q < Q; ++q) {
+    const int e = tid + NT * q, r = e / GCH, ch = e % GCH;
+    if (e < 256 * NB) {
+      u32x4* p = reinterpret_cast<u32x4*>(s + r * GRS + 16 * ch);
+      p[0] = v[q][0];
+      p[1] = v[q][1];
+    }
   }
 }
 
-// One wave's whole task for a domain of NB 16-blocks per side, owning the a
-// blocks w (and w + 4 when NAB == 2); every loop is unrolled and branch-free.
-template <int PREC, int NB, int NAB>
+// One wave's whole task for a domain of NB 16-blocks per side, in a
+// workgroup of NW waves, owning the a blocks w + NW ab for ab < NAB (NAB = 0:
+// the wave only helps stage L_J); every loop is unrolled and branch-free.
+template <int PREC, int NB, int NAB, int NW>
 __device__ __forceinline__ void fused128_body(const FusedTask& t, int d, int j0, int j1, uint16_t* cs, uint16_t* ls,
                                               int tid, int w, int li, int lg, float s_t, float s_r) {
   typedef typename XT<PREC>::T T16;
   typedef typename XT<PREC>::V V;
   typedef unsigned u32x2 __attribute__((ext_vector_type(2)));
   constexpr int NKS = (NB + 1) / 2;  // 32-wide k-steps
-  u32x4 pf[NB][2];
-  gload<NB>(t.C, t.ldx, d, tid, pf);
-  gstore<NB>(tid, pf, cs);
-  gload<NB>(t.L + (int64_t)j0 * t.jstep, t.ldx, d, tid, pf);
-  gstore<NB>(tid, pf, ls);
+  constexpr int NT = 64 * NW, QS = (256 * NB + NT - 1) / NT;
+  constexpr int NA = NAB > 0 ? NAB : 1;
+  u32x4 pf[QS][2];
+  gload<NB, NT>(t.C, t.ldx, d, tid, pf);
+  gstore<NB, NT>(tid, pf, cs);
+  gload<NB, NT>(t.L + (int64_t)j0 * t.jstep, t.ldx, d, tid, pf);
+  gstore<NB, NT>(tid, pf, ls);
   __syncthreads();
-  f4 acc2[NAB][NB];
+  f4 acc2[NA][NB];
 #pragma unroll
   for (int ab = 0; ab < NAB; ++ab)
 #pragma unroll
@@ -89,7 +101,8 @@ __device__ __forceinline__ void fused128_body(const FusedTask& t, int d, int j0,
   const int c0 = 16 * (lg >> 1) + 4 * (lg & 1);  // element of c = 32s + 4lg in its row
   for (int J = j0; J < j1; ++J) {
     const bool more = J + 1 < j1;
-    if (more) gload<NB>(t.L + (int64_t)(J + 1) * t.jstep, t.ldx, d, tid, pf);
+    if (more) gload<NB, NT>(t.L + (int64_t)(J + 1) * t.jstep, t.ldx, d, tid, pf);
+    if constexpr (NAB > 0) {
     // ---- phase 1: acc1[cb][ab] lane holds T'[16(w + 4ab) + li][16cb + 4lg + r].
     // Steps i = (k-step s, block pair cp); the fragments of step i + 1 are read
     // from LDS before the MFMAs of step i.
@@ -113,7 +126,7 @@ __device__ __forceinline__ void fused128_body(const FusedTask& t, int d, int j0,
       const int ko = 16 * (4 * sk + lg);
 #pragma unroll
       for (int ab = 0; ab < NAB; ++ab) {
-        const uint16_t* p = ls + (16 * (w + 4 * ab) + li) * GRS + ko;
+        const uint16_t* p = ls + (16 * (w + NW * ab) + li) * GRS + ko;
         h[ab] = *reinterpret_cast<const V*>(p);
         l[ab] = *reinterpret_cast<const V*>(p + 8);
       }
@@ -227,16 +240,17 @@ __device__ __forceinline__ void fused128_body(const FusedTask& t, int d, int j0,
         }
       }
     }
+    }
     if (more) {
       __syncthreads();  // every wave is done with L_J
-      gstore<NB>(tid, pf, ls);
+      gstore<NB, NT>(tid, pf, ls);
       __syncthreads();
     }
   }
-  // ---- R: lane holds R[16 nb2 + li][16(w + 4ab) + 4lg .. +3]
+  // ---- R: lane holds R[16 nb2 + li][16(w + NW ab) + 4lg .. +3]
 #pragma unroll
   for (int ab = 0; ab < NAB; ++ab) {
-    const int a = 16 * (w + 4 * ab) + 4 * lg;
+    const int a = 16 * (w + NW * ab) + 4 * lg;
     if (a >= d) continue;
 #pragma unroll
     for (int nb2 = 0; nb2 < NB; ++nb2) {
@@ -261,9 +275,9 @@ __device__ __forceinline__ void fused128_body(const FusedTask& t, int d, int j0,
   }
 }
 
-template <int PREC>
-__global__ __launch_bounds__(256, 1) void xladder_fused128_kernel(const FusedTask* __restrict__ tasks,
-                                                                  const float* __restrict__ scales, int remap) {
+template <int PREC, int NW>
+__global__ __launch_bounds__(64 * NW, 1) void xladder_fused128_kernel(const FusedTask* __restrict__ tasks,
+                                                                      const float* __restrict__ scales, int remap) {
   const FusedTask t = tasks[remap ? xcd_task(blockIdx.x, gridDim.x) : blockIdx.x];
   __shared__ __attribute__((aligned(16))) uint16_t cs[GW * GRS];
   __shared__ __attribute__((aligned(16))) uint16_t ls[GW * GRS];
@@ -272,17 +286,20 @@ __global__ __launch_bounds__(256, 1) void xladder_fused128_kernel(const FusedTas
   const int li = lane & 15, lg = lane >> 4;
   const int d = __builtin_amdgcn_readfirstlane(t.d);
   const int j0 = __builtin_amdgcn_readfirstlane(t.j0), j1 = __builtin_amdgcn_readfirstlane(t.j1);
-  const int nb = (d + 15) >> 4;  // 5..8 for domains of 65..128: every wave owns block w
+  const int nb = (d + 15) >> 4;  // 5..8 for domains of 65..128
   const float s_t = scales ? scales[3] : 1.f, s_r = scales ? scales[4] : 1.f;
-  // waves whose second block w + 4 lies in the domain run the two-block body;
-  // both bodies meet the same barriers (two per Cholesky vector)
+  // a wave runs the body for its count of blocks w + NW ab inside the domain
+  // (four waves: 1 or 2; eight waves: 0 or 1); all bodies meet the same
+  // barriers (one after staging, two per Cholesky vector after the first)
+  const int nab = (w < nb ? 1 : 0) + (w + NW < nb ? 1 : 0);
 #define TCI_F128(NB_, NAB_) \
-  case NB_ * 2 + NAB_ - 1:  \
-    fused128_body<PREC, NB_, NAB_>(t, d, j0, j1, cs, ls, tid, w, li, lg, s_t, s_r); \
+  case NB_ * 4 + NAB_:      \
+    if constexpr (NAB_ * NW <= 8)                                                     \
+      fused128_body<PREC, NB_, NAB_, NW>(t, d, j0, j1, cs, ls, tid, w, li, lg, s_t, s_r); \
     break;
-  switch (nb * 2 + (w + 4 < nb ? 1 : 0)) {
-    TCI_F128(5, 1) TCI_F128(5, 2) TCI_F128(6, 1) TCI_F128(6, 2)
-    TCI_F128(7, 1) TCI_F128(7, 2) TCI_F128(8, 1) TCI_F128(8, 2)
+  switch (nb * 4 + nab) {
+    TCI_F128(5, 0) TCI_F128(5, 1) TCI_F128(5, 2) TCI_F128(6, 0) TCI_F128(6, 1) TCI_F128(6, 2)
+    TCI_F128(7, 0) TCI_F128(7, 1) TCI_F128(7, 2) TCI_F128(8, 0) TCI_F128(8, 1) TCI_F128(8, 2)
     default: break;
   }
 #undef TCI_F128
@@ -291,17 +308,20 @@ __global__ __launch_bounds__(256, 1) void xladder_fused128_kernel(const FusedTas
 }  // namespace
 
 void launch_xladder_fused128(int precision, const FusedTask* tasks, int64_t ntasks, const float* scales,
-                             hipStream_t st, int remap) {
+                             hipStream_t st, int remap, int waves) {
   if (ntasks <= 0) return;
   TCI_REQUIRE(ntasks < ((int64_t)1 << 31), "too many fused ladder tasks");
-  if (precision == TCI_SIGMA_BF16X3)
-    hipLaunchKernelGGL(xladder_fused128_kernel<TCI_SIGMA_BF16X3>, dim3((unsigned)ntasks), dim3(256), 0, st, tasks,
-                       scales, remap);
-  else if (precision == TCI_SIGMA_FP16X3)
-    hipLaunchKernelGGL(xladder_fused128_kernel<TCI_SIGMA_FP16X3>, dim3((unsigned)ntasks), dim3(256), 0, st, tasks,
-                       scales, remap);
+  TCI_REQUIRE(waves == 4 || waves == 8, "fused ladder workgroups have 4 or 8 waves");
+  TCI_REQUIRE(precision == TCI_SIGMA_BF16X3 || precision == TCI_SIGMA_FP16X3, "fused ladder needs a split precision");
+  const dim3 g((unsigned)ntasks);
+  if (precision == TCI_SIGMA_BF16X3 && waves == 4)
+    hipLaunchKernelGGL((xladder_fused128_kernel<TCI_SIGMA_BF16X3, 4>), g, dim3(256), 0, st, tasks, scales, remap);
+  else if (precision == TCI_SIGMA_BF16X3)
+    hipLaunchKernelGGL((xladder_fused128_kernel<TCI_SIGMA_BF16X3, 8>), g, dim3(512), 0, st, tasks, scales, remap);
+  else if (waves == 4)
+    hipLaunchKernelGGL((xladder_fused128_kernel<TCI_SIGMA_FP16X3, 4>), g, dim3(256), 0, st, tasks, scales, remap);
   else
-    throw Error(TCI_ERR_ARG, "fused ladder needs a split precision");
+    hipLaunchKernelGGL((xladder_fused128_kernel<TCI_SIGMA_FP16X3, 8>), g, dim3(512), 0, st, tasks, scales, remap);
   TCI_HIP(hipGetLastError());
 }
 

@@ -671,8 +671,9 @@ void mixed_prepare(tci_plan& P, int precision, hipStream_t st) {
     const char* f = std::getenv("TCI_SPLIT_FUSED");
     const char* f2 = std::getenv("TCI_SPLIT_FUSED128");
     const char* f3 = std::getenv("TCI_SPLIT_FUSED192");
-    build(def_target, e && e[0] == '1', f && f[0] == '1', 1024, f2 && f2[0] == '1' ? 1024 : 0,
+    build(def_target, e && e[0] == '1', f && f[0] == '1', 1024, f2 && (f2[0] == '1' || f2[0] == '8') ? 1024 : 0,
           f3 && f3[0] == '1' ? 1024 : 0);
+    M.fused2_waves = f2 && f2[0] == '8' ? 8 : 4;  // TCI_SPLIT_FUSED128=8: eight-wave workgroups
     const char* fd = std::getenv("TCI_SPLIT_DMA");
     set_dma(fd && fd[0] == '1');
     const char* fd1 = std::getenv("TCI_SPLIT_DMA1");
@@ -684,6 +685,7 @@ void mixed_prepare(tci_plan& P, int precision, hipStream_t st) {
           M.tuned_fused3_target);
     M.fused_remap = M.tuned_fused_order;
     M.fused2_remap = M.tuned_fused2_order;
+    M.fused2_waves = M.tuned_fused2_waves;
     M.fused3_remap = M.tuned_fused3_order;
     set_dma(M.tuned_dma);
     set_dma1(M.tuned_dma1);
@@ -778,36 +780,43 @@ void mixed_prepare(tci_plan& P, int precision, hipStream_t st) {
     }
     // then the whole-walk fused kernel for the walks of 65-128
     int64_t bft2 = 0;
-    int bfo2 = 0;
+    int bfo2 = 0, bnw2 = 4;
     bool any_fusable2 = false;
     for (int Q = 0; Q < n_pairs && !M.merged; ++Q) any_fusable2 = any_fusable2 || fusable2(P.h_dom_off[Q + 1] - P.h_dom_off[Q]);
     if (any_fusable2) {
       double tb = time_ms(st, 2, [&] { mixed_ladder_run(P, precision, st); });
       log += "; 65-128 two-phase:" + fmt_ms(tb);
+      // eight-wave workgroups (TCI_SPLIT_FUSED128=8) measured 2% slower at
+      // 34/192/1200 and are not a candidate
       for (int64_t ft : {512, 1024, 2048})
-        for (int fo : {0, 1}) {
-          build(bt, bb, bf, bft, ft, 0);
-          set_dma(bdma);
-          set_dma1(bdma1);
-          M.fused2_remap = fo;
-          TCI_HIP(hipStreamSynchronize(st));
-          const double tf = time_ms(st, 2, [&] { mixed_ladder_run(P, precision, st); });
-          log += " fused128-" + std::to_string(ft) + (fo ? "x" : "") + ":" + fmt_ms(tf);
-          if (tf < tb * 0.98) {
-            tb = tf;
-            bft2 = ft;
-            bfo2 = fo;
+        for (int fo : {0, 1})
+          for (int nw : {4}) {
+            build(bt, bb, bf, bft, ft, 0);
+            set_dma(bdma);
+            set_dma1(bdma1);
+            M.fused2_remap = fo;
+            M.fused2_waves = nw;
+            TCI_HIP(hipStreamSynchronize(st));
+            const double tf = time_ms(st, 2, [&] { mixed_ladder_run(P, precision, st); });
+            log += " fused128-" + std::to_string(ft) + (fo ? "x" : "") + (nw == 8 ? "w8" : "") + ":" + fmt_ms(tf);
+            if (tf < tb * 0.98) {
+              tb = tf;
+              bft2 = ft;
+              bfo2 = fo;
+              bnw2 = nw;
+            }
           }
-        }
       build(bt, bb, bf, bft, bft2, 0);
       set_dma(bdma);
       set_dma1(bdma1);
       M.fused2_remap = bfo2;
+      M.fused2_waves = bnw2;
       TCI_HIP(hipStreamSynchronize(st));
-      log += bft2 ? " -> fused128-" + std::to_string(bft2) + (bfo2 ? "x" : "") : " -> two-phase";
+      log += bft2 ? " -> fused128-" + std::to_string(bft2) + (bfo2 ? "x" : "") + (bnw2 == 8 ? "w8" : "") : " -> two-phase";
     }
     M.tuned_fused2_target = bft2;
     M.tuned_fused2_order = bfo2;
+    M.tuned_fused2_waves = bnw2;
     // the whole-walk kernel with C in LDS for the walks of 129-192
     // (TCI_SPLIT_FUSED192) is not a candidate: at 34/192/1200 it measured 15%
     // slower than the two phases for those walks
@@ -867,7 +876,7 @@ void mixed_ladder_run(tci_plan& P, int precision, hipStream_t st) {
                          M.fused_remap < 0 || xcd_remap_forced() ? xcd_remap_default() : M.fused_remap);
   if (M.fused2)
     launch_xladder_fused128(precision, M.fused2_d.as<FusedTask>(), (int64_t)M.fused2_h.size(), sc, st,
-                            xcd_remap_forced() ? xcd_remap_default() : M.fused2_remap);
+                            xcd_remap_forced() ? xcd_remap_default() : M.fused2_remap, M.fused2_waves);
   if (M.fused3)
     launch_xladder_fused192(precision, M.fused3_d.as<FusedTask>(), (int64_t)M.fused3_h.size(), sc, st,
                             xcd_remap_forced() ? xcd_remap_default() : M.fused3_remap);

@@ -103,6 +103,8 @@ struct tci_plan {
     std::vector<tci::FusedTask> fused2_h;
     tci::DevBuf fused2_d;
     int fused2_remap = 0;
+    int fused2_waves = 4;           // workgroup of the 65-128 kernel: 4 or 8 waves
+    int tuned_fused2_waves = 4;
     int tuned_fused2_order = 0;
     int64_t tuned_fused2_target = 0;
     // fused ladder of whole walks of 129-192 (prepacked L, C in LDS), by descending domain
