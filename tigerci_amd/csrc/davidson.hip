@@ -1,14 +1,18 @@
 // Davidson-Liu on device (davidson_liu, davidson.f90:47-239).
 //
 // The expansion vectors c_k and sigma_k = H c_k stay in HBM (the reference
-// keeps them in files, davidson_io.f90:26-52).  Per iteration and root:
+// keeps them in files, davidson_io.f90:26-52).  Per iteration, for the roots
+// that still take new vectors together (every pass over the basis serves up
+// to four of them, the vector work being HBM-bound):
 //   h(k,idx) = <c_k, sigma_idx>                  one multi-dot pass
 //   r = sum_k y_k (sigma_k - E c_k)              one multi-axpy pass
 //   r <- -r / (H_ii - E), 0 where H_ii - E == 0  (build_next_vector_std_eig)
 //   orthonormalise against all c_k              classical Gram-Schmidt applied
-//                                                twice, then normalise; zero if
-//                                                the norm < 1e-14.  The
-//                                                reference's modified GS with
+//                                                twice, then against the new
+//                                                vectors before it, then
+//                                                normalise; zero if the norm
+//                                                < 1e-14.  The reference's
+//                                                modified GS with
 //                                                renormalisation after every
 //                                                projection spans the same vector.
 // The m x m subspace problem (m <= 50 roots) is solved on the host (Jacobi).
@@ -71,6 +75,78 @@ __global__ void mdot_reduce_kernel(const double* __restrict__ partial, int nblk,
   if (threadIdx.x == 0) out[k] = red[0];
 }
 
+// Several vectors at once: partial[(k R + j) nblk + b] = block b's share of
+// X_k . V_j for R <= 4 vectors V_j, so one pass over the X_k serves all of
+// them (each V_j is re-read once per group of MD_GROUP X's).  256 threads.
+template <int R>
+__global__ __launch_bounds__(256) void mdotm_kernel(const double* const* __restrict__ X, int m,
+                                                    const double* const* __restrict__ V, int64_t n,
+                                                    double* __restrict__ partial, int nblk) {
+  const int k0 = blockIdx.y * MD_GROUP;
+  const int g = min(MD_GROUP, m - k0);
+  const double* x[MD_GROUP];
+#pragma unroll
+  for (int q = 0; q < MD_GROUP; ++q) x[q] = X[k0 + min(q, g - 1)];
+  const double* v[R];
+#pragma unroll
+  for (int j = 0; j < R; ++j) v[j] = V[j];
+  double acc[MD_GROUP][R];
+#pragma unroll
+  for (int q = 0; q < MD_GROUP; ++q)
+#pragma unroll
+    for (int j = 0; j < R; ++j) acc[q][j] = 0.0;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n; e += (int64_t)nblk * blockDim.x) {
+    double ve[R];
+#pragma unroll
+    for (int j = 0; j < R; ++j) ve[j] = v[j][e];
+#pragma unroll
+    for (int q = 0; q < MD_GROUP; ++q)
+      if (q < g) {
+        const double xe = x[q][e];
+#pragma unroll
+        for (int j = 0; j < R; ++j) acc[q][j] += xe * ve[j];
+      }
+  }
+  // wave sums by lane exchange, then the four waves through LDS
+  __shared__ double red[4][MD_GROUP * R];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+  for (int q = 0; q < MD_GROUP; ++q)
+#pragma unroll
+    for (int j = 0; j < R; ++j) {
+      double a = acc[q][j];
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) a += __shfl_xor(a, off);
+      if (lane == 0) red[w][q * R + j] = a;
+    }
+  __syncthreads();
+  const int t = threadIdx.x;
+  if (t < MD_GROUP * R && t / R < g)
+    partial[((int64_t)(k0 + t / R) * R + t % R) * nblk + blockIdx.x] = red[0][t] + red[1][t] + red[2][t] + red[3][t];
+}
+
+// out_j = beta out_j + sum_k coef[j m + k] X_k for R <= 4 outputs: one pass
+// over the X_k for all of them
+template <int R>
+__global__ void lincombm_kernel(double* const* __restrict__ out, double beta, const double* const* __restrict__ X,
+                                const double* __restrict__ coef, int m, int64_t n) {
+  double* o[R];
+#pragma unroll
+  for (int j = 0; j < R; ++j) o[j] = out[j];
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x) {
+    double a[R];
+#pragma unroll
+    for (int j = 0; j < R; ++j) a[j] = beta == 0.0 ? 0.0 : beta * o[j][e];
+    for (int k = 0; k < m; ++k) {
+      const double xe = X[k][e];
+#pragma unroll
+      for (int j = 0; j < R; ++j) a[j] += coef[j * m + k] * xe;
+    }
+#pragma unroll
+    for (int j = 0; j < R; ++j) o[j][e] = a[j];
+  }
+}
+
 // out = beta*out + sum_k coef[k] X_k
 __global__ void lincomb_kernel(double* __restrict__ out, double beta, const double* const* __restrict__ X,
                                const double* __restrict__ coef, int m, int64_t n) {
@@ -121,6 +197,55 @@ __global__ void gdiag_kernel(double* __restrict__ g, int64_t n, int64_t n_single
 __global__ void scale_kernel(double* __restrict__ v, double a, int64_t n) {
   for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x)
     v[e] *= a;
+}
+
+// (value, index) order: by value, ties to the lower index
+__device__ __host__ inline bool vi_less(double v, int64_t i, double bv, int64_t bi) {
+  return bi < 0 || v < bv || (v == bv && i < bi);
+}
+
+// Block b's smallest (value, index) pair of d that comes after (pv, pi) in
+// that order (index -1: none).  Called once per root with the previous pick,
+// it lists the lowest diagonal elements in order.  256 threads.
+__global__ __launch_bounds__(256) void argmin_after_kernel(const double* __restrict__ d, int64_t n, double pv,
+                                                           int64_t pi, double* __restrict__ bv,
+                                                           int64_t* __restrict__ bi) {
+  double best = 0.0;
+  int64_t bidx = -1;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x) {
+    const double v = d[e];
+    const bool after = v > pv || (v == pv && e > pi);
+    if (after && vi_less(v, e, best, bidx)) {
+      best = v;
+      bidx = e;
+    }
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    const double ov = __shfl_xor(best, off);
+    const long long oi = __shfl_xor((long long)bidx, off);
+    if (oi >= 0 && vi_less(ov, oi, best, bidx)) {
+      best = ov;
+      bidx = oi;
+    }
+  }
+  __shared__ double sv[4];
+  __shared__ int64_t si[4];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (lane == 0) {
+    sv[w] = best;
+    si[w] = bidx;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int q = 1; q < 4; ++q)
+      if (si[q] >= 0 && vi_less(sv[q], si[q], best, bidx)) {
+        best = sv[q];
+        bidx = si[q];
+      }
+    bv[blockIdx.x] = best;
+    bi[blockIdx.x] = bidx;
+  }
 }
 
 __global__ void unit_kernel(double* __restrict__ v, int64_t n, int64_t k) {
@@ -229,12 +354,77 @@ struct Dav {
   int nblk = 1024;
   DevBuf ptrs, coefs, partial, dots;
   std::vector<double*> hptr;
+  // pointer table: X vectors from 0, the second operand set from XMAX
+  static constexpr int XMAX = 2048, RB = 4;
   explicit Dav(tci_plan& p) : P(p), n(p.n_csf) {
     nblk = (int)std::min<int64_t>(1024, std::max<int64_t>(1, cdiv(n, 256 * 4)));
-    ptrs.alloc(1024 * sizeof(double*));
-    coefs.alloc(1024 * sizeof(double));
-    partial.alloc((size_t)1024 * nblk * sizeof(double));
-    dots.alloc(1024 * sizeof(double));
+    ptrs.alloc((XMAX + RB) * sizeof(double*));
+    coefs.alloc((size_t)RB * XMAX * sizeof(double));
+    partial.alloc((size_t)RB * XMAX * nblk * sizeof(double));
+    dots.alloc((size_t)RB * XMAX * sizeof(double));
+  }
+  // out[k R + j] = X_k . V_j (R = V.size()): one pass over X per group of RB
+  // vectors V_j
+  void mdotm(const std::vector<double*>& X, const std::vector<double*>& V, std::vector<double>& out) {
+    const int m = (int)X.size(), R = (int)V.size();
+    out.assign((size_t)m * R, 0.0);
+    if (m == 0 || R == 0) return;
+    TCI_REQUIRE(m <= XMAX, "too many vectors");
+    TCI_HIP(hipMemcpyAsync(ptrs.p, X.data(), m * sizeof(double*), hipMemcpyHostToDevice, st));
+    std::vector<double> part;
+    for (int j0 = 0; j0 < R; j0 += RB) {
+      const int r = std::min(RB, R - j0);
+      double** vp = static_cast<double**>(ptrs.p) + XMAX;
+      TCI_HIP(hipMemcpyAsync(vp, V.data() + j0, r * sizeof(double*), hipMemcpyHostToDevice, st));
+      const dim3 grid(nblk, (unsigned)cdiv(m, MD_GROUP));
+      const auto* xp = (const double* const*)ptrs.p;
+      const auto* vq = (const double* const*)vp;
+      switch (r) {
+        case 1: hipLaunchKernelGGL(mdotm_kernel<1>, grid, dim3(256), 0, st, xp, m, vq, n, partial.as<double>(), nblk); break;
+        case 2: hipLaunchKernelGGL(mdotm_kernel<2>, grid, dim3(256), 0, st, xp, m, vq, n, partial.as<double>(), nblk); break;
+        case 3: hipLaunchKernelGGL(mdotm_kernel<3>, grid, dim3(256), 0, st, xp, m, vq, n, partial.as<double>(), nblk); break;
+        default: hipLaunchKernelGGL(mdotm_kernel<4>, grid, dim3(256), 0, st, xp, m, vq, n, partial.as<double>(), nblk); break;
+      }
+      TCI_HIP(hipGetLastError());
+      hipLaunchKernelGGL(mdot_reduce_kernel, dim3(m * r), dim3(256), 0, st, partial.as<double>(), nblk, m * r,
+                         dots.as<double>());
+      TCI_HIP(hipGetLastError());
+      part.resize((size_t)m * r);
+      TCI_HIP(hipMemcpyAsync(part.data(), dots.p, part.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+      TCI_HIP(hipStreamSynchronize(st));
+      for (int k = 0; k < m; ++k)
+        for (int j = 0; j < r; ++j) out[(size_t)k * R + j0 + j] = part[(size_t)k * r + j];
+    }
+  }
+  // outs_j = beta outs_j + sum_k c[j m + k] X_k (m = X.size()): one pass over
+  // X per group of RB outputs
+  void lincombm(const std::vector<double*>& outs, double beta, const std::vector<double*>& X,
+                const std::vector<double>& c) {
+    const int m = (int)X.size(), R = (int)outs.size();
+    if (R == 0) return;
+    TCI_REQUIRE(m <= XMAX, "too many vectors");
+    TCI_REQUIRE(c.size() == (size_t)m * R, "coefficient count");
+    if (m > 0) TCI_HIP(hipMemcpyAsync(ptrs.p, X.data(), m * sizeof(double*), hipMemcpyHostToDevice, st));
+    const unsigned grid = (unsigned)std::min<int64_t>(cdiv(n, 256), 8192);
+    for (int j0 = 0; j0 < R; j0 += RB) {
+      const int r = std::min(RB, R - j0);
+      double** op = static_cast<double**>(ptrs.p) + XMAX;
+      TCI_HIP(hipMemcpyAsync(op, outs.data() + j0, r * sizeof(double*), hipMemcpyHostToDevice, st));
+      if (m > 0)
+        TCI_HIP(hipMemcpyAsync(coefs.p, c.data() + (size_t)j0 * m, (size_t)r * m * sizeof(double),
+                               hipMemcpyHostToDevice, st));
+      const auto* xp = (const double* const*)ptrs.p;
+      const double* cp = coefs.as<double>();
+      switch (r) {
+        case 1: hipLaunchKernelGGL(lincombm_kernel<1>, dim3(grid), dim3(256), 0, st, op, beta, xp, cp, m, n); break;
+        case 2: hipLaunchKernelGGL(lincombm_kernel<2>, dim3(grid), dim3(256), 0, st, op, beta, xp, cp, m, n); break;
+        case 3: hipLaunchKernelGGL(lincombm_kernel<3>, dim3(grid), dim3(256), 0, st, op, beta, xp, cp, m, n); break;
+        default: hipLaunchKernelGGL(lincombm_kernel<4>, dim3(grid), dim3(256), 0, st, op, beta, xp, cp, m, n); break;
+      }
+      TCI_HIP(hipGetLastError());
+      // the pointer/coef tables are reused by the next launch
+      TCI_HIP(hipStreamSynchronize(st));
+    }
   }
   // out[k] = X_k . v for k < m
   void mdot(const std::vector<double*>& X, const double* v, std::vector<double>& out) {
@@ -290,6 +480,22 @@ struct Dav {
     scale(v, 1.0 / nr);
     return true;
   }
+  // orthonormalise each of W against X and the W before it: CGS2 against X
+  // with one pass over X per sweep for all of W, then orthonormalize() against
+  // the preceding W.  W_j spans what orthonormalising W_0, W_1, ... one at a
+  // time against X and their predecessors spans; a vanished W_j is zeroed.
+  void orthonormalize_block(const std::vector<double*>& W, const std::vector<double*>& X) {
+    const int K = (int)W.size(), m = (int)X.size();
+    if (K == 0) return;
+    std::vector<double> ov, c((size_t)K * m);
+    for (int pass = 0; pass < 2 && m > 0; ++pass) {
+      mdotm(X, W, ov);
+      for (int j = 0; j < K; ++j)
+        for (int k = 0; k < m; ++k) c[(size_t)j * m + k] = -ov[(size_t)k * K + j];
+      lincombm(W, 1.0, X, c);
+    }
+    for (int j = 0; j < K; ++j) orthonormalize(W[j], std::vector<double*>(W.begin(), W.begin() + j));
+  }
 };
 
 }  // namespace
@@ -333,13 +539,12 @@ int davidson_impl(tci_plan* plan, const tci_davidson_params* prm, const double* 
     DevBuf Vb, Sb, work;
     Vb.alloc((size_t)mv * n * 8);
     Sb.alloc((size_t)mv * n * 8);
-    work.alloc((size_t)2 * n * 8);
+    work.alloc((size_t)n * 8);
     double* Vp = Vb.as<double>();
     double* Sp = Sb.as<double>();
     auto Vk = [&](int k) { return Vp + (int64_t)k * n; };
     auto Sk = [&](int k) { return Sp + (int64_t)k * n; };
-    double* res = work.as<double>();
-    double* tmp = res + n;
+    double* tmp = work.as<double>();  // G c of the generalized residual
     const double* diag = plan->diag.as<double>();
     const unsigned gxn = (unsigned)std::min<int64_t>(cdiv(n, 256), 8192);
     // generalized problem: G on the device, its subspace matrix gs beside h
@@ -390,14 +595,38 @@ int davidson_impl(tci_plan* plan, const tci_davidson_params* prm, const double* 
       if (precision == TCI_SIGMA_FP64) n64 += (int)ks.size();
       else nmix += (int)ks.size();
     };
-    // initial guess: unit vectors at the lowest diagonal elements (init_from_diag)
-    std::vector<double> hd(n);
-    TCI_HIP(hipMemcpy(hd.data(), diag, n * 8, hipMemcpyDeviceToHost));
-    std::vector<int64_t> idx(n);
-    std::iota(idx.begin(), idx.end(), 0);
-    std::partial_sort(idx.begin(), idx.begin() + roots, idx.end(), [&](int64_t a, int64_t b) {
-      return hd[a] < hd[b] || (hd[a] == hd[b] && a < b);
-    });
+    // initial guess: unit vectors at the lowest diagonal elements
+    // (init_from_diag), ties to the lower index; one device pass per root
+    std::vector<int64_t> idx;
+    {
+      const int nb = (int)std::min<int64_t>(1024, std::max<int64_t>(1, cdiv(n, 256 * 4)));
+      DevBuf sel;
+      sel.alloc((size_t)nb * 16);
+      double* bv = sel.as<double>();
+      int64_t* bi = reinterpret_cast<int64_t*>(bv + nb);
+      std::vector<double> hv(nb);
+      std::vector<int64_t> hi(nb);
+      double pv = -INFINITY;
+      int64_t pi = -1;
+      for (int r = 0; r < roots; ++r) {
+        hipLaunchKernelGGL(argmin_after_kernel, dim3(nb), dim3(256), 0, st, diag, n, pv, pi, bv, bi);
+        TCI_HIP(hipGetLastError());
+        TCI_HIP(hipMemcpyAsync(hv.data(), bv, nb * sizeof(double), hipMemcpyDeviceToHost, st));
+        TCI_HIP(hipMemcpyAsync(hi.data(), bi, nb * sizeof(int64_t), hipMemcpyDeviceToHost, st));
+        TCI_HIP(hipStreamSynchronize(st));
+        double best = 0.0;
+        int64_t bidx = -1;
+        for (int b = 0; b < nb; ++b)
+          if (hi[b] >= 0 && vi_less(hv[b], hi[b], best, bidx)) {
+            best = hv[b];
+            bidx = hi[b];
+          }
+        TCI_REQUIRE(bidx >= 0, "fewer ordered diagonal elements than roots");
+        idx.push_back(bidx);
+        pv = best;
+        pi = bidx;
+      }
+    }
     int m = 0;
     for (int r = 0; r < roots; ++r) {
       hipLaunchKernelGGL(unit_kernel, dim3((unsigned)std::min<int64_t>(cdiv(n, 256), 8192)), dim3(256), 0, st, Vk(m), n,
@@ -431,23 +660,30 @@ int davidson_impl(tci_plan* plan, const tci_davidson_params* prm, const double* 
     bool all_conv = false;
     for (it = 1; it <= maxit; ++it) {
       {
+        // new vectors of the unsettled roots: their sigmas, then
+        // h(q,k) = <c_q, sigma_k> for q <= k in one pass over the basis
         std::vector<int> ks;
         for (int r = 0; r < roots; ++r)
           if (!conv[r] && !ready[r]) ks.push_back(root2idx[r]);
         sigma_many(ks);
-      }
-      for (int r = 0; r < roots; ++r) {
-        if (conv[r] || ready[r]) continue;  // no new vector for a settled root
-        const int k = root2idx[r];
-        std::vector<double*> X;
-        for (int q = 0; q <= k; ++q) X.push_back(Vk(q));
-        std::vector<double> hv;
-        D.mdot(X, Sk(k), hv);
-        for (int q = 0; q <= k; ++q) h[(size_t)q * mv + k] = h[(size_t)k * mv + q] = hv[q];
-        if (gen) {
-          const std::vector<double> gv = g_column(k, k + 1);
-          for (int q = 0; q <= k; ++q) gs[(size_t)q * mv + k] = gs[(size_t)k * mv + q] = gv[q];
+        if (!ks.empty()) {
+          const int kmax = *std::max_element(ks.begin(), ks.end());
+          std::vector<double*> X, Ss;
+          for (int q = 0; q <= kmax; ++q) X.push_back(Vk(q));
+          for (int k : ks) Ss.push_back(Sk(k));
+          std::vector<double> hv;
+          D.mdotm(X, Ss, hv);
+          const int R = (int)ks.size();
+          for (int j = 0; j < R; ++j) {
+            const int k = ks[j];
+            for (int q = 0; q <= k; ++q) h[(size_t)q * mv + k] = h[(size_t)k * mv + q] = hv[(size_t)q * R + j];
+          }
         }
+        if (gen)
+          for (int k : ks) {
+            const std::vector<double> gv = g_column(k, k + 1);
+            for (int q = 0; q <= k; ++q) gs[(size_t)q * mv + k] = gs[(size_t)k * mv + q] = gv[q];
+          }
       }
       subspace_solve(m, w, Y);
       for (int r = 0; r < roots; ++r) {
@@ -570,29 +806,49 @@ int davidson_impl(tci_plan* plan, const tci_davidson_params* prm, const double* 
         Xv.push_back(Vk(q));
         Xs.push_back(Sk(q));
       }
-      for (int r = 0; r < roots; ++r) {
-        if (conv[r] || ready[r]) continue;
-        // residual r = sum_k y_k (sigma_k - E G c_k)  (build_residual_gen_eig
-        // for the generalized problem, G = 1 otherwise)
-        if (gen) {
+      // residuals r = sum_k y_k (sigma_k - E G c_k) of the unsettled roots
+      // (build_residual_gen_eig for the generalized problem, G = 1 otherwise)
+      // go straight into the free basis slots m, m + 1, ...: the restart rule
+      // above keeps m + roots <= mv.  For the standard problem one pass over
+      // the basis and its sigmas builds all of them.
+      std::vector<int> act;
+      for (int r = 0; r < roots; ++r)
+        if (!conv[r] && !ready[r]) act.push_back(r);
+      const int na = (int)act.size();
+      TCI_REQUIRE(m + na <= mv, "Davidson subspace budget exhausted");
+      std::vector<double*> rs;
+      for (int i = 0; i < na; ++i) rs.push_back(Vk(m + i));
+      if (gen) {
+        for (int i = 0; i < na; ++i) {
+          const int r = act[i];
           std::vector<double> c(mcur);
           for (int q = 0; q < mcur; ++q) c[q] = Y[(size_t)q * mcur + r];
-          D.lincomb(res, 0.0, Xs, c);
+          D.lincomb(rs[i], 0.0, Xs, c);
           D.lincomb(tmp, 0.0, Xv, c);
-          hipLaunchKernelGGL(gaxpy_kernel, dim3(gxn), dim3(256), 0, st, res, -energy[r], gd, tmp, n);
+          hipLaunchKernelGGL(gaxpy_kernel, dim3(gxn), dim3(256), 0, st, rs[i], -energy[r], gd, tmp, n);
           TCI_HIP(hipGetLastError());
-        } else {
-          std::vector<double*> X;
-          std::vector<double> c;
-          for (int q = 0; q < mcur; ++q) {
-            X.push_back(Sk(q));
-            c.push_back(Y[(size_t)q * mcur + r]);
-            X.push_back(Vk(q));
-            c.push_back(-energy[r] * Y[(size_t)q * mcur + r]);
-          }
-          D.lincomb(res, 0.0, X, c);
         }
-        const double rn = D.norm(res);
+      } else if (na > 0) {
+        std::vector<double*> X;
+        for (int q = 0; q < mcur; ++q) {
+          X.push_back(Sk(q));
+          X.push_back(Vk(q));
+        }
+        std::vector<double> c((size_t)na * 2 * mcur);
+        for (int i = 0; i < na; ++i)
+          for (int q = 0; q < mcur; ++q) {
+            const double y = Y[(size_t)q * mcur + act[i]];
+            c[(size_t)i * 2 * mcur + 2 * q] = y;
+            c[(size_t)i * 2 * mcur + 2 * q + 1] = -energy[act[i]] * y;
+          }
+        D.lincombm(rs, 0.0, X, c);
+      }
+      std::vector<double> rdots;
+      D.mdotm(rs, rs, rdots);
+      std::vector<int> kept;  // indices into act of the roots that get a new vector
+      for (int i = 0; i < na; ++i) {
+        const int r = act[i];
+        const double rn = std::sqrt(std::max(rdots[(size_t)i * na + i], 0.0));
         if (prm->verbose)
           std::printf("  %5d %5d %22.12f %15.8e %15.8e\n", it, r + 1, energy[r] + plan->e_ref, rn, dE[r]);
         // is_converged (davidson.f90:564-602)
@@ -613,20 +869,29 @@ int davidson_impl(tci_plan* plan, const tci_davidson_params* prm, const double* 
           conv[r] = 1;
           continue;
         }
-        // preconditioner: r_i <- -r_i / (H_ii - E_ref - E g_i), 0 where that is 0
-        // (build_next_vector_std_eig / build_next_vector_gen_eig)
+        kept.push_back(i);
+      }
+      // the kept residuals move to the front of the free slots (in root
+      // order, so a source slot is never overwritten before it is read) and
+      // are preconditioned in place: r_i <- -r_i / (H_ii - E_ref - E g_i), 0
+      // where that is 0 (build_next_vector_std_eig / build_next_vector_gen_eig)
+      std::vector<double*> W;
+      for (int t = 0; t < (int)kept.size(); ++t) {
+        const int i = kept[t], r = act[i];
+        if (i != t) TCI_HIP(hipMemcpyAsync(Vk(m + t), Vk(m + i), n * 8, hipMemcpyDeviceToDevice, st));
         if (gen)
-          hipLaunchKernelGGL(precond_gen_kernel, dim3(gxn), dim3(256), 0, st, res, diag, gd, energy[r], n);
+          hipLaunchKernelGGL(precond_gen_kernel, dim3(gxn), dim3(256), 0, st, Vk(m + t), diag, gd, energy[r], n);
         else
-          hipLaunchKernelGGL(precond_kernel, dim3(gxn), dim3(256), 0, st, res, diag, energy[r], n);
+          hipLaunchKernelGGL(precond_kernel, dim3(gxn), dim3(256), 0, st, Vk(m + t), diag, energy[r], n);
         TCI_HIP(hipGetLastError());
-        TCI_REQUIRE(m < mv, "Davidson subspace budget exhausted");
-        TCI_HIP(hipMemcpyAsync(Vk(m), res, n * 8, hipMemcpyDeviceToDevice, st));
+        W.push_back(Vk(m + t));
+      }
+      if (!W.empty()) {
         std::vector<double*> prev;
         for (int q = 0; q < m; ++q) prev.push_back(Vk(q));
-        D.orthonormalize(Vk(m), prev);
-        root2idx[r] = m;
-        ++m;
+        D.orthonormalize_block(W, prev);
+        for (int t = 0; t < (int)kept.size(); ++t) root2idx[act[kept[t]]] = m + t;
+        m += (int)kept.size();
       }
       all_conv = std::all_of(conv.begin(), conv.end(), [](char c) { return c != 0; });
       if (all_conv) break;
