@@ -3,8 +3,8 @@
 The bf16x3 products keep ~17 significant bits per operand (hi + lo bf16) and
 accumulate in FP32, so sigma agrees with the oracle to a few 1e-6 relative
 (bound used: 5e-5 in the 2-norm); scaled fp16x3 keeps ~22 bits (bound 2e-6).  Split precision covers the ladder and,
-where the plan runs them as dense products, the ring and 3-external terms;
-the remaining terms stay FP64.  The mixed
+where the plan runs them as dense products, the ring, 3-external and
+4-internal terms; the remaining terms stay FP64.  The mixed
 Davidson refines in FP64 after the switch, so its roots must match the FP64
 solver's to 1 microhartree (they typically agree to 1e-9).
 """

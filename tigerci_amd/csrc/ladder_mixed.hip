@@ -270,6 +270,9 @@ void mixed_prepare(tci_plan& P, int precision, hipStream_t st) {
                          P.Jbig.as<double>(), niv * niv, mx + 2);
       hipLaunchKernelGGL(absmax_kernel<double>, dim3(blocks_for(nov, 8192)), dim3(256), 0, st, P.Lov.as<double>(),
                          nov, mx + 2);
+      const int64_t nh = 2 * (int64_t)P.n_pairs * P.n_pairs;  // 4-internal coefficients
+      hipLaunchKernelGGL(absmax_kernel<double>, dim3(blocks_for(nh, 8192)), dim3(256), 0, st, P.holeA.as<double>(),
+                         nh, mx + 2);
     }
     hipLaunchKernelGGL(set_scales_kernel, dim3(1), dim3(64), 0, st, 0, mx, sc, (float)P.dmax);
     TCI_HIP(hipGetLastError());

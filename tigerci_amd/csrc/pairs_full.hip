@@ -22,8 +22,10 @@
 // 4-internal term sum_{Q=(k,l)} [(ki|lj) C_Q + (li|kj) C_Q^T] is one product of
 // an n_pairs x 2 n_pairs coefficient matrix with the embedded [C; C^T]
 // (2 n_pairs x n_v^2 rows), gathered back the same way.  All products run on
-// the dense FP64 kernel, or for the split precisions Z, Z3 and W on the
-// split-row kernel (k padded per n_v block to a multiple of 8).  The plan
+// the dense FP64 kernel, or for the split precisions on the split-row kernel
+// (Z, Z3 and W with k padded per n_v block to a multiple of 8; the 4-internal
+// product over k = walk, its B operand the embedded blocks written directly
+// as split rows over k).  The plan
 // takes this path when the products cost at most a few times the gathered
 // per-walk work (full or wide domains).
 #include "plan.hpp"
@@ -282,6 +284,39 @@ bool dense_pairs_choose(const tci_plan& P, double f_gather) {
 
 // FP64 tables: Jbig, work matrices, coefficient matrix and the dense programs;
 // o4 is the host copy of (ki|lj) = o4[((k n + i) n + l) n + j]
+// Split rows over k = walk of the embedded blocks, for the 4-internal product
+// in split precision: row (a, b) of n_v^2 holds C_Q[pos a][pos b] at k = Q
+// and C_Q[pos b][pos a] at k = n_pairs + Q (the Cemb and CTemb entries; 0
+// outside D_Q), zero-padded to a multiple of 8.  One thread per row and
+// 8-wide k chunk (blockIdx.y), so that a block's reads of each C_Q row are
+// contiguous.
+__global__ void embed_split_kernel(PairTab T, EmbedTab E, const double* __restrict__ C, uint16_t* __restrict__ out,
+                                   int64_t ldo, const float* scl, int f16) {
+  const int nv = T.n_v, np = T.n_pairs;
+  const int64_t row = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (row >= (int64_t)nv * nv) return;
+  const int a = (int)(row / nv), b = (int)(row - (int64_t)a * nv);
+  const int k0 = 8 * (int)blockIdx.y;
+  float v[8];
+#pragma unroll
+  for (int t = 0; t < 8; ++t) {
+    const int k = k0 + t;
+    double x = 0.0;
+    if (k < 2 * np) {
+      const int Q = k < np ? k : k - np;
+      const int32_t* ps = E.pos + (int64_t)Q * nv;
+      const int pa = ps[a], pb = ps[b];
+      if (pa >= 0 && pb >= 0) {
+        const int d = (int)(T.dom_off[Q + 1] - T.dom_off[Q]);
+        const double* blk = C + T.cd_off[Q];
+        x = k < np ? blk[(int64_t)pa * d + pb] : blk[(int64_t)pb * d + pa];
+      }
+    }
+    v[t] = (float)x;
+  }
+  put_split8(out + row * ldo + xidx(k0), v, scale_at(scl), f16);
+}
+
 void dense_pairs_build(tci_plan& P, const std::vector<double>& o4, hipStream_t st) {
   const int n = P.n_int, nv = P.n_v, np = P.n_pairs;
   const int64_t N = (int64_t)n * nv, v2 = (int64_t)nv * nv;
@@ -389,6 +424,26 @@ void dense_pairs_build_split(tci_plan& P, hipStream_t st) {
                 Blocking(), 5);
   M.ringZ3x.dma = zdma;
   M.ringZ3x.upload_tables(st);
+  // 4-internal term: Remb = holeA [Cemb; CTemb] with both operands as split
+  // rows over k = walk (2 n_pairs, padded); holeA carries the constants' scale
+  // and the embedded C that of the pair products' C, so the output scale is
+  // the pair products' one
+  {
+    const int np = P.n_pairs;
+    const int64_t v2 = (int64_t)nv * nv, kh = xpad(2 * (int64_t)np), ldh = 2 * kh;
+    M.holeAx.alloc((size_t)np * ldh * 2);
+    M.embTx.alloc((size_t)v2 * ldh * 2);
+    M.bytes += (int64_t)(M.holeAx.bytes + M.embTx.bytes);
+    hipLaunchKernelGGL(split_blocked_kernel, dim3(grid_for(np * (kh / 8))), dim3(256), 0, st, P.holeA.as<double>(),
+                       (int64_t)np, 2 * (int64_t)np, 1, 2 * np, M.holeAx.as<uint16_t>(), ldh, s0, f16);
+    TCI_HIP(hipGetLastError());
+    M.holeGx.clear();
+    M.holeGx.split = true;
+    M.holeGx.tile = 128;
+    M.holeGx.add(P.Remb.as<double>(), v2, np, (int)v2, {dseg(xop(M.holeAx, ldh), xop(M.embTx, ldh), (int)kh, 1.0)},
+                 false, false, 0, Blocking(), 5);
+    M.holeGx.upload_tables(st);
+  }
   Blocking bw;
   bw.cdiv = nv;
   bw.cstride = P.n_chol * nv;
@@ -439,11 +494,18 @@ void dense_pairs_terms(tci_plan& P, int precision, hipStream_t st) {
     M.ringZx.launch_split(precision, st, 0, sc);
     M.ringZ3x.launch_split(precision, st, 0, sc);
     M.progWx.launch_split(precision, st, 0, sc);
+    const int64_t v2 = (int64_t)nv * nv, kh = xpad(2 * (int64_t)P.n_pairs);
+    hipLaunchKernelGGL(embed_split_kernel, dim3((unsigned)cdiv(v2, 256), (unsigned)(kh / 8)), b, 0, st, P.tab, E,
+                       P.C.as<double>(), M.embTx.as<uint16_t>(), 2 * kh, sv, f16);
+    TCI_HIP(hipGetLastError());
+    M.holeGx.launch_split(precision, st, 0, sc);
   }
-  hipLaunchKernelGGL(embed_blocks_kernel, dim3(nt * nt, P.n_pairs), b, 0, st, P.tab, E, P.C.as<double>(),
-                     P.Cemb.as<double>(), P.CTemb.as<double>());
-  TCI_HIP(hipGetLastError());
-  P.holeG.launch(st);
+  if (precision == TCI_SIGMA_FP64) {
+    hipLaunchKernelGGL(embed_blocks_kernel, dim3(nt * nt, P.n_pairs), b, 0, st, P.tab, E, P.C.as<double>(),
+                       P.Cemb.as<double>(), P.CTemb.as<double>());
+    TCI_HIP(hipGetLastError());
+    P.holeG.launch(st);
+  }
   const int ntd = (P.dmax + 31) / 32;
   hipLaunchKernelGGL(ring_combine_kernel, dim3(ntd * ntd, P.n_pairs), b, 0, st, P.tab, P.Zb.as<double>(),
                      P.Z3b.as<double>(), P.Remb.as<double>(), P.R.as<double>());

@@ -71,7 +71,8 @@ struct tci_plan {
     int n_local = 0;
     bool prepacked = false;         // Lpx holds every J (packed once per plan)
     int prec = -1;                  // split precision the copies are made for
-    // scales (fp16x3; all 1 for bf16x3): [0] constants (L, Kov, Jbig, Lov),
+    // scales (fp16x3; all 1 for bf16x3): [0] constants (L, Kov, Jbig, Lov,
+    // the 4-internal coefficients),
     // [1] C of the ladder, [2] ladder intermediate, [3] phase-1 output,
     // [4] phase-2 output, [5] pair-product output, [6] C and Yt of the pair
     // products, [8] max of the constants; maxbits: [0] max|C|, [1] max|Yt|,
@@ -87,6 +88,10 @@ struct tci_plan {
     // dense pair terms in split precision (full domains)
     tci::DevBuf Kx, Jx, CRx, CtRx, CCx, Lovx, LovTx, Ytx;
     tci::GemmProgram ringZx, ringZ3x, progWx;
+    // 4-internal product in split precision: the coefficient matrix and the
+    // embedded [C; C^T] blocks as split rows over k = walk
+    tci::DevBuf holeAx, embTx;
+    tci::GemmProgram holeGx;
     std::vector<LadderPhase> phase;
     std::vector<std::pair<int64_t, int>> chunks;
     int64_t bytes = 0;
