@@ -453,6 +453,26 @@ void dense_pairs_build_split(tci_plan& P, hipStream_t st) {
   M.progWx.add(P.W.as<double>(), nv, (int)P.n_chol, iN, {dseg(xop(M.Lovx, ldx), xop(M.CtRx, ldx), K, 1.0)}, false,
                false, 0, bw, 5);
   M.progWx.upload_tables(st);
+  // With 128 x 128 tiles the four products go out as one launch, tasks with
+  // the longest k first (Z, then Z3 and W, then the 4-internal product):
+  // workgroups take tasks in order as slots free up, so the short tasks fill
+  // the last round instead of each launch ending in a partly empty one
+  M.pairsx.clear();
+  M.pairs_merged = ztile == 128 && !zdma;
+  if (M.pairs_merged) {
+    M.pairsx.split = true;
+    M.pairsx.tile = 128;
+    for (const GemmProgram* g : {&M.ringZx, &M.ringZ3x, &M.progWx, &M.holeGx}) {
+      const int s0 = (int)M.pairsx.segs.size();
+      for (const Seg& x : g->segs) M.pairsx.segs.push_back(x);
+      for (Task t : g->tasks) {
+        t.seg0 += s0;
+        t.seg1 += s0;
+        M.pairsx.tasks.push_back(t);
+      }
+    }
+    M.pairsx.upload_tables(st);
+  }
 }
 
 // ring, 3-external (X and W) and 4-internal terms into R and W; needs Y
@@ -491,14 +511,18 @@ void dense_pairs_terms(tci_plan& P, int precision, hipStream_t st) {
     hipLaunchKernelGGL(split_blocked_kernel, dim3(grid_for(N * (xpad(P.n_chol) / 8))), b, 0, st, P.Yt.as<double>(), N,
                        P.n_chol, 1, (int)P.n_chol, M.Ytx.as<uint16_t>(), ldj, sv, f16);
     TCI_HIP(hipGetLastError());
-    M.ringZx.launch_split(precision, st, 0, sc);
-    M.ringZ3x.launch_split(precision, st, 0, sc);
-    M.progWx.launch_split(precision, st, 0, sc);
     const int64_t v2 = (int64_t)nv * nv, kh = xpad(2 * (int64_t)P.n_pairs);
     hipLaunchKernelGGL(embed_split_kernel, dim3((unsigned)cdiv(v2, 256), (unsigned)(kh / 8)), b, 0, st, P.tab, E,
                        P.C.as<double>(), M.embTx.as<uint16_t>(), 2 * kh, sv, f16);
     TCI_HIP(hipGetLastError());
-    M.holeGx.launch_split(precision, st, 0, sc);
+    if (M.pairs_merged) {
+      M.pairsx.launch_split(precision, st, 0, sc);
+    } else {
+      M.ringZx.launch_split(precision, st, 0, sc);
+      M.ringZ3x.launch_split(precision, st, 0, sc);
+      M.progWx.launch_split(precision, st, 0, sc);
+      M.holeGx.launch_split(precision, st, 0, sc);
+    }
   }
   if (precision == TCI_SIGMA_FP64) {
     hipLaunchKernelGGL(embed_blocks_kernel, dim3(nt * nt, P.n_pairs), b, 0, st, P.tab, E, P.C.as<double>(),

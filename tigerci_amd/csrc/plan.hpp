@@ -92,6 +92,9 @@ struct tci_plan {
     // embedded [C; C^T] blocks as split rows over k = walk
     tci::DevBuf holeAx, embTx;
     tci::GemmProgram holeGx;
+    // Z, Z3, W and the 4-internal product as one task list (128 x 128 tiles)
+    tci::GemmProgram pairsx;
+    bool pairs_merged = false;
     std::vector<LadderPhase> phase;
     std::vector<std::pair<int64_t, int>> chunks;
     int64_t bytes = 0;
