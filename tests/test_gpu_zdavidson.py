@@ -35,3 +35,18 @@ def test_davidson_matches_oracle_trajectory():
     assert res["converged"]
     assert np.abs(e - plan.info.e_ref - E).max() < 1e-8
     assert res["iterations"] == hist[-1][0]
+
+
+@pytest.mark.parametrize("precision", ["fp64", "bf16x3"])
+def test_davidson_six_roots_vs_dense(precision):
+    """Six roots: a pass over the basis serves at most four roots, so the
+    subspace columns, the residuals and the Gram-Schmidt sweeps each run in
+    two groups (four, then two) and the new vectors are orthonormalised
+    across both groups."""
+    print("case: davidson six roots", precision, flush=True)
+    prob = make_problem(3, 10, 12, 7)
+    ref = np.linalg.eigvalsh(dense_hamiltonian(prob, Integrals(prob)))[:6]
+    plan = tci.Plan.from_problem(prob)
+    e, res = plan.davidson(roots=6, energy_tol=1e-11, norm_tol=1e-8, precision=precision)
+    assert res["converged"]
+    assert np.abs(e - plan.info.e_ref - ref).max() < 1e-8
