@@ -71,31 +71,77 @@ __global__ void unpack_kernel(PairTab T, const double* __restrict__ x, double* _
   }
 }
 
-__global__ void rterms_kernel(PairTab T, const double* __restrict__ x, const double* __restrict__ s,
-                              const double* __restrict__ Kov, const double* __restrict__ O3,
-                              double* __restrict__ R, const IoSlot* __restrict__ io) {
+// R_P[r][c] += c0 (ia|jb) - sum_k [(ki|bj) s_k^a + (kj|ai) s_k^b] at
+// a = D_P[r], b = D_P[c] (O3[k][p][q][b] = (kp|bq)).  A workgroup takes a
+// strip of RT_ROWS rows of one pair block: the strip's row factors
+// s_k^a and (kj|ai) go through LDS, RT_K values of k at a time, and each
+// thread keeps its column's factors (ki|bj) and s_k^b in registers while it
+// accumulates all rows of the strip, in the same order over k as a
+// per-element loop.
+constexpr int RT_ROWS = 16, RT_K = 32, RT_THREADS = 128;
+__global__ __launch_bounds__(RT_THREADS) void rterms_kernel(PairTab T, const double* __restrict__ x,
+                                                             const double* __restrict__ s,
+                                                             const double* __restrict__ Kov,
+                                                             const double* __restrict__ O3, double* __restrict__ R,
+                                                             const IoSlot* __restrict__ io) {
   if (io) x = io->x;
   const int P = blockIdx.y;
-  const int i = T.pairs[2 * P], j = T.pairs[2 * P + 1];
   const int64_t off = T.dom_off[P];
   const int d = (int)(T.dom_off[P + 1] - off);
-  const int64_t dd = (int64_t)d * d;
+  const int r0 = blockIdx.x * RT_ROWS;
+  if (r0 >= d) return;  // whole workgroup: no barrier is skipped by part of it
+  const int nr = min(RT_ROWS, d - r0);
+  const int i = T.pairs[2 * P], j = T.pairs[2 * P + 1];
   const int n = T.n_int, nv = T.n_v;
   const int64_t niv = (int64_t)n * nv;
   const double c0 = x[0];
   double* Rb = R + T.cd_off[P];
-  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < dd; e += (int64_t)gridDim.x * blockDim.x) {
-    const int r = (int)(e / d), c = (int)(e - (int64_t)r * d);
-    const int a = T.dom_idx[off + r], b = T.dom_idx[off + c];
-    double v = c0 * Kov[((int64_t)i * nv + a) * niv + (int64_t)j * nv + b];
-    // O3[k][p][q][b] = (kp|bq): for fixed k the lanes of a row read one
-    // n_v-long row of O3 and of s (gathered by the domain), the rest broadcast
-    double t = 0.0;
-    for (int k = 0; k < n; ++k) {
-      t += O3[(((int64_t)k * n + i) * n + j) * nv + b] * s[(int64_t)k * nv + a];
-      t += O3[(((int64_t)k * n + j) * n + i) * nv + a] * s[(int64_t)k * nv + b];
+  __shared__ double sa[RT_K][RT_ROWS], oa[RT_K][RT_ROWS];
+  __shared__ int arow[RT_ROWS];
+  if ((int)threadIdx.x < RT_ROWS) arow[threadIdx.x] = T.dom_idx[off + r0 + min((int)threadIdx.x, nr - 1)];
+  __syncthreads();
+  for (int cb = 0; cb < d; cb += RT_THREADS) {
+    const int c = cb + (int)threadIdx.x;
+    const bool col = c < d;
+    const int b = col ? T.dom_idx[off + c] : 0;
+    double acc[RT_ROWS];
+#pragma unroll
+    for (int rr = 0; rr < RT_ROWS; ++rr) acc[rr] = 0.0;
+    for (int k0 = 0; k0 < n; k0 += RT_K) {
+      const int kn = min(RT_K, n - k0);
+      __syncthreads();  // the previous factors are consumed
+      for (int t = threadIdx.x; t < RT_K * RT_ROWS; t += RT_THREADS) {
+        const int kk = t / RT_ROWS, rr = t - kk * RT_ROWS;
+        double vs = 0.0, vo = 0.0;
+        if (kk < kn && rr < nr) {
+          const int k = k0 + kk, a = arow[rr];
+          vs = s[(int64_t)k * nv + a];
+          vo = O3[(((int64_t)k * n + j) * n + i) * nv + a];
+        }
+        sa[kk][rr] = vs;
+        oa[kk][rr] = vo;
+      }
+      __syncthreads();
+      if (col)
+        for (int kk = 0; kk < kn; ++kk) {
+          const int k = k0 + kk;
+          const double ob = O3[(((int64_t)k * n + i) * n + j) * nv + b];
+          const double sb = s[(int64_t)k * nv + b];
+#pragma unroll
+          for (int rr = 0; rr < RT_ROWS; ++rr) {
+            acc[rr] += ob * sa[kk][rr];
+            acc[rr] += oa[kk][rr] * sb;
+          }
+        }
     }
-    Rb[e] += v - t;
+    if (col)
+#pragma unroll
+      for (int rr = 0; rr < RT_ROWS; ++rr)
+        if (rr < nr) {
+          const int a = arow[rr];
+          const int64_t e = (int64_t)(r0 + rr) * d + c;
+          Rb[e] += c0 * Kov[((int64_t)i * nv + a) * niv + (int64_t)j * nv + b] - acc[rr];
+        }
   }
 }
 
@@ -529,8 +575,8 @@ void launch_unpack(const PairTab& T, int dmax, const double* x, double* C, doubl
 void launch_rterms(const PairTab& T, int dmax, const double* x, const double* s, const double* Kov,
                    const double* O3, double* R, hipStream_t st, const IoSlot* io) {
   if (T.n_pairs == 0) return;
-  hipLaunchKernelGGL(rterms_kernel, dim3(std::min<unsigned>(gx((int64_t)dmax * dmax), 256), T.n_pairs), dim3(256),
-                     0, st, T, x, s, Kov, O3, R, io);
+  hipLaunchKernelGGL(rterms_kernel, dim3((unsigned)cdiv(dmax, RT_ROWS), T.n_pairs), dim3(RT_THREADS), 0, st, T, x, s,
+                     Kov, O3, R, io);
   TCI_HIP(hipGetLastError());
 }
 
