@@ -270,8 +270,11 @@ __global__ __launch_bounds__(256) void s3int_singles_kernel(PairTab T, const int
         sW[ii][cc] = (gi < n && gc < d) ? O3[(((int64_t)jj * n + gi) * n + kk) * nv + T.dom_idx[off + gc]] : 0.0;
       }
       __syncthreads();
+      // rows i >= n hold zeros: skip them (ig is the wave index, so the test
+      // is uniform over a wave)
 #pragma unroll
       for (int q = 0; q < 16; ++q) {
+        if (i0 + ig + 4 * q >= n) continue;
         double a = 0.0;
 #pragma unroll 8
         for (int cc = 0; cc < S3C; ++cc) a += sW[ig + 4 * q][cc] * sC[r][cc];
