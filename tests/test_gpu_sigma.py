@@ -167,3 +167,23 @@ def test_local_domain_pair_paths(dense):
         ref = oracle_sigma(prob, x, ints)
         assert _rel(_gpu_sigma(plan, x), ref) < TOL
         assert _rel(_gpu_sigma(plan, x, precision="bf16x3"), ref) < 5e-5
+
+
+@pytest.mark.parametrize("args", [
+    dict(n_int=34, n_v=6, n_chol=10, seed=81),
+    dict(n_int=34, n_v=8, n_chol=12, seed=83, domain_range=(3, 8), weak_below=2),
+    dict(n_int=2, n_v=150, n_chol=20, seed=82),
+])
+def test_sigma_pair_block_strips(args):
+    """The (ia|jb) and (ki|bj) terms of the pair blocks run in 16-row strips
+    over 128 columns and 32 internal orbitals at a time: n_int = 34 takes two
+    passes over k (also with local domains and weak pairs), d = 150 two
+    column passes.  FP64 and bf16x3 against the oracle."""
+    _say("case: pair-block strips", args)
+    prob = make_problem(**args)
+    ints = Integrals(prob)
+    plan = tci.Plan.from_problem(prob)
+    x = random_ci_vector(prob, 90)
+    ref = oracle_sigma(prob, x, ints)
+    assert _rel(_gpu_sigma(plan, x), ref) < TOL
+    assert _rel(_gpu_sigma(plan, x, precision="bf16x3"), ref) < 5e-5
