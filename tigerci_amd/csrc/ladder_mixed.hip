@@ -92,11 +92,16 @@ __global__ void split_lvv_kernel(const LT* __restrict__ Lvv, int64_t rows, int n
   }
 }
 
-// Lpx[q] rows (jj, r) = split row r of L^{J0+jj}[D_P, D_P] for local pair q
+// Lpx[q] rows (jj, r) = split row r of L^{J0+jj}[D_P, D_P] for local pair q.
+// One wave per row: the lanes first gather up to 512 domain columns of the row
+// in order (lane l takes columns l, l + 64, ..., so that one load instruction
+// reads increasing addresses), the row passes through LDS, and lane l then
+// splits columns 8l .. 8l + 7 into one chunk.  256 threads.
 template <class LT>
-__global__ void pack_L_split_kernel(PairTab T, const int32_t* __restrict__ pairs_local,
-                                    const int64_t* __restrict__ lp_off, const LT* __restrict__ Lvv, int64_t J0,
-                                    int len, uint16_t* __restrict__ Lpx, const float* scl, int f16) {
+__global__ __launch_bounds__(256) void pack_L_split_kernel(PairTab T, const int32_t* __restrict__ pairs_local,
+                                                           const int64_t* __restrict__ lp_off,
+                                                           const LT* __restrict__ Lvv, int64_t J0, int len,
+                                                           uint16_t* __restrict__ Lpx, const float* scl, int f16) {
   const float sc = scale_at(scl);
   const int q = blockIdx.y;
   const int P = pairs_local[q];
@@ -104,21 +109,35 @@ __global__ void pack_L_split_kernel(PairTab T, const int32_t* __restrict__ pairs
   const int d = (int)(T.dom_off[P + 1] - off);
   const int nch = (d + 7) >> 3;
   const int64_t nv = T.n_v;
-  const int64_t n = (int64_t)len * d * nch;
+  const int64_t rows = (int64_t)len * d;
+  const int32_t* dix = T.dom_idx + off;
   uint16_t* out = Lpx + lp_off[q];
-  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t row = e / nch;  // jj * d + r
-    const int ch = (int)(e - row * nch);
-    const int64_t jj = row / d;
-    const int r = (int)(row - jj * d);
-    const LT* L = Lvv + (J0 + jj) * nv * nv + (int64_t)T.dom_idx[off + r] * nv;
-    float v[8];
-#pragma unroll
-    for (int t = 0; t < 8; ++t) {
-      const int k = 8 * ch + t;
-      v[t] = k < d ? (float)L[T.dom_idx[off + min(k, d - 1)]] : 0.f;
+  __shared__ float rowbuf[4][512];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  // the workgroup steps through rows together, so every barrier is reached
+  // by all of its threads
+  for (int64_t base = (int64_t)blockIdx.x * 4; base < rows; base += (int64_t)gridDim.x * 4) {
+    const int64_t row = base + w;  // jj d + r
+    const bool ok = row < rows;
+    const LT* L = Lvv;
+    if (ok) {
+      const int64_t jj = row / d;
+      const int r = (int)(row - jj * d);
+      L = Lvv + (J0 + jj) * nv * nv + (int64_t)dix[r] * nv;
     }
-    put_split8(out + row * 16 * nch + 16 * ch, v, sc, f16);
+    for (int c0 = 0; c0 < d; c0 += 512) {
+      const int cn = min(512, d - c0);
+      for (int t = lane; t < 512; t += 64) rowbuf[w][t] = (ok && t < cn) ? (float)L[dix[c0 + t]] : 0.f;
+      __syncthreads();
+      if (ok)
+        for (int ch = lane; 8 * ch < cn; ch += 64) {
+          float v[8];
+#pragma unroll
+          for (int t = 0; t < 8; ++t) v[t] = rowbuf[w][8 * ch + t];
+          put_split8(out + row * 16 * nch + 16 * ((c0 >> 3) + ch), v, sc, f16);
+        }
+      __syncthreads();
+    }
   }
 }
 
