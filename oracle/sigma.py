@@ -160,10 +160,11 @@ def sigma(prob: Problem, x: np.ndarray, ints: Integrals | None = None,
         blk = R[i, j].copy()
         if "ladder" in terms:
             D = prob.domain(P)
-            Lp = ints.Lvv[:, D][:, :, D]                # (n_chol, d, d)
             Cp = C[i, j][np.ix_(D, D)]
-            lad = np.einsum('Jac,cd,Jbd->ab', Lp, Cp, Lp, optimize=True)
-            blk[np.ix_(D, D)] += lad
+            if np.any(Cp):                              # a zero block adds nothing
+                Lp = ints.Lvv[:, D][:, :, D]            # (n_chol, d, d)
+                lad = np.einsum('Jac,cd,Jbd->ab', Lp, Cp, Lp, optimize=True)
+                blk[np.ix_(D, D)] += lad
         Rd[(i, j)] = blk
 
     # ---- singles -----------------------------------------------------------

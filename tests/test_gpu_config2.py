@@ -1,0 +1,84 @@
+"""The 16 internal / 300 virtual / 1200 Cholesky-vector full-domain workload
+(the precision study): FP64 sigma against the oracle on a vector whose
+doubles fill two pair blocks (every other term over the whole space), the
+hermiticity and valence-component properties on dense random vectors, and
+the error of both split precisions against FP64 sigma: the relative 2-norm
+of sigma and the Rayleigh-quotient error c.sigma in Eh."""
+import sys
+
+import numpy as np
+import pytest
+
+import tigerci_amd as tci
+from oracle.problem import Problem
+from oracle.sigma import Integrals, sigma as oracle_sigma
+from tigerci_amd.synthetic import workload, random_vector
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def _rel(a, b):
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+@pytest.fixture(scope="module")
+def c2():
+    syn = workload("sigma_16x300x1200_full")
+    plan = tci.Plan.from_problem(syn)
+    _say(f"  plan n_csf={plan.n_csf} device {plan.info.device_bytes / 2**30:.1f} GiB")
+    return syn, plan
+
+
+def _sigma(plan, x, precision="fp64"):
+    c = tci.DeviceArray.from_numpy(x)
+    out = tci.DeviceArray(plan.n_csf)
+    plan.sigma(c, out, precision=precision)
+    return out.numpy()
+
+
+def test_config2_fp64_against_oracle(c2):
+    syn, plan = c2
+    _say("case: 16x300x1200 FP64 vs oracle (reference, singles, two pair blocks)")
+    prob = Problem(syn.n_int, syn.n_v, syn.n_chol, syn.eps, syn.L, syn.pairs, syn.dom_off, syn.dom_idx)
+    lay = prob.layout()
+    x = np.zeros(plan.n_csf)
+    rng = np.random.default_rng(7)
+    x[: 1 + syn.n_int * syn.n_v] = rng.standard_normal(1 + syn.n_int * syn.n_v)
+    for P in (5, 8):                  # walks (2,2) and (3,2): one i == j, one i > j block
+        end = lay.aa_off[P + 1] if P + 1 < prob.n_pairs else lay.n_csf
+        x[lay.aa_off[P]:end] = rng.standard_normal(end - lay.aa_off[P])
+    x /= np.linalg.norm(x)
+    ref = oracle_sigma(prob, x, Integrals(prob))
+    got = _sigma(plan, x)
+    _say(f"  fp64 rel err {_rel(got, ref):.2e}")
+    assert _rel(got, ref) < 1e-10
+
+
+def test_config2_hermitian_and_valence(c2):
+    syn, plan = c2
+    _say("case: 16x300x1200 hermiticity / valence component")
+    x1, x2 = random_vector(plan.n_csf, 1), random_vector(plan.n_csf, 2)
+    s1, s2 = _sigma(plan, x1), _sigma(plan, x2)
+    asym = abs(x2 @ s1 - x1 @ s2) / (np.linalg.norm(s1) * np.linalg.norm(x2))
+    _say(f"  hermiticity defect {asym:.2e}")
+    assert asym < 1e-12
+
+
+@pytest.mark.parametrize("precision,tol", [("bf16x3", 2e-5), ("fp16x3", 1e-5)])
+def test_config2_precision_study(c2, precision, tol):
+    """Split precision against FP64 on a dense random vector; the measured
+    figures are printed (the workload reports them, it sets no bound)."""
+    syn, plan = c2
+    _say("case: 16x300x1200 precision study", precision)
+    x = random_vector(plan.n_csf, 3)
+    s64, smx = _sigma(plan, x), _sigma(plan, x, precision)
+    err = _rel(smx, s64)
+    ray = abs(x @ smx - x @ s64)
+    _say(f"  {precision}: sigma rel err {err:.2e}, Rayleigh error {ray:.2e} Eh (|c.sigma| = {abs(x @ s64):.3e})")
+    assert err < tol
+    assert ray < tol * abs(x @ s64) * 10 + 1e-12

@@ -6,8 +6,10 @@ at any size:
   * the valence component sigma_0 = sum_P <K^P, Ct^P> recomputed in numpy from
     the Cholesky records;
   * bf16x3 sigma against FP64 sigma.
-They take minutes, so they run only with TCI_LARGE_TESTS=1."""
-import os
+The workload's mixed-precision bound is 1e-6 relative: scaled fp16x3 meets
+it and is the split precision that carries that gate here; bf16x3 (about 6e-6
+at this size, from FP32 accumulation over the long contraction lengths) is
+held to 1e-5."""
 import sys
 
 import numpy as np
@@ -16,8 +18,7 @@ import pytest
 import tigerci_amd as tci
 from tigerci_amd.synthetic import workload, random_vector
 
-pytestmark = [pytest.mark.gpu,
-              pytest.mark.skipif(os.environ.get("TCI_LARGE_TESTS") != "1", reason="set TCI_LARGE_TESTS=1")]
+pytestmark = pytest.mark.gpu
 
 
 def _say(*a):
@@ -82,7 +83,7 @@ def test_config3_hermitian_and_valence(big):
     assert abs(s1[0] - s0) <= 1e-10 * max(1.0, abs(s0))
 
 
-@pytest.mark.parametrize("precision,tol", [("bf16x3", 5e-5), ("fp16x3", 1e-6)])
+@pytest.mark.parametrize("precision,tol", [("bf16x3", 1e-5), ("fp16x3", 1e-6)])
 def test_config3_split_vs_fp64(big, precision, tol):
     """Split-precision sigma against FP64 sigma; the scaled-FP16 split meets
     the 1e-6 relative bound of this workload."""

@@ -188,6 +188,14 @@ def _ptr(t) -> int:
     return t.data_ptr()
 
 
+def _need(t, n: int, what: str) -> None:
+    """A DeviceArray passed as a vector must hold at least n doubles; the
+    library reads and writes n of them (raw addresses are the caller's
+    responsibility)."""
+    if isinstance(t, DeviceArray) and t.n < n:
+        raise TigerSigmaError(f"{what} holds {t.n} doubles, needs {n}")
+
+
 def _stream(stream):
     if stream is None:
         return None
@@ -261,12 +269,16 @@ class Plan:
         ``graph``: replay the build from a HIP graph captured on first use."""
         flags = PRECISION[precision] | (TCI_ACCUMULATE if accumulate else 0) | (TCI_ADD_EREF if add_eref else 0)
         flags |= TCI_SIGMA_GRAPH if graph else 0
+        _need(c, self.n_csf, "c")
+        _need(out, self.n_csf, "out")
         _check(self._lib.tci_sigma_apply(self._h, _ptr(c), _ptr(out), self.n_csf, flags, _stream(stream)))
 
     def sigma_batch(self, c, out, nvec: int, precision: str = "fp64", accumulate: bool = False,
                     add_eref: bool = False, stream=None) -> None:
         """nvec sigma builds at once; c and out hold nvec vectors back to back."""
         flags = PRECISION[precision] | (TCI_ACCUMULATE if accumulate else 0) | (TCI_ADD_EREF if add_eref else 0)
+        _need(c, nvec * self.n_csf, "c")
+        _need(out, nvec * self.n_csf, "out")
         _check(self._lib.tci_sigma_apply_batch(self._h, _ptr(c), _ptr(out), self.n_csf, nvec, flags,
                                                _stream(stream)))
 
@@ -276,6 +288,7 @@ class Plan:
             d = DeviceArray(self.n_csf)
             _check(self._lib.tci_diagonal(self._h, d.ptr, self.n_csf, None))
             return d.numpy()
+        _need(out, self.n_csf, "out")
         _check(self._lib.tci_diagonal(self._h, _ptr(out), self.n_csf, _stream(stream)))
         return None
 
@@ -318,6 +331,8 @@ class Plan:
                        1 if verbose else 0)
         e = np.zeros(roots)
         r = _DavResult()
+        if vectors is not None:
+            _need(vectors, roots * self.n_csf, "vectors")
         vp = _ptr(vectors) if vectors is not None else None
         if g_values is None:
             _check(self._lib.tci_davidson_run(self._h, ctypes.byref(p), e.ctypes.data, vp, ctypes.byref(r)))

@@ -1,6 +1,6 @@
 """Build libtigersigma.so for gfx950 with hipcc (in-tree, no JIT cache).
 
-    python -m tigerci_amd.build [--jobs N]
+    python -m tigerci_amd.build [--jobs N] [--incremental]
 
 Each .hip source compiles to an object in tigerci_amd/_build/ (in parallel),
 then one hipcc link produces tigerci_amd/libtigersigma.so.
@@ -27,8 +27,26 @@ SOURCES = ["plan.hip", "gemm.hip", "dense_gemm.hip", "dense_split.hip", "pairs_f
            "ladder_fused128.hip", "ladder_fused192.hip", "xgemm192d.hip"]
 
 
-def _compile(src: str) -> str:
+def _deps(path: str, seen: set) -> set:
+    """Files a source includes with quotes, transitively."""
+    with open(path) as f:
+        for line in f:
+            t = line.strip()
+            if t.startswith("#include") and '"' in t:
+                inc = os.path.normpath(os.path.join(os.path.dirname(path), t.split('"')[1]))
+                if inc not in seen and os.path.exists(inc):
+                    seen.add(inc)
+                    _deps(inc, seen)
+    return seen
+
+
+def _compile(src: str, incremental: bool = False) -> str:
     obj = os.path.join(BUILD, os.path.splitext(src)[0] + ".o")
+    if incremental and os.path.exists(obj):
+        path = os.path.join(CSRC, src)
+        newest = max(os.path.getmtime(f) for f in _deps(path, {path}))
+        if os.path.getmtime(obj) > newest:
+            return obj
     cmd = [HIPCC, *FLAGS, "-c", os.path.join(CSRC, src), "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
@@ -39,11 +57,13 @@ def _compile(src: str) -> str:
     return obj
 
 
-def build(jobs: int | None = None) -> str:
+def build(jobs: int | None = None, incremental: bool = False) -> str:
+    """Compile every source (or, with ``incremental``, those whose object is
+    older than the source or a file it includes) and link the library."""
     os.makedirs(BUILD, exist_ok=True)
     jobs = jobs or min(len(SOURCES), os.cpu_count() or 4)
     with cf.ThreadPoolExecutor(jobs) as ex:
-        objs = list(ex.map(_compile, SOURCES))
+        objs = list(ex.map(lambda f: _compile(f, incremental), SOURCES))
     tmp = LIB + ".tmp"
     cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", *objs, "-o", tmp]
     r = subprocess.run(cmd, capture_output=True, text=True)
@@ -56,5 +76,7 @@ def build(jobs: int | None = None) -> str:
 if __name__ == "__main__":
     ap = argparse.ArgumentParser()
     ap.add_argument("--jobs", type=int, default=None)
+    ap.add_argument("--incremental", action="store_true",
+                    help="recompile only sources newer than their objects or the files they include")
     a = ap.parse_args()
-    print(build(a.jobs))
+    print(build(a.jobs, a.incremental))

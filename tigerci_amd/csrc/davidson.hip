@@ -18,9 +18,13 @@
 // The m x m subspace problem (m <= 50 roots) is solved on the host (Jacobi).
 // Mixed precision: expansion sigmas use the split ladder until every root's
 // |dE| < switch_tol; then the subspace collapses to the current Ritz vectors,
-// their sigmas are rebuilt in FP64 and the solve finishes in FP64.  The first
-// iteration after the switch never counts as converged (its dE compares a
-// mixed-precision eigenvalue with an FP64 one).
+// their sigmas are rebuilt in FP64 and the solve finishes in FP64.  The
+// switch is not a restart: in the iteration after it, dE of a root is the
+// difference between its FP64 Ritz value in the collapsed space and its last
+// mixed-precision one (the Rayleigh-quotient error of the split sigma), and
+// the root converges there when that dE < energy_tol or its FP64 residual
+// norm < norm_tol.  A budget collapse is a restart: neither that iteration
+// nor the next counts as converged (davidson_restart.f90).
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -517,13 +521,18 @@ int davidson_impl(tci_plan* plan, const tci_davidson_params* prm, const double* 
     // default subspace budget: every iteration's vectors, capped so that the
     // basis and its sigma vectors take at most 32 GB (the reference's disk
     // budget rule, davidson_restart.f90:26-60, with device memory as the store)
+    // The basis and sigma stores hold mv + 2*roots vectors: the collapse
+    // builds its (at most 2*roots) new vectors in the slots beyond the
+    // current basis before moving them to the front, and a collapse can fall
+    // due at any m <= mv.
     int mv = prm->max_vectors;
     if (mv <= 0) {
-      const int64_t cap = (int64_t)(32.0e9 / (16.0 * (double)plan->n_csf));
+      const int64_t cap = (int64_t)(32.0e9 / (16.0 * (double)plan->n_csf)) - 2 * roots;
       mv = (int)std::max<int64_t>(5 * roots, std::min<int64_t>((int64_t)maxit * roots + roots, cap));
     }
     mv = std::min(mv, 1000);
     TCI_REQUIRE(mv >= 5 * roots, "max_vectors must be at least 5*roots");
+    const int mslots = mv + 2 * roots;
     // low byte: tci_precision of the expansion sigmas; TCI_SIGMA_GRAPH replays
     // every sigma build from a captured HIP graph
     TCI_REQUIRE((prm->precision & ~(0xff | TCI_SIGMA_GRAPH)) == 0, "unknown bits in precision");
@@ -537,8 +546,8 @@ int davidson_impl(tci_plan* plan, const tci_davidson_params* prm, const double* 
     double t_sigma = 0.0;
     int n64 = 0, nmix = 0;
     DevBuf Vb, Sb, work;
-    Vb.alloc((size_t)mv * n * 8);
-    Sb.alloc((size_t)mv * n * 8);
+    Vb.alloc((size_t)mslots * n * 8);
+    Sb.alloc((size_t)mslots * n * 8);
     work.alloc((size_t)n * 8);
     double* Vp = Vb.as<double>();
     double* Sp = Sb.as<double>();
@@ -721,8 +730,8 @@ int davidson_impl(tci_plan* plan, const tci_davidson_params* prm, const double* 
             cols.push_back(cp);
           }
         }
-        // new vectors go to slots m .. m+ncols-1 (space reserved by the budget check)
-        TCI_REQUIRE(m + (int)cols.size() <= mv, "subspace budget too small for a collapse");
+        // new vectors go to slots m .. m+ncols-1 (m <= mv, ncols <= 2*roots)
+        TCI_REQUIRE(m + (int)cols.size() <= mslots, "subspace store too small for a collapse");
         std::vector<double*> Xv, Xs;
         for (int q = 0; q < m; ++q) {
           Xv.push_back(Vk(q));

@@ -1092,7 +1092,7 @@ int tci_sigma_apply(tci_plan* plan, const double* c, double* sigma, int64_t n, i
   TCI_GUARD_BEGIN
   TCI_REQUIRE(plan && c && sigma, "null argument");
   TCI_REQUIRE(n == plan->n_csf, "vector length does not match the plan's CSF count");
-  TCI_REQUIRE((const void*)c != (const void*)sigma, "c and sigma must not alias");
+  TCI_REQUIRE(c + n <= sigma || sigma + n <= c, "c and sigma must not overlap");
   plan->sigma(c, sigma, flags, static_cast<hipStream_t>(stream));
   return TCI_OK;
   TCI_GUARD_END
