@@ -68,78 +68,135 @@ def ladder_alg_bytes(syn, precision: str, fused=()) -> float:
     return float(per_j * syn.n_chol)
 
 
-def roofline(info, syn, precision: str, seconds: float, traffic, kernel: str, fused=()) -> dict:
-    """Roofline of the ladder section: the binding roof is the larger of the
-    MFMA time (algorithmic flops, x3 MFMAs for a split product) and the HBM
-    time of the algorithm's minimum bytes; achieved is that quantity over the
-    measured section time."""
-    flops = info.flops_ladder
-    mfma_flops = flops * (1 if precision == "fp64" else 3)
-    alg_bytes = ladder_alg_bytes(syn, precision, fused)
-    t_mfma, t_hbm = mfma_flops / PEAK[precision], alg_bytes / HBM_PEAK
-    r = {"kernel": kernel, "avg_launch_ms": seconds * 1e3, "algorithmic_flops": flops, "algorithmic_bytes": alg_bytes,
-         "traffic": traffic, "traffic_gbs": traffic / seconds / 1e9 if traffic else None,
-         "tflops_algorithmic": flops / seconds / 1e12, "mfma_busy_frac": mfma_flops / seconds / PEAK[precision]}
-    if t_hbm > t_mfma:
-        ach = alg_bytes / seconds
-        r.update({"bound": "hbm", "achieved": ach / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": ach / HBM_PEAK})
-    else:
-        ach = flops / seconds  # algorithmic flops against the published dense peak of the precision
-        r.update({"bound": "mfma", "achieved": ach / 1e12, "peak": PEAK[precision] / 1e12, "unit": "TFLOP/s",
-                  "frac": ach / PEAK[precision]})
-    return r
+def ladder_floor_bytes(syn, precision: str) -> float:
+    """Byte floor of one sigma's ladder: every walk's L^J[D,D] block read once
+    (4 bytes per element in split form or FP32 storage, 8 in FP64), the
+    full-domain walks sharing L^J itself; C and R are negligible beside it."""
+    d = np.diff(np.asarray(syn.dom_off)).astype(np.float64)
+    full = d == syn.n_v
+    e_l = 4.0 if (precision != "fp64" or syn.L.dtype == np.float32) else 8.0
+    per_j = e_l * ((d[~full] ** 2).sum() + (syn.n_v ** 2 if full.any() else 0.0))
+    return float(per_j * syn.n_chol)
 
 
-def measured_traffic(workload: str, precision: str):
-    """HBM bytes of one sigma's ladder section, measured with rocprofv3 --pmc
-    (profiles/ladder_traffic.txt), or None for an unmeasured workload."""
-    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "ladder_traffic.txt")
+def kernel_table(plan, tci, x_host, precision: str, n: int) -> list:
+    """Per-kernel event times of n eager profiled sigma builds of one precision."""
+    xp = tci.DeviceArray.from_numpy(x_host)
+    yp = tci.DeviceArray(plan.n_csf)
+    plan.sigma(xp, yp, precision=precision)
+    plan.profile(True)
+    for _ in range(n):
+        plan.sigma(xp, yp, precision=precision)
+    tab = plan.profile_kernels(precision)
+    lad = plan.profile_read(precision)
+    plan.profile(False)
+    for k in tab:
+        k["ms_per_sigma"] = k["ms"] / n
+        k["launches_per_sigma"] = k["launches"] / n
+    return tab, lad
+
+
+def kernel_traffic(workload: str, precision: str, kernel: str):
+    """HBM bytes per launch of one kernel (rocprofv3 --pmc FETCH_SIZE and
+    WRITE_SIZE, each scaled by the copy-kernel calibration), from the
+    machine-readable lines of profiles/kernel_traffic.txt, or None."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "kernel_traffic.txt")
     if not os.path.exists(path):
         return None
     for line in open(path):
-        f = line.split()
-        if len(f) == 3 and not line.startswith("#") and f[0] == workload and f[1] == precision:
-            return float(f[2])
+        if line.startswith("#") or not line.strip():
+            continue
+        f = line.rstrip("\n").split("|")
+        if len(f) >= 4 and f[0] == workload and f[1] == precision and f[2] == kernel:
+            return float(f[3])
     return None
 
 
-def cpu_baseline(syn, plan, n_sigma_fp64: int, budget_s: float = 15.0) -> dict:
-    """Time the oracle's dense ladder restatement on a bounded sample of pair
-    blocks of the same workload, and scale to one FP64 Davidson solve
-    (n_sigma_fp64 sigma builds, each plan.info.flops_sigma flops)."""
+def kernel_roofline(k: dict, precision: str, traffic) -> dict:
+    """Roofline of one kernel: algorithmic flops per launch (2 m n k over the
+    valid extents of its products) over its average launch time, against the
+    dense MFMA peak of the precision the products run in (the published
+    2.5 PFLOP/s bf16 for a split product, whose three MFMAs make the units
+    busy 3x that fraction)."""
+    t = k["ms"] / max(k["launches"], 1) / 1e3
+    ach = k["flops"] / t
+    r = {"kernel": k["name"], "bound": "mfma", "achieved": ach / 1e12, "peak": PEAK[precision] / 1e12,
+         "unit": "TFLOP/s", "frac": ach / PEAK[precision], "traffic": traffic,
+         "avg_launch_ms": t * 1e3, "launches_per_sigma": k["launches_per_sigma"],
+         "algorithmic_flops_per_launch": k["flops"]}
+    if precision != "fp64":
+        r["mfma_busy_frac"] = 3.0 * ach / PEAK[precision]
+    if traffic:
+        r["traffic_gbs"] = traffic / t / 1e9
+    return r
+
+
+def ladder_section(info, syn, precision: str, seconds: float, fused=()) -> dict:
+    """The whole ladder of one sigma (several launches) against section 8(d)'s
+    roof: F_lad = sum_P 4 d_P^3 n_chol algorithmic flops over the section time,
+    against the dense MFMA peak; the byte floor (L^J[D,D] read once) and the
+    chosen schedule's own minimum bytes beside it."""
+    flops = info.flops_ladder
+    return {"ms": seconds * 1e3, "algorithmic_flops": flops, "tflops": flops / seconds / 1e12,
+            "frac_mfma": flops / seconds / PEAK[precision],
+            "byte_floor": ladder_floor_bytes(syn, precision),
+            "floor_gbs": ladder_floor_bytes(syn, precision) / seconds / 1e9,
+            "schedule_bytes": ladder_alg_bytes(syn, precision, fused)}
+
+
+def cpu_baseline(syn, x, s64_gpu, n_sigma_fp64: int, budget_s: float = 25.0) -> dict:
+    """The reference's CPU sigma path, restated in C with its loop structure
+    (oracle/cpu_ladder.c: OpenMP over the outer virtual a, one DGEMM per a
+    assembling (ab|cd) from L^J, per-walk DGEMV products,
+    three_and_four_seg_mod_1.f90:318-632,3240-3627) and every other sigma term
+    from the numpy oracle, timed on this host's cores in FP64.  The other
+    terms run whole; the ladder runs whole when it fits the budget, otherwise
+    over an evenly strided sample of the outer virtuals, scaled by the sampled
+    fraction.  A whole run is checked against the GPU FP64 sigma of the same
+    vector.  value = seconds of the FP64 Davidson solve's sigma builds on the
+    CPU (its vector work is not counted)."""
     from threadpoolctl import threadpool_info
-    from oracle.scepper import scepper_p1, scepper_m1
-    rng = np.random.default_rng(7)
-    n_int = syn.n_int
-    L = syn.L
-    order = np.argsort(-np.diff(syn.dom_off))
-    sample = list(order[::max(1, len(order) // 16)])
-    flops = 0.0
+    from oracle.problem import Problem
+    from oracle.sigma import Integrals, sigma as oracle_sigma, ALL_TERMS
+    from oracle.cpu_sigma import cpu_ladder, host_cores
+    cores = host_cores()
+    prob = Problem(syn.n_int, syn.n_v, syn.n_chol, syn.eps, syn.L, syn.pairs, syn.dom_off, syn.dom_idx)
+    L64 = np.ascontiguousarray(syn.L, dtype=np.float64)
     t0 = time.perf_counter()
-    used = 0
-    while time.perf_counter() - t0 < budget_s:
-        P = sample[used % len(sample)]
-        D = syn.dom_idx[syn.dom_off[P]:syn.dom_off[P + 1]]
-        d = len(D)
-        a = n_int + D
-        rec = (np.maximum.outer(a, a) * (np.maximum.outer(a, a) + 1) // 2 + np.minimum.outer(a, a))
-        Lp = np.ascontiguousarray(L[rec.ravel()].T.reshape(syn.n_chol, d, d), dtype=np.float64)
-        m2 = d * (d - 1) // 2
-        C = scepper_p1(rng.standard_normal(m2), rng.standard_normal(d), d) / 2 + \
-            scepper_m1(rng.standard_normal(m2), d) / 3.4641016151377544
-        (Lp @ C @ Lp).sum(axis=0)
-        flops += 4.0 * d ** 3 * syn.n_chol
-        used += 1
-    dt = time.perf_counter() - t0
-    rate = flops / dt
-    threads = max([x.get("num_threads", 1) for x in threadpool_info() if x.get("user_api") == "blas"] or [1])
-    sigma_s = plan.info.flops_sigma / rate
-    return {"value": sigma_s * n_sigma_fp64, "unit": "s", "cores": int(threads), "kind": "port",
-            "sample": f"oracle dense ladder (numpy BLAS, FP64), {used} pair-block ladders drawn from "
-                      f"{len(sample)} of {len(order)} walks "
-                      f"({flops:.3g} flop in {dt:.1f} s = {rate / 1e9:.1f} GFLOP/s), scaled by flops to "
-                      f"{n_sigma_fp64} FP64 sigma builds of {plan.info.flops_sigma:.3g} flop",
-            "sigma_seconds": sigma_s}
+    ints = Integrals(prob)
+    t_int = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    rest = oracle_sigma(prob, x, ints, terms=ALL_TERMS - {"ladder"})
+    t_rest = time.perf_counter() - t0
+    del ints
+    # one outer virtual on one thread, to size the sample
+    probe = np.zeros_like(rest)
+    t0 = time.perf_counter()
+    cpu_ladder(prob, x, probe, a_list=[syn.n_v // 2], L=L64)
+    t_one = time.perf_counter() - t0
+    n_s = int(min(syn.n_v, max(cores, (budget_s - t_rest) / max(t_one, 1e-6) * cores)))
+    a_list = np.unique(np.linspace(0, syn.n_v - 1, n_s).round().astype(np.int64))
+    lad = np.zeros_like(rest)
+    t0 = time.perf_counter()
+    cpu_ladder(prob, x, lad, a_list=a_list, L=L64)
+    t_lad = time.perf_counter() - t0
+    whole = a_list.size == syn.n_v
+    t_sigma = t_rest + t_lad * syn.n_v / a_list.size
+    rel = None
+    if whole and s64_gpu is not None:
+        cs = rest + lad
+        rel = float(np.linalg.norm(cs - s64_gpu) / np.linalg.norm(s64_gpu))
+    blas = [f"{b.get('internal_api')} {b.get('version')}" for b in threadpool_info() if b.get("user_api") == "blas"]
+    return {"value": t_sigma * n_sigma_fp64, "unit": "s", "cores": cores, "kind": "port",
+            "sample": (f"one FP64 sigma of this workload on {cores} host cores: the ladder in C with the "
+                       f"reference's loop structure (OpenMP over a, DGEMM per a, DGEMV per walk; BLAS "
+                       f"{', '.join(blas) or 'unknown'}, one BLAS thread per OpenMP thread) over "
+                       f"{a_list.size} of {syn.n_v} outer virtuals ({t_lad:.1f} s"
+                       f"{'' if whole else ', scaled by the sampled fraction'}), the other terms in numpy "
+                       f"({t_rest:.1f} s); times the {n_sigma_fp64} sigma builds of the FP64 solve"),
+            "sigma_seconds": t_sigma, "ladder_seconds_measured": t_lad, "ladder_outer_virtuals": int(a_list.size),
+            "rest_seconds": t_rest, "integral_assembly_seconds": t_int,
+            "check_rel_err_vs_gpu_fp64": rel}
 
 
 def main() -> int:
@@ -196,18 +253,6 @@ def main() -> int:
            "data": f"synthetic (seeded N(0,1) exp(-J/80) x "
                    f"{a.l_scale if a.l_scale is not None else WORKLOADS[a.workload][6]} Cholesky vectors)"}
 
-    def ladder_probe(prec, n):
-        """ladder and sigma event times of n eager builds (graph replays time only the whole build)"""
-        xp = tci.DeviceArray.from_numpy(random_vector(plan.n_csf, a.seed + 1))
-        yp = tci.DeviceArray(plan.n_csf)
-        plan.sigma(xp, yp, precision=prec)
-        plan.profile(True)
-        for _ in range(n):
-            plan.sigma(xp, yp, precision=prec)
-        r = plan.profile_read(prec)
-        plan.profile(False)
-        return r
-
     cfg["sigma_graph"] = a.graph
     cfg["autotune"] = os.environ.get("TCI_AUTOTUNE", "1") != "0"
     if a.mode == "sigma":
@@ -222,7 +267,7 @@ def main() -> int:
             plan.sigma(x, y, precision=a.precision, graph=a.graph)
         tci.synchronize()
         dt = (time.perf_counter() - t0) / a.steps
-        pr = ladder_probe(a.precision, a.steps) if a.graph else plan.profile_read(a.precision)
+        tab, pr = kernel_table(plan, tci, random_vector(plan.n_csf, a.seed + 1), a.precision, max(a.steps, 3))
         cfg["schedule"] = plan.tuning().strip().splitlines()
         lad = pr["ladder_ms"] / max(pr["ladder_launches"], 1) / 1e3
         if a.precision != "fp64":  # precision study: this sigma against the FP64 one
@@ -232,10 +277,15 @@ def main() -> int:
             xv = x.numpy()
             out["sigma_check"] = {"sigma_rel_err": float(np.linalg.norm(s_mx - s_64) / np.linalg.norm(s_64)),
                                   "rayleigh_err_Eh": float(abs(xv @ s_mx - xv @ s_64))}
+        dom = max(tab, key=lambda k: k["ms_per_sigma"])
         out.update({"value": info.flops_sigma / dt / 1e12, "unit": "TFLOP/s", "higher_is_better": True,
                     "ms_per_step": dt * 1e3, "dtype": a.precision, "config": cfg,
-                    "roofline": roofline(info, syn, a.precision, lad, measured_traffic(a.workload, a.precision),
-                                         f"{a.precision} ladder (per sigma)", fused_ranges(plan.tuning()))})
+                    "roofline": kernel_roofline(dom, a.precision, kernel_traffic(a.workload, a.precision, dom["name"])),
+                    "ladder_section": ladder_section(info, syn, a.precision, lad, fused_ranges(plan.tuning())),
+                    "kernels": [dict(name=k["name"], ms_per_sigma=k["ms_per_sigma"],
+                                     launches_per_sigma=k["launches_per_sigma"],
+                                     tflops=(k["flops"] * k["launches"] / k["ms"] / 1e9) if k["ms"] > 0 else None)
+                                for k in sorted(tab, key=lambda k: -k["ms_per_sigma"])[:8]]})
         if a.copy_probe_mib:
             n = a.copy_probe_mib * (1 << 20) // 8
             src, dst = tci.DeviceArray(n), tci.DeviceArray(n)
@@ -286,12 +336,15 @@ def main() -> int:
     t_64 = time.perf_counter() - t0
     p64 = plan.profile_read("fp64")
     log(f"[bench] fp64 solve: {res_64} E={e_64}")
-    if a.graph:
-        sig_ms = {k: v["sigma_ms"] / max(v["sigma_calls"], 1) for k, v in (("fp64", p64), ("mixed", pm))}
-        p64 = dict(ladder_probe("fp64", 3), sigma_ms=p64["sigma_ms"], sigma_calls=p64["sigma_calls"])
-        if a.precision != "fp64":
-            pm = dict(ladder_probe(a.precision, 3), sigma_ms=pm["sigma_ms"], sigma_calls=pm["sigma_calls"])
-        log(f"[bench] graph sigma ms {sig_ms}")
+    # per-kernel tables from eager profiled builds after the timed region
+    # (graph replays time only whole builds)
+    xk = random_vector(plan.n_csf, a.seed + 1)
+    tab64, lp64 = kernel_table(plan, tci, xk, "fp64", 3)
+    p64 = dict(lp64, sigma_ms=p64["sigma_ms"], sigma_calls=p64["sigma_calls"])
+    tabm = []
+    if a.precision != "fp64":
+        tabm, lpm = kernel_table(plan, tci, xk, a.precision, 3)
+        pm = dict(lpm, sigma_ms=pm["sigma_ms"], sigma_calls=pm["sigma_calls"])
     lad = p64["ladder_ms"] / max(p64["ladder_launches"], 1) / 1e3
     sig64 = p64["sigma_ms"] / max(p64["sigma_calls"], 1) / 1e3
     out.update({
@@ -308,23 +361,29 @@ def main() -> int:
         "sigma_tflops_fp64": info.flops_sigma / sig64 / 1e12,
     })
     out["config"]["schedule"] = plan.tuning().strip().splitlines()
-    roof64 = roofline(info, syn, "fp64", lad, measured_traffic(a.workload, "fp64"),
-                      "FP64 ladder (dgemm phases 1+2, per sigma)")
-    roof64["launches"] = p64["ladder_launches"]
+    # the dominant kernel of the timed step: per-sigma kernel time weighted by
+    # the mixed solve's sigma counts in each precision
+    n_mix, n_ref = res_m["sigma_mixed"], res_m["sigma_fp64"]
+    cands = [(k["ms_per_sigma"] * n_mix, a.precision, k) for k in tabm] + \
+            [(k["ms_per_sigma"] * n_ref, "fp64", k) for k in tab64]
+    _, dprec, dk = max(cands, key=lambda t: t[0])
+    out["roofline"] = kernel_roofline(dk, dprec, kernel_traffic(a.workload, dprec, dk["name"]))
+    out["roofline"]["share_of_step"] = dk["ms_per_sigma"] * (n_mix if dprec != "fp64" else n_ref) / 1e3 / t_m
+    k64 = max(tab64, key=lambda k: k["ms_per_sigma"])
+    out["roofline_fp64"] = kernel_roofline(k64, "fp64", kernel_traffic(a.workload, "fp64", k64["name"]))
+    out["ladder_section"] = {"fp64": ladder_section(info, syn, "fp64", lad)}
+    out["kernels"] = {prec: [dict(name=k["name"], ms_per_sigma=k["ms_per_sigma"],
+                                  launches_per_sigma=k["launches_per_sigma"],
+                                  tflops=(k["flops"] * k["launches"] / k["ms"] / 1e9) if k["ms"] > 0 else None)
+                             for k in sorted(t, key=lambda k: -k["ms_per_sigma"])[:8]]
+                      for prec, t in (("fp64", tab64), (a.precision, tabm)) if t}
     if a.precision != "fp64" and pm["ladder_launches"]:
         lm = pm["ladder_ms"] / pm["ladder_launches"] / 1e3
         sgm = pm["sigma_ms"] / max(pm["sigma_calls"], 1) / 1e3
         out["sigma_tflops_mixed"] = info.flops_sigma / sgm / 1e12
-        fz = fused_ranges(plan.tuning())
-        kind = "two-phase xgemm" + "".join(f", fused {lo}-{hi}" for lo, hi in fz)
-        out["roofline"] = roofline(info, syn, a.precision, lm, measured_traffic(a.workload, a.precision),
-                                   f"{a.precision} ladder ({kind}; per sigma)", fz)
-        out["roofline"]["launches"] = pm["ladder_launches"]
-        out["roofline_fp64"] = roof64
-    else:
-        out["roofline"] = roof64
+        out["ladder_section"][a.precision] = ladder_section(info, syn, a.precision, lm, fused_ranges(plan.tuning()))
     if not a.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(syn, plan, res_64["sigma_fp64"])
+        out["cpu_baseline"] = cpu_baseline(syn, xr, s64, res_64["sigma_fp64"])
     print(json.dumps(out), flush=True)
     return 0
 

@@ -61,15 +61,21 @@ enum tci_precision {
  * n_int..n_int+n_v-1.  `chol` holds one record of n_chol values per orbital
  * pair p>=q, record index p(p+1)/2+q (the MO Cholesky record layout of
  * ThreeIndexWriter.cpp:18-35 / cholesky_structs.f90:35-39, 0-based).
- * eps are the orbital energies: the core Hamiltonian is the one whose
- * closed-shell Fock matrix is diag(eps).  Domains are sorted virtual indices
- * (0-based over the virtual space) per strong pair walk (allowed virtuals,
+ * One-electron part: either `h`, the one-electron integrals in the packed
+ * lower-triangle order of the reference's pp array (element (p,q), p >= q, at
+ * p(p+1)/2+q; getpp, get_integrals_mod.f90:61-82, index2 of utilities_mod.f90),
+ * from which the plan builds the closed-shell Fock matrix f = h + G on the
+ * device (localized orbitals: off-diagonal f_ij, f_ia and f_ab all enter
+ * sigma); or, when h is NULL, `eps`, orbital energies of canonical orbitals
+ * (the core Hamiltonian is then the one whose closed-shell Fock matrix is
+ * diag(eps)).  Domains are sorted virtual indices (0-based over the virtual
+ * space) per strong pair walk (allowed virtuals,
  * allowed_virtuals_mod.f90:163-224); pairs left out carry no doubles. */
 typedef struct tci_problem {
   int64_t n_int;
   int64_t n_v;
   int64_t n_chol;
-  const double* eps;       /* n_int + n_v */
+  const double* eps;       /* n_int + n_v orbital energies (ignored when h is given) */
   const void* chol;        /* (n_orb(n_orb+1)/2) x n_chol, dtype chol_dtype */
   int32_t chol_dtype;      /* TCI_F64 or TCI_F32 */
   int32_t ladder_dtype;    /* device storage of L^J_ab for the ladder: TCI_F64 or TCI_F32 */
@@ -77,6 +83,7 @@ typedef struct tci_problem {
   const int32_t* pairs;    /* n_pairs x 2: (i, j), i >= j */
   const int64_t* dom_off;  /* n_pairs + 1 */
   const int32_t* dom_idx;  /* dom_off[n_pairs] entries */
+  const double* h;         /* NULL, or n_orb(n_orb+1)/2 packed one-electron integrals */
 } tci_problem;
 
 typedef struct tci_plan tci_plan;
@@ -144,6 +151,7 @@ typedef struct tci_davidson_result {
   int32_t converged;      /* all roots met a convergence criterion before maxit */
   double seconds;         /* wall time of the solve */
   double seconds_sigma;   /* of which inside sigma builds */
+  int32_t collapses;      /* subspace collapses on the vector budget (restarts) */
 } tci_davidson_result;
 
 /* Davidson-Liu for the lowest roots (davidson_liu, davidson.f90:47-239).
@@ -194,6 +202,14 @@ int tci_device_synchronize(void);
 int tci_profile_enable(tci_plan* plan, int32_t on);
 int tci_profile_read(const tci_plan* plan, int32_t precision, double* ladder_ms,
                      int64_t* ladder_launches, double* sigma_ms, int64_t* sigma_calls);
+
+/* Per-kernel event timing of the same profiled sigma builds: one text line per
+ * kernel role ("name|launches|total_ms|algorithmic_flops_per_launch"), for the
+ * builds of one precision since profiling was enabled.  The algorithmic flops
+ * of a launch count 2 m n k over the valid (unpadded) extents of its
+ * products.  Copies at most len-1 characters plus a terminating 0.  No
+ * reference counterpart. */
+int tci_profile_kernels(const tci_plan* plan, int32_t precision, char* buf, int64_t len);
 
 /* Launch schedules the plan picked by timing candidates on the device: K-split
  * fill targets and workgroup orders of the ladder programs (FP64 at plan

@@ -50,6 +50,9 @@ class Problem:
     pairs: np.ndarray          # (n_pairs, 2) int32 strong pairs (i>=j), walk order
     dom_off: np.ndarray        # (n_pairs+1,) int64 offsets into dom_idx
     dom_idx: np.ndarray        # (sum d_P,) int32 sorted virtual indices
+    # one-electron integrals h_pq (n_orb, n_orb), symmetric; None: the core
+    # Hamiltonian whose closed-shell Fock matrix is diag(eps)
+    h: np.ndarray | None = None
 
     # ---- derived -------------------------------------------------------
     @property
@@ -114,7 +117,8 @@ class Layout:
 def make_problem(n_int: int, n_v: int, n_chol: int, seed: int,
                  domain_range: tuple[int, int] | None = None,
                  weak_below: int = 0,
-                 decay: float = 80.0, l_scale: float = 1.0) -> Problem:
+                 decay: float = 80.0, l_scale: float = 1.0,
+                 h_offdiag: float = 0.0) -> Problem:
     """Seeded synthetic problem.
 
     * eps_i ~ U[-1.5,-0.3], eps_a ~ U[0.5,5.0]
@@ -124,6 +128,10 @@ def make_problem(n_int: int, n_v: int, n_chol: int, seed: int,
     * full pair domains unless ``domain_range=(lo,hi)``: then each pair's
       domain is a sorted random subset of size U[lo,hi] (inclusive) and pairs
       with size < ``weak_below`` carry no doubles.
+    * ``h_offdiag > 0``: explicit one-electron integrals h = h_canonical + V,
+      V symmetric with N(0,1) * h_offdiag entries (stream [seed,4]), so the
+      closed-shell Fock matrix diag(eps) + V has off-diagonal occupied-occupied,
+      occupied-virtual and virtual-virtual blocks (localized orbitals).
     Independent sub-streams keep each piece reproducible on its own.
     """
     n_orb = n_int + n_v
@@ -152,7 +160,22 @@ def make_problem(n_int: int, n_v: int, n_chol: int, seed: int,
     dom_off = np.zeros(len(doms) + 1, np.int64)
     dom_off[1:] = np.cumsum([len(d) for d in doms])
     dom_idx = (np.concatenate(doms) if doms else np.zeros(0)).astype(np.int32)
-    return Problem(n_int, n_v, n_chol, eps, L, pairs_a, dom_off, dom_idx)
+    prob = Problem(n_int, n_v, n_chol, eps, L, pairs_a, dom_off, dom_idx)
+    if h_offdiag > 0.0:
+        V = np.random.default_rng([seed, 4]).standard_normal((n_orb, n_orb)) * h_offdiag
+        prob.h = core_hamiltonian(prob) + 0.5 * (V + V.T)
+    return prob
+
+
+def packed_lower(M: np.ndarray) -> np.ndarray:
+    """Symmetric matrix -> packed lower triangle, element (p,q), p>=q, at
+    p(p+1)/2+q: the layout of the reference's one-electron integral array
+    pp(num_orbitals(num_orbitals+1)/2) (F/get_integrals_mod.f90 getpp,
+    index2 of F/utilities_mod.f90 shifted to 0-based)."""
+    r, c = np.tril_indices(M.shape[0])
+    out = np.zeros(M.shape[0] * (M.shape[0] + 1) // 2)
+    out[r * (r + 1) // 2 + c] = M[r, c]
+    return out
 
 
 def random_ci_vector(prob: Problem, seed: int) -> np.ndarray:
@@ -173,20 +196,38 @@ def eri_full(prob: Problem) -> np.ndarray:
     return (M.T @ M).reshape(n, n, n, n)
 
 
+def closed_shell_g(prob: Problem) -> np.ndarray:
+    """G_pq = sum_k [2(pq|kk) - (pk|kq)] over the doubly occupied internal
+    orbitals k, from the Cholesky vectors (no four-index array)."""
+    Lf = prob.L_full()
+    o = slice(0, prob.n_int)
+    dJ = np.einsum('Jkk->J', Lf[:, o, o])
+    Lo = Lf[:, :, o]                                   # (J, p, k)
+    return 2.0 * np.einsum('J,Jpq->pq', dJ, Lf) - np.einsum('Jpk,Jqk->pq', Lo, Lo)
+
+
 def core_hamiltonian(prob: Problem, eri: np.ndarray | None = None) -> np.ndarray:
-    """h_pq such that the closed-shell Fock matrix is diag(eps):
-    f_pq = h_pq + sum_k [2(pq|kk) - (pk|kq)]  ->  h = diag(eps) - G."""
+    """h_pq: ``prob.h`` when given, otherwise the one whose closed-shell Fock
+    matrix is diag(eps): f_pq = h_pq + G_pq  ->  h = diag(eps) - G."""
+    if prob.h is not None:
+        return np.array(prob.h, dtype=np.float64)
     if eri is None:
-        eri = eri_full(prob)
+        return np.diag(prob.eps) - closed_shell_g(prob)
     o = slice(0, prob.n_int)
     G = 2.0 * np.einsum('pqkk->pq', eri[:, :, o, o]) - np.einsum('pkkq->pq', eri[:, o, o, :])
     return np.diag(prob.eps) - G
 
 
+def fock(prob: Problem) -> np.ndarray:
+    """Closed-shell Fock matrix f = h + G (diag(eps) when no h is given)."""
+    if prob.h is None:
+        return np.diag(prob.eps).astype(np.float64)
+    return np.array(prob.h, dtype=np.float64) + closed_shell_g(prob)
+
+
 def reference_energy(prob: Problem, eri: np.ndarray | None = None) -> float:
-    """E_ref = sum_i (h_ii + eps_i) of the closed-shell reference."""
-    if eri is None:
-        eri = eri_full(prob)
+    """E_ref = sum_i (h_ii + f_ii) of the closed-shell reference."""
     h = core_hamiltonian(prob, eri)
+    f = fock(prob)
     o = np.arange(prob.n_int)
-    return float(np.sum(h[o, o] + prob.eps[o]))
+    return float(np.sum(h[o, o] + f[o, o]))

@@ -44,13 +44,13 @@ from __future__ import annotations
 import math
 import numpy as np
 
-from .problem import Problem, Layout
+from .problem import Problem, Layout, fock
 from .scepper import (scepper_p1, scepper_m1, unscepper_p1, unscepper_m12,
                       SQRT2)
 
 SQRT3 = math.sqrt(3.0)
 ALL_TERMS = frozenset({"ref", "fock", "ladder", "hole", "ring", "s3ext",
-                       "s3int", "ss"})
+                       "s3int", "ss", "fov"})
 
 
 class Integrals:
@@ -71,6 +71,7 @@ class Integrals:
         # O3[k,i,b,j] = (ki|bj)
         self.O3 = (self.Loo.reshape(nc, -1).T @ self.Lov.transpose(0, 2, 1).reshape(nc, -1)
                    ).reshape(no, no, nv, no)
+        self.F = fock(prob)                           # closed-shell Fock matrix f = h + G
 
 
 def unpack(prob: Problem, x: np.ndarray, lay: Layout | None = None):
@@ -126,7 +127,8 @@ def sigma(prob: Problem, x: np.ndarray, ints: Integrals | None = None,
     ints = ints or Integrals(prob)
     lay = prob.layout()
     no, nv = prob.n_int, prob.n_v
-    eo, ev = prob.eps[:no], prob.eps[no:]
+    F = ints.F
+    Foo, Fov, Fvv = F[:no, :no], F[:no, no:], F[no:, no:]
     c0, s, C = unpack(prob, x, lay)
     Ct = 2.0 * C - C.transpose(0, 1, 3, 2)
     K, J = ints.K, ints.J
@@ -136,9 +138,15 @@ def sigma(prob: Problem, x: np.ndarray, ints: Integrals | None = None,
     if "ref" in terms:
         R += c0 * K
     if "fock" in terms:
-        de = (ev[None, None, :, None] + ev[None, None, None, :]
-              - eo[:, None, None, None] - eo[None, :, None, None])
-        R += de * C
+        # sum_c f_ac C_cb + C_ac f_cb - sum_k f_ki C^{kj} - C^{ik} f_kj; with a
+        # diagonal Fock matrix this is (e_a+e_b-e_i-e_j) C^{ij}
+        R += (np.einsum('ac,ijcb->ijab', Fvv, C, optimize=True)
+              + np.einsum('ijac,cb->ijab', C, Fvv, optimize=True)
+              - np.einsum('ki,kjab->ijab', Foo, C, optimize=True)
+              - np.einsum('kj,ikab->ijab', Foo, C, optimize=True))
+    if "fov" in terms:
+        # doubles <- singles through the occupied-virtual Fock block
+        R += np.einsum('ia,jb->ijab', s, Fov) + np.einsum('ia,jb->ijab', Fov, s)
     if "hole" in terms:
         R += np.einsum('kilj,klab->ijab', ints.O4, C, optimize=True)
     if "ring" in terms:
@@ -170,7 +178,10 @@ def sigma(prob: Problem, x: np.ndarray, ints: Integrals | None = None,
     # ---- singles -----------------------------------------------------------
     g = np.zeros((no, nv))
     if "fock" in terms:
-        g += (ev[None, :] - eo[:, None]) * s
+        g += s @ Fvv - Foo @ s
+    if "fov" in terms:
+        # singles <- reference and singles <- doubles through f_ia
+        g += c0 * Fov + np.einsum('jb,ijab->ia', Fov, Ct, optimize=True)
     if "ss" in terms:
         g += np.einsum('ijab,jb->ia', 2.0 * K - J, s, optimize=True)
     if "s3ext" in terms:
@@ -182,6 +193,8 @@ def sigma(prob: Problem, x: np.ndarray, ints: Integrals | None = None,
 
     # ---- reference ---------------------------------------------------------
     sig0 = float(np.sum(K * Ct)) if "ref" in terms else 0.0
+    if "fov" in terms:
+        sig0 += 2.0 * float(np.sum(Fov * s))
     return pack(prob, sig0, sig_s, Rd, lay)
 
 
@@ -191,7 +204,8 @@ def diagonal(prob: Problem, ints: Integrals | None = None) -> np.ndarray:
     ints = ints or Integrals(prob)
     lay = prob.layout()
     no, nv = prob.n_int, prob.n_v
-    eo, ev = prob.eps[:no], prob.eps[no:]
+    fd = np.diag(ints.F)                              # only the diagonal Fock elements enter H_ii
+    eo, ev = fd[:no], fd[no:]
     Lvv = ints.Lvv
     vv_diag = np.einsum('Jaa,Jbb->ab', Lvv, Lvv)        # (aa|bb)
     vv_exch = np.einsum('Jab,Jab->ab', Lvv, Lvv)        # (ab|ab)

@@ -25,6 +25,7 @@ program abi_probe
   prob%pairs = c_null_ptr
   prob%dom_off = c_null_ptr
   prob%dom_idx = c_null_ptr
+  prob%h = c_null_ptr
   rc = tci_plan_create(prob, plan)
   write (*, '(a,i0)') 'status ', rc
   write (*, '(a,a)') 'error ', trim(tci_last_error_string())

@@ -63,17 +63,23 @@ def test_fused_ranges_from_tuning_record():
     assert b.fused_ranges(t3) == [(65, 128), (129, 192)]
 
 
-def test_roofline_picks_the_binding_roof():
+def test_kernel_roofline_and_ladder_section():
+    """A kernel's roofline is its algorithmic flops per launch over its mean
+    launch time against the dense MFMA peak of its precision; the ladder
+    section is F_lad over the section time, with the L^J[D,D]-once byte floor."""
     b = _bench()
-    syn = _syn(192, [192] * 0 + [150] * 100, 1000)
+    k = dict(name="xgemm_kernel<128,192>", launches=6, ms=18.0, flops=1.5e12, launches_per_sigma=6.0)
+    r = b.kernel_roofline(k, "bf16x3", 3.0e9)
+    t = 18.0 / 6 / 1e3
+    assert r["bound"] == "mfma" and r["unit"] == "TFLOP/s"
+    assert abs(r["achieved"] - 1.5e12 / t / 1e12) < 1e-9 and abs(r["frac"] - 1.5e12 / t / 2.5e15) < 1e-12
+    assert abs(r["mfma_busy_frac"] - 3 * r["frac"]) < 1e-12 and abs(r["traffic_gbs"] - 3.0e9 / t / 1e9) < 1e-9
+    r64 = b.kernel_roofline(dict(k, name="dgemm_kernel<64>"), "fp64", None)
+    assert abs(r64["frac"] - 1.5e12 / t / 78.6e12) < 1e-12 and "mfma_busy_frac" not in r64
+    syn = _syn(192, [150] * 100, 1000)
     info = SimpleNamespace(flops_ladder=4.0 * 100 * 150 ** 3 * 1000)
-    alg = b.ladder_alg_bytes(syn, "bf16x3")
-    r = b.roofline(info, syn, "bf16x3", 0.05, None, "x")
-    # 3 MFMAs per split product at 2.5 PF/s against the minimum bytes at 8 TB/s
-    if alg / 8e12 > 3 * info.flops_ladder / 2.5e15:
-        assert r["bound"] == "hbm" and abs(r["achieved"] - alg / 0.05 / 1e9) < 1e-6 and r["unit"] == "GB/s"
-    else:
-        assert r["bound"] == "mfma"
-    r64 = b.roofline(info, syn, "fp64", 0.2, 1e12, "x")
-    assert r64["bound"] == "mfma" and abs(r64["frac"] - info.flops_ladder / 0.2 / 78.6e12) < 1e-12
-    assert abs(r64["traffic_gbs"] - 1e12 / 0.2 / 1e9) < 1e-9
+    sec = b.ladder_section(info, syn, "bf16x3", 0.05)
+    assert abs(sec["frac_mfma"] - info.flops_ladder / 0.05 / 2.5e15) < 1e-12
+    assert sec["byte_floor"] == 4.0 * 100 * 150 ** 2 * 1000           # L^J[D,D] once, 4 bytes split
+    assert b.ladder_floor_bytes(syn, "fp64") == 8.0 * 100 * 150 ** 2 * 1000
+    assert sec["schedule_bytes"] > sec["byte_floor"]                    # T round trip and two reads of L

@@ -103,8 +103,16 @@ def test_sigma_config1_size():
     prob = make_problem(10, 100, 400, 2024)
     plan = tci.Plan.from_problem(prob)
     x = random_ci_vector(prob, 1)
-    ref = oracle_sigma(prob, x)
-    assert _rel(_gpu_sigma(plan, x), ref) < TOL
+    ints = Integrals(prob)
+    ref = oracle_sigma(prob, x, ints)
+    got = _gpu_sigma(plan, x)
+    assert _rel(got, ref) < TOL
+    # the CPU baseline's sigma (the ladder in C with the reference's loop
+    # structure) agrees with the GPU FP64 sigma as well
+    from oracle.cpu_sigma import cpu_sigma
+    cs = cpu_sigma(prob, x, ints)
+    _say(f"  GPU vs oracle {_rel(got, ref):.2e}, GPU vs CPU baseline sigma {_rel(got, cs):.2e}")
+    assert _rel(got, cs) < TOL
 
 
 def test_dense_pair_path_matches_gather_path():

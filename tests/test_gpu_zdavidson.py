@@ -50,3 +50,26 @@ def test_davidson_six_roots_vs_dense(precision):
     e, res = plan.davidson(roots=6, energy_tol=1e-11, norm_tol=1e-8, precision=precision)
     assert res["converged"]
     assert np.abs(e - plan.info.e_ref - ref).max() < 1e-8
+
+
+@pytest.mark.parametrize("roots,max_vectors,precision", [(2, 10, "fp64"), (3, 15, "fp64"), (2, 12, "bf16x3")])
+def test_davidson_budget_collapse(roots, max_vectors, precision):
+    """A subspace budget of 5-6 vectors per root forces collapses to the
+    current and previous Ritz vectors of every root (davidson_restart.f90),
+    including at budgets below 7 per root, where the collapse builds its
+    vectors in the store beyond the budget.  Roots against the dense spectrum
+    and the oracle Davidson with the same budget."""
+    print("case: davidson collapse", roots, max_vectors, precision, flush=True)
+    prob = make_problem(3, 10, 12, 7)
+    ints = Integrals(prob)
+    ref = np.linalg.eigvalsh(dense_hamiltonian(prob, ints))[:roots]
+    E, X, hist = oracle_davidson(lambda v: oracle_sigma(prob, v, ints), oracle_diagonal(prob, ints),
+                                 roots=roots, energy_tol=1e-10, norm_tol=1e-7, max_vectors=max_vectors)
+    plan = tci.Plan.from_problem(prob)
+    e, res = plan.davidson(roots=roots, energy_tol=1e-10, norm_tol=1e-7, max_vectors=max_vectors,
+                           precision=precision)
+    print(f"  {res['iterations']} iterations, {res['collapses']} collapses, "
+          f"root error {np.abs(e - plan.info.e_ref - ref).max():.2e}", flush=True)
+    assert res["converged"] and res["collapses"] >= 1
+    assert np.abs(e - plan.info.e_ref - ref).max() < 1e-8
+    assert np.abs(E - ref).max() < 1e-8

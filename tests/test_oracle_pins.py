@@ -22,6 +22,10 @@ CASES = [
     dict(n_int=2, n_v=3, n_chol=6, seed=1),
     dict(n_int=3, n_v=3, n_chol=8, seed=2),
     dict(n_int=3, n_v=5, n_chol=10, seed=3, domain_range=(2, 5), weak_below=3),
+    # explicit one-electron integrals with off-diagonal Fock blocks (oo, ov, vv)
+    dict(n_int=2, n_v=3, n_chol=6, seed=4, h_offdiag=0.3),
+    dict(n_int=3, n_v=4, n_chol=8, seed=5, h_offdiag=0.2),
+    dict(n_int=3, n_v=5, n_chol=10, seed=6, domain_range=(2, 5), weak_below=3, h_offdiag=0.25),
 ]
 
 
@@ -55,8 +59,9 @@ def test_sigma_equals_determinant_ci(case):
 
 
 def test_each_term_is_needed():
-    """Dropping any single term breaks agreement with the determinant CI."""
-    prob = make_problem(n_int=3, n_v=3, n_chol=8, seed=2, decay=4.0)
+    """Dropping any single term breaks agreement with the determinant CI
+    (off-diagonal Fock blocks present, so the occupied-virtual terms count)."""
+    prob = make_problem(n_int=3, n_v=3, n_chol=8, seed=2, decay=4.0, h_offdiag=0.3)
     ci, Hd, e0, U = _brute(prob)
     Hb = U.T @ (Hd - e0 * np.eye(Hd.shape[0])) @ U
     ints = Integrals(prob)

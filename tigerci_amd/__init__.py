@@ -43,7 +43,7 @@ class _Problem(ctypes.Structure):
     _fields_ = [("n_int", ctypes.c_int64), ("n_v", ctypes.c_int64), ("n_chol", ctypes.c_int64),
                 ("eps", ctypes.c_void_p), ("chol", ctypes.c_void_p), ("chol_dtype", ctypes.c_int32),
                 ("ladder_dtype", ctypes.c_int32), ("n_pairs", ctypes.c_int64), ("pairs", ctypes.c_void_p),
-                ("dom_off", ctypes.c_void_p), ("dom_idx", ctypes.c_void_p)]
+                ("dom_off", ctypes.c_void_p), ("dom_idx", ctypes.c_void_p), ("h", ctypes.c_void_p)]
 
 
 class _Info(ctypes.Structure):
@@ -60,7 +60,8 @@ class _DavParams(ctypes.Structure):
 
 class _DavResult(ctypes.Structure):
     _fields_ = [("iterations", ctypes.c_int32), ("sigma_fp64", ctypes.c_int32), ("sigma_mixed", ctypes.c_int32),
-                ("converged", ctypes.c_int32), ("seconds", ctypes.c_double), ("seconds_sigma", ctypes.c_double)]
+                ("converged", ctypes.c_int32), ("seconds", ctypes.c_double), ("seconds_sigma", ctypes.c_double),
+                ("collapses", ctypes.c_int32)]
 
 
 _lib = None
@@ -95,14 +96,16 @@ def load_library(path: str | None = None) -> ctypes.CDLL:
     lib.tci_profile_read.argtypes = [vp, i32, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(i64),
                                      ctypes.POINTER(ctypes.c_double), ctypes.POINTER(i64)]
     lib.tci_plan_tuning.argtypes = [vp, ctypes.c_char_p, i64]
+    lib.tci_profile_kernels.argtypes = [vp, i32, ctypes.c_char_p, i64]
     lib.tci_sigma_apply_batch.argtypes = [vp, vp, vp, i64, i32, i32, vp]
     lib.tci_davidson_gen_run.argtypes = [vp, ctypes.POINTER(_DavParams), vp, vp, vp, ctypes.POINTER(_DavResult)]
     lib.tci_acpf_gvalues.argtypes = [i32, i32, ctypes.c_double, vp]
     lib.tci_last_error.restype = ctypes.c_char_p
-    for f in ("tci_plan_create", "tci_plan_get_info", "tci_sigma_apply", "tci_htimesc", "tci_diagonal",
+    for f in ("tci_plan_create", "tci_plan_get_info", "tci_sigma_apply", "tci_htimesc",
+              "tci_diagonal",
               "tci_davidson_run", "tci_selftest_mfma", "tci_device_alloc", "tci_device_free", "tci_memcpy_h2d",
               "tci_memcpy_d2h", "tci_memcpy_d2d", "tci_device_synchronize", "tci_profile_enable",
-              "tci_profile_read", "tci_plan_tuning", "tci_davidson_gen_run", "tci_acpf_gvalues",
+              "tci_profile_read", "tci_profile_kernels", "tci_plan_tuning", "tci_davidson_gen_run", "tci_acpf_gvalues",
               "tci_sigma_apply_batch"):
         getattr(lib, f).restype = ctypes.c_int
     if path is None:
@@ -217,24 +220,43 @@ class Plan:
     """Device plan of one CI problem (see the module docstring)."""
 
     def __init__(self, n_int: int, n_v: int, n_chol: int, eps, chol, pairs, dom_off, dom_idx,
-                 ladder_dtype: str = "f64"):
+                 ladder_dtype: str = "f64", h=None):
+        """``h``: optional one-electron integrals, an (n_orb, n_orb) symmetric
+        matrix or its packed lower triangle (the reference's pp order); when
+        given, the Fock matrix is built from it and ``eps`` may be None."""
         self._lib = load_library()
+        n_orb = n_int + n_v
+        hp = None
+        if h is not None:
+            h = np.asarray(h, dtype=np.float64)
+            if h.shape == (n_orb, n_orb):
+                r, c = np.tril_indices(n_orb)
+                hp = np.zeros(n_orb * (n_orb + 1) // 2)
+                hp[r * (r + 1) // 2 + c] = h[r, c]
+            elif h.size == n_orb * (n_orb + 1) // 2:
+                hp = np.ascontiguousarray(h.ravel())
+            else:
+                raise TigerSigmaError("h must be (n_orb, n_orb) or packed n_orb(n_orb+1)/2")
+        if eps is None:
+            if hp is None:
+                raise TigerSigmaError("eps or h is required")
+            eps = np.zeros(n_orb)
         eps = np.ascontiguousarray(eps, dtype=np.float64)
         chol = np.ascontiguousarray(chol)
         if chol.dtype not in (np.float64, np.float32):
             chol = chol.astype(np.float64)
-        n_orb = n_int + n_v
         if chol.size != n_orb * (n_orb + 1) // 2 * n_chol:
             raise TigerSigmaError("chol must hold n_orb(n_orb+1)/2 records of n_chol values")
         pairs = np.ascontiguousarray(pairs, dtype=np.int32).reshape(-1, 2)
         dom_off = np.ascontiguousarray(dom_off, dtype=np.int64)
         dom_idx = np.ascontiguousarray(dom_idx, dtype=np.int32)
-        self._keep = (eps, chol, pairs, dom_off, dom_idx)
+        self._keep = (eps, chol, pairs, dom_off, dom_idx, hp)
         pr = _Problem(n_int, n_v, n_chol, eps.ctypes.data, chol.ctypes.data,
                       TCI_F64 if chol.dtype == np.float64 else TCI_F32,
                       TCI_F64 if ladder_dtype == "f64" else TCI_F32,
                       pairs.shape[0], pairs.ctypes.data if pairs.size else None,
-                      dom_off.ctypes.data, dom_idx.ctypes.data if dom_idx.size else None)
+                      dom_off.ctypes.data, dom_idx.ctypes.data if dom_idx.size else None,
+                      hp.ctypes.data if hp is not None else None)
         h = ctypes.c_void_p()
         _check(self._lib.tci_plan_create(ctypes.byref(pr), ctypes.byref(h)))
         self._h = h
@@ -248,9 +270,9 @@ class Plan:
     @classmethod
     def from_problem(cls, prob, ladder_dtype: str = "f64") -> "Plan":
         """Build from any object with the attributes n_int, n_v, n_chol, eps, L,
-        pairs, dom_off, dom_idx (L: records x n_chol)."""
+        pairs, dom_off, dom_idx (L: records x n_chol) and optionally h."""
         return cls(prob.n_int, prob.n_v, prob.n_chol, prob.eps, prob.L, prob.pairs, prob.dom_off,
-                   prob.dom_idx, ladder_dtype=ladder_dtype)
+                   prob.dom_idx, ladder_dtype=ladder_dtype, h=getattr(prob, "h", None))
 
     def close(self):
         if getattr(self, "_h", None):
@@ -315,6 +337,17 @@ class Plan:
                                           ctypes.byref(sm), ctypes.byref(sn)))
         return dict(ladder_ms=lm.value, ladder_launches=ln.value, sigma_ms=sm.value, sigma_calls=sn.value)
 
+    def profile_kernels(self, precision: str = "fp64") -> list:
+        """Per-kernel event times of the profiled eager builds of one precision:
+        [{name, launches, ms (total), flops (algorithmic, per launch)}]."""
+        buf = ctypes.create_string_buffer(1 << 16)
+        _check(self._lib.tci_profile_kernels(self._h, PRECISION[precision], buf, len(buf)))
+        out = []
+        for line in buf.value.decode().splitlines():
+            name, n, ms, fl = line.rsplit("|", 3)
+            out.append(dict(name=name, launches=int(n), ms=float(ms), flops=float(fl)))
+        return out
+
     def davidson(self, roots: int = 1, energy_tol: float = 1e-6, norm_tol: float = 1e-6,
                  precision: str = "fp64", switch_tol: float | None = None, maxit: int = 0,
                  max_vectors: int = 0, vectors=None, verbose: bool = False, graph: bool = False,
@@ -343,7 +376,8 @@ class Plan:
             _check(self._lib.tci_davidson_gen_run(self._h, ctypes.byref(p), g.ctypes.data, e.ctypes.data, vp,
                                                   ctypes.byref(r)))
         return e, dict(iterations=r.iterations, sigma_fp64=r.sigma_fp64, sigma_mixed=r.sigma_mixed,
-                       converged=bool(r.converged), seconds=r.seconds, seconds_sigma=r.seconds_sigma)
+                       converged=bool(r.converged), seconds=r.seconds, seconds_sigma=r.seconds_sigma,
+                       collapses=r.collapses)
 
     def htimesc_host(self, civec: np.ndarray, sigmavec: np.ndarray) -> None:
         if civec.dtype != np.float64 or sigmavec.dtype != np.float64 or not sigmavec.flags.c_contiguous:

@@ -168,6 +168,7 @@ struct GemmProgram {
   bool scaled = false; // some task has an output scale (set by upload_tables)
   int dma = 0;         // split programs: LDS stages filled by direct loads (0: register-staged)
   int remap = -1;      // workgroup -> task order: 0 plain, 1 XCD-contiguous, -1 default
+  double alg_flops = 0.0;  // algorithmic flops of one launch: 2 m n k over valid extents (upload_tables)
   std::vector<Task> tasks;
   std::vector<Seg> segs;
   DevBuf d_tasks, d_segs;
@@ -177,6 +178,8 @@ struct GemmProgram {
            bool out_split = false, int orow_div = 0, Blocking blk = Blocking(), int oscale = -1);
   void upload_tables(hipStream_t st);
   void launch(hipStream_t st, int64_t J0 = 0) const;
+  // name of the kernel a launch runs, e.g. "xgemmd_kernel<192,192,3>"
+  std::string kernel_name() const;
   // scales: device array the tasks' oscale index into (may be null when unused)
   void launch_split(int precision, hipStream_t st, int64_t J0 = 0, const float* scales = nullptr) const;
 };

@@ -544,7 +544,7 @@ int davidson_impl(tci_plan* plan, const tci_davidson_params* prm, const double* 
     hipStream_t st = D.st;
     auto t_start = std::chrono::steady_clock::now();
     double t_sigma = 0.0;
-    int n64 = 0, nmix = 0;
+    int n64 = 0, nmix = 0, n_collapse = 0;
     DevBuf Vb, Sb, work;
     Vb.alloc((size_t)mslots * n * 8);
     Sb.alloc((size_t)mslots * n * 8);
@@ -805,6 +805,7 @@ int davidson_impl(tci_plan* plan, const tci_davidson_params* prm, const double* 
           energy[r] = w[r];
         }
         restarted = !do_switch;
+        if (restarted) ++n_collapse;
       }
       restarts.push_back(restarted ? 1 : 0);
       Yprev = Y;
@@ -928,6 +929,7 @@ int davidson_impl(tci_plan* plan, const tci_davidson_params* prm, const double* 
       result->converged = std::all_of(met.begin(), met.end(), [](char c) { return c != 0; }) ? 1 : 0;
       result->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
       result->seconds_sigma = t_sigma;
+      result->collapses = n_collapse;
     }
     return TCI_OK;
   } catch (const tci::Error& e) {

@@ -447,8 +447,22 @@ void GemmProgram::add(double* C, int64_t ldc, int M, int N, const std::vector<Se
 void GemmProgram::upload_tables(hipStream_t st) {
   scaled = false;
   for (const Task& t : tasks) scaled = scaled || t.oscale >= 0;
+  alg_flops = 0.0;
+  const int tm = tile, tn = (split && tile_n > 0) ? tile_n : tile;
+  for (const Task& t : tasks) {
+    const double mv = std::min(tm, t.M - t.m0), nv = std::min(tn, t.N - t.n0);
+    for (int q = t.seg0; q < t.seg1; ++q) alg_flops += 2.0 * mv * nv * segs[q].K;
+  }
   upload(d_tasks, tasks, st);
   upload(d_segs, segs, st);
+}
+
+std::string GemmProgram::kernel_name() const {
+  const std::string t = std::to_string(tile), tn = std::to_string(tile_n > 0 ? tile_n : tile);
+  if (split) return dma ? "xgemmd_kernel<" + t + "," + tn + "," + std::to_string(dma) + ">"
+                        : "xgemm_kernel<" + t + "," + tn + ">";
+  if (dense) return "dgemm_kernel<" + t + ">";
+  return "ggemm" + t + "_kernel";
 }
 
 void GemmProgram::launch(hipStream_t st, int64_t J0) const {

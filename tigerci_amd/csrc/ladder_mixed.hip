@@ -229,8 +229,10 @@ void mixed_ladder_chunk(tci_plan& P, int precision, const std::pair<int64_t, int
   const int f16 = precision == TCI_SIGMA_FP16X3 ? 1 : 0;
   const float* sc = M.scales.as<float>();
   const tci_plan::LadderPhase& ph = M.phase[ch.second];
+  const std::string pn = precision == TCI_SIGMA_FP16X3 ? "fp16x3" : "bf16x3";
   if (M.n_local && !M.prepacked) {
     const unsigned gxb = 1024;
+    const int e0 = P.kt_begin(st);
     if (P.lvv_f32)
       hipLaunchKernelGGL(pack_L_split_kernel<float>, dim3(gxb, M.n_local), dim3(256), 0, st, P.tab,
                          M.pairs_local.as<int32_t>(), ph.lp_off.as<int64_t>(), static_cast<const float*>(P.Lvv.p),
@@ -240,15 +242,19 @@ void mixed_ladder_chunk(tci_plan& P, int precision, const std::pair<int64_t, int
                          M.pairs_local.as<int32_t>(), ph.lp_off.as<int64_t>(), static_cast<const double*>(P.Lvv.p),
                          ch.first, (int)ph.Lc, M.Lpx.as<uint16_t>(), sc, f16);
     TCI_HIP(hipGetLastError());
+    P.kt_end(e0, pn + " ladder pack_L_split_kernel", 0.0, precision, st);
   }
+  auto run = [&](const std::string& role, const GemmProgram& g) {
+    P.kt_run(pn + " ladder " + role, g, precision, st, [&] { g.launch_split(precision, st, ch.first, sc); });
+  };
   if (M.merged) {
-    ph.p1.launch_split(precision, st, ch.first, sc);
-    ph.p2.launch_split(precision, st, ch.first, sc);
+    run("phase 1", ph.p1);
+    run("phase 2", ph.p2);
   } else {
-    for (const auto& g : ph.p1x) g.launch_split(precision, st, ch.first, sc);
-    for (const auto& g : ph.pp1) g.launch_split(precision, st, ch.first, sc);
-    for (const auto& g : ph.p2x) g.launch_split(precision, st, ch.first, sc);
-    for (const auto& g : ph.pp2) g.launch_split(precision, st, ch.first, sc);
+    for (const auto& g : ph.p1x) run("phase 1", g);
+    for (const auto& g : ph.pp1) run("phase 1 (one walk)", g);
+    for (const auto& g : ph.p2x) run("phase 2", g);
+    for (const auto& g : ph.pp2) run("phase 2 (one walk)", g);
   }
 }
 
@@ -893,15 +899,32 @@ void mixed_ladder_run(tci_plan& P, int precision, hipStream_t st) {
   TCI_HIP(hipGetLastError());
   double* Rz = M.rpart_n > 1 ? M.Rpart.as<double>() : P.R.as<double>();
   launch_fill(Rz, (int64_t)M.rpart_n * P.sum_d2, 0.0, st);
-  if (M.fused)
+  // algorithmic flops of a fused task list: 4 d^2 (panel width) per Cholesky vector
+  auto fflops = [](const std::vector<FusedTask>& v, bool panels) {
+    double f = 0.0;
+    for (const FusedTask& t : v)
+      f += 4.0 * t.d * t.d * (panels ? std::min(64, t.d - t.a0) : t.d) * (double)(t.j1 - t.j0);
+    return f;
+  };
+  const std::string pn = precision == TCI_SIGMA_FP16X3 ? "fp16x3" : "bf16x3";
+  if (M.fused) {
+    const int e0 = P.kt_begin(st);
     launch_xladder_fused(precision, M.fused_d.as<FusedTask>(), (int64_t)M.fused_h.size(), sc, st,
                          M.fused_remap < 0 || xcd_remap_forced() ? xcd_remap_default() : M.fused_remap);
-  if (M.fused2)
+    if (e0 >= 0) P.kt_end(e0, pn + " ladder xladder_fused_kernel (129-192 panels)", fflops(M.fused_h, true), precision, st);
+  }
+  if (M.fused2) {
+    const int e0 = P.kt_begin(st);
     launch_xladder_fused128(precision, M.fused2_d.as<FusedTask>(), (int64_t)M.fused2_h.size(), sc, st,
                             xcd_remap_forced() ? xcd_remap_default() : M.fused2_remap, M.fused2_waves);
-  if (M.fused3)
+    if (e0 >= 0) P.kt_end(e0, pn + " ladder xladder_fused128_kernel", fflops(M.fused2_h, false), precision, st);
+  }
+  if (M.fused3) {
+    const int e0 = P.kt_begin(st);
     launch_xladder_fused192(precision, M.fused3_d.as<FusedTask>(), (int64_t)M.fused3_h.size(), sc, st,
                             xcd_remap_forced() ? xcd_remap_default() : M.fused3_remap);
+    if (e0 >= 0) P.kt_end(e0, pn + " ladder xladder_fused192_kernel", fflops(M.fused3_h, false), precision, st);
+  }
   for (const auto& ch : M.chunks) mixed_ladder_chunk(P, precision, ch, st);
   if (P.timing()) TCI_HIP(hipEventRecord(P.ev[3], st));
   if (M.rpart_n > 1) ladder_reduce(M.Rpart.as<double>(), P.sum_d2, M.rpart_n, P.R.as<double>(), P.sum_d2, st);

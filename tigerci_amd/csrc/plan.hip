@@ -69,7 +69,7 @@ using namespace tci;
 void tci_plan::build(const tci_problem& p) {
   TCI_REQUIRE(p.n_int >= 1 && p.n_v >= 1 && p.n_chol >= 1, "n_int, n_v and n_chol must be positive");
   TCI_REQUIRE(p.n_int <= 256 && p.n_v <= 65536, "orbital counts out of range");
-  TCI_REQUIRE(p.eps && p.chol, "eps and chol are required");
+  TCI_REQUIRE((p.eps || p.h) && p.chol, "chol and one of eps or h are required");
   TCI_REQUIRE(p.chol_dtype == TCI_F64 || p.chol_dtype == TCI_F32, "chol_dtype must be TCI_F64 or TCI_F32");
   TCI_REQUIRE(p.ladder_dtype == TCI_F64 || p.ladder_dtype == TCI_F32, "ladder_dtype must be TCI_F64 or TCI_F32");
   TCI_REQUIRE(p.n_pairs >= 0 && (p.n_pairs == 0 || (p.pairs && p.dom_off && p.dom_idx)), "pair tables missing");
@@ -80,7 +80,9 @@ void tci_plan::build(const tci_problem& p) {
   npair_all = n_orb * (n_orb + 1) / 2;
   n_pairs = (int)p.n_pairs;
   lvv_f32 = p.ladder_dtype == TCI_F32;
-  h_eps.assign(p.eps, p.eps + n_orb);
+  explicit_h = p.h != nullptr;
+  if (explicit_h) h_eps.assign(n_orb, 0.0);  // diag(F), set once F is built
+  else h_eps.assign(p.eps, p.eps + n_orb);
   h_pairs.assign(p.pairs, p.pairs + 2 * n_pairs);
   h_dom_off.assign(p.dom_off, p.dom_off + n_pairs + 1);
   TCI_REQUIRE(h_dom_off[0] == 0, "dom_off[0] must be 0");
@@ -220,22 +222,87 @@ void tci_plan::build(const tci_problem& p) {
     launch_vv_exch(Lvv.p, lvv_f32, n_chol, n_v, vvx.as<double>(), st);
     Mss.alloc((size_t)niv * niv * 8);
     launch_build_mss(Kov.as<double>(), Jvv.as<double>(), n_int, n_v, Mss.as<double>(), st);
-    // E_ref = 2 sum_i eps_i - sum_ik [2 (ii|kk) - (ik|ik)]  (closed shell, Fock = diag(eps))
     TCI_HIP(hipMemcpy(o4.data(), O4.p, o4.size() * 8, hipMemcpyDeviceToHost));
-    auto O = [&](int a, int b, int c, int d) { return o4[(((size_t)a * n_int + b) * n_int + c) * n_int + d]; };
-    double e = 0.0;
-    for (int i = 0; i < n_int; ++i) {
-      e += 2.0 * h_eps[i];
-      for (int k = 0; k < n_int; ++k) e -= 2.0 * O(i, i, k, k) - O(i, k, i, k);
+    if (!explicit_h) {
+      // E_ref = 2 sum_i eps_i - sum_ik [2 (ii|kk) - (ik|ik)]  (closed shell, Fock = diag(eps))
+      auto O = [&](int a, int b, int c, int d) { return o4[(((size_t)a * n_int + b) * n_int + c) * n_int + d]; };
+      double e = 0.0;
+      for (int i = 0; i < n_int; ++i) {
+        e += 2.0 * h_eps[i];
+        for (int k = 0; k < n_int; ++k) e -= 2.0 * O(i, i, k, k) - O(i, k, i, k);
+      }
+      e_ref = e;
     }
-    e_ref = e;
     TCI_HIP(hipDeviceSynchronize());
   }
   Lpm.reset();
 
+  // ---- explicit one-electron integrals: Fock matrix, orbital energies, E_ref --
+  // The pp array of the reference (getpp, get_integrals_mod.f90) in its packed
+  // lower-triangle order.  F = h + G; the orbital energies are diag(F) and
+  // E_ref = sum_i (h_ii + f_ii).  Off-diagonal elements couple the CSFs:
+  // f_ij (i != j) through O4 and Mss, f_ab (a != b) through Mss and the
+  // per-walk products F'[D,D] C_P + C_P F'[D,D], f_ia through O3 (singles <->
+  // doubles) and the reference <-> singles kernel.
+  std::vector<double> hF;
+  if (explicit_h) {
+    const int64_t npk = n_orb * (n_orb + 1) / 2;
+    std::vector<double> hp(p.h, p.h + npk);
+    DevBuf d_h;
+    upload_now(d_h, hp);
+    Fm.alloc((size_t)n_orb * n_orb * 8);
+    launch_fock(d_h.as<double>(), O4.as<double>(), O3.as<double>(), Kov.as<double>(), Jvv.as<double>(), n_int, n_v,
+                Fm.as<double>(), st);
+    hF.resize((size_t)n_orb * n_orb);
+    TCI_HIP(hipMemcpy(hF.data(), Fm.p, hF.size() * 8, hipMemcpyDeviceToHost));
+    double e = 0.0;
+    for (int64_t q = 0; q < n_orb; ++q) {
+      h_eps[q] = hF[(size_t)q * n_orb + q];
+      TCI_REQUIRE(std::isfinite(h_eps[q]), "non-finite Fock matrix element");
+    }
+    for (int i = 0; i < n_int; ++i) e += hp[(size_t)i * (i + 1) / 2 + i] + h_eps[i];
+    e_ref = e;
+    TCI_HIP(hipMemcpy(eps.p, h_eps.data(), (size_t)n_orb * 8, hipMemcpyHostToDevice));
+  }
+
   diag.alloc((size_t)n_csf * 8);
   launch_diag(tab, dmax, Kov.as<double>(), Jvv.as<double>(), O4.as<double>(), vvd.as<double>(), vvx.as<double>(),
               diag.as<double>(), st);
+
+  if (explicit_h) {
+    // fold the off-diagonal Fock elements into the coupling tables (after the
+    // diagonal, which reads only O4 elements the fold leaves unchanged):
+    //   O4[k][i][l][j] -= f'_ki d_lj + d_ki f'_lj   (hole term over C^{kl})
+    //   O3[k][k][q][b] -= f_qb,  Mss += d_ij f'_ab - d_ab f'_ij
+    auto Fh = [&](int64_t a, int64_t b) { return hF[(size_t)a * n_orb + b]; };
+    const size_t n = n_int;
+    for (size_t k = 0; k < n; ++k)
+      for (size_t i = 0; i < n; ++i)
+        for (size_t l = 0; l < n; ++l)
+          for (size_t j = 0; j < n; ++j) {
+            double v = 0.0;
+            if (l == j && k != i) v += Fh(k, i);
+            if (k == i && l != j) v += Fh(l, j);
+            if (v != 0.0) o4[((k * n + i) * n + l) * n + j] -= v;
+          }
+    TCI_HIP(hipMemcpy(O4.p, o4.data(), o4.size() * 8, hipMemcpyHostToDevice));
+    launch_fold_fock(Fm.as<double>(), n_int, n_v, O3.as<double>(), Mss.as<double>(), st);
+    std::vector<double> fvv((size_t)n_v * n_v, 0.0), fov((size_t)niv, 0.0);
+    for (int a = 0; a < n_v; ++a)
+      for (int b = 0; b < n_v; ++b)
+        if (a != b) {
+          fvv[(size_t)a * n_v + b] = Fh(n_int + a, n_int + b);
+          has_fvv = has_fvv || fvv[(size_t)a * n_v + b] != 0.0;
+        }
+    for (int i = 0; i < n_int; ++i)
+      for (int a = 0; a < n_v; ++a) {
+        fov[(size_t)i * n_v + a] = Fh(i, n_int + a);
+        has_fov = has_fov || fov[(size_t)i * n_v + a] != 0.0;
+      }
+    if (has_fvv) upload_now(Fvvp, fvv);
+    if (has_fov) upload_now(Fov, fov);
+    TCI_HIP(hipDeviceSynchronize());
+  }
 
   // ---- domain maps ---------------------------------------------------------
   std::vector<int32_t> h_pos((size_t)std::max(n_pairs, 1) * n_v, -1);
@@ -400,6 +467,22 @@ void tci_plan::build(const tci_problem& p) {
     ring.add(R.as<double>() + h_cd_off[P], d, d, d, sg, true);
   }
   ring.upload_tables(st);
+  // off-diagonal virtual Fock elements on every pair block:
+  // R_P += F'[D,D] C_P + C_P F'[D,D]
+  fockvv.clear();
+  fockvv.tile = 128;
+  if (has_fvv) {
+    const double* Fv = Fvvp.as<double>();
+    for (int P = 0; P < n_pairs; ++P) {
+      const int d = dsz(P);
+      const int32_t* DP = DL(P);
+      fockvv.add(R.as<double>() + h_cd_off[P], d, d, d,
+                 {seg(op(Fv, n_v, 1, DP, DP), blk(Cp, P, false, nullptr, nullptr), d, 1.0),
+                  seg(blk(Cp, P, false, nullptr, nullptr), op(Fv, n_v, 1, DP, DP), d, 1.0)},
+                 true);
+    }
+    fockvv.upload_tables(st);
+  }
   if (dense_pairs) dense_pairs_build(*this, o4, st);
   // Y[(J,a)][i] = sum_c L^J_ac s_i^c
   progY.clear();
@@ -496,7 +579,8 @@ void tci_plan::build(const tci_problem& p) {
   for (DevBuf* b : {&pairs, &dom_off, &dom_idx, &cd_off, &aa_off, &ab_off, &eps, &pos, &xmap, &isect_off, &isect,
                     &inv_off, &inv_q, &inv_pos, &gpart_off, &Lvv, &Lov, &Kov, &Jvv, &Mss, &O4, &O3, &vvd, &vvx,
                     &diag, &C, &Ct, &R, &s, &g, &gss, &gW, &Y, &W, &gpart, &part, &sig0, &lad_part, &Jbig, &CRb,
-                    &CtRb, &CCb, &Zb, &Z3b, &Cemb, &CTemb, &Remb, &Yt, &holeA, &Tbuf, &Rpart, &Lp, &Cw, &lp_off_all})
+                    &CtRb, &CCb, &Zb, &Z3b, &Cemb, &CTemb, &Remb, &Yt, &holeA, &Tbuf, &Rpart, &Lp, &Cw, &lp_off_all, &Fm,
+                    &Fvvp, &Fov})
     device_bytes += (int64_t)b->bytes;
 }
 
@@ -513,6 +597,42 @@ tci_plan::~tci_plan() {
   if (gstream) (void)hipStreamDestroy(gstream);
   for (hipEvent_t& e : ev)
     if (e) (void)hipEventDestroy(e);
+  for (hipEvent_t& e : kev) (void)hipEventDestroy(e);
+}
+
+int tci_plan::kt_begin(hipStream_t st) {
+  if (!timing()) return -1;
+  if (kev_used + 2 > kev.size()) {
+    if (kev.size() >= 8192) return -1;  // pool exhausted: this launch goes untimed
+    for (int q = 0; q < 64; ++q) {
+      hipEvent_t e = nullptr;
+      TCI_HIP(hipEventCreate(&e));
+      kev.push_back(e);
+    }
+  }
+  TCI_HIP(hipEventRecord(kev[kev_used], st));
+  return (int)kev_used++;
+}
+
+void tci_plan::kt_end(int e0, const std::string& name, double flops, int prec, hipStream_t st) {
+  if (e0 < 0) return;
+  TCI_HIP(hipEventRecord(kev[kev_used], st));
+  kpend.push_back(KRec{name, flops, e0, (int)kev_used, prec});
+  ++kev_used;
+}
+
+void tci_plan::kt_collect() {
+  for (const KRec& r : kpend) {
+    TCI_HIP(hipEventSynchronize(kev[r.e1]));
+    float ms = 0.f;
+    TCI_HIP(hipEventElapsedTime(&ms, kev[r.e0], kev[r.e1]));
+    KStat& k = kstats[r.prec][r.name];
+    ++k.n;
+    k.ms += ms;
+    k.flops += r.flops;
+  }
+  kpend.clear();
+  kev_used = 0;
 }
 
 // Two-phase ladder tables (dense FP64 MFMA programs).  Per chunk of Lc
@@ -810,14 +930,19 @@ void tci_plan::ladder_chunk(const std::pair<int64_t, int>& ch, hipStream_t st) {
   const LadderPhase& ph = lphase[ch.second];
   if (n_local && !lp_all) {
     PackTab K{pairs_local.as<int32_t>(), ph.lp_off.as<int64_t>(), n_local};
+    const int e0 = kt_begin(st);
     launch_pack_L(tab, K, Lvv.p, lvv_f32, ch.first, ph.Lc, Lp.p, st);
+    kt_end(e0, "fp64 ladder pack_L_kernel", 0.0, TCI_SIGMA_FP64, st);
   }
+  auto run = [&](const char* role, const GemmProgram& g) {
+    kt_run(role, g, TCI_SIGMA_FP64, st, [&] { g.launch(st, ch.first); });
+  };
   if (ladder_merged) {
-    ph.p1.launch(st, ch.first);
-    ph.p2.launch(st, ch.first);
+    run("fp64 ladder phase 1", ph.p1);
+    run("fp64 ladder phase 2", ph.p2);
   } else {
-    for (const auto& g : ph.p1x) g.launch(st, ch.first);
-    for (const auto& g : ph.p2x) g.launch(st, ch.first);
+    for (const auto& g : ph.p1x) run("fp64 ladder phase 1", g);
+    for (const auto& g : ph.p2x) run("fp64 ladder phase 2", g);
   }
 }
 
@@ -847,7 +972,9 @@ void tci_plan::unpack_and_ladder(const double* x, const IoSlot* io, int precisio
   a.n_v = n_v;
   a.partial = lad_part.as<double>();
   a.chunk_stride = sum_d2;
+  const int e0 = kt_begin(st);
   ladder.launch(a, st);
+  kt_end(e0, "fp64 fused ladder", flops_ladder, TCI_SIGMA_FP64, st);
   if (timing()) TCI_HIP(hipEventRecord(ev[3], st));
   ladder_reduce(lad_part.as<double>(), sum_d2, lad_chunks, R.as<double>(), sum_d2, st);
 }
@@ -861,15 +988,21 @@ void tci_plan::sigma_body(const double* x, double* y, const IoSlot* io, int flag
 // ladder's R
 void tci_plan::sigma_rest(const double* x, double* y, const IoSlot* io, int flags, hipStream_t st) {
   const int precision = flags & 0xff;
+  int e0 = kt_begin(st);
   launch_rterms(tab, dmax, x, s.as<double>(), Kov.as<double>(), O3.as<double>(), R.as<double>(), st, io);
-  progY.launch(st);
+  kt_end(e0, "rterms_kernel", 0.0, precision, st);
+  kt_run("singles Y", progY, precision, st, [&] { progY.launch(st); });
   if (dense_pairs) {
+    e0 = kt_begin(st);
     dense_pairs_terms(*this, precision, st);
+    kt_end(e0, "dense pair products (section)", 0.0, precision, st);
   } else {
     HoleTab H{isect_off.as<int64_t>(), isect.as<int32_t>()};
+    e0 = kt_begin(st);
     launch_hole(tab, H, dmax, C.as<double>(), O4.as<double>(), R.as<double>(), st);
-    ring.launch(st);
-    progW.launch(st);
+    kt_end(e0, "hole_kernel", 0.0, precision, st);
+    kt_run("ring and 3-external", ring, precision, st, [&] { ring.launch(st); });
+    kt_run("singles W", progW, precision, st, [&] { progW.launch(st); });
   }
   launch_gemv(Mss.as<double>(), (int64_t)n_int * n_v, s.as<double>(), gss.as<double>(), st);
   progGW.launch(st);
@@ -878,6 +1011,9 @@ void tci_plan::sigma_rest(const double* x, double* y, const IoSlot* io, int flag
   launch_singles_reduce(tab, I, gpart_off.as<int64_t>(), gpart.as<double>(), gss.as<double>(), gW.as<double>(), n_split,
                         g.as<double>(), st);
   launch_ref_dot(tab, Kov.as<double>(), Ct.as<double>(), part.as<double>(), sig0.as<double>(), st);
+  if (has_fvv) kt_run("virtual Fock", fockvv, precision, st, [&] { fockvv.launch(st); });
+  if (has_fov) launch_fov_terms(x, s.as<double>(), Fov.as<double>(), (int64_t)n_int * n_v, g.as<double>(),
+                                sig0.as<double>(), st, io);
   launch_pack(tab, dmax, x, R.as<double>(), g.as<double>(), sig0.as<double>(), e_ref, flags, y, st, io);
 }
 
@@ -965,6 +1101,7 @@ void tci_plan::sigma_batch(const double* const* x, double* const* y, int nvec, i
   if (profiling) {
     TCI_HIP(hipEventRecord(ev[1], st));
     TCI_HIP(hipEventSynchronize(ev[1]));
+    kt_collect();
     float ms = 0.f;
     TCI_HIP(hipEventElapsedTime(&ms, ev[0], ev[1]));
     prof_sigma_ms[precision] += ms;
@@ -989,6 +1126,7 @@ void tci_plan::sigma(const double* x, double* y, int flags, hipStream_t st) {
   if (profiling) {
     TCI_HIP(hipEventRecord(ev[1], st));
     TCI_HIP(hipEventSynchronize(ev[1]));
+    kt_collect();
     float ms = 0.f;
     TCI_HIP(hipEventElapsedTime(&ms, ev[0], ev[1]));
     prof_sigma_ms[precision] += ms;
@@ -1131,7 +1269,10 @@ int tci_profile_enable(tci_plan* plan, int32_t on) {
   for (int q = 0; q < 3; ++q) {
     plan->prof_ladder_ms[q] = plan->prof_sigma_ms[q] = 0.0;
     plan->prof_ladder_n[q] = plan->prof_sigma_n[q] = 0;
+    plan->kstats[q].clear();
   }
+  plan->kpend.clear();
+  plan->kev_used = 0;
   return TCI_OK;
   TCI_GUARD_END
 }
@@ -1145,6 +1286,24 @@ int tci_profile_read(const tci_plan* plan, int32_t precision, double* ladder_ms,
   if (ladder_launches) *ladder_launches = plan->prof_ladder_n[precision];
   if (sigma_ms) *sigma_ms = plan->prof_sigma_ms[precision];
   if (sigma_calls) *sigma_calls = plan->prof_sigma_n[precision];
+  return TCI_OK;
+  TCI_GUARD_END
+}
+
+int tci_profile_kernels(const tci_plan* plan, int32_t precision, char* buf, int64_t len) {
+  TCI_GUARD_BEGIN
+  TCI_REQUIRE(plan && buf && len > 0, "null argument");
+  TCI_REQUIRE(precision >= 0 && precision < 3, "unknown precision");
+  std::string out;
+  char line[512];
+  for (const auto& kv : plan->kstats[precision]) {
+    std::snprintf(line, sizeof line, "%s|%lld|%.6f|%.6e\n", kv.first.c_str(), (long long)kv.second.n, kv.second.ms,
+                  kv.second.n ? kv.second.flops / (double)kv.second.n : 0.0);
+    out += line;
+  }
+  const size_t m = std::min<size_t>(out.size(), (size_t)len - 1);
+  std::memcpy(buf, out.data(), m);
+  buf[m] = 0;
   return TCI_OK;
   TCI_GUARD_END
 }

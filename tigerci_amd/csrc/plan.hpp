@@ -26,6 +26,13 @@ struct tci_plan {
   tci::DevBuf isect_off, isect, inv_off, inv_q, inv_pos, gpart_off;
   // integrals (device)
   tci::DevBuf Lvv, Lov, Kov, Jvv, Mss, O4, O3, vvd, vvx, diag;
+  // explicit one-electron integrals: closed-shell Fock matrix F = h + G
+  // (n_orb x n_orb), its virtual block off the diagonal (n_v x n_v) and its
+  // occupied-virtual block (n_int x n_v); the occupied off-diagonal elements
+  // act through O4, the occupied-virtual ones also through O3
+  bool explicit_h = false, has_fvv = false, has_fov = false;
+  tci::DevBuf Fm, Fvvp, Fov;
+  tci::GemmProgram fockvv;         // R_P += F'[D,D] C_P + C_P F'[D,D]
   // work buffers
   tci::DevBuf C, Ct, R, s, g, gss, gW, Y, W, gpart, part, sig0, lad_part;
   int n_split = 1;
@@ -155,6 +162,32 @@ struct tci_plan {
   std::map<int, hipGraphExec_t> graphs;
   bool capturing = false;
   bool timing() const { return profiling && !capturing; }
+  // per-kernel event timing of profiled eager builds (tci_profile_kernels):
+  // kt_begin records a start event (-1 when not timing), kt_end the stop event
+  // of one launch; kt_collect reads the pairs once the build has synchronised
+  struct KStat {
+    int64_t n = 0;
+    double ms = 0.0, flops = 0.0;  // launches, total milliseconds, total algorithmic flops
+  };
+  struct KRec {
+    std::string name;
+    double flops;
+    int e0, e1, prec;
+  };
+  std::map<std::string, KStat> kstats[3];
+  std::vector<hipEvent_t> kev;
+  std::vector<KRec> kpend;
+  size_t kev_used = 0;
+  int kt_begin(hipStream_t st);
+  void kt_end(int e0, const std::string& name, double flops, int prec, hipStream_t st);
+  void kt_collect();
+  // time one program launch under its role name and the program's algorithmic flops
+  template <class F>
+  void kt_run(const std::string& role, const tci::GemmProgram& g, int prec, hipStream_t st, F&& launch) {
+    const int e0 = kt_begin(st);
+    launch();
+    if (e0 >= 0) kt_end(e0, role + " " + g.kernel_name(), g.alg_flops, prec, st);
+  }
   void drop_graphs();
   ~tci_plan();
 

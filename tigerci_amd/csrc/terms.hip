@@ -563,6 +563,89 @@ __global__ void fill_kernel(double* p, int64_t n, double v) {
     p[e] = v;
 }
 
+// ---- explicit one-electron integrals ---------------------------------------
+// F_pq = h_pq + sum_k [2 (pq|kk) - (pk|kq)] (closed-shell Fock matrix), n_orb x
+// n_orb, from the packed h (element (p,q), p >= q, at p(p+1)/2 + q) and the
+// assembled blocks: O4[a][b][c][d] = (ab|cd) (internal), O3[k][p][q][b] =
+// (kp|bq), Kov[(i,a)][(j,b)] = (ia|jb), Jvv[ij pair][a][b] = (ij|ab).
+__global__ void fock_kernel(const double* __restrict__ hp, const double* __restrict__ O4,
+                            const double* __restrict__ O3, const double* __restrict__ Kov,
+                            const double* __restrict__ Jvv, int n_int, int n_v, double* __restrict__ F) {
+  const int64_t no = n_int, nv = n_v, n_orb = no + nv, niv = no * nv;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n_orb * n_orb;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t p = e / n_orb, q = e - p * n_orb;
+    if (p < q) continue;  // lower triangle, mirrored below
+    double g = 0.0;
+    if (p < no) {  // (internal, internal)
+      for (int64_t k = 0; k < no; ++k)
+        g += 2.0 * O4[((p * no + q) * no + k) * no + k] - O4[((p * no + k) * no + k) * no + q];
+    } else if (q < no) {  // (virtual a = p - no, internal i = q): 2 (kk|ai) - (ik|ak)
+      const int64_t a = p - no, i = q;
+      for (int64_t k = 0; k < no; ++k)
+        g += 2.0 * O3[((k * no + k) * no + i) * nv + a] - O3[((i * no + k) * no + k) * nv + a];
+    } else {  // (virtual, virtual): 2 (kk|ab) - (ka|kb)
+      const int64_t a = p - no, b = q - no;
+      for (int64_t k = 0; k < no; ++k)
+        g += 2.0 * Jvv[((k * (k + 1) / 2 + k) * nv + a) * nv + b] - Kov[(k * nv + a) * niv + k * nv + b];
+    }
+    const double f = hp[p * (p + 1) / 2 + q] + g;
+    F[p * n_orb + q] = f;
+    F[q * n_orb + p] = f;
+  }
+}
+
+// O3[k][k][q][b] -= f_{q,b}: the occupied-virtual Fock block joins the
+// 3-internal singles coupling (ki|bj) - delta_ki f_jb in both directions
+__global__ void fold_o3_kernel(double* __restrict__ O3, const double* __restrict__ F, int n_int, int n_v) {
+  const int64_t no = n_int, nv = n_v, n_orb = no + nv;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < no * no * nv;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t k = e / (no * nv), r = e - k * no * nv, q = r / nv, b = r - q * nv;
+    O3[((k * no + k) * no + q) * nv + b] -= F[q * n_orb + no + b];
+  }
+}
+
+// Mss[(i,a)][(j,b)] += delta_ij f'_ab - delta_ab f'_ij (off-diagonal Fock
+// elements; the diagonal ones act through the orbital energies)
+__global__ void fold_mss_kernel(double* __restrict__ Mss, const double* __restrict__ F, int n_int, int n_v) {
+  const int64_t no = n_int, nv = n_v, n_orb = no + nv, niv = no * nv;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < no * nv * nv;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t i = e / (nv * nv), r = e - i * nv * nv, a = r / nv, b = r - a * nv;
+    if (a != b) Mss[(i * nv + a) * niv + i * nv + b] += F[(no + a) * n_orb + no + b];
+  }
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < nv * no * no;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t a = e / (no * no), r = e - a * no * no, i = r / no, j = r - i * no;
+    if (i != j) Mss[(i * nv + a) * niv + j * nv + a] -= F[i * n_orb + j];
+  }
+}
+
+// Reference <-> singles through f_ia: g[i][a] += c0 f_ia and
+// sigma_0 += 2 sum_ia f_ia s_i^a (one workgroup; after the singles reduction
+// and the reference dot of the same build)
+__global__ void fov_terms_kernel(const double* x, const IoSlot* io, const double* __restrict__ s,
+                                 const double* __restrict__ Fov, int64_t niv, double* __restrict__ g,
+                                 double* __restrict__ sig0) {
+  if (io) x = io->x;
+  const double c0 = x[0];
+  __shared__ double red[256];
+  double acc = 0.0;
+  for (int64_t e = threadIdx.x; e < niv; e += blockDim.x) {
+    const double f = Fov[e];
+    g[e] += c0 * f;
+    acc += f * s[e];
+  }
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *sig0 += 2.0 * red[0];
+}
+
 inline unsigned gx(int64_t work) { return (unsigned)std::min<int64_t>(std::max<int64_t>(cdiv(work, 256), 1), 4096); }
 
 }  // namespace
@@ -687,6 +770,27 @@ void launch_gemv(const double* A, int64_t n, const double* x, double* y, hipStre
 void launch_fill(double* p, int64_t n, double v, hipStream_t st) {
   if (n <= 0) return;
   hipLaunchKernelGGL(fill_kernel, dim3(gx(n)), dim3(256), 0, st, p, n, v);
+  TCI_HIP(hipGetLastError());
+}
+
+void launch_fock(const double* hp, const double* O4, const double* O3, const double* Kov, const double* Jvv,
+                 int n_int, int n_v, double* F, hipStream_t st) {
+  const int64_t n = (int64_t)(n_int + n_v) * (n_int + n_v);
+  hipLaunchKernelGGL(fock_kernel, dim3(gx(n)), dim3(256), 0, st, hp, O4, O3, Kov, Jvv, n_int, n_v, F);
+  TCI_HIP(hipGetLastError());
+}
+
+void launch_fold_fock(const double* F, int n_int, int n_v, double* O3, double* Mss, hipStream_t st) {
+  const int64_t n = (int64_t)n_int * n_int * n_v + (int64_t)n_int * n_v * n_v;
+  hipLaunchKernelGGL(fold_o3_kernel, dim3(gx(n)), dim3(256), 0, st, O3, F, n_int, n_v);
+  TCI_HIP(hipGetLastError());
+  hipLaunchKernelGGL(fold_mss_kernel, dim3(gx(n)), dim3(256), 0, st, Mss, F, n_int, n_v);
+  TCI_HIP(hipGetLastError());
+}
+
+void launch_fov_terms(const double* x, const double* s, const double* Fov, int64_t niv, double* g, double* sig0,
+                      hipStream_t st, const IoSlot* io) {
+  hipLaunchKernelGGL(fov_terms_kernel, dim3(1), dim3(256), 0, st, x, io, s, Fov, niv, g, sig0);
   TCI_HIP(hipGetLastError());
 }
 

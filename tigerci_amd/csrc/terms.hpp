@@ -82,6 +82,15 @@ void launch_vv_exch(const void* Lvv, bool f32, int64_t n_chol, int n_v, double* 
 void launch_build_mss(const double* Kov, const double* Jvv, int n_int, int n_v, double* Mss,
                       hipStream_t st);
 void launch_fill(double* p, int64_t n, double v, hipStream_t st);
+// explicit one-electron integrals: F = h + G (closed-shell Fock matrix, n_orb x
+// n_orb) from the packed h and the assembled integral blocks
+void launch_fock(const double* h_packed, const double* O4, const double* O3, const double* Kov, const double* Jvv,
+                 int n_int, int n_v, double* F, hipStream_t st);
+// O3[k][k][q][b] -= f_qb and Mss += delta_ij f'_ab - delta_ab f'_ij (f' = F off its diagonal)
+void launch_fold_fock(const double* F, int n_int, int n_v, double* O3, double* Mss, hipStream_t st);
+// g += c0 f_ov, *sig0 += 2 <f_ov, s>  (c0 read from x[0])
+void launch_fov_terms(const double* x, const double* s, const double* Fov, int64_t niv, double* g, double* sig0,
+                      hipStream_t st, const IoSlot* io = nullptr);
 // y = A x, A n x n row-major
 void launch_gemv(const double* A, int64_t n, const double* x, double* y, hipStream_t st);
 // Cw[k][P d + c] = C_P[k][c] for n_pairs blocks of size d x d stored back to back

@@ -28,6 +28,7 @@ module tigersigma
     type(c_ptr) :: pairs          ! n_pairs x 2 int32, (i, j) with i >= j, 0-based
     type(c_ptr) :: dom_off        ! n_pairs + 1 int64
     type(c_ptr) :: dom_idx        ! int32 sorted virtual indices, 0-based
+    type(c_ptr) :: h              ! c_null_ptr, or the packed one-electron integrals pp(n_orb(n_orb+1)/2)
   end type tci_problem
 
   type, bind(c), public :: tci_plan_info
@@ -47,12 +48,14 @@ module tigersigma
   type, bind(c), public :: tci_davidson_result
     integer(c_int32_t) :: iterations, sigma_fp64, sigma_mixed, converged
     real(c_double) :: seconds, seconds_sigma
+    integer(c_int32_t) :: collapses
   end type tci_davidson_result
 
-  public :: tci_plan_create, tci_plan_destroy, tci_plan_get_info, tci_sigma_apply, tci_htimesc, &
-            tci_diagonal, tci_davidson_run, tci_davidson_gen_run, tci_acpf_gvalues, tci_device_alloc, &
-            tci_device_free, tci_memcpy_h2d, &
-            tci_memcpy_d2h, tci_device_synchronize, tci_last_error_string
+  public :: tci_plan_create, tci_plan_destroy, tci_plan_get_info, tci_sigma_apply, tci_sigma_apply_batch, &
+            tci_htimesc, tci_diagonal, tci_davidson_run, tci_davidson_gen_run, tci_acpf_gvalues, &
+            tci_device_alloc, tci_device_free, tci_memcpy_h2d, tci_memcpy_d2h, tci_memcpy_d2d, &
+            tci_device_synchronize, tci_profile_enable, tci_profile_read, tci_plan_tuning, tci_selftest_mfma, &
+            tci_last_error_string
 
   interface
     integer(c_int) function tci_plan_create(prob, plan) bind(c, name="tci_plan_create")
@@ -75,6 +78,14 @@ module tigersigma
       type(c_ptr), value :: plan, c, sigma, stream
       integer(c_int64_t), value :: n
       integer(c_int32_t), value :: flags
+    end function
+    ! nvec device vectors back to back in c and sigma (not overlapping)
+    integer(c_int) function tci_sigma_apply_batch(plan, c, sigma, n, nvec, flags, stream) &
+        bind(c, name="tci_sigma_apply_batch")
+      import :: c_int, c_ptr, c_int64_t, c_int32_t
+      type(c_ptr), value :: plan, c, sigma, stream
+      integer(c_int64_t), value :: n
+      integer(c_int32_t), value :: nvec, flags
     end function
     ! host vectors, htimesc semantics: sigmavec += H civec
     integer(c_int) function tci_htimesc(plan, civec, sigmavec, n) bind(c, name="tci_htimesc")
@@ -135,7 +146,36 @@ module tigersigma
       type(c_ptr), value :: dst, src
       integer(c_int64_t), value :: bytes
     end function
+    integer(c_int) function tci_memcpy_d2d(dst, src, bytes) bind(c, name="tci_memcpy_d2d")
+      import :: c_int, c_ptr, c_int64_t
+      type(c_ptr), value :: dst, src
+      integer(c_int64_t), value :: bytes
+    end function
     integer(c_int) function tci_device_synchronize() bind(c, name="tci_device_synchronize")
+      import :: c_int
+    end function
+    ! HIP-event timing of every sigma build: ladder and whole-build milliseconds per precision
+    integer(c_int) function tci_profile_enable(plan, on) bind(c, name="tci_profile_enable")
+      import :: c_int, c_ptr, c_int32_t
+      type(c_ptr), value :: plan
+      integer(c_int32_t), value :: on
+    end function
+    integer(c_int) function tci_profile_read(plan, precision, ladder_ms, ladder_launches, sigma_ms, sigma_calls) &
+        bind(c, name="tci_profile_read")
+      import :: c_int, c_ptr, c_int32_t, c_int64_t, c_double
+      type(c_ptr), value :: plan
+      integer(c_int32_t), value :: precision
+      real(c_double), intent(out) :: ladder_ms, sigma_ms
+      integer(c_int64_t), intent(out) :: ladder_launches, sigma_calls
+    end function
+    ! autotuning record (text, NUL-terminated, at most len-1 characters)
+    integer(c_int) function tci_plan_tuning(plan, buf, len) bind(c, name="tci_plan_tuning")
+      import :: c_int, c_ptr, c_char, c_int64_t
+      type(c_ptr), value :: plan
+      character(kind=c_char), intent(out) :: buf(*)
+      integer(c_int64_t), value :: len
+    end function
+    integer(c_int) function tci_selftest_mfma() bind(c, name="tci_selftest_mfma")
       import :: c_int
     end function
     type(c_ptr) function tci_last_error_c() bind(c, name="tci_last_error")
