@@ -9,10 +9,14 @@
  * Vectors passed to tci_sigma_apply / tci_diagonal live in device memory;
  * tci_htimesc takes host vectors.
  *
- * CI vector layout (one entry per CSF, 0-based):
+ * CI vector layout (one entry per CSF, 0-based), walks in the reference's
+ * lexical DRT order:
  *   [0]                    valence (closed-shell reference)
- *   [1 + i*n_v + a]        N-1 walk of hole i, virtual a
- *   per strong pair walk P = (i,j), i >= j, in the order given:
+ *   [1 + (n_int-1-i)*n_v + a]  N-1 walk of hole i, virtual a (walks by
+ *                          descending hole, the DRT walk order of a
+ *                          closed-shell reference)
+ *   per strong pair walk P = (i,j), i >= j, in the order given (the
+ *   reference's order is i descending, then j descending):
  *       aa block  d_P entries                       (virtual doubly occupied)
  *       ab block  nsp * d_P (d_P-1)/2 entries       (nsp = 1 if i==j else 2:
  *                 singlet-coupled block, then triplet-coupled block; packed

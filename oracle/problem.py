@@ -13,11 +13,14 @@ p(p+1)/2+q in 0-based form (the reference's 1-based index2(p,q) =
 max(max-1)/2+min of F/utilities_mod.f90:199 shifted by one).
 
 CI vector layout (F/graph_mod.f90:1041-1092, 1430-1450) for the closed-shell
-single reference:
+single reference, walks in the reference's lexical walk order (oracle/drt.py):
   [valence]  1 coefficient (the reference determinant, fsn(0)=1)
-  [N-1]      per internal hole i (walk order i=0..n_int-1): n_v coefficients
-             (fsn(2)=1 spin function, virt-minor; singles use all virtuals)
-  [N-2]      per strong pair walk P=(i,j), i>=j, in walk order:
+  [N-1]      per internal hole i, holes in descending order (i = n_int-1
+             first): n_v coefficients (fsn(2)=1 spin function, virt-minor;
+             singles use all virtuals)
+  [N-2]      per strong pair walk P=(i,j), i>=j, in the order of ``pairs``
+             (the generators emit the reference's order: i descending, then
+             j descending, weak pairs left out):
                aa block: d_P coefficients           (fsn(s)=1)
                ab block: nsp * d_P(d_P-1)/2          (nsp=fsn(s+2): 1 for i==j,
                                                       2 for i>j: singlet-coupled
@@ -47,7 +50,7 @@ class Problem:
     n_chol: int
     eps: np.ndarray            # (n_orb,) orbital energies, internal first
     L: np.ndarray              # (npair_all, n_chol) float64, pair-major records
-    pairs: np.ndarray          # (n_pairs, 2) int32 strong pairs (i>=j), walk order
+    pairs: np.ndarray          # (n_pairs, 2) int32 strong pairs (i>=j), in CI-vector order
     dom_off: np.ndarray        # (n_pairs+1,) int64 offsets into dom_idx
     dom_idx: np.ndarray        # (sum d_P,) int32 sorted virtual indices
     # one-electron integrals h_pq (n_orb, n_orb), symmetric; None: the core
@@ -88,7 +91,7 @@ class Layout:
     """Offsets of every block of the CI vector (0-based)."""
     n_csf: int
     ref_off: int
-    single_off: int                  # singles of hole i at single_off + i*n_v
+    single_off: int                  # singles of hole i at single_off + (n_int-1-i)*n_v
     aa_off: np.ndarray               # (n_pairs,)
     ab_off: np.ndarray               # (n_pairs,)
     nsp: np.ndarray                  # (n_pairs,) 1 or 2
@@ -156,6 +159,11 @@ def make_problem(n_int: int, n_v: int, n_chol: int, seed: int,
                 dom = np.sort(rng_d.choice(n_v, size, replace=False)).astype(np.int32)
             pairs.append((i, j))
             doms.append(dom)
+    # the reference's walk order (i descending, then j descending); domains
+    # are drawn in ascending order above, so a pair keeps its domain
+    order = sorted(range(len(pairs)), key=lambda k: (-pairs[k][0], -pairs[k][1]))
+    pairs = [pairs[k] for k in order]
+    doms = [doms[k] for k in order]
     pairs_a = np.array(pairs, dtype=np.int32).reshape(-1, 2)
     dom_off = np.zeros(len(doms) + 1, np.int64)
     dom_off[1:] = np.cumsum([len(d) for d in doms])

@@ -10,9 +10,14 @@ excludes E_ref on the diagonal.
 Dense representation (all pair indices ordered, i,j over internal orbitals):
   C^{ij} (n_v x n_v), C^{ji} = C^{ij T};  for a strong walk P=(i,j) the dense
   block on its domain is
-     i>j:  C^{ij} = S+(x_ab+, x_aa)/2 + S-(x_ab-)/(2 sqrt3)
+     i>j:  C^{ij} = -S+(x_ab+, x_aa)/2 + S-(x_ab-)/(2 sqrt3)
      i=j:  C^{ii} = S+(x_ab, x_aa)/sqrt2
-  (S+ = scepper_diag_p1, S- = scepper_diag_m1); singles s_i^a = x_i^a/sqrt2;
+  (the minus sign is the phase of the reference's SGGA spin functions: the
+  restated coupling coefficients of the valence <-> N-2 loop give
+  <V|H|aa, ab singlet> = -<K, S+> and <V|H|ab triplet> = +sqrt3 <K, S->
+  relative to the valence CSF; tests/test_spin.py)
+  (S+ = scepper_diag_p1, S- = scepper_diag_m1); singles s_i^a = x_i^a/sqrt2
+  (x_i^a at 1 + (n_int-1-i) n_v + a: N-1 walks by descending hole);
   reference c0 = x_0.  Row index of C^{ij} pairs with i.
 Contravariant residual R^{ij}_{ab} (derived from the spin-orbital CISD
 projections; pinned by brute-force determinant CI in tests/):
@@ -25,7 +30,7 @@ projections; pinned by brute-force determinant CI in tests/):
          - sum_k [(ki|bj) s_k^a + (kj|ai) s_k^b]        3-internal singles
   Z^{ij} = sum_k [ Ct^{ik} K^{kj} - C^{ik} J^{kj} - J^{kj} C^{ik} ]
   Ct^{ik} = 2 C^{ik} - C^{ik T},  K^{kj}_{cb} = (kc|jb),  J^{kj}_{ac} = (kj|ac)
-Orthonormal sigma: i>j: sigma+ = S+^T R, sigma- = sqrt3 S-^T R; i=j:
+Orthonormal sigma: i>j: sigma+ = -S+^T R, sigma- = sqrt3 S-^T R; i=j:
 S+^T R / sqrt2 (unscepper adjoints).  Singles:
   sigma_i^a = sqrt2 [ (e_a-e_i) s_i^a + sum_jb (2(ai|jb)-(ab|ij)) s_j^b
                       + sum_jbc (ab|jc) Ct^{ij}_bc - sum_jkb (ji|kb) Ct^{jk}_ab ]
@@ -79,7 +84,7 @@ def unpack(prob: Problem, x: np.ndarray, lay: Layout | None = None):
     lay = lay or prob.layout()
     no, nv = prob.n_int, prob.n_v
     c0 = float(x[0])
-    s = x[1:1 + no * nv].reshape(no, nv) / SQRT2
+    s = x[1:1 + no * nv].reshape(no, nv)[::-1] / SQRT2      # N-1 walks by descending hole
     C = np.zeros((no, no, nv, nv))
     for P in range(prob.n_pairs):
         i, j = prob.pairs[P]
@@ -92,7 +97,7 @@ def unpack(prob: Problem, x: np.ndarray, lay: Layout | None = None):
             blk = scepper_p1(vs, w, d) / SQRT2
         else:
             vt = x[lay.ab_off[P] + m2:lay.ab_off[P] + 2 * m2]
-            blk = scepper_p1(vs, w, d) / 2.0 + scepper_m1(vt, d) / (2.0 * SQRT3)
+            blk = -scepper_p1(vs, w, d) / 2.0 + scepper_m1(vt, d) / (2.0 * SQRT3)
         C[i, j][np.ix_(D, D)] = blk
         if i != j:
             C[j, i] = C[i, j].T
@@ -104,7 +109,7 @@ def pack(prob: Problem, sig0: float, sig_s: np.ndarray, R: dict, lay: Layout | N
     lay = lay or prob.layout()
     out = np.zeros(lay.n_csf)
     out[0] = sig0
-    out[1:1 + prob.n_int * prob.n_v] = sig_s.ravel()
+    out[1:1 + prob.n_int * prob.n_v] = sig_s[::-1].ravel()
     for P in range(prob.n_pairs):
         i, j = prob.pairs[P]
         D = prob.domain(P)
@@ -114,6 +119,8 @@ def pack(prob: Problem, sig0: float, sig_s: np.ndarray, R: dict, lay: Layout | N
         v, w = unscepper_p1(RP)
         if i == j:
             v, w = v / SQRT2, w / SQRT2
+        else:
+            v, w = -v, -w
         out[lay.aa_off[P]:lay.aa_off[P] + d] = w
         out[lay.ab_off[P]:lay.ab_off[P] + m2] = v
         if i != j:
@@ -214,7 +221,7 @@ def diagonal(prob: Problem, ints: Integrals | None = None) -> np.ndarray:
     out = np.zeros(lay.n_csf)
     out[0] = 0.0
     sing = (ev[None, :] - eo[:, None]) + 2.0 * Kd - Jd
-    out[1:1 + no * nv] = sing.ravel()
+    out[1:1 + no * nv] = sing[::-1].ravel()
     for P in range(prob.n_pairs):
         i, j = prob.pairs[P]
         D = prob.domain(P)

@@ -49,7 +49,8 @@ def test_config2_fp64_against_oracle(c2):
     x = np.zeros(plan.n_csf)
     rng = np.random.default_rng(7)
     x[: 1 + syn.n_int * syn.n_v] = rng.standard_normal(1 + syn.n_int * syn.n_v)
-    for P in (5, 8):                  # walks (2,2) and (3,2): one i == j, one i > j block
+    walk = {tuple(p): P for P, p in enumerate(prob.pairs.tolist())}
+    for P in (walk[(2, 2)], walk[(3, 2)]):        # one i == j and one i > j block
         end = lay.aa_off[P + 1] if P + 1 < prob.n_pairs else lay.n_csf
         x[lay.aa_off[P]:end] = rng.standard_normal(end - lay.aa_off[P])
     x /= np.linalg.norm(x)

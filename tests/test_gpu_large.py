@@ -73,9 +73,9 @@ def test_config3_hermitian_and_valence(big):
             off += d + m2
         else:
             t = x1[off + d + m2:off + d + 2 * m2]
-            C[hi, lo] = v / 2 - t / (2 * np.sqrt(3.0))
-            C[lo, hi] = v / 2 + t / (2 * np.sqrt(3.0))
-            C[np.arange(d), np.arange(d)] = np.sqrt(2.0) * w / 2
+            C[hi, lo] = -v / 2 - t / (2 * np.sqrt(3.0))
+            C[lo, hi] = -v / 2 + t / (2 * np.sqrt(3.0))
+            C[np.arange(d), np.arange(d)] = -np.sqrt(2.0) * w / 2
             off += d + 2 * m2
         Ct = 2 * C - C.T
         s0 += (1.0 if i == j else 2.0) * np.sum(K * Ct)

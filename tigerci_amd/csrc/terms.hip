@@ -31,14 +31,24 @@ __device__ __forceinline__ void tri_decode(int64_t idx, int& hi, int& lo) {
   lo = (int)(idx - (int64_t)h * (h - 1) / 2);
 }
 
+// CSF index of the singles element e = i n_v + a: the N-1 walks are laid out
+// in the reference's lexical walk order, which for a closed-shell reference
+// runs over the hole orbital in descending order (index_vector_internal,
+// graph_mod.f90:1032-1589, with the DRT weights of drt_maker), virtuals
+// ascending within a walk
+__device__ __forceinline__ int64_t single_csf(const PairTab& T, int64_t e) {
+  const int64_t i = e / T.n_v, a = e - i * T.n_v;
+  return 1 + (T.n_int - 1 - i) * T.n_v + a;
+}
+
 __global__ void unpack_kernel(PairTab T, const double* __restrict__ x, double* __restrict__ C,
                               double* __restrict__ Ct, double* __restrict__ s, const IoSlot* __restrict__ io) {
   if (io) x = io->x;
   const int P = blockIdx.y;
-  if (P == T.n_pairs) {  // singles
+  if (P == T.n_pairs) {  // singles: s[i][a] from the N-1 walk of hole i (walks by descending hole)
     const int64_t n = (int64_t)T.n_int * T.n_v;
     for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x)
-      s[e] = x[1 + e] * INV_SQRT2;
+      s[e] = x[single_csf(T, e)] * INV_SQRT2;
     return;
   }
   const int i = T.pairs[2 * P], j = T.pairs[2 * P + 1];
@@ -52,15 +62,19 @@ __global__ void unpack_kernel(PairTab T, const double* __restrict__ x, double* _
   for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < dd; e += (int64_t)gridDim.x * blockDim.x) {
     const int r = (int)(e / d), c = (int)(e - (int64_t)r * d);
     double crc, ccr;
+    // i > j: the doubly occupied (aa) and singlet-coupled (ab) CSFs enter
+    // with a minus sign, the phase of the reference's SGGA spin functions
+    // relative to the valence CSF (coupling coefficients (-1, -sqrt3) of the
+    // valence <-> N-2 loop, three_and_four_seg_mod_2.f90:1524-1533,8007-8162)
     if (r == c) {
-      crc = ccr = (i == j) ? xa[r] : xa[r] * INV_SQRT2;
+      crc = ccr = (i == j) ? xa[r] : -xa[r] * INV_SQRT2;
     } else {
       const int hi = r > c ? r : c, lo = r > c ? c : r;
       const int64_t idx = (int64_t)hi * (hi - 1) / 2 + lo;
       if (i == j) {
         crc = ccr = xb[idx] * INV_SQRT2;
       } else {
-        const double vs = 0.5 * xb[idx];
+        const double vs = -0.5 * xb[idx];
         const double q = xb[m2 + idx] * INV_2SQRT3;
         crc = r < c ? vs + q : vs - q;
         ccr = r < c ? vs - q : vs + q;
@@ -350,8 +364,9 @@ __global__ void pack_kernel(PairTab T, const double* __restrict__ x, const doubl
       if (k == 0) {
         v = *sig0 + eadd * x[0];
       } else {
-        const int64_t e = k - 1;
-        const int i = (int)(e / nv), a = (int)(e - (int64_t)i * nv);
+        const int64_t w = (k - 1) / nv;  // N-1 walk rank: hole no - 1 - w
+        const int i = no - 1 - (int)w, a = (int)(k - 1 - w * nv);
+        const int64_t e = (int64_t)i * nv + a;
         v = SQRT2 * g[e] + (T.eps[no + a] - T.eps[i] + eadd) * x[k];
       }
       sigma[k] = accum ? sigma[k] + v : v;
@@ -372,7 +387,7 @@ __global__ void pack_kernel(PairTab T, const double* __restrict__ x, const doubl
     if (q < d) {
       const int r = (int)q;
       const double ea = T.eps[no + T.dom_idx[off + r]];
-      v = (i == j) ? Rb[(int64_t)r * d + r] : SQRT2 * Rb[(int64_t)r * d + r];
+      v = (i == j) ? Rb[(int64_t)r * d + r] : -SQRT2 * Rb[(int64_t)r * d + r];
       fock = 2.0 * ea - ei - ej;
     } else {
       int64_t idx = q - d;
@@ -382,7 +397,7 @@ __global__ void pack_kernel(PairTab T, const double* __restrict__ x, const doubl
       tri_decode(idx, hi, lo);
       const double up = Rb[(int64_t)lo * d + hi], dn = Rb[(int64_t)hi * d + lo];
       if (i == j) v = (up + dn) * INV_SQRT2;
-      else v = trip ? SQRT3 * (up - dn) : (up + dn);
+      else v = trip ? SQRT3 * (up - dn) : -(up + dn);   // i > j: aa and singlet CSFs with a minus sign
       fock = T.eps[no + T.dom_idx[off + hi]] + T.eps[no + T.dom_idx[off + lo]] - ei - ej;
     }
     const int64_t k = base + q;
@@ -408,8 +423,8 @@ __global__ void diag_kernel(PairTab T, const double* __restrict__ Kov, const dou
     const int64_t n = 1 + niv;
     for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < n; k += (int64_t)gridDim.x * blockDim.x) {
       if (k == 0) { diag[0] = 0.0; continue; }
-      const int64_t e = k - 1;
-      const int i = (int)(e / nv), a = (int)(e - (int64_t)i * nv);
+      const int64_t w = (k - 1) / nv;  // N-1 walk rank: hole no - 1 - w
+      const int i = no - 1 - (int)w, a = (int)(k - 1 - w * nv);
       diag[k] = T.eps[no + a] - T.eps[i] + 2.0 * Kd(i, a) - Jd(i, a);
     }
     return;

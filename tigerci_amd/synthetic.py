@@ -4,9 +4,11 @@ Every array follows the tci_problem layout of include/tigersigma.h:
   * eps_i ~ U[-1.5,-0.3] (internal), eps_a ~ U[0.5,5.0] (virtual)
   * L^J_pq = L^J_qp ~ N(0,1) exp(-J/80) l_scale, one record of n_chol values
     per orbital pair p>=q (record p(p+1)/2+q)
-  * pair walks (i,j), i>=j, in the order (0,0),(1,0),(1,1),(2,0),...; full
-    domains, or seeded random sorted subsets of size U[lo,hi] with pairs below
-    ``weak_below`` dropped (singles only)
+  * pair walks (i,j), i>=j, in the reference's CI-vector walk order (i
+    descending, then j descending: (n-1,n-1),(n-1,n-2),...,(0,0)); full
+    domains, or seeded random sorted subsets of size U[lo,hi] (drawn in the
+    order (0,0),(1,0),(1,1),(2,0),...) with pairs below ``weak_below``
+    dropped (singles only)
 Independent numpy sub-streams [seed,0..3] make each piece reproducible alone;
 L is drawn in row chunks so an FP32 copy never needs an FP64 intermediate of
 the full size.
@@ -71,6 +73,9 @@ def make_synthetic(n_int: int, n_v: int, n_chol: int, seed: int, domain_range=No
                 dom = np.sort(rng_d.choice(n_v, size, replace=False)).astype(np.int32)
             pairs.append((i, j))
             doms.append(dom)
+    order = sorted(range(len(pairs)), key=lambda k: (-pairs[k][0], -pairs[k][1]))
+    pairs = [pairs[k] for k in order]
+    doms = [doms[k] for k in order]
     dom_off = np.zeros(len(doms) + 1, np.int64)
     dom_off[1:] = np.cumsum([len(d) for d in doms])
     dom_idx = (np.concatenate(doms) if doms else np.zeros(0)).astype(np.int32)
