@@ -131,6 +131,16 @@ int tci_sigma_apply_batch(tci_plan* plan, const double* c, double* sigma, int64_
  * FP64).  Copies over PCIe; synchronous. */
 int tci_htimesc(tci_plan* plan, const double* civec, double* sigmavec, int64_t n);
 
+/* tci_htimesc without device allocation: the same semantics (sigmavec +=
+ * H civec, bare H, FP64, host vectors, synchronous), with civec and sigmavec
+ * staged through a fixed 64 MiB device array that the library holds on every
+ * GPU this is called on, for the life of the process.  n is limited to
+ * 4,194,304 CSFs (TCI_ERR_ARG beyond).  Calls on one GPU share that array and
+ * are serialised by the library.  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc). */
+int tci_htimesc_noalloc(tci_plan* plan, const double* civec, double* sigmavec, int64_t n);
+
 /* diag[k] = H_kk - E_ref (device array of n doubles); the values
  * diag_element (diag_element_mod_2.f90:83) writes to iodiagstore. */
 int tci_diagonal(tci_plan* plan, double* diag, int64_t n, void* stream);

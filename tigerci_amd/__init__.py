@@ -83,6 +83,7 @@ def load_library(path: str | None = None) -> ctypes.CDLL:
     lib.tci_plan_get_info.argtypes = [vp, ctypes.POINTER(_Info)]
     lib.tci_sigma_apply.argtypes = [vp, vp, vp, i64, i32, vp]
     lib.tci_htimesc.argtypes = [vp, vp, vp, i64]
+    lib.tci_htimesc_noalloc.argtypes = [vp, vp, vp, i64]
     lib.tci_diagonal.argtypes = [vp, vp, i64, vp]
     lib.tci_davidson_run.argtypes = [vp, ctypes.POINTER(_DavParams), vp, vp, ctypes.POINTER(_DavResult)]
     lib.tci_selftest_mfma.argtypes = []
@@ -101,7 +102,7 @@ def load_library(path: str | None = None) -> ctypes.CDLL:
     lib.tci_davidson_gen_run.argtypes = [vp, ctypes.POINTER(_DavParams), vp, vp, vp, ctypes.POINTER(_DavResult)]
     lib.tci_acpf_gvalues.argtypes = [i32, i32, ctypes.c_double, vp]
     lib.tci_last_error.restype = ctypes.c_char_p
-    for f in ("tci_plan_create", "tci_plan_get_info", "tci_sigma_apply", "tci_htimesc",
+    for f in ("tci_plan_create", "tci_plan_get_info", "tci_sigma_apply", "tci_htimesc", "tci_htimesc_noalloc",
               "tci_diagonal",
               "tci_davidson_run", "tci_selftest_mfma", "tci_device_alloc", "tci_device_free", "tci_memcpy_h2d",
               "tci_memcpy_d2h", "tci_memcpy_d2d", "tci_device_synchronize", "tci_profile_enable",
