@@ -131,17 +131,41 @@ def kernel_roofline(k: dict, precision: str, traffic) -> dict:
     return r
 
 
-def ladder_section(info, syn, precision: str, seconds: float, fused=()) -> dict:
-    """The whole ladder of one sigma (several launches) against section 8(d)'s
-    roof: F_lad = sum_P 4 d_P^3 n_chol algorithmic flops over the section time,
-    against the dense MFMA peak; the byte floor (L^J[D,D] read once) and the
-    chosen schedule's own minimum bytes beside it."""
-    flops = info.flops_ladder
-    return {"ms": seconds * 1e3, "algorithmic_flops": flops, "tflops": flops / seconds / 1e12,
+def resident_ladder(tuning: str, precision: str) -> bool:
+    """Whether the plan's sigma of this precision takes the ladder from the
+    resident (ab|cd) block (its tuning record line ends in "-> resident")."""
+    return re.search(r"^%s ladder \(ab\|cd\) resident:.*-> resident$" % re.escape(precision), tuning,
+                     re.MULTILINE) is not None
+
+
+def resident_counts(syn):
+    """(n_p, n_m, walks, walks with i != j) of the resident-(ab|cd) product."""
+    n_v = syn.n_v
+    pairs = np.asarray(syn.pairs).reshape(-1, 2)
+    return n_v * (n_v + 1) // 2, n_v * (n_v - 1) // 2, len(pairs), int((pairs[:, 0] != pairs[:, 1]).sum())
+
+
+def ladder_section(info, syn, precision: str, seconds: float, fused=(), resident: bool = False) -> dict:
+    """The whole ladder of one sigma (several launches) against the MFMA roof.
+    Cholesky-direct: F_lad = sum_P 4 d_P^3 n_chol algorithmic flops, byte floor
+    L^J[D,D] read once, and the chosen schedule's own minimum bytes.  Resident
+    (ab|cd): F = 2 n_p^2 n_walks + 2 n_m^2 n_walks(i != j) (the product over
+    the packed symmetric and antisymmetric blocks), byte floor V+- read once
+    (8 bytes an element in FP64, 4 in split form) plus X and S."""
+    if resident:
+        n_p, n_m, nw, nwm = resident_counts(syn)
+        flops = 2.0 * n_p * n_p * nw + 2.0 * n_m * n_m * nwm
+        e = 8.0 if precision == "fp64" else 4.0
+        floor = e * (n_p * n_p + n_m * n_m) + e * (n_p * nw + n_m * nwm) + 8.0 * (n_p * nw + n_m * nwm)
+        sched = floor
+    else:
+        flops = info.flops_ladder
+        floor = ladder_floor_bytes(syn, precision)
+        sched = ladder_alg_bytes(syn, precision, fused)
+    return {"algorithm": "resident (ab|cd)" if resident else "Cholesky-direct",
+            "ms": seconds * 1e3, "algorithmic_flops": flops, "tflops": flops / seconds / 1e12,
             "frac_mfma": flops / seconds / PEAK[precision],
-            "byte_floor": ladder_floor_bytes(syn, precision),
-            "floor_gbs": ladder_floor_bytes(syn, precision) / seconds / 1e9,
-            "schedule_bytes": ladder_alg_bytes(syn, precision, fused)}
+            "byte_floor": floor, "floor_gbs": floor / seconds / 1e9, "schedule_bytes": sched}
 
 
 def cpu_baseline(syn, x, s64_gpu, n_sigma_fp64: int, budget_s: float = 25.0) -> dict:
@@ -281,7 +305,8 @@ def main() -> int:
         out.update({"value": info.flops_sigma / dt / 1e12, "unit": "TFLOP/s", "higher_is_better": True,
                     "ms_per_step": dt * 1e3, "dtype": a.precision, "config": cfg,
                     "roofline": kernel_roofline(dom, a.precision, kernel_traffic(a.workload, a.precision, dom["name"])),
-                    "ladder_section": ladder_section(info, syn, a.precision, lad, fused_ranges(plan.tuning())),
+                    "ladder_section": ladder_section(info, syn, a.precision, lad, fused_ranges(plan.tuning()),
+                                                     resident_ladder(plan.tuning(), a.precision)),
                     "kernels": [dict(name=k["name"], ms_per_sigma=k["ms_per_sigma"],
                                      launches_per_sigma=k["launches_per_sigma"],
                                      tflops=(k["flops"] * k["launches"] / k["ms"] / 1e9) if k["ms"] > 0 else None)
@@ -371,7 +396,8 @@ def main() -> int:
     out["roofline"]["share_of_step"] = dk["ms_per_sigma"] * (n_mix if dprec != "fp64" else n_ref) / 1e3 / t_m
     k64 = max(tab64, key=lambda k: k["ms_per_sigma"])
     out["roofline_fp64"] = kernel_roofline(k64, "fp64", kernel_traffic(a.workload, "fp64", k64["name"]))
-    out["ladder_section"] = {"fp64": ladder_section(info, syn, "fp64", lad)}
+    out["ladder_section"] = {"fp64": ladder_section(info, syn, "fp64", lad,
+                                                    resident=resident_ladder(plan.tuning(), "fp64"))}
     out["kernels"] = {prec: [dict(name=k["name"], ms_per_sigma=k["ms_per_sigma"],
                                   launches_per_sigma=k["launches_per_sigma"],
                                   tflops=(k["flops"] * k["launches"] / k["ms"] / 1e9) if k["ms"] > 0 else None)
@@ -381,7 +407,8 @@ def main() -> int:
         lm = pm["ladder_ms"] / pm["ladder_launches"] / 1e3
         sgm = pm["sigma_ms"] / max(pm["sigma_calls"], 1) / 1e3
         out["sigma_tflops_mixed"] = info.flops_sigma / sgm / 1e12
-        out["ladder_section"][a.precision] = ladder_section(info, syn, a.precision, lm, fused_ranges(plan.tuning()))
+        out["ladder_section"][a.precision] = ladder_section(info, syn, a.precision, lm, fused_ranges(plan.tuning()),
+                                                            resident_ladder(plan.tuning(), a.precision))
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(syn, xr, s64, res_64["sigma_fp64"])
     print(json.dumps(out), flush=True)

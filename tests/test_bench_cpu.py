@@ -83,3 +83,26 @@ def test_kernel_roofline_and_ladder_section():
     assert sec["byte_floor"] == 4.0 * 100 * 150 ** 2 * 1000           # L^J[D,D] once, 4 bytes split
     assert b.ladder_floor_bytes(syn, "fp64") == 8.0 * 100 * 150 ** 2 * 1000
     assert sec["schedule_bytes"] > sec["byte_floor"]                    # T round trip and two reads of L
+
+
+def test_resident_ladder_record_and_section():
+    """The resident-(ab|cd) ladder is read from the tuning record per
+    precision; its section counts the packed products' flops, 2 n_p^2 n_walks
+    + 2 n_m^2 n_walks(i != j), and V+- read once as the byte floor."""
+    b = _bench()
+    t = ("fp64 ladder: -> 8192/64/p1 128\n"
+         "fp64 ladder (ab|cd) resident: 6.8e+11 flops against 6.2e+12 Cholesky-direct; 128/1:12.0ms -> 128/1 "
+         "order 1; resident 12.1ms, Cholesky-direct 200.3ms -> resident\n"
+         "bf16x3 ladder (ab|cd) resident: 128x192/1:3.1ms -> 128x192/1 order 1; resident 3.0ms, "
+         "Cholesky-direct 2.0ms -> Cholesky-direct\n")
+    assert b.resident_ladder(t, "fp64") and not b.resident_ladder(t, "bf16x3")
+    assert not b.resident_ladder(t, "fp16x3")
+    syn = _syn(10, [10, 6, 10], 7)
+    syn.pairs = np.array([[2, 1], [1, 1], [3, 0]], dtype=np.int32)
+    n_p, n_m = 55, 45
+    sec = b.ladder_section(SimpleNamespace(flops_ladder=1.0), syn, "fp64", 0.01, resident=True)
+    assert sec["algorithm"] == "resident (ab|cd)"
+    assert sec["algorithmic_flops"] == 2.0 * n_p ** 2 * 3 + 2.0 * n_m ** 2 * 2
+    assert sec["byte_floor"] >= 8.0 * (n_p ** 2 + n_m ** 2)
+    sx = b.ladder_section(SimpleNamespace(flops_ladder=1.0), syn, "bf16x3", 0.01, resident=True)
+    assert sx["byte_floor"] < sec["byte_floor"]

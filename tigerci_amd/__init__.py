@@ -324,7 +324,7 @@ class Plan:
 
     def tuning(self) -> str:
         """Launch schedules picked by the plan's autotuning (empty when off)."""
-        buf = ctypes.create_string_buffer(4096)
+        buf = ctypes.create_string_buffer(1 << 16)
         _check(self._lib.tci_plan_tuning(self._h, buf, len(buf)))
         return buf.value.decode()
 

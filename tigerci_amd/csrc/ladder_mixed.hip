@@ -871,6 +871,38 @@ void mixed_prepare(tci_plan& P, int precision, hipStream_t st) {
     P.tuning += log + "\n";
   }
   if (P.dense_pairs) dense_pairs_build_split(P, st);
+  // ladder from the resident (ab|cd) block when the plan holds it: timed
+  // against the split Cholesky-direct ladder above (kept when faster) unless
+  // autotuning is off or TCI_LADDER_VV=1; the loser's operand copies are freed
+  P.vv.split = false;
+  if (P.vv.built) {
+    const char* env = std::getenv("TCI_LADDER_VV");
+    const bool force = env && env[0] == '1';
+    const std::string pn = precision == TCI_SIGMA_FP16X3 ? "fp16x3" : "bf16x3";
+    vv_prepare_split(P, precision, st);
+    std::string log = pn + " ladder (ab|cd) resident:";
+    bool use = true;
+    if (autotune_enabled()) {
+      const double tv = vv_tune(P, precision, st, log);
+      if (!force) {
+        const double tc = time_ms(st, 1, [&] { mixed_ladder_run(P, precision, st); });
+        log += "; resident " + fmt_ms(tv) + ", Cholesky-direct " + fmt_ms(tc);
+        use = tv < tc;
+      }
+    }
+    P.vv.split = use;
+    P.tuning += log + (use ? " -> resident\n" : " -> Cholesky-direct\n");
+    TCI_HIP(hipStreamSynchronize(st));
+    if (use) {
+      for (DevBuf* b : {&M.Lx, &M.Lpx, &M.Tx, &M.Rpart}) {
+        M.bytes -= (int64_t)b->bytes;
+        b->reset();
+      }
+    } else {
+      for (DevBuf* b : {&P.vv.Vpx, &P.vv.Vmx, &P.vv.Xpx, &P.vv.Xmx}) b->reset();
+      P.vv.prec = -1;
+    }
+  }
   TCI_HIP(hipStreamSynchronize(st));
   M.ready = true;
 }
@@ -932,6 +964,11 @@ void mixed_ladder_run(tci_plan& P, int precision, hipStream_t st) {
 
 void mixed_ladder(tci_plan& P, int precision, hipStream_t st) {
   mixed_prepare(P, precision, st);
+  if (P.vv.split) {
+    vv_ladder(P, precision, st);
+    if (P.timing()) TCI_HIP(hipEventRecord(P.ev[3], st));
+    return;
+  }
   mixed_ladder_run(P, precision, st);
 }
 
@@ -946,7 +983,7 @@ void mixed_ladder(tci_plan& P, int precision, hipStream_t st) {
 // domain); the schedule reuses the single-vector tuning's fill target.
 bool mixed_batch_supported(const tci_plan& P) {
   const tci_plan::MixedLadder& M = P.mx;
-  return M.ready && !M.merged && (M.prepacked || M.n_local == 0) && P.n_pairs > 0;
+  return M.ready && !P.vv.split && !M.merged && (M.prepacked || M.n_local == 0) && P.n_pairs > 0;
 }
 
 void mixed_batch_prepare(tci_plan& P, int precision, int nvec, hipStream_t st) {

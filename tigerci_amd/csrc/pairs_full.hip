@@ -482,6 +482,14 @@ void dense_pairs_terms(tci_plan& P, int precision, hipStream_t st) {
   const int nt = (nv + 31) / 32;
   const dim3 g(n * nt, n * nt), b(256);
   const EmbedTab E = etab(P);
+  const std::string pn = precision == TCI_SIGMA_FP64 ? "fp64" : (precision == TCI_SIGMA_FP16X3 ? "fp16x3" : "bf16x3");
+  auto run = [&](const std::string& role, const GemmProgram& gp) {
+    P.kt_run(pn + " " + role, gp, precision, st, [&] {
+      if (precision == TCI_SIGMA_FP64) gp.launch(st);
+      else gp.launch_split(precision, st, 0, P.mx.scales.as<float>());
+    });
+  };
+  int e0 = P.kt_begin(st);
   hipLaunchKernelGGL(y_transpose_kernel, dim3((unsigned)cdiv(P.n_chol, 32), nt, n), b, 0, st, P.Y.as<double>(), n, nv,
                      P.n_chol, P.Yt.as<double>());
   if (precision == TCI_SIGMA_FP64) {
@@ -492,9 +500,10 @@ void dense_pairs_terms(tci_plan& P, int precision, hipStream_t st) {
     hipLaunchKernelGGL(relayout_kernel<false>, g, b, 0, st, P.tab, E, P.C.as<double>(), true, 1.0, P.CCb.p, N,
                        nullptr, 0);
     TCI_HIP(hipGetLastError());
-    P.ringZ.launch(st);
-    P.ringZ3.launch(st);
-    P.progWd.launch(st);
+    P.kt_end(e0, pn + " pair relayouts", 0.0, precision, st);
+    run("pair product Z", P.ringZ);
+    run("pair product Z3", P.ringZ3);
+    run("pair product W", P.progWd);
   } else {
     tci_plan::MixedLadder& M = P.mx;
     const int64_t ldx = 2 * (int64_t)n * xpad(nv), ldj = 2 * xpad(P.n_chol);
@@ -515,25 +524,30 @@ void dense_pairs_terms(tci_plan& P, int precision, hipStream_t st) {
     hipLaunchKernelGGL(embed_split_kernel, dim3((unsigned)cdiv(v2, 256), (unsigned)(kh / 8)), b, 0, st, P.tab, E,
                        P.C.as<double>(), M.embTx.as<uint16_t>(), 2 * kh, sv, f16);
     TCI_HIP(hipGetLastError());
+    P.kt_end(e0, pn + " pair relayouts", 0.0, precision, st);
     if (M.pairs_merged) {
-      M.pairsx.launch_split(precision, st, 0, sc);
+      run("pair products Z, Z3, W, 4-internal", M.pairsx);
     } else {
-      M.ringZx.launch_split(precision, st, 0, sc);
-      M.ringZ3x.launch_split(precision, st, 0, sc);
-      M.progWx.launch_split(precision, st, 0, sc);
-      M.holeGx.launch_split(precision, st, 0, sc);
+      run("pair product Z", M.ringZx);
+      run("pair product Z3", M.ringZ3x);
+      run("pair product W", M.progWx);
+      run("4-internal product", M.holeGx);
     }
   }
   if (precision == TCI_SIGMA_FP64) {
+    e0 = P.kt_begin(st);
     hipLaunchKernelGGL(embed_blocks_kernel, dim3(nt * nt, P.n_pairs), b, 0, st, P.tab, E, P.C.as<double>(),
                        P.Cemb.as<double>(), P.CTemb.as<double>());
     TCI_HIP(hipGetLastError());
-    P.holeG.launch(st);
+    P.kt_end(e0, pn + " pair relayouts", 0.0, precision, st);
+    run("4-internal product", P.holeG);
   }
   const int ntd = (P.dmax + 31) / 32;
+  e0 = P.kt_begin(st);
   hipLaunchKernelGGL(ring_combine_kernel, dim3(ntd * ntd, P.n_pairs), b, 0, st, P.tab, P.Zb.as<double>(),
                      P.Z3b.as<double>(), P.Remb.as<double>(), P.R.as<double>());
   TCI_HIP(hipGetLastError());
+  P.kt_end(e0, pn + " ring_combine_kernel", 0.0, precision, st);
 }
 
 }  // namespace tci

@@ -552,6 +552,49 @@ void tci_plan::build(const tci_problem& p) {
     }
   }
   if (!ladder_fused) build_two_phase_ladder(st);
+  // ladder from the resident (ab|cd) block: with autotuning, built when it
+  // fits and its product costs under twice the Cholesky-direct flops, then
+  // timed against the Cholesky-direct ladder (the faster is kept);
+  // TCI_LADDER_VV=1 forces it whenever it fits (with or without autotuning),
+  // TCI_LADDER_VV=0 switches it off
+  if (!ladder_fused && n_pairs > 0) {
+    const char* env = std::getenv("TCI_LADDER_VV");
+    const int force = env && env[0] == '1' ? 1 : (env && env[0] == '0' ? 0 : -1);
+    double f_chol = 0.0;
+    for (int P = 0; P < n_pairs; ++P) f_chol += 4.0 * std::pow((double)dsz(P), 3) * (double)n_chol;
+    const double f_vv = vv_flops(*this);
+    double need = 0.0;
+    const bool fits = force != 0 && vv_fits(*this, &need);
+    const bool tune = autotune_enabled();
+    const bool try_vv = fits && (force == 1 || (tune && f_vv < 2.0 * f_chol));
+    if (force == 1 && !fits) tuning += "fp64 ladder (ab|cd) resident: does not fit (" + std::to_string(need / 1e9) + " GB)\n";
+    if (try_vv) {
+      vv_build(*this, st);
+      char head[160];
+      std::snprintf(head, sizeof head, "fp64 ladder (ab|cd) resident: %.3g flops against %.3g Cholesky-direct;",
+                    f_vv, f_chol);
+      std::string log = head;
+      bool use = true;
+      if (tune) {
+        const double tv = vv_tune(*this, TCI_SIGMA_FP64, st, log);
+        if (force != 1) {
+          const double tc = time_ms(st, 1, [&] { two_phase_ladder(st); });
+          log += "; resident " + fmt_ms(tv) + ", Cholesky-direct " + fmt_ms(tc);
+          use = tv < tc;
+        }
+      }
+      vv.fp64 = use;
+      tuning += log + (use ? " -> resident\n" : " -> Cholesky-direct\n");
+      if (use) {  // the two-phase ladder's buffers are not used again
+        Tbuf.reset();
+        Lp.reset();
+        Rpart.reset();
+        Cw.reset();
+        lchunks.clear();
+        lphase.clear();
+      }
+    }
+  }
 
   // ---- accounting ---------------------------------------------------------------
   flops_ladder = 0.0;
@@ -582,6 +625,7 @@ void tci_plan::build(const tci_problem& p) {
                     &CtRb, &CCb, &Zb, &Z3b, &Cemb, &CTemb, &Remb, &Yt, &holeA, &Tbuf, &Rpart, &Lp, &Cw, &lp_off_all, &Fm,
                     &Fvvp, &Fov})
     device_bytes += (int64_t)b->bytes;
+  device_bytes += vv_device_bytes(*this);
 }
 
 void tci_plan::drop_graphs() {
@@ -946,6 +990,17 @@ void tci_plan::ladder_chunk(const std::pair<int64_t, int>& ch, hipStream_t st) {
   }
 }
 
+// The FP64 two-phase ladder of one sigma: R (zeroed here) += sum over the
+// chunks of Cholesky vectors, then the K-split partials reduced.
+void tci_plan::two_phase_ladder(hipStream_t st) {
+  double* Rz = rpart_n > 1 ? Rpart.as<double>() : R.as<double>();
+  launch_fill(Rz, (int64_t)(rpart_n > 1 ? rpart_n : 1) * sum_d2, 0.0, st);
+  if (ladder_merged) launch_concat_cols(C.as<double>(), n_pairs, n_v, Cw.as<double>(), st);
+  for (const auto& ch : lchunks) ladder_chunk(ch, st);
+  if (timing()) TCI_HIP(hipEventRecord(ev[3], st));
+  if (rpart_n > 1) ladder_reduce(Rpart.as<double>(), sum_d2, rpart_n, R.as<double>(), sum_d2, st);
+}
+
 void tci_plan::unpack_and_ladder(const double* x, const IoSlot* io, int precision, hipStream_t st) {
   launch_unpack(tab, dmax, x, C.as<double>(), Ct.as<double>(), s.as<double>(), st, io);
   if (timing()) TCI_HIP(hipEventRecord(ev[2], st));
@@ -953,13 +1008,13 @@ void tci_plan::unpack_and_ladder(const double* x, const IoSlot* io, int precisio
     mixed_ladder(*this, precision, st);
     return;
   }
-  if (!ladder_fused) {
-    double* Rz = rpart_n > 1 ? Rpart.as<double>() : R.as<double>();
-    launch_fill(Rz, (int64_t)(rpart_n > 1 ? rpart_n : 1) * sum_d2, 0.0, st);
-    if (ladder_merged) launch_concat_cols(C.as<double>(), n_pairs, n_v, Cw.as<double>(), st);
-    for (const auto& ch : lchunks) ladder_chunk(ch, st);
+  if (vv.fp64) {
+    vv_ladder(*this, TCI_SIGMA_FP64, st);
     if (timing()) TCI_HIP(hipEventRecord(ev[3], st));
-    if (rpart_n > 1) ladder_reduce(Rpart.as<double>(), sum_d2, rpart_n, R.as<double>(), sum_d2, st);
+    return;
+  }
+  if (!ladder_fused) {
+    two_phase_ladder(st);
     return;
   }
   LadderArgs a;
@@ -993,9 +1048,7 @@ void tci_plan::sigma_rest(const double* x, double* y, const IoSlot* io, int flag
   kt_end(e0, "rterms_kernel", 0.0, precision, st);
   kt_run("singles Y", progY, precision, st, [&] { progY.launch(st); });
   if (dense_pairs) {
-    e0 = kt_begin(st);
     dense_pairs_terms(*this, precision, st);
-    kt_end(e0, "dense pair products (section)", 0.0, precision, st);
   } else {
     HoleTab H{isect_off.as<int64_t>(), isect.as<int32_t>()};
     e0 = kt_begin(st);

@@ -145,6 +145,23 @@ struct tci_plan {
     int tuned_dma1 = 0;             // autotuned: stages of direct loads for the 128 x 192 phase-1 tiles (0: off)
     std::vector<int> tuned_orders;  // autotuned workgroup order per program role
   } mx;
+  // ladder from the resident virtual block (ab|cd) (ladder_vv.hip): V+- once
+  // per plan, every sigma one product S = X V over all walks
+  struct VVLadder {
+    bool built = false;             // FP64 V+- resident
+    bool fp64 = false;              // the FP64 sigma uses it
+    bool split = false;             // the split sigma of precision `prec` uses it
+    int prec = -1;                  // split copies made for this precision
+    int64_t np = 0, nm = 0;         // packed pair counts n_v(n_v+1)/2, n_v(n_v-1)/2
+    int ncm = 0;                    // walks with i != j (rows of X-)
+    tci::DevBuf Vp, Vm;             // FP64 [np][np], [nm][nm]
+    tci::DevBuf Vpx, Vmx;           // split rows of V+-
+    tci::DevBuf Xp, Xm, Xpx, Xmx;   // X+- (FP64 rows, split rows)
+    tci::DevBuf Sp, Sm;             // S+- partials of the K-splits
+    tci::DevBuf colm;               // [n_pairs] row of the walk in X- or -1
+    int ks = 1, ksx = 1;            // K-splits of the FP64 / split products
+    tci::GemmProgram g64, gx;       // the products (both parts in one launch)
+  } vv;
   tci::PairTab tab{};
   // accounting
   double flops_ladder = 0.0, flops_sigma = 0.0, bytes_ladder = 0.0;
@@ -201,6 +218,7 @@ struct tci_plan {
   void sigma_graph(const double* x, double* y, int flags, hipStream_t st);
   void unpack_and_ladder(const double* x, const tci::IoSlot* io, int precision, hipStream_t st);
   void build_two_phase_ladder(hipStream_t st);
+  void two_phase_ladder(hipStream_t st);
   void ladder_chunk(const std::pair<int64_t, int>& ch, hipStream_t st);
 };
 
@@ -225,6 +243,15 @@ void mixed_batch_ladder(tci_plan& P, int precision, hipStream_t st);
 // fp16x3 scales: mx[slot] = max |x[0..n)|, then set_scales mode (1: ladder, 2: pair products)
 void launch_split_scales(int mode, const double* x, int64_t n, unsigned* mx, int slot, float* sc, float dmax,
                          hipStream_t st);
+// ladder from the resident (ab|cd) block (ladder_vv.hip)
+bool vv_fits(const tci_plan& P, double* bytes_out);
+double vv_flops(const tci_plan& P);
+void vv_build(tci_plan& P, hipStream_t st);
+void vv_set_schedule(tci_plan& P, int precision, int tile, int tile_n, int dma, int ks, hipStream_t st);
+void vv_prepare_split(tci_plan& P, int precision, hipStream_t st);
+void vv_ladder(tci_plan& P, int precision, hipStream_t st);
+double vv_tune(tci_plan& P, int precision, hipStream_t st, std::string& log);
+int64_t vv_device_bytes(const tci_plan& P);
 // dense pair terms (pairs_full.hip)
 bool dense_pairs_choose(const tci_plan& P, double f_gather);
 void dense_pairs_build(tci_plan& P, const std::vector<double>& o4, hipStream_t st);
