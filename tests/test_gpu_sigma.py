@@ -177,6 +177,31 @@ def test_local_domain_pair_paths(dense):
         assert _rel(_gpu_sigma(plan, x, precision="bf16x3"), ref) < 5e-5
 
 
+@pytest.mark.parametrize("z", ["0", "1"])
+def test_dense_pair_z_forms(z):
+    """Z = C~R Kov - CR Jbig + Yt Lov of the dense pair products directly
+    (TCI_Z_LOWRANK=0) and as U Lov - CR Jbig with U = Yt + C~R Lov^T (=1, the
+    default; FP64 and bf16x3, fp16x3 keeps the direct form): both reproduce the
+    oracle with local domains and weak pairs, in every precision."""
+    _say("case: dense pair Z form, TCI_Z_LOWRANK =", z)
+    prob = make_problem(6, 45, 40, 43, domain_range=(9, 45), weak_below=14)
+    ints = Integrals(prob)
+    os.environ["TCI_DENSE_PAIRS"] = "1"
+    os.environ["TCI_Z_LOWRANK"] = z
+    try:
+        plan = tci.Plan.from_problem(prob)
+    finally:
+        del os.environ["TCI_DENSE_PAIRS"]
+        del os.environ["TCI_Z_LOWRANK"]
+    for k in range(2):
+        x = random_ci_vector(prob, 64 + k)
+        ref = oracle_sigma(prob, x, ints)
+        assert _rel(_gpu_sigma(plan, x), ref) < TOL
+        assert _rel(_gpu_sigma(plan, x, precision="bf16x3"), ref) < 5e-5
+        assert _rel(_gpu_sigma(plan, x, precision="fp16x3"), ref) < 2e-6
+        assert _rel(_gpu_sigma(plan, x, precision="bf16x3"), ref) < 5e-5  # back after the switch
+
+
 @pytest.mark.parametrize("args", [
     dict(n_int=34, n_v=6, n_chol=10, seed=81),
     dict(n_int=34, n_v=8, n_chol=12, seed=83, domain_range=(3, 8), weak_below=2),

@@ -13,6 +13,9 @@
 //   Lov[J][(j,b)] = L^J_jb.
 // Then
 //   Z  = CtR Kov - CR Jbig + Yt Lov,   Z3 = Jbig CC,   W[i][J][b] = (Lov CtR)[J][(i,b)]
+// and since Kov = Lov^T Lov, Z = U Lov - CR Jbig with U = Yt + CtR Lov^T
+// (N x n_chol): 2 N^2 n_chol flops for U and as many for U Lov, in place of
+// 2 N^3 for CtR Kov (FP64 and bf16x3; fp16x3 keeps CtR Kov).
 // hold for every ordered (i, j): Z(i,j) = sum_k [Ct^{ik} K^{kj} - C^{ik} J^{kj}] + X^{ij},
 // Z3(j,i) = sum_k J^{kj} C^{ik}, so the ring and 3-external terms of the walk
 // P = (i, j) are, at positions x, y of D_P (a = D_P[x], b = D_P[y]),
@@ -31,6 +34,7 @@
 #include "plan.hpp"
 
 #include <cstdlib>
+#include <initializer_list>
 
 namespace tci {
 
@@ -151,6 +155,22 @@ __global__ void transpose_split_kernel(const double* __restrict__ src, int64_t n
       for (int t = 0; t < 8; ++t) v[t] = tile[8 * ch + t][cy];
       put_split8(out + c * ldo + xidx(k), v, scale_at(scl), f16);
     }
+  }
+}
+
+// dst[c][r] = src[r][c] (rows x cols FP64, 32 x 32 tiles through LDS)
+__global__ void transpose_kernel(const double* __restrict__ src, int64_t rows, int64_t cols, double* __restrict__ dst) {
+  __shared__ double tile[32][33];
+  const int64_t r0 = (int64_t)blockIdx.y * 32, c0 = (int64_t)blockIdx.x * 32;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+  for (int yy = ty; yy < 32; yy += 8) {
+    const int64_t r = r0 + yy, c = c0 + tx;
+    tile[yy][tx] = (r < rows && c < cols) ? src[r * cols + c] : 0.0;
+  }
+  __syncthreads();
+  for (int yy = ty; yy < 32; yy += 8) {
+    const int64_t c = c0 + yy, r = r0 + tx;
+    if (r < rows && c < cols) dst[c * rows + r] = tile[tx][yy];
   }
 }
 
@@ -342,13 +362,34 @@ void dense_pairs_build(tci_plan& P, const std::vector<double>& o4, hipStream_t s
   }
   upload(P.holeA, hA, st);
   const int iN = (int)N, K = (int)P.n_chol;
+  {
+    const char* e = std::getenv("TCI_Z_LOWRANK");
+    P.z_lowrank = !(e && e[0] == '0');
+  }
   P.ringZ.clear();
   P.ringZ.dense = true;
   P.ringZ.tile = 128;
-  P.ringZ.add(P.Zb.as<double>(), N, iN, iN,
-              {dseg(dop(P.CtRb.p, N), dop(P.Kov.p, N), iN, 1.0), dseg(dop(P.CRb.p, N), dop(P.Jbig.p, N), iN, -1.0),
-               dseg(dop(P.Yt.p, P.n_chol), dop(P.Lov.p, N), K, 1.0)},
-              false);
+  P.progU.clear();
+  if (P.z_lowrank) {
+    // Kov = Lov^T Lov, so C~R Kov + Yt Lov = U Lov with U = Yt + C~R Lov^T
+    // (accumulated into Yt after its transpose in every sigma)
+    P.LovT.alloc((size_t)N * P.n_chol * 8);
+    hipLaunchKernelGGL(transpose_kernel, dim3((unsigned)cdiv(N, 32), (unsigned)cdiv(P.n_chol, 32)), dim3(256), 0, st,
+                       P.Lov.as<double>(), P.n_chol, N, P.LovT.as<double>());
+    TCI_HIP(hipGetLastError());
+    P.progU.dense = true;
+    P.progU.tile = 128;
+    P.progU.add(P.Yt.as<double>(), P.n_chol, iN, K, {dseg(dop(P.CtRb.p, N), dop(P.LovT.p, P.n_chol), iN, 1.0)}, true);
+    P.progU.upload_tables(st);
+    P.ringZ.add(P.Zb.as<double>(), N, iN, iN,
+                {dseg(dop(P.Yt.p, P.n_chol), dop(P.Lov.p, N), K, 1.0), dseg(dop(P.CRb.p, N), dop(P.Jbig.p, N), iN, -1.0)},
+                false);
+  } else {
+    P.ringZ.add(P.Zb.as<double>(), N, iN, iN,
+                {dseg(dop(P.CtRb.p, N), dop(P.Kov.p, N), iN, 1.0), dseg(dop(P.CRb.p, N), dop(P.Jbig.p, N), iN, -1.0),
+                 dseg(dop(P.Yt.p, P.n_chol), dop(P.Lov.p, N), K, 1.0)},
+                false);
+  }
   P.ringZ.upload_tables(st);
   P.ringZ3.clear();
   P.ringZ3.dense = true;
@@ -379,7 +420,14 @@ void dense_pairs_build_split(tci_plan& P, hipStream_t st) {
   tci_plan::MixedLadder& M = P.mx;
   const int n = P.n_int, nv = P.n_v;
   const int64_t N = (int64_t)n * nv, ldx = 2 * (int64_t)n * xpad(nv), ldj = 2 * xpad(P.n_chol);
+  // the low-rank Z in bf16x3 only: in fp16x3 the scale of U would have to be
+  // known before U exists (C~R is split with the scale of C, Yt and U together)
+  M.z_lowrank = P.z_lowrank && M.prec == TCI_SIGMA_BF16X3;
   for (DevBuf* b : {&M.Kx, &M.Jx, &M.CRx, &M.CtRx, &M.CCx}) {
+    if (b == &M.Kx && M.z_lowrank) {
+      b->reset();
+      continue;
+    }
     b->alloc((size_t)N * ldx * 2);
     M.bytes += (int64_t)b->bytes;
   }
@@ -390,8 +438,9 @@ void dense_pairs_build_split(tci_plan& P, hipStream_t st) {
   const int64_t work = N * n * (xpad(nv) / 8);
   const float* s0 = M.scales.as<float>();  // constants' scale
   const int f16 = M.prec == TCI_SIGMA_FP16X3 ? 1 : 0;
-  hipLaunchKernelGGL(split_blocked_kernel, dim3(grid_for(work)), dim3(256), 0, st, P.Kov.as<double>(), N, N, n, nv,
-                     M.Kx.as<uint16_t>(), ldx, s0, f16);
+  if (!M.z_lowrank)
+    hipLaunchKernelGGL(split_blocked_kernel, dim3(grid_for(work)), dim3(256), 0, st, P.Kov.as<double>(), N, N, n, nv,
+                       M.Kx.as<uint16_t>(), ldx, s0, f16);
   hipLaunchKernelGGL(split_blocked_kernel, dim3(grid_for(work)), dim3(256), 0, st, P.Jbig.as<double>(), N, N, n, nv,
                      M.Jx.as<uint16_t>(), ldx, s0, f16);
   hipLaunchKernelGGL(split_blocked_kernel, dim3(grid_for(P.n_chol * n * (xpad(nv) / 8))), dim3(256), 0, st,
@@ -411,10 +460,15 @@ void dense_pairs_build_split(tci_plan& P, hipStream_t st) {
   M.ringZx.clear();
   M.ringZx.split = true;
   M.ringZx.tile = ztile;
-  M.ringZx.add(P.Zb.as<double>(), N, iN, iN,
-               {dseg(xop(M.CtRx, ldx), xop(M.Kx, ldx), K, 1.0), dseg(xop(M.CRx, ldx), xop(M.Jx, ldx), K, 1.0),
-                dseg(xop(M.Ytx, ldj), xop(M.LovTx, ldj), KJ, 1.0)},
-               false, false, 0, Blocking(), 5);
+  if (M.z_lowrank)  // Ytx holds the split rows of U = Yt + C~R Lov^T
+    M.ringZx.add(P.Zb.as<double>(), N, iN, iN,
+                 {dseg(xop(M.CRx, ldx), xop(M.Jx, ldx), K, 1.0), dseg(xop(M.Ytx, ldj), xop(M.LovTx, ldj), KJ, 1.0)},
+                 false, false, 0, Blocking(), 5);
+  else
+    M.ringZx.add(P.Zb.as<double>(), N, iN, iN,
+                 {dseg(xop(M.CtRx, ldx), xop(M.Kx, ldx), K, 1.0), dseg(xop(M.CRx, ldx), xop(M.Jx, ldx), K, 1.0),
+                  dseg(xop(M.Ytx, ldj), xop(M.LovTx, ldj), KJ, 1.0)},
+                 false, false, 0, Blocking(), 5);
   M.ringZx.dma = zdma;
   M.ringZx.upload_tables(st);
   M.ringZ3x.clear();
@@ -453,25 +507,45 @@ void dense_pairs_build_split(tci_plan& P, hipStream_t st) {
   M.progWx.add(P.W.as<double>(), nv, (int)P.n_chol, iN, {dseg(xop(M.Lovx, ldx), xop(M.CtRx, ldx), K, 1.0)}, false,
                false, 0, bw, 5);
   M.progWx.upload_tables(st);
-  // With 128 x 128 tiles the four products go out as one launch, tasks with
-  // the longest k first (Z, then Z3 and W, then the 4-internal product):
+  // U[(i,a)][J] = Yt + sum_(j,b) C~R[(i,a)][(j,b)] Lov[J][(j,b)], accumulated into Yt
+  M.progUx.clear();
+  if (M.z_lowrank) {
+    M.progUx.split = true;
+    M.progUx.tile = 128;
+    M.progUx.add(P.Yt.as<double>(), P.n_chol, iN, (int)P.n_chol, {dseg(xop(M.CtRx, ldx), xop(M.Lovx, ldx), K, 1.0)},
+                 true, false, 0, Blocking(), 5);
+    M.progUx.upload_tables(st);
+  }
+  // With 128 x 128 tiles the products go out as one launch, tasks with the
+  // longest k first (Z, then Z3 and W, then the 4-internal product):
   // workgroups take tasks in order as slots free up, so the short tasks fill
-  // the last round instead of each launch ending in a partly empty one
-  M.pairsx.clear();
-  M.pairs_merged = ztile == 128 && !zdma;
-  if (M.pairs_merged) {
-    M.pairsx.split = true;
-    M.pairsx.tile = 128;
-    for (const GemmProgram* g : {&M.ringZx, &M.ringZ3x, &M.progWx, &M.holeGx}) {
-      const int s0 = (int)M.pairsx.segs.size();
-      for (const Seg& x : g->segs) M.pairsx.segs.push_back(x);
+  // the last round instead of each launch ending in a partly empty one.  With
+  // the low-rank Z, U and W go out first as a launch of their own (Z reads U).
+  auto merge = [](GemmProgram& out, std::initializer_list<const GemmProgram*> gs, hipStream_t s) {
+    out.clear();
+    out.split = true;
+    out.tile = 128;
+    for (const GemmProgram* g : gs) {
+      const int s0 = (int)out.segs.size();
+      for (const Seg& x : g->segs) out.segs.push_back(x);
       for (Task t : g->tasks) {
         t.seg0 += s0;
         t.seg1 += s0;
-        M.pairsx.tasks.push_back(t);
+        out.tasks.push_back(t);
       }
     }
-    M.pairsx.upload_tables(st);
+    out.upload_tables(s);
+  };
+  M.pairsx.clear();
+  M.pairsUWx.clear();
+  M.pairs_merged = ztile == 128 && !zdma;
+  if (M.pairs_merged) {
+    if (M.z_lowrank) {
+      merge(M.pairsUWx, {&M.progUx, &M.progWx}, st);
+      merge(M.pairsx, {&M.ringZx, &M.ringZ3x, &M.holeGx}, st);
+    } else {
+      merge(M.pairsx, {&M.ringZx, &M.ringZ3x, &M.progWx, &M.holeGx}, st);
+    }
   }
 }
 
@@ -501,6 +575,7 @@ void dense_pairs_terms(tci_plan& P, int precision, hipStream_t st) {
                        nullptr, 0);
     TCI_HIP(hipGetLastError());
     P.kt_end(e0, pn + " pair relayouts", 0.0, precision, st);
+    if (P.z_lowrank) run("pair product U", P.progU);
     run("pair product Z", P.ringZ);
     run("pair product Z3", P.ringZ3);
     run("pair product W", P.progWd);
@@ -517,15 +592,35 @@ void dense_pairs_terms(tci_plan& P, int precision, hipStream_t st) {
     hipLaunchKernelGGL(relayout_kernel<true>, g, b, 0, st, P.tab, E, P.C.as<double>(), false, -1.0, M.CRx.p, ldx, sv,
                        f16);
     hipLaunchKernelGGL(relayout_kernel<true>, g, b, 0, st, P.tab, E, P.C.as<double>(), true, 1.0, M.CCx.p, ldx, sv, f16);
-    hipLaunchKernelGGL(split_blocked_kernel, dim3(grid_for(N * (xpad(P.n_chol) / 8))), b, 0, st, P.Yt.as<double>(), N,
-                       P.n_chol, 1, (int)P.n_chol, M.Ytx.as<uint16_t>(), ldj, sv, f16);
+    if (!M.z_lowrank)
+      hipLaunchKernelGGL(split_blocked_kernel, dim3(grid_for(N * (xpad(P.n_chol) / 8))), b, 0, st, P.Yt.as<double>(), N,
+                         P.n_chol, 1, (int)P.n_chol, M.Ytx.as<uint16_t>(), ldj, sv, f16);
     TCI_HIP(hipGetLastError());
     const int64_t v2 = (int64_t)nv * nv, kh = xpad(2 * (int64_t)P.n_pairs);
     hipLaunchKernelGGL(embed_split_kernel, dim3((unsigned)cdiv(v2, 256), (unsigned)(kh / 8)), b, 0, st, P.tab, E,
                        P.C.as<double>(), M.embTx.as<uint16_t>(), 2 * kh, sv, f16);
     TCI_HIP(hipGetLastError());
     P.kt_end(e0, pn + " pair relayouts", 0.0, precision, st);
-    if (M.pairs_merged) {
+    if (M.z_lowrank) {  // U (into Yt) and W, then the split rows of U, then Z, Z3 and the 4-internal product
+      if (M.pairs_merged) {
+        run("pair products U, W", M.pairsUWx);
+      } else {
+        run("pair product U", M.progUx);
+        run("pair product W", M.progWx);
+      }
+      e0 = P.kt_begin(st);
+      hipLaunchKernelGGL(split_blocked_kernel, dim3(grid_for(N * (xpad(P.n_chol) / 8))), b, 0, st, P.Yt.as<double>(),
+                         N, P.n_chol, 1, (int)P.n_chol, M.Ytx.as<uint16_t>(), ldj, sv, f16);
+      TCI_HIP(hipGetLastError());
+      P.kt_end(e0, pn + " pair relayouts", 0.0, precision, st);
+      if (M.pairs_merged) {
+        run("pair products Z, Z3, 4-internal", M.pairsx);
+      } else {
+        run("pair product Z", M.ringZx);
+        run("pair product Z3", M.ringZ3x);
+        run("4-internal product", M.holeGx);
+      }
+    } else if (M.pairs_merged) {
       run("pair products Z, Z3, W, 4-internal", M.pairsx);
     } else {
       run("pair product Z", M.ringZx);

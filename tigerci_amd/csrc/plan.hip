@@ -612,9 +612,10 @@ void tci_plan::build(const tci_problem& p) {
     f_ring += 2.0 * 2.0 * d * d * (double)n_chol;
   }
   bytes_ladder = bytes;
-  if (dense_pairs) {  // executed products: Z (three terms), Z3, W, the 4-internal GEMM
+  if (dense_pairs) {  // executed products: Z (two or three terms), U, Z3, W, the 4-internal GEMM
     const double N = (double)niv, d2 = (double)n_v * n_v;
-    f_ring = 6.0 * N * N * N + 4.0 * N * N * (double)n_chol + 2.0 * n_pairs * 2.0 * n_pairs * d2;
+    f_ring = (z_lowrank ? 4.0 * N * N * N + 6.0 * N * N * (double)n_chol : 6.0 * N * N * N + 4.0 * N * N * (double)n_chol) +
+             2.0 * n_pairs * 2.0 * n_pairs * d2;
   }
   flops_sigma = flops_ladder + f_ring + 2.0 * (double)n_chol * n_v * niv * 3.0 + 2.0 * (double)niv * niv;
   TCI_HIP(hipDeviceSynchronize());
@@ -622,7 +623,7 @@ void tci_plan::build(const tci_problem& p) {
   for (DevBuf* b : {&pairs, &dom_off, &dom_idx, &cd_off, &aa_off, &ab_off, &eps, &pos, &xmap, &isect_off, &isect,
                     &inv_off, &inv_q, &inv_pos, &gpart_off, &Lvv, &Lov, &Kov, &Jvv, &Mss, &O4, &O3, &vvd, &vvx,
                     &diag, &C, &Ct, &R, &s, &g, &gss, &gW, &Y, &W, &gpart, &part, &sig0, &lad_part, &Jbig, &CRb,
-                    &CtRb, &CCb, &Zb, &Z3b, &Cemb, &CTemb, &Remb, &Yt, &holeA, &Tbuf, &Rpart, &Lp, &Cw, &lp_off_all, &Fm,
+                    &CtRb, &CCb, &Zb, &Z3b, &Cemb, &CTemb, &Remb, &Yt, &holeA, &LovT, &Tbuf, &Rpart, &Lp, &Cw, &lp_off_all, &Fm,
                     &Fvvp, &Fov})
     device_bytes += (int64_t)b->bytes;
   device_bytes += vv_device_bytes(*this);

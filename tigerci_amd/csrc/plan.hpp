@@ -41,8 +41,11 @@ struct tci_plan {
   tci::GemmProgram ring, progY, progW, progGW;
   // pair terms as dense products over embedded pair blocks (pairs_full.hip)
   bool dense_pairs = false;
-  tci::DevBuf pairid_d, Jbig, CRb, CtRb, CCb, Zb, Z3b, Cemb, CTemb, Remb, Yt, holeA;
-  tci::GemmProgram ringZ, ringZ3, holeG, progWd;
+  tci::DevBuf pairid_d, Jbig, CRb, CtRb, CCb, Zb, Z3b, Cemb, CTemb, Remb, Yt, holeA, LovT;
+  tci::GemmProgram ringZ, ringZ3, holeG, progWd, progU;
+  // Kov = Lov^T Lov: the C~R Kov part of Z runs as U = Yt + C~R Lov^T, then
+  // U Lov (2 N^2 n_chol flops in place of 2 N^3); TCI_Z_LOWRANK=0 keeps C~R Kov
+  bool z_lowrank = false;
   tci::LadderLaunch ladder;        // fused single-kernel ladder (TCI_LADDER=fused)
   bool ladder_fused = false;
   // two-phase ladder: per J-chunk of length Lc, T_P = L[D,D] C_P then R_P += sum T L
@@ -95,6 +98,9 @@ struct tci_plan {
     // dense pair terms in split precision (full domains)
     tci::DevBuf Kx, Jx, CRx, CtRx, CCx, Lovx, LovTx, Ytx;
     tci::GemmProgram ringZx, ringZ3x, progWx;
+    // bf16x3 with the low-rank Z: U (into Yt) and W as one launch before Z
+    bool z_lowrank = false;
+    tci::GemmProgram progUx, pairsUWx;
     // 4-internal product in split precision: the coefficient matrix and the
     // embedded [C; C^T] blocks as split rows over k = walk
     tci::DevBuf holeAx, embTx;
