@@ -236,6 +236,42 @@ int tci_plan_tuning(const tci_plan* plan, char* buf, int64_t len);
  * scalar product on the device; 0 when they agree.  No reference counterpart. */
 int tci_selftest_mfma(void);
 
+/* Internal walk table of the CSF space: every walk of the internal DRT that
+ * ends with n_elec - n_ext electrons (n_ext = 0, 1, 2: valence, N-1, N-2
+ * walks), in the order of the reference's lexical walk index, with the keep
+ * flag of its tree search.  Replaces graph_border and drt_maker
+ * (F/graph_mod.f90:101-380), the walk enumeration of tree_searchX
+ * (F/new_tree_search.f90:45-202) with index = 1 + sum of arc weights
+ * (F/locist_mod.f90:394), and the excitation-level and frozen-level tests of
+ * tree_search_internal (F/graph_mod.f90:700-745,963-1028).  references:
+ * n_ref x n_int occupations (0, 1, 2) of the internal levels; the first
+ * n_frozen levels are frozen.  occ (capacity x n_int) and kept (capacity) are
+ * host arrays; *n_walks returns the walk count, and capacity 0 only queries
+ * it.  The walks are generated on the GPU (synchronous). */
+int tci_internal_walks(int32_t n_int, int32_t n_v, int32_t n_elec, int32_t spin_mult, int32_t n_ref,
+                       const int8_t* references, int32_t n_frozen, int32_t n_ext, int64_t capacity,
+                       int8_t* occ, uint8_t* kept, int64_t* n_walks);
+
+/* Allowed virtuals of internal walks under the weak-pairs and truncation-of-
+ * virtuals approximations: find_hole_pairs and find_allowed_virtuals
+ * (F/locist_mod.f90:453-607), WP_throw_out (:210-245) and the table of
+ * distinct lists of add_new_virtual_list (F/allowed_virtuals_mod.f90:163-219).
+ * Inputs (host): n_ref <= 16 references and n_walks walks as internal
+ * occupations (n_ref x n_int, n_walks x n_int), wp[i][j] (n_int x n_int) and
+ * tov[i][j][a] (n_int x n_int x n_v), nonzero where are_interacting_WP(i, j) /
+ * are_interacting_TOV(i, j, a) (F/wp_tov_mod.f90:254-323) hold, read for hole
+ * pairs i <= j.  Outputs (host): status[w] = 0 reference, 1 single excitation
+ * (every virtual), 2 double excitation, 3 double excitation removed by WP
+ * (empty list); list_of_walk[w] = index of its list (-1 for a reference); the
+ * distinct lists in order of first appearance as list_off[0..*n_lists] (room
+ * for n_walks + 1) into list_virt (sorted 0-based virtuals, list_capacity
+ * entries).  A walk that is not a reference, single or double excitation of
+ * the references is an error, as in the reference.  Computed on the GPU. */
+int tci_allowed_virtuals(int32_t n_int, int32_t n_v, int32_t n_ref, const int8_t* references, int64_t n_walks,
+                         const int8_t* walk_occ, const uint8_t* wp, const uint8_t* tov, int32_t* status,
+                         int64_t* list_of_walk, int64_t* list_off, int32_t* list_virt, int64_t list_capacity,
+                         int64_t* n_lists);
+
 const char* tci_last_error(void);
 
 #ifdef __cplusplus

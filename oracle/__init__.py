@@ -18,6 +18,14 @@ sigma     dense pair-block restatement of every sigma term of htimesc
           (htimesc.f90:46-147) for the closed-shell single-reference CSF space,
           plus the exact diagonal H_ii (diag_element_mod_2.f90:83 role).
 davidson  Davidson-Liu restatement (davidson.f90:47-239, 478-723).
+drt       the distinct row table, lexical walk order and the walk-keeping
+          tests of the internal tree search (graph_mod.f90:101-380,963-1028,
+          new_tree_search.f90:45-202).
+spin      SGGA spin tables and contracted cycles (spin_mod.f90:99-1312).
+domains   hole pairs, allowed virtuals and the weak-pairs test
+          (locist_mod.f90:210-245,453-607, allowed_virtuals_mod.f90:163-219).
+cpu_ladder / cpu_sigma  the reference ladder loop in C with OpenMP and BLAS
+          (the timed CPU baseline) and its Python driver.
 
 How each function is pinned (the checks live in ``tests/``):
   * every sigma term and the diagonal: brute-force Slater-determinant CI with
@@ -28,7 +36,13 @@ How each function is pinned (the checks live in ``tests/``):
   * packed ladder vs dense ladder: agreement to round-off;
   * Davidson: converged roots equal numpy.linalg.eigh of the dense H;
   * hermiticity c1.(H c2) == c2.(H c1) and H_ii == (H e_i)_i.
-Parity unpinned: the SGGA spin-function phase of the triplet-coupled pair
-functions and the walk order relative to the reference DRT (neither can be
-fixed without running the reference, which cannot be built here).
+  * the DRT, walk order and kept-walk counts: brute-force walk enumeration
+    and the counts the reference's test outputs print (Methane SDCI and LSDCI,
+    methanol MRACPF);
+  * allowed virtuals: the methane LSDCI truncation counts and an independent
+    statement by hole/particle counting on seeded multireference spaces.
+Parity unpinned: the phase of the N-1 CSFs and of the i = j pair walks
+relative to the valence CSF (the loop types that would fix them are not
+restated); the CSF phase of i > j pair walks is pinned by the restated
+valence <-> N-2 coupling coefficients.
 """

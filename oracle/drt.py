@@ -129,6 +129,32 @@ class DRT:
         return out
 
 
+def check_excitation(occ, references, n_elec: int) -> bool:
+    """check_excitation (F/graph_mod.f90:963-1002): the walk is at most doubly
+    excited from some reference: sum |occ - ref| over the internal levels, plus
+    1 or 2 for one or two electrons outside the internal space, is at most 4."""
+    ext = n_elec - sum(occ)
+    return any(sum(abs(int(o) - int(r)) for o, r in zip(occ, ref)) + ext <= 4 for ref in references)
+
+
+def check_frozen(occ, n_frozen: int) -> bool:
+    """check_frozen (F/graph_mod.f90:1004-1026): the first n_frozen internal
+    levels stay doubly occupied."""
+    return all(int(occ[j]) == 2 for j in range(n_frozen))
+
+
+def internal_walks(n_int: int, n_v: int, n_elec: int, references, n_ext: int, n_frozen: int = 0):
+    """(walks, kept): every walk of the internal graph with n_elec - n_ext
+    electrons, in lexical (index) order, and whether tree_search_internal
+    keeps it (F/graph_mod.f90:700-745: kept walks get their singles count in
+    the index vector, the others kill_number).  ``references`` are internal
+    occupations."""
+    g = DRT(n_int + n_v, n_elec, [list(r) + [0] * n_v for r in references])
+    walks = sorted(g.tree_search(n_int, n_elec - n_ext), key=g.weight)
+    kept = [check_excitation(w, references, n_elec) and check_frozen(w, n_frozen) for w in walks]
+    return walks, kept
+
+
 def closed_shell_walk_order(n_int: int, n_v: int = 4):
     """(singles, pairs): the N-1 walks as their hole orbital (0-based) and the
     N-2 walks as their hole pair (i, j), i >= j (0-based), in the order of the

@@ -101,13 +101,16 @@ def load_library(path: str | None = None) -> ctypes.CDLL:
     lib.tci_sigma_apply_batch.argtypes = [vp, vp, vp, i64, i32, i32, vp]
     lib.tci_davidson_gen_run.argtypes = [vp, ctypes.POINTER(_DavParams), vp, vp, vp, ctypes.POINTER(_DavResult)]
     lib.tci_acpf_gvalues.argtypes = [i32, i32, ctypes.c_double, vp]
+    lib.tci_internal_walks.argtypes = [i32, i32, i32, i32, i32, vp, i32, i32, i64, vp, vp, ctypes.POINTER(i64)]
+    lib.tci_allowed_virtuals.argtypes = [i32, i32, i32, vp, i64, vp, vp, vp, vp, vp, vp, vp, i64,
+                                         ctypes.POINTER(i64)]
     lib.tci_last_error.restype = ctypes.c_char_p
     for f in ("tci_plan_create", "tci_plan_get_info", "tci_sigma_apply", "tci_htimesc", "tci_htimesc_noalloc",
               "tci_diagonal",
               "tci_davidson_run", "tci_selftest_mfma", "tci_device_alloc", "tci_device_free", "tci_memcpy_h2d",
               "tci_memcpy_d2h", "tci_memcpy_d2d", "tci_device_synchronize", "tci_profile_enable",
               "tci_profile_read", "tci_profile_kernels", "tci_plan_tuning", "tci_davidson_gen_run", "tci_acpf_gvalues",
-              "tci_sigma_apply_batch"):
+              "tci_sigma_apply_batch", "tci_internal_walks", "tci_allowed_virtuals"):
         getattr(lib, f).restype = ctypes.c_int
     if path is None:
         _lib = lib
@@ -392,3 +395,45 @@ def htimesc(civec: np.ndarray, sigmavec: np.ndarray, data: Plan) -> None:
     ``subroutine htimesc(civec, sigmavec, data)`` (htimesc.f90:46): civec is
     read-only, sigmavec accumulates (the caller zeroes it, davidson.f90:126)."""
     data.htimesc_host(civec, sigmavec)
+
+
+def internal_walks(n_int: int, n_v: int, n_elec: int, references, n_ext: int, n_frozen: int = 0,
+                   spin_mult: int = 1):
+    """(occupations, kept) of every walk of the internal graph with
+    n_elec - n_ext electrons, in lexical index order (tci_internal_walks).
+    ``references``: internal occupations, one row per reference."""
+    lib = load_library()
+    refs = np.ascontiguousarray(np.asarray(references, dtype=np.int8).reshape(-1, n_int))
+    n = ctypes.c_int64(0)
+    _check(lib.tci_internal_walks(n_int, n_v, n_elec, spin_mult, refs.shape[0], refs.ctypes.data, n_frozen, n_ext,
+                                  0, None, None, ctypes.byref(n)))
+    occ = np.zeros((max(n.value, 1), n_int), np.int8)
+    kept = np.zeros(max(n.value, 1), np.uint8)
+    if n.value:
+        _check(lib.tci_internal_walks(n_int, n_v, n_elec, spin_mult, refs.shape[0], refs.ctypes.data, n_frozen,
+                                      n_ext, n.value, occ.ctypes.data, kept.ctypes.data, ctypes.byref(n)))
+    return occ[:n.value], kept[:n.value].astype(bool)
+
+
+def allowed_virtuals(references, walks, wp, tov):
+    """(status, list_of_walk, lists) of tci_allowed_virtuals for internal walk
+    occupations ``walks`` (n_walks x n_int), wp (n_int x n_int) and tov
+    (n_int x n_int x n_v) interaction tables."""
+    lib = load_library()
+    walks = np.ascontiguousarray(np.asarray(walks, dtype=np.int8))
+    n_walks, n_int = walks.shape
+    refs = np.ascontiguousarray(np.asarray(references, dtype=np.int8).reshape(-1, n_int))
+    wp = np.ascontiguousarray(np.asarray(wp, dtype=np.uint8))
+    tov = np.ascontiguousarray(np.asarray(tov, dtype=np.uint8))
+    n_v = tov.shape[2]
+    status = np.zeros(n_walks, np.int32)
+    low = np.zeros(n_walks, np.int64)
+    off = np.zeros(n_walks + 1, np.int64)
+    cap = n_walks * n_v
+    virt = np.zeros(max(cap, 1), np.int32)
+    nl = ctypes.c_int64(0)
+    _check(lib.tci_allowed_virtuals(n_int, n_v, refs.shape[0], refs.ctypes.data, n_walks, walks.ctypes.data,
+                                    wp.ctypes.data, tov.ctypes.data, status.ctypes.data, low.ctypes.data,
+                                    off.ctypes.data, virt.ctypes.data, cap, ctypes.byref(nl)))
+    lists = [virt[off[k]:off[k + 1]].tolist() for k in range(nl.value)]
+    return status, low, lists
