@@ -178,6 +178,26 @@ module tigersigma
     integer(c_int) function tci_selftest_mfma() bind(c, name="tci_selftest_mfma")
       import :: c_int
     end function
+    ! internal walk table (DRT walks in lexical order, keep flags); capacity 0 queries the count
+    integer(c_int) function tci_internal_walks(n_int, n_v, n_elec, spin_mult, n_ref, references, n_frozen, n_ext, &
+        capacity, occ, kept, n_walks) bind(c, name="tci_internal_walks")
+      import :: c_int, c_int32_t, c_int64_t, c_int8_t
+      integer(c_int32_t), value :: n_int, n_v, n_elec, spin_mult, n_ref, n_frozen, n_ext
+      integer(c_int8_t), intent(in) :: references(*)
+      integer(c_int64_t), value :: capacity
+      integer(c_int8_t), intent(out) :: occ(*), kept(*)
+      integer(c_int64_t), intent(out) :: n_walks
+    end function
+    ! allowed-virtual lists of walks under WP/TOV (find_allowed_virtuals)
+    integer(c_int) function tci_allowed_virtuals(n_int, n_v, n_ref, references, n_walks, walk_occ, wp, tov, status, &
+        list_of_walk, list_off, list_virt, list_capacity, n_lists) bind(c, name="tci_allowed_virtuals")
+      import :: c_int, c_int32_t, c_int64_t, c_int8_t
+      integer(c_int32_t), value :: n_int, n_v, n_ref
+      integer(c_int8_t), intent(in) :: references(*), walk_occ(*), wp(*), tov(*)
+      integer(c_int64_t), value :: n_walks, list_capacity
+      integer(c_int32_t), intent(out) :: status(*), list_virt(*)
+      integer(c_int64_t), intent(out) :: list_of_walk(*), list_off(*), n_lists
+    end function
     type(c_ptr) function tci_last_error_c() bind(c, name="tci_last_error")
       import :: c_ptr
     end function
