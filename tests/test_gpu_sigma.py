@@ -220,3 +220,30 @@ def test_sigma_pair_block_strips(args):
     ref = oracle_sigma(prob, x, ints)
     assert _rel(_gpu_sigma(plan, x), ref) < TOL
     assert _rel(_gpu_sigma(plan, x, precision="bf16x3"), ref) < 5e-5
+
+
+@pytest.mark.parametrize("waves", ["256", "256w8"])
+@pytest.mark.parametrize("precision", ["bf16x3", "fp16x3"])
+def test_split_products_on_256_tiles(precision, waves):
+    """Every split pair product (U, W, Z, Z3, the 4-internal product) on the
+    256 x 256 tiles of the 32 x 32-MFMA kernel (TCI_PAIRS_TILE=256, or 256w8
+    for its eight-wave form), and the resident ladder's product on them too
+    (TCI_VV_TILE): sigma against the
+    oracle, with outputs that are partial tiles (n_v = 45, 6 internal), W's
+    blocked columns and U's accumulation."""
+    _say("case: split products on 256 tiles", precision)
+    prob = make_problem(6, 45, 40, 47, domain_range=(9, 45), weak_below=14)
+    ints = Integrals(prob)
+    env = {"TCI_DENSE_PAIRS": "1", "TCI_PAIRS_TILE": waves, "TCI_VV_TILE": waves, "TCI_LADDER_VV": "1",
+           "TCI_AUTOTUNE": "0"}
+    os.environ.update(env)
+    try:
+        plan = tci.Plan.from_problem(prob)
+        for k in range(2):
+            x = random_ci_vector(prob, 66 + k)
+            ref = oracle_sigma(prob, x, ints)
+            assert _rel(_gpu_sigma(plan, x, precision=precision), ref) < (5e-5 if precision == "bf16x3" else 2e-6)
+        assert "-> resident" in plan.tuning()
+    finally:
+        for k in env:
+            del os.environ[k]

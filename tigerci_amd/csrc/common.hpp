@@ -167,6 +167,7 @@ struct GemmProgram {
   bool split = false;  // split-row operands, C = sum A B^T: dense_split.hip kernel (tile 64 or 128)
   bool scaled = false; // some task has an output scale (set by upload_tables)
   int dma = 0;         // split programs: LDS stages filled by direct loads (0: register-staged)
+  int waves = 4;       // split programs on 256 x 256 tiles: 4 waves of 128 x 128 or 8 of 128 x 64
   int remap = -1;      // workgroup -> task order: 0 plain, 1 XCD-contiguous, -1 default
   double alg_flops = 0.0;  // algorithmic flops of one launch: 2 m n k over valid extents (upload_tables)
   std::vector<Task> tasks;
@@ -180,6 +181,12 @@ struct GemmProgram {
   void launch(hipStream_t st, int64_t J0 = 0) const;
   // name of the kernel a launch runs, e.g. "xgemmd_kernel<192,192,3>"
   std::string kernel_name() const;
+  // Reorder the tiles of every output grid (tasks with one C and one segment
+  // range) into blocks of G x G tiles, row blocks outer, keeping the grids in
+  // their order; G = 0 restores `base`.  Dealt XCD-contiguously (remap 1), the
+  // tiles that run together on one XCD then share G row and G column panels
+  // in its L2.  Results do not depend on the order.
+  void block_order(int G, const std::vector<Task>& base, hipStream_t st);
   // scales: device array the tasks' oscale index into (may be null when unused)
   void launch_split(int precision, hipStream_t st, int64_t J0 = 0, const float* scales = nullptr) const;
 };
@@ -192,6 +199,10 @@ bool autotune_enabled();
 std::string fmt_ms(double ms);
 double time_ms(hipStream_t st, int reps, const std::function<void()>& fn);
 int tune_order(GemmProgram& g, hipStream_t st, const std::function<void()>& launch);
+// tune_block_order: times plain order and XCD-contiguous order with tile
+// blocks of G = 0, 4, 8 and 16, keeps the fastest (unless within 3% of the
+// current order) and returns a short record such as "x8"
+std::string tune_block_order(GemmProgram& g, hipStream_t st, const std::function<void()>& launch);
 
 void launch_ggemm(int tile, const Task* tasks, const Seg* segs, int64_t ntasks, int64_t J0, hipStream_t st,
                   int remap);

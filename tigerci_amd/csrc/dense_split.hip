@@ -31,6 +31,10 @@ void GemmProgram::launch_split(int precision, hipStream_t st, int64_t J0, const 
   TCI_REQUIRE(split, "not a split program");
   TCI_REQUIRE(!scaled || scales, "scaled split tasks need a scales array");
   const int rm = remap < 0 || xcd_remap_forced() ? xcd_remap_default() : remap;
+  if (tile == 256) {
+    launch_xgemm256(precision, waves, d_tasks.as<Task>(), d_segs.as<Seg>(), (int64_t)tasks.size(), J0, scales, st, rm);
+    return;
+  }
   if (dma) {
     TCI_REQUIRE(precision == TCI_SIGMA_BF16X3 || precision == TCI_SIGMA_FP16X3, "split GEMM needs a split precision");
     launch_xgemmd(precision, tile, tile_n > 0 ? tile_n : tile, dma, d_tasks.as<Task>(), d_segs.as<Seg>(),

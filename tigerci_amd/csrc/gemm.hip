@@ -21,6 +21,8 @@
 //   D[i][j] at lane j + 16*(i%4), register i/4.
 #include <cstdlib>
 
+#include <algorithm>
+
 #include "common.hpp"
 
 namespace tci {
@@ -403,6 +405,8 @@ void GemmProgram::add(double* C, int64_t ldc, int M, int N, const std::vector<Se
               "blocked output needs a dense or split program");
   TCI_REQUIRE(oscale < 0 || split, "output scales belong to split programs");
   TCI_REQUIRE(!out_split || (split && !accumulate), "split-row output needs a split program and no accumulation");
+  TCI_REQUIRE(!(split && tile == 256) || (!out_split && orow_div == 0 && (tile_n == 0 || tile_n == 256)),
+              "256 x 256 split tiles write FP64 output only");
   if (split)
     for (const Seg& x : s)
       TCI_REQUIRE(x.alpha == 1.0 && orow_div == 0 && x.A.base && x.B.base && !x.A.rmap && !x.A.cmap && !x.B.rmap &&
@@ -457,8 +461,40 @@ void GemmProgram::upload_tables(hipStream_t st) {
   upload(d_segs, segs, st);
 }
 
+void GemmProgram::block_order(int G, const std::vector<Task>& base, hipStream_t st) {
+  tasks = base;
+  if (G > 0) {
+    const int tm = tile, tn = (split && tile_n > 0) ? tile_n : tile;
+    // grids in order of first appearance
+    std::vector<std::pair<const double*, int32_t>> keys;
+    std::vector<int> grid(tasks.size());
+    for (size_t k = 0; k < tasks.size(); ++k) {
+      const std::pair<const double*, int32_t> key(tasks[k].C, tasks[k].seg0);
+      size_t g = 0;
+      while (g < keys.size() && keys[g] != key) ++g;
+      if (g == keys.size()) keys.push_back(key);
+      grid[k] = (int)g;
+    }
+    std::vector<size_t> idx(tasks.size());
+    for (size_t k = 0; k < idx.size(); ++k) idx[k] = k;
+    std::stable_sort(idx.begin(), idx.end(), [&](size_t a, size_t b) {
+      const Task &x = base[a], &y = base[b];
+      if (grid[a] != grid[b]) return grid[a] < grid[b];
+      const int xb = x.m0 / (G * tm), yb = y.m0 / (G * tm);
+      if (xb != yb) return xb < yb;
+      const int xc = x.n0 / (G * tn), yc = y.n0 / (G * tn);
+      if (xc != yc) return xc < yc;
+      if (x.m0 != y.m0) return x.m0 < y.m0;
+      return x.n0 < y.n0;
+    });
+    for (size_t k = 0; k < idx.size(); ++k) tasks[k] = base[idx[k]];
+  }
+  upload_tables(st);
+}
+
 std::string GemmProgram::kernel_name() const {
   const std::string t = std::to_string(tile), tn = std::to_string(tile_n > 0 ? tile_n : tile);
+  if (split && tile == 256) return "xgemm256_kernel<" + std::to_string(waves) + ">";
   if (split) return dma ? "xgemmd_kernel<" + t + "," + tn + "," + std::to_string(dma) + ">"
                         : "xgemm_kernel<" + t + "," + tn + ">";
   if (dense) return "dgemm_kernel<" + t + ">";

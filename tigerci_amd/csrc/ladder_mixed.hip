@@ -870,7 +870,60 @@ void mixed_prepare(tci_plan& P, int precision, hipStream_t st) {
     M.tuned_fused_target = bft;
     P.tuning += log + "\n";
   }
-  if (P.dense_pairs) dense_pairs_build_split(P, st);
+  if (P.dense_pairs) {
+    const float* scl = M.scales.as<float>();
+    auto pair_programs = [&] {
+      std::vector<GemmProgram*> gs;
+      if (M.pairs_merged) gs = {&M.pairsUWx, &M.pairsx};
+      else gs = {&M.progUx, &M.ringZx, &M.ringZ3x, &M.progWx, &M.holeGx};
+      return gs;
+    };
+    auto run_pairs = [&] {
+      for (GemmProgram* g : pair_programs())
+        if (!g->tasks.empty()) g->launch_split(precision, st, 0, scl);
+    };
+    const char* te = std::getenv("TCI_PAIRS_TILE");
+    if (autotune_enabled() && !te) {
+      // tiles of the split pair products (128 x 128, or 256 x 256 on 32 x 32
+      // MFMAs with 4 or 8 waves): Z, Z3 and the 4-internal product first, then
+      // U and W; then their tile order
+      std::string log = (precision == TCI_SIGMA_FP16X3 ? "fp16x3" : "bf16x3") + std::string(" pair products: tile");
+      static const char* cname[3] = {"128", "256w4", "256w8"};
+      const int64_t bytes0 = M.bytes;  // each build counts its split copies again
+      auto trial = [&](int cz, int cu) {
+        M.bytes = bytes0;
+        dense_pairs_build_split(P, st, cz, cu);
+        return time_ms(st, 2, run_pairs);
+      };
+      double best = 1e300;
+      int bz = 0, bu = 0;
+      for (int cz = 0; cz < 3; ++cz) {
+        const double ms = trial(cz, 0);
+        log += std::string(" z") + cname[cz] + ":" + fmt_ms(ms);
+        if (ms < best * 0.98) {
+          best = ms;
+          bz = cz;
+        }
+      }
+      if (M.z_lowrank) {  // U and W go out as a launch of their own
+        const double ms = trial(bz, 2);
+        log += std::string(" uw256w8:") + fmt_ms(ms);
+        if (ms < best * 0.98) {
+          best = ms;
+          bu = 2;
+        }
+      } else {
+        bu = bz;
+      }
+      if (!(M.z_lowrank && bu == 2)) trial(bz, bu);
+      log += std::string(" -> z") + cname[bz] + " uw" + cname[bu] + "; order";
+      for (GemmProgram* g : pair_programs())
+        if (!g->tasks.empty()) log += " " + tune_block_order(*g, st, [&] { g->launch_split(precision, st, 0, scl); });
+      P.tuning += log + "\n";
+    } else {
+      dense_pairs_build_split(P, st);
+    }
+  }
   // ladder from the resident (ab|cd) block when the plan holds it: timed
   // against the split Cholesky-direct ladder above (kept when faster) unless
   // autotuning is off or TCI_LADDER_VV=1; the loser's operand copies are freed

@@ -35,6 +35,7 @@
 
 #include <cstdlib>
 #include <initializer_list>
+#include <string>
 
 namespace tci {
 
@@ -416,7 +417,7 @@ void dense_pairs_build(tci_plan& P, const std::vector<double>& o4, hipStream_t s
 }
 
 // split copies of Kov, Jbig, Lov, Lov^T and the split programs for Z, Z3 and W
-void dense_pairs_build_split(tci_plan& P, hipStream_t st) {
+void dense_pairs_build_split(tci_plan& P, hipStream_t st, int cfg_z, int cfg_uw) {
   tci_plan::MixedLadder& M = P.mx;
   const int n = P.n_int, nv = P.n_v;
   const int64_t N = (int64_t)n * nv, ldx = 2 * (int64_t)n * xpad(nv), ldj = 2 * xpad(P.n_chol);
@@ -454,11 +455,21 @@ void dense_pairs_build_split(tci_plan& P, hipStream_t st) {
   // Z and Z3 are square N x N outputs: 128 x 128 tiles, or 192 x 192 with
   // TCI_PAIRS_TILE=192 (each operand panel fetched a third less often), fed by
   // direct-load stages with TCI_PAIRS_TILE=192d
+  // cfg_z (Z, Z3 and the 4-internal product) and cfg_uw (U and W): 0 for
+  // 128 x 128 tiles, 1 and 2 for 256 x 256 tiles of the 32 x 32-MFMA kernel
+  // with 4 or 8 waves; -1 for TCI_PAIRS_TILE (128, 192 or 192d: Z and Z3
+  // alone on 192 x 192 tiles, 256 or 256w8: every product on 256 x 256)
   const char* te = std::getenv("TCI_PAIRS_TILE");
-  const int ztile = te && std::atoi(te) == 192 ? 192 : 128;
-  const int zdma = ztile == 192 && te && te[3] == 'd' ? 3 : 0;
+  const std::string tes = te ? te : "128";
+  if (cfg_z < 0) cfg_z = tes == "256" ? 1 : (tes == "256w8" ? 2 : 0);
+  if (cfg_uw < 0) cfg_uw = cfg_z;
+  const int ztile = cfg_z ? 256 : (tes == "192" || tes == "192d" ? 192 : 128);
+  const int zdma = ztile == 192 && tes == "192d" ? 3 : 0;
+  const int htile = cfg_z ? 256 : 128, pw = cfg_z == 2 ? 8 : 4;      // 4-internal product, waves of Z's 256 tiles
+  const int utile = cfg_uw ? 256 : 128, uw = cfg_uw == 2 ? 8 : 4;   // U and W
   M.ringZx.clear();
   M.ringZx.split = true;
+  M.ringZx.waves = pw;
   M.ringZx.tile = ztile;
   if (M.z_lowrank)  // Ytx holds the split rows of U = Yt + C~R Lov^T
     M.ringZx.add(P.Zb.as<double>(), N, iN, iN,
@@ -473,6 +484,7 @@ void dense_pairs_build_split(tci_plan& P, hipStream_t st) {
   M.ringZx.upload_tables(st);
   M.ringZ3x.clear();
   M.ringZ3x.split = true;
+  M.ringZ3x.waves = pw;
   M.ringZ3x.tile = ztile;
   M.ringZ3x.add(P.Z3b.as<double>(), N, iN, iN, {dseg(xop(M.Jx, ldx), xop(M.CCx, ldx), K, 1.0)}, false, false, 0,
                 Blocking(), 5);
@@ -493,7 +505,8 @@ void dense_pairs_build_split(tci_plan& P, hipStream_t st) {
     TCI_HIP(hipGetLastError());
     M.holeGx.clear();
     M.holeGx.split = true;
-    M.holeGx.tile = 128;
+    M.holeGx.waves = pw;
+    M.holeGx.tile = htile;
     M.holeGx.add(P.Remb.as<double>(), v2, np, (int)v2, {dseg(xop(M.holeAx, ldh), xop(M.embTx, ldh), (int)kh, 1.0)},
                  false, false, 0, Blocking(), 5);
     M.holeGx.upload_tables(st);
@@ -503,7 +516,8 @@ void dense_pairs_build_split(tci_plan& P, hipStream_t st) {
   bw.cstride = P.n_chol * nv;
   M.progWx.clear();
   M.progWx.split = true;
-  M.progWx.tile = 128;
+  M.progWx.waves = uw;
+  M.progWx.tile = utile;
   M.progWx.add(P.W.as<double>(), nv, (int)P.n_chol, iN, {dseg(xop(M.Lovx, ldx), xop(M.CtRx, ldx), K, 1.0)}, false,
                false, 0, bw, 5);
   M.progWx.upload_tables(st);
@@ -511,7 +525,8 @@ void dense_pairs_build_split(tci_plan& P, hipStream_t st) {
   M.progUx.clear();
   if (M.z_lowrank) {
     M.progUx.split = true;
-    M.progUx.tile = 128;
+    M.progUx.waves = uw;
+    M.progUx.tile = utile;
     M.progUx.add(P.Yt.as<double>(), P.n_chol, iN, (int)P.n_chol, {dseg(xop(M.CtRx, ldx), xop(M.Lovx, ldx), K, 1.0)},
                  true, false, 0, Blocking(), 5);
     M.progUx.upload_tables(st);
@@ -524,7 +539,8 @@ void dense_pairs_build_split(tci_plan& P, hipStream_t st) {
   auto merge = [](GemmProgram& out, std::initializer_list<const GemmProgram*> gs, hipStream_t s) {
     out.clear();
     out.split = true;
-    out.tile = 128;
+    out.tile = (*gs.begin())->tile;
+    out.waves = (*gs.begin())->waves;
     for (const GemmProgram* g : gs) {
       const int s0 = (int)out.segs.size();
       for (const Seg& x : g->segs) out.segs.push_back(x);
@@ -538,7 +554,8 @@ void dense_pairs_build_split(tci_plan& P, hipStream_t st) {
   };
   M.pairsx.clear();
   M.pairsUWx.clear();
-  M.pairs_merged = ztile == 128 && !zdma;
+  // one launch holds tasks of one tile shape only
+  M.pairs_merged = (ztile == 128 || ztile == 256) && !zdma && (M.z_lowrank || utile == ztile);
   if (M.pairs_merged) {
     if (M.z_lowrank) {
       merge(M.pairsUWx, {&M.progUx, &M.progWx}, st);

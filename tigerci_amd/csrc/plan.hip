@@ -596,6 +596,15 @@ void tci_plan::build(const tci_problem& p) {
     }
   }
 
+  // order of the dense pair products' tiles (plain, or XCD-contiguous in
+  // blocks of tiles that share operand panels in one XCD's L2)
+  if (dense_pairs && autotune_enabled()) {
+    std::string log = "fp64 pair products: order";
+    for (GemmProgram* g : {&ringZ, &ringZ3, &progU, &progWd, &holeG})
+      if (!g->tasks.empty()) log += " " + tune_block_order(*g, st, [&] { g->launch(st); });
+    tuning += log + "\n";
+  }
+
   // ---- accounting ---------------------------------------------------------------
   flops_ladder = 0.0;
   double f_ring = 0.0, bytes = 0.0;

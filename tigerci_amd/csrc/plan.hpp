@@ -261,6 +261,8 @@ int64_t vv_device_bytes(const tci_plan& P);
 // dense pair terms (pairs_full.hip)
 bool dense_pairs_choose(const tci_plan& P, double f_gather);
 void dense_pairs_build(tci_plan& P, const std::vector<double>& o4, hipStream_t st);
-void dense_pairs_build_split(tci_plan& P, hipStream_t st);
+// cfg_z (Z, Z3, 4-internal) and cfg_uw (U, W): 0 = 128 x 128 tiles, 1 / 2 =
+// 256 x 256 with 4 / 8 waves, -1 = the TCI_PAIRS_TILE choice
+void dense_pairs_build_split(tci_plan& P, hipStream_t st, int cfg_z = -1, int cfg_uw = -1);
 void dense_pairs_terms(tci_plan& P, int precision, hipStream_t st);
 }  // namespace tci

@@ -7,6 +7,8 @@
 //                  D[i][j] lane j+16(i%4), register i/4
 //   bf16/f16 16x16x32: A[i][k] lane i+16(k/8), element k%8; B likewise
 //                  D[i][j] lane j+16(i/4), register i%4
+//   bf16/f16 32x32x16: A[i][k] lane i+32(k/8), element k%8; B likewise
+//                  D[i][j] lane j+32((i/4)%2), register 4(i/8)+i%4
 // Inputs are small integers so every product and sum is exact.
 #include "common.hpp"
 
@@ -60,6 +62,29 @@ __global__ void mfma_layout_kernel(int* __restrict__ bad) {
       double ref = 0.0;
       for (int k = 0; k < 32; ++k) ref += Af(row, k) * Bf(k, col);
       if ((double)acc[r] != ref) ++nbad;
+    }
+  }
+  {  // bf16 and f16 32x32x16
+    typedef float f16v __attribute__((ext_vector_type(16)));
+    const int j = l & 31, g = l >> 5;
+    bf16x8 a, b;
+    h8 ah, bh;
+    for (int t = 0; t < 8; ++t) {
+      a[t] = (__bf16)(float)Af(j, 8 * g + t);
+      b[t] = (__bf16)(float)Bf(8 * g + t, j);
+      ah[t] = (_Float16)(float)Af(j, 8 * g + t);
+      bh[t] = (_Float16)(float)Bf(8 * g + t, j);
+    }
+    f16v acc, acch;
+    for (int r = 0; r < 16; ++r) acc[r] = acch[r] = 0.f;
+    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
+    acch = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, bh, acch, 0, 0, 0);
+    for (int r = 0; r < 16; ++r) {
+      const int row = 8 * (r / 4) + 4 * g + r % 4, col = j;
+      double ref = 0.0;
+      for (int k = 0; k < 16; ++k) ref += Af(row, k) * Bf(k, col);
+      if ((double)acc[r] != ref) ++nbad;
+      if ((double)acch[r] != ref) ++nbad;
     }
   }
   bad[l] = nbad;

@@ -58,4 +58,34 @@ int tune_order(GemmProgram& g, hipStream_t st, const std::function<void()>& laun
   return g.remap;
 }
 
+std::string tune_block_order(GemmProgram& g, hipStream_t st, const std::function<void()>& launch) {
+  if (xcd_remap_forced() || g.tasks.empty()) return "-";
+  const std::vector<Task> base = g.tasks;
+  const int cur_remap = g.remap < 0 ? 0 : g.remap;
+  g.remap = cur_remap;
+  const double t0 = time_ms(st, 2, launch);
+  double best = t0;
+  int bg = 0, bo = cur_remap;
+  for (int o = 0; o < 2; ++o)
+    for (int G : {0, 4, 8, 16}) {
+      if (G == 0 && o == cur_remap) continue;
+      g.block_order(G, base, st);
+      g.remap = o;
+      const double t = time_ms(st, 2, launch);
+      if (t < best) {
+        best = t;
+        bg = G;
+        bo = o;
+      }
+    }
+  if (best > 0.97 * t0) {  // keep the order it was built with
+    bg = 0;
+    bo = cur_remap;
+  }
+  g.block_order(bg, base, st);
+  g.remap = bo;
+  TCI_HIP(hipStreamSynchronize(st));
+  return (bo ? "x" : "p") + std::to_string(bg);
+}
+
 }  // namespace tci

@@ -116,6 +116,54 @@ __device__ __forceinline__ void xstage(const Seg& S, const Task& t, int64_t J0, 
   xload<TN>(S.B, J0, t.N, t.n0, kb, S.K, tid, vb);
 }
 
+// Direct loads of one stage (k-step kb of segment S) into stage buffer sb
+// (A hi, A lo planes of TM rows, then B hi, B lo of TN rows): a plane of R rows
+// is R / 16 instructions of 64 lanes; wave w of NW issues instructions w, w + NW, ...
+template <int TM, int TN, int NW = 4>
+__device__ __forceinline__ void dstage(const Seg& S, const Task& t, int64_t J0, int kb, int w, int lane,
+                                       uint16_t* sb) {
+  constexpr int PA = TM * XK, PB = TN * XK;
+#pragma unroll
+  for (int op = 0; op < 2; ++op) {
+    const Op& O = op ? S.B : S.A;
+    const int rows = op ? t.N : t.M, r0 = op ? t.n0 : t.m0;
+    const uint16_t* base = static_cast<const uint16_t*>(O.base) + J0 * O.jstride;
+    uint16_t* plane = sb + (op ? 2 * PA : 0);
+    const int pl = op ? PB : PA;
+#pragma unroll
+    for (int q = 0; q < (op ? TN : TM) / (16 * NW); ++q) {
+      const int ins = w + NW * q, s = 64 * ins + lane;
+      const int row = s >> 2, ch = (s & 3) ^ ((row >> 1) & 3);
+      const int k = kb + 8 * ch, r = r0 + row;
+      const u32x4* p = (k < S.K && r < rows)
+                           ? reinterpret_cast<const u32x4*>(base + (int64_t)r * O.rs + ((int64_t)(k >> 3) << 4))
+                           : xzero;
+      uint16_t* hi = plane + ins * 512;
+      uint16_t* lo = plane + pl + ins * 512;
+      __builtin_amdgcn_global_load_lds((const void __attribute__((address_space(1)))*)p,
+                                       (void __attribute__((address_space(3)))*)hi, 16, 0, 0);
+      __builtin_amdgcn_global_load_lds((const void __attribute__((address_space(1)))*)(p + 1),
+                                       (void __attribute__((address_space(3)))*)lo, 16, 0, 0);
+    }
+  }
+}
+
+// Next k-step of a task's segment list with the segment index kept in
+// scalar registers, so that the segment reads are scalar loads: a vector load
+// here would make the compiler wait for every outstanding load, the direct
+// loads of the stages in flight included.
+__device__ __forceinline__ bool dadvance(const Seg* segs, int seg1, int& seg, int& kb) {
+  kb += XK;
+  if (kb >= segs[__builtin_amdgcn_readfirstlane(seg)].K) {
+    kb = 0;
+    seg = __builtin_amdgcn_readfirstlane(seg + 1);
+    while (seg < seg1 && segs[seg].K <= 0) seg = __builtin_amdgcn_readfirstlane(seg + 1);
+  }
+  seg = __builtin_amdgcn_readfirstlane(seg);
+  kb = __builtin_amdgcn_readfirstlane(kb);
+  return seg < seg1;
+}
+
 // distance between a wave's consecutive 16 x 16 blocks: 16 (quadrants) or 32
 // (round-robin over two waves, 192 x 192 tiles)
 template <int TM, int TN>
@@ -342,6 +390,10 @@ void launch_xgemm192_bf16(const Task* tasks, const Seg* segs, int64_t ntasks, in
                           hipStream_t st, int remap);
 void launch_xgemm192_f16(const Task* tasks, const Seg* segs, int64_t ntasks, int64_t J0, const float* scales,
                          hipStream_t st, int remap);
+// 256 x 256 tiles on 32 x 32 MFMAs with direct-load stages (xgemm256.hip);
+// FP64 output only
+void launch_xgemm256(int precision, int waves, const Task* tasks, const Seg* segs, int64_t ntasks, int64_t J0,
+                     const float* scales, hipStream_t st, int remap);
 // LDS stages filled by direct loads (xgemm192d.hip): 192 x 192 with 3 stages,
 // 128 x 192 with 2 or 3
 void launch_xgemmd(int precision, int tm, int tn, int nst, const Task* tasks, const Seg* segs, int64_t ntasks,

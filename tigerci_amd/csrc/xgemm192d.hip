@@ -27,54 +27,6 @@ namespace tci {
 
 namespace {
 
-// Direct loads of one stage (k-step kb of segment S) into stage buffer sb
-// (A hi, A lo planes of TM rows, then B hi, B lo of TN rows): a plane of R rows
-// is R / 16 instructions of 64 lanes; wave w issues instructions w, w + 4, ...
-template <int TM, int TN>
-__device__ __forceinline__ void dstage(const Seg& S, const Task& t, int64_t J0, int kb, int w, int lane,
-                                       uint16_t* sb) {
-  constexpr int PA = TM * XK, PB = TN * XK;
-#pragma unroll
-  for (int op = 0; op < 2; ++op) {
-    const Op& O = op ? S.B : S.A;
-    const int rows = op ? t.N : t.M, r0 = op ? t.n0 : t.m0;
-    const uint16_t* base = static_cast<const uint16_t*>(O.base) + J0 * O.jstride;
-    uint16_t* plane = sb + (op ? 2 * PA : 0);
-    const int pl = op ? PB : PA;
-#pragma unroll
-    for (int q = 0; q < (op ? TN : TM) / 64; ++q) {
-      const int ins = w + 4 * q, s = 64 * ins + lane;
-      const int row = s >> 2, ch = (s & 3) ^ ((row >> 1) & 3);
-      const int k = kb + 8 * ch, r = r0 + row;
-      const u32x4* p = (k < S.K && r < rows)
-                           ? reinterpret_cast<const u32x4*>(base + (int64_t)r * O.rs + ((int64_t)(k >> 3) << 4))
-                           : xzero;
-      uint16_t* hi = plane + ins * 512;
-      uint16_t* lo = plane + pl + ins * 512;
-      __builtin_amdgcn_global_load_lds((const void __attribute__((address_space(1)))*)p,
-                                       (void __attribute__((address_space(3)))*)hi, 16, 0, 0);
-      __builtin_amdgcn_global_load_lds((const void __attribute__((address_space(1)))*)(p + 1),
-                                       (void __attribute__((address_space(3)))*)lo, 16, 0, 0);
-    }
-  }
-}
-
-// Next k-step of a task's segment list with the segment index kept in
-// scalar registers, so that the segment reads are scalar loads: a vector load
-// here would make the compiler wait for every outstanding load, the direct
-// loads of the stages in flight included.
-__device__ __forceinline__ bool dadvance(const Seg* segs, int seg1, int& seg, int& kb) {
-  kb += XK;
-  if (kb >= segs[__builtin_amdgcn_readfirstlane(seg)].K) {
-    kb = 0;
-    seg = __builtin_amdgcn_readfirstlane(seg + 1);
-    while (seg < seg1 && segs[seg].K <= 0) seg = __builtin_amdgcn_readfirstlane(seg + 1);
-  }
-  seg = __builtin_amdgcn_readfirstlane(seg);
-  kb = __builtin_amdgcn_readfirstlane(kb);
-  return seg < seg1;
-}
-
 template <int TM, int TN, int NST, int PREC, int NBM, int NBN>
 __device__ __forceinline__ void xgemmd_body(const Task& t, const Seg* __restrict__ segs, int64_t J0, int seg0,
                                             const float* __restrict__ scales, uint16_t* sm, int w, int lane,
