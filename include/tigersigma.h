@@ -272,6 +272,19 @@ int tci_allowed_virtuals(int32_t n_int, int32_t n_v, int32_t n_ref, const int8_t
                          int64_t* list_of_walk, int64_t* list_off, int32_t* list_virt, int64_t list_capacity,
                          int64_t* n_lists);
 
+/* AO -> MO transformation of Cholesky vectors: first_half_transform,
+ * second_half_transform and write_MO_cho_vec (F/cholesky_transform.f90:305-411).
+ * ao_pairs: n_ao_pairs pairs (mu, nu) of 0-based AO indices, each unordered
+ * pair at most once; ao_vec: n_chol x n_ao_pairs, vector J's values on those
+ * pairs (L^J is symmetric); mo_coef: n_ao x n_mo column-major (C[mu + n_ao p],
+ * the reference's molecular_orbitals(mu, p)).  Output: mo_records, the
+ * n_mo (n_mo + 1) / 2 records of n_chol values in the layout tci_problem.chol
+ * reads (record p(p+1)/2 + q, p >= q, holds sum_{mu nu} C[mu][q] C[nu][p]
+ * L^J(mu, nu) over J), and, if norms is not NULL, the 2-norm of every record
+ * (cho_norms).  Host arrays; the transform runs on the GPU (synchronous). */
+int tci_cholesky_ao_to_mo(int32_t n_ao, int32_t n_mo, int64_t n_chol, int64_t n_ao_pairs, const int32_t* ao_pairs,
+                          const double* ao_vec, const double* mo_coef, double* mo_records, double* norms);
+
 const char* tci_last_error(void);
 
 #ifdef __cplusplus

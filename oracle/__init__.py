@@ -24,6 +24,8 @@ drt       the distinct row table, lexical walk order and the walk-keeping
 spin      SGGA spin tables and contracted cycles (spin_mod.f90:99-1312).
 domains   hole pairs, allowed virtuals and the weak-pairs test
           (locist_mod.f90:210-245,453-607, allowed_virtuals_mod.f90:163-219).
+cholesky  the AO -> MO transformation of Cholesky vectors
+          (cholesky_transform.f90:305-411).
 cpu_ladder / cpu_sigma  the reference ladder loop in C with OpenMP and BLAS
           (the timed CPU baseline) and its Python driver.
 
@@ -39,6 +41,8 @@ How each function is pinned (the checks live in ``tests/``):
   * the DRT, walk order and kept-walk counts: brute-force walk enumeration
     and the counts the reference's test outputs print (Methane SDCI and LSDCI,
     methanol MRACPF);
+  * the AO -> MO transform: an independent dense transformation and the
+    four-index identity (pq|rs) = sum_J L_pq L_rs against transformed AO integrals;
   * allowed virtuals: the methane LSDCI truncation counts and an independent
     statement by hole/particle counting on seeded multireference spaces.
 Parity unpinned: the phase of the N-1 CSFs and of the i = j pair walks

@@ -104,13 +104,14 @@ def load_library(path: str | None = None) -> ctypes.CDLL:
     lib.tci_internal_walks.argtypes = [i32, i32, i32, i32, i32, vp, i32, i32, i64, vp, vp, ctypes.POINTER(i64)]
     lib.tci_allowed_virtuals.argtypes = [i32, i32, i32, vp, i64, vp, vp, vp, vp, vp, vp, vp, i64,
                                          ctypes.POINTER(i64)]
+    lib.tci_cholesky_ao_to_mo.argtypes = [i32, i32, i64, i64, vp, vp, vp, vp, vp]
     lib.tci_last_error.restype = ctypes.c_char_p
     for f in ("tci_plan_create", "tci_plan_get_info", "tci_sigma_apply", "tci_htimesc", "tci_htimesc_noalloc",
               "tci_diagonal",
               "tci_davidson_run", "tci_selftest_mfma", "tci_device_alloc", "tci_device_free", "tci_memcpy_h2d",
               "tci_memcpy_d2h", "tci_memcpy_d2d", "tci_device_synchronize", "tci_profile_enable",
               "tci_profile_read", "tci_profile_kernels", "tci_plan_tuning", "tci_davidson_gen_run", "tci_acpf_gvalues",
-              "tci_sigma_apply_batch", "tci_internal_walks", "tci_allowed_virtuals"):
+              "tci_sigma_apply_batch", "tci_internal_walks", "tci_allowed_virtuals", "tci_cholesky_ao_to_mo"):
         getattr(lib, f).restype = ctypes.c_int
     if path is None:
         _lib = lib
@@ -437,3 +438,21 @@ def allowed_virtuals(references, walks, wp, tov):
                                     off.ctypes.data, virt.ctypes.data, cap, ctypes.byref(nl)))
     lists = [virt[off[k]:off[k + 1]].tolist() for k in range(nl.value)]
     return status, low, lists
+
+
+def cholesky_ao_to_mo(ao_pairs, ao_vec, mo_coef, norms: bool = True):
+    """(records, norms) of tci_cholesky_ao_to_mo: ao_pairs (n_pairs x 2, 0-based
+    AO indices), ao_vec (n_chol x n_pairs), mo_coef (n_ao x n_mo, C[mu][p])."""
+    lib = load_library()
+    pairs = np.ascontiguousarray(np.asarray(ao_pairs, dtype=np.int32).reshape(-1, 2))
+    vec = np.ascontiguousarray(np.asarray(ao_vec, dtype=np.float64))
+    C = np.asarray(mo_coef, dtype=np.float64)
+    n_ao, n_mo = C.shape
+    cf = np.ascontiguousarray(C.T)  # column-major C: C[mu + n_ao p]
+    n_chol = vec.shape[0]
+    n_rec = n_mo * (n_mo + 1) // 2
+    rec = np.zeros((n_rec, n_chol))
+    nrm = np.zeros(n_rec) if norms else None
+    _check(lib.tci_cholesky_ao_to_mo(n_ao, n_mo, n_chol, pairs.shape[0], pairs.ctypes.data, vec.ctypes.data,
+                                     cf.ctypes.data, rec.ctypes.data, nrm.ctypes.data if norms else None))
+    return rec, nrm

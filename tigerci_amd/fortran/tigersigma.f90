@@ -198,6 +198,17 @@ module tigersigma
       integer(c_int32_t), intent(out) :: status(*), list_virt(*)
       integer(c_int64_t), intent(out) :: list_of_walk(*), list_off(*), n_lists
     end function
+    ! AO -> MO transform of Cholesky vectors (records in the tci_problem chol layout)
+    integer(c_int) function tci_cholesky_ao_to_mo(n_ao, n_mo, n_chol, n_ao_pairs, ao_pairs, ao_vec, mo_coef, &
+        mo_records, norms) bind(c, name="tci_cholesky_ao_to_mo")
+      import :: c_int, c_int32_t, c_int64_t, c_double, c_ptr
+      integer(c_int32_t), value :: n_ao, n_mo
+      integer(c_int64_t), value :: n_chol, n_ao_pairs
+      integer(c_int32_t), intent(in) :: ao_pairs(*)
+      real(c_double), intent(in) :: ao_vec(*), mo_coef(*)
+      real(c_double), intent(out) :: mo_records(*)
+      type(c_ptr), value :: norms
+    end function
     type(c_ptr) function tci_last_error_c() bind(c, name="tci_last_error")
       import :: c_ptr
     end function
