@@ -106,9 +106,9 @@ def kernel_traffic(workload: str, precision: str, kernel: str):
     for line in open(path):
         if line.startswith("#") or not line.strip():
             continue
-        f = line.rstrip("\n").split("|")
-        if len(f) >= 4 and f[0] == workload and f[1] == precision and f[2] == kernel:
-            return float(f[3])
+        f = line.rstrip("\n").split("|")  # kernel names may hold '|', as in "(ab|cd)"
+        if len(f) >= 4 and f[0] == workload and f[1] == precision and "|".join(f[2:-1]) == kernel:
+            return float(f[-1])
     return None
 
 
@@ -390,6 +390,8 @@ def main() -> int:
         "energies_mixed": [float(v) for v in e_m], "energies_fp64": [float(v) for v in e_64],
         "energy_diff_uEh": float(np.abs(e_m - e_64).max() * 1e6),
         "converged": {"mixed": res_m["converged"], "fp64": res_64["converged"]},
+        "solver_seconds": {"mixed": {"total": res_m["seconds"], "sigma": res_m["seconds_sigma"]},
+                           "fp64": {"total": res_64["seconds"], "sigma": res_64["seconds_sigma"]}},
         "iterations": {"mixed": res_m["iterations"], "fp64": res_64["iterations"],
                        "sigma_mixed": res_m["sigma_mixed"], "sigma_fp64_in_mixed": res_m["sigma_fp64"],
                        "sigma_fp64": res_64["sigma_fp64"]},

@@ -106,3 +106,19 @@ def test_resident_ladder_record_and_section():
     assert sec["byte_floor"] >= 8.0 * (n_p ** 2 + n_m ** 2)
     sx = b.ladder_section(SimpleNamespace(flops_ladder=1.0), syn, "bf16x3", 0.01, resident=True)
     assert sx["byte_floor"] < sec["byte_floor"]
+
+
+def test_kernel_traffic_lookup_keeps_pipes_in_names():
+    """Kernel names such as "fp64 ladder (ab|cd) product ..." hold the field
+    separator; the lookup joins the middle fields back and takes the bytes
+    from the last one.  Every machine-readable line of the committed file is
+    found again under its own name."""
+    b = _bench()
+    path = os.path.join(ROOT, "profiles", "kernel_traffic.txt")
+    lines = [l.rstrip("\n") for l in open(path) if l.strip() and not l.startswith("#")]
+    assert lines
+    for l in lines:
+        f = l.split("|")
+        assert b.kernel_traffic(f[0], f[1], "|".join(f[2:-1])) == float(f[-1])
+    assert any("(ab|cd)" in l for l in lines)
+    assert b.kernel_traffic("davidson_34x192x1200_dom40-192", "fp64", "no such kernel") is None
