@@ -66,6 +66,16 @@ struct XT<TCI_SIGMA_FP16X3> {
 // element offset of k-chunk ch (8 values) of LDS row `row`
 __device__ __forceinline__ int xsw(int row, int ch) { return row * XK + (((ch ^ (row >> 1)) & 3) << 3); }
 
+// The same for the 32 x 32 MFMA fragments of the 256-tile kernel.  The LDS
+// serves a 128-bit read 16 lanes per pass, lanes 8q..8q+7 with the eight lanes
+// 16 further (the grouping the bank-conflict counters of both fragment
+// shapes fit), over 64 banks: rows 4 apart share banks.  The 16 x 16 fragments
+// read chunks 0 and 1 (or 2 and 3) of rows r..r+7 in a pass, which xsw keeps
+// apart; the 32 x 32 fragments read one chunk of rows r..r+7 and r+16..r+23,
+// so the chunk position takes row bits 2 and 4.
+__device__ __forceinline__ int xsw32f(int row) { return ((row >> 1) & 2) | ((row >> 4) & 1); }
+__device__ __forceinline__ int xsw32(int row, int ch) { return row * XK + (((ch ^ xsw32f(row)) & 3) << 3); }
+
 __device__ __forceinline__ bool xadvance(const Seg* segs, int seg1, int& seg, int& kb) {
   kb += XK;
   if (kb >= segs[seg].K) {
@@ -119,7 +129,8 @@ __device__ __forceinline__ void xstage(const Seg& S, const Task& t, int64_t J0, 
 // Direct loads of one stage (k-step kb of segment S) into stage buffer sb
 // (A hi, A lo planes of TM rows, then B hi, B lo of TN rows): a plane of R rows
 // is R / 16 instructions of 64 lanes; wave w of NW issues instructions w, w + NW, ...
-template <int TM, int TN, int NW = 4>
+// Chunks are placed by xsw, or by xsw32 when SW32 is set.
+template <int TM, int TN, int NW = 4, bool SW32 = false>
 __device__ __forceinline__ void dstage(const Seg& S, const Task& t, int64_t J0, int kb, int w, int lane,
                                        uint16_t* sb) {
   constexpr int PA = TM * XK, PB = TN * XK;
@@ -133,7 +144,7 @@ __device__ __forceinline__ void dstage(const Seg& S, const Task& t, int64_t J0, 
 #pragma unroll
     for (int q = 0; q < (op ? TN : TM) / (16 * NW); ++q) {
       const int ins = w + NW * q, s = 64 * ins + lane;
-      const int row = s >> 2, ch = (s & 3) ^ ((row >> 1) & 3);
+      const int row = s >> 2, ch = (s & 3) ^ (SW32 ? xsw32f(row) : ((row >> 1) & 3));
       const int k = kb + 8 * ch, r = r0 + row;
       const u32x4* p = (k < S.K && r < rows)
                            ? reinterpret_cast<const u32x4*>(base + (int64_t)r * O.rs + ((int64_t)(k >> 3) << 4))

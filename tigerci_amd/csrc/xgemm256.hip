@@ -8,7 +8,8 @@
 // 256 accumulator registers): per k-step 32 KiB of fragments feed 96 MFMAs of
 // twice the work, and one workgroup per CU writes 64 KiB of operands into LDS,
 // so the LDS runs at about half its bandwidth.  Both stages (2 x 64 KiB) are
-// filled by direct global-to-LDS loads (dstage, xgemm.hpp); the loads of the
+// filled by direct global-to-LDS loads (dstage, xgemm.hpp, chunks placed by
+// xsw32 so that the fragment reads do not conflict); the loads of the
 // next stage are in flight during the 3 x 32 MFMAs of the current one.
 //
 // An eight-wave form gives each wave 128 x 64 (4 x 2 blocks, 128 accumulator
@@ -57,13 +58,13 @@ __device__ __forceinline__ void x256_step(const uint16_t* base, f16v (&acc)[NBM]
     V ah[NBM], al[NBM], bh[NBN], bl[NBN];
 #pragma unroll
     for (int i = 0; i < NBM; ++i) {
-      const int e = xsw(wm + 32 * i + r, 2 * h + hsel);
+      const int e = xsw32(wm + 32 * i + r, 2 * h + hsel);
       ah[i] = *reinterpret_cast<const V*>(base + e);
       al[i] = *reinterpret_cast<const V*>(base + PA + e);
     }
 #pragma unroll
     for (int j = 0; j < NBN; ++j) {
-      const int e = xsw(wn + 32 * j + r, 2 * h + hsel);
+      const int e = xsw32(wn + 32 * j + r, 2 * h + hsel);
       bh[j] = *reinterpret_cast<const V*>(base + 2 * PA + e);
       bl[j] = *reinterpret_cast<const V*>(base + 3 * PA + e);
     }
@@ -143,7 +144,7 @@ __device__ __forceinline__ void x256_body(const Task& t, const Seg* __restrict__
 #pragma unroll
       for (int e = 0; e < 16; ++e) acc[i][j][e] = 0.f;
   int lseg = seg0, lkb = 0;
-  dstage<T256, T256, NW>(segs[lseg], t, J0, lkb, w, lane, sm);
+  dstage<T256, T256, NW, true>(segs[lseg], t, J0, lkb, w, lane, sm);
   bool lmore = dadvance(segs, t.seg1, lseg, lkb);
   int cb = 0;
   while (true) {
@@ -152,7 +153,7 @@ __device__ __forceinline__ void x256_body(const Task& t, const Seg* __restrict__
     __builtin_amdgcn_s_barrier();  // every wave's loads of this stage are in LDS; the other buffer is free
     bool issued = false;
     if (lmore) {
-      dstage<T256, T256, NW>(segs[lseg], t, J0, lkb, w, lane, sm + (cb ^ 1) * STG);
+      dstage<T256, T256, NW, true>(segs[lseg], t, J0, lkb, w, lane, sm + (cb ^ 1) * STG);
       lmore = dadvance(segs, t.seg1, lseg, lkb);
       issued = true;
     }
@@ -193,18 +194,22 @@ __global__ __launch_bounds__(64 * NW, 1) void xgemm256_kernel(const Task* __rest
     return;
   }
   // every wave takes part in the loads and barriers; a wave whose part lies
-  // outside the output (nbm or nbn 0) computes nothing
-#define TCI_X2(M_, N_)                                                                  \
-  case M_ * 8 + N_:                                                                     \
-    if constexpr (N_ <= BN) x256_body<PREC, NW, M_, N_>(t, segs, J0, seg0, scales, sm, w, lane, wm, wn); \
-    break;
-#define TCI_X2ROW(M_) TCI_X2(M_, 0) TCI_X2(M_, 1) TCI_X2(M_, 2) TCI_X2(M_, 3) TCI_X2(M_, 4)
-  switch (nbm * 8 + nbn) {
-    TCI_X2ROW(0) TCI_X2ROW(1) TCI_X2ROW(2) TCI_X2ROW(3) TCI_X2ROW(4)
-    default: break;
-  }
-#undef TCI_X2ROW
-#undef TCI_X2
+  // outside the output (nbm or nbn 0) computes nothing, and a wave at the edge
+  // computes all its blocks (rows and columns beyond M and N are loaded as
+  // zeros and not stored), which keeps one instantiation per kernel
+  if (nbm > 0 && nbn > 0) x256_body<PREC, NW, 4, BN>(t, segs, J0, seg0, scales, sm, w, lane, wm, wn);
+  else x256_body<PREC, NW, 0, 0>(t, segs, J0, seg0, scales, sm, w, lane, wm, wn);
+}
+
+template <int NW>
+void launch256(int precision, dim3 g, const Task* tasks, const Seg* segs, int64_t J0, const float* scales,
+               hipStream_t st, int remap) {
+  if (precision == TCI_SIGMA_BF16X3)
+    hipLaunchKernelGGL((xgemm256_kernel<TCI_SIGMA_BF16X3, NW>), g, dim3(64 * NW), 0, st, tasks, segs, J0, scales,
+                       remap);
+  else
+    hipLaunchKernelGGL((xgemm256_kernel<TCI_SIGMA_FP16X3, NW>), g, dim3(64 * NW), 0, st, tasks, segs, J0, scales,
+                       remap);
 }
 
 }  // namespace
@@ -215,17 +220,8 @@ void launch_xgemm256(int precision, int waves, const Task* tasks, const Seg* seg
   TCI_REQUIRE(ntasks < ((int64_t)1 << 31), "too many gemm tasks");
   TCI_REQUIRE(precision == TCI_SIGMA_BF16X3 || precision == TCI_SIGMA_FP16X3, "split GEMM needs a split precision");
   const dim3 g((unsigned)ntasks);
-  if (waves == 8) {
-    if (precision == TCI_SIGMA_BF16X3)
-      hipLaunchKernelGGL((xgemm256_kernel<TCI_SIGMA_BF16X3, 8>), g, dim3(512), 0, st, tasks, segs, J0, scales, remap);
-    else
-      hipLaunchKernelGGL((xgemm256_kernel<TCI_SIGMA_FP16X3, 8>), g, dim3(512), 0, st, tasks, segs, J0, scales, remap);
-  } else {
-    if (precision == TCI_SIGMA_BF16X3)
-      hipLaunchKernelGGL((xgemm256_kernel<TCI_SIGMA_BF16X3, 4>), g, dim3(256), 0, st, tasks, segs, J0, scales, remap);
-    else
-      hipLaunchKernelGGL((xgemm256_kernel<TCI_SIGMA_FP16X3, 4>), g, dim3(256), 0, st, tasks, segs, J0, scales, remap);
-  }
+  if (waves == 8) launch256<8>(precision, g, tasks, segs, J0, scales, st, remap);
+  else launch256<4>(precision, g, tasks, segs, J0, scales, st, remap);
   TCI_HIP(hipGetLastError());
 }
 
