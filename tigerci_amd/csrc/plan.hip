@@ -383,11 +383,12 @@ void tci_plan::build(const tci_problem& p) {
   Ct.alloc((size_t)sum_d2 * 8);
   R.alloc((size_t)sum_d2 * 8);
   s.alloc((size_t)niv * 8);
+  sT.alloc((size_t)niv * 8);
   g.alloc((size_t)niv * 8);
   gss.alloc((size_t)niv * 8);
   Y.alloc((size_t)n_chol * niv * 8);
   W.alloc((size_t)n_chol * niv * 8);
-  part.alloc((size_t)std::max(n_pairs, 1) * 8);
+  part.alloc((size_t)std::max(n_pairs, 1) * ref_dot_slices() * 8);
   sig0.alloc(8);
   {
     // K-splits of gW = W L (a product with only n_int rows that streams all of
@@ -485,9 +486,12 @@ void tci_plan::build(const tci_problem& p) {
   }
   if (dense_pairs) dense_pairs_build(*this, o4, st);
   // Y[(J,a)][i] = sum_c L^J_ac s_i^c
+  // (a dense product: L^J rows are contiguous, and s is transposed into sT per sigma)
   progY.clear();
+  progY.dense = true;
+  progY.tile = 64;
   progY.add(Y.as<double>(), n_int, (int)(n_chol * n_v), n_int,
-            {seg(op(Lvv.p, n_v, 1, nullptr, nullptr, lvv_f32), op(s.as<double>(), 1, n_v), n_v, 1.0)}, false);
+            {seg(op(Lvv.p, n_v, 1, nullptr, nullptr, lvv_f32), op(sT.as<double>(), n_int, 1), n_v, 1.0)}, false);
   progY.upload_tables(st);
   // W[i][J][b] = sum_j sum_c Ct^{ij}[pos(b)][c] L^J_{j D[c]}
   progW.clear();
@@ -508,8 +512,10 @@ void tci_plan::build(const tci_problem& p) {
     progW.add(W.as<double>() + (int64_t)i * n_chol * n_v, n_v, (int)n_chol, n_v, sg, false);
   }
   progW.upload_tables(st);
-  // gW[s][i][a] = sum_{J in split s} sum_b W[i][J][b] L^J_ba
+  // gW[s][i][a] = sum_{J in split s} sum_b W[i][J][b] L^J_ba (a dense product)
   progGW.clear();
+  progGW.dense = true;
+  progGW.tile = 64;
   for (int sp = 0; sp < n_split; ++sp) {
     const int64_t J0 = n_chol * sp / n_split, J1 = n_chol * (sp + 1) / n_split;
     const void* Lb = lvv_f32 ? (const void*)(Lvv.as<float>() + J0 * n_v * n_v) : (const void*)(Lvv.as<double>() + J0 * n_v * n_v);
@@ -631,7 +637,7 @@ void tci_plan::build(const tci_problem& p) {
   device_bytes = 0;
   for (DevBuf* b : {&pairs, &dom_off, &dom_idx, &cd_off, &aa_off, &ab_off, &eps, &pos, &xmap, &isect_off, &isect,
                     &inv_off, &inv_q, &inv_pos, &gpart_off, &Lvv, &Lov, &Kov, &Jvv, &Mss, &O4, &O3, &vvd, &vvx,
-                    &diag, &C, &Ct, &R, &s, &g, &gss, &gW, &Y, &W, &gpart, &part, &sig0, &lad_part, &Jbig, &CRb,
+                    &diag, &C, &Ct, &R, &s, &sT, &g, &gss, &gW, &Y, &W, &gpart, &part, &sig0, &lad_part, &Jbig, &CRb,
                     &CtRb, &CCb, &Zb, &Z3b, &Cemb, &CTemb, &Remb, &Yt, &holeA, &LovT, &Tbuf, &Rpart, &Lp, &Cw, &lp_off_all, &Fm,
                     &Fvvp, &Fov})
     device_bytes += (int64_t)b->bytes;
@@ -1056,6 +1062,7 @@ void tci_plan::sigma_rest(const double* x, double* y, const IoSlot* io, int flag
   int e0 = kt_begin(st);
   launch_rterms(tab, dmax, x, s.as<double>(), Kov.as<double>(), O3.as<double>(), R.as<double>(), st, io);
   kt_end(e0, "rterms_kernel", 0.0, precision, st);
+  launch_transpose_small(s.as<double>(), n_int, n_v, sT.as<double>(), st);
   kt_run("singles Y", progY, precision, st, [&] { progY.launch(st); });
   if (dense_pairs) {
     dense_pairs_terms(*this, precision, st);

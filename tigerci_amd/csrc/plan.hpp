@@ -34,7 +34,7 @@ struct tci_plan {
   tci::DevBuf Fm, Fvvp, Fov;
   tci::GemmProgram fockvv;         // R_P += F'[D,D] C_P + C_P F'[D,D]
   // work buffers
-  tci::DevBuf C, Ct, R, s, g, gss, gW, Y, W, gpart, part, sig0, lad_part;
+  tci::DevBuf C, Ct, R, s, sT, g, gss, gW, Y, W, gpart, part, sig0, lad_part;
   int n_split = 1;
   int lad_chunks = 1;
   // programs

@@ -199,30 +199,34 @@ __global__ void hole_kernel(PairTab T, HoleTab H, const double* __restrict__ C, 
   }
 }
 
-__global__ void ref_dot_kernel(PairTab T, const double* __restrict__ Kov, const double* __restrict__ Ct,
-                               double* __restrict__ part) {
-  const int P = blockIdx.x;
+// Row slices of a pair block per ref_dot block: each of a block's four waves
+// takes rows y*4 + w, stepping by the slices' 32 rows, lanes over columns
+// (coalesced reads of C and of the gathered row of Kov).
+constexpr int kRefSlices = 8;
+
+__global__ __launch_bounds__(256) void ref_dot_kernel(PairTab T, const double* __restrict__ Kov,
+                                                      const double* __restrict__ Ct, double* __restrict__ part) {
+  const int P = blockIdx.x, y = blockIdx.y;
   const int i = T.pairs[2 * P], j = T.pairs[2 * P + 1];
   const int64_t off = T.dom_off[P];
   const int d = (int)(T.dom_off[P + 1] - off);
-  const int64_t dd = (int64_t)d * d;
   const int nv = T.n_v;
   const int64_t niv = (int64_t)T.n_int * nv;
   const double* Cb = Ct + T.cd_off[P];
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
   double acc = 0.0;
-  for (int64_t e = threadIdx.x; e < dd; e += blockDim.x) {
-    const int r = (int)(e / d), c = (int)(e - (int64_t)r * d);
-    const int a = T.dom_idx[off + r], b = T.dom_idx[off + c];
-    acc += Kov[((int64_t)i * nv + a) * niv + (int64_t)j * nv + b] * Cb[e];
+  for (int r = y * 4 + wv; r < d; r += kRefSlices * 4) {
+    const double* krow = Kov + ((int64_t)i * nv + T.dom_idx[off + r]) * niv + (int64_t)j * nv;
+    const double* crow = Cb + (int64_t)r * d;
+    for (int c = lane; c < d; c += 64) acc += krow[T.dom_idx[off + c]] * crow[c];
   }
-  __shared__ double red[256];
-  red[threadIdx.x] = acc;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) acc += __shfl_down(acc, o, 64);
+  __shared__ double red[4];
+  if (lane == 0) red[wv] = acc;
   __syncthreads();
-  for (int w = 128; w > 0; w >>= 1) {
-    if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) part[P] = (i == j ? 1.0 : 2.0) * red[0];
+  if (threadIdx.x == 0)
+    part[(int64_t)P * kRefSlices + y] = (i == j ? 1.0 : 2.0) * ((red[0] + red[1]) + (red[2] + red[3]));
 }
 
 __global__ void sum_kernel(const double* __restrict__ part, int n, double* __restrict__ out) {
@@ -663,7 +667,22 @@ __global__ void fov_terms_kernel(const double* x, const IoSlot* io, const double
 
 inline unsigned gx(int64_t work) { return (unsigned)std::min<int64_t>(std::max<int64_t>(cdiv(work, 256), 1), 4096); }
 
+// out[c][r] = in[r][c] (rows x cols, small matrices)
+__global__ void transpose_small_kernel(const double* __restrict__ in, int rows, int cols, double* __restrict__ out) {
+  const int64_t n = (int64_t)rows * cols;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x) {
+    const int r = (int)(e / cols), c = (int)(e - (int64_t)r * cols);
+    out[(int64_t)c * rows + r] = in[e];
+  }
+}
+
 }  // namespace
+
+void launch_transpose_small(const double* in, int rows, int cols, double* out, hipStream_t st) {
+  if ((int64_t)rows * cols <= 0) return;
+  hipLaunchKernelGGL(transpose_small_kernel, dim3(gx((int64_t)rows * cols)), dim3(256), 0, st, in, rows, cols, out);
+  TCI_HIP(hipGetLastError());
+}
 
 void launch_unpack(const PairTab& T, int dmax, const double* x, double* C, double* Ct, double* s,
                    hipStream_t st, const IoSlot* io) {
@@ -688,13 +707,15 @@ void launch_hole(const PairTab& T, const HoleTab& H, int dmax, const double* C, 
   TCI_HIP(hipGetLastError());
 }
 
+int ref_dot_slices() { return kRefSlices; }
+
 void launch_ref_dot(const PairTab& T, const double* Kov, const double* Ct, double* pair_part, double* sig0,
                     hipStream_t st) {
   if (T.n_pairs > 0) {
-    hipLaunchKernelGGL(ref_dot_kernel, dim3(T.n_pairs), dim3(256), 0, st, T, Kov, Ct, pair_part);
+    hipLaunchKernelGGL(ref_dot_kernel, dim3(T.n_pairs, kRefSlices), dim3(256), 0, st, T, Kov, Ct, pair_part);
     TCI_HIP(hipGetLastError());
   }
-  hipLaunchKernelGGL(sum_kernel, dim3(1), dim3(256), 0, st, pair_part, T.n_pairs, sig0);
+  hipLaunchKernelGGL(sum_kernel, dim3(1), dim3(256), 0, st, pair_part, T.n_pairs * kRefSlices, sig0);
   TCI_HIP(hipGetLastError());
 }
 

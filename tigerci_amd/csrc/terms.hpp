@@ -43,9 +43,14 @@ struct HoleTab {
 void launch_hole(const PairTab& T, const HoleTab& H, int dmax, const double* C, const double* O4,
                  double* R, hipStream_t st);
 
-// per-pair <K^{ij}, Ct^{ij}> (x2 for i>j), summed into *sig0
+// per-pair <K^{ij}, Ct^{ij}> (x2 for i>j), summed into *sig0; pair_part holds
+// ref_dot_slices() partials per pair
+int ref_dot_slices();
 void launch_ref_dot(const PairTab& T, const double* Kov, const double* Ct, double* pair_part,
                     double* sig0, hipStream_t st);
+
+// out[c][r] = in[r][c] for a small rows x cols matrix
+void launch_transpose_small(const double* in, int rows, int cols, double* out, hipStream_t st);
 
 // singles <- doubles through (ji|kb): gpart[(2Q+o)][i][pos] = -sum_c Ct^{jk}[pos][c] (ji|kb)
 void launch_s3int_singles(const PairTab& T, int dmax, const int64_t* gpart_off, const double* Ct,
