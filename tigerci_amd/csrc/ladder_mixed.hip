@@ -885,10 +885,10 @@ void mixed_prepare(tci_plan& P, int precision, hipStream_t st) {
     const char* te = std::getenv("TCI_PAIRS_TILE");
     if (autotune_enabled() && !te) {
       // tiles of the split pair products (128 x 128, or 256 x 256 on 32 x 32
-      // MFMAs with 4 or 8 waves): Z, Z3 and the 4-internal product first, then
-      // U and W; then their tile order
+      // MFMAs with 4, 8 or 16 waves): Z, Z3 and the 4-internal product first,
+      // then U and W; then their tile order
       std::string log = (precision == TCI_SIGMA_FP16X3 ? "fp16x3" : "bf16x3") + std::string(" pair products: tile");
-      static const char* cname[3] = {"128", "256w4", "256w8"};
+      static const char* cname[4] = {"128", "256w4", "256w8", "256w16"};
       const int64_t bytes0 = M.bytes;  // each build counts its split copies again
       auto trial = [&](int cz, int cu) {
         M.bytes = bytes0;
@@ -897,7 +897,7 @@ void mixed_prepare(tci_plan& P, int precision, hipStream_t st) {
       };
       double best = 1e300;
       int bz = 0, bu = 0;
-      for (int cz = 0; cz < 3; ++cz) {
+      for (int cz = 0; cz < 4; ++cz) {
         const double ms = trial(cz, 0);
         log += std::string(" z") + cname[cz] + ":" + fmt_ms(ms);
         if (ms < best * 0.98) {

@@ -246,9 +246,9 @@ void build_products(tci_plan& P, GemmProgram& g, bool split, int tile, int tile_
   g.split = split;
   g.tile = tile;
   g.tile_n = split ? tile_n : 0;
-  // 256 x 256 split tiles take `dma` as their wave count (4 or 8); others as direct-load stages
+  // 256 x 256 split tiles take `dma` as their wave count (4, 8 or 16); others as direct-load stages
   g.dma = split && tile == 256 ? 0 : dma;
-  g.waves = split && tile == 256 && dma == 8 ? 8 : 4;
+  g.waves = split && tile == 256 && (dma == 8 || dma == 16) ? dma : 4;
   const int tn = split ? tile_n : tile;
   struct Part {
     const void* X;
@@ -421,7 +421,9 @@ void vv_prepare_split(tci_plan& P, int precision, hipStream_t st) {
   V.prec = precision;
   // untuned schedule: 128 x 192 tiles, or TCI_VV_TILE=256 for the 32 x 32-MFMA kernel
   const char* te = std::getenv("TCI_VV_TILE");
-  if (te && std::atoi(te) == 256) vv_set_schedule(P, precision, 256, 256, std::string(te) == "256w8" ? 8 : 0, 2, st);
+  if (te && std::atoi(te) == 256)
+    vv_set_schedule(P, precision, 256, 256, std::string(te) == "256w8" ? 8 : (std::string(te) == "256w16" ? 16 : 0), 2,
+                    st);
   else vv_set_schedule(P, precision, 128, 192, 0, 1, st);
   TCI_HIP(hipStreamSynchronize(st));
 }
@@ -482,6 +484,7 @@ double vv_tune(tci_plan& P, int precision, hipStream_t st, std::string& log) {
       cands.push_back({192, 192, 3, ks});
       cands.push_back({256, 256, 0, ks});
       cands.push_back({256, 256, 8, ks});  // dma field: eight waves
+      cands.push_back({256, 256, 16, ks});  // sixteen waves
     }
   }
   double best = 1e300;
@@ -490,7 +493,7 @@ double vv_tune(tci_plan& P, int precision, hipStream_t st, std::string& log) {
     vv_set_schedule(P, precision, c.tile, c.tile_n, c.dma, c.ks, st);
     const double ms = time_ms(st, 2, [&] { vv_ladder(P, precision, st); });
     log += " " + std::to_string(c.tile) + (c.tile_n ? "x" + std::to_string(c.tile_n) : "") +
-           (c.dma ? (c.tile == 256 ? "w8" : "d") : "") + "/" + std::to_string(c.ks) + ":" + fmt_ms(ms);
+           (c.dma ? (c.tile == 256 ? "w" + std::to_string(c.dma) : "d") : "") + "/" + std::to_string(c.ks) + ":" + fmt_ms(ms);
     if (ms < best * 0.98) {
       best = ms;
       bc = c;
@@ -504,7 +507,7 @@ double vv_tune(tci_plan& P, int precision, hipStream_t st, std::string& log) {
     else g.launch_split(precision, st, 0, sc);
   });
   log += " -> " + std::to_string(bc.tile) + (bc.tile_n ? "x" + std::to_string(bc.tile_n) : "") +
-         (bc.dma ? (bc.tile == 256 ? "w8" : "d") : "") + "/" + std::to_string(bc.ks) + " order " + std::to_string(o);
+         (bc.dma ? (bc.tile == 256 ? "w" + std::to_string(bc.dma) : "d") : "") + "/" + std::to_string(bc.ks) + " order " + std::to_string(o);
   TCI_HIP(hipStreamSynchronize(st));
   return time_ms(st, 2, [&] { vv_ladder(P, precision, st); });
 }

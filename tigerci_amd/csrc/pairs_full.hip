@@ -456,17 +456,19 @@ void dense_pairs_build_split(tci_plan& P, hipStream_t st, int cfg_z, int cfg_uw)
   // TCI_PAIRS_TILE=192 (each operand panel fetched a third less often), fed by
   // direct-load stages with TCI_PAIRS_TILE=192d
   // cfg_z (Z, Z3 and the 4-internal product) and cfg_uw (U and W): 0 for
-  // 128 x 128 tiles, 1 and 2 for 256 x 256 tiles of the 32 x 32-MFMA kernel
-  // with 4 or 8 waves; -1 for TCI_PAIRS_TILE (128, 192 or 192d: Z and Z3
-  // alone on 192 x 192 tiles, 256 or 256w8: every product on 256 x 256)
+  // 128 x 128 tiles, 1, 2 and 3 for 256 x 256 tiles of the 32 x 32-MFMA
+  // kernel with 4, 8 or 16 waves; -1 for TCI_PAIRS_TILE (128, 192 or 192d: Z
+  // and Z3 alone on 192 x 192 tiles, 256, 256w8 or 256w16: every product on
+  // 256 x 256)
   const char* te = std::getenv("TCI_PAIRS_TILE");
   const std::string tes = te ? te : "128";
-  if (cfg_z < 0) cfg_z = tes == "256" ? 1 : (tes == "256w8" ? 2 : 0);
+  if (cfg_z < 0) cfg_z = tes == "256" ? 1 : (tes == "256w8" ? 2 : (tes == "256w16" ? 3 : 0));
   if (cfg_uw < 0) cfg_uw = cfg_z;
+  static const int waves_of[4] = {4, 4, 8, 16};
   const int ztile = cfg_z ? 256 : (tes == "192" || tes == "192d" ? 192 : 128);
   const int zdma = ztile == 192 && tes == "192d" ? 3 : 0;
-  const int htile = cfg_z ? 256 : 128, pw = cfg_z == 2 ? 8 : 4;      // 4-internal product, waves of Z's 256 tiles
-  const int utile = cfg_uw ? 256 : 128, uw = cfg_uw == 2 ? 8 : 4;   // U and W
+  const int htile = cfg_z ? 256 : 128, pw = waves_of[cfg_z];      // 4-internal product, waves of Z's 256 tiles
+  const int utile = cfg_uw ? 256 : 128, uw = waves_of[cfg_uw];   // U and W
   M.ringZx.clear();
   M.ringZx.split = true;
   M.ringZx.waves = pw;

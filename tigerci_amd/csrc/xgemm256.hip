@@ -166,18 +166,20 @@ __device__ __forceinline__ void x256_body(const Task& t, const Seg* __restrict__
 
 // NW = 4: four waves of 128 x 128 (4 x 4 blocks, one wave per SIMD);
 // NW = 8: eight waves of 128 x 64 (4 x 2 blocks, two waves per SIMD to hide
-// each other's LDS and barrier waits)
+// each other's LDS and barrier waits); NW = 16: sixteen waves of 64 x 64
+// (2 x 2 blocks, four per SIMD)
 template <int PREC, int NW>
 __global__ __launch_bounds__(64 * NW, 1) void xgemm256_kernel(const Task* __restrict__ tasks,
                                                               const Seg* __restrict__ segs, int64_t J0,
                                                               const float* __restrict__ scales, int remap) {
-  constexpr int WN = NW == 4 ? 128 : 64, BN = WN / 32;  // wave tile columns, blocks
+  constexpr int WM = NW == 16 ? 64 : 128, WN = NW == 4 ? 128 : 64;  // wave tile
+  constexpr int BM = WM / 32, BN = WN / 32, WR = T256 / WM;         // blocks; waves along M
   const Task t = tasks[remap ? xcd_task(blockIdx.x, gridDim.x) : blockIdx.x];
   __shared__ __attribute__((aligned(16))) uint16_t sm[2 * 4 * T256 * XK];
   const int tid = threadIdx.x;
   const int lane = tid & 63, w = __builtin_amdgcn_readfirstlane(tid >> 6);
-  const int wm = (w & 1) * 128, wn = (w >> 1) * WN;
-  const int nbm = __builtin_amdgcn_readfirstlane(max(0, min(4, (t.M - (t.m0 + wm) + 31) / 32)));
+  const int wm = (w % WR) * WM, wn = (w / WR) * WN;
+  const int nbm = __builtin_amdgcn_readfirstlane(max(0, min(BM, (t.M - (t.m0 + wm) + 31) / 32)));
   const int nbn = __builtin_amdgcn_readfirstlane(max(0, min(BN, (t.N - (t.n0 + wn) + 31) / 32)));
   int seg0 = t.seg0;
   while (seg0 < t.seg1 && segs[seg0].K <= 0) ++seg0;
@@ -197,7 +199,7 @@ __global__ __launch_bounds__(64 * NW, 1) void xgemm256_kernel(const Task* __rest
   // outside the output (nbm or nbn 0) computes nothing, and a wave at the edge
   // computes all its blocks (rows and columns beyond M and N are loaded as
   // zeros and not stored), which keeps one instantiation per kernel
-  if (nbm > 0 && nbn > 0) x256_body<PREC, NW, 4, BN>(t, segs, J0, seg0, scales, sm, w, lane, wm, wn);
+  if (nbm > 0 && nbn > 0) x256_body<PREC, NW, BM, BN>(t, segs, J0, seg0, scales, sm, w, lane, wm, wn);
   else x256_body<PREC, NW, 0, 0>(t, segs, J0, seg0, scales, sm, w, lane, wm, wn);
 }
 
@@ -220,7 +222,8 @@ void launch_xgemm256(int precision, int waves, const Task* tasks, const Seg* seg
   TCI_REQUIRE(ntasks < ((int64_t)1 << 31), "too many gemm tasks");
   TCI_REQUIRE(precision == TCI_SIGMA_BF16X3 || precision == TCI_SIGMA_FP16X3, "split GEMM needs a split precision");
   const dim3 g((unsigned)ntasks);
-  if (waves == 8) launch256<8>(precision, g, tasks, segs, J0, scales, st, remap);
+  if (waves == 16) launch256<16>(precision, g, tasks, segs, J0, scales, st, remap);
+  else if (waves == 8) launch256<8>(precision, g, tasks, segs, J0, scales, st, remap);
   else launch256<4>(precision, g, tasks, segs, J0, scales, st, remap);
   TCI_HIP(hipGetLastError());
 }
