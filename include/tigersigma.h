@@ -285,6 +285,35 @@ int tci_allowed_virtuals(int32_t n_int, int32_t n_v, int32_t n_ref, const int8_t
 int tci_cholesky_ao_to_mo(int32_t n_ao, int32_t n_mo, int64_t n_chol, int64_t n_ao_pairs, const int32_t* ao_pairs,
                           const double* ao_vec, const double* mo_coef, double* mo_records, double* norms);
 
+/* Spin functions of n_open open shells (2..12) with multiplicity spin_mult,
+ * in the reference's order, and the symmetric-group representation matrices
+ * built on them: branching_diagram and sft_graph (F/spin_mod.f90:99-521),
+ * make_cycles (:523-698) and make_transpositions (:700-860).  With nf =
+ * fsn(n_open, spin_mult), host outputs (each may be NULL):
+ *   sft[(i-1) nf + (w-1)], i = 1..n_open-1: the reference's sft(i, w), the
+ *     signed 1-based index of spin function w under the elementary
+ *     transposition (i, i+1) (0: w changes sign only);
+ *   spin_matrix[v nf + (w-1)], v = 0..n_open: 2S+1 after the shells above v;
+ *   cycles[c nf nf + r nf + s]: matrix of the cycle (l..k), l < k, at label
+ *     c = (k-1)(k-2)/2 + l - 1 (row r, column s, 0-based), (-1)^P included;
+ *   transpositions: matrix of the transposition (i, j), i > j, at label
+ *     (i-1)(i-2)/2 + j - 1, same layout.
+ * Computed on the GPU (synchronous). */
+int tci_spin_tables(int32_t n_open, int32_t spin_mult, int32_t* sft, int32_t* spin_matrix, double* cycles,
+                    double* transpositions);
+
+/* Contracted cycles (l..k)(j..i)|d1,d2 between configurations of `left` and
+ * `right` open shells: contracted_cycle_stateless (F/spin_mod.f90:861-1312),
+ * the spin part of the loop coupling coefficients Px / Py
+ * (F/three_and_four_seg_mod_2.f90:638-997,2267-2383).  records: n_rec x 8
+ * int32 (i, j, k, l, d1, d2, left, right) with the reference's 1-based shell
+ * indices, left <= n_open (<= 12), right <= left; record r writes its
+ * fsn(left) x fsn(right) matrix (row-major) at out + out_off[r], blocks in
+ * record order and not overlapping.  Singlet spin functions (spin_mult 1).
+ * Host arrays; computed on the GPU (synchronous). */
+int tci_contracted_cycles(int32_t n_open, int32_t spin_mult, int64_t n_rec, const int32_t* records,
+                          const int64_t* out_off, double* out);
+
 const char* tci_last_error(void);
 
 #ifdef __cplusplus

@@ -20,6 +20,7 @@ with ``python -m tigerci_amd.build``.
 from __future__ import annotations
 
 import ctypes
+import math
 import os
 from dataclasses import dataclass
 
@@ -105,13 +106,16 @@ def load_library(path: str | None = None) -> ctypes.CDLL:
     lib.tci_allowed_virtuals.argtypes = [i32, i32, i32, vp, i64, vp, vp, vp, vp, vp, vp, vp, i64,
                                          ctypes.POINTER(i64)]
     lib.tci_cholesky_ao_to_mo.argtypes = [i32, i32, i64, i64, vp, vp, vp, vp, vp]
+    lib.tci_spin_tables.argtypes = [i32, i32, vp, vp, vp, vp]
+    lib.tci_contracted_cycles.argtypes = [i32, i32, i64, vp, vp, vp]
     lib.tci_last_error.restype = ctypes.c_char_p
     for f in ("tci_plan_create", "tci_plan_get_info", "tci_sigma_apply", "tci_htimesc", "tci_htimesc_noalloc",
               "tci_diagonal",
               "tci_davidson_run", "tci_selftest_mfma", "tci_device_alloc", "tci_device_free", "tci_memcpy_h2d",
               "tci_memcpy_d2h", "tci_memcpy_d2d", "tci_device_synchronize", "tci_profile_enable",
               "tci_profile_read", "tci_profile_kernels", "tci_plan_tuning", "tci_davidson_gen_run", "tci_acpf_gvalues",
-              "tci_sigma_apply_batch", "tci_internal_walks", "tci_allowed_virtuals", "tci_cholesky_ao_to_mo"):
+              "tci_sigma_apply_batch", "tci_internal_walks", "tci_allowed_virtuals", "tci_cholesky_ao_to_mo",
+              "tci_spin_tables", "tci_contracted_cycles"):
         getattr(lib, f).restype = ctypes.c_int
     if path is None:
         _lib = lib
@@ -456,3 +460,44 @@ def cholesky_ao_to_mo(ao_pairs, ao_vec, mo_coef, norms: bool = True):
     _check(lib.tci_cholesky_ao_to_mo(n_ao, n_mo, n_chol, pairs.shape[0], pairs.ctypes.data, vec.ctypes.data,
                                      cf.ctypes.data, rec.ctypes.data, nrm.ctypes.data if norms else None))
     return rec, nrm
+
+
+def _fsn(n: int, mult: int = 1) -> int:
+    if n < 0 or (n + mult - 1) % 2 or mult - 1 > n:
+        return 0
+    s2 = mult - 1
+    return mult * math.factorial(n) // (math.factorial((n + s2) // 2 + 1) * math.factorial((n - s2) // 2))
+
+
+def spin_tables(n_open: int, spin_mult: int = 1):
+    """dict of tci_spin_tables: sft ((n_open-1) x nf), spin_matrix
+    ((n_open+1) x nf), cycles and transpositions ({(l, k): nf x nf} for l < k,
+    {(i, j): nf x nf} for i > j, 1-based shells as in the reference)."""
+    lib = load_library()
+    nf = _fsn(n_open, spin_mult)
+    nc = n_open * (n_open - 1) // 2
+    sft = np.zeros((max(n_open - 1, 0), nf), dtype=np.int32)
+    sm = np.zeros((n_open + 1, nf), dtype=np.int32)
+    cyc = np.zeros((nc, nf, nf))
+    tr = np.zeros((nc, nf, nf))
+    _check(lib.tci_spin_tables(n_open, spin_mult, sft.ctypes.data, sm.ctypes.data, cyc.ctypes.data, tr.ctypes.data))
+    lab = lambda a, b: (b - 1) * (b - 2) // 2 + a - 1
+    cycles = {(l, k): cyc[lab(l, k)] for k in range(2, n_open + 1) for l in range(1, k)}
+    transp = {(i, j): tr[lab(j, i)] for i in range(2, n_open + 1) for j in range(1, i)}
+    return {"nf": nf, "sft": sft, "spin_matrix": sm, "cycles": cycles, "transpositions": transp}
+
+
+def contracted_cycles(n_open: int, records, spin_mult: int = 1):
+    """List of the matrices of tci_contracted_cycles for records (i, j, k, l,
+    d1, d2, left, right), 1-based shells; each fsn(left) x fsn(right)."""
+    lib = load_library()
+    rec = np.ascontiguousarray(np.asarray(records, dtype=np.int32).reshape(-1, 8))
+    shapes = [(_fsn(int(r[6]), spin_mult), _fsn(int(r[7]), spin_mult)) for r in rec]
+    off = np.zeros(len(rec), dtype=np.int64)
+    tot = 0
+    for q, (a, b) in enumerate(shapes):
+        off[q] = tot
+        tot += a * b
+    out = np.zeros(max(tot, 1))
+    _check(lib.tci_contracted_cycles(n_open, spin_mult, len(rec), rec.ctypes.data, off.ctypes.data, out.ctypes.data))
+    return [out[o:o + a * b].reshape(a, b) for o, (a, b) in zip(off, shapes)]

@@ -209,6 +209,21 @@ module tigersigma
       real(c_double), intent(out) :: mo_records(*)
       type(c_ptr), value :: norms
     end function
+    integer(c_int) function tci_spin_tables(n_open, spin_mult, sft, spin_matrix, cycles, transpositions) &
+        bind(c, name="tci_spin_tables")
+      import :: c_int, c_int32_t, c_ptr
+      integer(c_int32_t), value :: n_open, spin_mult
+      type(c_ptr), value :: sft, spin_matrix, cycles, transpositions
+    end function
+    integer(c_int) function tci_contracted_cycles(n_open, spin_mult, n_rec, records, out_off, out) &
+        bind(c, name="tci_contracted_cycles")
+      import :: c_int, c_int32_t, c_int64_t, c_double
+      integer(c_int32_t), value :: n_open, spin_mult
+      integer(c_int64_t), value :: n_rec
+      integer(c_int32_t), intent(in) :: records(*)
+      integer(c_int64_t), intent(in) :: out_off(*)
+      real(c_double), intent(out) :: out(*)
+    end function
     type(c_ptr) function tci_last_error_c() bind(c, name="tci_last_error")
       import :: c_ptr
     end function
