@@ -314,6 +314,25 @@ int tci_spin_tables(int32_t n_open, int32_t spin_mult, int32_t* sft, int32_t* sp
 int tci_contracted_cycles(int32_t n_open, int32_t spin_mult, int64_t n_rec, const int32_t* records,
                           const int64_t* out_off, double* out);
 
+/* Ladder loop records of the reference's first sigma: the iteration-1 dry run
+ * of all_external_3_4_loops_cho_2 with build_ext_34seg_dryrun
+ * (F/three_and_four_seg_mod_1.f90:318-469,3142-3238) and the record of
+ * abcd_seg_writeData (F/abcd_seg_info_replacement.f90:42-62).  For every
+ * virtual pair a >= c (a outer, both ascending) that is not ignorable, and
+ * every one of the n_walks kept N-2 walks (in the caller's order, the tree
+ * search's) whose allowed-virtual list holds both: one record of 7 int32
+ * (a_g (a_g - 1) / 2 + c_g with orbital indices a_g = n_int + 1 + a,
+ * head_weight, fsn(s), fsn(s - 2) (0 for s < 2), fsn(s + 2), 1-based position
+ * of a and of c in the walk's list), s the walk's open shells num_singles[w],
+ * fsn of multiplicity spin_mult.  Lists: sorted 0-based virtuals at
+ * list_virt[list_off[w] .. list_off[w+1]) (as tci_allowed_virtuals returns
+ * them per walk); ignorable: n_v x n_v flags [c][a] of ignorable_pair, or
+ * NULL for none.  *n_records returns the count; capacity 0 only queries it.
+ * Host arrays; computed on the GPU (synchronous). */
+int tci_ladder_records(int32_t n_int, int32_t n_v, int32_t spin_mult, int64_t n_walks, const int64_t* head_weight,
+                       const int32_t* num_singles, const int64_t* list_off, const int32_t* list_virt,
+                       const uint8_t* ignorable, int64_t capacity, int32_t* records, int64_t* n_records);
+
 const char* tci_last_error(void);
 
 #ifdef __cplusplus

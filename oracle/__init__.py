@@ -26,6 +26,8 @@ domains   hole pairs, allowed virtuals and the weak-pairs test
           (locist_mod.f90:210-245,453-607, allowed_virtuals_mod.f90:163-219).
 cholesky  the AO -> MO transformation of Cholesky vectors
           (cholesky_transform.f90:305-411).
+records   the ladder loop records of the iteration-1 dry run
+          (three_and_four_seg_mod_1.f90:318-469,3142-3238).
 cpu_ladder / cpu_sigma  the reference ladder loop in C with OpenMP and BLAS
           (the timed CPU baseline) and its Python driver.
 
@@ -44,7 +46,10 @@ How each function is pinned (the checks live in ``tests/``):
   * the AO -> MO transform: an independent dense transformation and the
     four-index identity (pq|rs) = sum_J L_pq L_rs against transformed AO integrals;
   * allowed virtuals: the methane LSDCI truncation counts and an independent
-    statement by hole/particle counting on seeded multireference spaces.
+    statement by hole/particle counting on seeded multireference spaces;
+  * ladder records: the closed-form count (every unordered pair of a walk's
+    virtuals), positions that point back at the record's virtuals, the
+    reference's order, and ignorable pairs removing exactly their records.
 Parity unpinned: the phase of the N-1 CSFs and of the i = j pair walks
 relative to the valence CSF (the loop types that would fix them are not
 restated); the CSF phase of i > j pair walks is pinned by the restated

@@ -108,6 +108,7 @@ def load_library(path: str | None = None) -> ctypes.CDLL:
     lib.tci_cholesky_ao_to_mo.argtypes = [i32, i32, i64, i64, vp, vp, vp, vp, vp]
     lib.tci_spin_tables.argtypes = [i32, i32, vp, vp, vp, vp]
     lib.tci_contracted_cycles.argtypes = [i32, i32, i64, vp, vp, vp]
+    lib.tci_ladder_records.argtypes = [i32, i32, i32, i64, vp, vp, vp, vp, vp, i64, vp, vp]
     lib.tci_last_error.restype = ctypes.c_char_p
     for f in ("tci_plan_create", "tci_plan_get_info", "tci_sigma_apply", "tci_htimesc", "tci_htimesc_noalloc",
               "tci_diagonal",
@@ -115,7 +116,7 @@ def load_library(path: str | None = None) -> ctypes.CDLL:
               "tci_memcpy_d2h", "tci_memcpy_d2d", "tci_device_synchronize", "tci_profile_enable",
               "tci_profile_read", "tci_profile_kernels", "tci_plan_tuning", "tci_davidson_gen_run", "tci_acpf_gvalues",
               "tci_sigma_apply_batch", "tci_internal_walks", "tci_allowed_virtuals", "tci_cholesky_ao_to_mo",
-              "tci_spin_tables", "tci_contracted_cycles"):
+              "tci_spin_tables", "tci_contracted_cycles", "tci_ladder_records"):
         getattr(lib, f).restype = ctypes.c_int
     if path is None:
         _lib = lib
@@ -501,3 +502,24 @@ def contracted_cycles(n_open: int, records, spin_mult: int = 1):
     out = np.zeros(max(tot, 1))
     _check(lib.tci_contracted_cycles(n_open, spin_mult, len(rec), rec.ctypes.data, off.ctypes.data, out.ctypes.data))
     return [out[o:o + a * b].reshape(a, b) for o, (a, b) in zip(off, shapes)]
+
+
+def ladder_records(n_int: int, n_v: int, head_weight, num_singles, lists, ignorable=None, spin_mult: int = 1):
+    """n_records x 7 int32 array of tci_ladder_records; lists: per walk the
+    sorted 0-based virtuals; ignorable: n_v x n_v flags [c][a] or None."""
+    lib = load_library()
+    hw = np.ascontiguousarray(np.asarray(head_weight, dtype=np.int64))
+    ns = np.ascontiguousarray(np.asarray(num_singles, dtype=np.int32))
+    off = np.zeros(len(lists) + 1, dtype=np.int64)
+    off[1:] = np.cumsum([len(x) for x in lists])
+    lv = np.ascontiguousarray(np.concatenate([np.asarray(x, dtype=np.int32) for x in lists]) if off[-1] else
+                              np.zeros(1, dtype=np.int32))
+    ig = None if ignorable is None else np.ascontiguousarray(np.asarray(ignorable, dtype=np.uint8))
+    n = ctypes.c_int64(0)
+    args = (n_int, n_v, spin_mult, len(lists), hw.ctypes.data, ns.ctypes.data, off.ctypes.data, lv.ctypes.data,
+            None if ig is None else ig.ctypes.data)
+    _check(lib.tci_ladder_records(*args, 0, None, ctypes.byref(n)))
+    rec = np.zeros((max(n.value, 1), 7), dtype=np.int32)
+    if n.value:
+        _check(lib.tci_ladder_records(*args, n.value, rec.ctypes.data, ctypes.byref(n)))
+    return rec[:n.value]

@@ -24,7 +24,7 @@ FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-Wall", "-Wno-
          "-I", os.path.join(HERE, "..", "include")]
 SOURCES = ["plan.hip", "gemm.hip", "dense_gemm.hip", "dense_split.hip", "pairs_full.hip", "ladder.hip", "ladder_mixed.hip", "terms.hip", "davidson.hip",
            "selftest.hip", "tune.hip", "xgemm192_bf16.hip", "xgemm192_f16.hip", "ladder_fused_split.hip",
-           "ladder_fused128.hip", "ladder_fused192.hip", "xgemm192d.hip", "noalloc.hip", "ladder_vv.hip", "walks.hip", "xgemm256.hip", "chol_transform.hip", "spin.hip"]
+           "ladder_fused128.hip", "ladder_fused192.hip", "xgemm192d.hip", "noalloc.hip", "ladder_vv.hip", "walks.hip", "xgemm256.hip", "chol_transform.hip", "spin.hip", "records.hip"]
 
 
 def _deps(path: str, seen: set) -> set:

@@ -224,6 +224,16 @@ module tigersigma
       integer(c_int64_t), intent(in) :: out_off(*)
       real(c_double), intent(out) :: out(*)
     end function
+    integer(c_int) function tci_ladder_records(n_int, n_v, spin_mult, n_walks, head_weight, num_singles, list_off, &
+        list_virt, ignorable, capacity, records, n_records) bind(c, name="tci_ladder_records")
+      import :: c_int, c_int32_t, c_int64_t, c_ptr
+      integer(c_int32_t), value :: n_int, n_v, spin_mult
+      integer(c_int64_t), value :: n_walks, capacity
+      integer(c_int64_t), intent(in) :: head_weight(*), list_off(*)
+      integer(c_int32_t), intent(in) :: num_singles(*), list_virt(*)
+      type(c_ptr), value :: ignorable, records
+      integer(c_int64_t), intent(out) :: n_records
+    end function
     type(c_ptr) function tci_last_error_c() bind(c, name="tci_last_error")
       import :: c_ptr
     end function
