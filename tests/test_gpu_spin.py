@@ -97,3 +97,5 @@ def test_spin_entry_points_reject_bad_input():
         tci.contracted_cycles(4, [(1, 2, 1, 2, 0, 0, 6, 4)])  # left beyond the tables
     with pytest.raises(tci.TigerSigmaError):
         tci.contracted_cycles(4, [(1, 2, 1, 2, 0, 0, 4, 4)], spin_mult=3)
+    with pytest.raises(tci.TigerSigmaError):
+        tci.contracted_cycles(-3, [(1, 2, 1, 2, 0, 0, 4, 4)])

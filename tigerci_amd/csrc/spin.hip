@@ -354,6 +354,7 @@ extern "C" int tci_contracted_cycles(int32_t n_open, int32_t spin_mult, int64_t 
   try {
     TCI_REQUIRE(records && out_off && out, "null argument");
     TCI_REQUIRE(spin_mult == 1, "contracted cycles are evaluated for singlet spin functions");
+    TCI_REQUIRE(n_open >= 2 && n_open <= kMaxOpen, "open shells must be in 2.." + std::to_string(kMaxOpen));
     TCI_REQUIRE(n_rec >= 1 && n_rec < ((int64_t)1 << 31), "invalid record count");
     std::vector<int32_t> fs(n_open + 1);
     for (int m = 0; m <= n_open; ++m) fs[m] = fsn_of(m, spin_mult);
