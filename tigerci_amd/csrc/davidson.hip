@@ -7,8 +7,11 @@
 //   h(k,idx) = <c_k, sigma_idx>                  one multi-dot pass
 //   r = sum_k y_k (sigma_k - E c_k)              one multi-axpy pass
 //   r <- -r / (H_ii - E), 0 where H_ii - E == 0  (build_next_vector_std_eig)
-//   orthonormalise against all c_k              classical Gram-Schmidt applied
-//                                                twice, then against the new
+//   orthonormalise against all c_k              classical Gram-Schmidt, a
+//                                                second sweep when the first
+//                                                removed over half of a
+//                                                vector's squared norm, then
+//                                                against the new
 //                                                vectors before it, then
 //                                                normalise; zero if the norm
 //                                                < 1e-14.  The reference's
@@ -488,15 +491,30 @@ struct Dav {
   // with one pass over X per sweep for all of W, then orthonormalize() against
   // the preceding W.  W_j spans what orthonormalising W_0, W_1, ... one at a
   // time against X and their predecessors spans; a vanished W_j is zeroed.
+  // The second sweep runs only when the first removed more than half of some
+  // W_j's squared norm (|W_j'|^2 = |W_j|^2 - sum_k <X_k, W_j>^2 for
+  // orthonormal X): otherwise one classical sweep is already orthogonal to
+  // working precision (the twice-is-enough test), and the pass over X is saved.
   void orthonormalize_block(const std::vector<double*>& W, const std::vector<double*>& X) {
     const int K = (int)W.size(), m = (int)X.size();
     if (K == 0) return;
-    std::vector<double> ov, c((size_t)K * m);
+    std::vector<double> ov, wn, c((size_t)K * m);
+    if (m > 0) mdotm(W, W, wn);
     for (int pass = 0; pass < 2 && m > 0; ++pass) {
       mdotm(X, W, ov);
       for (int j = 0; j < K; ++j)
         for (int k = 0; k < m; ++k) c[(size_t)j * m + k] = -ov[(size_t)k * K + j];
       lincombm(W, 1.0, X, c);
+      if (pass == 0) {
+        bool again = false;
+        for (int j = 0; j < K; ++j) {
+          double proj = 0.0;
+          for (int k = 0; k < m; ++k) proj += ov[(size_t)k * K + j] * ov[(size_t)k * K + j];
+          const double w2 = wn[(size_t)j * K + j];
+          if (!(w2 - proj > 0.5 * w2)) again = true;
+        }
+        if (!again) break;
+      }
     }
     for (int j = 0; j < K; ++j) orthonormalize(W[j], std::vector<double*>(W.begin(), W.begin() + j));
   }

@@ -897,25 +897,34 @@ void mixed_prepare(tci_plan& P, int precision, hipStream_t st) {
       };
       double best = 1e300;
       int bz = 0, bu = 0;
-      for (int cz = 0; cz < 4; ++cz) {
-        const double ms = trial(cz, 0);
-        log += std::string(" z") + cname[cz] + ":" + fmt_ms(ms);
-        if (ms < best * 0.98) {
-          best = ms;
-          bz = cz;
-        }
-      }
-      if (M.z_lowrank) {  // U and W go out as a launch of their own
-        const double ms = trial(bz, 2);
-        log += std::string(" uw256w8:") + fmt_ms(ms);
-        if (ms < best * 0.98) {
-          best = ms;
-          bu = 2;
-        }
+      if (M.tuned_pairs_z >= 0) {  // tuned for the other split precision: same tiles, same sums
+        bz = M.tuned_pairs_z;
+        bu = M.tuned_pairs_uw;
+        log += " kept";
+        trial(bz, bu);
       } else {
-        bu = bz;
+        for (int cz = 0; cz < 4; ++cz) {
+          const double ms = trial(cz, 0);
+          log += std::string(" z") + cname[cz] + ":" + fmt_ms(ms);
+          if (ms < best * 0.98) {
+            best = ms;
+            bz = cz;
+          }
+        }
+        if (M.z_lowrank) {  // U and W go out as a launch of their own
+          const double ms = trial(bz, 2);
+          log += std::string(" uw256w8:") + fmt_ms(ms);
+          if (ms < best * 0.98) {
+            best = ms;
+            bu = 2;
+          }
+        } else {
+          bu = bz;
+        }
+        if (!(M.z_lowrank && bu == 2)) trial(bz, bu);
+        M.tuned_pairs_z = bz;
+        M.tuned_pairs_uw = bu;
       }
-      if (!(M.z_lowrank && bu == 2)) trial(bz, bu);
       log += std::string(" -> z") + cname[bz] + " uw" + cname[bu] + "; order";
       for (GemmProgram* g : pair_programs())
         if (!g->tasks.empty()) log += " " + tune_block_order(*g, st, [&] { g->launch_split(precision, st, 0, scl); });
@@ -935,13 +944,21 @@ void mixed_prepare(tci_plan& P, int precision, hipStream_t st) {
     vv_prepare_split(P, precision, st);
     std::string log = pn + " ladder (ab|cd) resident:";
     bool use = true;
-    if (autotune_enabled()) {
+    if (P.vv.split_use >= 0) {  // decided for the other split precision: same schedule, same sums
+      use = P.vv.split_use == 1;
+      if (use && P.vv.split_tile > 0) {
+        vv_set_schedule(P, precision, P.vv.split_tile, P.vv.split_tile_n, P.vv.split_dma, P.vv.split_ks, st);
+        P.vv.gx.remap = P.vv.split_order;
+      }
+      log += " kept";
+    } else if (autotune_enabled()) {
       const double tv = vv_tune(P, precision, st, log);
       if (!force) {
         const double tc = time_ms(st, 1, [&] { mixed_ladder_run(P, precision, st); });
         log += "; resident " + fmt_ms(tv) + ", Cholesky-direct " + fmt_ms(tc);
         use = tv < tc;
       }
+      P.vv.split_use = use ? 1 : 0;
     }
     P.vv.split = use;
     P.tuning += log + (use ? " -> resident\n" : " -> Cholesky-direct\n");

@@ -506,6 +506,13 @@ double vv_tune(tci_plan& P, int precision, hipStream_t st, std::string& log) {
     if (precision == TCI_SIGMA_FP64) g.launch(st);
     else g.launch_split(precision, st, 0, sc);
   });
+  if (precision != TCI_SIGMA_FP64) {
+    V.split_tile = bc.tile;
+    V.split_tile_n = bc.tile_n;
+    V.split_dma = bc.dma;
+    V.split_ks = bc.ks;
+    V.split_order = o;
+  }
   log += " -> " + std::to_string(bc.tile) + (bc.tile_n ? "x" + std::to_string(bc.tile_n) : "") +
          (bc.dma ? (bc.tile == 256 ? "w" + std::to_string(bc.dma) : "d") : "") + "/" + std::to_string(bc.ks) + " order " + std::to_string(o);
   TCI_HIP(hipStreamSynchronize(st));

@@ -108,6 +108,9 @@ struct tci_plan {
     // Z, Z3, W and the 4-internal product as one task list (128 x 128 tiles)
     tci::GemmProgram pairsx;
     bool pairs_merged = false;
+    // tile configurations of the split pair products, tuned on the first split
+    // preparation and kept for both split precisions (-1: not yet tuned)
+    int tuned_pairs_z = -1, tuned_pairs_uw = -1;
     std::vector<LadderPhase> phase;
     std::vector<std::pair<int64_t, int>> chunks;
     int64_t bytes = 0;
@@ -167,6 +170,11 @@ struct tci_plan {
     tci::DevBuf colm;               // [n_pairs] row of the walk in X- or -1
     int ks = 1, ksx = 1;            // K-splits of the FP64 / split products
     tci::GemmProgram g64, gx;       // the products (both parts in one launch)
+    // split schedule chosen on the first split preparation (kept for both
+    // split precisions, so a plan's split sigma does not change): tile,
+    // tile_n, dma (wave count for 256 tiles), K-split, order; use: -1 not yet
+    // decided, 0 Cholesky-direct, 1 resident
+    int split_tile = 0, split_tile_n = 0, split_dma = 0, split_ks = 1, split_order = 0, split_use = -1;
   } vv;
   tci::PairTab tab{};
   // accounting
