@@ -247,6 +247,12 @@ def main() -> int:
                     help="replay every sigma build from a captured HIP graph (default; ladder times then come "
                          "from separate eager profiled builds)")
     ap.add_argument("--no-graph", dest="graph", action="store_false", help="launch every sigma build eagerly")
+    ap.add_argument("--iteration-graph", dest="iteration_graph", action="store_true", default=True,
+                    help="Davidson: replay each iteration's device work (residuals, preconditioner, "
+                         "Gram-Schmidt, sigma builds, subspace columns) from a captured HIP graph (default "
+                         "with --graph)")
+    ap.add_argument("--sigma-graph-only", dest="iteration_graph", action="store_false",
+                    help="Davidson: graphs for the sigma builds only, the vector work eager")
     ap.add_argument("--mode", default="davidson", choices=["davidson", "sigma"])
     ap.add_argument("--switch-tol", type=float, default=1e-6)
     ap.add_argument("--no-cpu", action="store_true")
@@ -349,9 +355,11 @@ def main() -> int:
     del xd, yd
 
     # ---- Davidson: mixed (timed steps) and FP64 reference -------------------
+    dav_graph = ("iteration" if a.iteration_graph else True) if a.graph else False
+    cfg["davidson_graph"] = dav_graph or "none"
     def solve(prec):
         return plan.davidson(roots=a.roots, energy_tol=1e-6, norm_tol=1e-5, precision=prec,
-                             switch_tol=a.switch_tol, graph=a.graph)
+                             switch_tol=a.switch_tol, graph=dav_graph)
 
     for _ in range(a.warmup):
         solve(a.precision)

@@ -59,6 +59,14 @@ enum tci_precision {
  * any civec/sigmavec pair.  Calls on one plan must be ordered on one stream.
  * With profiling on, only the whole build is timed (no ladder time). */
 #define TCI_SIGMA_GRAPH 0x400
+/* tci_davidson_params.precision only: run each Davidson iteration's device
+ * work (residuals and their norms, the preconditioner, Gram-Schmidt, the
+ * sigma builds of the new vectors, the new columns of the subspace matrix)
+ * as one HIP graph launch, with one host synchronisation per iteration for
+ * the subspace eigenproblem.  Standard problem only; tci_davidson_gen_run
+ * runs the vector work eagerly.  Implies TCI_SIGMA_GRAPH for the sigma
+ * builds outside the iteration (the guess and collapses). */
+#define TCI_DAVIDSON_GRAPH 0x800
 
 /* Problem description.  All pointers are host pointers, read during
  * tci_plan_create only.  Orbitals: internal 0..n_int-1, virtual
@@ -151,7 +159,8 @@ typedef struct tci_davidson_params {
   double energy_tol;      /* |dE| convergence (ENERGY TOLERANCE, default 1e-6) */
   double norm_tol;        /* residual norm convergence (RESIDUAL TOLERANCE) */
   int32_t precision;      /* tci_precision of the expansion sigmas, | TCI_SIGMA_GRAPH to replay
-                             every sigma build from a captured HIP graph */
+                             every sigma build from a captured HIP graph, | TCI_DAVIDSON_GRAPH
+                             to replay whole iterations */
   double switch_tol;      /* mixed: switch to FP64 sigma once every root has |dE| < switch_tol
                              or a residual below norm_tol (forced at maxit/2) */
   int32_t max_vectors;    /* subspace collapse budget (0 -> maxit*roots + roots, capped at 32 GB of vectors) */
@@ -164,7 +173,7 @@ typedef struct tci_davidson_result {
   int32_t sigma_mixed;    /* number of mixed-precision sigma builds */
   int32_t converged;      /* all roots met a convergence criterion before maxit */
   double seconds;         /* wall time of the solve */
-  double seconds_sigma;   /* of which inside sigma builds */
+  double seconds_sigma;   /* of which inside sigma builds (in iteration graphs: by the device clock) */
   int32_t collapses;      /* subspace collapses on the vector budget (restarts) */
 } tci_davidson_result;
 

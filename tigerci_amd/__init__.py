@@ -34,6 +34,7 @@ PRECISION = {"fp64": 0, "bf16x3": 1, "fp16x3": 2}
 TCI_ACCUMULATE = 0x100
 TCI_ADD_EREF = 0x200
 TCI_SIGMA_GRAPH = 0x400
+TCI_DAVIDSON_GRAPH = 0x800
 
 
 class TigerSigmaError(RuntimeError):
@@ -360,16 +361,21 @@ class Plan:
 
     def davidson(self, roots: int = 1, energy_tol: float = 1e-6, norm_tol: float = 1e-6,
                  precision: str = "fp64", switch_tol: float | None = None, maxit: int = 0,
-                 max_vectors: int = 0, vectors=None, verbose: bool = False, graph: bool = False,
+                 max_vectors: int = 0, vectors=None, verbose: bool = False, graph: bool | str = False,
                  g_values=None):
         """Lowest roots; returns (total energies, result dict).  ``vectors`` is an
         optional DeviceArray of roots*n_csf doubles for the eigenvectors;
-        ``graph`` replays every sigma build from a captured HIP graph;
+        ``graph=True`` replays every sigma build from a captured HIP graph,
+        ``graph="iteration"`` every iteration's device work (standard problem;
+        the generalized one keeps its vector work eager);
         ``g_values`` (five g values, see ``acpf_gvalues``) solves the
         generalized ACPF-type problem (H - E_ref) c = E G c instead."""
         if switch_tol is None:
             switch_tol = energy_tol
-        prec = PRECISION[precision] | (TCI_SIGMA_GRAPH if graph else 0)
+        if graph not in (False, True, "iteration"):
+            raise ValueError("graph must be False, True or 'iteration'")
+        prec = PRECISION[precision] | (TCI_DAVIDSON_GRAPH if graph == "iteration" else
+                                       TCI_SIGMA_GRAPH if graph else 0)
         p = _DavParams(roots, maxit, energy_tol, norm_tol, prec, switch_tol, max_vectors,
                        1 if verbose else 0)
         e = np.zeros(roots)

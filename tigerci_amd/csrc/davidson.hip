@@ -19,6 +19,14 @@
 //                                                renormalisation after every
 //                                                projection spans the same vector.
 // The m x m subspace problem (m <= 50 roots) is solved on the host (Jacobi).
+// With TCI_DAVIDSON_GRAPH (standard problem) the device work of an iteration,
+// from the residuals to the new columns of h, is one HIP graph replay: the
+// kernels read the basis size, the Ritz coefficients and the energies from a
+// device block the host uploads, the second Gram-Schmidt sweep is switched on
+// the device, and a root whose residual norm is below norm_tol has its new
+// vector zeroed in the graph and dropped by the host afterwards (its sigma is
+// built and discarded).  The host synchronises once per iteration, for the
+// subspace problem and the convergence bookkeeping.
 // Mixed precision: expansion sigmas use the split ladder until every root's
 // |dE| < switch_tol; then the subspace collapses to the current Ritz vectors,
 // their sigmas are rebuilt in FP64 and the solve finishes in FP64.  The
@@ -32,6 +40,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstdlib>
+#include <cstring>
 #include <numeric>
 
 #include "plan.hpp"
@@ -258,6 +267,231 @@ __global__ __launch_bounds__(256) void argmin_after_kernel(const double* __restr
 __global__ void unit_kernel(double* __restrict__ v, int64_t n, int64_t k) {
   for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x)
     v[e] = (e == k) ? 1.0 : 0.0;
+}
+
+// ---- kernels of the graph-captured iteration ----
+// A replay serves every iteration with the same numbers of active roots and
+// new vectors, so the kernels read what changes between iterations from a
+// device block: the basis size m (vector slots are addressed relative to it),
+// the store bases, the residual coefficients and the energies.
+
+struct IterState {
+  double* V;  // basis store: slot k at V + k n
+  double* S;  // sigma store, same slots
+  int64_t n;
+  int m;      // basis size when the iteration starts
+  int pad;
+  double norm_tol;
+};
+
+// slot mmul*m + off of the basis (sel 0) or sigma (sel 1) store
+struct SlotSpec {
+  int sel, mmul, off;
+};
+// a count of mmul*m + off vectors
+struct CountSpec {
+  int mmul, off;
+};
+
+__device__ __forceinline__ double* islot(const IterState& s, SlotSpec q, int k) {
+  return (q.sel ? s.S : s.V) + (int64_t)(q.mmul * s.m + q.off + k) * s.n;
+}
+
+// mdotm_kernel over slot ranges: partial[(k R + j) nblk + b] = block b's share
+// of X_k . W_j for k < count.  A set gate that reads 0 skips the launch's work
+// (the second Gram-Schmidt sweep is switched on the device).  256 threads.
+template <int R>
+__global__ __launch_bounds__(256) void sdots_kernel(const IterState* __restrict__ ps, SlotSpec xs, CountSpec xc,
+                                                    SlotSpec ws, double* __restrict__ partial, int nblk,
+                                                    const int* __restrict__ gate) {
+  if (gate && *gate == 0) return;
+  const IterState s = *ps;
+  const int m = xc.mmul * s.m + xc.off;
+  const int k0 = blockIdx.y * MD_GROUP;
+  if (k0 >= m) return;
+  const int g = min(MD_GROUP, m - k0);
+  const double* x[MD_GROUP];
+#pragma unroll
+  for (int q = 0; q < MD_GROUP; ++q) x[q] = islot(s, xs, k0 + min(q, g - 1));
+  const double* v[R];
+#pragma unroll
+  for (int j = 0; j < R; ++j) v[j] = islot(s, ws, j);
+  double acc[MD_GROUP][R];
+#pragma unroll
+  for (int q = 0; q < MD_GROUP; ++q)
+#pragma unroll
+    for (int j = 0; j < R; ++j) acc[q][j] = 0.0;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < s.n; e += (int64_t)nblk * blockDim.x) {
+    double ve[R];
+#pragma unroll
+    for (int j = 0; j < R; ++j) ve[j] = v[j][e];
+#pragma unroll
+    for (int q = 0; q < MD_GROUP; ++q)
+      if (q < g) {
+        const double xe = x[q][e];
+#pragma unroll
+        for (int j = 0; j < R; ++j) acc[q][j] += xe * ve[j];
+      }
+  }
+  __shared__ double red[4][MD_GROUP * R];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+  for (int q = 0; q < MD_GROUP; ++q)
+#pragma unroll
+    for (int j = 0; j < R; ++j) {
+      double a = acc[q][j];
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) a += __shfl_xor(a, off);
+      if (lane == 0) red[w][q * R + j] = a;
+    }
+  __syncthreads();
+  const int t = threadIdx.x;
+  if (t < MD_GROUP * R && t / R < g)
+    partial[((int64_t)(k0 + t / R) * R + t % R) * nblk + blockIdx.x] = red[0][t] + red[1][t] + red[2][t] + red[3][t];
+}
+
+// out[k R + j] = sum over blocks of partial, k < count.  256 threads.
+__global__ void sreduce_kernel(const IterState* __restrict__ ps, CountSpec xc, int R, const double* __restrict__ partial,
+                               int nblk, double* __restrict__ out, const int* __restrict__ gate) {
+  if (gate && *gate == 0) return;
+  const int k = blockIdx.x;
+  if (k >= (xc.mmul * ps->m + xc.off) * R) return;
+  __shared__ double red[256];
+  double a = 0.0;
+  for (int b = threadIdx.x; b < nblk; b += blockDim.x) a += partial[(int64_t)k * nblk + b];
+  red[threadIdx.x] = a;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[k] = red[0];
+}
+
+// W_j -= sum_{k < count} d[k R + j] X_k (d as sreduce_kernel writes it)
+template <int R>
+__global__ void sproject_kernel(const IterState* __restrict__ ps, SlotSpec ws, SlotSpec xs, CountSpec xc,
+                                const double* __restrict__ d, const int* __restrict__ gate) {
+  if (gate && *gate == 0) return;
+  const IterState s = *ps;
+  const int m = xc.mmul * s.m + xc.off;
+  double* o[R];
+#pragma unroll
+  for (int j = 0; j < R; ++j) o[j] = islot(s, ws, j);
+  const double* x0 = islot(s, xs, 0);
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < s.n; e += (int64_t)gridDim.x * blockDim.x) {
+    double a[R];
+#pragma unroll
+    for (int j = 0; j < R; ++j) a[j] = o[j][e];
+    for (int k = 0; k < m; ++k) {
+      const double xe = x0[(int64_t)k * s.n + e];
+#pragma unroll
+      for (int j = 0; j < R; ++j) a[j] += (-d[k * R + j]) * xe;
+    }
+#pragma unroll
+    for (int j = 0; j < R; ++j) o[j][e] = a[j];
+  }
+}
+
+// residuals r_j = sum_{q < m} c[j][2q] sigma_q + c[j][2q+1] c_q into basis
+// slots m + j0 + j; c rows of length ld, c[j][2q+1] = -E y_q
+template <int R>
+__global__ void sres_kernel(const IterState* __restrict__ ps, int j0, const double* __restrict__ coef, int ld) {
+  const IterState s = *ps;
+  double* o[R];
+#pragma unroll
+  for (int j = 0; j < R; ++j) o[j] = s.V + (int64_t)(s.m + j0 + j) * s.n;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < s.n; e += (int64_t)gridDim.x * blockDim.x) {
+    double a[R];
+#pragma unroll
+    for (int j = 0; j < R; ++j) a[j] = 0.0;
+    for (int q = 0; q < s.m; ++q) {
+      const double se = s.S[(int64_t)q * s.n + e];
+#pragma unroll
+      for (int j = 0; j < R; ++j) a[j] += coef[(j0 + j) * ld + 2 * q] * se;
+      const double ce = s.V[(int64_t)q * s.n + e];
+#pragma unroll
+      for (int j = 0; j < R; ++j) a[j] += coef[(j0 + j) * ld + 2 * q + 1] * ce;
+    }
+#pragma unroll
+    for (int j = 0; j < R; ++j) o[j][e] = a[j];
+  }
+}
+
+// dst[j0 + j] = d[j R + j], the squared norms among a group's dots
+__global__ void sdiag_kernel(const double* __restrict__ d, int R, int j0, double* __restrict__ dst) {
+  const int j = threadIdx.x;
+  if (j < R) dst[j0 + j] = d[j * R + j];
+}
+
+// residual norm test of the new vectors: mask[j] = 1 where the root may stop
+// on its residual norm (rnok) and |r_j| < norm_tol
+__global__ void smask_kernel(const IterState* __restrict__ ps, int ne, const double* __restrict__ r2,
+                             const int* __restrict__ rnok, double* __restrict__ mask) {
+  const int j = threadIdx.x;
+  if (j >= ne) return;
+  const double rn = sqrt(fmax(r2[j], 0.0));
+  mask[j] = rnok[j] && rn < ps->norm_tol ? 1.0 : 0.0;
+}
+
+// r <- -r / (H_ii - E_j) (0 where that is 0) on slot m + j; zero if masked
+__global__ void sprecond_kernel(const IterState* __restrict__ ps, int j, const double* __restrict__ diag,
+                                const double* __restrict__ eng, const double* __restrict__ mask) {
+  const IterState s = *ps;
+  double* r = s.V + (int64_t)(s.m + j) * s.n;
+  const double E = eng[j];
+  const bool off = mask[j] != 0.0;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < s.n; e += (int64_t)gridDim.x * blockDim.x) {
+    const double p = diag[e] - E;
+    r[e] = (off || p == 0.0) ? 0.0 : -r[e] / p;
+  }
+}
+
+// second-sweep test of one group after the first sweep's dots d (k R + j, k <
+// m): set *flag when some unmasked W_j lost more than half of its squared
+// norm w2 (|W_j'|^2 = w2 - sum_k d^2 for orthonormal X)
+__global__ void sagain_kernel(const IterState* __restrict__ ps, const double* __restrict__ d, int R, int j0,
+                              const double* __restrict__ w2, const double* __restrict__ mask, int* __restrict__ flag) {
+  const int j = threadIdx.x;
+  if (j >= R || mask[j0 + j] != 0.0) return;
+  const int m = ps->m;
+  double proj = 0.0;
+  for (int k = 0; k < m; ++k) proj += d[k * R + j] * d[k * R + j];
+  const double w = w2[j0 + j];
+  if (!(w - proj > 0.5 * w)) flag[0] = 1;
+}
+
+__global__ void sfill_int_kernel(int* __restrict__ p, int v) {
+  if (threadIdx.x == 0) p[0] = v;
+}
+
+// fac[j] = 1/|W_j| from its squared norm d[0], 0 below 1e-14
+__global__ void snorm_kernel(const double* __restrict__ d, int j, double* __restrict__ fac) {
+  if (threadIdx.x != 0) return;
+  const double nr = sqrt(fmax(d[0], 0.0));
+  fac[j] = nr < 1e-14 ? 0.0 : 1.0 / nr;
+}
+
+__global__ void sscale_kernel(const IterState* __restrict__ ps, int j, const double* __restrict__ fac) {
+  const IterState s = *ps;
+  double* v = s.V + (int64_t)(s.m + j) * s.n;
+  const double a = fac[j];
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < s.n; e += (int64_t)gridDim.x * blockDim.x)
+    v[e] *= a;
+}
+
+// constant-rate device clock, taken once the preceding kernels have finished
+// (brackets the sigma builds of a replay)
+__global__ void sstamp_kernel(long long* __restrict__ t) {
+  if (threadIdx.x == 0) t[0] = wall_clock64();
+}
+
+// the sigma build of new vector j reads c and writes sigma through the slot
+__global__ void sset_io_kernel(const IterState* __restrict__ ps, int j, IoSlot* __restrict__ io) {
+  if (threadIdx.x != 0) return;
+  const IterState s = *ps;
+  io->x = s.V + (int64_t)(s.m + j) * s.n;
+  io->y = s.S + (int64_t)(s.m + j) * s.n;
 }
 
 // cyclic Jacobi eigen-solver for a small symmetric matrix (ascending eigenvalues)
@@ -520,6 +754,205 @@ struct Dav {
   }
 };
 
+// Byte layout of the plan's iteration block for S basis slots: the part the
+// host uploads before a replay (state, energies, norm-test switches, residual
+// coefficients), the part copied back after it (squared residual norms,
+// vanished-vector flags, new subspace columns: group g of four new vectors at
+// hd + 4 S g doubles, entry q R + j), then device scratch.  The host mirror
+// (pinned) uses the same offsets.
+struct IterLayout {
+  static constexpr int RMAX = 64;  // roots
+  int S, ld;
+  size_t eng, rnok, cres, up_end, rd, mask, ts, hd, out_end, w2, fac, flag, dots, partial, total;
+  IterLayout(int S_, int nblk) : S(S_), ld(2 * S_) {
+    auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
+    eng = 256;  // IterState at 0
+    rnok = al(eng + RMAX * 8);
+    cres = al(rnok + RMAX * 4);
+    up_end = al(cres + (size_t)RMAX * ld * 8);
+    rd = up_end;
+    mask = al(rd + RMAX * 8);
+    ts = al(mask + RMAX * 8);  // device clock before and after the sigma builds
+    hd = al(ts + 16);
+    out_end = al(hd + (size_t)RMAX * S * 8);
+    w2 = out_end;
+    fac = al(w2 + RMAX * 8);
+    flag = al(fac + RMAX * 8);
+    dots = al(flag + 256);
+    partial = al(dots + (size_t)4 * (S + RMAX) * 8);
+    total = al(partial + (size_t)4 * (S + RMAX) * nblk * 8);
+  }
+};
+
+void drop_iter_graphs(tci_plan& P) {
+  if (P.iter_graphs.empty()) return;
+  TCI_HIP(hipDeviceSynchronize());
+  for (auto& kv : P.iter_graphs) (void)hipGraphExecDestroy(kv.second);
+  P.iter_graphs.clear();
+}
+
+// the block and its host mirror for S slots (graphs that point into smaller
+// ones are dropped first)
+void iter_ensure(tci_plan& P, const IterLayout& L) {
+  if (P.iter_buf.bytes < L.total) {
+    drop_iter_graphs(P);
+    P.iter_buf.alloc(L.total);
+  }
+  if (P.iter_host_bytes < L.out_end) {
+    drop_iter_graphs(P);
+    if (P.iter_host) (void)hipHostFree(P.iter_host);
+    P.iter_host = nullptr;
+    P.iter_host_bytes = 0;
+    TCI_HIP(hipHostMalloc(&P.iter_host, L.out_end, hipHostMallocDefault));
+    P.iter_host_bytes = L.out_end;
+  }
+  if (P.iter_clock_khz <= 0) {
+    int dev = 0;
+    TCI_HIP(hipGetDevice(&dev));
+    TCI_HIP(hipDeviceGetAttribute(&P.iter_clock_khz, hipDeviceAttributeWallClockRate, dev));
+  }
+  if (!P.io.p) P.io.alloc(sizeof(IoSlot));
+  if (!P.gstream) TCI_HIP(hipStreamCreateWithFlags(&P.gstream, hipStreamNonBlocking));
+}
+
+// Record one iteration of the standard problem with na active roots, the
+// first ne of which take a new vector (residual slots m .. m + na - 1, the
+// new vectors first), on stream cs:
+//   residuals and their squared norms; the norm test; the preconditioner;
+//   one classical Gram-Schmidt sweep against the basis and a second one that
+//   runs only when the first removed more than half of some vector's squared
+//   norm (decided on the device); CGS2 within the new block and
+//   normalisation (zero below 1e-14); the sigma builds of the new vectors
+//   (between two reads of the device clock); the new columns <c_q, sigma_k>, q < m + ne;
+//   one copy of the results to the host mirror.
+void iter_record(tci_plan& P, const IterLayout& L, int nblk, int precision, int na, int ne, hipStream_t cs) {
+  char* base = static_cast<char*>(P.iter_buf.p);
+  const IterState* ps = reinterpret_cast<const IterState*>(base);
+  const double* eng = reinterpret_cast<const double*>(base + L.eng);
+  const int* rnok = reinterpret_cast<const int*>(base + L.rnok);
+  const double* cres = reinterpret_cast<const double*>(base + L.cres);
+  double* rd = reinterpret_cast<double*>(base + L.rd);
+  double* mask = reinterpret_cast<double*>(base + L.mask);
+  double* hd = reinterpret_cast<double*>(base + L.hd);
+  double* w2 = reinterpret_cast<double*>(base + L.w2);
+  double* fac = reinterpret_cast<double*>(base + L.fac);
+  int* flag = reinterpret_cast<int*>(base + L.flag);
+  double* dots = reinterpret_cast<double*>(base + L.dots);
+  double* partial = reinterpret_cast<double*>(base + L.partial);
+  const int64_t n = P.n_csf;
+  const unsigned gx = (unsigned)std::min<int64_t>(cdiv(n, 256), 8192);
+  const double* diag = P.diag.as<double>();
+  // dots of X (count xc, at most xmax) with R vectors W, reduced into out
+  auto dotsR = [&](SlotSpec xs, CountSpec xc, int xmax, SlotSpec ws, int R, double* out, const int* gate) {
+    const dim3 grid(nblk, (unsigned)cdiv(xmax, MD_GROUP));
+    switch (R) {
+      case 1: hipLaunchKernelGGL(sdots_kernel<1>, grid, dim3(256), 0, cs, ps, xs, xc, ws, partial, nblk, gate); break;
+      case 2: hipLaunchKernelGGL(sdots_kernel<2>, grid, dim3(256), 0, cs, ps, xs, xc, ws, partial, nblk, gate); break;
+      case 3: hipLaunchKernelGGL(sdots_kernel<3>, grid, dim3(256), 0, cs, ps, xs, xc, ws, partial, nblk, gate); break;
+      default: hipLaunchKernelGGL(sdots_kernel<4>, grid, dim3(256), 0, cs, ps, xs, xc, ws, partial, nblk, gate); break;
+    }
+    TCI_HIP(hipGetLastError());
+    hipLaunchKernelGGL(sreduce_kernel, dim3((unsigned)(xmax * R)), dim3(256), 0, cs, ps, xc, R, partial, nblk, out,
+                       gate);
+    TCI_HIP(hipGetLastError());
+  };
+  auto project = [&](SlotSpec ws, SlotSpec xs, CountSpec xc, int R, const int* gate) {
+    switch (R) {
+      case 1: hipLaunchKernelGGL(sproject_kernel<1>, dim3(gx), dim3(256), 0, cs, ps, ws, xs, xc, dots, gate); break;
+      case 2: hipLaunchKernelGGL(sproject_kernel<2>, dim3(gx), dim3(256), 0, cs, ps, ws, xs, xc, dots, gate); break;
+      case 3: hipLaunchKernelGGL(sproject_kernel<3>, dim3(gx), dim3(256), 0, cs, ps, ws, xs, xc, dots, gate); break;
+      default: hipLaunchKernelGGL(sproject_kernel<4>, dim3(gx), dim3(256), 0, cs, ps, ws, xs, xc, dots, gate); break;
+    }
+    TCI_HIP(hipGetLastError());
+  };
+  const SlotSpec basis{0, 0, 0};
+  const CountSpec all_m{1, 0};
+  hipLaunchKernelGGL(sfill_int_kernel, dim3(1), dim3(64), 0, cs, flag, 0);
+  for (int j0 = 0; j0 < na; j0 += 4) {
+    const int R = std::min(4, na - j0);
+    switch (R) {
+      case 1: hipLaunchKernelGGL(sres_kernel<1>, dim3(gx), dim3(256), 0, cs, ps, j0, cres, L.ld); break;
+      case 2: hipLaunchKernelGGL(sres_kernel<2>, dim3(gx), dim3(256), 0, cs, ps, j0, cres, L.ld); break;
+      case 3: hipLaunchKernelGGL(sres_kernel<3>, dim3(gx), dim3(256), 0, cs, ps, j0, cres, L.ld); break;
+      default: hipLaunchKernelGGL(sres_kernel<4>, dim3(gx), dim3(256), 0, cs, ps, j0, cres, L.ld); break;
+    }
+    TCI_HIP(hipGetLastError());
+  }
+  for (int j0 = 0; j0 < na; j0 += 4) {
+    const int R = std::min(4, na - j0);
+    dotsR({0, 1, j0}, {0, R}, R, {0, 1, j0}, R, dots, nullptr);
+    hipLaunchKernelGGL(sdiag_kernel, dim3(1), dim3(64), 0, cs, dots, R, j0, rd);
+  }
+  hipLaunchKernelGGL(smask_kernel, dim3(1), dim3(64), 0, cs, ps, ne, rd, rnok, mask);
+  for (int j = 0; j < ne; ++j) hipLaunchKernelGGL(sprecond_kernel, dim3(gx), dim3(256), 0, cs, ps, j, diag, eng, mask);
+  TCI_HIP(hipGetLastError());
+  for (int j0 = 0; j0 < ne; j0 += 4) {
+    const int R = std::min(4, ne - j0);
+    dotsR({0, 1, j0}, {0, R}, R, {0, 1, j0}, R, dots, nullptr);
+    hipLaunchKernelGGL(sdiag_kernel, dim3(1), dim3(64), 0, cs, dots, R, j0, w2);
+  }
+  for (int sweep = 0; sweep < 2; ++sweep)
+    for (int j0 = 0; j0 < ne; j0 += 4) {
+      const int R = std::min(4, ne - j0);
+      const int* gate = sweep ? flag : nullptr;
+      dotsR(basis, all_m, L.S, {0, 1, j0}, R, dots, gate);
+      if (sweep == 0) hipLaunchKernelGGL(sagain_kernel, dim3(1), dim3(64), 0, cs, ps, dots, R, j0, w2, mask, flag);
+      project({0, 1, j0}, basis, all_m, R, gate);
+    }
+  for (int j = 0; j < ne; ++j) {
+    for (int pass = 0; pass < 2 && j > 0; ++pass) {
+      dotsR({0, 1, 0}, {0, j}, j, {0, 1, j}, 1, dots, nullptr);
+      project({0, 1, j}, {0, 1, 0}, {0, j}, 1, nullptr);
+    }
+    dotsR({0, 1, j}, {0, 1}, 1, {0, 1, j}, 1, dots, nullptr);
+    hipLaunchKernelGGL(snorm_kernel, dim3(1), dim3(64), 0, cs, dots, j, fac);
+    hipLaunchKernelGGL(sscale_kernel, dim3(gx), dim3(256), 0, cs, ps, j, fac);
+  }
+  TCI_HIP(hipGetLastError());
+  long long* ts = reinterpret_cast<long long*>(base + L.ts);
+  hipLaunchKernelGGL(sstamp_kernel, dim3(1), dim3(64), 0, cs, ts);
+  IoSlot* io = P.io.as<IoSlot>();
+  for (int j = 0; j < ne; ++j) {
+    hipLaunchKernelGGL(sset_io_kernel, dim3(1), dim3(64), 0, cs, ps, j, io);
+    TCI_HIP(hipGetLastError());
+    P.sigma_body(nullptr, nullptr, io, precision, cs);
+  }
+  hipLaunchKernelGGL(sstamp_kernel, dim3(1), dim3(64), 0, cs, ts + 1);
+  TCI_HIP(hipGetLastError());
+  for (int j0 = 0, g = 0; j0 < ne; j0 += 4, ++g) {
+    const int R = std::min(4, ne - j0);
+    dotsR(basis, {1, ne}, L.S, {1, 1, j0}, R, hd + (size_t)4 * L.S * g, nullptr);
+  }
+  const size_t out_bytes = L.hd - L.rd + (size_t)cdiv(ne, 4) * 4 * L.S * 8;
+  TCI_HIP(hipMemcpyAsync(static_cast<char*>(P.iter_host) + L.rd, base + L.rd, out_bytes, hipMemcpyDeviceToHost, cs));
+}
+
+hipGraphExec_t iter_graph(tci_plan& P, const IterLayout& L, int nblk, int precision, int na, int ne) {
+  const int64_t key = ((int64_t)L.S << 24) | ((int64_t)precision << 16) | ((int64_t)na << 8) | ne;
+  auto it = P.iter_graphs.find(key);
+  if (it != P.iter_graphs.end()) return it->second;
+  hipStream_t cs = P.gstream;
+  hipGraph_t g = nullptr;
+  TCI_HIP(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+  P.capturing = true;
+  try {
+    iter_record(P, L, nblk, precision, na, ne, cs);
+  } catch (...) {
+    P.capturing = false;
+    (void)hipStreamEndCapture(cs, &g);
+    if (g) (void)hipGraphDestroy(g);
+    throw;
+  }
+  P.capturing = false;
+  TCI_HIP(hipStreamEndCapture(cs, &g));
+  hipGraphExec_t ex = nullptr;
+  const hipError_t e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  TCI_HIP(e);
+  P.iter_graphs.emplace(key, ex);
+  return ex;
+}
+
 }  // namespace
 }  // namespace tci
 
@@ -552,9 +985,11 @@ int davidson_impl(tci_plan* plan, const tci_davidson_params* prm, const double* 
     TCI_REQUIRE(mv >= 5 * roots, "max_vectors must be at least 5*roots");
     const int mslots = mv + 2 * roots;
     // low byte: tci_precision of the expansion sigmas; TCI_SIGMA_GRAPH replays
-    // every sigma build from a captured HIP graph
-    TCI_REQUIRE((prm->precision & ~(0xff | TCI_SIGMA_GRAPH)) == 0, "unknown bits in precision");
-    const int graph_flag = prm->precision & TCI_SIGMA_GRAPH;
+    // every sigma build from a captured HIP graph, TCI_DAVIDSON_GRAPH every
+    // iteration of the standard problem (and the builds outside it)
+    TCI_REQUIRE((prm->precision & ~(0xff | TCI_SIGMA_GRAPH | TCI_DAVIDSON_GRAPH)) == 0, "unknown bits in precision");
+    const int graph_flag = prm->precision & (TCI_SIGMA_GRAPH | TCI_DAVIDSON_GRAPH) ? TCI_SIGMA_GRAPH : 0;
+    const bool igraph = (prm->precision & TCI_DAVIDSON_GRAPH) != 0 && gvals == nullptr;
     const int prec0 = prm->precision & 0xff;
     TCI_REQUIRE(prec0 == TCI_SIGMA_FP64 || prec0 == TCI_SIGMA_BF16X3 || prec0 == TCI_SIGMA_FP16X3, "unknown precision");
     const int64_t n = plan->n_csf;
@@ -685,8 +1120,15 @@ int davidson_impl(tci_plan* plan, const tci_davidson_params* prm, const double* 
     std::vector<char> restarts;
     int it = 0;
     bool all_conv = false;
+    bool sig_done = false;  // the last iteration graph built the new vectors' sigmas and h columns
+    bool sigma_timed = true;
+    const IterLayout IL(mslots, D.nblk);
+    if (igraph) {
+      TCI_REQUIRE(roots <= IterLayout::RMAX, "too many roots for iteration graphs");
+      iter_ensure(*plan, IL);
+    }
     for (it = 1; it <= maxit; ++it) {
-      {
+      if (!sig_done) {
         // new vectors of the unsettled roots: their sigmas, then
         // h(q,k) = <c_q, sigma_k> for q <= k in one pass over the basis
         std::vector<int> ks;
@@ -712,6 +1154,7 @@ int davidson_impl(tci_plan* plan, const tci_davidson_params* prm, const double* 
             for (int q = 0; q <= k; ++q) gs[(size_t)q * mv + k] = gs[(size_t)k * mv + q] = gv[q];
           }
       }
+      sig_done = false;
       subspace_solve(m, w, Y);
       for (int r = 0; r < roots; ++r) {
         dE[r] = std::fabs(w[r] - energy[r]);
@@ -844,6 +1287,99 @@ int davidson_impl(tci_plan* plan, const tci_davidson_params* prm, const double* 
         if (!conv[r] && !ready[r]) act.push_back(r);
       const int na = (int)act.size();
       TCI_REQUIRE(m + na <= mv, "Davidson subspace budget exhausted");
+      if (igraph && na > 0 && it < maxit) {
+        // the rules the host can apply before the residuals exist (dE); the
+        // residual-norm rule runs inside the graph
+        const bool in_restart = restarts.back() || (restarts.size() >= 2 && restarts[restarts.size() - 2]);
+        const double etol = precision != TCI_SIGMA_FP64 ? prm->switch_tol : prm->energy_tol;
+        std::vector<int> order, stop;  // indices into act: new vectors first, then roots that stop on dE
+        for (int i = 0; i < na; ++i) (!in_restart && it > 1 && dE[act[i]] < etol ? stop : order).push_back(i);
+        const int ne = (int)order.size();
+        order.insert(order.end(), stop.begin(), stop.end());
+        if (ne > 0) {
+          if (precision != TCI_SIGMA_FP64) mixed_prepare(*plan, precision, st);
+          iter_ensure(*plan, IL);
+          char* hb = static_cast<char*>(plan->iter_host);
+          IterState hs{Vp, Sp, n, m, 0, prm->norm_tol};
+          std::memcpy(hb, &hs, sizeof(hs));
+          double* heng = reinterpret_cast<double*>(hb + IL.eng);
+          int* hrnok = reinterpret_cast<int*>(hb + IL.rnok);
+          double* hcres = reinterpret_cast<double*>(hb + IL.cres);
+          for (int t = 0; t < ne; ++t) {
+            heng[t] = energy[act[order[t]]];
+            hrnok[t] = in_restart ? 0 : 1;
+          }
+          for (int i = 0; i < na; ++i) {
+            const int r = act[order[i]];
+            for (int q = 0; q < m; ++q) {
+              const double y = Y[(size_t)q * m + r];
+              hcres[(size_t)i * IL.ld + 2 * q] = y;
+              hcres[(size_t)i * IL.ld + 2 * q + 1] = -energy[r] * y;
+            }
+          }
+          hipGraphExec_t ex = iter_graph(*plan, IL, D.nblk, precision, na, ne);
+          TCI_HIP(hipMemcpyAsync(plan->iter_buf.p, hb, IL.cres + (size_t)na * IL.ld * 8, hipMemcpyHostToDevice, st));
+          TCI_HIP(hipGraphLaunch(ex, st));
+          TCI_HIP(hipStreamSynchronize(st));
+          const long long* hts = reinterpret_cast<const long long*>(hb + IL.ts);
+          if (plan->iter_clock_khz > 0 && hts[1] >= hts[0])
+            t_sigma += (double)(hts[1] - hts[0]) / (1e3 * plan->iter_clock_khz);
+          else
+            sigma_timed = false;
+          if (precision == TCI_SIGMA_FP64) n64 += ne;
+          else nmix += ne;
+          const double* hrd = reinterpret_cast<const double*>(hb + IL.rd);
+          const double* hmask = reinterpret_cast<const double*>(hb + IL.mask);
+          const double* hhd = reinterpret_cast<const double*>(hb + IL.hd);
+          std::vector<int> pos(na);
+          for (int i = 0; i < na; ++i) pos[order[i]] = i;
+          if (prm->verbose)
+            for (int i = 0; i < na; ++i) {
+              const int r = act[i];
+              std::printf("  %5d %5d %22.12f %15.8e %15.8e\n", it, r + 1, energy[r] + plan->e_ref,
+                          std::sqrt(std::max(hrd[pos[i]], 0.0)), dE[r]);
+            }
+          // roots that stop: on dE (host) or on the residual norm (masked in the graph)
+          std::vector<int> kept;  // new-vector indices t that join the basis
+          for (int i = 0; i < na; ++i) {
+            const int r = act[order[i]];
+            if (i < ne && hmask[i] == 0.0) {
+              kept.push_back(i);
+              continue;
+            }
+            if (precision != TCI_SIGMA_FP64) {
+              ready[r] = 1;
+            } else {
+              met[r] = 1;
+              conv[r] = 1;
+            }
+          }
+          // kept vectors to consecutive slots (a masked vector is zero, its sigma too)
+          for (int u = 0; u < (int)kept.size(); ++u)
+            if (kept[u] != u) {
+              TCI_HIP(hipMemcpyAsync(Vk(m + u), Vk(m + kept[u]), n * 8, hipMemcpyDeviceToDevice, st));
+              TCI_HIP(hipMemcpyAsync(Sk(m + u), Sk(m + kept[u]), n * 8, hipMemcpyDeviceToDevice, st));
+            }
+          // h(q, k) = <c_q, sigma_k> for the new k, q <= k
+          auto hcol = [&](int q_old, int t) {
+            const int g = t / 4, R = std::min(4, ne - 4 * g);
+            return hhd[(size_t)4 * IL.S * g + (size_t)q_old * R + t % 4];
+          };
+          for (int u = 0; u < (int)kept.size(); ++u) {
+            const int k = m + u, t = kept[u];
+            for (int q = 0; q <= k; ++q) {
+              const int q_old = q < m ? q : m + kept[q - m];
+              h[(size_t)q * mv + k] = h[(size_t)k * mv + q] = hcol(q_old, t);
+            }
+            root2idx[act[order[t]]] = k;
+          }
+          m += (int)kept.size();
+          sig_done = true;
+          all_conv = std::all_of(conv.begin(), conv.end(), [](char c) { return c != 0; });
+          if (all_conv) break;
+          continue;
+        }
+      }
       std::vector<double*> rs;
       for (int i = 0; i < na; ++i) rs.push_back(Vk(m + i));
       if (gen) {
@@ -946,7 +1482,7 @@ int davidson_impl(tci_plan* plan, const tci_davidson_params* prm, const double* 
       result->sigma_mixed = nmix;
       result->converged = std::all_of(met.begin(), met.end(), [](char c) { return c != 0; }) ? 1 : 0;
       result->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
-      result->seconds_sigma = t_sigma;
+      result->seconds_sigma = sigma_timed ? t_sigma : -1.0;
       result->collapses = n_collapse;
     }
     return TCI_OK;

@@ -645,15 +645,19 @@ void tci_plan::build(const tci_problem& p) {
 }
 
 void tci_plan::drop_graphs() {
-  if (graphs.empty()) return;
+  if (graphs.empty() && iter_graphs.empty()) return;
   // a replay may still be in flight on any stream
   TCI_HIP(hipDeviceSynchronize());
   for (auto& kv : graphs) (void)hipGraphExecDestroy(kv.second);
   graphs.clear();
+  for (auto& kv : iter_graphs) (void)hipGraphExecDestroy(kv.second);
+  iter_graphs.clear();
 }
 
 tci_plan::~tci_plan() {
   for (auto& kv : graphs) (void)hipGraphExecDestroy(kv.second);
+  for (auto& kv : iter_graphs) (void)hipGraphExecDestroy(kv.second);
+  if (iter_host) (void)hipHostFree(iter_host);
   if (gstream) (void)hipStreamDestroy(gstream);
   for (hipEvent_t& e : ev)
     if (e) (void)hipEventDestroy(e);

@@ -192,6 +192,16 @@ struct tci_plan {
   hipStream_t gstream = nullptr;    // capture stream (default priority)
   std::map<int, hipGraphExec_t> graphs;
   bool capturing = false;
+  // whole Davidson iterations captured as HIP graphs (davidson.hip): one
+  // executable per (precision, active roots, new vectors, basis slots), the
+  // device block their replays read the iteration's state from and write
+  // their results to, its pinned host mirror, and the rate of the device
+  // clock that times the captured sigma builds
+  std::map<int64_t, hipGraphExec_t> iter_graphs;
+  tci::DevBuf iter_buf;
+  void* iter_host = nullptr;
+  size_t iter_host_bytes = 0;
+  int iter_clock_khz = 0;
   bool timing() const { return profiling && !capturing; }
   // per-kernel event timing of profiled eager builds (tci_profile_kernels):
   // kt_begin records a start event (-1 when not timing), kt_end the stop event

@@ -18,6 +18,7 @@ module tigersigma
   integer(c_int32_t), parameter, public :: TCI_ACPF = 1, TCI_ACPF2 = 2, TCI_MRAQCC = 3, TCI_ACPF_CUSTOM = 4, &
                                            TCI_ACPF_TEST = 9
   integer(c_int32_t), parameter, public :: TCI_ACCUMULATE = 256, TCI_ADD_EREF = 512, TCI_SIGMA_GRAPH = 1024
+  integer(c_int32_t), parameter, public :: TCI_DAVIDSON_GRAPH = 2048
 
   type, bind(c), public :: tci_problem
     integer(c_int64_t) :: n_int, n_v, n_chol
