@@ -126,3 +126,25 @@ def test_vv_davidson_roots():
     assert r2["converged"]
     assert np.max(np.abs(np.asarray(e1) - np.asarray(e0))) < 1e-9
     assert np.max(np.abs(np.asarray(e2) - np.asarray(e0))) < 1e-6
+
+
+@pytest.mark.parametrize("tile", ["128x192T", "128x192T2"])
+@pytest.mark.parametrize("precision", ["bf16x3", "fp16x3"])
+@pytest.mark.parametrize("args", CASES)
+def test_vv_transposed_split_product(args, precision, tile):
+    """S^T = V X^T (walks along the tiles' N side), one and two K-splits,
+    transposed back before the gather: against the oracle at the split
+    bound, and against the untransposed product to FP32 summation order."""
+    _say("case: resident ladder transposed", precision, tile, args)
+    prob = make_problem(**args)
+    ints = Integrals(prob)
+    xs = [random_ci_vector(prob, 110 + k) for k in range(2)]
+    with _env(TCI_LADDER_VV=1, TCI_AUTOTUNE=0):
+        ref = [_sigma(tci.Plan.from_problem(prob), x, precision=precision) for x in xs[:1]]
+    with _env(TCI_LADDER_VV=1, TCI_AUTOTUNE=0, TCI_VV_TILE=tile):
+        plan = tci.Plan.from_problem(prob)
+        got = [_sigma(plan, x, precision=precision) for x in xs]
+        assert _rel(got[0], ref[0]) < 1e-6
+        for x, g in zip(xs, got):
+            assert _rel(g, oracle_sigma(prob, x, ints)) < SPLIT_TOL[precision]
+        assert np.array_equal(_sigma(plan, xs[0], precision=precision, graph=True), got[0])

@@ -947,7 +947,8 @@ void mixed_prepare(tci_plan& P, int precision, hipStream_t st) {
     if (P.vv.split_use >= 0) {  // decided for the other split precision: same schedule, same sums
       use = P.vv.split_use == 1;
       if (use && P.vv.split_tile > 0) {
-        vv_set_schedule(P, precision, P.vv.split_tile, P.vv.split_tile_n, P.vv.split_dma, P.vv.split_ks, st);
+        vv_set_schedule(P, precision, P.vv.split_tile, P.vv.split_tile_n, P.vv.split_dma, P.vv.split_ks, st,
+                        P.vv.split_trans);
         P.vv.gx.remap = P.vv.split_order;
       }
       log += " kept";

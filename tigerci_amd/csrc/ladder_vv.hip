@@ -26,7 +26,11 @@
 // X+ [P][n_p], X- [P'][n_m]; S+, S- the same shapes, one copy per K-split.
 // Split precisions keep V+- and X+- as split rows (hi/lo 16-bit parts,
 // common.hpp) and compute S = X V^T on the split GEMM; FP64 uses the dense
-// FP64 GEMM with V as the K x N operand.
+// FP64 GEMM with V as the K x N operand.  The split product can also run as
+// S^T = V X^T: with a few hundred walks, the walks then lie along the tiles'
+// N side, where 192-wide tiles cover 518 walks in three tiles (V read three
+// times) against five 128-row tiles the other way round; S^T is transposed
+// back before the gather.
 #include <algorithm>
 #include <cmath>
 
@@ -207,6 +211,27 @@ __global__ void vv_gather_kernel(PairTab T, const double* __restrict__ Sp, int64
   }
 }
 
+// S[r][c] = sum over the nparts K-split partials of ST[c][r] (ST: [cols][rows]
+// per partial, S: [rows][cols]); 32 x 32 tiles through LDS, 256 threads
+__global__ void vv_untranspose_kernel(const double* __restrict__ ST, int64_t part, int64_t rows, int64_t cols,
+                                      int nparts, double* __restrict__ S) {
+  __shared__ double t[32][33];
+  const int64_t c0 = (int64_t)blockIdx.x * 32, r0 = (int64_t)blockIdx.y * 32;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+  for (int i = ty; i < 32; i += 8) {
+    const int64_t c = c0 + i, r = r0 + tx;
+    double v = 0.0;
+    if (c < cols && r < rows)
+      for (int k = 0; k < nparts; ++k) v += ST[k * part + c * rows + r];
+    t[i][tx] = v;
+  }
+  __syncthreads();
+  for (int i = ty; i < 32; i += 8) {
+    const int64_t r = r0 + i, c = c0 + tx;
+    if (r < rows && c < cols) S[r * cols + c] = t[tx][i];
+  }
+}
+
 inline unsigned blocks(int64_t work, unsigned cap) {
   return (unsigned)std::min<int64_t>(std::max<int64_t>(cdiv(work, 256), 1), cap);
 }
@@ -239,7 +264,8 @@ constexpr int kScV = 9, kScOut = 10, kScC = 1;
 // S = X V (FP64: V as K x N; split: X V^T over split rows) for both parts as
 // one program: tasks ordered by column block of V, then K-split, then row
 // block of X, so the row tiles that read one V panel are neighbours.
-void build_products(tci_plan& P, GemmProgram& g, bool split, int tile, int tile_n, int dma, int ks, hipStream_t st) {
+void build_products(tci_plan& P, GemmProgram& g, bool split, int tile, int tile_n, int dma, int ks, bool trans,
+                    hipStream_t st) {
   tci_plan::VVLadder& V = P.vv;
   g.clear();
   g.dense = !split;
@@ -257,10 +283,33 @@ void build_products(tci_plan& P, GemmProgram& g, bool split, int tile, int tile_
     int64_t n, rows, ldx;
   };
   const int64_t lx_p = split ? 2 * xpad(V.np) : V.np, lx_m = split ? 2 * xpad(V.nm) : V.nm;
-  std::vector<Part> parts = {{split ? V.Xpx.p : V.Xp.p, split ? V.Vpx.p : V.Vp.p, V.Sp.as<double>(), V.np,
-                              (int64_t)P.n_pairs, lx_p}};
+  std::vector<Part> parts = {{split ? V.Xpx.p : V.Xp.p, split ? V.Vpx.p : V.Vp.p,
+                              (trans ? V.STp : V.Sp).as<double>(), V.np, (int64_t)P.n_pairs, lx_p}};
   if (V.ncm > 0)
-    parts.push_back({split ? V.Xmx.p : V.Xm.p, split ? V.Vmx.p : V.Vm.p, V.Sm.as<double>(), V.nm, (int64_t)V.ncm, lx_m});
+    parts.push_back({split ? V.Xmx.p : V.Xm.p, split ? V.Vmx.p : V.Vm.p, (trans ? V.STm : V.Sm).as<double>(), V.nm,
+                     (int64_t)V.ncm, lx_m});
+  if (trans) {
+    TCI_REQUIRE(split, "the transposed product is a split-precision schedule");
+    // S^T = V X^T: tasks by row block of V (tile rows of pairs), then K-split;
+    // the program tiles the walks along N
+    for (const Part& pt : parts) {
+      const int64_t K = xpad(pt.n);
+      const int64_t spart = pt.rows * pt.n;
+      for (int64_t m0 = 0; m0 < pt.n; m0 += tile) {
+        const int mm = (int)std::min<int64_t>(tile, pt.n - m0);
+        for (int k = 0; k < ks; ++k) {
+          const int64_t k0 = (K * k / ks) & ~(int64_t)15, k1 = k + 1 == ks ? K : (K * (k + 1) / ks) & ~(int64_t)15;
+          TCI_REQUIRE(k1 > k0, "empty K-split range");
+          const uint16_t* Vb = static_cast<const uint16_t*>(pt.Vm) + m0 * pt.ldx + 2 * k0;
+          const uint16_t* Xb = static_cast<const uint16_t*>(pt.X) + 2 * k0;
+          g.add(pt.S + k * spart + m0 * pt.rows, pt.rows, mm, (int)pt.rows,
+                {vseg(vop(Vb, pt.ldx), vop(Xb, pt.ldx), (int)(k1 - k0))}, false, false, 0, Blocking(), kScOut);
+        }
+      }
+    }
+    g.upload_tables(st);
+    return;
+  }
   for (const Part& pt : parts) {
     const int64_t K = split ? xpad(pt.n) : pt.n;
     const int64_t spart = pt.rows * pt.n;  // elements of one partial S
@@ -287,11 +336,15 @@ void build_products(tci_plan& P, GemmProgram& g, bool split, int tile, int tile_
   g.upload_tables(st);
 }
 
-void alloc_partials(tci_plan& P, int ks) {
+void alloc_partials(tci_plan& P, int ks, bool trans) {
   tci_plan::VVLadder& V = P.vv;
   const size_t need_p = (size_t)ks * P.n_pairs * V.np * 8, need_m = (size_t)ks * std::max(V.ncm, 1) * V.nm * 8;
   if (V.Sp.bytes < need_p) V.Sp.alloc(need_p);
   if (V.Sm.bytes < need_m) V.Sm.alloc(need_m);
+  if (trans) {
+    if (V.STp.bytes < need_p) V.STp.alloc(need_p);
+    if (V.STm.bytes < need_m) V.STm.alloc(need_m);
+  }
 }
 
 }  // namespace
@@ -372,18 +425,20 @@ void vv_build(tci_plan& P, hipStream_t st) {
   TCI_HIP(hipStreamSynchronize(st));
 }
 
-void vv_set_schedule(tci_plan& P, int precision, int tile, int tile_n, int dma, int ks, hipStream_t st) {
+void vv_set_schedule(tci_plan& P, int precision, int tile, int tile_n, int dma, int ks, hipStream_t st, bool trans) {
   tci_plan::VVLadder& V = P.vv;
   // every K-split range non-empty (at least 64 long), so every partial of S is written
   const int64_t kmin = V.ncm > 0 ? std::min(V.np, V.nm) : V.np;
   ks = (int)std::max<int64_t>(1, std::min<int64_t>(ks, kmin / 64));
-  alloc_partials(P, ks);
   if (precision == TCI_SIGMA_FP64) {
+    alloc_partials(P, ks, false);
     V.ks = ks;
-    build_products(P, V.g64, false, tile, 0, 0, ks, st);
+    build_products(P, V.g64, false, tile, 0, 0, ks, false, st);
   } else {
+    alloc_partials(P, ks, trans);
     V.ksx = ks;
-    build_products(P, V.gx, true, tile, tile_n, dma, ks, st);
+    V.xtrans = trans;
+    build_products(P, V.gx, true, tile, tile_n, dma, ks, trans, st);
   }
 }
 
@@ -419,11 +474,15 @@ void vv_prepare_split(tci_plan& P, int precision, hipStream_t st) {
   launch_fill(static_cast<double*>(V.Xpx.p), (int64_t)(V.Xpx.bytes / 8), 0.0, st);
   launch_fill(static_cast<double*>(V.Xmx.p), (int64_t)(V.Xmx.bytes / 8), 0.0, st);
   V.prec = precision;
-  // untuned schedule: 128 x 192 tiles, or TCI_VV_TILE=256 for the 32 x 32-MFMA kernel
+  // untuned schedule: 128 x 192 tiles, TCI_VV_TILE=256 (256w8, 256w16) for the
+  // 32 x 32-MFMA kernel, TCI_VV_TILE=128x192T (128x192T<k>: k K-splits) for
+  // the transposed product
   const char* te = std::getenv("TCI_VV_TILE");
+  const std::string ts = te ? te : "";
   if (te && std::atoi(te) == 256)
-    vv_set_schedule(P, precision, 256, 256, std::string(te) == "256w8" ? 8 : (std::string(te) == "256w16" ? 16 : 0), 2,
-                    st);
+    vv_set_schedule(P, precision, 256, 256, ts == "256w8" ? 8 : (ts == "256w16" ? 16 : 0), 2, st);
+  else if (ts.rfind("128x192T", 0) == 0)
+    vv_set_schedule(P, precision, 128, 192, 0, ts.size() > 8 ? std::max(1, std::atoi(te + 8)) : 1, st, true);
   else vv_set_schedule(P, precision, 128, 192, 0, 1, st);
   TCI_HIP(hipStreamSynchronize(st));
 }
@@ -456,7 +515,19 @@ void vv_ladder(tci_plan& P, int precision, hipStream_t st) {
     if (split) g.launch_split(precision, st, 0, sc);
     else g.launch(st);
   });
-  const int ks = split ? V.ksx : V.ks;
+  int ks = split ? V.ksx : V.ks;
+  if (split && V.xtrans) {  // S^T partials -> S (summed)
+    e0 = P.kt_begin(st);
+    const int64_t rp = P.n_pairs, rm = std::max(V.ncm, 1);
+    hipLaunchKernelGGL(vv_untranspose_kernel, dim3((unsigned)cdiv(V.np, 32), (unsigned)cdiv(rp, 32)), dim3(256), 0,
+                       st, V.STp.as<double>(), rp * V.np, rp, V.np, ks, V.Sp.as<double>());
+    if (V.ncm > 0)
+      hipLaunchKernelGGL(vv_untranspose_kernel, dim3((unsigned)cdiv(V.nm, 32), (unsigned)cdiv(rm, 32)), dim3(256), 0,
+                         st, V.STm.as<double>(), rm * V.nm, rm, V.nm, ks, V.Sm.as<double>());
+    TCI_HIP(hipGetLastError());
+    P.kt_end(e0, pn + " ladder vv_untranspose_kernel", 0.0, precision, st);
+    ks = 1;
+  }
   e0 = P.kt_begin(st);
   hipLaunchKernelGGL(vv_gather_kernel, dim3(blocks((int64_t)P.dmax * P.dmax, 64), (unsigned)P.n_pairs), dim3(256), 0,
                      st, P.tab, V.Sp.as<double>(), (int64_t)P.n_pairs * V.np, V.np, V.Sm.as<double>(),
@@ -471,6 +542,7 @@ double vv_tune(tci_plan& P, int precision, hipStream_t st, std::string& log) {
   tci_plan::VVLadder& V = P.vv;
   struct Cand {
     int tile, tile_n, dma, ks;
+    bool trans = false;
   };
   std::vector<Cand> cands;
   if (precision == TCI_SIGMA_FP64) {
@@ -485,21 +557,28 @@ double vv_tune(tci_plan& P, int precision, hipStream_t st, std::string& log) {
       cands.push_back({256, 256, 0, ks});
       cands.push_back({256, 256, 8, ks});  // dma field: eight waves
       cands.push_back({256, 256, 16, ks});  // sixteen waves
+      cands.push_back({128, 192, 0, ks, true});  // S^T = V X^T
+      cands.push_back({128, 128, 0, ks, true});
+      cands.push_back({192, 192, 3, ks, true});
     }
   }
+  auto label = [](const Cand& c) {
+    return std::to_string(c.tile) + (c.tile_n ? "x" + std::to_string(c.tile_n) : "") +
+           (c.dma ? (c.tile == 256 ? "w" + std::to_string(c.dma) : "d") : "") + (c.trans ? "T" : "") + "/" +
+           std::to_string(c.ks);
+  };
   double best = 1e300;
   Cand bc = cands[0];
   for (const Cand& c : cands) {
-    vv_set_schedule(P, precision, c.tile, c.tile_n, c.dma, c.ks, st);
+    vv_set_schedule(P, precision, c.tile, c.tile_n, c.dma, c.ks, st, c.trans);
     const double ms = time_ms(st, 2, [&] { vv_ladder(P, precision, st); });
-    log += " " + std::to_string(c.tile) + (c.tile_n ? "x" + std::to_string(c.tile_n) : "") +
-           (c.dma ? (c.tile == 256 ? "w" + std::to_string(c.dma) : "d") : "") + "/" + std::to_string(c.ks) + ":" + fmt_ms(ms);
+    log += " " + label(c) + ":" + fmt_ms(ms);
     if (ms < best * 0.98) {
       best = ms;
       bc = c;
     }
   }
-  vv_set_schedule(P, precision, bc.tile, bc.tile_n, bc.dma, bc.ks, st);
+  vv_set_schedule(P, precision, bc.tile, bc.tile_n, bc.dma, bc.ks, st, bc.trans);
   GemmProgram& g = precision == TCI_SIGMA_FP64 ? V.g64 : V.gx;
   const float* sc = P.mx.scales.as<float>();
   const int o = tune_order(g, st, [&] {
@@ -512,9 +591,9 @@ double vv_tune(tci_plan& P, int precision, hipStream_t st, std::string& log) {
     V.split_dma = bc.dma;
     V.split_ks = bc.ks;
     V.split_order = o;
+    V.split_trans = bc.trans;
   }
-  log += " -> " + std::to_string(bc.tile) + (bc.tile_n ? "x" + std::to_string(bc.tile_n) : "") +
-         (bc.dma ? (bc.tile == 256 ? "w" + std::to_string(bc.dma) : "d") : "") + "/" + std::to_string(bc.ks) + " order " + std::to_string(o);
+  log += " -> " + label(bc) + " order " + std::to_string(o);
   TCI_HIP(hipStreamSynchronize(st));
   return time_ms(st, 2, [&] { vv_ladder(P, precision, st); });
 }
@@ -522,7 +601,8 @@ double vv_tune(tci_plan& P, int precision, hipStream_t st, std::string& log) {
 int64_t vv_device_bytes(const tci_plan& P) {
   const tci_plan::VVLadder& V = P.vv;
   int64_t b = 0;
-  for (const DevBuf* x : {&V.Vp, &V.Vm, &V.Vpx, &V.Vmx, &V.Xp, &V.Xm, &V.Xpx, &V.Xmx, &V.Sp, &V.Sm, &V.colm})
+  for (const DevBuf* x : {&V.Vp, &V.Vm, &V.Vpx, &V.Vmx, &V.Xp, &V.Xm, &V.Xpx, &V.Xmx, &V.Sp, &V.Sm, &V.STp, &V.STm,
+                          &V.colm})
     b += (int64_t)x->bytes;
   return b;
 }

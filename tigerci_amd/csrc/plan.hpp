@@ -167,6 +167,10 @@ struct tci_plan {
     tci::DevBuf Vpx, Vmx;           // split rows of V+-
     tci::DevBuf Xp, Xm, Xpx, Xmx;   // X+- (FP64 rows, split rows)
     tci::DevBuf Sp, Sm;             // S+- partials of the K-splits
+    // split product run transposed, S^T = V X^T (pairs x walks, the walks as
+    // the tiles' N side), then transposed back into Sp / Sm
+    bool xtrans = false;
+    tci::DevBuf STp, STm;
     tci::DevBuf colm;               // [n_pairs] row of the walk in X- or -1
     int ks = 1, ksx = 1;            // K-splits of the FP64 / split products
     tci::GemmProgram g64, gx;       // the products (both parts in one launch)
@@ -175,6 +179,7 @@ struct tci_plan {
     // tile_n, dma (wave count for 256 tiles), K-split, order; use: -1 not yet
     // decided, 0 Cholesky-direct, 1 resident
     int split_tile = 0, split_tile_n = 0, split_dma = 0, split_ks = 1, split_order = 0, split_use = -1;
+    bool split_trans = false;
   } vv;
   tci::PairTab tab{};
   // accounting
@@ -271,7 +276,8 @@ void launch_split_scales(int mode, const double* x, int64_t n, unsigned* mx, int
 bool vv_fits(const tci_plan& P, double* bytes_out);
 double vv_flops(const tci_plan& P);
 void vv_build(tci_plan& P, hipStream_t st);
-void vv_set_schedule(tci_plan& P, int precision, int tile, int tile_n, int dma, int ks, hipStream_t st);
+void vv_set_schedule(tci_plan& P, int precision, int tile, int tile_n, int dma, int ks, hipStream_t st,
+                     bool trans = false);
 void vv_prepare_split(tci_plan& P, int precision, hipStream_t st);
 void vv_ladder(tci_plan& P, int precision, hipStream_t st);
 double vv_tune(tci_plan& P, int precision, hipStream_t st, std::string& log);
