@@ -511,11 +511,12 @@ void vv_ladder(tci_plan& P, int precision, hipStream_t st) {
   TCI_HIP(hipGetLastError());
   P.kt_end(e0, pn + " ladder vv_scatter_kernel", 0.0, precision, st);
   const GemmProgram& g = split ? V.gx : V.g64;
-  P.kt_run(pn + " ladder (ab|cd) product", g, precision, st, [&] {
+  int ks = split ? V.ksx : V.ks;  // K-splits change the launch (and its traffic), so they name it
+  P.kt_run(pn + " ladder (ab|cd) product" + (ks > 1 ? " K-split " + std::to_string(ks) : std::string()), g,
+           precision, st, [&] {
     if (split) g.launch_split(precision, st, 0, sc);
     else g.launch(st);
   });
-  int ks = split ? V.ksx : V.ks;
   if (split && V.xtrans) {  // S^T partials -> S (summed)
     e0 = P.kt_begin(st);
     const int64_t rp = P.n_pairs, rm = std::max(V.ncm, 1);
