@@ -63,9 +63,9 @@ enum tci_precision {
  * work (residuals and their norms, the preconditioner, Gram-Schmidt, the
  * sigma builds of the new vectors, the new columns of the subspace matrix)
  * as one HIP graph launch, with one host synchronisation per iteration for
- * the subspace eigenproblem.  Standard problem only; tci_davidson_gen_run
- * runs the vector work eagerly.  Implies TCI_SIGMA_GRAPH for the sigma
- * builds outside the iteration (the guess and collapses). */
+ * the subspace eigenproblem (tci_davidson_run and tci_davidson_gen_run).
+ * Implies TCI_SIGMA_GRAPH for the sigma builds outside the iteration (the
+ * guess and collapses). */
 #define TCI_DAVIDSON_GRAPH 0x800
 
 /* Problem description.  All pointers are host pointers, read during

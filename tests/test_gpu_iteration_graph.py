@@ -9,7 +9,8 @@ It must follow the eager solver: the same iterations, sigma counts and roots
 vectors may differ in the last bits), against the dense spectrum and the
 oracle Davidson, across groups of four roots, budget collapses, the
 mixed -> FP64 switch, roots that stop on their residual norm inside a replay,
-and repeated solves that reuse the captured graphs.
+repeated solves that reuse the captured graphs, and the generalized (ACPF)
+problem.
 """
 import sys
 
@@ -131,13 +132,18 @@ def test_iteration_graph_residual_norm_stops(precision, roots):
     _same_vectors(v0, v1, 1e-6)
 
 
-def test_iteration_graph_leaves_generalized_problem_eager():
-    _say("case: iteration graph flag on the generalized problem")
+@pytest.mark.parametrize("functional", ["ACPF", "ACPF_TEST"])
+def test_iteration_graph_generalized_problem(functional):
+    """(H - E_ref) c = E G c: G in the residuals and the preconditioner, the
+    metric columns <c_q, G c_k> beside those of h, all inside the replay."""
+    _say("case: iteration graph, generalized problem", functional)
     prob = make_problem(4, 30, 40, 12)
     plan = tci.Plan.from_problem(prob)
-    g = tci.acpf_gvalues(tci.ACPF, 10)
+    g = tci.acpf_gvalues(getattr(tci, functional), 10)
     kw = dict(roots=2, energy_tol=1e-9, norm_tol=1e-6, g_values=g)
     e0, r0 = plan.davidson(**kw)
     e1, r1 = plan.davidson(graph="iteration", **kw)
-    assert r1["converged"] and r1["iterations"] == r0["iterations"]
-    assert np.abs(e0 - e1).max() < 1e-12
+    _say(f"  eager {e0} ({r0['iterations']} it)  graph {e1} ({r1['iterations']} it)")
+    assert r0["converged"] and r1["converged"]
+    assert r1["iterations"] == r0["iterations"]
+    assert np.abs(e0 - e1).max() < 1e-10

@@ -366,8 +366,7 @@ class Plan:
         """Lowest roots; returns (total energies, result dict).  ``vectors`` is an
         optional DeviceArray of roots*n_csf doubles for the eigenvectors;
         ``graph=True`` replays every sigma build from a captured HIP graph,
-        ``graph="iteration"`` every iteration's device work (standard problem;
-        the generalized one keeps its vector work eager);
+        ``graph="iteration"`` every iteration's device work;
         ``g_values`` (five g values, see ``acpf_gvalues``) solves the
         generalized ACPF-type problem (H - E_ref) c = E G c instead."""
         if switch_tol is None:

@@ -26,7 +26,9 @@
 // the device, and a root whose residual norm is below norm_tol has its new
 // vector zeroed in the graph and dropped by the host afterwards (its sigma is
 // built and discarded).  The host synchronises once per iteration, for the
-// subspace problem and the convergence bookkeeping.
+// subspace problem and the convergence bookkeeping.  The generalized problem
+// adds its metric: G in the residuals and the preconditioner, and the new
+// columns <c_q, G c_k> beside those of h.
 // Mixed precision: expansion sigmas use the split ladder until every root's
 // |dE| < switch_tol; then the subspace collapses to the current Ritz vectors,
 // their sigmas are rebuilt in FP64 and the solve finishes in FP64.  The
@@ -278,6 +280,7 @@ __global__ void unit_kernel(double* __restrict__ v, int64_t n, int64_t k) {
 struct IterState {
   double* V;  // basis store: slot k at V + k n
   double* S;  // sigma store, same slots
+  const double* G;  // diagonal metric of the generalized problem, null for the standard one
   int64_t n;
   int m;      // basis size when the iteration starts
   int pad;
@@ -300,10 +303,11 @@ __device__ __forceinline__ double* islot(const IterState& s, SlotSpec q, int k) 
 // mdotm_kernel over slot ranges: partial[(k R + j) nblk + b] = block b's share
 // of X_k . W_j for k < count.  A set gate that reads 0 skips the launch's work
 // (the second Gram-Schmidt sweep is switched on the device).  256 threads.
+// (with wg, W_j weighted by the metric: X_k . G W_j)
 template <int R>
 __global__ __launch_bounds__(256) void sdots_kernel(const IterState* __restrict__ ps, SlotSpec xs, CountSpec xc,
                                                     SlotSpec ws, double* __restrict__ partial, int nblk,
-                                                    const int* __restrict__ gate) {
+                                                    const int* __restrict__ gate, int wg) {
   if (gate && *gate == 0) return;
   const IterState s = *ps;
   const int m = xc.mmul * s.m + xc.off;
@@ -323,8 +327,9 @@ __global__ __launch_bounds__(256) void sdots_kernel(const IterState* __restrict_
     for (int j = 0; j < R; ++j) acc[q][j] = 0.0;
   for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < s.n; e += (int64_t)nblk * blockDim.x) {
     double ve[R];
+    const double ge = wg ? s.G[e] : 1.0;
 #pragma unroll
-    for (int j = 0; j < R; ++j) ve[j] = v[j][e];
+    for (int j = 0; j < R; ++j) ve[j] = wg ? ge * v[j][e] : v[j][e];
 #pragma unroll
     for (int q = 0; q < MD_GROUP; ++q)
       if (q < g) {
@@ -393,25 +398,41 @@ __global__ void sproject_kernel(const IterState* __restrict__ ps, SlotSpec ws, S
   }
 }
 
-// residuals r_j = sum_{q < m} c[j][2q] sigma_q + c[j][2q+1] c_q into basis
-// slots m + j0 + j; c rows of length ld, c[j][2q+1] = -E y_q
+// residuals into basis slots m + j0 + j, from c rows of length ld:
+//   standard:     r_j = sum_{q < m} c[j][2q] sigma_q + c[j][2q+1] c_q, c[j][2q+1] = -E_j y_q
+//   generalized:  r_j = sum_q c[j][2q] sigma_q - E_j G (sum_q c[j][2q+1] c_q), c[j][2q+1] = y_q
 template <int R>
-__global__ void sres_kernel(const IterState* __restrict__ ps, int j0, const double* __restrict__ coef, int ld) {
+__global__ void sres_kernel(const IterState* __restrict__ ps, int j0, const double* __restrict__ coef, int ld,
+                            const double* __restrict__ eres) {
   const IterState s = *ps;
   double* o[R];
 #pragma unroll
   for (int j = 0; j < R; ++j) o[j] = s.V + (int64_t)(s.m + j0 + j) * s.n;
   for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < s.n; e += (int64_t)gridDim.x * blockDim.x) {
-    double a[R];
+    double a[R], b[R];
 #pragma unroll
-    for (int j = 0; j < R; ++j) a[j] = 0.0;
-    for (int q = 0; q < s.m; ++q) {
-      const double se = s.S[(int64_t)q * s.n + e];
+    for (int j = 0; j < R; ++j) a[j] = b[j] = 0.0;
+    if (!s.G) {
+      for (int q = 0; q < s.m; ++q) {
+        const double se = s.S[(int64_t)q * s.n + e];
 #pragma unroll
-      for (int j = 0; j < R; ++j) a[j] += coef[(j0 + j) * ld + 2 * q] * se;
-      const double ce = s.V[(int64_t)q * s.n + e];
+        for (int j = 0; j < R; ++j) a[j] += coef[(j0 + j) * ld + 2 * q] * se;
+        const double ce = s.V[(int64_t)q * s.n + e];
 #pragma unroll
-      for (int j = 0; j < R; ++j) a[j] += coef[(j0 + j) * ld + 2 * q + 1] * ce;
+        for (int j = 0; j < R; ++j) a[j] += coef[(j0 + j) * ld + 2 * q + 1] * ce;
+      }
+    } else {
+      for (int q = 0; q < s.m; ++q) {
+        const double se = s.S[(int64_t)q * s.n + e], ce = s.V[(int64_t)q * s.n + e];
+#pragma unroll
+        for (int j = 0; j < R; ++j) {
+          a[j] += coef[(j0 + j) * ld + 2 * q] * se;
+          b[j] += coef[(j0 + j) * ld + 2 * q + 1] * ce;
+        }
+      }
+      const double ge = s.G[e];
+#pragma unroll
+      for (int j = 0; j < R; ++j) a[j] += -eres[j0 + j] * ge * b[j];
     }
 #pragma unroll
     for (int j = 0; j < R; ++j) o[j][e] = a[j];
@@ -434,7 +455,8 @@ __global__ void smask_kernel(const IterState* __restrict__ ps, int ne, const dou
   mask[j] = rnok[j] && rn < ps->norm_tol ? 1.0 : 0.0;
 }
 
-// r <- -r / (H_ii - E_j) (0 where that is 0) on slot m + j; zero if masked
+// r <- -r / (H_ii - E_j g_i) (g = 1 for the standard problem; 0 where that
+// is 0) on slot m + j; zero if masked
 __global__ void sprecond_kernel(const IterState* __restrict__ ps, int j, const double* __restrict__ diag,
                                 const double* __restrict__ eng, const double* __restrict__ mask) {
   const IterState s = *ps;
@@ -442,7 +464,7 @@ __global__ void sprecond_kernel(const IterState* __restrict__ ps, int j, const d
   const double E = eng[j];
   const bool off = mask[j] != 0.0;
   for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < s.n; e += (int64_t)gridDim.x * blockDim.x) {
-    const double p = diag[e] - E;
+    const double p = s.G ? diag[e] - E * s.G[e] : diag[e] - E;
     r[e] = (off || p == 0.0) ? 0.0 : -r[e] / p;
   }
 }
@@ -757,24 +779,28 @@ struct Dav {
 // Byte layout of the plan's iteration block for S basis slots: the part the
 // host uploads before a replay (state, energies, norm-test switches, residual
 // coefficients), the part copied back after it (squared residual norms,
-// vanished-vector flags, new subspace columns: group g of four new vectors at
-// hd + 4 S g doubles, entry q R + j), then device scratch.  The host mirror
+// vanished-vector flags, device clock stamps, new subspace columns: group g of
+// four new vectors at hd + 4 S g doubles, entry q R + j, and for the
+// generalized problem the new metric columns at gsd, laid out the same), then
+// device scratch.  The host mirror
 // (pinned) uses the same offsets.
 struct IterLayout {
   static constexpr int RMAX = 64;  // roots
   int S, ld;
-  size_t eng, rnok, cres, up_end, rd, mask, ts, hd, out_end, w2, fac, flag, dots, partial, total;
+  size_t eng, eres, rnok, cres, up_end, rd, mask, ts, hd, gsd, out_end, w2, fac, flag, dots, partial, total;
   IterLayout(int S_, int nblk) : S(S_), ld(2 * S_) {
     auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
     eng = 256;  // IterState at 0
-    rnok = al(eng + RMAX * 8);
+    eres = al(eng + RMAX * 8);
+    rnok = al(eres + RMAX * 8);
     cres = al(rnok + RMAX * 4);
     up_end = al(cres + (size_t)RMAX * ld * 8);
     rd = up_end;
     mask = al(rd + RMAX * 8);
     ts = al(mask + RMAX * 8);  // device clock before and after the sigma builds
     hd = al(ts + 16);
-    out_end = al(hd + (size_t)RMAX * S * 8);
+    gsd = al(hd + (size_t)RMAX * S * 8);  // metric columns <c_q, G c_k>, laid out as hd
+    out_end = al(gsd + (size_t)RMAX * S * 8);
     w2 = out_end;
     fac = al(w2 + RMAX * 8);
     flag = al(fac + RMAX * 8);
@@ -825,10 +851,11 @@ void iter_ensure(tci_plan& P, const IterLayout& L) {
 //   normalisation (zero below 1e-14); the sigma builds of the new vectors
 //   (between two reads of the device clock); the new columns <c_q, sigma_k>, q < m + ne;
 //   one copy of the results to the host mirror.
-void iter_record(tci_plan& P, const IterLayout& L, int nblk, int precision, int na, int ne, hipStream_t cs) {
+void iter_record(tci_plan& P, const IterLayout& L, int nblk, int precision, int na, int ne, bool gen, hipStream_t cs) {
   char* base = static_cast<char*>(P.iter_buf.p);
   const IterState* ps = reinterpret_cast<const IterState*>(base);
   const double* eng = reinterpret_cast<const double*>(base + L.eng);
+  const double* eres = reinterpret_cast<const double*>(base + L.eres);
   const int* rnok = reinterpret_cast<const int*>(base + L.rnok);
   const double* cres = reinterpret_cast<const double*>(base + L.cres);
   double* rd = reinterpret_cast<double*>(base + L.rd);
@@ -843,13 +870,13 @@ void iter_record(tci_plan& P, const IterLayout& L, int nblk, int precision, int 
   const unsigned gx = (unsigned)std::min<int64_t>(cdiv(n, 256), 8192);
   const double* diag = P.diag.as<double>();
   // dots of X (count xc, at most xmax) with R vectors W, reduced into out
-  auto dotsR = [&](SlotSpec xs, CountSpec xc, int xmax, SlotSpec ws, int R, double* out, const int* gate) {
+  auto dotsR = [&](SlotSpec xs, CountSpec xc, int xmax, SlotSpec ws, int R, double* out, const int* gate, int wg = 0) {
     const dim3 grid(nblk, (unsigned)cdiv(xmax, MD_GROUP));
     switch (R) {
-      case 1: hipLaunchKernelGGL(sdots_kernel<1>, grid, dim3(256), 0, cs, ps, xs, xc, ws, partial, nblk, gate); break;
-      case 2: hipLaunchKernelGGL(sdots_kernel<2>, grid, dim3(256), 0, cs, ps, xs, xc, ws, partial, nblk, gate); break;
-      case 3: hipLaunchKernelGGL(sdots_kernel<3>, grid, dim3(256), 0, cs, ps, xs, xc, ws, partial, nblk, gate); break;
-      default: hipLaunchKernelGGL(sdots_kernel<4>, grid, dim3(256), 0, cs, ps, xs, xc, ws, partial, nblk, gate); break;
+      case 1: hipLaunchKernelGGL(sdots_kernel<1>, grid, dim3(256), 0, cs, ps, xs, xc, ws, partial, nblk, gate, wg); break;
+      case 2: hipLaunchKernelGGL(sdots_kernel<2>, grid, dim3(256), 0, cs, ps, xs, xc, ws, partial, nblk, gate, wg); break;
+      case 3: hipLaunchKernelGGL(sdots_kernel<3>, grid, dim3(256), 0, cs, ps, xs, xc, ws, partial, nblk, gate, wg); break;
+      default: hipLaunchKernelGGL(sdots_kernel<4>, grid, dim3(256), 0, cs, ps, xs, xc, ws, partial, nblk, gate, wg); break;
     }
     TCI_HIP(hipGetLastError());
     hipLaunchKernelGGL(sreduce_kernel, dim3((unsigned)(xmax * R)), dim3(256), 0, cs, ps, xc, R, partial, nblk, out,
@@ -871,10 +898,10 @@ void iter_record(tci_plan& P, const IterLayout& L, int nblk, int precision, int 
   for (int j0 = 0; j0 < na; j0 += 4) {
     const int R = std::min(4, na - j0);
     switch (R) {
-      case 1: hipLaunchKernelGGL(sres_kernel<1>, dim3(gx), dim3(256), 0, cs, ps, j0, cres, L.ld); break;
-      case 2: hipLaunchKernelGGL(sres_kernel<2>, dim3(gx), dim3(256), 0, cs, ps, j0, cres, L.ld); break;
-      case 3: hipLaunchKernelGGL(sres_kernel<3>, dim3(gx), dim3(256), 0, cs, ps, j0, cres, L.ld); break;
-      default: hipLaunchKernelGGL(sres_kernel<4>, dim3(gx), dim3(256), 0, cs, ps, j0, cres, L.ld); break;
+      case 1: hipLaunchKernelGGL(sres_kernel<1>, dim3(gx), dim3(256), 0, cs, ps, j0, cres, L.ld, eres); break;
+      case 2: hipLaunchKernelGGL(sres_kernel<2>, dim3(gx), dim3(256), 0, cs, ps, j0, cres, L.ld, eres); break;
+      case 3: hipLaunchKernelGGL(sres_kernel<3>, dim3(gx), dim3(256), 0, cs, ps, j0, cres, L.ld, eres); break;
+      default: hipLaunchKernelGGL(sres_kernel<4>, dim3(gx), dim3(256), 0, cs, ps, j0, cres, L.ld, eres); break;
     }
     TCI_HIP(hipGetLastError());
   }
@@ -922,13 +949,17 @@ void iter_record(tci_plan& P, const IterLayout& L, int nblk, int precision, int 
   for (int j0 = 0, g = 0; j0 < ne; j0 += 4, ++g) {
     const int R = std::min(4, ne - j0);
     dotsR(basis, {1, ne}, L.S, {1, 1, j0}, R, hd + (size_t)4 * L.S * g, nullptr);
+    if (gen)  // metric columns <c_q, G c_k>
+      dotsR(basis, {1, ne}, L.S, {0, 1, j0}, R, reinterpret_cast<double*>(base + L.gsd) + (size_t)4 * L.S * g, nullptr,
+            1);
   }
-  const size_t out_bytes = L.hd - L.rd + (size_t)cdiv(ne, 4) * 4 * L.S * 8;
+  const size_t out_bytes = (gen ? L.gsd : L.hd) - L.rd + (size_t)cdiv(ne, 4) * 4 * L.S * 8;
   TCI_HIP(hipMemcpyAsync(static_cast<char*>(P.iter_host) + L.rd, base + L.rd, out_bytes, hipMemcpyDeviceToHost, cs));
 }
 
-hipGraphExec_t iter_graph(tci_plan& P, const IterLayout& L, int nblk, int precision, int na, int ne) {
-  const int64_t key = ((int64_t)L.S << 24) | ((int64_t)precision << 16) | ((int64_t)na << 8) | ne;
+hipGraphExec_t iter_graph(tci_plan& P, const IterLayout& L, int nblk, int precision, int na, int ne, bool gen) {
+  const int64_t key =
+      ((int64_t)L.S << 25) | ((int64_t)gen << 24) | ((int64_t)precision << 16) | ((int64_t)na << 8) | ne;
   auto it = P.iter_graphs.find(key);
   if (it != P.iter_graphs.end()) return it->second;
   hipStream_t cs = P.gstream;
@@ -936,7 +967,7 @@ hipGraphExec_t iter_graph(tci_plan& P, const IterLayout& L, int nblk, int precis
   TCI_HIP(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
   P.capturing = true;
   try {
-    iter_record(P, L, nblk, precision, na, ne, cs);
+    iter_record(P, L, nblk, precision, na, ne, gen, cs);
   } catch (...) {
     P.capturing = false;
     (void)hipStreamEndCapture(cs, &g);
@@ -989,7 +1020,7 @@ int davidson_impl(tci_plan* plan, const tci_davidson_params* prm, const double* 
     // iteration of the standard problem (and the builds outside it)
     TCI_REQUIRE((prm->precision & ~(0xff | TCI_SIGMA_GRAPH | TCI_DAVIDSON_GRAPH)) == 0, "unknown bits in precision");
     const int graph_flag = prm->precision & (TCI_SIGMA_GRAPH | TCI_DAVIDSON_GRAPH) ? TCI_SIGMA_GRAPH : 0;
-    const bool igraph = (prm->precision & TCI_DAVIDSON_GRAPH) != 0 && gvals == nullptr;
+    const bool igraph = (prm->precision & TCI_DAVIDSON_GRAPH) != 0;
     const int prec0 = prm->precision & 0xff;
     TCI_REQUIRE(prec0 == TCI_SIGMA_FP64 || prec0 == TCI_SIGMA_BF16X3 || prec0 == TCI_SIGMA_FP16X3, "unknown precision");
     const int64_t n = plan->n_csf;
@@ -1300,9 +1331,10 @@ int davidson_impl(tci_plan* plan, const tci_davidson_params* prm, const double* 
           if (precision != TCI_SIGMA_FP64) mixed_prepare(*plan, precision, st);
           iter_ensure(*plan, IL);
           char* hb = static_cast<char*>(plan->iter_host);
-          IterState hs{Vp, Sp, n, m, 0, prm->norm_tol};
+          IterState hs{Vp, Sp, gen ? gd : nullptr, n, m, 0, prm->norm_tol};
           std::memcpy(hb, &hs, sizeof(hs));
           double* heng = reinterpret_cast<double*>(hb + IL.eng);
+          double* heres = reinterpret_cast<double*>(hb + IL.eres);
           int* hrnok = reinterpret_cast<int*>(hb + IL.rnok);
           double* hcres = reinterpret_cast<double*>(hb + IL.cres);
           for (int t = 0; t < ne; ++t) {
@@ -1311,13 +1343,14 @@ int davidson_impl(tci_plan* plan, const tci_davidson_params* prm, const double* 
           }
           for (int i = 0; i < na; ++i) {
             const int r = act[order[i]];
+            heres[i] = energy[r];
             for (int q = 0; q < m; ++q) {
               const double y = Y[(size_t)q * m + r];
               hcres[(size_t)i * IL.ld + 2 * q] = y;
-              hcres[(size_t)i * IL.ld + 2 * q + 1] = -energy[r] * y;
+              hcres[(size_t)i * IL.ld + 2 * q + 1] = gen ? y : -energy[r] * y;
             }
           }
-          hipGraphExec_t ex = iter_graph(*plan, IL, D.nblk, precision, na, ne);
+          hipGraphExec_t ex = iter_graph(*plan, IL, D.nblk, precision, na, ne, gen);
           TCI_HIP(hipMemcpyAsync(plan->iter_buf.p, hb, IL.cres + (size_t)na * IL.ld * 8, hipMemcpyHostToDevice, st));
           TCI_HIP(hipGraphLaunch(ex, st));
           TCI_HIP(hipStreamSynchronize(st));
@@ -1361,15 +1394,17 @@ int davidson_impl(tci_plan* plan, const tci_davidson_params* prm, const double* 
               TCI_HIP(hipMemcpyAsync(Sk(m + u), Sk(m + kept[u]), n * 8, hipMemcpyDeviceToDevice, st));
             }
           // h(q, k) = <c_q, sigma_k> for the new k, q <= k
-          auto hcol = [&](int q_old, int t) {
+          const double* hgsd = reinterpret_cast<const double*>(hb + IL.gsd);
+          auto col = [&](const double* blk, int q_old, int t) {
             const int g = t / 4, R = std::min(4, ne - 4 * g);
-            return hhd[(size_t)4 * IL.S * g + (size_t)q_old * R + t % 4];
+            return blk[(size_t)4 * IL.S * g + (size_t)q_old * R + t % 4];
           };
           for (int u = 0; u < (int)kept.size(); ++u) {
             const int k = m + u, t = kept[u];
             for (int q = 0; q <= k; ++q) {
               const int q_old = q < m ? q : m + kept[q - m];
-              h[(size_t)q * mv + k] = h[(size_t)k * mv + q] = hcol(q_old, t);
+              h[(size_t)q * mv + k] = h[(size_t)k * mv + q] = col(hhd, q_old, t);
+              if (gen) gs[(size_t)q * mv + k] = gs[(size_t)k * mv + q] = col(hgsd, q_old, t);
             }
             root2idx[act[order[t]]] = k;
           }
