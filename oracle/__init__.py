@@ -50,8 +50,9 @@ How each function is pinned (the checks live in ``tests/``):
   * ladder records: the closed-form count (every unordered pair of a walk's
     virtuals), positions that point back at the record's virtuals, the
     reference's order, and ignorable pairs removing exactly their records.
-Parity unpinned: the phase of the N-1 CSFs and of the i = j pair walks
-relative to the valence CSF (the loop types that would fix them are not
-restated); the CSF phase of i > j pair walks is pinned by the restated
-valence <-> N-2 coupling coefficients.
+Parity unpinned: the phase of the i = j pair walks relative to the valence
+CSF (the loop types that would fix it are not restated).  The CSF phase of
+i > j pair walks is pinned by the restated valence <-> N-2 coupling
+coefficients, and that of the N-1 walks by the valence <-> N-1 coefficient
+(+sqrt 2 times the reference's G(a), tests/test_spin.py).
 """

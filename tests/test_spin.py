@@ -130,3 +130,44 @@ def test_csf_phase_of_pair_walks():
             assert abs(sigma(prob, x, ints)[0] - ref) < 1e-12 * max(1.0, abs(ref))
             checked += 1
     assert checked > 20
+
+
+def test_csf_phase_of_single_walks():
+    """The valence <-> N-1 element of the reference: in
+    one_intern_seg_compl_vec_lmo (F/two_seg_mod_3.f90:3018-3105, loop type 2
+    of one_internal_seg_2_cho, :617-745) Ka = G(a) from make_gij_a_pao
+    (:3332-3427), h_ia + sum_{k != i} [2(ia|kk) - (ik|ka)] + (ia|ii), added on
+    the diagonal by add_in_Gij_vec (case 3, :849-879), times
+    Px = contracted_cycle_stateless(jbar, ibar, 0, 0, jbar, 0, 2, 0) with
+    jbar = 1 (the open shell i) and ibar = 2 (one_info_simplifier, :747-843).
+    sigma_0 of every unit vector on a single, and sigma of the valence unit
+    vector on every single, equal Px G(a): the N-1 CSFs carry the reference's
+    phase relative to the valence (a flipped single CSF would give -Px G)."""
+    px = contracted_cycle(SpinTables(2, 1), 1, 2, 0, 0, 1, 0, 2, 0)
+    assert px.shape == (1, 1) and abs(px[0, 0] - np.sqrt(2.0)) < 1e-14
+    prob = make_problem(3, 5, 6, 4, h_offdiag=0.2)
+    ints = Integrals(prob)
+    lay = prob.layout()
+    no, nv = prob.n_int, prob.n_v
+    Lf = prob.L_full()
+
+    def eri(p, q, r, s):
+        return float(np.dot(Lf[:, p, q], Lf[:, r, s]))
+
+    xv = np.zeros(lay.n_csf)
+    xv[lay.ref_off] = 1.0
+    s_v = sigma(prob, xv, ints)
+    big = 0
+    for i in range(no):
+        for a in range(nv):
+            A = no + a
+            g = prob.h[i, A] + eri(i, A, i, i)
+            g += sum(2.0 * eri(i, A, k, k) - eri(i, k, k, A) for k in range(no) if k != i)
+            ref = px[0, 0] * g
+            k = lay.single_off + (no - 1 - i) * nv + a
+            x = np.zeros(lay.n_csf)
+            x[k] = 1.0
+            assert abs(sigma(prob, x, ints)[0] - ref) < 1e-12 * max(1.0, abs(ref))
+            assert abs(s_v[k] - ref) < 1e-12 * max(1.0, abs(ref))
+            big += abs(ref) > 1e-2
+    assert big > no * nv // 2
