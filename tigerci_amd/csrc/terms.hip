@@ -311,16 +311,21 @@ __global__ __launch_bounds__(256) void s3int_singles_kernel(PairTab T, const int
 // g[i][a] = gss + sum of the K-split partials of gW + every gpart entry of a
 // walk whose domain holds a.  One block per virtual a; thread (i, slice) sums
 // the partials and inverse-map entries of its slice for internal i (64 at a
-// time), and the four slices meet in LDS (fixed order: deterministic).
-__global__ __launch_bounds__(256) void singles_reduce_kernel(PairTab T, InvTab I, const int64_t* __restrict__ gpart_off,
-                                                             const double* __restrict__ gpart,
-                                                             const double* __restrict__ gss,
-                                                             const double* __restrict__ gW, int n_split,
-                                                             double* __restrict__ g) {
+// time), and the sixteen slices meet in LDS (fixed order: deterministic).
+// Sixteen slices keep each thread's chain of dependent loads (inverse map,
+// walk tables, then the partial) short: a launch has only n_v blocks.
+constexpr int kRedSlices = 16;
+
+__global__ __launch_bounds__(64 * kRedSlices) void singles_reduce_kernel(PairTab T, InvTab I,
+                                                                         const int64_t* __restrict__ gpart_off,
+                                                                         const double* __restrict__ gpart,
+                                                                         const double* __restrict__ gss,
+                                                                         const double* __restrict__ gW, int n_split,
+                                                                         double* __restrict__ g) {
   const int nv = T.n_v, no = T.n_int, a = blockIdx.x;
   const int64_t n = (int64_t)no * nv;
   const int il = threadIdx.x & 63, sl = threadIdx.x >> 6;
-  __shared__ double part[4][64];
+  __shared__ double part[kRedSlices][64];
   const int64_t q0 = I.inv_off[a], q1 = I.inv_off[a + 1];
   for (int i0 = 0; i0 < no; i0 += 64) {
     const int i = i0 + il;
@@ -328,8 +333,8 @@ __global__ __launch_bounds__(256) void singles_reduce_kernel(PairTab T, InvTab I
     if (i < no) {
       const int64_t e = (int64_t)i * nv + a;
       if (sl == 0) v = gss[e];
-      for (int sp = sl; sp < n_split; sp += 4) v += gW[(int64_t)sp * n + e];
-      for (int64_t q = q0 + sl; q < q1; q += 4) {
+      for (int sp = sl; sp < n_split; sp += kRedSlices) v += gW[(int64_t)sp * n + e];
+      for (int64_t q = q0 + sl; q < q1; q += kRedSlices) {
         const int Q = I.inv_q[q], pos = I.inv_pos[q];
         const int d = (int)(T.dom_off[Q + 1] - T.dom_off[Q]);
         v += gpart[gpart_off[2 * Q] + (int64_t)i * d + pos];
@@ -338,7 +343,11 @@ __global__ __launch_bounds__(256) void singles_reduce_kernel(PairTab T, InvTab I
     }
     part[sl][il] = v;
     __syncthreads();
-    if (sl == 0 && i < no) g[(int64_t)i * nv + a] = ((part[0][il] + part[1][il]) + part[2][il]) + part[3][il];
+    if (sl == 0 && i < no) {
+      double s = part[0][il];
+      for (int k = 1; k < kRedSlices; ++k) s += part[k][il];
+      g[(int64_t)i * nv + a] = s;
+    }
     __syncthreads();
   }
 }
@@ -729,7 +738,7 @@ void launch_s3int_singles(const PairTab& T, int dmax, const int64_t* gpart_off, 
 
 void launch_singles_reduce(const PairTab& T, const InvTab& I, const int64_t* gpart_off, const double* gpart,
                            const double* gss, const double* gW, int n_split, double* g, hipStream_t st) {
-  hipLaunchKernelGGL(singles_reduce_kernel, dim3((unsigned)T.n_v), dim3(256), 0, st, T, I, gpart_off,
+  hipLaunchKernelGGL(singles_reduce_kernel, dim3((unsigned)T.n_v), dim3(64 * kRedSlices), 0, st, T, I, gpart_off,
                      gpart, gss, gW, n_split, g);
   TCI_HIP(hipGetLastError());
 }
