@@ -4,9 +4,10 @@ One replay runs an iteration's device work: the residuals and their norms,
 the norm test, the preconditioner, Gram-Schmidt against the basis (the second
 sweep switched on the device), orthonormalisation within the new block, the
 sigma builds of the new vectors and the new columns of the subspace matrix.
-It must follow the eager solver: the same iterations, sigma counts and roots
-(to 1e-10 Eh; the block-internal dots reduce in another order, so the
-vectors may differ in the last bits), against the dense spectrum and the
+It must follow the eager solver, whose dots, projections and norms sum in
+the same order: the same iterations, sigma counts and roots (to 1e-10 Eh;
+the second-sweep test runs on the device there and on the host here, so the
+last bits may differ), against the dense spectrum and the
 oracle Davidson, across groups of four roots, budget collapses, the
 mixed -> FP64 switch, roots that stop on their residual norm inside a replay,
 repeated solves that reuse the captured graphs, and the generalized (ACPF)

@@ -718,9 +718,11 @@ struct Dav {
     // the pointer/coef tables are reused by the next call: keep ordering simple
     TCI_HIP(hipStreamSynchronize(st));
   }
+  // (norms and the projections of orthonormalize() go through mdotm, whose
+  // summation order the graph-captured iteration reproduces)
   double norm(const double* v) {
     std::vector<double> o;
-    mdot({const_cast<double*>(v)}, v, o);
+    mdotm({const_cast<double*>(v)}, {const_cast<double*>(v)}, o);
     return std::sqrt(std::max(o[0], 0.0));
   }
   void scale(double* v, double a) {
@@ -731,7 +733,7 @@ struct Dav {
   bool orthonormalize(double* v, const std::vector<double*>& X) {
     std::vector<double> ov;
     for (int pass = 0; pass < 2 && !X.empty(); ++pass) {
-      mdot(X, v, ov);
+      mdotm(X, {v}, ov);
       for (double& o : ov) o = -o;
       lincomb(v, 1.0, X, ov);
     }
