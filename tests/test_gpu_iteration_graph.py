@@ -43,7 +43,7 @@ def _same_vectors(a, b, tol):
         assert np.abs(x - s * y).max() < tol
 
 
-@pytest.mark.parametrize("precision", ["fp64", "bf16x3"])
+@pytest.mark.parametrize("precision", ["fp64", "bf16x3", "fp16x3"])
 @pytest.mark.parametrize("roots", [2, 3])
 def test_iteration_graph_matches_eager(precision, roots):
     _say("case: iteration graph vs eager", precision, roots)
@@ -148,3 +148,20 @@ def test_iteration_graph_generalized_problem(functional):
     assert r0["converged"] and r1["converged"]
     assert r1["iterations"] == r0["iterations"]
     assert np.abs(e0 - e1).max() < 1e-10
+
+
+def test_iteration_graphs_across_solves_on_one_plan():
+    """One plan, solves with 2, 3 and again 2 roots and both split precisions
+    between FP64 ones: each solve equals its eager twin, whichever graphs the
+    earlier solves captured (a split-precision switch drops them all)."""
+    _say("case: iteration graphs across solves")
+    prob = make_problem(5, 60, 80, 24, domain_range=(20, 60), weak_below=25, l_scale=0.03)
+    plan = tci.Plan.from_problem(prob)
+    for roots, precision in [(2, "fp64"), (3, "bf16x3"), (2, "fp16x3"), (3, "fp64"), (2, "bf16x3")]:
+        kw = dict(roots=roots, energy_tol=1e-9, norm_tol=1e-6, precision=precision, switch_tol=1e-6)
+        e0, r0 = plan.davidson(**kw)
+        e1, r1 = plan.davidson(graph="iteration", **kw)
+        _say(f"  {roots} roots {precision}: {r0['iterations']} / {r1['iterations']} iterations")
+        assert r0["converged"] and r1["converged"]
+        assert r1["iterations"] == r0["iterations"]
+        assert np.abs(e0 - e1).max() < 1e-10
