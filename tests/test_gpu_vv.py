@@ -128,13 +128,14 @@ def test_vv_davidson_roots():
     assert np.max(np.abs(np.asarray(e2) - np.asarray(e0))) < 1e-6
 
 
-@pytest.mark.parametrize("tile", ["128x192T", "128x192T2"])
+@pytest.mark.parametrize("tile", ["128x192T", "128x192T2", "128x192d2", "128x192d3"])
 @pytest.mark.parametrize("precision", ["bf16x3", "fp16x3"])
 @pytest.mark.parametrize("args", CASES)
 def test_vv_transposed_split_product(args, precision, tile):
     """S^T = V X^T (walks along the tiles' N side), one and two K-splits,
-    transposed back before the gather: against the oracle at the split
-    bound, and against the untransposed product to FP32 summation order."""
+    transposed back before the gather, and the untransposed product with two
+    or three direct-load stages: against the oracle at the split bound, and
+    against the default schedule to FP32 summation order."""
     _say("case: resident ladder transposed", precision, tile, args)
     prob = make_problem(**args)
     ints = Integrals(prob)

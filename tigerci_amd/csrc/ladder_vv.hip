@@ -476,13 +476,16 @@ void vv_prepare_split(tci_plan& P, int precision, hipStream_t st) {
   V.prec = precision;
   // untuned schedule: 128 x 192 tiles, TCI_VV_TILE=256 (256w8, 256w16) for the
   // 32 x 32-MFMA kernel, TCI_VV_TILE=128x192T (128x192T<k>: k K-splits) for
-  // the transposed product
+  // the transposed product, TCI_VV_TILE=128x192d2 or 128x192d3 for two or
+  // three direct-load stages
   const char* te = std::getenv("TCI_VV_TILE");
   const std::string ts = te ? te : "";
   if (te && std::atoi(te) == 256)
     vv_set_schedule(P, precision, 256, 256, ts == "256w8" ? 8 : (ts == "256w16" ? 16 : 0), 2, st);
   else if (ts.rfind("128x192T", 0) == 0)
     vv_set_schedule(P, precision, 128, 192, 0, ts.size() > 8 ? std::max(1, std::atoi(te + 8)) : 1, st, true);
+  else if (ts == "128x192d2" || ts == "128x192d3")
+    vv_set_schedule(P, precision, 128, 192, ts == "128x192d2" ? 2 : 3, 1, st);
   else vv_set_schedule(P, precision, 128, 192, 0, 1, st);
   TCI_HIP(hipStreamSynchronize(st));
 }
@@ -558,15 +561,19 @@ double vv_tune(tci_plan& P, int precision, hipStream_t st, std::string& log) {
       cands.push_back({256, 256, 0, ks});
       cands.push_back({256, 256, 8, ks});  // dma field: eight waves
       cands.push_back({256, 256, 16, ks});  // sixteen waves
+      cands.push_back({128, 192, 2, ks});  // 128 x 192 with two or three direct-load stages
+      cands.push_back({128, 192, 3, ks});
       cands.push_back({128, 192, 0, ks, true});  // S^T = V X^T
       cands.push_back({128, 128, 0, ks, true});
       cands.push_back({192, 192, 3, ks, true});
+      cands.push_back({128, 192, 3, ks, true});
     }
   }
   auto label = [](const Cand& c) {
     return std::to_string(c.tile) + (c.tile_n ? "x" + std::to_string(c.tile_n) : "") +
-           (c.dma ? (c.tile == 256 ? "w" + std::to_string(c.dma) : "d") : "") + (c.trans ? "T" : "") + "/" +
-           std::to_string(c.ks);
+           (c.dma ? (c.tile == 256 ? "w" + std::to_string(c.dma) : "d" + (c.tile == 192 ? "" : std::to_string(c.dma)))
+                  : "") +
+           (c.trans ? "T" : "") + "/" + std::to_string(c.ks);
   };
   double best = 1e300;
   Cand bc = cands[0];
