@@ -222,13 +222,14 @@ def test_sigma_pair_block_strips(args):
     assert _rel(_gpu_sigma(plan, x, precision="bf16x3"), ref) < 5e-5
 
 
-@pytest.mark.parametrize("waves", ["256", "256w8", "256w16"])
+@pytest.mark.parametrize("waves", ["256", "256w8", "256w16", "128x192d2"])
 @pytest.mark.parametrize("precision", ["bf16x3", "fp16x3"])
 def test_split_products_on_256_tiles(precision, waves):
     """Every split pair product (U, W, Z, Z3, the 4-internal product) on the
     256 x 256 tiles of the 32 x 32-MFMA kernel (TCI_PAIRS_TILE=256, or 256w8
-    and 256w16 for its eight- and sixteen-wave forms), and the resident
-    ladder's product on them too
+    and 256w16 for its eight- and sixteen-wave forms) or on 128 x 192 tiles
+    with two direct-load stages (128x192d2), and the resident ladder's
+    product on them too
     (TCI_VV_TILE): sigma against the
     oracle, with outputs that are partial tiles (n_v = 45, 6 internal), W's
     blocked columns and U's accumulation."""

@@ -457,22 +457,28 @@ void dense_pairs_build_split(tci_plan& P, hipStream_t st, int cfg_z, int cfg_uw)
   // direct-load stages with TCI_PAIRS_TILE=192d
   // cfg_z (Z, Z3 and the 4-internal product) and cfg_uw (U and W): 0 for
   // 128 x 128 tiles, 1, 2 and 3 for 256 x 256 tiles of the 32 x 32-MFMA
-  // kernel with 4, 8 or 16 waves; -1 for TCI_PAIRS_TILE (128, 192 or 192d: Z
-  // and Z3 alone on 192 x 192 tiles, 256, 256w8 or 256w16: every product on
-  // 256 x 256)
+  // kernel with 4, 8 or 16 waves, 4 for 128 x 192 tiles with two direct-load
+  // stages; -1 for TCI_PAIRS_TILE (128, 192 or 192d: Z and Z3 alone on
+  // 192 x 192 tiles, 256, 256w8 or 256w16: every product on 256 x 256,
+  // 128x192d2: every product on 128 x 192 with direct loads)
   const char* te = std::getenv("TCI_PAIRS_TILE");
   const std::string tes = te ? te : "128";
-  if (cfg_z < 0) cfg_z = tes == "256" ? 1 : (tes == "256w8" ? 2 : (tes == "256w16" ? 3 : 0));
+  if (cfg_z < 0)
+    cfg_z = tes == "256" ? 1 : (tes == "256w8" ? 2 : (tes == "256w16" ? 3 : (tes == "128x192d2" ? 4 : 0)));
   if (cfg_uw < 0) cfg_uw = cfg_z;
-  static const int waves_of[4] = {4, 4, 8, 16};
-  const int ztile = cfg_z ? 256 : (tes == "192" || tes == "192d" ? 192 : 128);
-  const int zdma = ztile == 192 && tes == "192d" ? 3 : 0;
-  const int htile = cfg_z ? 256 : 128, pw = waves_of[cfg_z];      // 4-internal product, waves of Z's 256 tiles
-  const int utile = cfg_uw ? 256 : 128, uw = waves_of[cfg_uw];   // U and W
+  static const int waves_of[5] = {4, 4, 8, 16, 4};
+  auto tile_of = [](int c) { return c == 1 || c == 2 || c == 3 ? 256 : 128; };
+  const int ztile = cfg_z ? tile_of(cfg_z) : (tes == "192" || tes == "192d" ? 192 : 128);
+  const int ztn = cfg_z == 4 ? 192 : 0;                            // tile_n of the 128 x 192 form
+  const int zdma = cfg_z == 4 ? 2 : (ztile == 192 && tes == "192d" ? 3 : 0);
+  const int htile = tile_of(cfg_z), pw = waves_of[cfg_z];          // 4-internal product, waves of Z's 256 tiles
+  const int utile = tile_of(cfg_uw), uw = waves_of[cfg_uw];        // U and W
+  const int utn = cfg_uw == 4 ? 192 : 0, udma = cfg_uw == 4 ? 2 : 0;
   M.ringZx.clear();
   M.ringZx.split = true;
   M.ringZx.waves = pw;
   M.ringZx.tile = ztile;
+  M.ringZx.tile_n = ztn;  // tasks are cut by tile x tile_n when added
   if (M.z_lowrank)  // Ytx holds the split rows of U = Yt + C~R Lov^T
     M.ringZx.add(P.Zb.as<double>(), N, iN, iN,
                  {dseg(xop(M.CRx, ldx), xop(M.Jx, ldx), K, 1.0), dseg(xop(M.Ytx, ldj), xop(M.LovTx, ldj), KJ, 1.0)},
@@ -488,6 +494,7 @@ void dense_pairs_build_split(tci_plan& P, hipStream_t st, int cfg_z, int cfg_uw)
   M.ringZ3x.split = true;
   M.ringZ3x.waves = pw;
   M.ringZ3x.tile = ztile;
+  M.ringZ3x.tile_n = ztn;
   M.ringZ3x.add(P.Z3b.as<double>(), N, iN, iN, {dseg(xop(M.Jx, ldx), xop(M.CCx, ldx), K, 1.0)}, false, false, 0,
                 Blocking(), 5);
   M.ringZ3x.dma = zdma;
@@ -509,8 +516,10 @@ void dense_pairs_build_split(tci_plan& P, hipStream_t st, int cfg_z, int cfg_uw)
     M.holeGx.split = true;
     M.holeGx.waves = pw;
     M.holeGx.tile = htile;
+    M.holeGx.tile_n = ztn;
     M.holeGx.add(P.Remb.as<double>(), v2, np, (int)v2, {dseg(xop(M.holeAx, ldh), xop(M.embTx, ldh), (int)kh, 1.0)},
                  false, false, 0, Blocking(), 5);
+    M.holeGx.dma = cfg_z == 4 ? 2 : 0;
     M.holeGx.upload_tables(st);
   }
   Blocking bw;
@@ -520,8 +529,10 @@ void dense_pairs_build_split(tci_plan& P, hipStream_t st, int cfg_z, int cfg_uw)
   M.progWx.split = true;
   M.progWx.waves = uw;
   M.progWx.tile = utile;
+  M.progWx.tile_n = utn;
   M.progWx.add(P.W.as<double>(), nv, (int)P.n_chol, iN, {dseg(xop(M.Lovx, ldx), xop(M.CtRx, ldx), K, 1.0)}, false,
                false, 0, bw, 5);
+  M.progWx.dma = udma;
   M.progWx.upload_tables(st);
   // U[(i,a)][J] = Yt + sum_(j,b) C~R[(i,a)][(j,b)] Lov[J][(j,b)], accumulated into Yt
   M.progUx.clear();
@@ -529,8 +540,10 @@ void dense_pairs_build_split(tci_plan& P, hipStream_t st, int cfg_z, int cfg_uw)
     M.progUx.split = true;
     M.progUx.waves = uw;
     M.progUx.tile = utile;
+    M.progUx.tile_n = utn;
     M.progUx.add(P.Yt.as<double>(), P.n_chol, iN, (int)P.n_chol, {dseg(xop(M.CtRx, ldx), xop(M.Lovx, ldx), K, 1.0)},
                  true, false, 0, Blocking(), 5);
+    M.progUx.dma = udma;
     M.progUx.upload_tables(st);
   }
   // With 128 x 128 tiles the products go out as one launch, tasks with the
@@ -542,6 +555,8 @@ void dense_pairs_build_split(tci_plan& P, hipStream_t st, int cfg_z, int cfg_uw)
     out.clear();
     out.split = true;
     out.tile = (*gs.begin())->tile;
+    out.tile_n = (*gs.begin())->tile_n;
+    out.dma = (*gs.begin())->dma;
     out.waves = (*gs.begin())->waves;
     for (const GemmProgram* g : gs) {
       const int s0 = (int)out.segs.size();
@@ -557,7 +572,8 @@ void dense_pairs_build_split(tci_plan& P, hipStream_t st, int cfg_z, int cfg_uw)
   M.pairsx.clear();
   M.pairsUWx.clear();
   // one launch holds tasks of one tile shape only
-  M.pairs_merged = (ztile == 128 || ztile == 256) && !zdma && (M.z_lowrank || utile == ztile);
+  M.pairs_merged = ((ztile == 128 || ztile == 256) && !zdma && (M.z_lowrank || utile == ztile)) ||
+                   (cfg_z == 4 && (M.z_lowrank || cfg_uw == 4));
   if (M.pairs_merged) {
     if (M.z_lowrank) {
       merge(M.pairsUWx, {&M.progUx, &M.progWx}, st);
