@@ -222,7 +222,7 @@ def test_sigma_pair_block_strips(args):
     assert _rel(_gpu_sigma(plan, x, precision="bf16x3"), ref) < 5e-5
 
 
-@pytest.mark.parametrize("waves", ["256", "256w8", "256w16", "128x192d2"])
+@pytest.mark.parametrize("waves", ["256", "256w8", "256w16", "128x192d2", "128d2"])
 @pytest.mark.parametrize("precision", ["bf16x3", "fp16x3"])
 def test_split_products_on_256_tiles(precision, waves):
     """Every split pair product (U, W, Z, Z3, the 4-internal product) on the

@@ -128,7 +128,7 @@ def test_vv_davidson_roots():
     assert np.max(np.abs(np.asarray(e2) - np.asarray(e0))) < 1e-6
 
 
-@pytest.mark.parametrize("tile", ["128x192T", "128x192T2", "128x192d2", "128x192d3"])
+@pytest.mark.parametrize("tile", ["128x192T", "128x192T2", "128x192d2", "128x192d3", "128x128d2"])
 @pytest.mark.parametrize("precision", ["bf16x3", "fp16x3"])
 @pytest.mark.parametrize("args", CASES)
 def test_vv_transposed_split_product(args, precision, tile):

@@ -888,7 +888,7 @@ void mixed_prepare(tci_plan& P, int precision, hipStream_t st) {
       // MFMAs with 4, 8 or 16 waves): Z, Z3 and the 4-internal product first,
       // then U and W; then their tile order
       std::string log = (precision == TCI_SIGMA_FP16X3 ? "fp16x3" : "bf16x3") + std::string(" pair products: tile");
-      static const char* cname[5] = {"128", "256w4", "256w8", "256w16", "128x192d2"};
+      static const char* cname[6] = {"128", "256w4", "256w8", "256w16", "128x192d2", "128d2"};
       const int64_t bytes0 = M.bytes;  // each build counts its split copies again
       auto trial = [&](int cz, int cu) {
         M.bytes = bytes0;
@@ -903,7 +903,7 @@ void mixed_prepare(tci_plan& P, int precision, hipStream_t st) {
         log += " kept";
         trial(bz, bu);
       } else {
-        for (int cz = 0; cz < 5; ++cz) {
+        for (int cz = 0; cz < 6; ++cz) {
           const double ms = trial(cz, 0);
           log += std::string(" z") + cname[cz] + ":" + fmt_ms(ms);
           if (ms < best * 0.98) {
@@ -912,7 +912,7 @@ void mixed_prepare(tci_plan& P, int precision, hipStream_t st) {
           }
         }
         if (M.z_lowrank) {  // U and W go out as a launch of their own
-          for (int cu : {2, 4}) {
+          for (int cu : {2, 4, 5}) {
             const double ms = trial(bz, cu);
             log += std::string(" uw") + cname[cu] + ":" + fmt_ms(ms);
             if (ms < best * 0.98) {
@@ -923,7 +923,7 @@ void mixed_prepare(tci_plan& P, int precision, hipStream_t st) {
         } else {
           bu = bz;
         }
-        if (!(M.z_lowrank && bu == 4)) trial(bz, bu);  // unless the last trial built the chosen tiles
+        if (!(M.z_lowrank && bu == 5)) trial(bz, bu);  // unless the last trial built the chosen tiles
         M.tuned_pairs_z = bz;
         M.tuned_pairs_uw = bu;
       }

@@ -486,6 +486,8 @@ void vv_prepare_split(tci_plan& P, int precision, hipStream_t st) {
     vv_set_schedule(P, precision, 128, 192, 0, ts.size() > 8 ? std::max(1, std::atoi(te + 8)) : 1, st, true);
   else if (ts == "128x192d2" || ts == "128x192d3")
     vv_set_schedule(P, precision, 128, 192, ts == "128x192d2" ? 2 : 3, 1, st);
+  else if (ts == "128x128d2")
+    vv_set_schedule(P, precision, 128, 128, 2, 1, st);
   else vv_set_schedule(P, precision, 128, 192, 0, 1, st);
   TCI_HIP(hipStreamSynchronize(st));
 }
@@ -563,6 +565,7 @@ double vv_tune(tci_plan& P, int precision, hipStream_t st, std::string& log) {
       cands.push_back({256, 256, 16, ks});  // sixteen waves
       cands.push_back({128, 192, 2, ks});  // 128 x 192 with two or three direct-load stages
       cands.push_back({128, 192, 3, ks});
+      cands.push_back({128, 128, 2, ks});
       cands.push_back({128, 192, 0, ks, true});  // S^T = V X^T
       cands.push_back({128, 128, 0, ks, true});
       cands.push_back({192, 192, 3, ks, true});

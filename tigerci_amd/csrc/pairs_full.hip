@@ -457,23 +457,28 @@ void dense_pairs_build_split(tci_plan& P, hipStream_t st, int cfg_z, int cfg_uw)
   // direct-load stages with TCI_PAIRS_TILE=192d
   // cfg_z (Z, Z3 and the 4-internal product) and cfg_uw (U and W): 0 for
   // 128 x 128 tiles, 1, 2 and 3 for 256 x 256 tiles of the 32 x 32-MFMA
-  // kernel with 4, 8 or 16 waves, 4 for 128 x 192 tiles with two direct-load
-  // stages; -1 for TCI_PAIRS_TILE (128, 192 or 192d: Z and Z3 alone on
-  // 192 x 192 tiles, 256, 256w8 or 256w16: every product on 256 x 256,
-  // 128x192d2: every product on 128 x 192 with direct loads)
+  // kernel with 4, 8 or 16 waves, 4 and 5 for 128 x 192 and 128 x 128 tiles
+  // with two direct-load stages; -1 for TCI_PAIRS_TILE (128, 192 or 192d: Z
+  // and Z3 alone on 192 x 192 tiles, 256, 256w8 or 256w16: every product on
+  // 256 x 256, 128x192d2 or 128d2: every product on those direct-load tiles)
   const char* te = std::getenv("TCI_PAIRS_TILE");
   const std::string tes = te ? te : "128";
   if (cfg_z < 0)
-    cfg_z = tes == "256" ? 1 : (tes == "256w8" ? 2 : (tes == "256w16" ? 3 : (tes == "128x192d2" ? 4 : 0)));
+    cfg_z = tes == "256" ? 1
+            : tes == "256w8" ? 2
+            : tes == "256w16" ? 3
+            : tes == "128x192d2" ? 4
+            : tes == "128d2" ? 5
+                             : 0;
   if (cfg_uw < 0) cfg_uw = cfg_z;
-  static const int waves_of[5] = {4, 4, 8, 16, 4};
+  static const int waves_of[6] = {4, 4, 8, 16, 4, 4};
   auto tile_of = [](int c) { return c == 1 || c == 2 || c == 3 ? 256 : 128; };
   const int ztile = cfg_z ? tile_of(cfg_z) : (tes == "192" || tes == "192d" ? 192 : 128);
   const int ztn = cfg_z == 4 ? 192 : 0;                            // tile_n of the 128 x 192 form
-  const int zdma = cfg_z == 4 ? 2 : (ztile == 192 && tes == "192d" ? 3 : 0);
+  const int zdma = cfg_z >= 4 ? 2 : (ztile == 192 && tes == "192d" ? 3 : 0);
   const int htile = tile_of(cfg_z), pw = waves_of[cfg_z];          // 4-internal product, waves of Z's 256 tiles
   const int utile = tile_of(cfg_uw), uw = waves_of[cfg_uw];        // U and W
-  const int utn = cfg_uw == 4 ? 192 : 0, udma = cfg_uw == 4 ? 2 : 0;
+  const int utn = cfg_uw == 4 ? 192 : 0, udma = cfg_uw >= 4 ? 2 : 0;
   M.ringZx.clear();
   M.ringZx.split = true;
   M.ringZx.waves = pw;
@@ -519,7 +524,7 @@ void dense_pairs_build_split(tci_plan& P, hipStream_t st, int cfg_z, int cfg_uw)
     M.holeGx.tile_n = ztn;
     M.holeGx.add(P.Remb.as<double>(), v2, np, (int)v2, {dseg(xop(M.holeAx, ldh), xop(M.embTx, ldh), (int)kh, 1.0)},
                  false, false, 0, Blocking(), 5);
-    M.holeGx.dma = cfg_z == 4 ? 2 : 0;
+    M.holeGx.dma = cfg_z >= 4 ? 2 : 0;
     M.holeGx.upload_tables(st);
   }
   Blocking bw;
@@ -572,8 +577,9 @@ void dense_pairs_build_split(tci_plan& P, hipStream_t st, int cfg_z, int cfg_uw)
   M.pairsx.clear();
   M.pairsUWx.clear();
   // one launch holds tasks of one tile shape only
-  M.pairs_merged = ((ztile == 128 || ztile == 256) && !zdma && (M.z_lowrank || utile == ztile)) ||
-                   (cfg_z == 4 && (M.z_lowrank || cfg_uw == 4));
+  // (programs merged into one launch share tile, tile_n and stages)
+  const bool zmerge = ((ztile == 128 || ztile == 256) && !zdma) || cfg_z >= 4;
+  M.pairs_merged = zmerge && (M.z_lowrank || (utile == ztile && utn == ztn && udma == zdma));
   if (M.pairs_merged) {
     if (M.z_lowrank) {
       merge(M.pairsUWx, {&M.progUx, &M.progWx}, st);

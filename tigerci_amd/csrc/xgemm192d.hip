@@ -1,4 +1,4 @@
-// Split GEMM tiles (192 x 192 and 128 x 192) with LDS stages filled by direct
+// Split GEMM tiles (192 x 192, 128 x 192, 128 x 128) with LDS stages filled by direct
 // global-to-LDS loads (global_load_lds_dwordx4).
 //
 // The register-staged kernel (xgemm.hpp) has one stage in flight while it
@@ -132,6 +132,8 @@ void launch_xgemmd(int precision, int tm, int tn, int nst, const Task* tasks, co
     launch_xgemmd_t<128, 192, 2>(precision, tasks, segs, ntasks, J0, scales, st, remap);
   else if (tm == 128 && tn == 192 && nst == 3)
     launch_xgemmd_t<128, 192, 3>(precision, tasks, segs, ntasks, J0, scales, st, remap);
+  else if (tm == 128 && tn == 128 && nst == 2)
+    launch_xgemmd_t<128, 128, 2>(precision, tasks, segs, ntasks, J0, scales, st, remap);
   else
     throw Error(TCI_ERR_ARG, "no direct-load split GEMM for " + std::to_string(tm) + "x" + std::to_string(tn) + ", " +
                                  std::to_string(nst) + " stages");
